@@ -1,0 +1,109 @@
+/*
+ * mibwt — MI355X-native Burrows-Wheeler transform builder (HIP, gfx950).
+ *
+ * C ABI.  Host pointers unless a function name ends in _device.  No C++ or HIP types cross
+ * this boundary; streams are passed as opaque pointers (a hipStream_t, or NULL for the
+ * storage's own stream).
+ *
+ * Two conventions for the output:
+ *   - libcubwt format (libcubwt.cuh:69-77, libcubwt.cu:2242-2254): L[0] = T[n-1],
+ *     L[1..i-1] and L[i..n-1] are the BWT rows with the "$" row removed, and the return
+ *     value is the primary index i >= 1;
+ *   - reference format (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:4776-4790):
+ *     O[j] = T[(SA[j]-1) mod n] over the n non-empty suffixes in $-order.
+ *     O = L[1..i-1] ++ [L[0]] ++ L[i..n-1] (libcubwt.cu:3003-3015).
+ */
+#ifndef MIBWT_H
+#define MIBWT_H 1
+
+#include <stdint.h>
+
+#define LIBCUBWT_VERSION_MAJOR 1
+#define LIBCUBWT_VERSION_MINOR 6
+#define LIBCUBWT_VERSION_PATCH 3
+#define LIBCUBWT_VERSION_STRING "1.6.3"
+
+/* Error codes: identical values to libcubwt.cuh:32-38. */
+#define LIBCUBWT_NO_ERROR 0
+#define LIBCUBWT_BAD_PARAMETER -1
+#define LIBCUBWT_NOT_ENOUGH_MEMORY -2
+#define LIBCUBWT_GPU_ERROR -7
+#define LIBCUBWT_GPU_NOT_SUPPORTED -8
+#define LIBCUBWT_GPU_NOT_ENOUGH_MEMORY -9
+/* mibwt-specific: a 64-bit phrase fingerprint collision survived every re-seed. */
+#define MIBWT_HASH_COLLISION -20
+
+/* BWT methods for mibwt_bwt / mibwt_batch_bwt. */
+#define MIBWT_METHOD_PFP 0
+#define MIBWT_METHOD_LYNDON 1
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- libcubwt drop-in */
+
+/* Replaces libcubwt_allocate_device_storage (libcubwt.cuh:60, libcubwt.cu:2256).
+ * Creates a storage bound to the current HIP device with its own stream.  Device buffers
+ * are sized for max_length and grow on demand.  BAD_PARAMETER if device_storage is NULL or
+ * max_length >= 2^31 - 2^20 (32-bit text positions). */
+int64_t libcubwt_allocate_device_storage(void** device_storage, int64_t max_length);
+
+/* Replaces libcubwt_free_device_storage (libcubwt.cuh:67, libcubwt.cu:2394). */
+int64_t libcubwt_free_device_storage(void* device_storage);
+
+/* Replaces libcubwt_bwt (libcubwt.cuh:77, libcubwt.cu:2414-2431): host T[0..n) -> host
+ * L[0..n) in libcubwt format (L may alias T); returns the primary index or an error code.
+ * 16 <= n <= max_length, as in libcubwt.  Computed by prefix-free parsing with the (W,P)
+ * from mibwt_select_wp (fixed W=5,P=16 when the selector is not trained). */
+int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t n);
+
+/* ---------------------------------------------------------------- mibwt extensions */
+
+/* Prefix-free-parsing BWT with runtime window W (1..64) and modulus P (1..65535).
+ * Replaces the compile-time variants Prefix_Free_Parsing_W=<W>_P=<P>.cu (main at
+ * Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:4793-4815, Compute_D :2415, Compute_S :4330).
+ * Host buffers, libcubwt format; 1 <= n <= max_length. */
+int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int32_t W,
+                      int32_t P);
+
+/* Same on device buffers already resident in HBM: dT[0..n) -> dO[0..n) in REFERENCE
+ * format (O), on the storage's stream (stream == NULL) or on `stream`.  Synchronous.
+ * Returns the primary index. */
+int64_t mibwt_pfp_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
+                             int32_t W, int32_t P, void* stream);
+
+/* Lyndon-grammar BWT (Lyndon_Grammar_BWT.cu main :5547-5576: build_Lyndon_Grammar :2592,
+ * Sort_Lyndon_Grammar :3621, Build_BWT :4628).  Host buffers, libcubwt format. */
+int64_t mibwt_lyndon_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n);
+
+/* Device-buffer variant, reference format, like mibwt_pfp_bwt_device. */
+int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
+                                void* stream);
+
+/* Per-phase statistics of the last call on this storage. */
+#define MIBWT_MAX_PHASES 16
+struct mibwt_phase_stats {
+    int32_t num_phases;
+    char name[MIBWT_MAX_PHASES][24];
+    double ms[MIBWT_MAX_PHASES];         /* HIP-event time of the phase */
+    double bytes[MIBWT_MAX_PHASES];      /* algorithmic HBM bytes of the phase */
+    int64_t n, W, P;
+    int64_t triggers, phrases, distinct_phrases, dict_length, alpha_groups;
+    int64_t mixed_entries, doubling_rounds_dict, doubling_rounds_parse;
+    double total_ms;                     /* HIP-event time from T resident to O resident */
+};
+int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out);
+
+/* Time a kernel of the last configuration: re-runs the named phase's dominant kernel
+ * `reps` times on the storage stream and returns its mean duration in ms via *ms.
+ * Names: "scan" (trigger scan) and "sort" (one 8-bit LSD radix pass of the phrase
+ * fingerprint sort).  Inputs must be those of the last mibwt_pfp_bwt_device call. */
+int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms,
+                          double* bytes_per_launch);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIBWT_H */

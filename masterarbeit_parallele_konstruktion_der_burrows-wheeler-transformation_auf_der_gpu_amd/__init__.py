@@ -1,0 +1,206 @@
+"""mibwt — MI355X-native Burrows-Wheeler transform builder.
+
+Python mirror of the C ABI in include/mibwt.h (ctypes over lib/libmibwt.so).  The library
+is the product; this module only binds it.  There is no CPU fallback: if the HIP library is
+missing every call raises.
+
+Entry points mirror the reference programs:
+  - ``bwt(T, method="pfp", W=5, P=16)`` -> (L, primary_index) in libcubwt format
+    (libcubwt.cuh:77, libcubwt.cu:2414);
+  - ``bwt_reference_format(...)`` -> O as the reference programs print it
+    (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:4776-4790);
+  - ``Storage`` wraps libcubwt_allocate_device_storage / libcubwt_free_device_storage.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmibwt.so")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+NO_ERROR = 0
+BAD_PARAMETER = -1
+NOT_ENOUGH_MEMORY = -2
+GPU_ERROR = -7
+GPU_NOT_SUPPORTED = -8
+GPU_NOT_ENOUGH_MEMORY = -9
+HASH_COLLISION = -20
+
+METHOD_PFP = 0
+METHOD_LYNDON = 1
+
+EXPORTED = [
+    "libcubwt_allocate_device_storage",
+    "libcubwt_free_device_storage",
+    "libcubwt_bwt",
+    "mibwt_pfp_bwt",
+    "mibwt_pfp_bwt_device",
+    "mibwt_lyndon_bwt",
+    "mibwt_lyndon_bwt_device",
+    "mibwt_last_phase_stats",
+    "mibwt_time_kernel",
+]
+
+_lib = None
+
+
+class BwtError(RuntimeError):
+    def __init__(self, code: int, what: str = ""):
+        super().__init__(f"mibwt error {code} {what}".strip())
+        self.code = code
+
+
+MAX_PHASES = 16
+
+
+class PhaseStats(ctypes.Structure):
+    _fields_ = [
+        ("num_phases", ctypes.c_int32),
+        ("name", (ctypes.c_char * 24) * MAX_PHASES),
+        ("ms", ctypes.c_double * MAX_PHASES),
+        ("bytes", ctypes.c_double * MAX_PHASES),
+        ("n", ctypes.c_int64), ("W", ctypes.c_int64), ("P", ctypes.c_int64),
+        ("triggers", ctypes.c_int64), ("phrases", ctypes.c_int64),
+        ("distinct_phrases", ctypes.c_int64), ("dict_length", ctypes.c_int64),
+        ("alpha_groups", ctypes.c_int64), ("mixed_entries", ctypes.c_int64),
+        ("doubling_rounds_dict", ctypes.c_int64), ("doubling_rounds_parse", ctypes.c_int64),
+        ("total_ms", ctypes.c_double),
+    ]
+
+
+def lib():
+    """Load lib/libmibwt.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: run build.py (hipcc, gfx950) first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, i64, i32 = ctypes.c_void_p, ctypes.c_int64, ctypes.c_int32
+        u8p = ctypes.POINTER(ctypes.c_uint8)
+        L.libcubwt_allocate_device_storage.argtypes = [ctypes.POINTER(vp), i64]
+        L.libcubwt_free_device_storage.argtypes = [vp]
+        L.libcubwt_bwt.argtypes = [vp, u8p, u8p, i64]
+        L.mibwt_pfp_bwt.argtypes = [vp, u8p, u8p, i64, i32, i32]
+        L.mibwt_pfp_bwt_device.argtypes = [vp, vp, vp, i64, i32, i32, vp]
+        L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
+        L.mibwt_lyndon_bwt_device.argtypes = [vp, vp, vp, i64, vp]
+        L.mibwt_last_phase_stats.argtypes = [vp, ctypes.POINTER(PhaseStats)]
+        L.mibwt_time_kernel.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_double)]
+        for name in EXPORTED:
+            getattr(L, name).restype = i64
+        _lib = L
+    return _lib
+
+
+def _u8(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _as_text(T) -> np.ndarray:
+    if isinstance(T, (bytes, bytearray, memoryview)):
+        return np.frombuffer(bytes(T), dtype=np.uint8).copy()
+    return np.ascontiguousarray(T, dtype=np.uint8)
+
+
+class Storage:
+    """A libcubwt-style device storage (one HIP stream, grow-only buffers)."""
+
+    def __init__(self, max_length: int):
+        self.handle = ctypes.c_void_p()
+        rc = lib().libcubwt_allocate_device_storage(ctypes.byref(self.handle), int(max_length))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "libcubwt_allocate_device_storage")
+        self.max_length = int(max_length)
+
+    def close(self):
+        if self.handle:
+            rc = lib().libcubwt_free_device_storage(self.handle)
+            self.handle = ctypes.c_void_p()
+            if rc != NO_ERROR:
+                raise BwtError(rc, "libcubwt_free_device_storage")
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- host-buffer entry points (libcubwt format)
+    def bwt(self, T, method: str = "pfp", W: int = 5, P: int = 16):
+        T = _as_text(T)
+        L = np.empty_like(T)
+        if method == "pfp":
+            rc = lib().mibwt_pfp_bwt(self.handle, _u8(T), _u8(L), len(T), W, P)
+        elif method == "lyndon":
+            rc = lib().mibwt_lyndon_bwt(self.handle, _u8(T), _u8(L), len(T))
+        elif method == "cubwt":
+            rc = lib().libcubwt_bwt(self.handle, _u8(T), _u8(L), len(T))
+        else:
+            raise ValueError(method)
+        if rc < 0:
+            raise BwtError(rc, method)
+        return L, int(rc)
+
+    # ---- device-buffer entry points (reference format)
+    def bwt_device(self, dT_ptr: int, dO_ptr: int, n: int, method: str = "pfp", W: int = 5,
+                   P: int = 16, stream: int | None = None) -> int:
+        if method == "pfp":
+            rc = lib().mibwt_pfp_bwt_device(self.handle, dT_ptr, dO_ptr, n, W, P, stream)
+        else:
+            rc = lib().mibwt_lyndon_bwt_device(self.handle, dT_ptr, dO_ptr, n, stream)
+        if rc < 0:
+            raise BwtError(rc, method + "_device")
+        return int(rc)
+
+    def stats(self) -> dict:
+        st = PhaseStats()
+        rc = lib().mibwt_last_phase_stats(self.handle, ctypes.byref(st))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_last_phase_stats")
+        phases = []
+        for i in range(st.num_phases):
+            phases.append({"name": st.name[i].value.decode(), "ms": st.ms[i], "bytes": st.bytes[i]})
+        out = {k: getattr(st, k) for k, _ in PhaseStats._fields_ if k not in ("name", "ms", "bytes")}
+        out["phases"] = phases
+        return out
+
+    def time_kernel(self, name: str, reps: int = 20):
+        ms, b = ctypes.c_double(), ctypes.c_double()
+        rc = lib().mibwt_time_kernel(self.handle, name.encode(), reps, ctypes.byref(ms), ctypes.byref(b))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_time_kernel")
+        return ms.value, b.value
+
+
+def bwt(T, method: str = "pfp", W: int = 5, P: int = 16):
+    """(L, primary_index) in libcubwt format."""
+    T = _as_text(T)
+    with Storage(max(len(T), 16)) as s:
+        return s.bwt(T, method, W, P)
+
+
+def cubwt_to_reference(L: np.ndarray, index: int) -> np.ndarray:
+    """O = L[1..i-1] ++ [L[0]] ++ L[i..n-1] (libcubwt.cu:3003-3015, without its i == n quirk)."""
+    L = np.asarray(L, dtype=np.uint8)
+    return np.concatenate([L[1:index], L[:1], L[index:]])
+
+
+def reference_to_cubwt(O: np.ndarray, index: int) -> np.ndarray:
+    O = np.asarray(O, dtype=np.uint8)
+    return np.concatenate([O[index - 1:index], O[:index - 1], O[index:]])
+
+
+def bwt_reference_format(T, method: str = "pfp", W: int = 5, P: int = 16) -> np.ndarray:
+    L, idx = bwt(T, method, W, P)
+    return cubwt_to_reference(L, idx)

@@ -1,0 +1,243 @@
+// Shared host/device infrastructure of the mibwt library: error handling, the per-storage
+// stream and grow-only device buffers, phase timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/mibwt.h"
+
+namespace mibwt {
+
+// An error that maps onto a libcubwt-style return code.
+struct Error : std::runtime_error {
+    int64_t code;
+    Error(int64_t c, const std::string& what) : std::runtime_error(what), code(c) {}
+};
+
+inline int64_t hip_error_code(hipError_t e)
+{
+    switch (e) {
+    case hipSuccess: return LIBCUBWT_NO_ERROR;
+    case hipErrorOutOfMemory: return LIBCUBWT_GPU_NOT_ENOUGH_MEMORY;
+    case hipErrorNoDevice:
+    case hipErrorInvalidDevice:
+    case hipErrorNoBinaryForGpu:
+    case hipErrorInsufficientDriver: return LIBCUBWT_GPU_NOT_SUPPORTED;
+    default: return LIBCUBWT_GPU_ERROR;
+    }
+}
+
+#define MIBWT_HIP(call)                                                                    \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            char b_[256];                                                                  \
+            snprintf(b_, sizeof(b_), "%s:%d %s: %s", __FILE__, __LINE__, #call,            \
+                     hipGetErrorString(e_));                                               \
+            throw ::mibwt::Error(::mibwt::hip_error_code(e_), b_);                         \
+        }                                                                                  \
+    } while (0)
+
+#define MIBWT_LAUNCH_CHECK() MIBWT_HIP(hipGetLastError())
+
+#define MIBWT_REQUIRE(cond, code, msg)                                                     \
+    do {                                                                                   \
+        if (!(cond)) throw ::mibwt::Error((code), (msg));                                  \
+    } while (0)
+
+inline uint32_t ceil_div(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+
+inline int bits_for(uint64_t v)  // number of bits needed to represent values 0..v
+{
+    int b = 0;
+    while (b < 64 && (v >> b) != 0) b++;
+    return b;
+}
+
+// A grow-only device allocation owned by a Storage.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t cap = 0;
+};
+
+struct Phase {
+    std::string name;
+    hipEvent_t start = nullptr, stop = nullptr;
+    double bytes = 0;
+};
+
+// Everything one libcubwt-style storage owns.  One stream; not safe for concurrent calls.
+struct Storage {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int64_t max_length = 0;
+    std::vector<DevBuf> bufs;
+    void* pinned = nullptr;          // small pinned host buffer for scalar read-backs
+    size_t pinned_cap = 0;
+    // phase timing
+    std::vector<Phase> phases;
+    int phase_used = 0;
+    hipEvent_t t_begin = nullptr, t_end = nullptr;
+    mibwt_phase_stats stats{};
+    // state kept for mibwt_time_kernel
+    const uint8_t* last_text = nullptr;
+    int64_t last_n = 0;
+    int last_W = 0, last_P = 0;
+    int64_t last_sort_items = 0;
+
+    // device buffer #id with capacity >= bytes; contents are not preserved on growth
+    template <typename T>
+    T* buf(int id, size_t count)
+    {
+        size_t bytes = count * sizeof(T) + 256;
+        if ((size_t)id >= bufs.size()) bufs.resize(id + 1);
+        DevBuf& b = bufs[id];
+        if (b.cap < bytes) {
+            if (b.ptr) {
+                MIBWT_HIP(hipStreamSynchronize(stream));
+                MIBWT_HIP(hipFree(b.ptr));
+                b.ptr = nullptr;
+                b.cap = 0;
+            }
+            size_t want = bytes + bytes / 8;
+            hipError_t e = hipMalloc(&b.ptr, want);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                b.ptr = nullptr;
+                throw Error(LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "hipMalloc failed");
+            }
+            b.cap = want;
+        }
+        return reinterpret_cast<T*>(b.ptr);
+    }
+
+    template <typename T>
+    T* host(size_t count)
+    {
+        size_t bytes = count * sizeof(T);
+        if (pinned_cap < bytes) {
+            if (pinned) MIBWT_HIP(hipHostFree(pinned));
+            pinned = nullptr;
+            MIBWT_HIP(hipHostMalloc(&pinned, bytes < 4096 ? 4096 : bytes, hipHostMallocDefault));
+            pinned_cap = bytes < 4096 ? 4096 : bytes;
+        }
+        return reinterpret_cast<T*>(pinned);
+    }
+
+    // read `count` device values of type T (synchronises the stream)
+    template <typename T>
+    void read(T* dst, const T* dev, size_t count = 1)
+    {
+        T* h = host<T>(count);
+        MIBWT_HIP(hipMemcpyAsync(h, dev, sizeof(T) * count, hipMemcpyDeviceToHost, stream));
+        MIBWT_HIP(hipStreamSynchronize(stream));
+        memcpy(dst, h, sizeof(T) * count);
+    }
+
+    void phases_reset()
+    {
+        phase_used = 0;
+        memset(&stats, 0, sizeof(stats));
+    }
+    int phase_begin(const char* name)
+    {
+        if ((int)phases.size() <= phase_used) {
+            phases.emplace_back();
+            MIBWT_HIP(hipEventCreate(&phases.back().start));
+            MIBWT_HIP(hipEventCreate(&phases.back().stop));
+        }
+        Phase& p = phases[phase_used];
+        p.name = name;
+        p.bytes = 0;
+        MIBWT_HIP(hipEventRecord(p.start, stream));
+        return phase_used++;
+    }
+    void phase_end(int id, double bytes)
+    {
+        phases[id].bytes = bytes;
+        MIBWT_HIP(hipEventRecord(phases[id].stop, stream));
+    }
+    void phases_collect()
+    {
+        MIBWT_HIP(hipStreamSynchronize(stream));
+        stats.num_phases = phase_used < MIBWT_MAX_PHASES ? phase_used : MIBWT_MAX_PHASES;
+        for (int i = 0; i < stats.num_phases; i++) {
+            float ms = 0;
+            MIBWT_HIP(hipEventElapsedTime(&ms, phases[i].start, phases[i].stop));
+            snprintf(stats.name[i], sizeof(stats.name[i]), "%s", phases[i].name.c_str());
+            stats.ms[i] = ms;
+            stats.bytes[i] = phases[i].bytes;
+        }
+    }
+};
+
+// Buffer ids (one grow-only allocation each).  Buffers that are live at the same time have
+// different ids.
+enum BufId : int {
+    B_TEXT = 0,     // padded copy of the text
+    B_TRIG_MASK,    // 16-bit trigger masks
+    B_TILE_CNT,     // per-tile trigger counts
+    B_TRIG,         // trigger end positions + last phrase end (= phrase ends)
+    B_SCAN_TMP,     // 4 consecutive ids for scan partials (one per recursion level)
+    B_SCAN_TMP1,
+    B_SCAN_TMP2,
+    B_SCAN_TMP3,
+    B_SORT_K0,
+    B_SORT_K1,
+    B_SORT_V0,
+    B_SORT_V1,
+    B_SORT_HIST,
+    B_PH_FLAG,      // dictionary head flags / ids
+    B_PH_PID,       // phrase id per occurrence
+    B_DU_HEAD,
+    B_DU_START,
+    B_DU_LEN,
+    B_DU_LEN64,
+    B_DU_FREQ,
+    B_DU_OFF,
+    B_LEXRANK,
+    B_DT_OWNER,
+    B_DT_RANK,
+    B_DT_SA,
+    B_ACT_X,
+    B_ACT_POS,
+    B_ACT_X2,
+    B_ACT_POS2,
+    B_ACT_FLAG,
+    B_ACT_HEAD,
+    B_P_RANK,
+    B_P_SA,
+    B_G_OFF,
+    B_G_MIX,
+    B_G_ECNT,
+    B_G_DELTA,
+    B_MX_FLAG,
+    B_MX_SCAN,
+    B_MX_ENT,
+    B_Q_CNT,
+    B_Q_OFF,
+    B_SCALARS,
+    B_STAGE,
+    B_COUNT_
+};
+
+// Slots of the B_SCALARS buffer.
+enum ScalarSlot : int {
+    SC_TRIGGERS = 0,
+    SC_DISTINCT,
+    SC_COLLISION,
+    SC_ACTIVE,
+    SC_PIDX,
+    SC_ENTRIES,
+    SC_TOTAL,
+    SC_COUNT_ = 16
+};
+
+}  // namespace mibwt

@@ -1,0 +1,918 @@
+// Prefix-free-parsing BWT on gfx950.
+//
+// Reference: Prefix_Free_Parsing_W=5_P=16_kommentiert.cu (Compute_D :2415, Compute_S :4330).
+// The trigger set, the phrases and therefore the parse are exactly the reference's; the
+// later phases compute the same BWT by a different, GPU-shaped route:
+//
+//  1. scan      trigger windows (value mod P == 0, or equal to T[0..W-1], never the window
+//               at 0), one rolling Karp-Rabin value per thread over 16 window ends staged
+//               in LDS, 16-bit flag masks, then an ordered emit of the end positions
+//               (reference Compute_D1/D2 :443-571, D3/D4 :573-592).
+//  2. phrases   64-bit fingerprint per phrase occurrence.
+//  3. dict      LSD radix sort of the fingerprints; equal neighbours are compared byte by
+//               byte (a collision re-seeds the hash); dense phrase ids, one representative
+//               occurrence and the frequency per distinct phrase.
+//  4. dsort     suffix sorting of the dictionary (all suffixes of all distinct phrases) by
+//               prefix doubling with segment limits; equal suffixes of different phrases
+//               end up in one group.  These groups are the reference's S set
+//               (:2509-2592) and their order its S sort (:2767-3240); the order of the
+//               whole phrases is the lexicographic phrase order (:1990-2164, 616-691).
+//  5. psort     suffix array of the parse by prefix doubling (:2342-2412, 787-807).
+//  6. assemble  each dictionary suffix alpha of length > W owns cnt(alpha) consecutive BWT
+//               slots.  Groups whose preceding characters agree are filled directly (the
+//               reference's single-character intervals, :3337-3350).  Mixed groups and
+//               whole phrases are ordered by the rank of the next parse suffix
+//               (:4217-4248): entries (alpha, char) are generated in parse-suffix order and
+//               stably radix sorted by alpha; the last radix pass writes the BWT bytes.
+//
+// The terminator is virtual: positions >= n of the extended text T.$^W compare below every
+// byte, so all 256 byte values are accepted (the reference needs bytes >= 2).
+#include "prims.hpp"
+
+namespace mibwt {
+
+constexpr int kPad = 64;             // zero bytes after the text copy
+constexpr int kTrigPerThread = 16;   // window ends per thread
+constexpr int kTrigTile = kBlock * kTrigPerThread;
+constexpr int kMaxW = 64;
+
+// --------------------------------------------------------------------- helpers
+
+// 8 bytes of the padded text starting at any position p (little endian).
+__device__ __forceinline__ uint64_t load8(const uint8_t* __restrict__ T, uint64_t p)
+{
+    const uint64_t a = p & ~7ull;
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(T + a);
+    const uint64_t w0 = w[0];
+    const int sh = (int)(p & 7) * 8;
+    if (sh == 0) return w0;
+    const uint64_t w1 = w[1];
+    return (w0 >> sh) | (w1 << (64 - sh));
+}
+
+__device__ __forceinline__ uint64_t mix64(uint64_t h)
+{
+    h ^= h >> 33;
+    h *= 0xff51afd7ed558ccdull;
+    h ^= h >> 33;
+    h *= 0xc4ceb9fe1a85ec53ull;
+    h ^= h >> 33;
+    return h;
+}
+
+// Phrase k of the parse: [start, end] in the extended text, end >= n means sentinels.
+__device__ __forceinline__ void phrase_bounds(const uint32_t* __restrict__ E, uint32_t k, int W,
+                                              uint32_t& start, uint32_t& end)
+{
+    start = (k == 0) ? 0u : E[k - 1] - (uint32_t)W + 1u;
+    end = E[k];
+}
+
+// Fingerprint of the phrase [s, e] (virtual terminator beyond n).
+__device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint64_t n, uint32_t s, uint32_t e,
+                                uint64_t seed)
+{
+    const uint32_t len = e - s + 1;
+    const uint32_t text_end = (e < n) ? e + 1 : (uint32_t)n;  // exclusive
+    uint64_t h = seed ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
+    for (uint32_t p = s; p < text_end; p += 8) {
+        uint64_t w = load8(T, p);
+        const uint32_t c = text_end - p;
+        if (c < 8) w &= (~0ull) >> (64 - 8 * c);
+        h = mix64(h ^ (w + 0x2545F4914F6CDD1Dull * (p - s + 1)));
+    }
+    if (e >= n) h = mix64(h ^ 0xA24BAED4963EE407ull ^ (uint64_t)(e - n + 1));
+    return h;
+}
+
+__device__ bool phrase_equal(const uint8_t* __restrict__ T, uint64_t n, uint32_t s1, uint32_t e1,
+                             uint32_t s2, uint32_t e2)
+{
+    if (e1 - s1 != e2 - s2) return false;
+    const bool x1 = e1 >= n, x2 = e2 >= n;
+    if (x1 != x2) return false;  // only the last phrase reaches into the terminator
+    const uint32_t len = (x1 ? (uint32_t)n - s1 : e1 - s1 + 1);
+    if (x1 && (uint32_t)n - s2 != len) return false;
+    for (uint32_t i = 0; i < len; i += 8) {
+        uint64_t a = load8(T, s1 + i), b = load8(T, s2 + i);
+        const uint32_t c = len - i;
+        if (c < 8) {
+            const uint64_t m = (~0ull) >> (64 - 8 * c);
+            a &= m;
+            b &= m;
+        }
+        if (a != b) return false;
+    }
+    return true;
+}
+
+// --------------------------------------------------------------------- 1. trigger scan
+
+struct TrigTabs {
+    uint16_t in_tab[256];   // c mod P
+    uint16_t out_tab[256];  // -(c * 2^W) mod P
+};
+
+__device__ __forceinline__ uint32_t modP(uint32_t x, uint32_t P)
+{
+    // x < 4P
+    x = (x >= 2 * P) ? x - 2 * P : x;
+    x = (x >= P) ? x - P : x;
+    return x;
+}
+
+// Flags of the 16 window ends [e0, e0+16) of one thread as a 16-bit mask; LDS `txt` holds
+// the tile's text starting at tile_base - kMaxW.
+__device__ __forceinline__ uint32_t trig_mask16(const uint8_t* txt, const uint8_t* first,
+                                                const TrigTabs& tb, uint32_t h0, int W, uint32_t P,
+                                                uint64_t e0, uint64_t n, uint32_t loc)
+{
+    // loc = local index of e0 in txt
+    uint32_t h = 0;
+    for (int k = 0; k < W; k++) h = modP(2 * h + tb.in_tab[txt[loc - W + 1 + k]], P);
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < kTrigPerThread; i++) {
+        if (i > 0) {
+            const uint32_t l = loc + i;
+            h = modP(2 * h + tb.in_tab[txt[l]] + tb.out_tab[txt[l - W]], P);
+        }
+        const uint64_t e = e0 + i;
+        bool trig = (h == 0);
+        if (!trig && h == h0) {
+            bool eq = true;
+            for (int k = 0; k < W && eq; k++) eq = (txt[loc + i - W + 1 + k] == first[k]);
+            trig = eq;
+        }
+        if (trig && e >= (uint64_t)W && e < n) mask |= 1u << i;
+    }
+    return mask;
+}
+
+__device__ void trig_load_tile(const uint8_t* __restrict__ T, uint64_t n, uint64_t tile_base,
+                               uint8_t* txt, uint8_t* first, TrigTabs& tb, uint32_t& h0_out, int W,
+                               uint32_t P, uint32_t* h0_lds)
+{
+    // text [tile_base - kMaxW, tile_base + kTrigTile) in 16-byte pieces
+    const int pieces = (kTrigTile + kMaxW) / 16;
+    for (int q = threadIdx.x; q < pieces; q += kBlock) {
+        const int64_t g = (int64_t)tile_base - kMaxW + (int64_t)q * 16;
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (g >= 0 && (uint64_t)g < n + kPad - 16) v = *reinterpret_cast<const uint4*>(T + g);
+        *reinterpret_cast<uint4*>(txt + q * 16) = v;
+    }
+    if (threadIdx.x < kMaxW) first[threadIdx.x] = T[threadIdx.x];
+    {
+        const uint32_t c = threadIdx.x;  // kBlock == 256
+        uint32_t p2w = 1 % P;
+        for (int k = 0; k < W; k++) p2w = (p2w * 2) % P;
+        tb.in_tab[c] = (uint16_t)(c % P);
+        tb.out_tab[c] = (uint16_t)((P - (c * p2w) % P) % P);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t h = 0;
+        for (int k = 0; k < W; k++) h = modP(2 * h + tb.in_tab[first[k]], P);
+        *h0_lds = h;
+    }
+    __syncthreads();
+    h0_out = *h0_lds;
+}
+
+// Pass 1: per-thread 16-bit trigger masks and per-tile counts.
+__global__ __launch_bounds__(kBlock) void k_trig_mask(const uint8_t* __restrict__ T, uint64_t n, int W,
+                                                     uint32_t P, uint16_t* __restrict__ masks,
+                                                     uint32_t* __restrict__ tile_cnt)
+{
+    __shared__ __align__(16) uint8_t txt[kTrigTile + kMaxW];
+    __shared__ uint8_t first[kMaxW];
+    __shared__ TrigTabs tb;
+    __shared__ uint32_t h0s;
+    __shared__ uint32_t wsum[kWaves];
+    const uint64_t tile_base = (uint64_t)blockIdx.x * kTrigTile;
+    uint32_t h0;
+    trig_load_tile(T, n, tile_base, txt, first, tb, h0, W, P, &h0s);
+    const uint32_t loc = kMaxW + threadIdx.x * kTrigPerThread;
+    const uint32_t m = trig_mask16(txt, first, tb, h0, W, P, tile_base + threadIdx.x * kTrigPerThread, n, loc);
+    masks[(uint64_t)blockIdx.x * kBlock + threadIdx.x] = (uint16_t)m;
+    uint32_t tot;
+    (void)block_exclusive_scan((uint32_t)__popc(m), 0u, OpSum{}, wsum, &tot);
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = tot;
+}
+
+// Pass 2: ordered emit of trigger end positions.
+__global__ __launch_bounds__(kBlock) void k_trig_emit(const uint16_t* __restrict__ masks,
+                                                     const uint32_t* __restrict__ tile_off,
+                                                     uint32_t* __restrict__ E)
+{
+    __shared__ uint32_t wsum[kWaves];
+    const uint32_t m = masks[(uint64_t)blockIdx.x * kBlock + threadIdx.x];
+    uint32_t tot;
+    uint32_t off = block_exclusive_scan((uint32_t)__popc(m), 0u, OpSum{}, wsum, &tot) + tile_off[blockIdx.x];
+    const uint32_t e0 = blockIdx.x * kTrigTile + threadIdx.x * kTrigPerThread;
+    uint32_t mm = m;
+    while (mm) {
+        const int b = __ffs(mm) - 1;
+        E[off++] = e0 + b;
+        mm &= mm - 1;
+    }
+}
+
+// --------------------------------------------------------------------- 2/3. dictionary
+
+__global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
+                              const uint32_t* __restrict__ E, uint32_t d, uint64_t seed,
+                              uint64_t* __restrict__ H, uint32_t* __restrict__ V)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    uint32_t s, e;
+    phrase_bounds(E, k, W, s, e);
+    H[k] = phrase_hash(T, n, s, e, seed);
+    V[k] = k;
+}
+
+__global__ void k_dict_mark(const uint8_t* __restrict__ T, uint64_t n, int W,
+                            const uint32_t* __restrict__ E, const uint64_t* __restrict__ H,
+                            const uint32_t* __restrict__ V, uint32_t d, uint32_t* __restrict__ flag,
+                            uint32_t* __restrict__ collision)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    uint32_t f = 1;
+    if (j > 0 && H[j] == H[j - 1]) {
+        f = 0;
+        uint32_t s1, e1, s2, e2;
+        phrase_bounds(E, V[j], W, s1, e1);
+        phrase_bounds(E, V[j - 1], W, s2, e2);
+        if (!phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
+    }
+    flag[j] = f;
+}
+
+// ids[j] = inclusive count of heads (1-based id of the distinct phrase at sorted position j)
+__global__ void k_dict_assign(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ V,
+                              const uint32_t* __restrict__ flag, const uint32_t* __restrict__ E,
+                              uint32_t d, int W, uint32_t* __restrict__ pid,
+                              uint32_t* __restrict__ head, uint32_t* __restrict__ du_start,
+                              uint32_t* __restrict__ du_len)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= d) return;
+    const uint32_t u = ids[j] - 1;
+    const uint32_t k = V[j];
+    pid[k] = u;
+    if (flag[j]) {
+        uint32_t s, e;
+        phrase_bounds(E, k, W, s, e);
+        head[u] = j;
+        du_start[u] = s;
+        du_len[u] = e - s + 1;
+    }
+}
+
+__global__ void k_dict_freq(const uint32_t* __restrict__ head, uint32_t du, uint32_t d,
+                            uint32_t* __restrict__ freq)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= du) return;
+    freq[u] = ((u + 1 < du) ? head[u + 1] : d) - head[u];
+}
+
+// --------------------------------------------------------------------- 4/5. prefix doubling
+
+// Segment limits of a string family.  Dictionary: the suffix at DT position x ends at the
+// end of its phrase.  Parse: one string of length d.
+struct LimDict {
+    const uint32_t* owner;
+    const uint64_t* off;
+    const uint32_t* len;
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const
+    {
+        const uint32_t u = owner[x];
+        return (uint32_t)off[u] + len[u];
+    }
+};
+struct LimParse {
+    uint32_t d;
+    __device__ __forceinline__ uint32_t operator()(uint32_t) const { return d; }
+};
+
+__global__ void k_dt_owner(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                           uint32_t du, uint32_t* __restrict__ owner)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= du) return;
+    const uint64_t o = off[u];
+    for (uint32_t j = 0; j < len[u]; j++) owner[o + j] = u;
+}
+
+// Initial key of DT position x: 7 symbols of 9 bits (0 = past the phrase end,
+// 1 = terminator, byte + 2), most significant first.
+__global__ void k_dt_initkey(const uint8_t* __restrict__ T, uint64_t n,
+                             const uint32_t* __restrict__ owner, const uint64_t* __restrict__ off,
+                             const uint32_t* __restrict__ len, const uint32_t* __restrict__ start,
+                             uint32_t L, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L) return;
+    const uint32_t u = owner[x];
+    const uint32_t j = x - (uint32_t)off[u];
+    const uint32_t rem = len[u] - j;
+    const uint64_t tp = (uint64_t)start[u] + j;
+    const uint64_t w = (tp < n) ? load8(T, tp) : 0ull;
+    uint64_t key = 0;
+#pragma unroll
+    for (int i = 0; i < 7; i++) {
+        uint64_t s;
+        if ((uint32_t)i >= rem) s = 0;
+        else if (tp + i >= n) s = 1;
+        else s = ((w >> (8 * i)) & 0xFF) + 2;
+        key = (key << 9) | s;
+    }
+    K[x] = key;
+    V[x] = x;
+}
+
+__global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid,
+                                uint32_t d, int B, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
+                                uint32_t* __restrict__ par)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    const uint64_t a = lexrank[pid[k]] + 1;
+    const uint64_t b = (k + 1 < d) ? (uint64_t)lexrank[pid[k + 1]] + 1 : 0;
+    K[k] = (a << B) | b;
+    V[k] = k;
+    par[k] = (uint32_t)a;
+}
+
+// Rank update after a sort of the active set A (keys K, elements V, global positions G,
+// all of length m).  head index via hidx (inclusive max scan of head flags * i).
+__global__ void k_dbl_headflag(const uint64_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    hf[i] = (i == 0 || K[i] != K[i - 1]) ? i : 0;
+}
+
+template <typename Lim>
+__global__ void k_dbl_update(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V,
+                             const uint32_t* __restrict__ G, const uint32_t* __restrict__ hidx,
+                             uint32_t m, uint32_t* __restrict__ rank, uint32_t* __restrict__ SA,
+                             uint32_t* __restrict__ keep, uint32_t h_cov, Lim lim)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t x = V[i];
+    const uint32_t g = G[i];
+    const uint32_t hi = hidx[i];
+    rank[x] = G[hi] + 1;
+    SA[g] = x;
+    // the group [hi, next head) stays active if it has >= 2 members and its strings are
+    // not yet known to be equal: the compared prefix of length h_cov ends inside them
+    const bool single = (hi == i) && (i + 1 == m || K[i + 1] != K[i]);
+    bool active = !single;
+    if (active) {
+        const uint32_t xh = V[hi];
+        const uint32_t rem = lim(xh) - xh;
+        active = rem >= h_cov;
+    }
+    keep[i] = active ? 1u : 0u;
+}
+
+__global__ void k_dbl_compact(const uint32_t* __restrict__ V, const uint32_t* __restrict__ G,
+                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                              uint32_t m, uint32_t* __restrict__ V2, uint32_t* __restrict__ G2)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    if (keep[i]) {
+        V2[pos[i]] = V[i];
+        G2[pos[i]] = G[i];
+    }
+}
+
+template <typename Lim>
+__global__ void k_dbl_key(const uint32_t* __restrict__ V, uint32_t m, const uint32_t* __restrict__ rank,
+                          uint32_t h, int B, Lim lim, uint64_t* __restrict__ K)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t x = V[i];
+    const uint64_t r1 = rank[x];
+    const uint32_t lx = lim(x);
+    const uint64_t r2 = ((uint64_t)x + h < lx) ? rank[x + h] : 0;
+    K[i] = (r1 << B) | r2;
+}
+
+__global__ void k_iota(uint32_t* __restrict__ p, uint32_t m)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) p[i] = i;
+}
+
+struct DoublingResult {
+    int rounds = 0;
+    double bytes = 0;
+};
+
+// Prefix doubling over N suffix positions of a family of strings.  On entry K0/V0 hold the
+// initial (key, position) pairs whose keys (bits [0, key_bits)) order the suffixes by their
+// first h0 symbols, padded with a terminator smaller than every symbol.  On exit
+// rank[x] = 1 + position of the head of x's group in the sorted order and SA[] is the
+// sorted order; suffixes with identical strings end in one group.  A group stops being
+// refined when it is a singleton or when its strings are shorter than the prefix length
+// already compared (then they are equal).
+template <typename Lim>
+DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t* V0, uint32_t* V1,
+                               uint32_t N, int key_bits, uint32_t h0, Lim lim, uint32_t* rank,
+                               uint32_t* SA)
+{
+    DoublingResult res;
+    uint32_t* G0 = S.buf<uint32_t>(B_ACT_POS, N);
+    uint32_t* G1 = S.buf<uint32_t>(B_ACT_POS2, N);
+    uint32_t* V2 = S.buf<uint32_t>(B_ACT_X2, N);
+    uint32_t* hidx = S.buf<uint32_t>(B_ACT_HEAD, N);
+    uint32_t* keep = S.buf<uint32_t>(B_ACT_FLAG, N);
+    uint32_t* pos = S.buf<uint32_t>(B_ACT_X, N);
+    uint32_t* cnt_dev = S.buf<uint32_t>(B_SCALARS, SC_COUNT_) + SC_ACTIVE;
+
+    uint32_t m = N;
+    uint32_t h = h0;
+    int bits = key_bits;
+    k_iota<<<grid_for(m), kBlock, 0, S.stream>>>(G0, m);
+    MIBWT_LAUNCH_CHECK();
+    const int B = bits_for((uint64_t)N + 1);
+    for (;;) {
+        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, m, 0, bits);
+        res.bytes += (double)m * (8 + 4) * 2 * ((bits + 7) / 8);
+        k_dbl_headflag<<<grid_for(m), kBlock, 0, S.stream>>>(K0, m, hidx);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_max<uint32_t>(S, hidx, hidx, m);
+        k_dbl_update<Lim><<<grid_for(m), kBlock, 0, S.stream>>>(K0, V0, G0, hidx, m, rank, SA, keep,
+                                                                 h, lim);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, keep, pos, m, cnt_dev);
+        uint32_t m2 = 0;
+        S.read(&m2, cnt_dev);
+        res.rounds++;
+        if (m2 == 0) break;
+        k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(V0, G0, keep, pos, m, V2, G1);
+        MIBWT_LAUNCH_CHECK();
+        std::swap(G0, G1);
+        std::swap(V0, V2);
+        m = m2;
+        k_dbl_key<Lim><<<grid_for(m), kBlock, 0, S.stream>>>(V0, m, rank, h, B, lim, K0);
+        MIBWT_LAUNCH_CHECK();
+        res.bytes += (double)m * (4 + 4 + 4 + 8) + (double)m * 16;
+        bits = 2 * B;
+        h = 2 * h;
+    }
+    return res;
+}
+
+// --------------------------------------------------------------------- 5. phrase order
+
+__global__ void k_phrase_start_flag(const uint32_t* __restrict__ SA, const uint32_t* __restrict__ owner,
+                                    const uint64_t* __restrict__ off, uint32_t L, uint32_t* __restrict__ F)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= L) return;
+    const uint32_t x = SA[g];
+    F[g] = (x == (uint32_t)off[owner[x]]) ? 1u : 0u;
+}
+
+__global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __restrict__ off,
+                          const uint32_t* __restrict__ Fs, uint32_t du, uint32_t* __restrict__ lexrank)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= du) return;
+    lexrank[u] = Fs[rank[(uint32_t)off[u]] - 1];
+}
+
+// --------------------------------------------------------------------- 6. BWT assembly
+
+struct DictView {
+    const uint32_t* SA;     // sorted DT positions
+    const uint32_t* rank;   // 1 + group head position
+    const uint32_t* owner;  // distinct phrase of a DT position
+    const uint64_t* off;    // first DT position of a distinct phrase
+    const uint32_t* len;    // phrase length
+    const uint32_t* start;  // text position of the representative occurrence
+    const uint32_t* freq;   // occurrences of the phrase
+    const uint8_t* T;
+    uint32_t L;
+    int W;
+    uint64_t n;
+};
+
+// character preceding the suffix at DT position x (j >= 1)
+__device__ __forceinline__ uint8_t dv_prev(const DictView& v, uint32_t x, uint32_t u, uint32_t j)
+{
+    return v.T[v.start[u] + j - 1];
+}
+
+// BWT slot counts per sorted position and the "mixed" mark of groups whose preceding
+// characters differ.
+__global__ void k_asm_count(DictView v, uint32_t* __restrict__ cnt, uint8_t* __restrict__ gmix)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= v.L) return;
+    const uint32_t x = v.SA[g];
+    const uint32_t u = v.owner[x];
+    const uint32_t j = x - (uint32_t)v.off[u];
+    const uint32_t rem = v.len[u] - j;
+    if (rem <= (uint32_t)v.W) {
+        cnt[g] = 0;
+        return;
+    }
+    cnt[g] = v.freq[u];
+    if (j >= 1) {
+        const uint32_t gh = v.rank[x] - 1;
+        if (gh != g) {
+            const uint32_t xh = v.SA[gh];
+            const uint32_t uh = v.owner[xh];
+            const uint32_t jh = xh - (uint32_t)v.off[uh];
+            if (dv_prev(v, x, u, j) != dv_prev(v, xh, uh, jh)) gmix[gh] = 1;
+        }
+    }
+}
+
+// slots written through the ordered entries: whole phrases and mixed groups
+__global__ void k_asm_ecnt(DictView v, const uint8_t* __restrict__ gmix, uint32_t* __restrict__ ecnt)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= v.L) return;
+    const uint32_t x = v.SA[g];
+    const uint32_t u = v.owner[x];
+    const uint32_t j = x - (uint32_t)v.off[u];
+    const uint32_t rem = v.len[u] - j;
+    uint32_t c = 0;
+    if (rem > (uint32_t)v.W && (j == 0 || gmix[v.rank[x] - 1])) c = v.freq[u];
+    ecnt[g] = c;
+}
+
+__global__ void k_asm_delta(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ escan,
+                            uint32_t L, uint32_t* __restrict__ delta)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= L) return;
+    delta[g] = goff[g] - escan[g];
+}
+
+// direct fill of the single-character groups: every member writes its own freq slots
+__global__ void k_asm_fill(DictView v, const uint8_t* __restrict__ gmix, const uint32_t* __restrict__ goff,
+                           uint8_t* __restrict__ O)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= v.L) return;
+    const uint32_t x = v.SA[g];
+    const uint32_t u = v.owner[x];
+    const uint32_t j = x - (uint32_t)v.off[u];
+    const uint32_t rem = v.len[u] - j;
+    if (rem <= (uint32_t)v.W || j == 0) return;
+    if (gmix[v.rank[x] - 1]) return;
+    const uint8_t c = dv_prev(v, x, u, j);
+    const uint64_t o = goff[g];
+    const uint64_t f = v.freq[u];
+    const uint64_t e = (o + f <= v.n) ? o + f : v.n;  // guard: never write past the output
+    for (uint64_t i = o; i < e; i++) O[i] = c;
+}
+
+// mixed-group members in DT order (grouped by phrase)
+__global__ void k_mx_flag(DictView v, const uint8_t* __restrict__ gmix, uint32_t* __restrict__ mxf)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x > v.L) return;
+    if (x == v.L) {
+        mxf[x] = 0;
+        return;
+    }
+    const uint32_t u = v.owner[x];
+    const uint32_t j = x - (uint32_t)v.off[u];
+    const uint32_t rem = v.len[u] - j;
+    mxf[x] = (rem > (uint32_t)v.W && j >= 1 && gmix[v.rank[x] - 1]) ? 1u : 0u;
+}
+
+__global__ void k_mx_write(DictView v, const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
+                           uint64_t* __restrict__ mx)
+{
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= v.L || !mxf[x]) return;
+    const uint32_t u = v.owner[x];
+    const uint32_t j = x - (uint32_t)v.off[u];
+    mx[mxs[x]] = ((uint64_t)(v.rank[x] - 1) << 9) | dv_prev(v, x, u, j);
+}
+
+// entries per parse-suffix rank r: the occurrence k = psa[r]-1 contributes its whole-phrase
+// entry and one entry per mixed suffix of its phrase
+__global__ void k_q_count(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
+                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                          const uint32_t* __restrict__ mxs, uint32_t* __restrict__ qc)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d) return;
+    const uint32_t k1 = psa[r];
+    if (k1 == 0) {
+        qc[r] = 0;
+        return;
+    }
+    const uint32_t u = pid[k1 - 1];
+    const uint32_t o = (uint32_t)off[u];
+    qc[r] = 1 + mxs[o + len[u]] - mxs[o];
+}
+
+__device__ __forceinline__ uint64_t whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
+                                                       uint64_t n, int W, uint32_t k, uint32_t alpha0)
+{
+    uint32_t s, e;
+    phrase_bounds(E, k, W, s, e);
+    const uint8_t c = (s > 0) ? T[s - 1] : T[n - 1];
+    return ((uint64_t)alpha0 << 9) | ((k == 0) ? 0x100ull : 0ull) | c;
+}
+
+__global__ void k_q_emit(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
+                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                         const uint32_t* __restrict__ mxs, const uint64_t* __restrict__ mx,
+                         const uint32_t* __restrict__ rank, const uint32_t* __restrict__ E,
+                         const uint8_t* __restrict__ T, uint64_t n, int W,
+                         const uint32_t* __restrict__ qo, uint64_t* __restrict__ ent)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d) return;
+    const uint32_t k1 = psa[r];
+    if (k1 == 0) return;
+    const uint32_t k = k1 - 1;
+    const uint32_t u = pid[k];
+    const uint32_t o = (uint32_t)off[u];
+    uint32_t q = qo[r];
+    ent[q++] = whole_phrase_entry(E, T, n, W, k, rank[o] - 1);
+    const uint32_t a = mxs[o], b = mxs[o + len[u]];
+    for (uint32_t i = a; i < b; i++) ent[q++] = mx[i];
+}
+
+// the last occurrence has no next parse suffix; its whole-phrase group is a singleton
+__global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const uint64_t* __restrict__ off,
+                         const uint32_t* __restrict__ rank, const uint32_t* __restrict__ E,
+                         const uint8_t* __restrict__ T, uint64_t n, int W, uint32_t pos,
+                         uint64_t* __restrict__ ent)
+{
+    const uint32_t k = d - 1;
+    const uint32_t u = pid[k];
+    ent[pos] = whole_phrase_entry(E, T, n, W, k, rank[(uint32_t)off[u]] - 1);
+}
+
+// last radix pass of the entry sort: slot = sorted position + delta(alpha)
+struct EmitBwt {
+    const uint32_t* delta;
+    uint8_t* O;
+    uint32_t* pidx;
+    uint64_t n;
+    __device__ __forceinline__ void operator()(uint64_t pos, uint64_t key, uint32_t) const
+    {
+        const uint64_t slot = pos + delta[(uint32_t)(key >> 9)];
+        if (slot >= n) return;  // guard: a slot past the output is a bug caught by the checks
+        O[slot] = (uint8_t)(key & 0xFF);
+        if (key & 0x100) *pidx = (uint32_t)slot + 1;
+    }
+};
+
+__global__ void k_u32_to_u64(const uint32_t* __restrict__ a, uint64_t* __restrict__ b, uint32_t m)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) b[i] = a[i];
+    if (i == m) b[i] = 0;
+}
+
+// --------------------------------------------------------------------- driver
+
+int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    MIBWT_REQUIRE(W >= 1 && W <= kMaxW && P >= 1 && P <= 65535, LIBCUBWT_BAD_PARAMETER, "bad W/P");
+    const uint64_t n = (uint64_t)n64;
+    S.phases_reset();
+    S.stats.n = n64;
+    S.stats.W = W;
+    S.stats.P = P;
+    if (!S.t_begin) {
+        MIBWT_HIP(hipEventCreate(&S.t_begin));
+        MIBWT_HIP(hipEventCreate(&S.t_end));
+    }
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    const uint32_t tiles = ceil_div(n + 1, kTrigTile);
+    const uint64_t text_bytes = (uint64_t)tiles * kTrigTile + kPad;
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
+
+    // the text: a padded private copy (zero tail so that 16-byte tile loads and 8-byte word
+    // loads past the end are in bounds)
+    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
+    if (src_host) {
+        MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
+    }
+    MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    if (!src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyDeviceToDevice, S.stream));
+
+    // ---- 1. trigger scan
+    int ph = S.phase_begin("scan");
+    uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
+    uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
+    k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, (uint32_t)P, masks, tcnt);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, tcnt, tcnt, tiles, sc + SC_TRIGGERS);
+    uint32_t t = 0;
+    S.read(&t, sc + SC_TRIGGERS);
+    const uint32_t d = t + 1;
+    uint32_t* E = S.buf<uint32_t>(B_TRIG, d);
+    k_trig_emit<<<tiles, kBlock, 0, S.stream>>>(masks, tcnt, E);
+    MIBWT_LAUNCH_CHECK();
+    fill<uint32_t>(S, E + t, 1, (uint32_t)(n + W - 1));
+    S.phase_end(ph, (double)n + 2.0 * tiles * kBlock * 2 + 4.0 * d);
+    S.stats.triggers = t;
+    S.stats.phrases = d;
+
+    // ---- 2/3. phrase fingerprints and the dictionary
+    ph = S.phase_begin("dict");
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, d);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, d);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, d);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, d);
+    uint32_t* flag = S.buf<uint32_t>(B_PH_FLAG, d);
+    uint32_t du = 0;
+    uint64_t seed = 0x6A09E667F3BCC909ull;
+    for (int attempt = 0;; attempt++) {
+        MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
+        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, K0, V0);
+        MIBWT_LAUNCH_CHECK();
+        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, 64);
+        fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+        k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, K0, V0, d, flag, sc + SC_COLLISION);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_sum<uint32_t>(S, flag, K1 ? reinterpret_cast<uint32_t*>(K1) : nullptr, d, sc + SC_DISTINCT);
+        uint32_t hv[SC_COUNT_];
+        S.read(hv, sc, SC_COUNT_);
+        du = hv[SC_DISTINCT];
+        if (hv[SC_COLLISION] == 0) break;
+        seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+    }
+    const uint32_t* ids = reinterpret_cast<const uint32_t*>(K1);
+    uint32_t* pid = S.buf<uint32_t>(B_PH_PID, d);
+    uint32_t* head = S.buf<uint32_t>(B_DU_HEAD, du);
+    uint32_t* dstart = S.buf<uint32_t>(B_DU_START, du);
+    uint32_t* dlen = S.buf<uint32_t>(B_DU_LEN, du);
+    uint32_t* freq = S.buf<uint32_t>(B_DU_FREQ, du);
+    k_dict_assign<<<grid_for(d), kBlock, 0, S.stream>>>(ids, V0, flag, E, d, W, pid, head, dstart, dlen);
+    MIBWT_LAUNCH_CHECK();
+    k_dict_freq<<<grid_for(du), kBlock, 0, S.stream>>>(head, du, d, freq);
+    MIBWT_LAUNCH_CHECK();
+    uint64_t* dlen64 = S.buf<uint64_t>(B_DU_LEN64, du + 1);
+    uint64_t* doff = S.buf<uint64_t>(B_DU_OFF, du + 1);
+    k_u32_to_u64<<<grid_for(du + 1), kBlock, 0, S.stream>>>(dlen, dlen64, du);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint64_t>(S, dlen64, doff, du + 1);
+    uint64_t L64 = 0;
+    S.read(&L64, doff + du);
+    MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "dictionary too large");
+    const uint32_t L = (uint32_t)L64;
+    S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 24.0 * d * 8 + 12.0 * d);
+    S.stats.distinct_phrases = du;
+    S.stats.dict_length = L;
+
+    // ---- 4. dictionary suffix sort
+    ph = S.phase_begin("dsort");
+    uint32_t* owner = S.buf<uint32_t>(B_DT_OWNER, L);
+    uint32_t* drank = S.buf<uint32_t>(B_DT_RANK, L);
+    uint32_t* dsa = S.buf<uint32_t>(B_DT_SA, L);
+    k_dt_owner<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, du, owner);
+    MIBWT_LAUNCH_CHECK();
+    K0 = S.buf<uint64_t>(B_SORT_K0, L);
+    K1 = S.buf<uint64_t>(B_SORT_K1, L);
+    V0 = S.buf<uint32_t>(B_SORT_V0, L);
+    V1 = S.buf<uint32_t>(B_SORT_V1, L);
+    k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, L, K0, V0);
+    MIBWT_LAUNCH_CHECK();
+    LimDict limd{owner, doff, dlen};
+    DoublingResult rd = prefix_doubling<LimDict>(S, K0, K1, V0, V1, L, 63, 7, limd, drank, dsa);
+    S.phase_end(ph, rd.bytes);
+    S.stats.doubling_rounds_dict = rd.rounds;
+
+    // ---- 5. phrase order and parse suffix array
+    ph = S.phase_begin("psort");
+    uint32_t* F = S.buf<uint32_t>(B_G_ECNT, L);
+    k_phrase_start_flag<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, owner, doff, L, F);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, F, F, L);
+    uint32_t* lexrank = S.buf<uint32_t>(B_LEXRANK, du);
+    k_lexrank<<<grid_for(du), kBlock, 0, S.stream>>>(drank, doff, F, du, lexrank);
+    MIBWT_LAUNCH_CHECK();
+    K0 = S.buf<uint64_t>(B_SORT_K0, d);
+    K1 = S.buf<uint64_t>(B_SORT_K1, d);
+    V0 = S.buf<uint32_t>(B_SORT_V0, d);
+    V1 = S.buf<uint32_t>(B_SORT_V1, d);
+    uint32_t* par = S.buf<uint32_t>(B_Q_CNT, d);
+    const int B0 = bits_for((uint64_t)du + 1);
+    k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, K0, V0, par);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);
+    uint32_t* psa = S.buf<uint32_t>(B_P_SA, d);
+    DoublingResult rp = prefix_doubling<LimParse>(S, K0, K1, V0, V1, d, 2 * B0, 2, LimParse{d}, prank, psa);
+    S.phase_end(ph, rp.bytes);
+    S.stats.doubling_rounds_parse = rp.rounds;
+
+    // ---- 6. BWT assembly
+    ph = S.phase_begin("assemble");
+    DictView v{dsa, drank, owner, doff, dlen, dstart, freq, T, L, W, n};
+    uint32_t* goff = S.buf<uint32_t>(B_G_OFF, L);
+    uint8_t* gmix = S.buf<uint8_t>(B_G_MIX, L);
+    MIBWT_HIP(hipMemsetAsync(gmix, 0, L, S.stream));
+    k_asm_count<<<grid_for(L), kBlock, 0, S.stream>>>(v, goff, gmix);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
+    uint32_t slots = 0;
+    S.read(&slots, sc + SC_TOTAL);
+    MIBWT_REQUIRE(slots == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
+    uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
+    k_asm_ecnt<<<grid_for(L), kBlock, 0, S.stream>>>(v, gmix, ecnt);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
+    uint32_t* delta = S.buf<uint32_t>(B_G_DELTA, L);
+    k_asm_delta<<<grid_for(L), kBlock, 0, S.stream>>>(goff, ecnt, L, delta);
+    MIBWT_LAUNCH_CHECK();
+    k_asm_fill<<<grid_for(L), kBlock, 0, S.stream>>>(v, gmix, goff, dO);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
+    uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(v, gmix, mxf);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
+    uint32_t nmx = 0;
+    S.read(&nmx, sc + SC_ENTRIES);
+    uint64_t* mx = S.buf<uint64_t>(B_MX_ENT, nmx + 1);
+    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(v, mxf, mxs, mx);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* qc = S.buf<uint32_t>(B_Q_CNT, d);
+    uint32_t* qo = S.buf<uint32_t>(B_Q_OFF, d);
+    k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, doff, dlen, mxs, qc);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
+    uint32_t hv[SC_COUNT_];
+    S.read(hv, sc, SC_COUNT_);
+    const uint32_t ne = hv[SC_ENTRIES] + 1;
+    uint64_t* ent0 = S.buf<uint64_t>(B_SORT_K0, ne);
+    uint64_t* ent1 = S.buf<uint64_t>(B_SORT_K1, ne);
+    k_q_emit<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, doff, dlen, mxs, mx, drank, E, T, n, W, qo,
+                                                   ent0);
+    MIBWT_LAUNCH_CHECK();
+    k_q_last<<<1, 1, 0, S.stream>>>(pid, d, doff, drank, E, T, n, W, ne - 1, ent0);
+    MIBWT_LAUNCH_CHECK();
+    fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
+    EmitBwt em{delta, dO, sc + SC_PIDX, n};
+    const int abits = bits_for(L);
+    radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + abits, &em);
+    S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + 8.0 * ne * 2 * ((abits + 7) / 8) + 4.0 * d * 3);
+    S.stats.alpha_groups = L;
+    S.stats.mixed_entries = ne;
+
+    MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
+    uint32_t pidx = 0;
+    S.read(&pidx, sc + SC_PIDX);
+    S.phases_collect();
+    float tot = 0;
+    MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
+    S.stats.total_ms = tot;
+    MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
+    S.last_text = src_host ? nullptr : src;
+    S.last_n = n64;
+    S.last_W = W;
+    S.last_P = P;
+    S.last_sort_items = d;
+    return (int64_t)pidx;
+}
+
+// Re-run the trigger scan kernel (pass 1) `reps` times on the text of the last call.
+double pfp_time_scan(Storage& S, int reps, double* bytes)
+{
+    const uint64_t n = (uint64_t)S.last_n;
+    const uint32_t tiles = ceil_div(n + 1, kTrigTile);
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, (uint64_t)tiles * kTrigTile + kPad);
+    uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
+    uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
+    hipEvent_t a, b;
+    MIBWT_HIP(hipEventCreate(&a));
+    MIBWT_HIP(hipEventCreate(&b));
+    k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, S.last_W, (uint32_t)S.last_P, masks, tcnt);
+    MIBWT_HIP(hipEventRecord(a, S.stream));
+    for (int i = 0; i < reps; i++)
+        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, S.last_W, (uint32_t)S.last_P, masks, tcnt);
+    MIBWT_HIP(hipEventRecord(b, S.stream));
+    MIBWT_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    MIBWT_HIP(hipEventElapsedTime(&ms, a, b));
+    MIBWT_HIP(hipEventDestroy(a));
+    MIBWT_HIP(hipEventDestroy(b));
+    *bytes = (double)n + 2.0 * tiles * kBlock + 4.0 * tiles;
+    return ms / reps;
+}
+
+}  // namespace mibwt

@@ -1,0 +1,94 @@
+"""Seeded synthetic texts for the BWT-build workloads (numpy, deterministic).
+
+Repetitive DNA collections: a uniform A/C/G/T base replicated `copies` times, each copy with
+independent substitutions (to a different base) and insertions/deletions (50/50, length
+uniform in [1, indel_max]).  Low-repetition control: an order-2 Markov chain over the 96
+printable bytes 32..127 with Dirichlet(0.5) transition rows.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+ACGT = np.frombuffer(b"ACGT", dtype=np.uint8)
+SEED_BASE = 0xB17C0DE
+
+
+def mutate(rng: np.random.Generator, base_idx: np.ndarray, sub_rate: float, indel_rate: float,
+           indel_max: int) -> np.ndarray:
+    """One mutated copy of a base given as symbol indices 0..3; returns indices."""
+    x = base_idx.copy()
+    if sub_rate > 0:
+        hit = np.flatnonzero(rng.random(len(x)) < sub_rate)
+        x[hit] = (x[hit] + rng.integers(1, 4, size=len(hit))) % 4
+    if indel_rate <= 0:
+        return x
+    nev = rng.binomial(len(x), indel_rate)
+    if nev == 0:
+        return x
+    pos = np.sort(rng.integers(0, len(x), size=nev))
+    kind = rng.random(nev) < 0.5
+    lens = rng.integers(1, indel_max + 1, size=nev)
+    pieces = []
+    cur = 0
+    for p, ins, ln in zip(pos.tolist(), kind.tolist(), lens.tolist()):
+        if p < cur:
+            continue
+        pieces.append(x[cur:p])
+        if ins:
+            pieces.append(rng.integers(0, 4, size=ln).astype(np.uint8))
+            cur = p
+        else:
+            cur = min(len(x), p + ln)
+    pieces.append(x[cur:])
+    return np.concatenate(pieces)
+
+
+def repetitive_dna(base_len: int, copies: int, sub_rate: float, indel_rate: float,
+                   indel_max: int, seed: int) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 4, size=base_len).astype(np.uint8)
+    parts = [mutate(rng, base, sub_rate, indel_rate, indel_max) for _ in range(copies)]
+    return ACGT[np.concatenate(parts)]
+
+
+def markov2_printable(n: int, seed: int, alpha: float = 0.5) -> np.ndarray:
+    rng = np.random.default_rng(seed)
+    k = 96
+    probs = rng.dirichlet(np.full(k, alpha), size=k * k)
+    cdf = np.cumsum(probs, axis=1)
+    cdf[:, -1] = 1.0
+    out = np.empty(n, dtype=np.uint8)
+    u = rng.random(n)
+    a, b = int(rng.integers(0, k)), int(rng.integers(0, k))
+    # sequential chain; vectorised per block would change the distribution, so loop in chunks
+    flat = cdf.reshape(k * k, k)
+    ctx = a * k + b
+    chunk = 1 << 16
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        for i in range(s, e):
+            c = int(np.searchsorted(flat[ctx], u[i]))
+            out[i] = c
+            ctx = (ctx % k) * k + c
+    return out + 32
+
+
+# Workloads as named in the bench/test configuration.
+CONFIGS = {
+    # 32 MiB: 1 MiB base x 32 copies, 0.1% substitutions + 0.01% indels of length 1..10
+    "dna32m": dict(base_len=1 << 20, copies=32, sub_rate=0.001, indel_rate=0.0001, indel_max=10,
+                   seed=SEED_BASE + 0),
+    # 256 MiB: 4 MiB base x 64 copies, 0.05% substitutions + 0.005% indels of length 1..50
+    "dna256m": dict(base_len=4 << 20, copies=64, sub_rate=0.0005, indel_rate=0.00005, indel_max=50,
+                    seed=SEED_BASE + 1),
+    # 1 GiB: 8 MiB base x 128 copies, 0.02% mutations split 10:1 substitutions:indels
+    "dna1g": dict(base_len=8 << 20, copies=128, sub_rate=0.0002 * 10 / 11,
+                  indel_rate=0.0002 / 11, indel_max=10, seed=SEED_BASE + 2),
+}
+
+
+def workload(name: str, scale: float = 1.0) -> np.ndarray:
+    """Text of a named workload; `scale` < 1 shrinks the base length (for quick tests)."""
+    cfg = dict(CONFIGS[name])
+    cfg["base_len"] = max(1024, int(cfg["base_len"] * scale))
+    return repetitive_dna(**cfg)
