@@ -1,0 +1,180 @@
+"""Benchmark: BWT build MB/s on one MI355X (prefix-free parsing, bit-exact).
+
+One step = one complete BWT build of the workload text, already resident in HBM, into a
+device output buffer (mibwt_pfp_bwt_device).  Default workload: the 256 MiB repetitive DNA
+collection (4 MiB uniform-ACGT base x 64 copies, 0.05% substitutions + 0.005% indels of
+length 1..50), W=5, P=16.  Prints one JSON line.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna256m] [--W 5 --P 16]
+                  [--no-verify] [--no-cpu-baseline]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.join(ROOT, "masterarbeit_parallele_konstruktion_der_burrows-wheeler-transformation_auf_der_gpu_amd")
+HBM_PEAK_GBS = 8000.0
+METRIC = "BWT build MB/s on 1 MI355X, bit-exact; learned (W,P) vs fixed W=5,P=16"
+
+
+def load_pkg():
+    if "mibwt" in sys.modules:
+        return sys.modules["mibwt"]
+    spec = importlib.util.spec_from_file_location("mibwt", os.path.join(PKG_DIR, "__init__.py"),
+                                                  submodule_search_locations=[PKG_DIR])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mibwt"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(T, W, P, sample_bytes):
+    """The oracle's restatement of the reference PFP pipeline (single thread) on a prefix."""
+    sys.path.insert(0, ROOT)
+    import oracle  # checker / CPU baseline only
+    sample = T[:sample_bytes]
+    t0 = time.perf_counter()
+    oracle.pfp_bwt(sample, W, P)
+    dt = time.perf_counter() - t0
+    return {"value": round(len(sample) / 1e6 / dt, 3), "unit": "MB/s", "cores": 1, "kind": "port",
+            "sample": f"first {len(sample) >> 20} MiB of the workload, PFP W={W},P={P}, "
+                      f"single-threaded C restatement of the reference pipeline, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="dna256m")
+    ap.add_argument("--W", type=int, default=5)
+    ap.add_argument("--P", type=int, default=16)
+    ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-mib", type=int, default=32)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also launch a plain 256 MiB device copy kernel (byte-counter reference)")
+    a = ap.parse_args()
+
+    if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    mibwt = load_pkg()
+    from mibwt import gen
+
+    log(f"generating workload {a.workload}")
+    T = gen.workload(a.workload)
+    n = len(T)
+    log(f"n = {n} bytes")
+    # device buffers through the library's own HIP runtime (the text stays resident in HBM)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(n)
+    mibwt.synchronize()
+    st = mibwt.Storage(n)
+
+    def step():
+        return st.bwt_device(dT.address, dO.address, n, "pfp", a.W, a.P)
+
+    for i in range(a.warmup):
+        idx = step()
+        log(f"warmup {i} done ({st.stats()['total_ms']:.2f} ms)")
+    mibwt.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        idx = step()
+    mibwt.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1000.0 / a.steps
+    value = n / 1e6 / (ms_per_step / 1000.0)
+    stats = st.stats()
+    log(f"timed {a.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} MB/s")
+
+    # dominant kernel: the LSD radix sort's ranking/scatter pass, every launch bracketed
+    # with HIP events on the storage stream during one extra (untimed) build
+    st.set_kernel_timing("radix_scatter")
+    step()
+    ks = st.stats()
+    st.set_kernel_timing("trig_mask")
+    step()
+    kscan = st.stats()
+    st.set_kernel_timing(None)
+    if a.calibrate_copy:
+        cms, cbytes = st.time_kernel("copy", reps=3, nbytes=float(256 << 20))
+        log(f"copy kernel: {cbytes / (cms / 1000) / 1e9 * 2:.1f} GB/s (read+write) over {cbytes:.0f} B")
+    avg_ms = ks["kt_ms_total"] / max(1, ks["kt_launches"])
+    avg_bytes = ks["kt_bytes_total"] / max(1, ks["kt_launches"])
+    achieved = avg_bytes / (avg_ms / 1000.0) / 1e9
+    scan_gbs = kscan["kt_bytes_total"] / (kscan["kt_ms_total"] / 1000.0) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_radix_scatter.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roofline = {"bound": "hbm", "kernel": "k_radix_scatter", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": traffic, "launches": ks["kt_launches"], "avg_launch_us": round(avg_ms * 1000, 2),
+                "algorithmic_bytes_per_launch": round(avg_bytes)}
+
+    verified = None
+    if not a.no_verify:
+        sys.path.insert(0, ROOT)
+        import oracle  # checker only
+        O = dO.download()
+        L = mibwt.reference_to_cubwt(O, idx)
+        t = time.perf_counter()
+        back = oracle.unbwt(L, idx)
+        ok = bool((back == T).all())
+        verified = "unbwt_roundtrip_ok" if ok else "MISMATCH"
+        log(f"verification (inverse BWT on the host): {verified} in {time.perf_counter() - t:.1f} s")
+        if not ok:
+            sys.exit(1)
+
+    cpu = None
+    if not a.no_cpu_baseline:
+        cpu = cpu_baseline(T, a.W, a.P, min(n, a.cpu_sample_mib << 20))
+        cpu["cores"] = 1
+        log(f"cpu baseline: {cpu['value']} MB/s")
+
+    phases = {p["name"]: {"ms": round(p["ms"], 3),
+                          "GBps": round(p["bytes"] / max(p["ms"], 1e-9) / 1e6, 1)} for p in stats["phases"]}
+    phases["scan_kernel"] = {"GBps": round(scan_gbs, 1), "frac": round(scan_gbs / HBM_PEAK_GBS, 4)}
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "MB/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic",
+        "config": {"workload": a.workload, "n_bytes": n, "method": "pfp", "W": a.W, "P": a.P,
+                   "wp_choice": "fixed", "parallelism": "single-gpu"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "verified": verified,
+        "phases": phases,
+        "counts": {k: stats[k] for k in ("triggers", "phrases", "distinct_phrases", "dict_length",
+                                          "mixed_entries", "doubling_rounds_dict", "doubling_rounds_parse")},
+    }
+    print(json.dumps(out), flush=True)
+    st.close()
+
+
+if __name__ == "__main__":
+    main()

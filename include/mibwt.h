@@ -81,6 +81,13 @@ int64_t mibwt_lyndon_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n)
 int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
                                 void* stream);
 
+/* Lyndon array of host T[0..n): lam[i] = length of the longest Lyndon word starting at i,
+ * computed by the block-local passes and the merge tree of the Lyndon pipeline (the forest
+ * the reference builds in Compute_Lyndon_Grammar_Local1/2, Lyndon_Grammar_BWT.cu:647-1868,
+ * and merges in Compute_Lyndon_Grammar_Global :1910-2062).  Returns the number of merge
+ * levels or an error code. */
+int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t* lam);
+
 /* Per-phase statistics of the last call on this storage. */
 #define MIBWT_MAX_PHASES 16
 struct mibwt_phase_stats {
@@ -92,15 +99,34 @@ struct mibwt_phase_stats {
     int64_t triggers, phrases, distinct_phrases, dict_length, alpha_groups;
     int64_t mixed_entries, doubling_rounds_dict, doubling_rounds_parse;
     double total_ms;                     /* HIP-event time from T resident to O resident */
+    /* per-launch timing of the kernel family chosen with mibwt_set_kernel_timing */
+    int64_t kt_launches;
+    double kt_ms_total;                  /* sum of the launches' HIP-event durations */
+    double kt_bytes_total;               /* sum of their algorithmic HBM bytes */
 };
 int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out);
 
-/* Time a kernel of the last configuration: re-runs the named phase's dominant kernel
- * `reps` times on the storage stream and returns its mean duration in ms via *ms.
- * Names: "scan" (trigger scan) and "sort" (one 8-bit LSD radix pass of the phrase
- * fingerprint sort).  Inputs must be those of the last mibwt_pfp_bwt_device call. */
+/* Bracket every launch of one kernel family with HIP events on the storage stream during
+ * the following calls; the totals appear in mibwt_phase_stats (kt_*).  kernel:
+ * "radix_scatter" (the LSD radix sort's ranking/scatter pass), "trig_mask" (the trigger
+ * scan) or NULL to switch timing off. */
+int64_t mibwt_set_kernel_timing(void* storage, const char* kernel);
+
+/* Diagnostics: re-run a kernel `reps` times on the storage stream and return its mean
+ * duration (ms) and algorithmic bytes per launch.  "scan": the trigger-scan kernel on the
+ * text of the last PFP call; "copy": a plain 16-byte-per-thread device copy of
+ * bytes_per_launch bytes (input), used to calibrate hardware byte counters. */
 int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms,
                           double* bytes_per_launch);
+
+/* Device-memory plumbing for callers that keep texts resident in HBM without another HIP
+ * runtime in the process (no counterpart in the reference, whose programs allocate inside
+ * main).  Current device; synchronous copies; kind: 0 host->device, 1 device->host,
+ * 2 device->device. */
+int64_t mibwt_device_alloc(void** ptr, int64_t bytes);
+int64_t mibwt_device_free(void* ptr);
+int64_t mibwt_copy(void* dst, const void* src, int64_t bytes, int32_t kind);
+int64_t mibwt_synchronize(void);
 
 #ifdef __cplusplus
 }

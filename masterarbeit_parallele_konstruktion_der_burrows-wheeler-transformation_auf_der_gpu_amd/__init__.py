@@ -41,8 +41,14 @@ EXPORTED = [
     "mibwt_pfp_bwt_device",
     "mibwt_lyndon_bwt",
     "mibwt_lyndon_bwt_device",
+    "mibwt_lyndon_array",
     "mibwt_last_phase_stats",
+    "mibwt_set_kernel_timing",
     "mibwt_time_kernel",
+    "mibwt_device_alloc",
+    "mibwt_device_free",
+    "mibwt_copy",
+    "mibwt_synchronize",
 ]
 
 _lib = None
@@ -69,6 +75,8 @@ class PhaseStats(ctypes.Structure):
         ("alpha_groups", ctypes.c_int64), ("mixed_entries", ctypes.c_int64),
         ("doubling_rounds_dict", ctypes.c_int64), ("doubling_rounds_parse", ctypes.c_int64),
         ("total_ms", ctypes.c_double),
+        ("kt_launches", ctypes.c_int64), ("kt_ms_total", ctypes.c_double),
+        ("kt_bytes_total", ctypes.c_double),
     ]
 
 
@@ -88,9 +96,15 @@ def lib():
         L.mibwt_pfp_bwt_device.argtypes = [vp, vp, vp, i64, i32, i32, vp]
         L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
         L.mibwt_lyndon_bwt_device.argtypes = [vp, vp, vp, i64, vp]
+        L.mibwt_lyndon_array.argtypes = [vp, u8p, i64, ctypes.POINTER(ctypes.c_uint32)]
         L.mibwt_last_phase_stats.argtypes = [vp, ctypes.POINTER(PhaseStats)]
+        L.mibwt_set_kernel_timing.argtypes = [vp, ctypes.c_char_p]
         L.mibwt_time_kernel.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
+        L.mibwt_device_alloc.argtypes = [ctypes.POINTER(vp), i64]
+        L.mibwt_device_free.argtypes = [vp]
+        L.mibwt_copy.argtypes = [vp, vp, i64, i32]
+        L.mibwt_synchronize.argtypes = []
         for name in EXPORTED:
             getattr(L, name).restype = i64
         _lib = L
@@ -163,6 +177,15 @@ class Storage:
             raise BwtError(rc, method + "_device")
         return int(rc)
 
+    def lyndon_array(self, T) -> np.ndarray:
+        T = _as_text(T)
+        lam = np.empty(len(T), dtype=np.uint32)
+        rc = lib().mibwt_lyndon_array(self.handle, _u8(T), len(T),
+                                      lam.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        if rc < 0:
+            raise BwtError(rc, "mibwt_lyndon_array")
+        return lam
+
     def stats(self) -> dict:
         st = PhaseStats()
         rc = lib().mibwt_last_phase_stats(self.handle, ctypes.byref(st))
@@ -175,12 +198,69 @@ class Storage:
         out["phases"] = phases
         return out
 
-    def time_kernel(self, name: str, reps: int = 20):
-        ms, b = ctypes.c_double(), ctypes.c_double()
+    def set_kernel_timing(self, kernel: str | None):
+        rc = lib().mibwt_set_kernel_timing(self.handle, kernel.encode() if kernel else None)
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_set_kernel_timing")
+
+    def time_kernel(self, name: str, reps: int = 20, nbytes: float = 0.0):
+        ms, b = ctypes.c_double(), ctypes.c_double(nbytes)
         rc = lib().mibwt_time_kernel(self.handle, name.encode(), reps, ctypes.byref(ms), ctypes.byref(b))
         if rc != NO_ERROR:
             raise BwtError(rc, "mibwt_time_kernel")
         return ms.value, b.value
+
+
+class DeviceBuffer:
+    """A device allocation made through the library's own HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        self.ptr = ctypes.c_void_p()
+        self.nbytes = int(nbytes)
+        rc = lib().mibwt_device_alloc(ctypes.byref(self.ptr), self.nbytes)
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_device_alloc")
+
+    @classmethod
+    def from_numpy(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    @property
+    def address(self) -> int:
+        return int(self.ptr.value)
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        rc = lib().mibwt_copy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, 0)
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_copy")
+
+    def download(self, dtype=np.uint8, count: int | None = None) -> np.ndarray:
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize if count is None else count, dtype=dtype)
+        rc = lib().mibwt_copy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, 1)
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_copy")
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().mibwt_device_free(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def synchronize():
+    rc = lib().mibwt_synchronize()
+    if rc != NO_ERROR:
+        raise BwtError(rc, "mibwt_synchronize")
 
 
 def bwt(T, method: str = "pfp", W: int = 5, P: int = 16):

@@ -6,7 +6,9 @@
 namespace mibwt {
 int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P);
 double pfp_time_scan(Storage& S, int reps, double* bytes);
+double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
+int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
 void selector_choose(const uint8_t* T, int64_t n, int* W, int* P);
 }  // namespace mibwt
 
@@ -170,6 +172,16 @@ int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, i
     return device_bwt(storage, dT, dO, n, MIBWT_METHOD_LYNDON, 5, 16, stream);
 }
 
+int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t* lam)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !T || !lam || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        return lyndon_array_run(*S, T, n, lam);
+    });
+}
+
 int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out)
 {
     Storage* S = reinterpret_cast<Storage*>(storage);
@@ -178,17 +190,75 @@ int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out)
     return LIBCUBWT_NO_ERROR;
 }
 
+int64_t mibwt_set_kernel_timing(void* storage, const char* kernel)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S) return LIBCUBWT_BAD_PARAMETER;
+    if (!kernel) S->kt_family = 0;
+    else if (strcmp(kernel, "radix_scatter") == 0) S->kt_family = 1;
+    else if (strcmp(kernel, "trig_mask") == 0) S->kt_family = 2;
+    else return LIBCUBWT_BAD_PARAMETER;
+    return LIBCUBWT_NO_ERROR;
+}
+
 int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms, double* bytes_per_launch)
 {
     Storage* S = reinterpret_cast<Storage*>(storage);
-    if (!S || !name || reps < 1 || !ms || !bytes_per_launch || S->last_n < 1) return LIBCUBWT_BAD_PARAMETER;
+    if (!S || !name || reps < 1 || !ms || !bytes_per_launch) return LIBCUBWT_BAD_PARAMETER;
+    if (strcmp(name, "copy") != 0 && S->last_n < 1) return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         if (strcmp(name, "scan") == 0) {
             *ms = pfp_time_scan(*S, reps, bytes_per_launch);
             return LIBCUBWT_NO_ERROR;
         }
+        if (strcmp(name, "copy") == 0) {
+            const uint64_t bytes = (uint64_t)(*bytes_per_launch) & ~15ull;
+            if (bytes < 16) return (int64_t)LIBCUBWT_BAD_PARAMETER;
+            *ms = time_copy(*S, bytes, reps);
+            *bytes_per_launch = (double)bytes;
+            return LIBCUBWT_NO_ERROR;
+        }
         return (int64_t)LIBCUBWT_BAD_PARAMETER;
+    });
+}
+
+int64_t mibwt_device_alloc(void** ptr, int64_t bytes)
+{
+    if (!ptr || bytes < 0) return LIBCUBWT_BAD_PARAMETER;
+    *ptr = nullptr;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipMalloc(ptr, (size_t)(bytes > 0 ? bytes : 1)));
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_device_free(void* ptr)
+{
+    if (!ptr) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipFree(ptr));
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_copy(void* dst, const void* src, int64_t bytes, int32_t kind)
+{
+    if (!dst || !src || bytes < 0 || kind < 0 || kind > 2) return LIBCUBWT_BAD_PARAMETER;
+    if (bytes == 0) return LIBCUBWT_NO_ERROR;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                        : hipMemcpyDeviceToDevice;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_synchronize(void)
+{
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipDeviceSynchronize());
+        return LIBCUBWT_NO_ERROR;
     });
 }
 

@@ -1,0 +1,170 @@
+// Command-line front ends that reproduce the reference programs' file boundary:
+// read ./input.txt ("n" then n decimal symbols), write ./out.txt ("n\n" then "s " per symbol,
+// no final newline), so that compare_script.sh / compare_script_lyndon.sh keep working.
+//
+//   pfp_bwt      [-W w] [-P p] [-i input.txt] [-o out.txt] [--raw]
+//                replaces Prefix_Free_Parsing_W=<w>_P=<p>[_small_alphabet].cu main (:4793-4815)
+//   lyndon_bwt   [-i ...] [-o ...] [--raw]     replaces Lyndon_Grammar_BWT.cu main (:5547-5576)
+//   cubwt_compat [-i ...] [-o ...] [--raw] [--driver-quirk]
+//                replaces the libcubwt.cu thesis driver (:3023-3037); --driver-quirk
+//                reproduces its output when the primary index equals n (it then prints n-1
+//                symbols, libcubwt.cu:3003-3015).
+// --raw reads the input as raw bytes and writes the BWT as raw bytes (reference order).
+#include <cerrno>
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "mibwt.h"
+
+namespace {
+
+bool read_file(const char* path, std::vector<char>& buf)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return false;
+    fseek(f, 0, SEEK_END);
+    long sz = ftell(f);
+    fseek(f, 0, SEEK_SET);
+    buf.resize((size_t)(sz > 0 ? sz : 0) + 1);
+    size_t got = sz > 0 ? fread(buf.data(), 1, (size_t)sz, f) : 0;
+    fclose(f);
+    buf.resize(got);
+    buf.push_back('\0');
+    return true;
+}
+
+// Parse "n v0 v1 ... v(n-1)" (whitespace separated decimals).  Returns false on bad input.
+bool parse_reference_text(const std::vector<char>& buf, std::vector<uint8_t>& T)
+{
+    const char* p = buf.data();
+    const char* e = p + buf.size() - 1;
+    auto next = [&](long long& v) -> bool {
+        while (p < e && (*p == ' ' || *p == '\n' || *p == '\r' || *p == '\t')) p++;
+        if (p >= e) return false;
+        bool neg = false;
+        if (*p == '-') { neg = true; p++; }
+        if (p >= e || *p < '0' || *p > '9') return false;
+        long long x = 0;
+        while (p < e && *p >= '0' && *p <= '9') x = x * 10 + (*p++ - '0');
+        v = neg ? -x : x;
+        return true;
+    };
+    long long n;
+    if (!next(n) || n < 0) return false;
+    T.resize((size_t)n);
+    for (long long i = 0; i < n; i++) {
+        long long v;
+        if (!next(v) || v < 0 || v > 255) return false;
+        T[(size_t)i] = (uint8_t)v;
+    }
+    return true;
+}
+
+bool write_reference_text(const char* path, const uint8_t* O, int64_t count, int64_t n)
+{
+    FILE* f = fopen(path, "wb");
+    if (!f) return false;
+    std::vector<char> out;
+    out.reserve((size_t)count * 4 + 32);
+    char tmp[32];
+    int k = snprintf(tmp, sizeof(tmp), "%lld\n", (long long)n);
+    out.insert(out.end(), tmp, tmp + k);
+    for (int64_t i = 0; i < count; i++) {
+        unsigned v = O[i];
+        if (v >= 100) out.push_back((char)('0' + v / 100));
+        if (v >= 10) out.push_back((char)('0' + (v / 10) % 10));
+        out.push_back((char)('0' + v % 10));
+        out.push_back(' ');
+    }
+    bool ok = fwrite(out.data(), 1, out.size(), f) == out.size();
+    return fclose(f) == 0 && ok;
+}
+
+int usage(const char* prog)
+{
+    fprintf(stderr, "usage: %s [-W w] [-P p] [-i input.txt] [-o out.txt] [--raw] [--driver-quirk]\n", prog);
+    return 2;
+}
+
+}  // namespace
+
+int main(int argc, char** argv)
+{
+    int W = 5, P = 16;
+    const char* in = "input.txt";
+    const char* outp = "out.txt";
+    bool raw = false, quirk = false;
+    for (int i = 1; i < argc; i++) {
+        std::string a = argv[i];
+        if (a == "-W" && i + 1 < argc) W = atoi(argv[++i]);
+        else if (a == "-P" && i + 1 < argc) P = atoi(argv[++i]);
+        else if (a == "-i" && i + 1 < argc) in = argv[++i];
+        else if (a == "-o" && i + 1 < argc) outp = argv[++i];
+        else if (a == "--raw") raw = true;
+        else if (a == "--driver-quirk") quirk = true;
+        else return usage(argv[0]);
+    }
+    std::vector<char> buf;
+    if (!read_file(in, buf)) {
+        fprintf(stderr, "cannot read %s: %s\n", in, strerror(errno));
+        return 1;
+    }
+    std::vector<uint8_t> T;
+    if (raw) {
+        T.assign(buf.begin(), buf.end() - 1);
+    } else if (!parse_reference_text(buf, T)) {
+        fprintf(stderr, "%s: malformed input (expected n followed by n symbols in 0..255)\n", in);
+        return 1;
+    }
+    buf.clear();
+    buf.shrink_to_fit();
+    const int64_t n = (int64_t)T.size();
+    if (n == 0) {
+        return write_reference_text(outp, nullptr, 0, 0) ? 0 : 1;
+    }
+    void* st = nullptr;
+    int64_t rc = libcubwt_allocate_device_storage(&st, n < 16 ? 16 : n);
+    if (rc != LIBCUBWT_NO_ERROR) {
+        fprintf(stderr, "device storage allocation failed (%lld)\n", (long long)rc);
+        return 1;
+    }
+    std::vector<uint8_t> L((size_t)n);
+#if defined(MIBWT_CLI_LYNDON_BWT)
+    int64_t idx = mibwt_lyndon_bwt(st, T.data(), L.data(), n);
+#elif defined(MIBWT_CLI_CUBWT_COMPAT)
+    int64_t idx = (n >= 16) ? libcubwt_bwt(st, T.data(), L.data(), n) : mibwt_pfp_bwt(st, T.data(), L.data(), n, 5, 16);
+#else
+    int64_t idx = mibwt_pfp_bwt(st, T.data(), L.data(), n, W, P);
+#endif
+    libcubwt_free_device_storage(st);
+    if (idx < 1) {
+        fprintf(stderr, "BWT failed (%lld)\n", (long long)idx);
+        return 1;
+    }
+    // reference order: O = L[1..i-1] ++ [L[0]] ++ L[i..n-1]
+    std::vector<uint8_t> O((size_t)n);
+    memcpy(O.data(), L.data() + 1, (size_t)(idx - 1));
+    O[(size_t)idx - 1] = L[0];
+    memcpy(O.data() + idx, L.data() + idx, (size_t)(n - idx));
+    int64_t count = n;
+    (void)quirk;
+#if defined(MIBWT_CLI_CUBWT_COMPAT)
+    if (quirk && idx == n) {
+        // the reference driver's loop over i = 1..n-1 never meets i == index == n and so
+        // prints L[1..n-1] only
+        memcpy(O.data(), L.data() + 1, (size_t)(n - 1));
+        count = n - 1;
+    }
+#endif
+    if (raw) {
+        FILE* f = fopen(outp, "wb");
+        bool ok = f && fwrite(O.data(), 1, (size_t)count, f) == (size_t)count;
+        if (f) ok = (fclose(f) == 0) && ok;
+        return ok ? 0 : 1;
+    }
+    return write_reference_text(outp, O.data(), count, n) ? 0 : 1;
+}

@@ -86,6 +86,11 @@ struct Storage {
     int phase_used = 0;
     hipEvent_t t_begin = nullptr, t_end = nullptr;
     mibwt_phase_stats stats{};
+    // per-launch timing of one kernel family (0 = off, 1 = radix scatter, 2 = trigger scan)
+    int kt_family = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;
+    std::vector<double> kt_bytes;
+    int kt_used = 0;
     // state kept for mibwt_time_kernel
     const uint8_t* last_text = nullptr;
     int64_t last_n = 0;
@@ -144,7 +149,27 @@ struct Storage {
     void phases_reset()
     {
         phase_used = 0;
+        kt_used = 0;
         memset(&stats, 0, sizeof(stats));
+    }
+    // bracket one launch of the timed kernel family; returns -1 when timing is off
+    int kt_begin(int family, double bytes)
+    {
+        if (kt_family != family) return -1;
+        if ((int)kt_events.size() <= kt_used) {
+            hipEvent_t a, b;
+            MIBWT_HIP(hipEventCreate(&a));
+            MIBWT_HIP(hipEventCreate(&b));
+            kt_events.emplace_back(a, b);
+            kt_bytes.push_back(0);
+        }
+        kt_bytes[kt_used] = bytes;
+        MIBWT_HIP(hipEventRecord(kt_events[kt_used].first, stream));
+        return kt_used++;
+    }
+    void kt_end(int id)
+    {
+        if (id >= 0) MIBWT_HIP(hipEventRecord(kt_events[id].second, stream));
     }
     int phase_begin(const char* name)
     {
@@ -174,6 +199,15 @@ struct Storage {
             snprintf(stats.name[i], sizeof(stats.name[i]), "%s", phases[i].name.c_str());
             stats.ms[i] = ms;
             stats.bytes[i] = phases[i].bytes;
+        }
+        stats.kt_launches = kt_used;
+        stats.kt_ms_total = 0;
+        stats.kt_bytes_total = 0;
+        for (int i = 0; i < kt_used; i++) {
+            float ms = 0;
+            MIBWT_HIP(hipEventElapsedTime(&ms, kt_events[i].first, kt_events[i].second));
+            stats.kt_ms_total += ms;
+            stats.kt_bytes_total += kt_bytes[i];
         }
     }
 };
@@ -225,6 +259,34 @@ enum BufId : int {
     B_Q_OFF,
     B_SCALARS,
     B_STAGE,
+    B_DB_KEY2,
+    B_DB_GID,
+    B_DB_GOFS,
+    B_DB_LARGE,
+    B_LG_K0,
+    B_LG_K1,
+    B_LG_V0,
+    B_LG_V1,
+    B_LG_I,
+    B_ALPHA,
+    B_LY_RES,
+    B_LY_FIN,
+    B_LY_LIST0,
+    B_LY_LIST1,
+    B_LY_CNT0,
+    B_LY_CNT1,
+    B_LY_LAM,
+    B_LY_HASH,
+    B_LY_SCAN0,     // 4 consecutive ids for the fingerprint scan partials
+    B_LY_SCAN1,
+    B_LY_SCAN2,
+    B_LY_SCAN3,
+    B_LY_HF,
+    B_LY_HF2,
+    B_LY_CLS,
+    B_LY_VL,
+    B_LY_JMP0,
+    B_LY_JMP1,
     B_COUNT_
 };
 
@@ -237,6 +299,9 @@ enum ScalarSlot : int {
     SC_PIDX,
     SC_ENTRIES,
     SC_TOTAL,
+    SC_GROUPS,
+    SC_LARGE,
+    SC_SIGMA,
     SC_COUNT_ = 16
 };
 

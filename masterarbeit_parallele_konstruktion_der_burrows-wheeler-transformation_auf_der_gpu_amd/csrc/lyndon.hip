@@ -1,12 +1,662 @@
-// Lyndon-grammar BWT (Lyndon_Grammar_BWT.cu).  Placeholder until the GPU pipeline lands.
-#include "prims.hpp"
+// Lyndon-grammar BWT on gfx950.
+//
+// Reference: Lyndon_Grammar_BWT.cu (build_Lyndon_Grammar :2592, Sort_Lyndon_Grammar :3621,
+// Build_BWT :4628).  Every position i of the text has a longest Lyndon prefix
+// P_i = T[i .. i+lam[i]); these are the maximal nodes of the Lyndon forest (stack words and
+// right children, the reference's rules).  The suffix order is the order of (P_i, suffix at
+// i+lam[i]): Lyndon prefixes compare lexicographically (a proper prefix is smaller), equal
+// prefixes are ordered by the suffix that follows, which starts at the next smaller suffix
+// and is therefore smaller.  The reference builds the grammar block-locally and merges it
+// right to left (:647-2062), deduplicates rules by Karp-Rabin fingerprints (:2183-2588),
+// sorts rules (:3129-4083) and induces the BWT over rank groups (:4114-4896).  Here:
+//
+//  1. lyndon   Lyndon array: every 256-byte block runs the right-to-left stack loop on its
+//              own; a position whose word may still grow with text beyond the block stays
+//              "open" with a resume point; open positions are resolved by a merge tree over
+//              neighbouring block ranges (one thread per range pair and level).
+//  2. rules    Karp-Rabin prefix fingerprints (mod 2^61-1) of the text in one device scan.
+//  3. rsort    positions ordered by P_i: radix sort of the first symbols packed with dense
+//              alphabet codes, then groups that tie on that prefix are ordered with a
+//              fingerprint-LCP comparator (equal fingerprints over the whole prefix = the same
+//              rule); the final order is checked pair by pair byte-exactly and the
+//              fingerprint base is re-drawn if a collision ever broke it.
+//  4. induce   groups of equal P_i are refined by the rank of the following suffix, along
+//              the next-smaller-suffix chain with pointer jumping (segmented group refinement
+//              shared with the prefix-free-parsing pipeline).
+//  5. output   O[j] = T[SA[j]-1] with the wrap to T[n-1].
+#include "doubling.hpp"
+#include "text.hpp"
 
 namespace mibwt {
 
-int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n)
+constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass
+constexpr uint64_t kM61 = (1ull << 61) - 1;
+
+// ------------------------------------------------------------------- 1. Lyndon array
+
+// longest common prefix of T[a..] and T[b..] (a < b), at most cap and never past n
+__device__ __forceinline__ uint32_t lcp_text(const uint8_t* __restrict__ T, uint64_t n, uint32_t a, uint32_t b,
+                                             uint32_t cap)
 {
-    (void)S; (void)src; (void)src_host; (void)dO; (void)n;
-    throw Error(LIBCUBWT_GPU_NOT_SUPPORTED, "Lyndon-grammar BWT is not built yet");
+    const uint64_t lim64 = ((uint64_t)cap < n - b) ? (uint64_t)cap : n - b;
+    const uint32_t lim = (uint32_t)lim64;
+    uint32_t k = 0;
+    while (k + 8 <= lim) {
+        const uint64_t x = load8(T, a + k) ^ load8(T, b + k);
+        if (x) return k + (uint32_t)(__builtin_ctzll(x) >> 3);
+        k += 8;
+    }
+    while (k < lim && T[a + k] == T[b + k]) k++;
+    return k;
+}
+
+enum : int { kStop = 0, kMerge = 1, kWait = 2 };
+
+// Right-to-left stack step: may the word w = T[i..j) absorb the word t starting at j (current
+// length lam_j; fin_j tells whether that length is final)?  Text beyond a block is readable;
+// only the final length of an open t is unknown, and the answer waits only when it depends
+// on that length.
+__device__ __forceinline__ int lyn_step(const uint8_t* __restrict__ T, uint64_t n, uint32_t i, uint32_t j,
+                                        uint32_t lam_j, bool fin_j)
+{
+    const uint32_t lw = j - i;
+    const uint32_t k = lcp_text(T, n, i, j, lw);
+    if (k < lw && (uint64_t)j + k < n) {
+        const uint8_t a = T[i + k], b = T[j + k];
+        if (k < lam_j) return (a < b) ? kMerge : kStop;
+        if (a > b) return kStop;            // w is larger than every extension of t
+        return fin_j ? kStop : kWait;       // t is a proper prefix of w unless it grows past k
+    }
+    if ((uint64_t)j + k >= n && k < lw) return kStop;  // the rest of the text is a proper prefix of w
+    if (lam_j > lw) return kMerge;          // w is a proper prefix of t
+    return fin_j ? kStop : kWait;
+}
+
+// One thread per block: right-to-left loop with the block end as a horizon.
+__global__ void k_lyn_local(const uint8_t* __restrict__ T, uint64_t n, uint32_t nb, uint32_t* __restrict__ lam,
+                            uint32_t* __restrict__ res, uint8_t* __restrict__ fin, uint32_t* __restrict__ olist,
+                            uint32_t* __restrict__ ocnt)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t s = b * kLynBlock;
+    const uint32_t e = (uint32_t)(((uint64_t)s + kLynBlock < n) ? s + kLynBlock : n);
+    uint32_t cnt = 0;
+    for (uint32_t i = e; i-- > s;) {
+        uint32_t j = i + 1;
+        bool ok = true;
+        for (;;) {
+            if (j == n) break;
+            if (j == e) {
+                ok = false;
+                break;
+            }
+            const uint32_t lj = lam[j];
+            const bool fj = fin[j] != 0;
+            const int r = lyn_step(T, n, i, j, lj, fj);
+            if (r == kMerge) {
+                if (!fj) {
+                    ok = false;
+                    break;
+                }
+                j += lj;
+            } else {
+                if (r == kWait) ok = false;
+                break;
+            }
+        }
+        lam[i] = j - i;
+        res[i] = j;
+        fin[i] = ok ? 1 : 0;
+        if (!ok) olist[s + cnt++] = i;
+    }
+    ocnt[b] = cnt;
+}
+
+// One thread per pair of neighbouring ranges of `size` blocks: the left range's open
+// positions (right to left) continue into the right range.
+__global__ void k_lyn_merge(const uint8_t* __restrict__ T, uint64_t n, uint32_t nb, uint32_t size,
+                            uint32_t* __restrict__ lam, uint32_t* __restrict__ res, uint8_t* __restrict__ fin,
+                            const uint32_t* __restrict__ in_list, const uint32_t* __restrict__ in_cnt,
+                            uint32_t* __restrict__ out_list, uint32_t* __restrict__ out_cnt)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint64_t a = (uint64_t)t * 2 * size;
+    if (a >= nb) return;
+    const uint64_t rb = a + size;
+    const uint64_t end64 = (a + 2 * size) * kLynBlock;
+    const uint32_t r_end = (uint32_t)(end64 < n ? end64 : n);
+    const uint64_t out = a * kLynBlock;
+    uint32_t oc = 0;
+    if (rb < nb) {
+        const uint32_t c = in_cnt[rb];
+        for (uint32_t q = 0; q < c; q++) out_list[out + oc++] = in_list[rb * kLynBlock + q];
+    }
+    const uint32_t cl = in_cnt[a];
+    for (uint32_t q = 0; q < cl; q++) {
+        const uint32_t i = in_list[a * kLynBlock + q];
+        uint32_t j = res[i];
+        bool ok = true;
+        for (;;) {
+            if (j == n) break;
+            if (j == r_end || !fin[j]) {
+                ok = false;
+                break;
+            }
+            const uint32_t lj = lam[j];
+            if (lyn_step(T, n, i, j, lj, true) == kMerge) j += lj;
+            else break;
+        }
+        res[i] = j;
+        if (ok) {
+            lam[i] = j - i;
+            fin[i] = 1;
+        } else {
+            out_list[out + oc++] = i;
+        }
+    }
+    out_cnt[a] = oc;
+}
+
+// lam[] of T[0..n) (the padded text copy T).  Returns the number of merge levels.
+int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, uint32_t* lam)
+{
+    const uint32_t nb = ceil_div(n, kLynBlock);
+    uint32_t* res = S.buf<uint32_t>(B_LY_RES, n);
+    uint8_t* fin = S.buf<uint8_t>(B_LY_FIN, n);
+    uint32_t* la = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * kLynBlock);
+    uint32_t* lb = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * kLynBlock);
+    uint32_t* ca = S.buf<uint32_t>(B_LY_CNT0, nb);
+    uint32_t* cb = S.buf<uint32_t>(B_LY_CNT1, nb);
+    k_lyn_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(T, n, nb, lam, res, fin, la, ca);
+    MIBWT_LAUNCH_CHECK();
+    int levels = 0;
+    for (uint64_t size = 1; size < nb; size *= 2) {
+        const uint32_t pairs = (uint32_t)((nb + 2 * size - 1) / (2 * size));
+        k_lyn_merge<<<ceil_div(pairs, 64), 64, 0, S.stream>>>(T, n, nb, (uint32_t)size, lam, res, fin, la, ca,
+                                                              lb, cb);
+        MIBWT_LAUNCH_CHECK();
+        std::swap(la, lb);
+        std::swap(ca, cb);
+        levels++;
+    }
+    return levels;
+}
+
+// ------------------------------------------------------------------- 2. fingerprints
+
+__device__ __forceinline__ uint64_t mulmod61(uint64_t a, uint64_t b)
+{
+    const uint64_t lo = a * b, hi = __umul64hi(a, b);
+    uint64_t r = (lo & kM61) + (lo >> 61) + (hi << 3);
+    r = (r & kM61) + (r >> 61);
+    return r >= kM61 ? r - kM61 : r;
+}
+
+__device__ __forceinline__ uint64_t addmod61(uint64_t a, uint64_t b)
+{
+    const uint64_t r = a + b;
+    return r >= kM61 ? r - kM61 : r;
+}
+
+struct HashPair {
+    uint64_t h, p;  // polynomial value and base^length, both mod 2^61-1
+};
+
+__device__ __forceinline__ HashPair shfl_up_val(HashPair x, int o)
+{
+    HashPair y;
+    y.h = __shfl_up(x.h, o, 64);
+    y.p = __shfl_up(x.p, o, 64);
+    return y;
+}
+
+struct OpHash {
+    __device__ __host__ HashPair operator()(HashPair a, HashPair b) const
+    {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return HashPair{addmod61(mulmod61(a.h, b.p), b.h), mulmod61(a.p, b.p)};
+#else
+        return a;  // host instantiation is never executed
+#endif
+    }
+};
+
+__global__ void k_hash_items(const uint8_t* __restrict__ T, uint64_t n, uint64_t base, HashPair* __restrict__ out)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = HashPair{(uint64_t)T[i] + 1, base};
+}
+
+struct PrefixHash {
+    const HashPair* pre;  // pre[k] = (H[k+1], base^(k+1))
+    // fingerprint of T[a..a+len)
+    __device__ __forceinline__ uint64_t sub(uint32_t a, uint32_t len) const
+    {
+        if (len == 0) return 0;
+        const uint64_t hb = pre[a + len - 1].h;
+        if (a == 0) return hb;
+        const uint64_t ha = pre[a - 1].h;
+        const uint64_t pw = pre[len - 1].p;
+        const uint64_t x = mulmod61(ha, pw);
+        return hb >= x ? hb - x : hb + kM61 - x;
+    }
+};
+
+// ------------------------------------------------------------------- 3. rule order
+
+// Key of position i: the first `per` symbols of P_i as dense codes (0 past its end).
+__global__ void k_lyn_initkey(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ lam,
+                              const uint32_t* __restrict__ code, int bits, int per, uint64_t* __restrict__ K,
+                              uint32_t* __restrict__ V)
+{
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t l = lam[i];
+    uint64_t key = 0, w = 0;
+    for (int k = 0; k < per; k++) {
+        if ((k & 7) == 0) w = (i + k < n) ? load8(T, i + k) : 0ull;
+        const uint64_t s = ((uint32_t)k < l) ? cs[(w >> (8 * (k & 7))) & 0xFF] : 0ull;
+        key = (key << bits) | s;
+    }
+    K[i] = key;
+    V[i] = (uint32_t)i;
+}
+
+struct LabelCmp {
+    const uint8_t* T;
+    const uint32_t* lam;
+    PrefixHash ph;
+    // -1 / 0 / +1 for P_x against P_y, knowing their first `known` symbols agree
+    __device__ int operator()(uint32_t x, uint32_t y, uint32_t known) const
+    {
+        if (x == y) return 0;
+        const uint32_t lx = lam[x], ly = lam[y];
+        const uint32_t m = lx < ly ? lx : ly;
+        uint32_t lo = known < m ? known : m;
+        if (lx == ly && ph.sub(x, lx) == ph.sub(y, ly)) return 0;
+        if (ph.sub(x, m) == ph.sub(y, m)) {
+            lo = m;
+        } else {
+            uint32_t hi = m - 1;  // largest candidate LCP
+            while (lo < hi) {
+                const uint32_t mid = lo + (hi - lo + 1) / 2;
+                if (ph.sub(x, mid) == ph.sub(y, mid)) lo = mid;
+                else hi = mid - 1;
+            }
+        }
+        if (lo < m) return T[x + lo] < T[y + lo] ? -1 : 1;
+        return lx < ly ? -1 : (lx > ly ? 1 : 0);
+    }
+};
+
+// Groups of equal initial keys whose members reach past the packed prefix: every member
+// counts the members ordered before it with the comparator (small groups only).
+__global__ void k_label_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
+                              const uint32_t* __restrict__ gofs, uint32_t m, uint32_t known, LabelCmp cmp,
+                              uint32_t* __restrict__ V2, uint32_t* __restrict__ eqhead, uint32_t* __restrict__ large)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t g = gid1[i] - 1;
+    const uint32_t s = gofs[g], e = gofs[g + 1];
+    if (e - s > kSmallGroup) {
+        large[i] = 1;
+        return;
+    }
+    large[i] = 0;
+    const uint32_t x = V[i];
+    uint32_t before = 0, equal_before = 0;
+    for (uint32_t j = s; j < e; j++) {
+        if (j == i) continue;
+        const int c = cmp(V[j], x, known);
+        before += (c < 0) || (c == 0 && j < i);
+        equal_before += (c == 0 && j < i);
+    }
+    V2[s + before] = x;
+    // the first of a run of equal labels starts a new label group
+    eqhead[s + before] = (equal_before == 0) ? 1u : 0u;
+}
+
+// Groups of the current label order: gid1 (1-based, inclusive scan of heads) and gofs.
+__global__ void k_lab_heads_from_keys(const uint64_t* __restrict__ K, uint64_t n, uint32_t* __restrict__ hf)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    hf[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+}
+
+// Per element: is its group a tie group (>= 2 members whose labels all reach past `known`
+// symbols), and is that group small or large?  cls: 0 = settled, 1 = small tie, 2 = large tie.
+__global__ void k_lab_classify(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
+                               const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ lam, uint64_t n,
+                               uint32_t known, uint8_t* __restrict__ cls, uint32_t* __restrict__ large)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = gid1[i] - 1;
+    const uint32_t s = gofs[g], e = gofs[g + 1];
+    uint8_t c = 0;
+    if (e - s >= 2 && lam[V[s]] >= known) c = (e - s > kSmallGroup) ? 2 : 1;
+    cls[i] = c;
+    large[i] = (c == 2) ? 1u : 0u;
+}
+
+// Small tie groups: comparator counting sort; settled groups are copied.  hf receives the
+// heads of the refined order (first of every run of equal labels).
+__global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
+                            const uint32_t* __restrict__ gofs, const uint8_t* __restrict__ cls, uint64_t n,
+                            uint32_t known, LabelCmp cmp, uint32_t* __restrict__ V2, uint32_t* __restrict__ hf)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t g = gid1[i] - 1;
+    const uint32_t s = gofs[g], e = gofs[g + 1];
+    const uint32_t x = V[i];
+    if (cls[i] == 2) return;  // large groups are written by the chunk pass
+    if (cls[i] == 0) {
+        V2[i] = x;
+        hf[i] = (i == s) ? 1u : 0u;
+        return;
+    }
+    uint32_t before = 0, equal_before = 0;
+    for (uint32_t j = s; j < e; j++) {
+        if (j == i) continue;
+        const int c = cmp(V[j], x, known);
+        before += (c < 0) || (c == 0 && j < (uint32_t)i);
+        equal_before += (c == 0 && j < (uint32_t)i);
+    }
+    V2[s + before] = x;
+    hf[s + before] = (equal_before == 0) ? 1u : 0u;
+}
+
+// Large tie groups: key = (group, next `per` symbols after `known`).
+__global__ void k_lab_chunk_gather(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ V,
+                                   const uint32_t* __restrict__ gid1, const uint32_t* __restrict__ large,
+                                   const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ lam,
+                                   const uint32_t* __restrict__ code, int bits, int per, uint32_t known, int cb,
+                                   uint64_t* __restrict__ LK, uint32_t* __restrict__ LV, uint32_t* __restrict__ LI)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !large[i]) return;
+    const uint32_t x = V[i];
+    const uint32_t l = lam[x];
+    uint64_t key = 0;
+    for (int k = 0; k < per; k++) {
+        const uint64_t q = (uint64_t)known + k;
+        const uint64_t s = (q < l) ? code[T[x + q]] : 0ull;
+        key = (key << bits) | s;
+    }
+    const uint32_t p = lpos[i];
+    LK[p] = ((uint64_t)gid1[i] << cb) | key;
+    LV[p] = x;
+    LI[p] = (uint32_t)i;
+}
+
+__global__ void k_lab_chunk_scatter(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
+                                    const uint32_t* __restrict__ LI, uint32_t cnt, uint32_t* __restrict__ V2,
+                                    uint32_t* __restrict__ hf)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= cnt) return;
+    const uint32_t i = LI[j];
+    V2[i] = LV[j];
+    hf[i] = (j == 0 || LK[j] != LK[j - 1]) ? 1u : 0u;
+}
+
+// exact check of the final label order: neighbours must be non-decreasing, and equal exactly
+// when they share a label group
+__global__ void k_lab_verify(const uint8_t* __restrict__ T, const uint32_t* __restrict__ V,
+                             const uint32_t* __restrict__ hf, const uint32_t* __restrict__ lam, uint64_t n,
+                             uint32_t* __restrict__ bad)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 || i >= n) return;
+    const uint32_t a = V[i - 1], b = V[i];
+    const uint32_t la = lam[a], lb = lam[b];
+    const uint32_t m = la < lb ? la : lb;
+    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint32_t k = lcp_text(T, n, lo, hi, m);
+    int c;
+    if (k < m) c = T[a + k] < T[b + k] ? -1 : 1;
+    else c = la < lb ? -1 : (la > lb ? 1 : 0);
+    const bool head = hf[i] != 0;
+    if (c > 0 || (c == 0) == head) atomicOr(bad, 1u);
+}
+
+__global__ void k_u32_to_key(const uint32_t* __restrict__ a, uint64_t n, uint64_t* __restrict__ K)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) K[i] = a[i];
+}
+
+// ------------------------------------------------------------------- 4. induction
+
+struct ChainPolicy {
+    const uint32_t* jmp;
+    uint32_t n;
+    __device__ __forceinline__ uint32_t next(uint32_t x) const
+    {
+        const uint32_t y = jmp[x];
+        return y < n ? y : kNoNext;
+    }
+    __device__ __forceinline__ bool open(uint32_t) const { return true; }
+};
+
+__global__ void k_jump_init(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ a,
+                            uint32_t* __restrict__ b)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = (uint32_t)(i + lam[i]);
+    a[i] = j;
+    b[i] = j;
+}
+
+// pointer jumping for the active elements: jmp2 = jmp o jmp
+__global__ void k_jump(const uint32_t* __restrict__ act, uint32_t m, const uint32_t* __restrict__ jmp,
+                       uint32_t* __restrict__ jmp2, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t x = act[i];
+    const uint32_t y = jmp[x];
+    jmp2[x] = (y < n) ? jmp[y] : n;
+}
+
+// ------------------------------------------------------------------- 5. output
+
+__global__ void k_bwt_from_sa(const uint32_t* __restrict__ SA, const uint8_t* __restrict__ T, uint64_t n,
+                              uint8_t* __restrict__ O, uint32_t* __restrict__ pidx)
+{
+    const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const uint32_t p = SA[g];
+    O[g] = T[p == 0 ? n - 1 : p - 1];
+    if (p == 0) *pidx = (uint32_t)g + 1;
+}
+
+// ------------------------------------------------------------------- driver
+
+int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    const uint64_t n = (uint64_t)n64;
+    S.phases_reset();
+    S.stats.n = n64;
+    if (!S.t_begin) {
+        MIBWT_HIP(hipEventCreate(&S.t_begin));
+        MIBWT_HIP(hipEventCreate(&S.t_end));
+    }
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    const uint64_t text_bytes = n + kPad + 4096;
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
+    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
+    if (src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
+    MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    if (!src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyDeviceToDevice, S.stream));
+
+    // ---- 1. Lyndon array
+    int ph = S.phase_begin("lyndon");
+    uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
+    const int levels = lyndon_array(S, T, n, lam);
+    uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
+    alphabet_codes(S, T, n, codes);
+    uint32_t sigma = 0;
+    S.read(&sigma, codes + 256);
+    S.phase_end(ph, (double)n * (1 + 4 + 4 + 1) * 2);
+    S.stats.doubling_rounds_dict = levels;
+
+    // ---- 2. prefix fingerprints
+    ph = S.phase_begin("rules");
+    HashPair* pre = S.buf<HashPair>(B_LY_HASH, n);
+    uint64_t base = 0x1F3D5B79A2C4E681ull % kM61;
+    S.phase_end(ph, 0);
+
+    // ---- 3. label order
+    ph = S.phase_begin("rsort");
+    const int bits = bits_for((uint64_t)sigma + 1);
+    const int per = 64 / bits;
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, n);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, n);
+    uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
+    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 1);
+    uint32_t* large = S.buf<uint32_t>(B_DB_LARGE, n);
+    uint32_t* lpos = S.buf<uint32_t>(B_ACT_X, n);
+    uint8_t* cls = S.buf<uint8_t>(B_LY_CLS, n);
+    uint32_t* Vl = S.buf<uint32_t>(B_LY_VL, n);
+    k_lyn_initkey<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, lam, codes, bits, per, K0, V0);
+    MIBWT_LAUNCH_CHECK();
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, n, 0, bits * per);
+    k_lab_heads_from_keys<<<grid_for(n), kBlock, 0, S.stream>>>(K0, n, hf);
+    MIBWT_LAUNCH_CHECK();
+    double rbytes = (double)n * 24 * ((bits * per + 7) / 8);
+    for (int attempt = 0;; attempt++) {
+        MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "label fingerprint collisions persist");
+        k_hash_items<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, base, pre);
+        MIBWT_LAUNCH_CHECK();
+        device_scan<HashPair, OpHash>(S, pre, pre, n, true, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN0);
+        LabelCmp cmp{T, lam, PrefixHash{pre}};
+        // working copies: V (order) and hf (heads) of the packed-key order
+        uint32_t* V = Vl;
+        MIBWT_HIP(hipMemcpyAsync(V, V0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+        uint32_t* H = S.buf<uint32_t>(B_LY_HF2, n);
+        MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+        uint32_t known = (uint32_t)per;
+        for (int round = 0;; round++) {
+            MIBWT_REQUIRE(round < 1 << 20, LIBCUBWT_GPU_ERROR, "label refinement does not converge");
+            inclusive_sum<uint32_t>(S, H, gid1, n, sc + SC_GROUPS);
+            k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(H, gid1, (uint32_t)n, gofs);
+            MIBWT_LAUNCH_CHECK();
+            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, cls, large);
+            MIBWT_LAUNCH_CHECK();
+            exclusive_sum<uint32_t>(S, large, lpos, n, sc + SC_LARGE);
+            uint32_t hv[SC_COUNT_];
+            S.read(hv, sc, SC_COUNT_);
+            const uint32_t nl = hv[SC_LARGE];
+            uint32_t* V2 = (V == Vl) ? V1 : Vl;
+            k_lab_small<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, cls, n, known, cmp, V2, H);
+            MIBWT_LAUNCH_CHECK();
+            if (nl > 0) {
+                uint64_t* LK0 = S.buf<uint64_t>(B_LG_K0, nl);
+                uint64_t* LK1 = S.buf<uint64_t>(B_LG_K1, nl);
+                uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
+                uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
+                uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
+                const int cb = bits * per;
+                MIBWT_REQUIRE(cb + bits_for(hv[SC_GROUPS]) <= 64, LIBCUBWT_GPU_ERROR, "chunk key too wide");
+                k_lab_chunk_gather<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, V, gid1, large, lpos, lam, codes, bits,
+                                                                         per, known, cb, LK0, LV0, LI);
+                MIBWT_LAUNCH_CHECK();
+                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, cb + bits_for(hv[SC_GROUPS]));
+                k_lab_chunk_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, V2, H);
+                MIBWT_LAUNCH_CHECK();
+                rbytes += (double)nl * 24 * 8;
+            }
+            V = V2;
+            if (nl == 0) break;
+            known += (uint32_t)per;
+        }
+        // exact verification of the label order
+        fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+        k_lab_verify<<<grid_for(n), kBlock, 0, S.stream>>>(T, V, H, lam, n, sc + SC_COLLISION);
+        MIBWT_LAUNCH_CHECK();
+        uint32_t bad = 0;
+        S.read(&bad, sc + SC_COLLISION);
+        if (bad == 0) {
+            if (V != Vl) MIBWT_HIP(hipMemcpyAsync(Vl, V, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+            MIBWT_HIP(hipMemcpyAsync(hf, H, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+            break;
+        }
+        base = (base * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) % kM61;
+    }
+    // label group ids as the initial keys of the induction
+    inclusive_sum<uint32_t>(S, hf, gid1, n, sc + SC_GROUPS);
+    k_u32_to_key<<<grid_for(n), kBlock, 0, S.stream>>>(gid1, n, K0);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t ngroups = 0;
+    S.read(&ngroups, sc + SC_GROUPS);
+    S.phase_end(ph, rbytes);
+    S.stats.distinct_phrases = ngroups;
+
+    // ---- 4. induction along the next-smaller-suffix chain
+    ph = S.phase_begin("induce");
+    uint32_t* jA = S.buf<uint32_t>(B_LY_JMP0, n);
+    uint32_t* jB = S.buf<uint32_t>(B_LY_JMP1, n);
+    k_jump_init<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, jA, jB);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* rank = S.buf<uint32_t>(B_DT_RANK, n);
+    uint32_t* SA = S.buf<uint32_t>(B_DT_SA, n);
+    ChainPolicy pol{jA, (uint32_t)n};
+    uint32_t* jcur = jA;
+    uint32_t* jnext = jB;
+    auto advance = [&](ChainPolicy& p, const uint32_t* act, uint32_t m) {
+        k_jump<<<grid_for(m), kBlock, 0, S.stream>>>(act, m, jcur, jnext, (uint32_t)n);
+        MIBWT_LAUNCH_CHECK();
+        std::swap(jcur, jnext);
+        p.jmp = jcur;
+    };
+    DoublingResult rr = refine_groups(S, K0, Vl, V1, (uint32_t)n, pol, advance, rank, SA);
+    S.phase_end(ph, rr.bytes);
+    S.stats.doubling_rounds_parse = rr.rounds;
+
+    // ---- 5. output
+    ph = S.phase_begin("output");
+    fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
+    k_bwt_from_sa<<<grid_for(n), kBlock, 0, S.stream>>>(SA, T, n, dO, sc + SC_PIDX);
+    MIBWT_LAUNCH_CHECK();
+    S.phase_end(ph, (double)n * 6);
+    MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
+    uint32_t pidx = 0;
+    S.read(&pidx, sc + SC_PIDX);
+    S.phases_collect();
+    float tot = 0;
+    MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
+    S.stats.total_ms = tot;
+    MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
+    return (int64_t)pidx;
+}
+
+// Lyndon array of a host text (diagnostics and parity tests of the local/merge passes).
+int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n64, uint32_t* lam_host)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    const uint64_t n = (uint64_t)n64;
+    const uint64_t text_bytes = n + kPad + 4096;
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
+    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
+    uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
+    const int levels = lyndon_array(S, T, n, lam);
+    MIBWT_HIP(hipMemcpyAsync(lam_host, lam, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, S.stream));
+    MIBWT_HIP(hipStreamSynchronize(S.stream));
+    return levels;
 }
 
 }  // namespace mibwt

@@ -27,38 +27,16 @@
 //
 // The terminator is virtual: positions >= n of the extended text T.$^W compare below every
 // byte, so all 256 byte values are accepted (the reference needs bytes >= 2).
-#include "prims.hpp"
+#include "doubling.hpp"
+#include "text.hpp"
 
 namespace mibwt {
 
-constexpr int kPad = 64;             // zero bytes after the text copy
 constexpr int kTrigPerThread = 16;   // window ends per thread
 constexpr int kTrigTile = kBlock * kTrigPerThread;
 constexpr int kMaxW = 64;
 
 // --------------------------------------------------------------------- helpers
-
-// 8 bytes of the padded text starting at any position p (little endian).
-__device__ __forceinline__ uint64_t load8(const uint8_t* __restrict__ T, uint64_t p)
-{
-    const uint64_t a = p & ~7ull;
-    const uint64_t* w = reinterpret_cast<const uint64_t*>(T + a);
-    const uint64_t w0 = w[0];
-    const int sh = (int)(p & 7) * 8;
-    if (sh == 0) return w0;
-    const uint64_t w1 = w[1];
-    return (w0 >> sh) | (w1 << (64 - sh));
-}
-
-__device__ __forceinline__ uint64_t mix64(uint64_t h)
-{
-    h ^= h >> 33;
-    h *= 0xff51afd7ed558ccdull;
-    h ^= h >> 33;
-    h *= 0xc4ceb9fe1a85ec53ull;
-    h ^= h >> 33;
-    return h;
-}
 
 // Phrase k of the parse: [start, end] in the extended text, end >= n means sentinels.
 __device__ __forceinline__ void phrase_bounds(const uint32_t* __restrict__ E, uint32_t k, int W,
@@ -307,169 +285,80 @@ __global__ void k_dt_owner(const uint64_t* __restrict__ off, const uint32_t* __r
     for (uint32_t j = 0; j < len[u]; j++) owner[o + j] = u;
 }
 
-// Initial key of DT position x: 7 symbols of 9 bits (0 = past the phrase end,
-// 1 = terminator, byte + 2), most significant first.
+// Initial key of DT position x: `per` symbols of `bits` bits each, most significant first;
+// symbol code 0 = past the phrase end, 1 = terminator, 2.. = dense code of the byte (codes
+// follow byte order, so key order is string order).
 __global__ void k_dt_initkey(const uint8_t* __restrict__ T, uint64_t n,
                              const uint32_t* __restrict__ owner, const uint64_t* __restrict__ off,
                              const uint32_t* __restrict__ len, const uint32_t* __restrict__ start,
+                             const uint32_t* __restrict__ code, int bits, int per,
                              uint32_t L, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
 {
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    __syncthreads();
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L) return;
     const uint32_t u = owner[x];
     const uint32_t j = x - (uint32_t)off[u];
     const uint32_t rem = len[u] - j;
     const uint64_t tp = (uint64_t)start[u] + j;
-    const uint64_t w = (tp < n) ? load8(T, tp) : 0ull;
     uint64_t key = 0;
-#pragma unroll
-    for (int i = 0; i < 7; i++) {
+    uint64_t w = 0;
+    for (int i = 0; i < per; i++) {
+        if ((i & 7) == 0) w = (tp + i < n) ? load8(T, tp + i) : 0ull;
         uint64_t s;
         if ((uint32_t)i >= rem) s = 0;
         else if (tp + i >= n) s = 1;
-        else s = ((w >> (8 * i)) & 0xFF) + 2;
-        key = (key << 9) | s;
+        else s = cs[(w >> (8 * (i & 7))) & 0xFF];
+        key = (key << bits) | s;
     }
     K[x] = key;
     V[x] = x;
 }
 
+// Initial parse key: `per` consecutive phrase ranks (+1; 0 past the end) of B bits each.
 __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid,
-                                uint32_t d, int B, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
-                                uint32_t* __restrict__ par)
+                                uint32_t d, int B, int per, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
-    const uint64_t a = lexrank[pid[k]] + 1;
-    const uint64_t b = (k + 1 < d) ? (uint64_t)lexrank[pid[k + 1]] + 1 : 0;
-    K[k] = (a << B) | b;
+    uint64_t key = 0;
+    for (int i = 0; i < per; i++) {
+        const uint64_t s = (k + i < d) ? (uint64_t)lexrank[pid[k + i]] + 1 : 0;
+        key = (key << B) | s;
+    }
+    K[k] = key;
     V[k] = k;
-    par[k] = (uint32_t)a;
 }
 
-// Rank update after a sort of the active set A (keys K, elements V, global positions G,
-// all of length m).  head index via hidx (inclusive max scan of head flags * i).
-__global__ void k_dbl_headflag(const uint64_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    hf[i] = (i == 0 || K[i] != K[i - 1]) ? i : 0;
-}
-
+// Family of strings compared h symbols at a time: suffix x continues at x + h inside its
+// segment [.., lim(x)); a group of equal h-prefixes is final once its strings are shorter
+// than h (they are then equal).
 template <typename Lim>
-__global__ void k_dbl_update(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V,
-                             const uint32_t* __restrict__ G, const uint32_t* __restrict__ hidx,
-                             uint32_t m, uint32_t* __restrict__ rank, uint32_t* __restrict__ SA,
-                             uint32_t* __restrict__ keep, uint32_t h_cov, Lim lim)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t x = V[i];
-    const uint32_t g = G[i];
-    const uint32_t hi = hidx[i];
-    rank[x] = G[hi] + 1;
-    SA[g] = x;
-    // the group [hi, next head) stays active if it has >= 2 members and its strings are
-    // not yet known to be equal: the compared prefix of length h_cov ends inside them
-    const bool single = (hi == i) && (i + 1 == m || K[i + 1] != K[i]);
-    bool active = !single;
-    if (active) {
-        const uint32_t xh = V[hi];
-        const uint32_t rem = lim(xh) - xh;
-        active = rem >= h_cov;
+struct StringPolicy {
+    Lim lim;
+    uint32_t h;
+    __device__ __forceinline__ uint32_t next(uint32_t x) const
+    {
+        return ((uint64_t)x + h < lim(x)) ? x + h : kNoNext;
     }
-    keep[i] = active ? 1u : 0u;
-}
-
-__global__ void k_dbl_compact(const uint32_t* __restrict__ V, const uint32_t* __restrict__ G,
-                              const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
-                              uint32_t m, uint32_t* __restrict__ V2, uint32_t* __restrict__ G2)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    if (keep[i]) {
-        V2[pos[i]] = V[i];
-        G2[pos[i]] = G[i];
-    }
-}
-
-template <typename Lim>
-__global__ void k_dbl_key(const uint32_t* __restrict__ V, uint32_t m, const uint32_t* __restrict__ rank,
-                          uint32_t h, int B, Lim lim, uint64_t* __restrict__ K)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t x = V[i];
-    const uint64_t r1 = rank[x];
-    const uint32_t lx = lim(x);
-    const uint64_t r2 = ((uint64_t)x + h < lx) ? rank[x + h] : 0;
-    K[i] = (r1 << B) | r2;
-}
-
-__global__ void k_iota(uint32_t* __restrict__ p, uint32_t m)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) p[i] = i;
-}
-
-struct DoublingResult {
-    int rounds = 0;
-    double bytes = 0;
+    __device__ __forceinline__ bool open(uint32_t xh) const { return lim(xh) - xh >= h; }
 };
 
-// Prefix doubling over N suffix positions of a family of strings.  On entry K0/V0 hold the
-// initial (key, position) pairs whose keys (bits [0, key_bits)) order the suffixes by their
-// first h0 symbols, padded with a terminator smaller than every symbol.  On exit
-// rank[x] = 1 + position of the head of x's group in the sorted order and SA[] is the
-// sorted order; suffixes with identical strings end in one group.  A group stops being
-// refined when it is a singleton or when its strings are shorter than the prefix length
-// already compared (then they are equal).
+// Prefix doubling over N suffix positions: K0/V0 hold the initial keys (bits [0, key_bits),
+// the first h0 symbols with a terminator padding) and positions; on exit rank[x] = 1 +
+// position of x's group head and SA = the sorted order (equal strings share a group).
 template <typename Lim>
-DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t* V0, uint32_t* V1,
-                               uint32_t N, int key_bits, uint32_t h0, Lim lim, uint32_t* rank,
-                               uint32_t* SA)
+DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t* V0, uint32_t* V1, uint32_t N,
+                               int key_bits, uint32_t h0, Lim lim, uint32_t* rank, uint32_t* SA)
 {
-    DoublingResult res;
-    uint32_t* G0 = S.buf<uint32_t>(B_ACT_POS, N);
-    uint32_t* G1 = S.buf<uint32_t>(B_ACT_POS2, N);
-    uint32_t* V2 = S.buf<uint32_t>(B_ACT_X2, N);
-    uint32_t* hidx = S.buf<uint32_t>(B_ACT_HEAD, N);
-    uint32_t* keep = S.buf<uint32_t>(B_ACT_FLAG, N);
-    uint32_t* pos = S.buf<uint32_t>(B_ACT_X, N);
-    uint32_t* cnt_dev = S.buf<uint32_t>(B_SCALARS, SC_COUNT_) + SC_ACTIVE;
-
-    uint32_t m = N;
-    uint32_t h = h0;
-    int bits = key_bits;
-    k_iota<<<grid_for(m), kBlock, 0, S.stream>>>(G0, m);
-    MIBWT_LAUNCH_CHECK();
-    const int B = bits_for((uint64_t)N + 1);
-    for (;;) {
-        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, m, 0, bits);
-        res.bytes += (double)m * (8 + 4) * 2 * ((bits + 7) / 8);
-        k_dbl_headflag<<<grid_for(m), kBlock, 0, S.stream>>>(K0, m, hidx);
-        MIBWT_LAUNCH_CHECK();
-        inclusive_max<uint32_t>(S, hidx, hidx, m);
-        k_dbl_update<Lim><<<grid_for(m), kBlock, 0, S.stream>>>(K0, V0, G0, hidx, m, rank, SA, keep,
-                                                                 h, lim);
-        MIBWT_LAUNCH_CHECK();
-        exclusive_sum<uint32_t>(S, keep, pos, m, cnt_dev);
-        uint32_t m2 = 0;
-        S.read(&m2, cnt_dev);
-        res.rounds++;
-        if (m2 == 0) break;
-        k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(V0, G0, keep, pos, m, V2, G1);
-        MIBWT_LAUNCH_CHECK();
-        std::swap(G0, G1);
-        std::swap(V0, V2);
-        m = m2;
-        k_dbl_key<Lim><<<grid_for(m), kBlock, 0, S.stream>>>(V0, m, rank, h, B, lim, K0);
-        MIBWT_LAUNCH_CHECK();
-        res.bytes += (double)m * (4 + 4 + 4 + 8) + (double)m * 16;
-        bits = 2 * B;
-        h = 2 * h;
-    }
-    return res;
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, N, 0, key_bits);
+    StringPolicy<Lim> pol{lim, h0};
+    DoublingResult r = refine_groups(
+        S, K0, V0, V1, N, pol, [](StringPolicy<Lim>& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA);
+    r.bytes += (double)N * 24 * ((key_bits + 7) / 8);
+    return r;
 }
 
 // --------------------------------------------------------------------- 5. phrase order
@@ -665,6 +554,7 @@ __global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const uin
 
 // last radix pass of the entry sort: slot = sorted position + delta(alpha)
 struct EmitBwt {
+    static constexpr double bytes_per_item = 1 + 4;  // BWT byte written, delta read
     const uint32_t* delta;
     uint8_t* O;
     uint32_t* pidx;
@@ -718,11 +608,23 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     int ph = S.phase_begin("scan");
     uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
     uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
-    k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, (uint32_t)P, masks, tcnt);
+    {
+        const int kt = S.kt_begin(2, (double)n + 2.0 * tiles * kBlock + 4.0 * tiles);
+        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, (uint32_t)P, masks, tcnt);
+        S.kt_end(kt);
+    }
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, tcnt, tcnt, tiles, sc + SC_TRIGGERS);
-    uint32_t t = 0;
-    S.read(&t, sc + SC_TRIGGERS);
+    uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
+    alphabet_codes(S, T, n, codes);
+    MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    uint32_t t = 0, sigma = 0;
+    {
+        uint32_t hv[SC_COUNT_];
+        S.read(hv, sc, SC_COUNT_);
+        t = hv[SC_TRIGGERS];
+        sigma = hv[SC_SIGMA];
+    }
     const uint32_t d = t + 1;
     uint32_t* E = S.buf<uint32_t>(B_TRIG, d);
     k_trig_emit<<<tiles, kBlock, 0, S.stream>>>(masks, tcnt, E);
@@ -790,10 +692,17 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     K1 = S.buf<uint64_t>(B_SORT_K1, L);
     V0 = S.buf<uint32_t>(B_SORT_V0, L);
     V1 = S.buf<uint32_t>(B_SORT_V1, L);
-    k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, L, K0, V0);
+    // symbols per initial key: dense codes of the bytes that occur (the reference's small-
+    // alphabet variant packs two symbols per radix digit, PFPs16:995-1002; here a 64-bit key
+    // holds as many codes as fit, 21 for DNA)
+    const int abits = bits_for((uint64_t)sigma + 1);
+    const int aper = 64 / abits;
+    k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, codes, abits, aper,
+                                                        L, K0, V0);
     MIBWT_LAUNCH_CHECK();
     LimDict limd{owner, doff, dlen};
-    DoublingResult rd = prefix_doubling<LimDict>(S, K0, K1, V0, V1, L, 63, 7, limd, drank, dsa);
+    DoublingResult rd = prefix_doubling<LimDict>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper, limd,
+                                                 drank, dsa);
     S.phase_end(ph, rd.bytes);
     S.stats.doubling_rounds_dict = rd.rounds;
 
@@ -810,13 +719,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     K1 = S.buf<uint64_t>(B_SORT_K1, d);
     V0 = S.buf<uint32_t>(B_SORT_V0, d);
     V1 = S.buf<uint32_t>(B_SORT_V1, d);
-    uint32_t* par = S.buf<uint32_t>(B_Q_CNT, d);
     const int B0 = bits_for((uint64_t)du + 1);
-    k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, K0, V0, par);
+    const int pper = 64 / B0;
+    k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, pper, K0, V0);
     MIBWT_LAUNCH_CHECK();
     uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);
     uint32_t* psa = S.buf<uint32_t>(B_P_SA, d);
-    DoublingResult rp = prefix_doubling<LimParse>(S, K0, K1, V0, V1, d, 2 * B0, 2, LimParse{d}, prank, psa);
+    DoublingResult rp = prefix_doubling<LimParse>(S, K0, K1, V0, V1, d, B0 * pper, (uint32_t)pper, LimParse{d},
+                                                  prank, psa);
     S.phase_end(ph, rp.bytes);
     S.stats.doubling_rounds_parse = rp.rounds;
 
@@ -868,9 +778,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_LAUNCH_CHECK();
     fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
     EmitBwt em{delta, dO, sc + SC_PIDX, n};
-    const int abits = bits_for(L);
-    radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + abits, &em);
-    S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + 8.0 * ne * 2 * ((abits + 7) / 8) + 4.0 * d * 3);
+    const int gbits = bits_for(L);
+    radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em);
+    S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + 8.0 * ne * 2 * ((gbits + 7) / 8) + 4.0 * d * 3);
     S.stats.alpha_groups = L;
     S.stats.mixed_entries = ne;
 
@@ -888,6 +798,34 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     S.last_P = P;
     S.last_sort_items = d;
     return (int64_t)pidx;
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t m)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    if (i < m) b[i] = a[i];
+}
+
+// Plain device copy of `bytes` bytes, `reps` times: the reference point for hardware byte
+// counters (it reads and writes exactly `bytes` each).
+double time_copy(Storage& S, uint64_t bytes, int reps)
+{
+    const uint64_t m = bytes / 16;
+    uint4* a = S.buf<uint4>(B_SORT_K0, m);
+    uint4* b = S.buf<uint4>(B_SORT_K1, m);
+    hipEvent_t e0, e1;
+    MIBWT_HIP(hipEventCreate(&e0));
+    MIBWT_HIP(hipEventCreate(&e1));
+    MIBWT_HIP(hipEventRecord(e0, S.stream));
+    for (int i = 0; i < reps; i++) k_copy16<<<ceil_div(m, kBlock), kBlock, 0, S.stream>>>(a, b, m);
+    MIBWT_LAUNCH_CHECK();
+    MIBWT_HIP(hipEventRecord(e1, S.stream));
+    MIBWT_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    MIBWT_HIP(hipEventElapsedTime(&ms, e0, e1));
+    MIBWT_HIP(hipEventDestroy(e0));
+    MIBWT_HIP(hipEventDestroy(e1));
+    return ms / reps;
 }
 
 // Re-run the trigger scan kernel (pass 1) `reps` times on the text of the last call.

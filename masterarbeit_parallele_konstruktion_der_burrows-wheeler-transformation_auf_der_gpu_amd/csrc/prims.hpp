@@ -26,6 +26,13 @@ __device__ __forceinline__ uint64_t lanemask_lt()
     return (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 }
 
+// Lane shuffle; aggregate value types overload it next to their definition.
+template <typename T>
+__device__ __forceinline__ T shfl_up_val(T x, int o)
+{
+    return __shfl_up(x, o, 64);
+}
+
 // Inclusive scan across one wavefront.
 template <typename T, typename Op>
 __device__ __forceinline__ T wave_inclusive_scan(T x, Op op)
@@ -33,7 +40,7 @@ __device__ __forceinline__ T wave_inclusive_scan(T x, Op op)
     const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        T y = __shfl_up(x, o, 64);
+        T y = shfl_up_val(x, o);
         if (lane >= o) x = op(y, x);
     }
     return x;
@@ -55,7 +62,7 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T identity, Op op, T* lds
         if (w < wid) wave_prefix = op(wave_prefix, s);
         tot = op(tot, s);
     }
-    T ex = __shfl_up(inc, 1, 64);
+    T ex = shfl_up_val(inc, 1);
     if (lane == 0) ex = identity;
     *total = tot;
     __syncthreads();
@@ -200,6 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ key
 // Default emitter: writes the sorted key/value pair.
 template <typename K, typename V>
 struct EmitPair {
+    static constexpr double bytes_per_item = sizeof(K) + sizeof(V);
     K* keys_out;
     V* vals_out;
     __device__ __forceinline__ void operator()(uint64_t pos, K key, V val) const
@@ -322,13 +330,18 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         MIBWT_LAUNCH_CHECK();
         exclusive_sum<uint32_t>(S, hist, hist, (uint64_t)kRadix * tiles);
         const bool is_last = (p == passes - 1);
+        const double vbytes = vin ? (double)sizeof(V) : 0.0;
         if (is_last && last) {
+            const int kt = S.kt_begin(1, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
             k_radix_scatter<K, V, LastEmit><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
                                                                            hist, tiles, *last);
+            S.kt_end(kt);
         } else {
+            const int kt = S.kt_begin(1, 2.0 * (double)n * (sizeof(K) + vbytes));
             EmitPair<K, V> e{kout, vout};
             k_radix_scatter<K, V, EmitPair<K, V>><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift,
                                                                                 mask, hist, tiles, e);
+            S.kt_end(kt);
         }
         MIBWT_LAUNCH_CHECK();
         std::swap(kin, kout);

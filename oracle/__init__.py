@@ -56,6 +56,12 @@ def lib():
         L.oracle_pfp_bwt.restype = i64
         L.oracle_lyndon_factors.argtypes = [u8p, i64, i64p]
         L.oracle_lyndon_factors.restype = i64
+        L.oracle_lyndon_array.argtypes = [u8p, i64, i32p]
+        L.oracle_lyndon_array.restype = ctypes.c_int
+        L.oracle_lyndon_ranks.argtypes = [u8p, i64, i32p]
+        L.oracle_lyndon_ranks.restype = i64
+        L.oracle_lyndon_bwt.argtypes = [u8p, i64, u8p]
+        L.oracle_lyndon_bwt.restype = i64
         _lib = L
     return _lib
 
@@ -159,3 +165,26 @@ def lyndon_factors(T) -> np.ndarray:
     F = np.empty(max(c, 1), dtype=np.int64)
     lib().oracle_lyndon_factors(_u8(T), len(T), F.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     return F[:c]
+
+
+def lyndon_array(T) -> np.ndarray:
+    T = _as_text(T)
+    lam = np.empty(len(T), dtype=np.int32)
+    lib().oracle_lyndon_array(_u8(T), len(T), _i32(lam))
+    return lam
+
+
+def lyndon_ranks(T):
+    """(dense rank of each position's longest Lyndon prefix, number of distinct prefixes)."""
+    T = _as_text(T)
+    r = np.empty(len(T), dtype=np.int32)
+    g = lib().oracle_lyndon_ranks(_u8(T), len(T), _i32(r))
+    return r, int(g)
+
+
+def lyndon_bwt(T):
+    """Reference-format BWT and primary index from the Lyndon-grammar restatement."""
+    T = _as_text(T)
+    O = np.empty(len(T), dtype=np.uint8)
+    idx = lib().oracle_lyndon_bwt(_u8(T), len(T), _u8(O)) if len(T) else 0
+    return O, int(idx)

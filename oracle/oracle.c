@@ -11,7 +11,13 @@
  *     dictionary-suffix set S with its intervals, S sort and BWT assembly.  Each step cites
  *     the reference lines it follows.  The terminator is virtual (value -1) instead of the
  *     reference's physical byte 1, so the restatement also covers bytes 0, 1 and 255.
- *  3. Duval's Lyndon factorisation (used to pin the stack of the Lyndon-grammar method).
+ *  3. Duval's Lyndon factorisation and a restatement of the Lyndon-grammar BWT
+ *     (Lyndon_Grammar_BWT.cu, "LYN" below): longest Lyndon prefix of every position (the
+ *     maximal nodes of the Lyndon forest: stack words and right children, LYN:647-1868,
+ *     1910-2062), their lexicographic ranks (the sorted grammar rules, LYN:3621-4083), and the
+ *     BWT by induction over rank groups (LYN:4628-4896), here as one left-to-right induced
+ *     pass: suffixes are grouped by their Lyndon prefix and, inside a group, ordered by the
+ *     suffix that follows the prefix, which is smaller and therefore already placed.
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -525,4 +531,99 @@ int64_t oracle_lyndon_factors(const uint8_t* T, int64_t n, int64_t* F)
         }
     }
     return cnt;
+}
+
+int oracle_lyndon_array(const uint8_t* T, int64_t n, int32_t* lam)
+{
+    /* right to left: the word starting at i absorbs following words while it is smaller */
+    for (int64_t i = n - 1; i >= 0; i--) {
+        int64_t j = i + 1;
+        while (j < n) {
+            /* compare w = T[i..j) with t = T[j..j+lam[j]) */
+            int64_t lw = j - i, lt = lam[j], k = 0;
+            while (k < lw && k < lt && T[i + k] == T[j + k]) k++;
+            int smaller = (k < lw && k < lt) ? (T[i + k] < T[j + k]) : (lw < lt);
+            if (!smaller) break;
+            j += lt;
+        }
+        lam[i] = (int32_t)(j - i);
+    }
+    return 0;
+}
+
+typedef struct {
+    const uint8_t* T;
+    const int32_t* lam;
+} lyn_ctx;
+
+static int cmp_lyn(const void* a, const void* b, void* arg)
+{
+    const lyn_ctx* c = (const lyn_ctx*)arg;
+    int64_t x = *(const int32_t*)a, y = *(const int32_t*)b;
+    int64_t lx = c->lam[x], ly = c->lam[y];
+    int64_t m = lx < ly ? lx : ly;
+    for (int64_t k = 0; k < m; k++)
+        if (c->T[x + k] != c->T[y + k]) return c->T[x + k] < c->T[y + k] ? -1 : 1;
+    return lx < ly ? -1 : (lx > ly ? 1 : 0);
+}
+
+int64_t oracle_lyndon_ranks(const uint8_t* T, int64_t n, int32_t* rank)
+{
+    if (n <= 0) return 0;
+    int32_t* lam = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* ord = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    oracle_lyndon_array(T, n, lam);
+    for (int64_t i = 0; i < n; i++) ord[i] = (int32_t)i;
+    lyn_ctx c = {T, lam};
+    qsort_r(ord, (size_t)n, sizeof(int32_t), cmp_lyn, &c);
+    int64_t r = -1;
+    for (int64_t i = 0; i < n; i++) {
+        if (i == 0 || cmp_lyn(&ord[i - 1], &ord[i], &c) != 0) r++;
+        rank[ord[i]] = (int32_t)r;
+    }
+    free(lam);
+    free(ord);
+    return r + 1;
+}
+
+int64_t oracle_lyndon_bwt(const uint8_t* T, int64_t n, uint8_t* O)
+{
+    if (n <= 0) return 0;
+    int32_t* lam = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* rank = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    oracle_lyndon_array(T, n, lam);
+    int64_t groups = oracle_lyndon_ranks(T, n, rank);
+    /* bucket starts by rank group */
+    int64_t* bucket = (int64_t*)calloc((size_t)groups + 1, sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) bucket[rank[i] + 1]++;
+    for (int64_t g = 0; g < groups; g++) bucket[g + 1] += bucket[g];
+    /* children lists: positions i whose following suffix starts at j = i + lam[i] */
+    int64_t* cstart = (int64_t*)calloc((size_t)n + 2, sizeof(int64_t));
+    int32_t* child = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    for (int64_t i = 0; i < n; i++) cstart[i + lam[i] + 1]++;
+    for (int64_t j = 0; j <= n; j++) cstart[j + 1] += cstart[j];
+    int64_t* cfill = (int64_t*)malloc(sizeof(int64_t) * ((size_t)n + 1));
+    memcpy(cfill, cstart, sizeof(int64_t) * ((size_t)n + 1));
+    for (int64_t i = 0; i < n; i++) child[cfill[i + lam[i]]++] = (int32_t)i;
+    int32_t* SA = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    /* the empty suffix ($) comes first: place the positions whose Lyndon prefix ends at n */
+    for (int64_t c = cstart[n]; c < cstart[n + 1]; c++) {
+        int32_t i = child[c];
+        SA[bucket[rank[i]]++] = i;
+    }
+    for (int64_t s = 0; s < n; s++) {
+        int32_t j = SA[s];
+        for (int64_t c = cstart[j]; c < cstart[j + 1]; c++) {
+            int32_t i = child[c];
+            SA[bucket[rank[i]]++] = i;
+        }
+    }
+    int64_t index = -1;
+    for (int64_t s = 0; s < n; s++) {
+        int64_t p = SA[s];
+        if (p == 0) index = s + 1;
+        O[s] = T[p == 0 ? n - 1 : p - 1];
+    }
+    free(SA); free(cfill); free(child); free(cstart); free(bucket); free(rank); free(lam);
+    return index;
 }

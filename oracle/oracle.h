@@ -62,6 +62,17 @@ int64_t oracle_pfp_bwt(const uint8_t* T, int64_t n, int W, int P, uint8_t* O,
 /* Lyndon factorisation of T: writes factor start positions to F (may be NULL); returns count. */
 int64_t oracle_lyndon_factors(const uint8_t* T, int64_t n, int64_t* F);
 
+/* Lyndon array: lam[i] = length of the longest Lyndon word starting at i (right-to-left
+ * stack loop).  0 on success. */
+int oracle_lyndon_array(const uint8_t* T, int64_t n, int32_t* lam);
+
+/* Dense lexicographic rank of every position's longest Lyndon prefix (equal strings share a
+ * rank); returns the number of distinct prefixes ("rules" plus stack words). */
+int64_t oracle_lyndon_ranks(const uint8_t* T, int64_t n, int32_t* rank);
+
+/* Lyndon-grammar BWT, reference format; returns the libcubwt primary index. */
+int64_t oracle_lyndon_bwt(const uint8_t* T, int64_t n, uint8_t* O);
+
 #ifdef __cplusplus
 }
 #endif

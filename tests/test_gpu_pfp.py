@@ -1,0 +1,138 @@
+"""GPU parity of the prefix-free-parsing BWT against the CPU oracle (bit-exact).
+
+Small and edge-case inputs are compared with the SA-IS oracle directly; the 32 MiB
+repetitive-DNA workload is compared in full; the 256 MiB workload is checked through
+size-independent properties (inverse-BWT round trip, symbol histogram).
+"""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def check(storage, mibwt, T, W=5, P=16):
+    T = np.frombuffer(bytes(T), dtype=np.uint8) if isinstance(T, (bytes, bytearray)) else T
+    L, idx = storage.bwt(T, "pfp", W, P)
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx, (len(T), W, P)
+    assert (mibwt.cubwt_to_reference(L, idx) == O).all(), (len(T), W, P)
+
+
+def test_kat_mississippi(storage, mibwt):
+    L, idx = storage.bwt(b"mississippi", "pfp", 5, 16)
+    assert idx == 5 and bytes(L) == b"ipssmpissii"
+    assert bytes(mibwt.cubwt_to_reference(L, idx)) == b"pssmipissii"
+
+
+def test_libcubwt_entry_min_length(storage, mibwt):
+    with pytest.raises(mibwt.BwtError) as e:
+        storage.bwt(b"mississippi", "cubwt")
+    assert e.value.code == mibwt.BAD_PARAMETER  # libcubwt rejects n < 16 (libcubwt.cu:2418)
+    T = b"mississippimississippi"
+    L, idx = storage.bwt(T, "cubwt")
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (L == oracle.ref_to_cubwt(O, oidx)).all()
+
+
+def test_n_above_max_length(storage, mibwt):
+    with pytest.raises(mibwt.BwtError) as e:
+        storage.bwt(np.zeros(storage.max_length + 1, dtype=np.uint8), "pfp")
+    assert e.value.code == mibwt.BAD_PARAMETER
+
+
+@pytest.mark.parametrize("W,P", [(1, 2), (2, 3), (3, 4), (5, 16), (5, 11), (5, 13), (5, 14),
+                                 (5, 23), (10, 31), (35, 81), (64, 997), (4, 1), (6, 65535)])
+def test_random_small(storage, mibwt, W, P):
+    rng = np.random.default_rng(W * 1000 + P)
+    for _ in range(25):
+        n = int(rng.integers(1, 3000))
+        k = int(rng.integers(1, 6))
+        T = (rng.integers(0, k, size=n) * int(rng.integers(1, 64))).astype(np.uint8)
+        check(storage, mibwt, T, W, P)
+
+
+EDGE = {
+    "one": b"x",
+    "two": b"ba",
+    "unary": b"a" * 5000,
+    "period2": b"ab" * 3000,
+    "period7": b"abcabcd" * 700,
+    "all_bytes": bytes(range(256)) * 20,
+    "zeros_ones": bytes([0, 1]) * 1000 + bytes([255]) * 10,
+    "first_window_repeats": b"ACGTA" * 1000,
+    "descending": bytes(range(255, -1, -1)) * 10,
+}
+
+
+@pytest.mark.parametrize("name", sorted(EDGE))
+@pytest.mark.parametrize("W,P", [(5, 16), (1, 2), (35, 81)])
+def test_edge_cases(storage, mibwt, name, W, P):
+    check(storage, mibwt, EDGE[name], W, P)
+
+
+def test_long_trigger_free_run(storage, mibwt):
+    # phrases far longer than the reference's 8192-byte limit (PFPk:581-585)
+    rng = np.random.default_rng(4)
+    T = np.concatenate([rng.integers(65, 69, size=500), np.full(40000, 67), rng.integers(65, 69, size=500)])
+    check(storage, mibwt, T.astype(np.uint8), 5, 16)
+    check(storage, mibwt, T.astype(np.uint8), 3, 60000)  # almost no triggers at all
+
+
+def test_in_place_libcubwt(storage, mibwt):
+    rng = np.random.default_rng(11)
+    T = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=100000)
+    O, oidx = oracle.bwt(T)
+    buf = T.copy()
+    import ctypes
+    p = buf.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    idx = mibwt.lib().libcubwt_bwt(storage.handle, p, p, len(buf))  # L aliases T
+    assert idx == oidx and (buf == oracle.ref_to_cubwt(O, oidx)).all()
+
+
+def test_repetitive_collection_various_wp(mibwt):
+    from mibwt import gen
+    T = gen.repetitive_dna(1 << 16, 16, 0.002, 0.0002, 10, seed=5)
+    O, oidx = oracle.bwt(T)
+    with mibwt.Storage(len(T)) as st:
+        for W, P in [(5, 16), (5, 11), (10, 23), (20, 41), (35, 81)]:
+            L, idx = st.bwt(T, "pfp", W, P)
+            assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all(), (W, P)
+            s = st.stats()
+            assert s["phrases"] == len(oracle.pfp_triggers(T, W, P)) + 1
+
+
+def test_dna32m_full(mibwt):
+    from mibwt import gen
+    T = gen.workload("dna32m")
+    O, oidx = oracle.bwt(T)
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "pfp", 5, 16)
+        s = st.stats()
+    assert idx == oidx
+    assert (mibwt.cubwt_to_reference(L, idx) == O).all()
+    assert s["triggers"] == len(oracle.pfp_triggers(T, 5, 16))
+
+
+def test_device_entry_reference_format(mibwt):
+    rng = np.random.default_rng(3)
+    T = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=1 << 18)
+    T = np.concatenate([T, T[1000:], T[:77777]])
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(len(T))
+    with mibwt.Storage(len(T)) as st:
+        idx = st.bwt_device(dT.address, dO.address, len(T), "pfp", 5, 16)
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (dO.download() == O).all()
+    dT.free()
+    dO.free()
+
+
+def test_dna256m_roundtrip(mibwt):
+    from mibwt import gen
+    T = gen.workload("dna256m")
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "pfp", 5, 16)
+    assert (np.bincount(L, minlength=256) == np.bincount(T, minlength=256)).all()
+    assert (oracle.unbwt(L, idx) == T).all()
