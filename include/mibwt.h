@@ -88,6 +88,15 @@ int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, i
  * levels or an error code. */
 int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t* lam);
 
+/* Batched BWT of k independent texts (host buffers, libcubwt format each; the reference has
+ * no batch mode, its benchmark loops over files, Benchmark_Skript_Linux.sh:7-138).  Texts are
+ * handed out longest first from one shared queue to `workers` (0 = 4, at most 16) worker
+ * storages, each with its own HIP stream and host thread.  method: MIBWT_METHOD_PFP (W = 0:
+ * learned (W,P) per text) or MIBWT_METHOD_LYNDON.  index[i] receives the primary index of
+ * text i or its error code; the return value is 0 or the first error. */
+int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* L, const int64_t* n,
+                        int64_t k, int64_t* index, int32_t method, int32_t W, int32_t P, int32_t workers);
+
 /* Per-phase statistics of the last call on this storage. */
 #define MIBWT_MAX_PHASES 16
 struct mibwt_phase_stats {
@@ -118,6 +127,41 @@ int64_t mibwt_set_kernel_timing(void* storage, const char* kernel);
  * bytes_per_launch bytes (input), used to calibrate hardware byte counters. */
 int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms,
                           double* bytes_per_launch);
+
+/* ---------------------------------------------------------------- learned (W,P) selector
+ * The reference chooses (W,P) per input by hand (Benchmark_Skript_Linux.sh:14-73) and compiles
+ * one binary per pair (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:19-20).  An MLP
+ * 64->128->128->3 (bf16 weights, fp32 accumulation, hand-written HIP kernels) predicts
+ * log(dictionary length/n), log(parse length/n) and log(build ms per byte) for each of the 40
+ * grid candidates W in {5,10,20,35} x P in {11,13,17,23,31,41,53,61,71,81}; the fastest is
+ * chosen.  The model file is `selector_model.bin` next to lib/ (or $MIBWT_SELECTOR). */
+
+/* The 40 grid candidates (W[i], P[i]); returns 40. */
+int64_t mibwt_wp_grid(int32_t* W, int32_t* P);
+
+/* Features of the 40 candidates for the device text dT[0..n): X[40][64]. */
+int64_t mibwt_wp_features(void* storage, const uint8_t* dT, int64_t n, float* X);
+
+/* (W,P) for the device text dT[0..n); pred (may be NULL) receives the 40 predicted
+ * log(ms per byte).  Returns 1 when a trained model chose, 0 for the fixed W=5,P=16 fallback. */
+int64_t mibwt_select_wp_device(void* storage, const uint8_t* dT, int64_t n, int32_t* W, int32_t* P,
+                               float* pred);
+
+/* Load a model file (replaces the default one). */
+int64_t mibwt_selector_load(const char* path);
+
+/* MLP on raw (un-normalised) parameters in the fp32 master layout W1[128][64] b1[128]
+ * W2[128][128] b2[128] W3[3][128] b3[3] (weights are used as bf16): mean-squared-error loss
+ * and its gradient for a batch, and plain inference. */
+int64_t mibwt_mlp_grad(void* storage, const float* params, const float* X, const float* Y, int64_t B,
+                       float* grads, float* loss);
+int64_t mibwt_mlp_predict(void* storage, const float* params, const float* X, int64_t B, float* Y);
+
+/* Train on B samples (X[B][64] raw features, Y[B][3] targets) with Adam (minibatch `batch`,
+ * 0 = full batch); normalisation is learned from the data; writes the model to out_path (may
+ * be NULL) and installs it. */
+int64_t mibwt_mlp_train(void* storage, const float* X, const float* Y, int64_t B, int32_t epochs,
+                        int32_t batch, float lr, uint32_t seed, const char* out_path, float* final_loss);
 
 /* Device-memory plumbing for callers that keep texts resident in HBM without another HIP
  * runtime in the process (no counterpart in the reference, whose programs allocate inside

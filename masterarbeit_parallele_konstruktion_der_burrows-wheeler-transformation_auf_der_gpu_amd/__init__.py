@@ -45,6 +45,14 @@ EXPORTED = [
     "mibwt_last_phase_stats",
     "mibwt_set_kernel_timing",
     "mibwt_time_kernel",
+    "mibwt_batch_bwt",
+    "mibwt_wp_grid",
+    "mibwt_wp_features",
+    "mibwt_select_wp_device",
+    "mibwt_selector_load",
+    "mibwt_mlp_grad",
+    "mibwt_mlp_predict",
+    "mibwt_mlp_train",
     "mibwt_device_alloc",
     "mibwt_device_free",
     "mibwt_copy",
@@ -101,6 +109,19 @@ def lib():
         L.mibwt_set_kernel_timing.argtypes = [vp, ctypes.c_char_p]
         L.mibwt_time_kernel.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_double)]
+        fp = ctypes.POINTER(ctypes.c_float)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.mibwt_batch_bwt.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
+                                      ctypes.POINTER(ctypes.c_int64), i64, ctypes.POINTER(ctypes.c_int64),
+                                      i32, i32, i32, i32]
+        L.mibwt_wp_grid.argtypes = [i32p, i32p]
+        L.mibwt_wp_features.argtypes = [vp, vp, i64, fp]
+        L.mibwt_select_wp_device.argtypes = [vp, vp, i64, i32p, i32p, fp]
+        L.mibwt_selector_load.argtypes = [ctypes.c_char_p]
+        L.mibwt_mlp_grad.argtypes = [vp, fp, fp, fp, i64, fp, fp]
+        L.mibwt_mlp_predict.argtypes = [vp, fp, fp, i64, fp]
+        L.mibwt_mlp_train.argtypes = [vp, fp, fp, i64, i32, i32, ctypes.c_float, ctypes.c_uint32,
+                                      ctypes.c_char_p, fp]
         L.mibwt_device_alloc.argtypes = [ctypes.POINTER(vp), i64]
         L.mibwt_device_free.argtypes = [vp]
         L.mibwt_copy.argtypes = [vp, vp, i64, i32]
@@ -113,6 +134,29 @@ def lib():
 
 def _u8(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def wp_grid():
+    W = np.zeros(40, dtype=np.int32)
+    P = np.zeros(40, dtype=np.int32)
+    lib().mibwt_wp_grid(W.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                        P.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return list(zip(W.tolist(), P.tolist()))
+
+
+def selector_load(path: str):
+    rc = lib().mibwt_selector_load(path.encode())
+    if rc != NO_ERROR:
+        raise BwtError(rc, "mibwt_selector_load")
+
+
+MLP_SHAPES = [("W1", (128, 64)), ("b1", (128,)), ("W2", (128, 128)), ("b2", (128,)),
+              ("W3", (3, 128)), ("b3", (3,))]
+MLP_PARAMS = sum(int(np.prod(s)) for _, s in MLP_SHAPES)
 
 
 def _as_text(T) -> np.ndarray:
@@ -185,6 +229,69 @@ class Storage:
         if rc < 0:
             raise BwtError(rc, "mibwt_lyndon_array")
         return lam
+
+    def batch_bwt(self, texts, method: str = "pfp", W: int = 5, P: int = 16, workers: int = 0):
+        """[(L, primary_index)] for independent texts; W=0 lets the selector choose per text."""
+        arrs = [_as_text(t) for t in texts]
+        outs = [np.empty_like(t) for t in arrs]
+        k = len(arrs)
+        Tp = (ctypes.c_void_p * k)(*[t.ctypes.data for t in arrs])
+        Lp = (ctypes.c_void_p * k)(*[o.ctypes.data for o in outs])
+        ns = (ctypes.c_int64 * k)(*[len(t) for t in arrs])
+        idx = (ctypes.c_int64 * k)()
+        m = METHOD_PFP if method == "pfp" else METHOD_LYNDON
+        rc = lib().mibwt_batch_bwt(self.handle, Tp, Lp, ns, k, idx, m, W, P, workers)
+        if rc < 0:
+            raise BwtError(rc, "mibwt_batch_bwt")
+        return [(outs[i], int(idx[i])) for i in range(k)]
+
+    # ---- learned (W,P) selector
+    def wp_features(self, dT_ptr: int, n: int) -> np.ndarray:
+        X = np.zeros((40, 64), dtype=np.float32)
+        rc = lib().mibwt_wp_features(self.handle, dT_ptr, n, _fp(X))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_wp_features")
+        return X
+
+    def select_wp(self, dT_ptr: int, n: int):
+        """((W, P), trained, predicted log-ms-per-byte of the 40 candidates)."""
+        W, P = ctypes.c_int32(), ctypes.c_int32()
+        pred = np.zeros(40, dtype=np.float32)
+        rc = lib().mibwt_select_wp_device(self.handle, dT_ptr, n, ctypes.byref(W), ctypes.byref(P), _fp(pred))
+        if rc < 0:
+            raise BwtError(rc, "mibwt_select_wp_device")
+        return (W.value, P.value), bool(rc), pred
+
+    def mlp_grad(self, params: np.ndarray, X: np.ndarray, Y: np.ndarray):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros_like(params)
+        loss = ctypes.c_float()
+        rc = lib().mibwt_mlp_grad(self.handle, _fp(params), _fp(X), _fp(Y), len(X), _fp(g), ctypes.byref(loss))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_mlp_grad")
+        return float(loss.value), g
+
+    def mlp_predict(self, params: np.ndarray, X: np.ndarray) -> np.ndarray:
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((len(X), 3), dtype=np.float32)
+        rc = lib().mibwt_mlp_predict(self.handle, _fp(params), _fp(X), len(X), _fp(Y))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_mlp_predict")
+        return Y
+
+    def mlp_train(self, X: np.ndarray, Y: np.ndarray, epochs: int = 200, batch: int = 512,
+                  lr: float = 1e-3, seed: int = 1, out_path: str | None = None) -> float:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        loss = ctypes.c_float()
+        rc = lib().mibwt_mlp_train(self.handle, _fp(X), _fp(Y), len(X), epochs, batch, lr, seed,
+                                   out_path.encode() if out_path else None, ctypes.byref(loss))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_mlp_train")
+        return float(loss.value)
 
     def stats(self) -> dict:
         st = PhaseStats()

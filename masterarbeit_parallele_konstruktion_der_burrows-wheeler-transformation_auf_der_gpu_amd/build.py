@@ -26,7 +26,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + CSRC,
           "-I" + os.path.join(HERE, "..", "include")]
 
-LIB_SOURCES = ["abi.hip", "pfp.hip", "lyndon.hip", "selector.cpp"]
+LIB_SOURCES = ["abi.hip", "pfp.hip", "lyndon.hip", "selector.hip"]
 CLI_SOURCES = {"pfp_bwt": "cli.cpp", "lyndon_bwt": "cli.cpp", "cubwt_compat": "cli.cpp"}
 
 
@@ -58,7 +58,7 @@ def build(jobs: int = 8, verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         objs = list(ex.map(lambda s: _compile(s, verbose), LIB_SOURCES))
     tmp = LIB + ".tmp"
-    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp], verbose)
+    _run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC", *objs, "-o", tmp, "-ldl"], verbose)
     os.replace(tmp, LIB)
     cli = os.path.join(CSRC, "cli.cpp")
     if os.path.exists(cli):

@@ -1,7 +1,11 @@
 // extern "C" boundary of libmibwt.so.  Every entry point validates its arguments before any
 // HIP call, maps HIP failures onto the libcubwt error codes (libcubwt.cuh:32-38,
 // libcubwt.cu:149-171) and never lets a C++ exception cross the ABI.
-#include "common.hpp"
+#include "text.hpp"
+
+#include <algorithm>
+#include <atomic>
+#include <thread>
 
 namespace mibwt {
 int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P);
@@ -9,7 +13,15 @@ double pfp_time_scan(Storage& S, int reps, double* bytes);
 double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
-void selector_choose(const uint8_t* T, int64_t n, int* W, int* P);
+void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred);
+bool selector_load(const char* path);
+bool selector_has_model();
+void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X);
+void wp_grid(int* W, int* P);
+float mlp_grad(Storage& S, const float* params, const float* X, const float* Y, uint32_t B, float* grads);
+void mlp_predict_raw(Storage& S, const float* params, const float* X, uint32_t B, float* Y);
+float mlp_train(Storage& S, const float* X, const float* Y, uint32_t B, int epochs, uint32_t mb, float lr,
+                uint32_t seed, const char* path);
 }  // namespace mibwt
 
 using namespace mibwt;
@@ -100,6 +112,8 @@ int64_t libcubwt_free_device_storage(void* device_storage)
 {
     Storage* S = reinterpret_cast<Storage*>(device_storage);
     if (!S) return LIBCUBWT_BAD_PARAMETER;
+    for (Storage* sub : S->pool) libcubwt_free_device_storage(sub);
+    S->pool.clear();
     int64_t rc = guarded([&]() -> int64_t {
         if (S->stream) MIBWT_HIP(hipStreamSynchronize(S->stream));
         for (auto& b : S->bufs)
@@ -123,9 +137,18 @@ int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t
 {
     Storage* S = reinterpret_cast<Storage*>(device_storage);
     if (!S || !T || !L || n < 16 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
-    int W = 5, P = 16;
-    selector_choose(T, n, &W, &P);
-    return host_bwt(device_storage, T, L, n, MIBWT_METHOD_PFP, W, P, 16);
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* dO = S->buf<uint8_t>(B_STAGE, (size_t)n);
+        const uint8_t wrap = T[n - 1];
+        // upload once; the (W,P) choice reads the staged text, the build reuses it
+        uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
+        int W = 5, P = 16;
+        selector_choose_device(*S, dT, (uint64_t)n, &W, &P, nullptr);
+        const int64_t idx = pfp_bwt_run(*S, nullptr, false, dO, n, W, P);
+        copy_out_cubwt(*S, dO, L, wrap, n, idx);
+        return idx;
+    });
 }
 
 int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int32_t W, int32_t P)
@@ -220,6 +243,147 @@ int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double*
             return LIBCUBWT_NO_ERROR;
         }
         return (int64_t)LIBCUBWT_BAD_PARAMETER;
+    });
+}
+
+int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* L, const int64_t* n, int64_t k,
+                        int64_t* index, int32_t method, int32_t W, int32_t P, int32_t workers)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !T || !L || !n || !index || k < 0 || workers < 0 || workers > 16) return LIBCUBWT_BAD_PARAMETER;
+    if (method != MIBWT_METHOD_PFP && method != MIBWT_METHOD_LYNDON) return LIBCUBWT_BAD_PARAMETER;
+    if (method == MIBWT_METHOD_PFP && W != 0 && (W < 1 || W > 64 || P < 1 || P > 65535)) return LIBCUBWT_BAD_PARAMETER;
+    for (int64_t i = 0; i < k; i++)
+        if (!T[i] || !L[i] || n[i] < 1 || n[i] > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    if (k == 0) return LIBCUBWT_NO_ERROR;
+    if (workers == 0) workers = 4;
+    // one storage (stream) per worker, created on first use and kept by the parent storage
+    int64_t rc = guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        while ((int)S->pool.size() < workers) {
+            void* sub = nullptr;
+            const int64_t r = libcubwt_allocate_device_storage(&sub, S->max_length);
+            if (r != LIBCUBWT_NO_ERROR) return r;
+            S->pool.push_back(reinterpret_cast<Storage*>(sub));
+        }
+        return LIBCUBWT_NO_ERROR;
+    });
+    if (rc != LIBCUBWT_NO_ERROR) return rc;
+    // longest texts first, handed out through one atomic counter
+    std::vector<int64_t> order((size_t)k);
+    for (int64_t i = 0; i < k; i++) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return n[a] > n[b]; });
+    std::atomic<int64_t> next{0};
+    auto work = [&](Storage* sub) {
+        (void)hipSetDevice(sub->device);
+        for (;;) {
+            const int64_t q = next.fetch_add(1);
+            if (q >= k) break;
+            const int64_t i = order[(size_t)q];
+            if (method == MIBWT_METHOD_LYNDON) {
+                index[i] = host_bwt(sub, T[i], L[i], n[i], MIBWT_METHOD_LYNDON, 5, 16, 1);
+            } else if (W != 0) {
+                index[i] = host_bwt(sub, T[i], L[i], n[i], MIBWT_METHOD_PFP, W, P, 1);
+            } else {
+                index[i] = guarded([&]() -> int64_t {
+                    uint8_t* dO = sub->buf<uint8_t>(B_STAGE, (size_t)n[i]);
+                    const uint8_t wrap = T[i][n[i] - 1];
+                    uint8_t* dT = stage_text(*sub, T[i], true, (uint64_t)n[i]);
+                    int w = 5, p = 16;
+                    selector_choose_device(*sub, dT, (uint64_t)n[i], &w, &p, nullptr);
+                    const int64_t idx = pfp_bwt_run(*sub, nullptr, false, dO, n[i], w, p);
+                    copy_out_cubwt(*sub, dO, L[i], wrap, n[i], idx);
+                    return idx;
+                });
+            }
+        }
+    };
+    std::vector<std::thread> th;
+    for (int w = 0; w < workers; w++) th.emplace_back(work, S->pool[(size_t)w]);
+    for (auto& t : th) t.join();
+    for (int64_t i = 0; i < k; i++)
+        if (index[i] < 0) return index[i];
+    return LIBCUBWT_NO_ERROR;
+}
+
+int64_t mibwt_select_wp_device(void* storage, const uint8_t* dT, int64_t n, int32_t* W, int32_t* P, float* pred)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !dT || !W || !P || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* T = stage_text(*S, dT, false, (uint64_t)n);
+        int w = 5, p = 16;
+        selector_choose_device(*S, T, (uint64_t)n, &w, &p, pred);
+        *W = w;
+        *P = p;
+        return selector_has_model() ? 1 : 0;
+    });
+}
+
+int64_t mibwt_wp_features(void* storage, const uint8_t* dT, int64_t n, float* X)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !dT || !X || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* T = stage_text(*S, dT, false, (uint64_t)n);
+        wp_features(*S, T, (uint64_t)n, X);
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_wp_grid(int32_t* W, int32_t* P)
+{
+    if (!W || !P) return LIBCUBWT_BAD_PARAMETER;
+    int w[64], p[64];
+    wp_grid(w, p);
+    for (int i = 0; i < 40; i++) {
+        W[i] = w[i];
+        P[i] = p[i];
+    }
+    return 40;
+}
+
+int64_t mibwt_selector_load(const char* path)
+{
+    if (!path) return LIBCUBWT_BAD_PARAMETER;
+    return selector_load(path) ? LIBCUBWT_NO_ERROR : LIBCUBWT_BAD_PARAMETER;
+}
+
+int64_t mibwt_mlp_grad(void* storage, const float* params, const float* X, const float* Y, int64_t B, float* grads,
+                       float* loss)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !params || !X || !Y || !grads || !loss || B < 1 || B > (1 << 24)) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        *loss = mlp_grad(*S, params, X, Y, (uint32_t)B, grads);
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_mlp_predict(void* storage, const float* params, const float* X, int64_t B, float* Y)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !params || !X || !Y || B < 1 || B > (1 << 24)) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        mlp_predict_raw(*S, params, X, (uint32_t)B, Y);
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_mlp_train(void* storage, const float* X, const float* Y, int64_t B, int32_t epochs, int32_t batch,
+                        float lr, uint32_t seed, const char* out_path, float* final_loss)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !X || !Y || B < 2 || B > (1 << 24) || epochs < 1 || batch < 0 || !(lr > 0)) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        const float l = mlp_train(*S, X, Y, (uint32_t)B, epochs, (uint32_t)batch, lr, seed, out_path);
+        if (final_loss) *final_loss = l;
+        return LIBCUBWT_NO_ERROR;
     });
 }
 

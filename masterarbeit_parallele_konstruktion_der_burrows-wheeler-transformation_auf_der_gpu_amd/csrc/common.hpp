@@ -91,6 +91,8 @@ struct Storage {
     std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;
     std::vector<double> kt_bytes;
     int kt_used = 0;
+    // worker storages of mibwt_batch_bwt (one stream each)
+    std::vector<Storage*> pool;
     // state kept for mibwt_time_kernel
     const uint8_t* last_text = nullptr;
     int64_t last_n = 0;
@@ -287,6 +289,25 @@ enum BufId : int {
     B_LY_VL,
     B_LY_JMP0,
     B_LY_JMP1,
+    B_SEL_HIST2,
+    B_SEL_COUNTS,
+    B_SEL_X,
+    B_SEL_T,
+    B_SEL_H1,
+    B_SEL_H2,
+    B_SEL_Y,
+    B_SEL_DY,
+    B_SEL_DH2,
+    B_SEL_DH1,
+    B_SEL_LOSS,
+    B_SEL_P,
+    B_SEL_PB,
+    B_SEL_G,
+    B_SEL_M1,
+    B_SEL_M2,
+    B_SEL_H1B,
+    B_SEL_H2B,
+    B_SEL_YB,
     B_COUNT_
 };
 

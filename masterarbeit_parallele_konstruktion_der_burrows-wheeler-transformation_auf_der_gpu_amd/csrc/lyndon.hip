@@ -492,12 +492,14 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         MIBWT_HIP(hipEventCreate(&S.t_end));
     }
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    const uint64_t text_bytes = n + kPad + 4096;
-    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
-    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
-    if (src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
-    MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
-    if (!src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyDeviceToDevice, S.stream));
+    uint8_t* T;
+    if (src_host) {
+        T = stage_text(S, src, true, n);
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    } else {
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        T = stage_text(S, src, false, n);
+    }
 
     // ---- 1. Lyndon array
     int ph = S.phase_begin("lyndon");
@@ -549,6 +551,7 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         uint32_t* H = S.buf<uint32_t>(B_LY_HF2, n);
         MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
         uint32_t known = (uint32_t)per;
+        int cper = per;
         for (int round = 0;; round++) {
             MIBWT_REQUIRE(round < 1 << 20, LIBCUBWT_GPU_ERROR, "label refinement does not converge");
             inclusive_sum<uint32_t>(S, H, gid1, n, sc + SC_GROUPS);
@@ -569,19 +572,22 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
                 uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
                 uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
                 uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
-                const int cb = bits * per;
-                MIBWT_REQUIRE(cb + bits_for(hv[SC_GROUPS]) <= 64, LIBCUBWT_GPU_ERROR, "chunk key too wide");
+                // (group, next symbols) must fit one 64-bit key
+                const int gb = bits_for(hv[SC_GROUPS]);
+                cper = (64 - gb) / bits;
+                MIBWT_REQUIRE(cper >= 1, LIBCUBWT_GPU_ERROR, "chunk key too wide");
+                const int cb = bits * cper;
                 k_lab_chunk_gather<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, V, gid1, large, lpos, lam, codes, bits,
-                                                                         per, known, cb, LK0, LV0, LI);
+                                                                         cper, known, cb, LK0, LV0, LI);
                 MIBWT_LAUNCH_CHECK();
-                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, cb + bits_for(hv[SC_GROUPS]));
+                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, cb + gb);
                 k_lab_chunk_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, V2, H);
                 MIBWT_LAUNCH_CHECK();
                 rbytes += (double)nl * 24 * 8;
             }
             V = V2;
             if (nl == 0) break;
-            known += (uint32_t)per;
+            known += (uint32_t)cper;
         }
         // exact verification of the label order
         fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
@@ -648,10 +654,7 @@ int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n64, uint32_t* 
 {
     MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
     const uint64_t n = (uint64_t)n64;
-    const uint64_t text_bytes = n + kPad + 4096;
-    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
-    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
-    MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
+    uint8_t* T = stage_text(S, src, true, n);
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
     const int levels = lyndon_array(S, T, n, lam);
     MIBWT_HIP(hipMemcpyAsync(lam_host, lam, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, S.stream));

@@ -592,17 +592,19 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     }
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     const uint32_t tiles = ceil_div(n + 1, kTrigTile);
-    const uint64_t text_bytes = (uint64_t)tiles * kTrigTile + kPad;
-    uint8_t* T = S.buf<uint8_t>(B_TEXT, text_bytes);
-
-    // the text: a padded private copy (zero tail so that 16-byte tile loads and 8-byte word
-    // loads past the end are in bounds)
-    MIBWT_HIP(hipMemsetAsync(T + n, 0, text_bytes - n, S.stream));
-    if (src_host) {
-        MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyHostToDevice, S.stream));
+    // the text: a padded private copy (src == nullptr: already staged by the caller); a copy
+    // from device memory is part of the timed build, an upload from the host is not
+    uint8_t* T;
+    if (src == nullptr) {
+        T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    } else if (src_host) {
+        T = stage_text(S, src, true, n);
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    } else {
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        T = stage_text(S, src, false, n);
     }
-    MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
-    if (!src_host) MIBWT_HIP(hipMemcpyAsync(T, src, n, hipMemcpyDeviceToDevice, S.stream));
 
     // ---- 1. trigger scan
     int ph = S.phase_begin("scan");
@@ -833,7 +835,7 @@ double pfp_time_scan(Storage& S, int reps, double* bytes)
 {
     const uint64_t n = (uint64_t)S.last_n;
     const uint32_t tiles = ceil_div(n + 1, kTrigTile);
-    uint8_t* T = S.buf<uint8_t>(B_TEXT, (uint64_t)tiles * kTrigTile + kPad);
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
     uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
     uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
     hipEvent_t a, b;

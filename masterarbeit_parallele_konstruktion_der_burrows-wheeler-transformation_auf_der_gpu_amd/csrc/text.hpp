@@ -30,25 +30,38 @@ __device__ __forceinline__ uint64_t mix64(uint64_t h)
     return h;
 }
 
-// symbol presence bitmap of the text (256 bits)
+// symbol presence bitmap of the text (256 bits): each thread ORs its 64 bytes into a
+// register bitmap, the wave reduces it with lane shuffles, one atomic per wave and word
 static __global__ __launch_bounds__(kBlock) void k_alpha_used(const uint8_t* __restrict__ T, uint64_t n,
                                                              uint32_t* __restrict__ used)
 {
-    __shared__ uint32_t bm[8];
-    if (threadIdx.x < 8) bm[threadIdx.x] = 0;
-    __syncthreads();
-    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * 16;
-    if (base < n) {
-        uint4 v = *reinterpret_cast<const uint4*>(T + base);
+    uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * 64;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const uint64_t p = base + q * 16;
+        if (p >= n) break;
+        const uint4 v = *reinterpret_cast<const uint4*>(T + p);
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-        const int cnt = (n - base) < 16 ? (int)(n - base) : 16;
+        const int cnt = (n - p) < 16 ? (int)(n - p) : 16;
         for (int i = 0; i < cnt; i++) {
             const uint32_t c = (w[i >> 2] >> (8 * (i & 3))) & 0xFF;
-            atomicOr(&bm[c >> 5], 1u << (c & 31));
+            bm[c >> 5] |= 1u << (c & 31);
         }
     }
-    __syncthreads();
-    if (threadIdx.x < 8 && bm[threadIdx.x]) atomicOr(&used[threadIdx.x], bm[threadIdx.x]);
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint32_t x = bm[k];
+        for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+        bm[k] = x;
+    }
+    if ((threadIdx.x & 63) < 8) {
+        const int k = threadIdx.x & 63;
+        uint32_t x = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) x = (j == k) ? bm[j] : x;
+        if (x) atomicOr(&used[k], x);
+    }
 }
 
 // dense symbol codes 2.. in byte order; codes[256] = number of distinct bytes
@@ -67,10 +80,23 @@ static __global__ void k_alpha_codes(const uint32_t* __restrict__ used, uint32_t
 inline void alphabet_codes(Storage& S, const uint8_t* T, uint64_t n, uint32_t* codes)
 {
     fill<uint32_t>(S, codes + 260, 8, 0u);
-    k_alpha_used<<<ceil_div(ceil_div(n, 16), kBlock), kBlock, 0, S.stream>>>(T, n, codes + 260);
+    k_alpha_used<<<ceil_div(ceil_div(n, 64), kBlock), kBlock, 0, S.stream>>>(T, n, codes + 260);
     MIBWT_LAUNCH_CHECK();
     k_alpha_codes<<<1, 256, 0, S.stream>>>(codes + 260, codes);
     MIBWT_LAUNCH_CHECK();
+}
+
+// Private padded copy of the text in B_TEXT: T[0..n) followed by zero bytes up to a 4096-byte
+// multiple plus kPad (tile loads and unaligned word loads stay inside the allocation).
+inline uint64_t staged_text_bytes(uint64_t n) { return ((n + 1 + 4095) / 4096) * 4096 + kPad + 4096; }
+
+inline uint8_t* stage_text(Storage& S, const uint8_t* src, bool src_host, uint64_t n)
+{
+    const uint64_t bytes = staged_text_bytes(n);
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, bytes);
+    MIBWT_HIP(hipMemsetAsync(T + n, 0, bytes - n, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(T, src, n, src_host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, S.stream));
+    return T;
 }
 
 }  // namespace mibwt

@@ -1,0 +1,172 @@
+"""Collect training runs for the learned (W,P) selector and train it on the GPU.
+
+For every synthetic text the prefix-free-parsing pipeline is run once per grid candidate
+(W in {5,10,20,35} x P in {11,...,81}); the selector features and the measured dictionary
+length, parse length and build time become one training sample each.  The MLP is trained by
+the library's own HIP kernels (mibwt_mlp_train) and written to selector_model.bin next to the
+library (or --out).  Held-out texts report the chosen candidate's time against the best grid
+point and against fixed W=5,P=16.
+
+  python train_selector.py [--texts 300] [--holdout 24] [--out PATH] [--data runs.npz]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _pkg():
+    if "mibwt" in sys.modules:
+        return sys.modules["mibwt"]
+    spec = importlib.util.spec_from_file_location("mibwt", os.path.join(HERE, "__init__.py"),
+                                                  submodule_search_locations=[HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules["mibwt"] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def random_text(rng: np.random.Generator, max_len: int) -> tuple[np.ndarray, dict]:
+    """A repetitive collection: random base (DNA, protein-like or bytes) x copies, mutated."""
+    from mibwt import gen
+    kind = rng.choice(["dna", "dna", "dna", "protein", "bytes"])
+    copies = int(rng.integers(2, 65))
+    total = int(np.exp(rng.uniform(np.log(4 << 20), np.log(max_len))))
+    base_len = max(4096, total // copies)
+    sub = float(np.exp(rng.uniform(np.log(1e-5), np.log(5e-2))))
+    indel = sub / float(rng.uniform(5, 20))
+    indel_max = int(rng.integers(1, 51))
+    seed = int(rng.integers(0, 2**31))
+    idx = gen.repetitive_dna(base_len, copies, sub, indel, indel_max, seed)  # ACGT bytes
+    if kind == "protein":
+        table = np.frombuffer(b"ACDEFGHIKLMNPQRSTVWY", dtype=np.uint8)
+        r2 = np.random.default_rng(seed ^ 0x5A5A)
+        # map 4-symbol pairs to 20 letters by re-drawing the base with a larger alphabet
+        base = r2.integers(0, 20, size=base_len).astype(np.uint8)
+        parts = [gen.mutate(r2, base, sub, indel, indel_max) for _ in range(copies)]
+        idx = table[np.concatenate(parts) % 20]
+    elif kind == "bytes":
+        r2 = np.random.default_rng(seed ^ 0xA5A5)
+        base = r2.integers(0, 256, size=base_len).astype(np.uint8)
+        parts = []
+        for _ in range(copies):
+            x = base.copy()
+            hit = np.flatnonzero(r2.random(len(x)) < sub)
+            x[hit] = r2.integers(0, 256, size=len(hit))
+            parts.append(x)
+        idx = np.concatenate(parts)
+    return idx, {"kind": str(kind), "copies": copies, "base_len": base_len, "sub": sub}
+
+
+def run_grid(mibwt, st, T: np.ndarray, grid):
+    """features, targets and ms of every grid candidate, and the ms of fixed W=5,P=16"""
+    n = len(T)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(n)
+    X = st.wp_features(dT.address, n)
+    Y = np.zeros((len(grid), 3), dtype=np.float32)
+    ms = np.zeros(len(grid))
+    st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)  # warm the buffers
+    st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)
+    fixed_ms = st.stats()["total_ms"]
+    for c, (W, P) in enumerate(grid):
+        st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
+        s = st.stats()
+        ms[c] = s["total_ms"]
+        Y[c] = [np.log(s["dict_length"] / n), np.log(s["phrases"] / n), np.log(s["total_ms"] / n)]
+    dT.free()
+    dO.free()
+    return X, Y, ms, fixed_ms
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--texts", type=int, default=300)
+    ap.add_argument("--holdout", type=int, default=24)
+    ap.add_argument("--max-mib", type=int, default=64)
+    ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--out", default=os.path.join(HERE, "selector_model.bin"))
+    ap.add_argument("--data", default=None, help="write / reuse the collected runs (npz)")
+    ap.add_argument("--report", default=None, help="write the held-out evaluation (json)")
+    ap.add_argument("--seed", type=int, default=20261203)
+    a = ap.parse_args()
+    mibwt = _pkg()
+    grid = mibwt.wp_grid()
+    rng = np.random.default_rng(a.seed)
+    st = mibwt.Storage(a.max_mib << 20)
+    t0 = time.time()
+    Xs, Ys, MS, FX, meta = [], [], [], [], []
+    total = a.texts + a.holdout
+    for i in range(total):
+        T, info = random_text(rng, a.max_mib << 20)
+        X, Y, ms, fms = run_grid(mibwt, st, T, grid)
+        Xs.append(X)
+        Ys.append(Y)
+        MS.append(ms)
+        FX.append(fms)
+        info["n"] = len(T)
+        meta.append(info)
+        print(f"[train_selector] text {i + 1}/{total} n={len(T)} {info['kind']} best "
+              f"{grid[int(ms.argmin())]} {ms.min():.2f} ms, W5P16 {fms:.2f} ms "
+              f"({time.time() - t0:.0f} s)", flush=True)
+    X = np.stack(Xs)
+    Y = np.stack(Ys)
+    MS = np.stack(MS)
+    FX = np.array(FX)
+    if a.data:
+        np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX)
+    ntr = a.texts
+    loss = st.mlp_train(X[:ntr].reshape(-1, 64), Y[:ntr].reshape(-1, 3), epochs=a.epochs, batch=512,
+                        lr=1e-3, seed=a.seed & 0xFFFF, out_path=a.out)
+    print(f"[train_selector] trained on {ntr * len(grid)} runs, final loss {loss:.4f}, model {a.out}", flush=True)
+    mibwt.selector_load(a.out)
+    # held-out evaluation: predicted choice against the measured grid
+    report = evaluate(mibwt, st, X[ntr:], MS[ntr:], FX[ntr:], grid, a.out)
+    report["train_runs"] = int(ntr * len(grid))
+    report["final_loss"] = loss
+    print(json.dumps(report), flush=True)
+    if a.report:
+        with open(a.report, "w") as f:
+            json.dump(report, f, indent=1)
+    st.close()
+
+
+def evaluate(mibwt, st, X, MS, FX, grid, model_path):
+    """Choice of the trained model on held-out texts (features X[t][40][64], measured ms)."""
+    with open(model_path, "rb") as f:
+        raw = np.frombuffer(f.read(), dtype=np.float32)
+    hdr = 4
+    xmean = raw[hdr:hdr + 64]
+    xstd = raw[hdr + 64:hdr + 128]
+    ymean = raw[hdr + 128:hdr + 131]
+    ystd = raw[hdr + 131:hdr + 134]
+    params = raw[hdr + 134:]
+    ratios_best, ratios_fixed, ratios_oracle = [], [], []
+    for t in range(len(X)):
+        Xn = (X[t] - xmean) / xstd
+        y = st.mlp_predict(params, Xn)
+        pred_ms = y[:, 2] * ystd[2] + ymean[2]
+        c = int(np.argmin(pred_ms))
+        ratios_best.append(MS[t][c] / MS[t].min())
+        ratios_fixed.append(FX[t] / MS[t][c])
+        ratios_oracle.append(FX[t] / MS[t].min())
+    out = {"holdout_texts": len(X),
+           "chosen_over_best_median": float(np.median(ratios_best)),
+           "chosen_over_best_mean": float(np.mean(ratios_best)),
+           "within_10pct_of_best": float(np.mean(np.array(ratios_best) <= 1.10))}
+    out["speedup_over_w5p16_median"] = float(np.median(ratios_fixed))
+    out["speedup_over_w5p16_geomean"] = float(np.exp(np.mean(np.log(ratios_fixed))))
+    out["best_grid_over_w5p16_geomean"] = float(np.exp(np.mean(np.log(ratios_oracle))))
+    return out
+
+
+if __name__ == "__main__":
+    main()

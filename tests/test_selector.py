@@ -1,0 +1,98 @@
+"""The (W,P) selector's MLP: the fp64 reference is pinned by finite differences; the device
+forward/backward kernels are checked against it; training lowers the loss; the selector's
+choice keeps the BWT bit-exact."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import mlp
+
+
+def bf16_params(rng, scale=0.2):
+    p = rng.normal(0, scale, mlp.COUNT).astype(np.float32)
+    return mlp.bf16_round(p)
+
+
+def test_reference_gradient_finite_differences():
+    rng = np.random.default_rng(0)
+    p = bf16_params(rng).astype(np.float64)
+    X = rng.normal(0, 1, (7, 64))
+    Y = rng.normal(0, 1, (7, 3))
+    loss, g = mlp.loss_grad(p, X, Y)
+    # biases are used unrounded: central differences on them are exact up to O(h^2)
+    off = 128 * 64
+    for idx in [off + 3, off + 77, off + 128 + 128 * 128 + 5, mlp.COUNT - 1, mlp.COUNT - 3]:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (mlp.loss_grad(pp, X, Y)[0] - mlp.loss_grad(pm, X, Y)[0]) / (2 * h)
+        assert abs(fd - g[idx]) < 1e-6 * max(1, abs(fd))
+    # weight gradients: compare against differences of the unrounded network (linear in W)
+    def loss_unrounded(q):
+        d = mlp.unpack(q)
+        a1 = X @ d["W1"].T + d["b1"]
+        h1 = np.maximum(a1, 0)
+        a2 = h1 @ d["W2"].T + d["b2"]
+        h2 = np.maximum(a2, 0)
+        y = h2 @ d["W3"].T + d["b3"]
+        return ((y - Y) ** 2).mean()
+    for idx in [5, 4000, off + 128 + 999, off + 128 + 128 * 128 + 128 + 200]:
+        h = 1e-6
+        pp, pm = p.copy(), p.copy()
+        pp[idx] += h
+        pm[idx] -= h
+        fd = (loss_unrounded(pp) - loss_unrounded(pm)) / (2 * h)
+        assert abs(fd - g[idx]) < 1e-5 * max(1, abs(fd))
+
+
+def test_bf16_rounding_is_nearest_even():
+    x = np.array([1.0, 1.00390625, 1.0078125, -2.5, 3.14159], dtype=np.float32)
+    r = mlp.bf16_round(x)
+    assert r[0] == 1.0 and r[2] == 1.0078125
+    assert r[1] == 1.0  # halfway case rounds to the even mantissa
+    assert abs(r[4] - 3.140625) < 1e-7
+
+
+@pytest.mark.gpu
+def test_device_gradient_matches_reference(storage):
+    rng = np.random.default_rng(1)
+    p = bf16_params(rng)
+    X = rng.normal(0, 1, (300, 64)).astype(np.float32)
+    Y = rng.normal(0, 1, (300, 3)).astype(np.float32)
+    loss_ref, g_ref = mlp.loss_grad(p, X, Y)
+    loss, g = storage.mlp_grad(p, X, Y)
+    assert abs(loss - loss_ref) < 1e-4 * max(1, loss_ref)
+    scale = np.abs(g_ref).max()
+    assert np.abs(g - g_ref).max() < 1e-4 * scale
+    y_ref, _ = mlp.forward(p, X)
+    y = storage.mlp_predict(p, X)
+    assert np.abs(y - y_ref).max() < 1e-3 * max(1, np.abs(y_ref).max())
+
+
+@pytest.mark.gpu
+def test_training_fits_a_smooth_target(storage, tmp_path):
+    rng = np.random.default_rng(2)
+    X = rng.normal(0, 1, (4000, 64)).astype(np.float32)
+    Y = np.stack([np.sin(X[:, 0]) + 0.5 * X[:, 1], X[:, 2] * X[:, 3], np.abs(X[:, 4])], 1).astype(np.float32)
+    path = str(tmp_path / "m.bin")
+    loss = storage.mlp_train(X, Y, epochs=60, batch=256, lr=2e-3, seed=3, out_path=path)
+    assert loss < 0.2  # normalised targets start near 1.0
+    import os
+    assert os.path.getsize(path) > 4 * mlp.COUNT
+
+
+@pytest.mark.gpu
+def test_selector_choice_keeps_bwt_exact(mibwt):
+    from mibwt import gen
+    T = gen.repetitive_dna(1 << 18, 8, 0.001, 0.0001, 10, seed=9)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    with mibwt.Storage(len(T)) as st:
+        (W, P), trained, pred = st.select_wp(dT.address, len(T))
+        assert (W, P) in mibwt.wp_grid() or (W, P) == (5, 16)
+        X = st.wp_features(dT.address, len(T))
+        assert np.isfinite(X).all() and X.shape == (40, 64)
+        L, idx = st.bwt(T, "cubwt")
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+    dT.free()
