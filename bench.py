@@ -6,7 +6,12 @@ collection (4 MiB uniform-ACGT base x 64 copies, 0.05% substitutions + 0.005% in
 length 1..50), W=5, P=16.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna256m] [--W 5 --P 16]
-                  [--no-verify] [--no-cpu-baseline]
+                  [--wp auto|fixed] [--lyndon-steps 3] [--no-verify] [--no-cpu-baseline]
+
+With --wp auto (default) every timed step runs the learned (W,P) selector on the resident text
+and then the build with its choice; `value` is that rate.  The fixed W/P build is timed
+beside it ("fixed_wp", "speedup_vs_fixed"), and the Lyndon-grammar BWT of the same text is
+timed and compared byte for byte ("lyndon").
 """
 from __future__ import annotations
 
@@ -62,6 +67,9 @@ def main():
     ap.add_argument("--no-verify", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=32)
+    ap.add_argument("--wp", choices=["auto", "fixed"], default="auto",
+                    help="auto: learned (W,P) choice timed per step (value), fixed W/P reported beside it")
+    ap.add_argument("--lyndon-steps", type=int, default=3, help="timed Lyndon-grammar builds (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a plain 256 MiB device copy kernel (byte-counter reference)")
     a = ap.parse_args()
@@ -84,22 +92,68 @@ def main():
     mibwt.synchronize()
     st = mibwt.Storage(n)
 
-    def step():
+    def run(step, steps, warmup, label):
+        idx = None
+        for i in range(warmup):
+            idx = step()
+            log(f"{label} warmup {i} done ({st.stats()['total_ms']:.2f} ms build)")
+        mibwt.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            idx = step()
+        mibwt.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        log(f"{label}: {ms:.3f} ms/step, {n / 1e3 / ms:.1f} MB/s")
+        return ms, idx
+
+    # fixed (W, P)
+    def step_fixed():
         return st.bwt_device(dT.address, dO.address, n, "pfp", a.W, a.P)
 
-    for i in range(a.warmup):
-        idx = step()
-        log(f"warmup {i} done ({st.stats()['total_ms']:.2f} ms)")
-    mibwt.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        idx = step()
-    mibwt.synchronize()
-    t1 = time.perf_counter()
-    ms_per_step = (t1 - t0) * 1000.0 / a.steps
+    ms_fixed, idx = run(step_fixed, a.steps, a.warmup, f"fixed W={a.W},P={a.P}")
+    stats_fixed = st.stats()
+    O_fixed = dO.download() if not a.no_verify else None
+
+    # learned (W, P): W = P = 0 asks the library to run the selector's feature pass and MLP
+    # inference on the resident text before the build, inside the timed step
+    chosen = {}
+
+    def step_tuned():
+        idx_t = st.bwt_device(dT.address, dO.address, n, "pfp", 0, 0)
+        s_t = st.stats()
+        chosen.update(W=int(s_t["W"]), P=int(s_t["P"]))
+        return idx_t
+
+    if a.wp == "auto":
+        ms_tuned, idx_t = run(step_tuned, a.steps, a.warmup, "learned (W,P)")
+        assert idx_t == idx, "primary index differs between (W,P) choices"
+        stats = st.stats()
+        W_run, P_run = chosen["W"], chosen["P"]
+        ms_per_step = ms_tuned
+    else:
+        stats = stats_fixed
+        W_run, P_run = a.W, a.P
+        ms_per_step = ms_fixed
     value = n / 1e6 / (ms_per_step / 1000.0)
-    stats = st.stats()
-    log(f"timed {a.steps} steps: {ms_per_step:.3f} ms/step, {value:.1f} MB/s")
+
+    # Lyndon-grammar BWT of the same text
+    lyndon = None
+    if a.lyndon_steps > 0:
+        def step_lyndon():
+            return st.bwt_device(dT.address, dO.address, n, "lyndon")
+        ms_l, idx_l = run(step_lyndon, a.lyndon_steps, 1, "lyndon")
+        lyndon = {"value": round(n / 1e6 / (ms_l / 1000.0), 2), "ms_per_step": round(ms_l, 4),
+                  "steps": a.lyndon_steps,
+                  "phases": {p["name"]: round(p["ms"], 3) for p in st.stats()["phases"]}}
+        if O_fixed is not None:
+            same = idx_l == idx and bool((dO.download() == O_fixed).all())
+            lyndon["identical_to_pfp"] = same
+            if not same:
+                log("Lyndon BWT differs from the PFP BWT")
+                sys.exit(1)
+
+    def step():
+        return st.bwt_device(dT.address, dO.address, n, "pfp", W_run, P_run)
 
     # dominant kernel: the LSD radix sort's ranking/scatter pass, every launch bracketed
     # with HIP events on the storage stream during one extra (untimed) build
@@ -131,7 +185,11 @@ def main():
     if not a.no_verify:
         sys.path.insert(0, ROOT)
         import oracle  # checker only
+        step()
         O = dO.download()
+        if not (O == O_fixed).all():
+            log("BWT differs between the fixed and the learned (W,P)")
+            sys.exit(1)
         L = mibwt.reference_to_cubwt(O, idx)
         t = time.perf_counter()
         back = oracle.unbwt(L, idx)
@@ -143,7 +201,7 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(T, a.W, a.P, min(n, a.cpu_sample_mib << 20))
+        cpu = cpu_baseline(T, W_run, P_run, min(n, a.cpu_sample_mib << 20))
         cpu["cores"] = 1
         log(f"cpu baseline: {cpu['value']} MB/s")
 
@@ -163,8 +221,13 @@ def main():
         "vs_baseline": None,
         "dtype": "u8",
         "data": "synthetic",
-        "config": {"workload": a.workload, "n_bytes": n, "method": "pfp", "W": a.W, "P": a.P,
-                   "wp_choice": "fixed", "parallelism": "single-gpu"},
+        "config": {"workload": a.workload, "n_bytes": n, "method": "pfp", "W": W_run, "P": P_run,
+                   "wp_choice": "learned" if a.wp == "auto" else "fixed", "parallelism": "single-gpu"},
+        "fixed_wp": {"W": a.W, "P": a.P, "value": round(n / 1e6 / (ms_fixed / 1000.0), 2),
+                     "ms_per_step": round(ms_fixed, 4),
+                     "phases": {p["name"]: round(p["ms"], 3) for p in stats_fixed["phases"]}},
+        "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
+        "lyndon": lyndon,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "verified": verified,

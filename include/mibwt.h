@@ -69,7 +69,8 @@ int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, in
 
 /* Same on device buffers already resident in HBM: dT[0..n) -> dO[0..n) in REFERENCE
  * format (O), on the storage's stream (stream == NULL) or on `stream`.  Synchronous.
- * Returns the primary index. */
+ * W = P = 0 lets the learned selector choose (W,P) on the resident text first (the
+ * choice is reported by mibwt_last_phase_stats).  Returns the primary index. */
 int64_t mibwt_pfp_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
                              int32_t W, int32_t P, void* stream);
 
@@ -80,6 +81,14 @@ int64_t mibwt_lyndon_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n)
 /* Device-buffer variant, reference format, like mibwt_pfp_bwt_device. */
 int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
                                 void* stream);
+
+/* Trigger end positions of host T[0..n) for window W and modulus P: the ends e (W <= e < n,
+ * ascending) of the windows T[e-W+1..e] whose value sum_k T[e-W+1+k]*2^(W-1-k) mod P is 0 or
+ * that equal T[0..W-1] (the reference's Compute_D1/Compute_D2 output,
+ * Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:443-571).  E receives the first min(t, cap)
+ * positions; returns t or an error code. */
+int64_t mibwt_pfp_triggers(void* storage, const uint8_t* T, int64_t n, int32_t W, int32_t P,
+                           uint32_t* E, int64_t cap);
 
 /* Lyndon array of host T[0..n): lam[i] = length of the longest Lyndon word starting at i,
  * computed by the block-local passes and the merge tree of the Lyndon pipeline (the forest
@@ -132,17 +141,17 @@ int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double*
  * The reference chooses (W,P) per input by hand (Benchmark_Skript_Linux.sh:14-73) and compiles
  * one binary per pair (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:19-20).  An MLP
  * 64->128->128->3 (bf16 weights, fp32 accumulation, hand-written HIP kernels) predicts
- * log(dictionary length/n), log(parse length/n) and log(build ms per byte) for each of the 40
- * grid candidates W in {5,10,20,35} x P in {11,13,17,23,31,41,53,61,71,81}; the fastest is
- * chosen.  The model file is `selector_model.bin` next to lib/ (or $MIBWT_SELECTOR). */
+ * log(dictionary length/n), log(parse length/n) and log(build ms per byte) for each of 41
+ * candidates: the grid W in {5,10,20,35} x P in {11,13,17,23,31,41,53,61,71,81} and the
+ * reference default W=5,P=16 (last); the fastest predicted candidate is chosen.  The model file is `selector_model.bin` next to lib/ (or $MIBWT_SELECTOR). */
 
-/* The 40 grid candidates (W[i], P[i]); returns 40. */
+/* The 41 candidates (W[i], P[i]) (arrays of at least 64); returns 41. */
 int64_t mibwt_wp_grid(int32_t* W, int32_t* P);
 
-/* Features of the 40 candidates for the device text dT[0..n): X[40][64]. */
+/* Features of the 41 candidates for the device text dT[0..n): X[41][64]. */
 int64_t mibwt_wp_features(void* storage, const uint8_t* dT, int64_t n, float* X);
 
-/* (W,P) for the device text dT[0..n); pred (may be NULL) receives the 40 predicted
+/* (W,P) for the device text dT[0..n); pred (may be NULL) receives the 41 predicted
  * log(ms per byte).  Returns 1 when a trained model chose, 0 for the fixed W=5,P=16 fallback. */
 int64_t mibwt_select_wp_device(void* storage, const uint8_t* dT, int64_t n, int32_t* W, int32_t* P,
                                float* pred);

@@ -42,6 +42,7 @@ EXPORTED = [
     "mibwt_lyndon_bwt",
     "mibwt_lyndon_bwt_device",
     "mibwt_lyndon_array",
+    "mibwt_pfp_triggers",
     "mibwt_last_phase_stats",
     "mibwt_set_kernel_timing",
     "mibwt_time_kernel",
@@ -105,6 +106,7 @@ def lib():
         L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
         L.mibwt_lyndon_bwt_device.argtypes = [vp, vp, vp, i64, vp]
         L.mibwt_lyndon_array.argtypes = [vp, u8p, i64, ctypes.POINTER(ctypes.c_uint32)]
+        L.mibwt_pfp_triggers.argtypes = [vp, u8p, i64, i32, i32, ctypes.POINTER(ctypes.c_uint32), i64]
         L.mibwt_last_phase_stats.argtypes = [vp, ctypes.POINTER(PhaseStats)]
         L.mibwt_set_kernel_timing.argtypes = [vp, ctypes.c_char_p]
         L.mibwt_time_kernel.argtypes = [vp, ctypes.c_char_p, i32, ctypes.POINTER(ctypes.c_double),
@@ -140,12 +142,16 @@ def _fp(a: np.ndarray):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
 
 
+NUM_CAND = 41  # the W x P grid (40) and the reference default W=5,P=16 (last)
+
+
 def wp_grid():
-    W = np.zeros(40, dtype=np.int32)
-    P = np.zeros(40, dtype=np.int32)
-    lib().mibwt_wp_grid(W.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
-                        P.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
-    return list(zip(W.tolist(), P.tolist()))
+    """The selector's candidates [(W, P)]: the 4 x 10 grid, then (5, 16)."""
+    W = np.zeros(64, dtype=np.int32)
+    P = np.zeros(64, dtype=np.int32)
+    count = lib().mibwt_wp_grid(W.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                P.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    return list(zip(W[:count].tolist(), P[:count].tolist()))
 
 
 def selector_load(path: str):
@@ -221,6 +227,17 @@ class Storage:
             raise BwtError(rc, method + "_device")
         return int(rc)
 
+    def pfp_triggers(self, T, W: int = 5, P: int = 16) -> np.ndarray:
+        """Trigger end positions (ascending) of the prefix-free parse of T."""
+        T = _as_text(T)
+        cap = max(1, len(T))
+        E = np.empty(cap, dtype=np.uint32)
+        rc = lib().mibwt_pfp_triggers(self.handle, _u8(T), len(T), W, P,
+                                      E.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), cap)
+        if rc < 0:
+            raise BwtError(rc, "mibwt_pfp_triggers")
+        return E[:rc].copy()
+
     def lyndon_array(self, T) -> np.ndarray:
         T = _as_text(T)
         lam = np.empty(len(T), dtype=np.uint32)
@@ -247,16 +264,16 @@ class Storage:
 
     # ---- learned (W,P) selector
     def wp_features(self, dT_ptr: int, n: int) -> np.ndarray:
-        X = np.zeros((40, 64), dtype=np.float32)
+        X = np.zeros((NUM_CAND, 64), dtype=np.float32)
         rc = lib().mibwt_wp_features(self.handle, dT_ptr, n, _fp(X))
         if rc != NO_ERROR:
             raise BwtError(rc, "mibwt_wp_features")
         return X
 
     def select_wp(self, dT_ptr: int, n: int):
-        """((W, P), trained, predicted log-ms-per-byte of the 40 candidates)."""
+        """((W, P), trained, predicted log-ms-per-byte of the NUM_CAND candidates)."""
         W, P = ctypes.c_int32(), ctypes.c_int32()
-        pred = np.zeros(40, dtype=np.float32)
+        pred = np.zeros(NUM_CAND, dtype=np.float32)
         rc = lib().mibwt_select_wp_device(self.handle, dT_ptr, n, ctypes.byref(W), ctypes.byref(P), _fp(pred))
         if rc < 0:
             raise BwtError(rc, "mibwt_select_wp_device")

@@ -10,6 +10,7 @@
 namespace mibwt {
 int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P);
 double pfp_time_scan(Storage& S, int reps, double* bytes);
+int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n, int W, int P, uint32_t* E, int64_t cap);
 double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
@@ -17,7 +18,7 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
 bool selector_load(const char* path);
 bool selector_has_model();
 void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X);
-void wp_grid(int* W, int* P);
+int wp_grid(int* W, int* P);
 float mlp_grad(Storage& S, const float* params, const float* X, const float* Y, uint32_t B, float* grads);
 void mlp_predict_raw(Storage& S, const float* params, const float* X, uint32_t B, float* Y);
 float mlp_train(Storage& S, const float* X, const float* Y, uint32_t B, int epochs, uint32_t mb, float lr,
@@ -166,7 +167,9 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
 {
     Storage* S = reinterpret_cast<Storage*>(storage);
     if (!S || !dT || !dO || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
-    if (method == MIBWT_METHOD_PFP && (W < 1 || W > 64 || P < 1 || P > 65535)) return LIBCUBWT_BAD_PARAMETER;
+    const bool learned = (method == MIBWT_METHOD_PFP && W == 0 && P == 0);
+    if (method == MIBWT_METHOD_PFP && !learned && (W < 1 || W > 64 || P < 1 || P > 65535))
+        return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
@@ -177,8 +180,18 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
             MIBWT_HIP(hipEventRecord(ev, caller));
             MIBWT_HIP(hipStreamWaitEvent(S->stream, ev, 0));
         }
-        int64_t idx = (method == MIBWT_METHOD_PFP) ? pfp_bwt_run(*S, dT, false, dO, n, W, P)
-                                                   : lyndon_bwt_run(*S, dT, false, dO, n);
+        int64_t idx;
+        if (learned) {
+            // stage once: the (W,P) choice reads the staged text, the build reuses it
+            uint8_t* T = stage_text(*S, dT, false, (uint64_t)n);
+            int w = 5, p = 16;
+            selector_choose_device(*S, T, (uint64_t)n, &w, &p, nullptr);
+            idx = pfp_bwt_run(*S, nullptr, false, dO, n, w, p);
+        } else if (method == MIBWT_METHOD_PFP) {
+            idx = pfp_bwt_run(*S, dT, false, dO, n, W, P);
+        } else {
+            idx = lyndon_bwt_run(*S, dT, false, dO, n);
+        }
         if (ev) MIBWT_HIP(hipEventDestroy(ev));
         return idx;
     });
@@ -193,6 +206,17 @@ int64_t mibwt_pfp_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int6
 int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n, void* stream)
 {
     return device_bwt(storage, dT, dO, n, MIBWT_METHOD_LYNDON, 5, 16, stream);
+}
+
+int64_t mibwt_pfp_triggers(void* storage, const uint8_t* T, int64_t n, int32_t W, int32_t P, uint32_t* E,
+                           int64_t cap)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !T || (!E && cap > 0) || cap < 0 || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        return pfp_triggers_run(*S, T, n, W, P, E, cap);
+    });
 }
 
 int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t* lam)
@@ -337,12 +361,12 @@ int64_t mibwt_wp_grid(int32_t* W, int32_t* P)
 {
     if (!W || !P) return LIBCUBWT_BAD_PARAMETER;
     int w[64], p[64];
-    wp_grid(w, p);
-    for (int i = 0; i < 40; i++) {
+    const int count = wp_grid(w, p);
+    for (int i = 0; i < count; i++) {
         W[i] = w[i];
         P[i] = p[i];
     }
-    return 40;
+    return count;
 }
 
 int64_t mibwt_selector_load(const char* path)

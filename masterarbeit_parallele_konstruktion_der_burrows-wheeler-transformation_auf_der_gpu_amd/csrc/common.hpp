@@ -73,6 +73,17 @@ struct Phase {
     double bytes = 0;
 };
 
+// State of one single-pass ("chained") scan launch.  Tiles take tickets in launch order and
+// publish status words: bits 63..34 the launch's epoch, 33..32 the state (1 = tile total,
+// 2 = inclusive prefix), 31..0 the value.  Words left by older launches carry an older epoch
+// and read as "not yet published", so the status array is never cleared between launches.
+// The tile that draws the last ticket resets the ticket counter for the next launch.
+struct Chain {
+    uint64_t* status;
+    uint32_t* ticket;
+    uint32_t epoch;
+};
+
 // Everything one libcubwt-style storage owns.  One stream; not safe for concurrent calls.
 struct Storage {
     int device = 0;
@@ -98,6 +109,10 @@ struct Storage {
     int64_t last_n = 0;
     int last_W = 0, last_P = 0;
     int64_t last_sort_items = 0;
+    // chained scans: epoch of the last launch
+    uint32_t chain_epoch = 0;
+    // selector sketch tables: epoch of the last use
+    uint32_t sketch_epoch = 0;
 
     // device buffer #id with capacity >= bytes; contents are not preserved on growth
     template <typename T>
@@ -124,6 +139,9 @@ struct Storage {
         }
         return reinterpret_cast<T*>(b.ptr);
     }
+
+    // status words for a chained scan over `tiles` tiles (cleared when first allocated)
+    Chain chain(uint32_t tiles);
 
     template <typename T>
     T* host(size_t count)
@@ -271,6 +289,7 @@ enum BufId : int {
     B_LG_V1,
     B_LG_I,
     B_ALPHA,
+    B_ALPHA_PART,
     B_LY_RES,
     B_LY_FIN,
     B_LY_LIST0,
@@ -289,6 +308,14 @@ enum BufId : int {
     B_LY_VL,
     B_LY_JMP0,
     B_LY_JMP1,
+    B_AS_REC,
+    B_AS_R,
+    B_AS_HD,
+    B_AS_PREC,
+    B_CHAIN_STATUS,
+    B_CHAIN_TICKET,
+    B_SEL_SKETCH,
+    B_SEL_PART,
     B_SEL_HIST2,
     B_SEL_COUNTS,
     B_SEL_X,
@@ -325,5 +352,20 @@ enum ScalarSlot : int {
     SC_SIGMA,
     SC_COUNT_ = 16
 };
+
+inline Chain Storage::chain(uint32_t tiles)
+{
+    const size_t have = ((size_t)B_CHAIN_STATUS < bufs.size()) ? bufs[B_CHAIN_STATUS].cap : 0;
+    if (chain_epoch == 0 || chain_epoch >= (1u << 30) - 1 || have < tiles * sizeof(uint64_t) + 256) {
+        uint64_t* st = buf<uint64_t>(B_CHAIN_STATUS, tiles);
+        uint32_t* tk = buf<uint32_t>(B_CHAIN_TICKET, 4);
+        MIBWT_HIP(hipMemsetAsync(st, 0, bufs[B_CHAIN_STATUS].cap, stream));
+        MIBWT_HIP(hipMemsetAsync(tk, 0, 4 * sizeof(uint32_t), stream));
+        chain_epoch = 0;
+    }
+    chain_epoch++;
+    return Chain{reinterpret_cast<uint64_t*>(bufs[B_CHAIN_STATUS].ptr), reinterpret_cast<uint32_t*>(bufs[B_CHAIN_TICKET].ptr),
+                 chain_epoch};
+}
 
 }  // namespace mibwt

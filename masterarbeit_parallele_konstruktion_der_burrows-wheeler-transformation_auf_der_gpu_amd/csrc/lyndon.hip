@@ -10,11 +10,12 @@
 // right to left (:647-2062), deduplicates rules by Karp-Rabin fingerprints (:2183-2588),
 // sorts rules (:3129-4083) and induces the BWT over rank groups (:4114-4896).  Here:
 //
-//  1. lyndon   Lyndon array: every 256-byte block runs the right-to-left stack loop on its
-//              own; a position whose word may still grow with text beyond the block stays
-//              "open" with a resume point; open positions are resolved by a merge tree over
-//              neighbouring block ranges (one thread per range pair and level).
-//  2. rules    Karp-Rabin prefix fingerprints (mod 2^61-1) of the text in one device scan.
+//  1. rules    Karp-Rabin prefix fingerprints (mod 2^61-1) of the text in one device scan.
+//  2. lyndon   Lyndon array as next-smaller-suffix distances: every 256-byte block chases
+//              NSS pointers right to left; positions without a smaller suffix in the block
+//              stay open and are resolved by a merge tree over neighbouring ranges, where an
+//              open position bisects the right range's prefix minima.  Suffix comparisons
+//              use the fingerprints for long common prefixes (O(log lcp) per comparison).
 //  3. rsort    positions ordered by P_i: radix sort of the first symbols packed with dense
 //              alphabet codes, then groups that tie on that prefix are ordered with a
 //              fingerprint-LCP comparator (equal fingerprints over the whole prefix = the same
@@ -48,139 +49,6 @@ __device__ __forceinline__ uint32_t lcp_text(const uint8_t* __restrict__ T, uint
     }
     while (k < lim && T[a + k] == T[b + k]) k++;
     return k;
-}
-
-enum : int { kStop = 0, kMerge = 1, kWait = 2 };
-
-// Right-to-left stack step: may the word w = T[i..j) absorb the word t starting at j (current
-// length lam_j; fin_j tells whether that length is final)?  Text beyond a block is readable;
-// only the final length of an open t is unknown, and the answer waits only when it depends
-// on that length.
-__device__ __forceinline__ int lyn_step(const uint8_t* __restrict__ T, uint64_t n, uint32_t i, uint32_t j,
-                                        uint32_t lam_j, bool fin_j)
-{
-    const uint32_t lw = j - i;
-    const uint32_t k = lcp_text(T, n, i, j, lw);
-    if (k < lw && (uint64_t)j + k < n) {
-        const uint8_t a = T[i + k], b = T[j + k];
-        if (k < lam_j) return (a < b) ? kMerge : kStop;
-        if (a > b) return kStop;            // w is larger than every extension of t
-        return fin_j ? kStop : kWait;       // t is a proper prefix of w unless it grows past k
-    }
-    if ((uint64_t)j + k >= n && k < lw) return kStop;  // the rest of the text is a proper prefix of w
-    if (lam_j > lw) return kMerge;          // w is a proper prefix of t
-    return fin_j ? kStop : kWait;
-}
-
-// One thread per block: right-to-left loop with the block end as a horizon.
-__global__ void k_lyn_local(const uint8_t* __restrict__ T, uint64_t n, uint32_t nb, uint32_t* __restrict__ lam,
-                            uint32_t* __restrict__ res, uint8_t* __restrict__ fin, uint32_t* __restrict__ olist,
-                            uint32_t* __restrict__ ocnt)
-{
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint32_t s = b * kLynBlock;
-    const uint32_t e = (uint32_t)(((uint64_t)s + kLynBlock < n) ? s + kLynBlock : n);
-    uint32_t cnt = 0;
-    for (uint32_t i = e; i-- > s;) {
-        uint32_t j = i + 1;
-        bool ok = true;
-        for (;;) {
-            if (j == n) break;
-            if (j == e) {
-                ok = false;
-                break;
-            }
-            const uint32_t lj = lam[j];
-            const bool fj = fin[j] != 0;
-            const int r = lyn_step(T, n, i, j, lj, fj);
-            if (r == kMerge) {
-                if (!fj) {
-                    ok = false;
-                    break;
-                }
-                j += lj;
-            } else {
-                if (r == kWait) ok = false;
-                break;
-            }
-        }
-        lam[i] = j - i;
-        res[i] = j;
-        fin[i] = ok ? 1 : 0;
-        if (!ok) olist[s + cnt++] = i;
-    }
-    ocnt[b] = cnt;
-}
-
-// One thread per pair of neighbouring ranges of `size` blocks: the left range's open
-// positions (right to left) continue into the right range.
-__global__ void k_lyn_merge(const uint8_t* __restrict__ T, uint64_t n, uint32_t nb, uint32_t size,
-                            uint32_t* __restrict__ lam, uint32_t* __restrict__ res, uint8_t* __restrict__ fin,
-                            const uint32_t* __restrict__ in_list, const uint32_t* __restrict__ in_cnt,
-                            uint32_t* __restrict__ out_list, uint32_t* __restrict__ out_cnt)
-{
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint64_t a = (uint64_t)t * 2 * size;
-    if (a >= nb) return;
-    const uint64_t rb = a + size;
-    const uint64_t end64 = (a + 2 * size) * kLynBlock;
-    const uint32_t r_end = (uint32_t)(end64 < n ? end64 : n);
-    const uint64_t out = a * kLynBlock;
-    uint32_t oc = 0;
-    if (rb < nb) {
-        const uint32_t c = in_cnt[rb];
-        for (uint32_t q = 0; q < c; q++) out_list[out + oc++] = in_list[rb * kLynBlock + q];
-    }
-    const uint32_t cl = in_cnt[a];
-    for (uint32_t q = 0; q < cl; q++) {
-        const uint32_t i = in_list[a * kLynBlock + q];
-        uint32_t j = res[i];
-        bool ok = true;
-        for (;;) {
-            if (j == n) break;
-            if (j == r_end || !fin[j]) {
-                ok = false;
-                break;
-            }
-            const uint32_t lj = lam[j];
-            if (lyn_step(T, n, i, j, lj, true) == kMerge) j += lj;
-            else break;
-        }
-        res[i] = j;
-        if (ok) {
-            lam[i] = j - i;
-            fin[i] = 1;
-        } else {
-            out_list[out + oc++] = i;
-        }
-    }
-    out_cnt[a] = oc;
-}
-
-// lam[] of T[0..n) (the padded text copy T).  Returns the number of merge levels.
-int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, uint32_t* lam)
-{
-    const uint32_t nb = ceil_div(n, kLynBlock);
-    uint32_t* res = S.buf<uint32_t>(B_LY_RES, n);
-    uint8_t* fin = S.buf<uint8_t>(B_LY_FIN, n);
-    uint32_t* la = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * kLynBlock);
-    uint32_t* lb = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * kLynBlock);
-    uint32_t* ca = S.buf<uint32_t>(B_LY_CNT0, nb);
-    uint32_t* cb = S.buf<uint32_t>(B_LY_CNT1, nb);
-    k_lyn_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(T, n, nb, lam, res, fin, la, ca);
-    MIBWT_LAUNCH_CHECK();
-    int levels = 0;
-    for (uint64_t size = 1; size < nb; size *= 2) {
-        const uint32_t pairs = (uint32_t)((nb + 2 * size - 1) / (2 * size));
-        k_lyn_merge<<<ceil_div(pairs, 64), 64, 0, S.stream>>>(T, n, nb, (uint32_t)size, lam, res, fin, la, ca,
-                                                              lb, cb);
-        MIBWT_LAUNCH_CHECK();
-        std::swap(la, lb);
-        std::swap(ca, cb);
-        levels++;
-    }
-    return levels;
 }
 
 // ------------------------------------------------------------------- 2. fingerprints
@@ -242,6 +110,207 @@ struct PrefixHash {
         return hb >= x ? hb - x : hb + kM61 - x;
     }
 };
+
+// ------------------------------------------------------------------- 1. Lyndon array
+//
+// lam[i] = NSS[i] - i, NSS[i] = the nearest j > i whose suffix is smaller (n if none).
+// Suffixes are compared through their longest common prefix: up to 32 bytes directly, then by
+// galloping and bisection on the prefix fingerprints, so the long repeats of a collection cost
+// O(log lcp) instead of O(lcp).
+struct SuffixCmp {
+    const uint8_t* T;
+    uint64_t n;
+    PrefixHash ph;
+    __device__ uint32_t lcp(uint32_t a, uint32_t b) const
+    {
+        const uint32_t m = (uint32_t)(n - (a > b ? a : b));  // comparable length
+        uint32_t k = 0;
+        while (k < 32 && k + 8 <= m) {
+            const uint64_t x = load8(T, a + k) ^ load8(T, b + k);
+            if (x) return k + (uint32_t)(__builtin_ctzll(x) >> 3);
+            k += 8;
+        }
+        if (k < 32) {
+            while (k < m && T[a + k] == T[b + k]) k++;
+            return k;
+        }
+        // prefixes agree on [0, lo); find the first disagreement in [lo, hi)
+        uint32_t lo = k, step = 32, hi;
+        for (;;) {
+            if (step > m - lo) {
+                hi = m + 1;
+                break;
+            }
+            if (ph.sub(a + lo, step) == ph.sub(b + lo, step)) {
+                lo += step;
+                step *= 2;
+            } else {
+                hi = lo + step;
+                break;
+            }
+        }
+        // largest l in [lo, hi) with equal prefixes of length l
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (ph.sub(a, mid) == ph.sub(b, mid)) lo = mid;
+            else hi = mid;
+        }
+        return lo;
+    }
+    // suffix(a) < suffix(b) (a != b); a suffix that is a prefix of the other is smaller
+    __device__ bool less(uint32_t a, uint32_t b) const
+    {
+        const uint32_t l = lcp(a, b);
+        if ((uint64_t)a + l == n) return true;
+        if ((uint64_t)b + l == n) return false;
+        return T[a + l] < T[b + l];
+    }
+};
+
+constexpr uint32_t kNoNss = 0xFFFFFFFFu;
+
+// One thread per 256-byte block, right to left: NSS by pointer chasing inside the block.
+// Positions without a smaller suffix inside the block stay open (kNoNss).  Per block: the
+// open positions in ascending order (O, increasing suffixes) and the chain of prefix minima
+// from the block start (M = s, NSS[s], NSS[NSS[s]], ..., decreasing suffixes).
+__global__ void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
+                            uint32_t* __restrict__ oc, uint32_t* __restrict__ M, uint32_t* __restrict__ mc)
+{
+    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nb) return;
+    const uint32_t s = b * kLynBlock;
+    const uint32_t e = (uint32_t)(((uint64_t)s + kLynBlock < sc.n) ? s + kLynBlock : sc.n);
+    for (uint32_t i = e; i-- > s;) {
+        uint32_t j = i + 1;
+        while (j < e && !sc.less(j, i)) j = nss[j];  // kNoNss ends the walk as well
+        nss[i] = (j < e) ? j : kNoNss;
+    }
+    uint32_t c = 0;
+    for (uint32_t i = s; i < e; i++)
+        if (nss[i] == kNoNss) O[s + c++] = i;
+    oc[b] = c;
+    c = 0;
+    for (uint32_t m = s; m != kNoNss; m = nss[m]) M[s + c++] = m;
+    mc[b] = c;
+}
+
+// One wavefront per pair of neighbouring ranges L, R of `size` blocks.  An open position i
+// of L finds its NSS among R's prefix minima: they decrease, so the first one below suffix(i)
+// is found by bisection.  The merged range keeps L's unresolved positions followed by R's
+// open ones, and L's prefix minima followed by the prefix minima of R below min(L).
+__global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uint32_t size,
+                                                  uint32_t* __restrict__ nss, const uint32_t* __restrict__ Oin,
+                                                  const uint32_t* __restrict__ ocin, const uint32_t* __restrict__ Min,
+                                                  const uint32_t* __restrict__ mcin, uint32_t* __restrict__ Oout,
+                                                  uint32_t* __restrict__ ocout, uint32_t* __restrict__ Mout,
+                                                  uint32_t* __restrict__ mcout)
+{
+    const int lane = threadIdx.x;
+    const uint64_t a = (uint64_t)blockIdx.x * 2 * size;
+    if (a >= nb) return;
+    const uint64_t rb = a + size;
+    const uint64_t sL = a * kLynBlock;
+    const uint32_t oL = ocin[a], mL = mcin[a];
+    const uint32_t* OL = Oin + sL;
+    const uint32_t* ML = Min + sL;
+    if (rb >= nb) {
+        for (uint32_t q = lane; q < oL; q += 64) Oout[sL + q] = OL[q];
+        for (uint32_t q = lane; q < mL; q += 64) Mout[sL + q] = ML[q];
+        if (lane == 0) {
+            ocout[a] = oL;
+            mcout[a] = mL;
+        }
+        return;
+    }
+    const uint64_t sR = rb * kLynBlock;
+    const uint32_t oR = ocin[rb], mR = mcin[rb];
+    const uint32_t* OR = Oin + sR;
+    const uint32_t* MR = Min + sR;
+    // bisection: first t in [0, mR) with suffix(MR[t]) < suffix(x), mR if none
+    auto first_below = [&](uint32_t x) -> uint32_t {
+        uint32_t lo = 0, hi = mR;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (sc.less(MR[mid], x)) hi = mid;
+            else lo = mid + 1;
+        }
+        return lo;
+    };
+    // L's open positions: resolve or keep (ordered compaction)
+    uint32_t kept = 0;
+    for (uint32_t q0 = 0; q0 < oL; q0 += 64) {
+        const uint32_t q = q0 + lane;
+        bool keep = false;
+        if (q < oL) {
+            const uint32_t i = OL[q];
+            const uint32_t t = first_below(i);
+            if (t < mR) nss[i] = MR[t];
+            else keep = true;
+        }
+        const uint64_t bal = __ballot(keep);
+        if (keep) Oout[sL + kept + (uint32_t)__popcll(bal & lanemask_lt())] = OL[q];
+        kept += (uint32_t)__popcll(bal);
+    }
+    for (uint32_t q = lane; q < oR; q += 64) Oout[sL + kept + q] = OR[q];
+    // prefix minima: all of L's, then R's below min(L) = the last of L's
+    const uint32_t tstar = first_below(ML[mL - 1]);  // every lane computes the same value
+    for (uint32_t q = lane; q < mL; q += 64) Mout[sL + q] = ML[q];
+    for (uint32_t q = tstar + lane; q < mR; q += 64) Mout[sL + mL + (q - tstar)] = MR[q];
+    if (lane == 0) {
+        ocout[a] = kept + oR;
+        mcout[a] = mL + (mR - tstar);
+    }
+}
+
+__global__ void k_nss_to_lam(uint32_t* __restrict__ a, uint64_t n)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t j = a[i];
+    a[i] = (j == kNoNss ? (uint32_t)n : j) - (uint32_t)i;
+}
+
+// lam[] of T[0..n) (the padded text copy T, prefix fingerprints ph).  Returns the number of
+// merge levels.
+int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32_t* lam)
+{
+    const uint32_t nb = ceil_div(n, kLynBlock);
+    uint32_t* oa = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * kLynBlock);
+    uint32_t* ob = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * kLynBlock);
+    uint32_t* ma = S.buf<uint32_t>(B_LY_RES, (size_t)nb * kLynBlock);
+    uint32_t* mb = S.buf<uint32_t>(B_LY_JMP0, (size_t)nb * kLynBlock);
+    uint32_t* oca = S.buf<uint32_t>(B_LY_CNT0, nb);
+    uint32_t* ocb = S.buf<uint32_t>(B_LY_CNT1, nb);
+    uint32_t* mca = S.buf<uint32_t>(B_LY_HF, nb);
+    uint32_t* mcb = S.buf<uint32_t>(B_LY_HF2, nb);
+    const SuffixCmp sc{T, n, ph};
+    k_nss_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(sc, nb, lam, oa, oca, ma, mca);
+    MIBWT_LAUNCH_CHECK();
+    int levels = 0;
+    for (uint64_t size = 1; size < nb; size *= 2) {
+        const uint32_t pairs = (uint32_t)((nb + 2 * size - 1) / (2 * size));
+        k_nss_merge<<<pairs, 64, 0, S.stream>>>(sc, nb, (uint32_t)size, lam, oa, oca, ma, mca, ob, ocb, mb, mcb);
+        MIBWT_LAUNCH_CHECK();
+        std::swap(oa, ob);
+        std::swap(oca, ocb);
+        std::swap(ma, mb);
+        std::swap(mca, mcb);
+        levels++;
+    }
+    k_nss_to_lam<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n);
+    MIBWT_LAUNCH_CHECK();
+    return levels;
+}
+
+// Prefix fingerprints pre[k] = (H(T[0..k]), base^(k+1)) mod 2^61-1 by one device scan.
+HashPair* prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base)
+{
+    HashPair* pre = S.buf<HashPair>(B_LY_HASH, n);
+    k_hash_items<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, base, pre);
+    MIBWT_LAUNCH_CHECK();
+    device_scan<HashPair, OpHash>(S, pre, pre, n, true, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN0);
+    return pre;
+}
 
 // ------------------------------------------------------------------- 3. rule order
 
@@ -501,22 +570,23 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         T = stage_text(S, src, false, n);
     }
 
-    // ---- 1. Lyndon array
-    int ph = S.phase_begin("lyndon");
+    // ---- 1. prefix fingerprints
+    int ph = S.phase_begin("rules");
+    uint64_t base = 0x1F3D5B79A2C4E681ull % kM61;
+    HashPair* pre = prefix_hashes(S, T, n, base);
+    bool pre_fresh = true;
+    S.phase_end(ph, (double)n * (1 + 16 * 4));
+
+    // ---- 2. Lyndon array
+    ph = S.phase_begin("lyndon");
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
-    const int levels = lyndon_array(S, T, n, lam);
+    const int levels = lyndon_array(S, T, n, PrefixHash{pre}, lam);
     uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
     alphabet_codes(S, T, n, codes);
     uint32_t sigma = 0;
     S.read(&sigma, codes + 256);
     S.phase_end(ph, (double)n * (1 + 4 + 4 + 1) * 2);
     S.stats.doubling_rounds_dict = levels;
-
-    // ---- 2. prefix fingerprints
-    ph = S.phase_begin("rules");
-    HashPair* pre = S.buf<HashPair>(B_LY_HASH, n);
-    uint64_t base = 0x1F3D5B79A2C4E681ull % kM61;
-    S.phase_end(ph, 0);
 
     // ---- 3. label order
     ph = S.phase_begin("rsort");
@@ -541,9 +611,8 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
     double rbytes = (double)n * 24 * ((bits * per + 7) / 8);
     for (int attempt = 0;; attempt++) {
         MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "label fingerprint collisions persist");
-        k_hash_items<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, base, pre);
-        MIBWT_LAUNCH_CHECK();
-        device_scan<HashPair, OpHash>(S, pre, pre, n, true, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN0);
+        if (!pre_fresh) pre = prefix_hashes(S, T, n, base);
+        pre_fresh = false;
         LabelCmp cmp{T, lam, PrefixHash{pre}};
         // working copies: V (order) and hf (heads) of the packed-key order
         uint32_t* V = Vl;
@@ -656,7 +725,8 @@ int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n64, uint32_t* 
     const uint64_t n = (uint64_t)n64;
     uint8_t* T = stage_text(S, src, true, n);
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
-    const int levels = lyndon_array(S, T, n, lam);
+    HashPair* pre = prefix_hashes(S, T, n, 0x1F3D5B79A2C4E681ull % kM61);
+    const int levels = lyndon_array(S, T, n, PrefixHash{pre}, lam);
     MIBWT_HIP(hipMemcpyAsync(lam_host, lam, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, S.stream));
     MIBWT_HIP(hipStreamSynchronize(S.stream));
     return levels;

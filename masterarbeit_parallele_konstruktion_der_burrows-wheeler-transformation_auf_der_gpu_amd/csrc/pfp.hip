@@ -63,6 +63,14 @@ __device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint64_t n, uint3
     return h;
 }
 
+// 8 text bytes at word index k of a stream starting sh/8 bytes into word w[0].
+__device__ __forceinline__ uint64_t funnel8(uint64_t lo, uint64_t hi, int sh)
+{
+    return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+}
+
+// Byte equality of two phrases.  Both are streamed as aligned words, 32 bytes per step with
+// all eight loads issued together (the padded text copy makes the over-read safe).
 __device__ bool phrase_equal(const uint8_t* __restrict__ T, uint64_t n, uint32_t s1, uint32_t e1,
                              uint32_t s2, uint32_t e2)
 {
@@ -71,15 +79,30 @@ __device__ bool phrase_equal(const uint8_t* __restrict__ T, uint64_t n, uint32_t
     if (x1 != x2) return false;  // only the last phrase reaches into the terminator
     const uint32_t len = (x1 ? (uint32_t)n - s1 : e1 - s1 + 1);
     if (x1 && (uint32_t)n - s2 != len) return false;
-    for (uint32_t i = 0; i < len; i += 8) {
-        uint64_t a = load8(T, s1 + i), b = load8(T, s2 + i);
-        const uint32_t c = len - i;
-        if (c < 8) {
-            const uint64_t m = (~0ull) >> (64 - 8 * c);
-            a &= m;
-            b &= m;
+    const uint64_t* w1 = reinterpret_cast<const uint64_t*>(T + (s1 & ~7u));
+    const uint64_t* w2 = reinterpret_cast<const uint64_t*>(T + (s2 & ~7u));
+    const int sh1 = (int)(s1 & 7u) * 8, sh2 = (int)(s2 & 7u) * 8;
+    uint64_t p1 = w1[0], p2 = w2[0];
+    for (uint32_t i = 0; i < len; i += 32) {
+        const uint32_t q = i >> 3;
+        uint64_t a[4], b[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            a[j] = w1[q + 1 + j];
+            b[j] = w2[q + 1 + j];
         }
-        if (a != b) return false;
+        uint64_t diff = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint64_t u = funnel8(j ? a[j - 1] : p1, a[j], sh1);
+            const uint64_t v = funnel8(j ? b[j - 1] : p2, b[j], sh2);
+            const uint32_t c = i + 8 * j;
+            const uint64_t m = (c >= len) ? 0ull : ((len - c >= 8) ? ~0ull : ((~0ull) >> (64 - 8 * (len - c))));
+            diff |= (u ^ v) & m;
+        }
+        if (diff) return false;
+        p1 = a[3];
+        p2 = b[3];
     }
     return true;
 }
@@ -196,6 +219,219 @@ __global__ __launch_bounds__(kBlock) void k_trig_emit(const uint16_t* __restrict
     }
 }
 
+// Single-pass trigger scan.  The window value V = sum_k T[e-W+1+k] * 2^(W-1-k) is kept
+// exactly (32 bits for W <= 24, 64 bits for W <= 56) and rolled per window end:
+// V' = 2V + T[e] - T[e-W] * 2^W.  V mod P == 0 is tested without division: for
+// P = 2^s * q (q odd), P | V  <=>  rotr(V * q^-1, s) <= floor((2^bits - 1) / P); for q = 1 it
+// is a mask test.  The first-window rule needs V == V(T[0..W-1]), confirmed byte by byte.
+// Each thread owns 32 consecutive window ends (its bytes arrive by two 16-byte loads), the
+// outgoing bytes T[e-W] come from an LDS copy of the tile, and the tile's trigger count is
+// chained to its predecessors by decoupled look-back, so the end positions are written in
+// order in the same pass.
+constexpr int kScanPer = 32;
+constexpr int kScanTileW = kBlock * kScanPer;  // window ends per tile
+constexpr int kScanHalo = 64;                   // bytes staged before the tile
+constexpr int kScanMaxW = 56;
+constexpr int kScanSub = 16;                    // sub-tiles per chained tile
+
+struct TrigScan {
+    const uint8_t* T;
+    uint64_t n;
+    int W;
+    uint32_t P;
+    uint32_t pshift;  // P = 2^pshift * q, q odd
+    uint64_t qinv;    // q^-1 mod 2^64
+    uint64_t plim;    // floor((2^bits - 1) / P) for the value width in use
+    uint32_t tiles;
+    uint32_t cap;     // capacity of E
+    uint32_t* E;
+    uint32_t* total;  // number of triggers (written by the last tile)
+    Chain chain;
+};
+
+template <typename V, bool kPow2>
+__device__ __forceinline__ bool trig_divisible(V v, const TrigScan& p)
+{
+    if (kPow2) return (v & (V)(p.P - 1)) == 0;
+    const V x = v * (V)p.qinv;
+    V r;
+    if (sizeof(V) == 4) r = (V)__builtin_rotateright32((uint32_t)x, p.pshift);
+    else r = (V)__builtin_rotateright64((uint64_t)x, p.pshift);
+    return r <= (V)p.plim;
+}
+
+// Trigger mask of the 32 window ends [e0, e0+32) of one thread; `txt` holds the current
+// sub-tile's bytes from kScanHalo before its first window end (LDS), a/b the thread's own 32
+// bytes.
+template <typename V, bool kPow2>
+__device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc, const uint4& a, const uint4& b,
+                                                const TrigScan& p, V v0, const uint8_t* first, uint64_t e0)
+{
+    const int W = p.W;
+    // value of the window ending at e0 - 1
+    V v = 0;
+    for (int k = 0; k < W; k++) v = (v << 1) + (V)txt[loc - W + k];
+    // outgoing bytes T[e0 - W + i], i < 32, re-aligned from dword reads
+    const uint32_t ob = loc - (uint32_t)W;
+    const uint32_t oa = ob & ~3u, sh = ob & 3u;
+    uint32_t ow[9];
+#pragma unroll
+    for (int m = 0; m < 9; m++) ow[m] = *reinterpret_cast<const uint32_t*>(txt + oa + 4 * m);
+    uint32_t out[8];
+#pragma unroll
+    for (int m = 0; m < 8; m++) out[m] = __builtin_amdgcn_alignbyte(ow[m + 1], ow[m], sh);
+    const uint32_t in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+
+    uint32_t mh = 0, mc = 0;  // bit 31 - i: window e0 + i (hash rule / value equals the first window's)
+#pragma unroll
+    for (int i = 0; i < kScanPer; i++) {
+        const uint32_t cin = (in[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        const uint32_t cout = (out[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+        v = (v << 1) + (V)cin - ((V)cout << W);
+        mh = (mh << 1) + (trig_divisible<V, kPow2>(v, p) ? 1u : 0u);
+        mc = (mc << 1) + ((v == v0) ? 1u : 0u);
+    }
+    uint32_t mask = __brev(mh);
+    uint32_t cand = __brev(mc) & ~mask;
+    // first-window rule: confirm the candidates byte by byte
+    while (cand) {
+        const int i = __ffs(cand) - 1;
+        cand &= cand - 1;
+        bool eq = true;
+        for (int k = 0; k < W && eq; k++) eq = (txt[loc + i - W + 1 + k] == first[k]);
+        if (eq) mask |= 1u << i;
+    }
+    // window ends W <= e < n
+    if (e0 + kScanPer > p.n) mask &= (e0 >= p.n) ? 0u : ((1u << (uint32_t)(p.n - e0)) - 1u);
+    if (e0 < (uint64_t)W) mask &= ~((W - e0 >= 32) ? ~0u : ((1u << (uint32_t)(W - e0)) - 1u));
+    return mask;
+}
+
+// One tile = kScanSub sub-tiles of kScanTileW window ends.  Each sub-tile is staged in LDS
+// (own bytes by two 16-byte loads per thread plus a 64-byte halo) and turned into one 32-bit
+// mask per thread; the masks stay in registers until the tile's offset is known.  Large tiles
+// keep the number of chained tiles (one ticket each) small.
+template <typename V, bool kPow2>
+__global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
+{
+    __shared__ __align__(16) uint8_t txt[kScanHalo + kScanTileW + 16];
+    __shared__ uint8_t first[64];
+    __shared__ uint32_t masks[kScanSub][kBlock];
+    __shared__ V v0s;
+    __shared__ uint32_t slot;
+    __shared__ uint32_t wsum[kWaves];
+    const int W = p.W;
+    const uint32_t tile = chain_ticket(p.chain, p.tiles, &slot);
+    if (threadIdx.x < 64) {
+        // value of the first window T[0..W-1], reduced over wavefront 0
+        const int k = threadIdx.x;
+        const uint8_t c = (k < W && (uint64_t)k < p.n) ? p.T[k] : 0;
+        first[k] = c;
+        V x = (k < W) ? ((V)c << (W - 1 - k)) : (V)0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            if (sizeof(V) == 4) x += (V)__shfl_xor((uint32_t)x, o, 64);
+            else x += (V)__shfl_xor((unsigned long long)x, o, 64);
+        }
+        if (k == 0) v0s = x;
+    }
+    const uint32_t loc = kScanHalo + threadIdx.x * kScanPer;
+    uint32_t cnt = 0;
+#pragma unroll 1
+    for (int sub = 0; sub < kScanSub; sub++) {
+        const uint64_t sb = ((uint64_t)tile * kScanSub + sub) * kScanTileW;
+        const uint64_t e0 = sb + (uint64_t)threadIdx.x * kScanPer;
+        uint4 a = make_uint4(0, 0, 0, 0), b = make_uint4(0, 0, 0, 0);
+        if (e0 < p.n) {
+            a = *reinterpret_cast<const uint4*>(p.T + e0);
+            b = *reinterpret_cast<const uint4*>(p.T + e0 + 16);
+        }
+        if (sub) __syncthreads();  // the previous sub-tile's LDS reads are done
+        *reinterpret_cast<uint4*>(txt + loc) = a;
+        *reinterpret_cast<uint4*>(txt + loc + 16) = b;
+        if (threadIdx.x < kScanHalo / 16) {
+            const int64_t g = (int64_t)sb - kScanHalo + (int64_t)threadIdx.x * 16;
+            uint4 h = make_uint4(0, 0, 0, 0);
+            if (g >= 0 && (uint64_t)g < p.n + kPad) h = *reinterpret_cast<const uint4*>(p.T + g);
+            *reinterpret_cast<uint4*>(txt + threadIdx.x * 16) = h;
+        } else if (threadIdx.x == kScanHalo / 16) {
+            *reinterpret_cast<uint4*>(txt + kScanHalo + kScanTileW) = make_uint4(0, 0, 0, 0);
+        }
+        __syncthreads();
+        const uint32_t m = (sb < p.n) ? trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0) : 0u;
+        masks[sub][threadIdx.x] = m;
+        cnt += (uint32_t)__popc(m);
+    }
+    // order inside the tile: sub-tile major, then thread; per-sub-tile offsets from scans
+    uint32_t tot_tile;
+    (void)block_exclusive_scan(cnt, 0u, OpSum{}, wsum, &tot_tile);
+    const uint32_t base = chain_prefix(p.chain, tile, tot_tile, OpSum{}, 0u, &slot);
+    if (tile == p.tiles - 1 && threadIdx.x == 0) *p.total = base + tot_tile;
+    uint32_t run = base;
+#pragma unroll 1
+    for (int sub = 0; sub < kScanSub; sub += 2) {
+        // two sub-tiles per scan: counts are at most 32 * kBlock < 2^16
+        const uint32_t m0 = masks[sub][threadIdx.x], m1 = masks[sub + 1][threadIdx.x];
+        const uint32_t pair = (uint32_t)__popc(m0) | ((uint32_t)__popc(m1) << 16);
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(pair, 0u, OpSum{}, wsum, &tot);
+        const uint32_t tot0 = tot & 0xFFFFu, tot1 = tot >> 16;
+        for (int h = 0; h < 2; h++) {
+            uint32_t o = run + (h ? tot0 + (ex >> 16) : (ex & 0xFFFFu));
+            uint32_t mm = h ? m1 : m0;
+            const uint64_t e0 = ((uint64_t)tile * kScanSub + sub + h) * kScanTileW + (uint64_t)threadIdx.x * kScanPer;
+            while (mm) {
+                const int i = __ffs(mm) - 1;
+                mm &= mm - 1;
+                if (o < p.cap) p.E[o] = (uint32_t)(e0 + i);
+                o++;
+            }
+        }
+        run += tot0 + tot1;
+    }
+}
+
+// Modular inverse of an odd q modulo 2^64 (Newton iteration).
+inline uint64_t inverse_mod_2_64(uint64_t q)
+{
+    uint64_t x = q;  // correct to 3 bits
+    for (int i = 0; i < 5; i++) x *= 2 - q * x;
+    return x;
+}
+
+// Trigger end positions in order into E (capacity cap); returns the number of triggers,
+// which may exceed cap (then E holds the first cap of them).  Requires W <= kScanMaxW.
+uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, uint32_t* E, uint32_t cap,
+                   uint32_t* total_dev)
+{
+    TrigScan p{};
+    p.T = T;
+    p.n = n;
+    p.W = W;
+    p.P = P;
+    uint32_t s = 0, q = P;
+    while ((q & 1u) == 0) {
+        q >>= 1;
+        s++;
+    }
+    p.pshift = s;
+    p.qinv = inverse_mod_2_64(q);
+    const bool narrow = W <= 24;
+    p.plim = narrow ? (uint64_t)(0xFFFFFFFFull / P) : (~0ull) / P;
+    p.tiles = (uint32_t)ceil_div(n, (uint64_t)kScanTileW * kScanSub);
+    p.cap = cap;
+    p.E = E;
+    p.total = total_dev;
+    p.chain = S.chain(p.tiles);
+    const bool pow2 = (q == 1);
+    if (narrow && pow2) k_trig_scan<uint32_t, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    else if (narrow) k_trig_scan<uint32_t, false><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    else if (pow2) k_trig_scan<uint64_t, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    else k_trig_scan<uint64_t, false><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    MIBWT_LAUNCH_CHECK();
+    return p.tiles;
+}
+
 // --------------------------------------------------------------------- 2/3. dictionary
 
 __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
@@ -210,6 +446,10 @@ __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
     V[k] = k;
 }
 
+// phrase fingerprints are sorted by their low 48 bits (6 radix passes)
+constexpr int kDictKeyBits = 48;
+constexpr uint64_t kDictKeyMask = (1ull << kDictKeyBits) - 1;
+
 __global__ void k_dict_mark(const uint8_t* __restrict__ T, uint64_t n, int W,
                             const uint32_t* __restrict__ E, const uint64_t* __restrict__ H,
                             const uint32_t* __restrict__ V, uint32_t d, uint32_t* __restrict__ flag,
@@ -218,12 +458,14 @@ __global__ void k_dict_mark(const uint8_t* __restrict__ T, uint64_t n, int W,
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d) return;
     uint32_t f = 1;
-    if (j > 0 && H[j] == H[j - 1]) {
+    if (j > 0 && (H[j] & kDictKeyMask) == (H[j - 1] & kDictKeyMask)) {
+        // the sort looked at the low kDictKeyBits only: equal sort keys must be equal
+        // fingerprints of equal phrases, anything else re-seeds the hash
         f = 0;
         uint32_t s1, e1, s2, e2;
         phrase_bounds(E, V[j], W, s1, e1);
         phrase_bounds(E, V[j - 1], W, s2, e2);
-        if (!phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
+        if (H[j] != H[j - 1] || !phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
     }
     flag[j] = f;
 }
@@ -382,134 +624,162 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 
 // --------------------------------------------------------------------- 6. BWT assembly
 
-struct DictView {
-    const uint32_t* SA;     // sorted DT positions
-    const uint32_t* rank;   // 1 + group head position
-    const uint32_t* owner;  // distinct phrase of a DT position
-    const uint64_t* off;    // first DT position of a distinct phrase
-    const uint32_t* len;    // phrase length
-    const uint32_t* start;  // text position of the representative occurrence
-    const uint32_t* freq;   // occurrences of the phrase
-    const uint8_t* T;
-    uint32_t L;
-    int W;
-    uint64_t n;
-};
+// Per dictionary position x (dictionary order, coalesced): a packed record
+//   bits  0..31  frequency of the phrase,
+//   bits 32..39  character preceding the suffix inside the phrase (j >= 1),
+//   bit  40      suffix longer than W (an alpha, owns BWT slots),
+//   bit  41      whole phrase (j == 0).
+constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
-// character preceding the suffix at DT position x (j >= 1)
-__device__ __forceinline__ uint8_t dv_prev(const DictView& v, uint32_t x, uint32_t u, uint32_t j)
+__global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
+                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq, uint32_t L, int W,
+                         uint64_t* __restrict__ rec)
 {
-    return v.T[v.start[u] + j - 1];
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= L) return;
+    const uint32_t u = owner[x];
+    const uint32_t j = x - (uint32_t)off[u];
+    const uint32_t rem = len[u] - j;
+    uint64_t r = freq[u];
+    if (rem > (uint32_t)W) r |= kRecAlpha;
+    if (j == 0) r |= kRecWhole;
+    else if (rem > (uint32_t)W) r |= (uint64_t)T[start[u] + j - 1] << 32;
+    rec[x] = r;
 }
 
-// BWT slot counts per sorted position and the "mixed" mark of groups whose preceding
-// characters differ.
-__global__ void k_asm_count(DictView v, uint32_t* __restrict__ cnt, uint8_t* __restrict__ gmix)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= v.L) return;
-    const uint32_t x = v.SA[g];
-    const uint32_t u = v.owner[x];
-    const uint32_t j = x - (uint32_t)v.off[u];
-    const uint32_t rem = v.len[u] - j;
-    if (rem <= (uint32_t)v.W) {
-        cnt[g] = 0;
-        return;
-    }
-    cnt[g] = v.freq[u];
-    if (j >= 1) {
-        const uint32_t gh = v.rank[x] - 1;
-        if (gh != g) {
-            const uint32_t xh = v.SA[gh];
-            const uint32_t uh = v.owner[xh];
-            const uint32_t jh = xh - (uint32_t)v.off[uh];
-            if (dv_prev(v, x, u, j) != dv_prev(v, xh, uh, jh)) gmix[gh] = 1;
-        }
-    }
-}
-
-// slots written through the ordered entries: whole phrases and mixed groups
-__global__ void k_asm_ecnt(DictView v, const uint8_t* __restrict__ gmix, uint32_t* __restrict__ ecnt)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= v.L) return;
-    const uint32_t x = v.SA[g];
-    const uint32_t u = v.owner[x];
-    const uint32_t j = x - (uint32_t)v.off[u];
-    const uint32_t rem = v.len[u] - j;
-    uint32_t c = 0;
-    if (rem > (uint32_t)v.W && (j == 0 || gmix[v.rank[x] - 1])) c = v.freq[u];
-    ecnt[g] = c;
-}
-
-__global__ void k_asm_delta(const uint32_t* __restrict__ goff, const uint32_t* __restrict__ escan,
-                            uint32_t L, uint32_t* __restrict__ delta)
+// Sorted order: gather each position's record and its group head once.
+__global__ void k_asm_gather(const uint32_t* __restrict__ SA, const uint32_t* __restrict__ rank,
+                             const uint64_t* __restrict__ rec, uint32_t L, uint64_t* __restrict__ R,
+                             uint32_t* __restrict__ Hd)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
-    delta[g] = goff[g] - escan[g];
+    const uint32_t x = SA[g];
+    R[g] = rec[x];
+    Hd[g] = rank[x] - 1;
 }
 
-// direct fill of the single-character groups: every member writes its own freq slots
-__global__ void k_asm_fill(DictView v, const uint8_t* __restrict__ gmix, const uint32_t* __restrict__ goff,
-                           uint8_t* __restrict__ O)
+// a group is mixed when a member's preceding character differs from its head's
+__global__ void k_asm_mix(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd, uint32_t L,
+                          uint8_t* __restrict__ gmix)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= v.L) return;
-    const uint32_t x = v.SA[g];
-    const uint32_t u = v.owner[x];
-    const uint32_t j = x - (uint32_t)v.off[u];
-    const uint32_t rem = v.len[u] - j;
-    if (rem <= (uint32_t)v.W || j == 0) return;
-    if (gmix[v.rank[x] - 1]) return;
-    const uint8_t c = dv_prev(v, x, u, j);
-    const uint64_t o = goff[g];
-    const uint64_t f = v.freq[u];
-    const uint64_t e = (o + f <= v.n) ? o + f : v.n;  // guard: never write past the output
-    for (uint64_t i = o; i < e; i++) O[i] = c;
+    if (g >= L) return;
+    const uint64_t r = R[g];
+    const uint32_t h = Hd[g];
+    if (!(r & kRecAlpha) || (r & kRecWhole) || h == g) return;
+    if (((r >> 32) & 0xFF) != ((R[h] >> 32) & 0xFF)) gmix[h] = 1;
 }
 
-// mixed-group members in DT order (grouped by phrase)
-__global__ void k_mx_flag(DictView v, const uint8_t* __restrict__ gmix, uint32_t* __restrict__ mxf)
+// slots per sorted position (cnt) and slots written through ordered entries (ecnt)
+__global__ void k_asm_counts(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd,
+                             const uint8_t* __restrict__ gmix, uint32_t L, uint32_t* __restrict__ cnt,
+                             uint32_t* __restrict__ ecnt)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= L) return;
+    const uint64_t r = R[g];
+    const uint32_t f = (uint32_t)r;
+    const bool alpha = (r & kRecAlpha) != 0;
+    cnt[g] = alpha ? f : 0u;
+    ecnt[g] = (alpha && ((r & kRecWhole) || gmix[Hd[g]])) ? f : 0u;
+}
+
+// Direct fill of the single-character groups, one wavefront per 64 consecutive sorted
+// positions: their runs are consecutive in the output, so the lanes sweep the wave's span
+// together (coalesced byte stores) and look their run up in LDS.  Also writes delta = slot
+// offset minus entry offset for every position.
+__global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd,
+                                                    const uint8_t* __restrict__ gmix, const uint32_t* __restrict__ goff,
+                                                    const uint32_t* __restrict__ escan, uint32_t L, uint64_t n,
+                                                    uint8_t* __restrict__ O, uint32_t* __restrict__ delta)
+{
+    __shared__ uint32_t rs[kBlock], rl[kBlock];
+    __shared__ uint8_t rc[kBlock];
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63, wb = threadIdx.x & ~63;
+    uint32_t st = 0, ln = 0;
+    uint8_t ch = 0;
+    if (g < L) {
+        const uint64_t r = R[g];
+        st = goff[g];
+        delta[g] = st - escan[g];
+        if ((r & kRecAlpha) && !(r & kRecWhole) && !gmix[Hd[g]]) {
+            ln = (uint32_t)r;
+            ch = (uint8_t)(r >> 32);
+        }
+    }
+    rs[threadIdx.x] = st;
+    rl[threadIdx.x] = ln;
+    rc[threadIdx.x] = ch;
+    __syncthreads();
+    // span of this wave: from its first run start to the end of its last run
+    const uint32_t g0 = blockIdx.x * blockDim.x + wb;
+    const uint32_t nv = (g0 >= L) ? 0u : ((L - g0) < 64u ? (L - g0) : 64u);
+    if (nv == 0) return;
+    const uint32_t lo = rs[wb];
+    const uint32_t hi = rs[wb + nv - 1] + rl[wb + nv - 1];
+    for (uint32_t p = lo + lane; p < hi; p += 64) {
+        // last run start <= p (runs are ordered by start)
+        int a = 0, b = (int)nv - 1;
+        while (a < b) {
+            const int m = (a + b + 1) >> 1;
+            if (rs[wb + m] <= p) a = m;
+            else b = m - 1;
+        }
+        if (p - rs[wb + a] < rl[wb + a] && p < n) O[p] = rc[wb + a];
+    }
+}
+
+// mixed-group members in dictionary order (grouped by phrase)
+__global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+                          const uint8_t* __restrict__ gmix, uint32_t L, uint32_t* __restrict__ mxf)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x > v.L) return;
-    if (x == v.L) {
+    if (x > L) return;
+    if (x == L) {
         mxf[x] = 0;
         return;
     }
-    const uint32_t u = v.owner[x];
-    const uint32_t j = x - (uint32_t)v.off[u];
-    const uint32_t rem = v.len[u] - j;
-    mxf[x] = (rem > (uint32_t)v.W && j >= 1 && gmix[v.rank[x] - 1]) ? 1u : 0u;
+    const uint64_t r = rec[x];
+    mxf[x] = ((r & kRecAlpha) && !(r & kRecWhole) && gmix[rank[x] - 1]) ? 1u : 0u;
 }
 
-__global__ void k_mx_write(DictView v, const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
+__global__ void k_mx_write(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+                           const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs, uint32_t L,
                            uint64_t* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= v.L || !mxf[x]) return;
-    const uint32_t u = v.owner[x];
-    const uint32_t j = x - (uint32_t)v.off[u];
-    mx[mxs[x]] = ((uint64_t)(v.rank[x] - 1) << 9) | dv_prev(v, x, u, j);
+    if (x >= L || !mxf[x]) return;
+    mx[mxs[x]] = ((uint64_t)(rank[x] - 1) << 9) | ((rec[x] >> 32) & 0xFF);
+}
+
+// per distinct phrase: first mixed entry, number of mixed entries, whole-phrase group
+struct PhraseRec {
+    uint32_t mx_begin, mx_count, alpha0, pad;
+};
+
+__global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                             const uint32_t* __restrict__ mxs, const uint32_t* __restrict__ rank, uint32_t du,
+                             PhraseRec* __restrict__ pr)
+{
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= du) return;
+    const uint32_t o = (uint32_t)off[u];
+    const uint32_t a = mxs[o], b = mxs[o + len[u]];
+    pr[u] = PhraseRec{a, b - a, rank[o] - 1, 0};
 }
 
 // entries per parse-suffix rank r: the occurrence k = psa[r]-1 contributes its whole-phrase
 // entry and one entry per mixed suffix of its phrase
 __global__ void k_q_count(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
-                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                          const uint32_t* __restrict__ mxs, uint32_t* __restrict__ qc)
+                          const PhraseRec* __restrict__ pr, uint32_t* __restrict__ qc)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= d) return;
     const uint32_t k1 = psa[r];
-    if (k1 == 0) {
-        qc[r] = 0;
-        return;
-    }
-    const uint32_t u = pid[k1 - 1];
-    const uint32_t o = (uint32_t)off[u];
-    qc[r] = 1 + mxs[o + len[u]] - mxs[o];
+    qc[r] = (k1 == 0) ? 0u : 1u + pr[pid[k1 - 1]].mx_count;
 }
 
 __device__ __forceinline__ uint64_t whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
@@ -522,10 +792,8 @@ __device__ __forceinline__ uint64_t whole_phrase_entry(const uint32_t* __restric
 }
 
 __global__ void k_q_emit(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
-                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                         const uint32_t* __restrict__ mxs, const uint64_t* __restrict__ mx,
-                         const uint32_t* __restrict__ rank, const uint32_t* __restrict__ E,
-                         const uint8_t* __restrict__ T, uint64_t n, int W,
+                         const PhraseRec* __restrict__ pr, const uint64_t* __restrict__ mx,
+                         const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
                          const uint32_t* __restrict__ qo, uint64_t* __restrict__ ent)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -533,33 +801,31 @@ __global__ void k_q_emit(const uint32_t* __restrict__ psa, uint32_t d, const uin
     const uint32_t k1 = psa[r];
     if (k1 == 0) return;
     const uint32_t k = k1 - 1;
-    const uint32_t u = pid[k];
-    const uint32_t o = (uint32_t)off[u];
-    uint32_t q = qo[r];
-    ent[q++] = whole_phrase_entry(E, T, n, W, k, rank[o] - 1);
-    const uint32_t a = mxs[o], b = mxs[o + len[u]];
-    for (uint32_t i = a; i < b; i++) ent[q++] = mx[i];
+    const PhraseRec q = pr[pid[k]];
+    uint32_t o = qo[r];
+    ent[o++] = whole_phrase_entry(E, T, n, W, k, q.alpha0);
+    for (uint32_t i = 0; i < q.mx_count; i++) ent[o++] = mx[q.mx_begin + i];
 }
 
 // the last occurrence has no next parse suffix; its whole-phrase group is a singleton
-__global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const uint64_t* __restrict__ off,
-                         const uint32_t* __restrict__ rank, const uint32_t* __restrict__ E,
-                         const uint8_t* __restrict__ T, uint64_t n, int W, uint32_t pos,
-                         uint64_t* __restrict__ ent)
+__global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
+                         const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
+                         uint32_t pos, uint64_t* __restrict__ ent)
 {
     const uint32_t k = d - 1;
-    const uint32_t u = pid[k];
-    ent[pos] = whole_phrase_entry(E, T, n, W, k, rank[(uint32_t)off[u]] - 1);
+    ent[pos] = whole_phrase_entry(E, T, n, W, k, pr[pid[k]].alpha0);
 }
 
 // last radix pass of the entry sort: slot = sorted position + delta(alpha)
 struct EmitBwt {
     static constexpr double bytes_per_item = 1 + 4;  // BWT byte written, delta read
+    static constexpr bool kValues = false;
     const uint32_t* delta;
     uint8_t* O;
     uint32_t* pidx;
     uint64_t n;
-    __device__ __forceinline__ void operator()(uint64_t pos, uint64_t key, uint32_t) const
+    __device__ __forceinline__ void val(uint64_t, uint32_t) const {}
+    __device__ __forceinline__ void key(uint64_t pos, uint64_t key) const
     {
         const uint64_t slot = pos + delta[(uint32_t)(key >> 9)];
         if (slot >= n) return;  // guard: a slot past the output is a bug caught by the checks
@@ -577,6 +843,88 @@ __global__ void k_u32_to_u64(const uint32_t* __restrict__ a, uint64_t* __restric
 
 // --------------------------------------------------------------------- driver
 
+// Trigger end positions of the staged text into B_TRIG (room for one more entry, the last
+// phrase end); returns their number.  With `codes` the dense alphabet codes are computed
+// in the same round trip (codes[256] = sigma).
+uint32_t pfp_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, uint32_t extra, uint32_t** E_out,
+                  double* bytes, uint32_t* codes, uint32_t* sigma)
+{
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    uint32_t t = 0;
+    uint32_t* E = nullptr;
+    double by = 0;
+    if (W <= kScanMaxW) {
+        // single pass; E is sized for the usual trigger density and the scan is repeated
+        // with the exact size when a text has more triggers
+        uint32_t cap = (uint32_t)std::min<uint64_t>(n, n / 8 + 65536);
+        for (int pass = 0;; pass++) {
+            E = S.buf<uint32_t>(B_TRIG, (size_t)cap + extra);
+            const int kt = S.kt_begin(2, (double)n);
+            trig_scan(S, T, n, W, P, E, cap, sc + SC_TRIGGERS);
+            S.kt_end(kt);
+            if (pass == 0 && codes) {
+                alphabet_codes(S, T, n, codes);
+                MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice,
+                                         S.stream));
+            }
+            uint32_t hv[SC_COUNT_];
+            S.read(hv, sc, SC_COUNT_);
+            t = hv[SC_TRIGGERS];
+            if (codes) *sigma = hv[SC_SIGMA];
+            if (kt >= 0) S.kt_bytes[kt] = (double)n + 4.0 * std::min(t, cap);
+            by += (double)n + 4.0 * std::min(t, cap);
+            if (t <= cap) break;
+            MIBWT_REQUIRE(pass == 0, LIBCUBWT_GPU_ERROR, "trigger count changed between scans");
+            cap = t;
+        }
+    } else {
+        // windows wider than the exact-value range: two passes over per-thread masks
+        const uint32_t tiles = ceil_div(n + 1, kTrigTile);
+        uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
+        uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
+        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, P, masks, tcnt);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, tcnt, tcnt, tiles, sc + SC_TRIGGERS);
+        if (codes) {
+            alphabet_codes(S, T, n, codes);
+            MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+        }
+        uint32_t hv[SC_COUNT_];
+        S.read(hv, sc, SC_COUNT_);
+        t = hv[SC_TRIGGERS];
+        if (codes) *sigma = hv[SC_SIGMA];
+        E = S.buf<uint32_t>(B_TRIG, (size_t)t + extra);
+        k_trig_emit<<<tiles, kBlock, 0, S.stream>>>(masks, tcnt, E);
+        MIBWT_LAUNCH_CHECK();
+        by = (double)n + 2.0 * tiles * kBlock * 2 + 4.0 * t;
+    }
+    *E_out = E;
+    *bytes = by;
+    return t;
+}
+
+// Trigger end positions of a host text (the reference's Compute_D1/D2 output, PFPk:443-571):
+// returns their number; E receives the first min(t, cap) of them.
+int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n64, int W, int P, uint32_t* Eh, int64_t cap)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    MIBWT_REQUIRE(W >= 1 && W <= kMaxW && P >= 1 && P <= 65535, LIBCUBWT_BAD_PARAMETER, "bad W/P");
+    const uint64_t n = (uint64_t)n64;
+    S.phases_reset();
+    uint8_t* T = stage_text(S, src, true, n);
+    uint32_t* E = nullptr;
+    double bytes = 0;
+    const uint32_t t = pfp_scan(S, T, n, W, (uint32_t)P, 1, &E, &bytes, nullptr, nullptr);
+    const int64_t m = std::min<int64_t>((int64_t)t, cap);
+    if (m > 0) {
+        MIBWT_HIP(hipMemcpyAsync(Eh, E, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, S.stream));
+        MIBWT_HIP(hipStreamSynchronize(S.stream));
+    }
+    S.stats.triggers = t;
+    return (int64_t)t;
+}
+
+
 int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P)
 {
     MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
@@ -591,7 +939,6 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_HIP(hipEventCreate(&S.t_end));
     }
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    const uint32_t tiles = ceil_div(n + 1, kTrigTile);
     // the text: a padded private copy (src == nullptr: already staged by the caller); a copy
     // from device memory is part of the timed build, an upload from the host is not
     uint8_t* T;
@@ -608,31 +955,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
 
     // ---- 1. trigger scan
     int ph = S.phase_begin("scan");
-    uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
-    uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
-    {
-        const int kt = S.kt_begin(2, (double)n + 2.0 * tiles * kBlock + 4.0 * tiles);
-        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, (uint32_t)P, masks, tcnt);
-        S.kt_end(kt);
-    }
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, tcnt, tcnt, tiles, sc + SC_TRIGGERS);
     uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
-    alphabet_codes(S, T, n, codes);
-    MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
-    uint32_t t = 0, sigma = 0;
-    {
-        uint32_t hv[SC_COUNT_];
-        S.read(hv, sc, SC_COUNT_);
-        t = hv[SC_TRIGGERS];
-        sigma = hv[SC_SIGMA];
-    }
+    uint32_t sigma = 0;
+    double scan_bytes = 0;
+    uint32_t* E = nullptr;
+    const uint32_t t = pfp_scan(S, T, n, W, (uint32_t)P, 1, &E, &scan_bytes, nullptr, nullptr);
     const uint32_t d = t + 1;
-    uint32_t* E = S.buf<uint32_t>(B_TRIG, d);
-    k_trig_emit<<<tiles, kBlock, 0, S.stream>>>(masks, tcnt, E);
-    MIBWT_LAUNCH_CHECK();
     fill<uint32_t>(S, E + t, 1, (uint32_t)(n + W - 1));
-    S.phase_end(ph, (double)n + 2.0 * tiles * kBlock * 2 + 4.0 * d);
+    S.phase_end(ph, scan_bytes + 4.0);
     S.stats.triggers = t;
     S.stats.phrases = d;
 
@@ -649,7 +979,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
         k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, K0, V0);
         MIBWT_LAUNCH_CHECK();
-        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, 64);
+        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, kDictKeyBits);
         fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
         k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, K0, V0, d, flag, sc + SC_COLLISION);
         MIBWT_LAUNCH_CHECK();
@@ -675,11 +1005,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     k_u32_to_u64<<<grid_for(du + 1), kBlock, 0, S.stream>>>(dlen, dlen64, du);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint64_t>(S, dlen64, doff, du + 1);
+    // alphabet of the dictionary = alphabet of the text (the phrases cover it)
+    alphabet_codes(S, T, n, codes, dstart, dlen, du);
     uint64_t L64 = 0;
     S.read(&L64, doff + du);
+    S.read(&sigma, codes + 256);
     MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "dictionary too large");
     const uint32_t L = (uint32_t)L64;
-    S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 24.0 * d * 8 + 12.0 * d);
+    S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 8.0 * d + 24.0 * d * (kDictKeyBits / 8) + 12.0 * d);
     S.stats.distinct_phrases = du;
     S.stats.dict_length = L;
 
@@ -734,38 +1067,45 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
 
     // ---- 6. BWT assembly
     ph = S.phase_begin("assemble");
-    DictView v{dsa, drank, owner, doff, dlen, dstart, freq, T, L, W, n};
-    uint32_t* goff = S.buf<uint32_t>(B_G_OFF, L);
+    uint64_t* rec = S.buf<uint64_t>(B_AS_REC, L);
+    uint64_t* R = S.buf<uint64_t>(B_AS_R, L);
+    uint32_t* Hd = S.buf<uint32_t>(B_AS_HD, L);
+    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, L, W, rec);
+    MIBWT_LAUNCH_CHECK();
+    k_asm_gather<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, drank, rec, L, R, Hd);
+    MIBWT_LAUNCH_CHECK();
     uint8_t* gmix = S.buf<uint8_t>(B_G_MIX, L);
-    MIBWT_HIP(hipMemsetAsync(gmix, 0, L, S.stream));
-    k_asm_count<<<grid_for(L), kBlock, 0, S.stream>>>(v, goff, gmix);
+    fill<uint8_t>(S, gmix, L, (uint8_t)0);
+    k_asm_mix<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, L, gmix);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* goff = S.buf<uint32_t>(B_G_OFF, L);
+    uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
+    k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, L, goff, ecnt);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
+    exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t slots = 0;
     S.read(&slots, sc + SC_TOTAL);
     MIBWT_REQUIRE(slots == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
-    uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
-    k_asm_ecnt<<<grid_for(L), kBlock, 0, S.stream>>>(v, gmix, ecnt);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t* delta = S.buf<uint32_t>(B_G_DELTA, L);
-    k_asm_delta<<<grid_for(L), kBlock, 0, S.stream>>>(goff, ecnt, L, delta);
-    MIBWT_LAUNCH_CHECK();
-    k_asm_fill<<<grid_for(L), kBlock, 0, S.stream>>>(v, gmix, goff, dO);
+    k_asm_fill<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, goff, ecnt, L, n, dO, delta);
     MIBWT_LAUNCH_CHECK();
     uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(v, gmix, mxf);
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, drank, gmix, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
     S.read(&nmx, sc + SC_ENTRIES);
     uint64_t* mx = S.buf<uint64_t>(B_MX_ENT, nmx + 1);
-    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(v, mxf, mxs, mx);
+    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(rec, drank, mxf, mxs, L, mx);
+    MIBWT_LAUNCH_CHECK();
+    PhraseRec* prec = S.buf<PhraseRec>(B_AS_PREC, du);
+    k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, drank, du, prec);
     MIBWT_LAUNCH_CHECK();
     uint32_t* qc = S.buf<uint32_t>(B_Q_CNT, d);
     uint32_t* qo = S.buf<uint32_t>(B_Q_OFF, d);
-    k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, doff, dlen, mxs, qc);
+    k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, qc);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
     uint32_t hv[SC_COUNT_];
@@ -773,10 +1113,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     const uint32_t ne = hv[SC_ENTRIES] + 1;
     uint64_t* ent0 = S.buf<uint64_t>(B_SORT_K0, ne);
     uint64_t* ent1 = S.buf<uint64_t>(B_SORT_K1, ne);
-    k_q_emit<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, doff, dlen, mxs, mx, drank, E, T, n, W, qo,
-                                                   ent0);
+    k_q_emit<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, ent0);
     MIBWT_LAUNCH_CHECK();
-    k_q_last<<<1, 1, 0, S.stream>>>(pid, d, doff, drank, E, T, n, W, ne - 1, ent0);
+    k_q_last<<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
     MIBWT_LAUNCH_CHECK();
     fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
     EmitBwt em{delta, dO, sc + SC_PIDX, n};
@@ -830,28 +1169,31 @@ double time_copy(Storage& S, uint64_t bytes, int reps)
     return ms / reps;
 }
 
-// Re-run the trigger scan kernel (pass 1) `reps` times on the text of the last call.
+// Re-run the trigger scan kernel `reps` times on the text of the last call.
 double pfp_time_scan(Storage& S, int reps, double* bytes)
 {
     const uint64_t n = (uint64_t)S.last_n;
-    const uint32_t tiles = ceil_div(n + 1, kTrigTile);
+    MIBWT_REQUIRE(S.last_W >= 1 && S.last_W <= kScanMaxW, LIBCUBWT_BAD_PARAMETER, "no single-pass scan for this W");
     uint8_t* T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
-    uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
-    uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    const uint32_t cap = (uint32_t)S.stats.triggers + 1;
+    uint32_t* E = S.buf<uint32_t>(B_TRIG, cap);
     hipEvent_t a, b;
     MIBWT_HIP(hipEventCreate(&a));
     MIBWT_HIP(hipEventCreate(&b));
-    k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, S.last_W, (uint32_t)S.last_P, masks, tcnt);
+    trig_scan(S, T, n, S.last_W, (uint32_t)S.last_P, E, cap, sc + SC_TRIGGERS);
     MIBWT_HIP(hipEventRecord(a, S.stream));
-    for (int i = 0; i < reps; i++)
-        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, S.last_W, (uint32_t)S.last_P, masks, tcnt);
+    for (int i = 0; i < reps; i++) trig_scan(S, T, n, S.last_W, (uint32_t)S.last_P, E, cap, sc + SC_TRIGGERS);
     MIBWT_HIP(hipEventRecord(b, S.stream));
     MIBWT_HIP(hipEventSynchronize(b));
     float ms = 0;
     MIBWT_HIP(hipEventElapsedTime(&ms, a, b));
     MIBWT_HIP(hipEventDestroy(a));
     MIBWT_HIP(hipEventDestroy(b));
-    *bytes = (double)n + 2.0 * tiles * kBlock + 4.0 * tiles;
+    uint32_t t = 0;
+    S.read(&t, sc + SC_TRIGGERS);
+    MIBWT_REQUIRE((int64_t)t == S.stats.triggers, LIBCUBWT_GPU_ERROR, "trigger count differs between runs");
+    *bytes = (double)n + 4.0 * t;
     return ms / reps;
 }
 

@@ -4,6 +4,8 @@
 // run leaves the tile as one coalesced write.
 #pragma once
 
+#include <type_traits>
+
 #include "common.hpp"
 
 namespace mibwt {
@@ -69,6 +71,85 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T identity, Op op, T* lds
     return op(wave_prefix, ex);
 }
 
+// ----------------------------------------------------------------------------- chained scan
+
+// Ticket of this tile in launch order (all threads of the block call it).
+__device__ __forceinline__ uint32_t chain_ticket(const Chain& c, uint32_t tiles, uint32_t* lds)
+{
+    if (threadIdx.x == 0) {
+        const uint32_t t = atomicAdd(c.ticket, 1u);
+        // every other tile of this launch has drawn its ticket: reset for the next launch
+        if (t == tiles - 1) __hip_atomic_store(c.ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        *lds = t;
+    }
+    __syncthreads();
+    const uint32_t t = *lds;
+    __syncthreads();
+    return t;
+}
+
+__device__ __forceinline__ void chain_publish(const Chain& c, uint32_t tile, uint32_t state, uint32_t v)
+{
+    const uint64_t w = ((uint64_t)c.epoch << 34) | ((uint64_t)state << 32) | v;
+    __hip_atomic_store(c.status + tile, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Exclusive prefix of `tile` from its predecessors, read 64 tiles at a time by one wavefront
+// (all 64 lanes call it).  Tiles with earlier tickets are running or done, so each wait ends.
+template <typename Op>
+__device__ uint32_t chain_lookback(const Chain& c, uint32_t tile, Op op, uint32_t identity)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t excl = identity;
+    int64_t j = (int64_t)tile - 1;
+    for (;;) {
+        const int64_t k = j - lane;
+        uint32_t state = 2, v = identity;
+        if (k >= 0) {
+            uint64_t w = __hip_atomic_load(c.status + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while ((uint32_t)(w >> 34) != c.epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                w = __hip_atomic_load(c.status + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            state = (uint32_t)(w >> 32) & 3u;
+            v = (uint32_t)w;
+        }
+        const uint64_t incl = __ballot(state == 2);
+        // nearest inclusive prefix: lanes 0..stop contribute
+        const int stop = incl ? (__ffsll((unsigned long long)incl) - 1) : 63;
+        uint32_t x = (lane <= stop) ? v : identity;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) x = op(x, (uint32_t)__shfl_xor(x, o, 64));
+        excl = op(excl, x);
+        if (incl) break;
+        j -= 64;
+    }
+    return excl;
+}
+
+// Exclusive prefix of this tile given its total; every thread of the block calls it and gets
+// the prefix.  `lds` is one word of shared memory.
+template <typename Op>
+__device__ __forceinline__ uint32_t chain_prefix(const Chain& c, uint32_t tile, uint32_t total, Op op,
+                                                 uint32_t identity, uint32_t* lds)
+{
+    if (threadIdx.x < 64) {
+        uint32_t excl = identity;
+        if (tile == 0) {
+            if (threadIdx.x == 0) chain_publish(c, 0, 2, total);
+        } else {
+            if (threadIdx.x == 0) chain_publish(c, tile, 1, total);
+            excl = chain_lookback(c, tile, op, identity);
+            if (threadIdx.x == 0) chain_publish(c, tile, 2, op(excl, total));
+        }
+        if (threadIdx.x == 0) *lds = excl;
+    }
+    __syncthreads();
+    const uint32_t r = *lds;
+    __syncthreads();
+    return r;
+}
+
 // ----------------------------------------------------------------------------- device scan
 
 constexpr int kScanItems = 8;
@@ -122,6 +203,64 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, 
     }
 }
 
+// Single-pass scan of 32-bit values: each tile scans its items, takes its prefix from its
+// predecessors by decoupled look-back and writes its outputs, one read and one write per item.
+// `out` may alias `in` (every thread reads its items before writing them).
+// Chained tiles are large (kChainItems per thread, 16384 per tile): every tile draws one
+// ticket from a single device-wide counter, and such atomics serialize at roughly 11 ns each
+// on MI355X, so a tile must carry far more than 11 ns of memory traffic.
+constexpr int kChainItems = 64;
+constexpr int kChainTile = kBlock * kChainItems;
+
+template <typename Op, bool kVec>
+__global__ __launch_bounds__(kBlock) void k_scan_chain(const uint32_t* in, uint32_t* out, uint64_t n,
+                                                      uint32_t identity, Op op, int inclusive,
+                                                      uint32_t* __restrict__ total_out, Chain c, uint32_t tiles)
+{
+    __shared__ uint32_t slot;
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t tile = chain_ticket(c, tiles, &slot);
+    const uint64_t base = (uint64_t)tile * kChainTile + (uint64_t)threadIdx.x * kChainItems;
+    uint32_t v[kChainItems];
+    const bool full = base + kChainItems <= n;
+    if (kVec && full) {
+#pragma unroll
+        for (int q = 0; q < kChainItems / 4; q++) {
+            const uint4 a = *reinterpret_cast<const uint4*>(in + base + 4 * q);
+            v[4 * q] = a.x;
+            v[4 * q + 1] = a.y;
+            v[4 * q + 2] = a.z;
+            v[4 * q + 3] = a.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kChainItems; i++) v[i] = (base + i < n) ? in[base + i] : identity;
+    }
+    uint32_t acc = identity;
+#pragma unroll
+    for (int i = 0; i < kChainItems; i++) acc = op(acc, v[i]);
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(acc, identity, op, lds, &tot);
+    const uint32_t pre = chain_prefix(c, tile, tot, op, identity, &slot);
+    run = op(pre, run);
+#pragma unroll
+    for (int i = 0; i < kChainItems; i++) {
+        const uint32_t nxt = op(run, v[i]);
+        v[i] = inclusive ? nxt : run;
+        run = nxt;
+    }
+    if (kVec && full) {
+#pragma unroll
+        for (int q = 0; q < kChainItems / 4; q++)
+            *reinterpret_cast<uint4*>(out + base + 4 * q) = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < kChainItems; i++)
+            if (base + i < n) out[base + i] = v[i];
+    }
+    if (total_out && tile == tiles - 1 && threadIdx.x == 0) *total_out = op(pre, tot);
+}
+
 // Device-wide scan.  `out` may alias `in`.  If total_dev != nullptr the reduction of all
 // n items is written there.  Scratch comes from buffers tmp_id, tmp_id+1, ... (recursion).
 template <typename T, typename Op>
@@ -136,6 +275,22 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
     if (tiles == 1) {
         k_scan_down<T, Op><<<1, kBlock, 0, S.stream>>>(in, out, nullptr, n, identity, op,
                                                         inclusive ? 1 : 0, total_dev);
+        MIBWT_LAUNCH_CHECK();
+        return;
+    }
+    if constexpr (std::is_same<T, uint32_t>::value) {
+        tiles = ceil_div(n, (uint64_t)kChainTile);
+        const Chain c = S.chain(tiles);
+        const uint32_t* in32 = reinterpret_cast<const uint32_t*>(in);
+        uint32_t* out32 = reinterpret_cast<uint32_t*>(out);
+        uint32_t* tot32 = reinterpret_cast<uint32_t*>(total_dev);
+        const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+        if (vec)
+            k_scan_chain<Op, true><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, (uint32_t)identity, op,
+                                                                   inclusive ? 1 : 0, tot32, c, tiles);
+        else
+            k_scan_chain<Op, false><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, (uint32_t)identity, op,
+                                                                    inclusive ? 1 : 0, tot32, c, tiles);
         MIBWT_LAUNCH_CHECK();
         return;
     }
@@ -167,12 +322,21 @@ void inclusive_max(Storage& S, const T* in, T* out, uint64_t n)
 }
 
 // ----------------------------------------------------------------------------- radix sort
+//
+// Stable LSD radix sort, 8-bit digits, one kernel per digit pass ("onesweep" shape):
+//   - one histogram kernel reads the keys once and counts the digits of every pass;
+//   - each pass kernel ranks its 4096-item tile with wavefront multi-split (8 ballots per
+//     item), publishes its per-digit counts, obtains the counts of all earlier tiles by
+//     decoupled look-back (one thread per digit), and scatters keys, then values, through
+//     LDS so that every digit run leaves the tile as one contiguous write.
 
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortItems = 16;                    // items per thread
 constexpr int kSortTile = kBlock * kSortItems;    // 4096 items per tile
 constexpr int kWaveTile = 64 * kSortItems;        // 1024 items per wavefront
+constexpr int kMaxPasses = 8;
+constexpr int kHistItems = 16;                    // consecutive keys per thread in the histogram
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
@@ -180,61 +344,97 @@ __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
     return (uint32_t)(key >> shift) & mask;
 }
 
+// Digit counts of all passes: hist[p * kRadix + d] (zeroed by the caller).  Each thread walks
+// kHistItems consecutive keys and adds runs of equal digits at once, which keeps the LDS
+// atomics cheap on presorted high digits.
 template <typename K>
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const K* __restrict__ keys, uint64_t n, int shift,
-                                                      uint32_t mask, uint32_t* __restrict__ hist,
-                                                      uint32_t tiles)
+__global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__ keys, uint64_t n, int lo, int hi,
+                                                          int passes, uint32_t* __restrict__ hist)
 {
-    __shared__ uint32_t cnt[kWaves][kRadix];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&cnt[0][0])[i] = 0;
+    __shared__ uint32_t h[kMaxPasses * kRadix];
+    for (int i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0;
     __syncthreads();
-    const uint64_t base = (uint64_t)blockIdx.x * kSortTile + (uint64_t)w * kWaveTile + lane;
+    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kHistItems;
+    K k[kHistItems];
+    int cnt_valid = 0;
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
-        uint64_t idx = base + (uint64_t)i * 64;
-        if (idx < n) atomicAdd(&cnt[w][digit_of(keys[idx], shift, mask)], 1u);
+    for (int i = 0; i < kHistItems; i++) {
+        k[i] = (base + i < n) ? keys[base + i] : K(0);
+        cnt_valid += (base + i < n) ? 1 : 0;
+    }
+    for (int p = 0; p < passes; p++) {
+        const int shift = lo + p * kRadixBits;
+        const int nb = (hi - shift) < kRadixBits ? (hi - shift) : kRadixBits;
+        const uint32_t mask = (1u << nb) - 1u;
+        uint32_t prev = 0xFFFFFFFFu, run = 0;
+#pragma unroll
+        for (int i = 0; i < kHistItems; i++) {
+            if (i < cnt_valid) {
+                const uint32_t d = digit_of(k[i], shift, mask);
+                if (d == prev) {
+                    run++;
+                } else {
+                    if (run) atomicAdd(&h[p * kRadix + prev], run);
+                    prev = d;
+                    run = 1;
+                }
+            }
+        }
+        if (run) atomicAdd(&h[p * kRadix + prev], run);
     }
     __syncthreads();
-    for (int d = threadIdx.x; d < kRadix; d += kBlock) {
-        uint32_t s = 0;
-#pragma unroll
-        for (int q = 0; q < kWaves; q++) s += cnt[q][d];
-        hist[(uint64_t)d * tiles + blockIdx.x] = s;
-    }
+    for (int i = threadIdx.x; i < passes * kRadix; i += kBlock)
+        if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
 // Default emitter: writes the sorted key/value pair.
 template <typename K, typename V>
 struct EmitPair {
     static constexpr double bytes_per_item = sizeof(K) + sizeof(V);
+    static constexpr bool kValues = true;
     K* keys_out;
     V* vals_out;
-    __device__ __forceinline__ void operator()(uint64_t pos, K key, V val) const
-    {
-        keys_out[pos] = key;
-        if (vals_out) vals_out[pos] = val;
-    }
+    __device__ __forceinline__ void key(uint64_t pos, K k) const { keys_out[pos] = k; }
+    __device__ __forceinline__ void val(uint64_t pos, V v) const { vals_out[pos] = v; }
 };
 
-template <typename K, typename V, typename Emit>
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ keys, const V* __restrict__ vals,
-                                                         uint64_t n, int shift, uint32_t mask,
-                                                         const uint32_t* __restrict__ hist_scanned,
-                                                         uint32_t tiles, Emit emit)
+// Exclusive prefix of digit d over the tiles before `tile` (one thread per digit).
+__device__ __forceinline__ uint32_t chain_lookback_digit(const Chain& c, uint32_t tile, uint32_t d)
 {
+    uint32_t excl = 0;
+    for (int64_t j = (int64_t)tile - 1; j >= 0; j--) {
+        const uint64_t* st = c.status + (uint64_t)j * kRadix + d;
+        uint64_t w = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while ((uint32_t)(w >> 34) != c.epoch) {
+            __builtin_amdgcn_s_sleep(1);
+            w = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        excl += (uint32_t)w;
+        if (((w >> 32) & 3u) == 2u) break;
+    }
+    return excl;
+}
+
+template <typename K, typename V, typename Emit>
+__global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ keys, const V* __restrict__ vals,
+                                                      uint64_t n, int shift, uint32_t mask,
+                                                      const uint32_t* __restrict__ hist_pass, uint32_t tiles,
+                                                      Chain c, Emit emit)
+{
+    static_assert(sizeof(V) <= sizeof(K), "values are staged in the key buffer");
     __shared__ uint32_t wcnt[kWaves][kRadix];
     __shared__ uint32_t dstart[kRadix];
     __shared__ uint32_t dglob[kRadix];
     __shared__ uint32_t scan_lds[kWaves];
-    __shared__ K skeys[kSortTile];
-    __shared__ V svals[kSortTile];
+    __shared__ uint32_t slot;
+    __shared__ K stage[kSortTile];
+    __shared__ uint8_t sdig[kSortTile];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
-    __syncthreads();
+    const uint32_t tile = chain_ticket(c, tiles, &slot);  // also orders the zeroing above
 
-    const uint64_t tile_base = (uint64_t)blockIdx.x * kSortTile;
+    const uint64_t tile_base = (uint64_t)tile * kSortTile;
     const uint64_t base = tile_base + (uint64_t)w * kWaveTile + lane;
     K k[kSortItems];
     V v[kSortItems];
@@ -245,7 +445,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
         const uint64_t idx = base + (uint64_t)i * 64;
         const bool valid = idx < n;
         k[i] = valid ? keys[idx] : K(0);
-        v[i] = (valid && vals) ? vals[idx] : V(0);
+        if (Emit::kValues) v[i] = (valid && vals) ? vals[idx] : V(0);
         const uint32_t d = digit_of(k[i], shift, mask);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -261,50 +461,78 @@ __global__ __launch_bounds__(kBlock) void k_radix_scatter(const K* __restrict__ 
         rank[i] = valid ? (uint16_t)(old + (uint32_t)__popcll(peers & lt)) : (uint16_t)0;
     }
     __syncthreads();
-    // per digit: wave prefixes, tile count, tile-local start, global start
     {
         const int d = threadIdx.x;  // kBlock == kRadix
         uint32_t run = 0;
 #pragma unroll
         for (int q = 0; q < kWaves; q++) {
-            uint32_t c = wcnt[q][d];
+            const uint32_t cq = wcnt[q][d];
             wcnt[q][d] = run;
-            run += c;
+            run += cq;
         }
         uint32_t tot;
-        uint32_t st = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
-        dstart[d] = st;
-        dglob[d] = hist_scanned[(uint64_t)d * tiles + blockIdx.x];
+        dstart[d] = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
+        const uint32_t gstart = block_exclusive_scan(hist_pass[d], 0u, OpSum{}, scan_lds, &tot);
+        uint32_t excl = 0;
+        if (tile == 0) {
+            chain_publish(c, d, 2, run);
+        } else {
+            chain_publish(c, tile * kRadix + d, 1, run);
+            excl = chain_lookback_digit(c, tile, d);
+            chain_publish(c, tile * kRadix + d, 2, excl + run);
+        }
+        dglob[d] = gstart + excl;
     }
     __syncthreads();
+    // rank[i] becomes the item's slot in the tile
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
         const uint64_t idx = base + (uint64_t)i * 64;
         if (idx < n) {
             const uint32_t d = digit_of(k[i], shift, mask);
             const uint32_t p = dstart[d] + wcnt[w][d] + rank[i];
-            skeys[p] = k[i];
-            svals[p] = v[i];
+            rank[i] = (uint16_t)p;
+            stage[p] = k[i];
+            sdig[p] = (uint8_t)d;
         }
     }
     __syncthreads();
     const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)kSortTile ? (n - tile_base) : kSortTile);
     for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) {
-        const K key = skeys[p];
-        const uint32_t d = digit_of(key, shift, mask);
-        emit((uint64_t)dglob[d] + (p - dstart[d]), key, svals[p]);
+        const uint32_t d = sdig[p];
+        emit.key((uint64_t)dglob[d] + (p - dstart[d]), stage[p]);
+    }
+    if (Emit::kValues && vals) {
+        __syncthreads();
+        V* sv = reinterpret_cast<V*>(stage);
+#pragma unroll
+        for (int i = 0; i < kSortItems; i++)
+            if (base + (uint64_t)i * 64 < n) sv[rank[i]] = v[i];
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) {
+            const uint32_t d = sdig[p];
+            emit.val((uint64_t)dglob[d] + (p - dstart[d]), sv[p]);
+        }
     }
 }
 
+template <typename T>
+__global__ void k_fill(T* p, uint64_t n, T v)
+{
+    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
 // Stable LSD radix sort of (key, value) pairs by key bits [lo, hi).  keys/vals hold the
-// input in buffer 0 and are sorted in place (ping-ponging through buffer 1); on return the
-// sorted data is in buffer 0 again.  vals0/vals1 may be nullptr (keys only).  If `last`
-// is given, the final pass calls it instead of writing buffer 0 (and buffer 0 is left
-// undefined).
+// input in buffer 0 and ping-pong through buffer 1.  Without `where` the sorted data ends in
+// buffer 0 (copied back after an odd number of passes); with `where` it stays where the last
+// pass wrote it and *where is set to 0 or 1.  vals0/vals1 may be nullptr (keys only).  If
+// `last` is given, the final pass calls it instead of writing a buffer.
 template <typename K, typename V, typename LastEmit = EmitPair<K, V>>
 void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, int lo, int hi,
-                const LastEmit* last = nullptr)
+                const LastEmit* last = nullptr, int* where = nullptr)
 {
+    if (where) *where = 0;
     if (n == 0 || hi <= lo) {
         if (last && n) {
             // nothing to sort: still route the data through the emitter with one trivial pass
@@ -316,8 +544,14 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
     }
     MIBWT_REQUIRE(n < (1ull << 32), LIBCUBWT_BAD_PARAMETER, "radix_sort: too many items");
     const uint32_t tiles = ceil_div(n, kSortTile);
-    uint32_t* hist = S.buf<uint32_t>(B_SORT_HIST, (size_t)kRadix * tiles);
-    int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
+    const int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
+    MIBWT_REQUIRE(passes <= kMaxPasses, LIBCUBWT_BAD_PARAMETER, "radix_sort: key too wide");
+    uint32_t* hist = S.buf<uint32_t>(B_SORT_HIST, (size_t)kMaxPasses * kRadix);
+    k_fill<uint32_t><<<ceil_div(kMaxPasses * kRadix, kBlock), kBlock, 0, S.stream>>>(hist, kMaxPasses * kRadix, 0u);
+    MIBWT_LAUNCH_CHECK();
+    k_radix_hist_all<K><<<ceil_div(n, (uint64_t)kBlock * kHistItems), kBlock, 0, S.stream>>>(keys0, n, lo, hi, passes,
+                                                                                           hist);
+    MIBWT_LAUNCH_CHECK();
     K* kin = keys0;
     K* kout = keys1;
     V* vin = vals0;
@@ -326,41 +560,37 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         const int shift = lo + p * kRadixBits;
         const int nb = (hi - shift) < kRadixBits ? (hi - shift) : kRadixBits;
         const uint32_t mask = (1u << nb) - 1u;
-        k_radix_hist<K><<<tiles, kBlock, 0, S.stream>>>(kin, n, shift, mask, hist, tiles);
-        MIBWT_LAUNCH_CHECK();
-        exclusive_sum<uint32_t>(S, hist, hist, (uint64_t)kRadix * tiles);
         const bool is_last = (p == passes - 1);
         const double vbytes = vin ? (double)sizeof(V) : 0.0;
+        const Chain c = S.chain(tiles * kRadix);
         if (is_last && last) {
             const int kt = S.kt_begin(1, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
-            k_radix_scatter<K, V, LastEmit><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                           hist, tiles, *last);
+            k_radix_pass<K, V, LastEmit><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, hist + p * kRadix,
+                                                                        tiles, c, *last);
             S.kt_end(kt);
         } else {
             const int kt = S.kt_begin(1, 2.0 * (double)n * (sizeof(K) + vbytes));
             EmitPair<K, V> e{kout, vout};
-            k_radix_scatter<K, V, EmitPair<K, V>><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift,
-                                                                                mask, hist, tiles, e);
+            k_radix_pass<K, V, EmitPair<K, V>><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+                                                                              hist + p * kRadix, tiles, c, e);
             S.kt_end(kt);
         }
         MIBWT_LAUNCH_CHECK();
         std::swap(kin, kout);
         std::swap(vin, vout);
     }
-    if (!last && kin != keys0) {
-        MIBWT_HIP(hipMemcpyAsync(keys0, kin, sizeof(K) * n, hipMemcpyDeviceToDevice, S.stream));
-        if (vals0 && vin) MIBWT_HIP(hipMemcpyAsync(vals0, vin, sizeof(V) * n, hipMemcpyDeviceToDevice, S.stream));
+    if (last) return;
+    if (kin != keys0) {
+        if (where) {
+            *where = 1;
+        } else {
+            MIBWT_HIP(hipMemcpyAsync(keys0, kin, sizeof(K) * n, hipMemcpyDeviceToDevice, S.stream));
+            if (vals0 && vin) MIBWT_HIP(hipMemcpyAsync(vals0, vin, sizeof(V) * n, hipMemcpyDeviceToDevice, S.stream));
+        }
     }
 }
 
 // ----------------------------------------------------------------------------- small kernels
-
-template <typename T>
-__global__ void k_fill(T* p, uint64_t n, T v)
-{
-    uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
 
 template <typename T>
 void fill(Storage& S, T* p, uint64_t n, T v)

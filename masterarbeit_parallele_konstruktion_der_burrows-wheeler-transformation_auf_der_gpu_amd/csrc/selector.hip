@@ -27,7 +27,12 @@ namespace mibwt {
 
 constexpr int kGridW[4] = {5, 10, 20, 35};
 constexpr int kGridP[10] = {11, 13, 17, 23, 31, 41, 53, 61, 71, 81};
-constexpr int kNumW = 4, kNumP = 10, kNumCand = kNumW * kNumP;
+constexpr int kNumW = 4, kNumP = 10, kNumGrid = kNumW * kNumP;
+// candidates: the grid, then the reference default W=5,P=16 (Benchmark_Skript_Linux.sh:70-71)
+constexpr int kNumCand = kNumGrid + 1;
+constexpr int kDefaultCand = kNumGrid;
+inline int cand_W(int c) { return c < kNumGrid ? kGridW[c / kNumP] : 5; }
+inline int cand_P(int c) { return c < kNumGrid ? kGridP[c % kNumP] : 16; }
 constexpr int kIn = 64, kHid = 128, kOut = 3;
 constexpr int kSampleChunks = 64, kChunk = 16384;  // 1 MiB sample
 
@@ -54,73 +59,180 @@ struct SampleCounts {
     uint32_t hist1[256];
     uint32_t trig[kNumCand];
     uint32_t positions;
+    uint32_t sketch[3];  // selected positions, distinct 16-mer and 64-mer fingerprints
 };
 
-// Sample statistics: byte histogram, order-1 pair counts (hist2, 65536 bins), and trigger
-// counts of every grid candidate (exact window values in 64-bit, mod P per candidate).
+// Divisibility by the odd grid moduli without division: v % P == 0 <=> v * P^-1 <= lim
+// (mod 2^32 for W <= 20, mod 2^64 for W = 35).
+struct GridDiv {
+    uint32_t inv32[kNumP], lim32[kNumP];
+    uint64_t inv64[kNumP], lim64[kNumP];
+};
+
+constexpr int kSampleParts = 4;                          // blocks per chunk
+constexpr int kSamplePer = kChunk / (kBlock * kSampleParts);  // 16 consecutive positions per thread
+
+// Sample statistics over kSampleChunks chunks of kChunk bytes (kSampleParts blocks each): byte and
+// byte-pair histograms (the pair histogram lives in LDS as packed 16-bit counters), and the
+// trigger count of every grid candidate from rolled exact window values.
 __global__ __launch_bounds__(kBlock) void k_feat_sample(const uint8_t* __restrict__ T, uint64_t n,
-                                                       uint64_t stride, uint32_t* __restrict__ hist2,
+                                                       uint64_t stride, GridDiv gd, uint32_t* __restrict__ hist2,
                                                        SampleCounts* __restrict__ out)
 {
+    __shared__ uint32_t h2[65536 / 2];
     __shared__ uint32_t h1[256];
     __shared__ uint32_t tc[kNumCand];
+    for (int i = threadIdx.x; i < 65536 / 2; i += kBlock) h2[i] = 0;
     h1[threadIdx.x] = 0;
     if (threadIdx.x < kNumCand) tc[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t chunk = blockIdx.x / (kChunk / (kBlock * 16));
-    const uint64_t part = blockIdx.x % (kChunk / (kBlock * 16));
-    const uint64_t cstart = chunk * stride;
-    uint32_t local_pos = 0;
-    for (int q = 0; q < 16; q++) {
-        const uint64_t p = cstart + part * kBlock * 16 + (uint64_t)threadIdx.x * 16 + q;
-        if (p >= n || p >= cstart + kChunk) break;
+    const uint64_t cstart = (uint64_t)(blockIdx.x / kSampleParts) * stride;
+    const uint64_t p0 = cstart + (uint64_t)(blockIdx.x % kSampleParts) * (kChunk / kSampleParts) +
+                        (uint64_t)threadIdx.x * kSamplePer;
+    uint32_t cnt[kNumCand];
+#pragma unroll
+    for (int c = 0; c < kNumCand; c++) cnt[c] = 0;
+    uint32_t v5 = 0, v10 = 0, v20 = 0;
+    uint64_t v35 = 0;
+    // windows ending before p0 (bytes before the text read as 0, windows there are skipped)
+    for (int k = 35; k >= 1; k--) {
+        const uint32_t c = (p0 >= (uint64_t)k && p0 - k < n) ? T[p0 - k] : 0u;
+        if (k <= 5) v5 = (v5 << 1) + c;
+        if (k <= 10) v10 = (v10 << 1) + c;
+        if (k <= 20) v20 = (v20 << 1) + c;
+        v35 = (v35 << 1) + c;
+    }
+    uint32_t pos = 0;
+    for (int q = 0; q < kSamplePer; q++) {
+        const uint64_t p = p0 + q;
+        if (p >= n) break;
         const uint32_t c = T[p];
+        const uint32_t c5 = p >= 5 ? T[p - 5] : 0u, c10 = p >= 10 ? T[p - 10] : 0u;
+        const uint32_t c20 = p >= 20 ? T[p - 20] : 0u, c35 = p >= 35 ? T[p - 35] : 0u;
+        v5 = (v5 << 1) + c - (c5 << 5);
+        v10 = (v10 << 1) + c - (c10 << 10);
+        v20 = (v20 << 1) + c - (c20 << 20);
+        v35 = (v35 << 1) + c - ((uint64_t)c35 << 35);
         atomicAdd(&h1[c], 1u);
-        if (p + 1 < n) atomicAdd(&hist2[(c << 8) | T[p + 1]], 1u);
-        local_pos++;
-        // window ending at p for every W
-        for (int wi = 0; wi < kNumW; wi++) {
-            const int W = kGridW[wi];
-            if (p + 1 < (uint64_t)W) continue;
-            uint64_t v = 0;
-            for (int k = 0; k < W; k++) v = (v << 1) + T[p - W + 1 + k];  // exact, < 2^43
-            for (int pi = 0; pi < kNumP; pi++)
-                if (v % (uint64_t)kGridP[pi] == 0) atomicAdd(&tc[wi * kNumP + pi], 1u);
+        if (p + 1 < n) {
+            const uint32_t pr = (c << 8) | T[p + 1];
+            atomicAdd(&h2[pr >> 1], 1u << (16 * (pr & 1u)));
+        }
+        pos++;
+#pragma unroll
+        for (int pi = 0; pi < kNumP; pi++) {
+            if (p >= 5) cnt[0 * kNumP + pi] += (v5 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+            if (p >= 10) cnt[1 * kNumP + pi] += (v10 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+            if (p >= 20) cnt[2 * kNumP + pi] += (v20 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+            if (p >= 35) cnt[3 * kNumP + pi] += (v35 * gd.inv64[pi] <= gd.lim64[pi]) ? 1u : 0u;
+        }
+        if (p >= 5) cnt[kDefaultCand] += ((v5 & 15u) == 0) ? 1u : 0u;
+    }
+#pragma unroll
+    for (int c = 0; c < kNumCand; c++)
+        if (cnt[c]) atomicAdd(&tc[c], cnt[c]);
+    if (pos) atomicAdd(&out->positions, pos);
+    __syncthreads();
+    if (h1[threadIdx.x]) atomicAdd(&out->hist1[threadIdx.x], h1[threadIdx.x]);
+    if (threadIdx.x < kNumCand && tc[threadIdx.x]) atomicAdd(&out->trig[threadIdx.x], tc[threadIdx.x]);
+    for (int i = threadIdx.x; i < 65536 / 2; i += kBlock) {
+        const uint32_t w = h2[i];
+        if (w & 0xFFFFu) atomicAdd(&hist2[2 * i], w & 0xFFFFu);
+        if (w >> 16) atomicAdd(&hist2[2 * i + 1], w >> 16);
+    }
+}
+
+constexpr int kSketchPer = 64;                              // positions per thread
+constexpr uint64_t kSketchBlock = (uint64_t)kBlock * kSketchPer;
+
+// insert a 32-bit fingerprint into an epoch-tagged open-addressing table (load factor at
+// most 1/4); true if new.  The probe sequence is bounded: a fingerprint that finds no slot
+// within kMaxProbe counts as new.
+constexpr int kMaxProbe = 256;
+__device__ __forceinline__ bool sketch_insert(uint64_t* __restrict__ tab, uint32_t mask, uint32_t epoch, uint32_t fp)
+{
+    const uint64_t mine = ((uint64_t)epoch << 32) | fp;
+    uint32_t h = (uint32_t)(mix64(fp) & mask);
+    for (int probe = 0; probe < kMaxProbe; probe++) {
+        uint64_t cur = __hip_atomic_load(tab + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (;;) {
+            if ((uint32_t)(cur >> 32) != epoch) {
+                const uint64_t seen = atomicCAS((unsigned long long*)(tab + h), (unsigned long long)cur,
+                                                (unsigned long long)mine);
+                if (seen == cur) return true;
+                cur = seen;
+                continue;
+            }
+            if ((uint32_t)cur == fp) return false;
+            break;
+        }
+        h = (h + 1) & mask;
+    }
+    return true;
+}
+
+// Content-defined sketch of the whole text: positions whose 8-byte window hashes into the
+// top 1/1024 contribute 32-bit fingerprints of the 16 and 64 bytes that start there; the
+// distinct ones are counted with two hash sets.  Per-block totals, no global counter.
+__global__ __launch_bounds__(kBlock) void k_feat_sketch(const uint8_t* __restrict__ T, uint64_t n,
+                                                       uint64_t* __restrict__ tab16, uint64_t* __restrict__ tab64,
+                                                       uint32_t mask, uint32_t epoch, uint32_t* __restrict__ part)
+{
+    __shared__ uint32_t red[3];
+    if (threadIdx.x < 3) red[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t p0 = (uint64_t)blockIdx.x * kSketchBlock + (uint64_t)threadIdx.x * kSketchPer;
+    uint32_t sel = 0, d16 = 0, d64 = 0;
+    if (p0 < n) {
+        const uint64_t* w = reinterpret_cast<const uint64_t*>(T + p0);  // p0 is a multiple of 64
+        uint64_t cur = w[0];
+        for (int q = 0; q < kSketchPer; q += 8) {
+            const uint64_t nxt = w[q / 8 + 1];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const uint64_t p = p0 + q + r;
+                const uint64_t win = r ? ((cur >> (8 * r)) | (nxt << (64 - 8 * r))) : cur;
+                if (((win * 0x9E3779B97F4A7C15ull) >> 54) == 0 && p + 64 <= n) {
+                    uint64_t h = 0x9E3779B97F4A7C15ull, a = 0;
+                    for (int k = 0; k < 8; k++) {
+                        h = mix64(h ^ load8(T, p + 8 * k));
+                        if (k == 1) a = h;
+                    }
+                    sel++;
+                    d16 += sketch_insert(tab16, mask, epoch, (uint32_t)(a >> 32)) ? 1u : 0u;
+                    d64 += sketch_insert(tab64, mask, epoch, (uint32_t)(h >> 32)) ? 1u : 0u;
+                }
+            }
+            cur = nxt;
         }
     }
-    atomicAdd(&out->positions, local_pos);
+    if (sel) {
+        atomicAdd(&red[0], sel);
+        atomicAdd(&red[1], d16);
+        atomicAdd(&red[2], d64);
+    }
     __syncthreads();
-    atomicAdd(&out->hist1[threadIdx.x], h1[threadIdx.x]);
-    if (threadIdx.x < kNumCand) atomicAdd(&out->trig[threadIdx.x], tc[threadIdx.x]);
+    if (threadIdx.x < 3) part[(uint64_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x];
 }
 
-// Content-defined sketch: positions whose 8-byte window hashes to 0 mod 256 contribute the
-// fingerprints of the 16 and 64 bytes that follow.
-__global__ void k_feat_sketch(const uint8_t* __restrict__ T, uint64_t n, uint64_t* __restrict__ f16,
-                              uint64_t* __restrict__ f64, uint32_t* __restrict__ cnt, uint32_t cap)
+__global__ __launch_bounds__(kBlock) void k_sum3(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                uint32_t* __restrict__ out)
 {
-    const uint64_t p = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p + 64 > n) return;
-    const uint64_t w = load8(T, p);
-    if ((mix64(w) & 255) != 0) return;
-    uint64_t h = 0x9E3779B97F4A7C15ull;
-    uint64_t a = 0;
-    for (int k = 0; k < 8; k++) {
-        h = mix64(h ^ load8(T, p + 8 * k));
-        if (k == 1) a = h;
-    }
-    const uint32_t slot = atomicAdd(cnt, 1u);
-    if (slot < cap) {
-        f16[slot] = a;
-        f64[slot] = h;
+    __shared__ uint32_t lds[kWaves];
+    for (int c = 0; c < 3; c++) {
+        uint32_t acc = 0;
+        for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) acc += part[(uint64_t)b * 3 + c];
+        uint32_t tot;
+        (void)block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
+        if (threadIdx.x == 0) out[c] = tot;
     }
 }
 
-__global__ void k_count_distinct(const uint64_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ out)
+static uint32_t inv_odd32(uint32_t q)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    if (i == 0 || K[i] != K[i - 1]) atomicAdd(out, 1u);
+    uint32_t x = q;
+    for (int i = 0; i < 5; i++) x *= 2 - q * x;
+    return x;
 }
 
 // Features of all candidates (host side assembly after one read-back).
@@ -130,36 +242,44 @@ void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X /* [kNumCand
     SampleCounts* sc = S.buf<SampleCounts>(B_SEL_COUNTS, 1);
     fill<uint32_t>(S, hist2, 65536, 0u);
     fill<uint32_t>(S, reinterpret_cast<uint32_t*>(sc), sizeof(SampleCounts) / 4, 0u);
-    const uint64_t stride = (n > (uint64_t)kChunk * kSampleChunks) ? n / kSampleChunks : kChunk;
-    const uint32_t blocks_per_chunk = kChunk / (kBlock * 16);
-    k_feat_sample<<<kSampleChunks * blocks_per_chunk, kBlock, 0, S.stream>>>(T, n, stride, hist2, sc);
-    MIBWT_LAUNCH_CHECK();
-    // sketch of the whole text
-    const uint32_t cap = (uint32_t)(n / 128 + 1024);
-    uint64_t* f16 = S.buf<uint64_t>(B_SORT_K0, cap);
-    uint64_t* f64 = S.buf<uint64_t>(B_LG_K0, cap);
-    uint64_t* tmp = S.buf<uint64_t>(B_SORT_K1, cap);
-    uint32_t* cnt = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    fill<uint32_t>(S, cnt, 4, 0u);
-    k_feat_sketch<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, f16, f64, cnt, cap);
-    MIBWT_LAUNCH_CHECK();
-    uint32_t m = 0;
-    S.read(&m, cnt);
-    if (m > cap) m = cap;
-    uint32_t dist[2] = {0, 0};
-    if (m > 0) {
-        radix_sort<uint64_t, uint32_t>(S, f16, tmp, nullptr, nullptr, m, 0, 64);
-        radix_sort<uint64_t, uint32_t>(S, f64, tmp, nullptr, nullptr, m, 0, 64);
-        k_count_distinct<<<grid_for(m), kBlock, 0, S.stream>>>(f16, m, cnt + 1);
-        k_count_distinct<<<grid_for(m), kBlock, 0, S.stream>>>(f64, m, cnt + 2);
-        MIBWT_LAUNCH_CHECK();
-        uint32_t hv[3];
-        S.read(hv, cnt, 3);
-        dist[0] = hv[1];
-        dist[1] = hv[2];
+    GridDiv gd;
+    for (int pi = 0; pi < kNumP; pi++) {
+        const uint32_t P = (uint32_t)kGridP[pi];
+        gd.inv32[pi] = inv_odd32(P);
+        gd.lim32[pi] = 0xFFFFFFFFu / P;
+        uint64_t x = P;
+        for (int i = 0; i < 6; i++) x *= 2 - (uint64_t)P * x;
+        gd.inv64[pi] = x;
+        gd.lim64[pi] = ~0ull / P;
     }
+    const uint64_t stride = (n > (uint64_t)kChunk * kSampleChunks) ? n / kSampleChunks : kChunk;
+    const uint32_t chunks = (uint32_t)std::min<uint64_t>(kSampleChunks, ceil_div(n, (uint64_t)kChunk));
+    k_feat_sample<<<chunks * kSampleParts, kBlock, 0, S.stream>>>(T, n, stride, gd, hist2, sc);
+    MIBWT_LAUNCH_CHECK();
+    // sketch of the whole text: two epoch-tagged hash sets of 2^k >= n/256 slots (about n/1024
+    // positions are selected, so the load factor stays below 1/4)
+    uint32_t bits = 12;
+    while ((1ull << bits) < n / 256 && bits < 30) bits++;
+    const uint64_t slots = 1ull << bits;
+    const size_t have = ((size_t)B_SEL_SKETCH < S.bufs.size()) ? S.bufs[B_SEL_SKETCH].cap : 0;
+    uint64_t* tab = S.buf<uint64_t>(B_SEL_SKETCH, 2 * slots);
+    if (have != S.bufs[B_SEL_SKETCH].cap || S.sketch_epoch == 0 || S.sketch_epoch == 0xFFFFFFFFu) {
+        MIBWT_HIP(hipMemsetAsync(tab, 0, S.bufs[B_SEL_SKETCH].cap, S.stream));
+        S.sketch_epoch = 0;
+    }
+    S.sketch_epoch++;
+    const uint32_t blocks = (uint32_t)ceil_div(n, kSketchBlock);
+    uint32_t* part = S.buf<uint32_t>(B_SEL_PART, (size_t)blocks * 3);
+    k_feat_sketch<<<blocks, kBlock, 0, S.stream>>>(T, n, tab, tab + slots, (uint32_t)(slots - 1), S.sketch_epoch,
+                                                   part);
+    MIBWT_LAUNCH_CHECK();
+    k_sum3<<<1, kBlock, 0, S.stream>>>(part, blocks, sc->sketch);
+    MIBWT_LAUNCH_CHECK();
     SampleCounts h;
     S.read(&h, sc);
+    const uint32_t m = h.sketch[0];
+    uint32_t dist[2] = {h.sketch[1], h.sketch[2]};
+
     std::vector<uint32_t> h2(65536);
     S.read(h2.data(), hist2, 65536);
     // order-0 / order-1 entropies
@@ -182,12 +302,14 @@ void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X /* [kNumCand
     const double rep16 = m ? (double)dist[0] / m : 1.0;
     const double rep64 = m ? (double)dist[1] / m : 1.0;
     const double pos = std::max(1u, h.positions);
-    for (int wi = 0; wi < kNumW; wi++)
-        for (int pi = 0; pi < kNumP; pi++) {
-            float* x = X + (wi * kNumP + pi) * kIn;
+    for (int c = 0; c < kNumCand; c++) {
+        {
+            float* x = X + c * kIn;
             for (int k = 0; k < kIn; k++) x[k] = 0.f;
-            const int W = kGridW[wi], P = kGridP[pi];
-            const double dens = (h.trig[wi * kNumP + pi] + 1.0) / (pos + 1.0);
+            const int W = cand_W(c), P = cand_P(c);
+            int wi = 0;
+            while (kGridW[wi] != W) wi++;
+            const double dens = (h.trig[c] + 1.0) / (pos + 1.0);
             x[0] = (float)(std::log2((double)n) / 32.0);
             x[1] = (float)(std::log2(std::max(1.0, sigma)) / 8.0);
             x[2] = (float)(H0 / 8.0);
@@ -208,6 +330,7 @@ void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X /* [kNumCand
             x[20] = (float)(rep64 / dens / 100.0);                     // distinct content x phrase length
             x[21] = (float)(std::log(rep64 + 1e-6) + std::log(1.0 / dens + W));
         }
+    }
 }
 
 // ------------------------------------------------------------------------ MLP kernels
@@ -604,8 +727,8 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
             best = c;
         }
     }
-    *W = kGridW[best / kNumP];
-    *P = kGridP[best % kNumP];
+    *W = cand_W(best);
+    *P = cand_P(best);
 }
 
 bool selector_has_model()
@@ -615,12 +738,13 @@ bool selector_has_model()
     return g_model.loaded;
 }
 
-void wp_grid(int* W, int* P)
+int wp_grid(int* W, int* P)
 {
     for (int c = 0; c < kNumCand; c++) {
-        W[c] = kGridW[c / kNumP];
-        P[c] = kGridP[c % kNumP];
+        W[c] = cand_W(c);
+        P[c] = cand_P(c);
     }
+    return kNumCand;
 }
 
 }  // namespace mibwt

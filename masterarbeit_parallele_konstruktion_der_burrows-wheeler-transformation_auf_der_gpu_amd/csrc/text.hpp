@@ -30,10 +30,35 @@ __device__ __forceinline__ uint64_t mix64(uint64_t h)
     return h;
 }
 
-// symbol presence bitmap of the text (256 bits): each thread ORs its 64 bytes into a
-// register bitmap, the wave reduces it with lane shuffles, one atomic per wave and word
-static __global__ __launch_bounds__(kBlock) void k_alpha_used(const uint8_t* __restrict__ T, uint64_t n,
-                                                             uint32_t* __restrict__ used)
+// Symbol presence bitmaps (256 bits as 8 words).  A thread keeps its bitmap in registers
+// (one compare and select per word, no dynamic register indexing), the wavefront ORs the
+// bitmaps with lane shuffles, and each block writes one partial bitmap (no global atomics:
+// same-address device atomics serialize); k_alpha_merge ORs the partials.
+__device__ __forceinline__ void bitmap_add(uint32_t (&bm)[8], uint32_t c)
+{
+    const uint32_t w = c >> 5, bit = 1u << (c & 31);
+#pragma unroll
+    for (int k = 0; k < 8; k++) bm[k] |= (w == (uint32_t)k) ? bit : 0u;
+}
+
+__device__ __forceinline__ void bitmap_block_store(uint32_t (&bm)[8], uint32_t* __restrict__ part)
+{
+    __shared__ uint32_t acc[8];
+    if (threadIdx.x < 8) acc[threadIdx.x] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        uint32_t x = bm[k];
+        for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
+        if ((threadIdx.x & 63) == 0 && x) atomicOr(&acc[k], x);
+    }
+    __syncthreads();
+    if (threadIdx.x < 8) part[(uint64_t)blockIdx.x * 8 + threadIdx.x] = acc[threadIdx.x];
+}
+
+// bytes of T[0..n), 64 per thread
+static __global__ __launch_bounds__(kBlock) void k_alpha_text(const uint8_t* __restrict__ T, uint64_t n,
+                                                             uint32_t* __restrict__ part)
 {
     uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * 64;
@@ -44,24 +69,47 @@ static __global__ __launch_bounds__(kBlock) void k_alpha_used(const uint8_t* __r
         const uint4 v = *reinterpret_cast<const uint4*>(T + p);
         const uint32_t w[4] = {v.x, v.y, v.z, v.w};
         const int cnt = (n - p) < 16 ? (int)(n - p) : 16;
-        for (int i = 0; i < cnt; i++) {
-            const uint32_t c = (w[i >> 2] >> (8 * (i & 3))) & 0xFF;
-            bm[c >> 5] |= 1u << (c & 31);
-        }
+#pragma unroll
+        for (int i = 0; i < 16; i++)
+            if (i < cnt) bitmap_add(bm, (w[i >> 2] >> (8 * (i & 3))) & 0xFF);
     }
+    bitmap_block_store(bm, part);
+}
+
+// bytes of the ranges T[start[u] .. start[u] + len[u]) clipped to n, one range per thread
+static __global__ __launch_bounds__(kBlock) void k_alpha_ranges(const uint8_t* __restrict__ T, uint64_t n,
+                                                               const uint32_t* __restrict__ start,
+                                                               const uint32_t* __restrict__ len, uint32_t m,
+                                                               uint32_t* __restrict__ part)
+{
+    uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u < m) {
+        const uint64_t s = start[u];
+        const uint64_t e = (s + len[u] < n) ? s + len[u] : n;
+        for (uint64_t p = s; p < e; p++) bitmap_add(bm, T[p]);
+    }
+    bitmap_block_store(bm, part);
+}
+
+static __global__ __launch_bounds__(kBlock) void k_alpha_merge(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                              uint32_t* __restrict__ used)
+{
+    __shared__ uint32_t acc[8];
+    if (threadIdx.x < 8) acc[threadIdx.x] = 0;
+    __syncthreads();
+    uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock)
+#pragma unroll
+        for (int k = 0; k < 8; k++) bm[k] |= part[(uint64_t)b * 8 + k];
 #pragma unroll
     for (int k = 0; k < 8; k++) {
         uint32_t x = bm[k];
         for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
-        bm[k] = x;
+        if ((threadIdx.x & 63) == 0 && x) atomicOr(&acc[k], x);
     }
-    if ((threadIdx.x & 63) < 8) {
-        const int k = threadIdx.x & 63;
-        uint32_t x = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) x = (j == k) ? bm[j] : x;
-        if (x) atomicOr(&used[k], x);
-    }
+    __syncthreads();
+    if (threadIdx.x < 8) used[threadIdx.x] = acc[threadIdx.x];
 }
 
 // dense symbol codes 2.. in byte order; codes[256] = number of distinct bytes
@@ -75,12 +123,17 @@ static __global__ void k_alpha_codes(const uint32_t* __restrict__ used, uint32_t
     if (c == 255) codes[256] = below + ((used[7] >> 31) & 1u);
 }
 
-// Dense codes of the bytes that occur in T[0..n): codes[c] for c < 256, codes[256] = sigma.
-// Uses codes[260..268) as scratch.  Returns nothing; read codes[256] for sigma.
-inline void alphabet_codes(Storage& S, const uint8_t* T, uint64_t n, uint32_t* codes)
+// Dense codes of the bytes that occur in T[0..n) (or, with ranges, in those ranges):
+// codes[c] for c < 256, codes[256] = sigma.  codes[260..268) holds the presence bitmap.
+inline void alphabet_codes(Storage& S, const uint8_t* T, uint64_t n, uint32_t* codes,
+                           const uint32_t* start = nullptr, const uint32_t* len = nullptr, uint32_t m = 0)
 {
-    fill<uint32_t>(S, codes + 260, 8, 0u);
-    k_alpha_used<<<ceil_div(ceil_div(n, 64), kBlock), kBlock, 0, S.stream>>>(T, n, codes + 260);
+    const uint32_t blocks = start ? ceil_div(m ? m : 1, kBlock) : ceil_div(ceil_div(n, 64), kBlock);
+    uint32_t* part = S.buf<uint32_t>(B_ALPHA_PART, (size_t)blocks * 8);
+    if (start) k_alpha_ranges<<<blocks, kBlock, 0, S.stream>>>(T, n, start, len, m, part);
+    else k_alpha_text<<<blocks, kBlock, 0, S.stream>>>(T, n, part);
+    MIBWT_LAUNCH_CHECK();
+    k_alpha_merge<<<1, kBlock, 0, S.stream>>>(part, blocks, codes + 260);
     MIBWT_LAUNCH_CHECK();
     k_alpha_codes<<<1, 256, 0, S.stream>>>(codes + 260, codes);
     MIBWT_LAUNCH_CHECK();

@@ -1,13 +1,14 @@
 """Collect training runs for the learned (W,P) selector and train it on the GPU.
 
-For every synthetic text the prefix-free-parsing pipeline is run once per grid candidate
-(W in {5,10,20,35} x P in {11,...,81}); the selector features and the measured dictionary
+For every synthetic text the prefix-free-parsing pipeline is run once per candidate
+(W in {5,10,20,35} x P in {11,...,81}, and the default W=5,P=16); the selector features and the measured dictionary
 length, parse length and build time become one training sample each.  The MLP is trained by
 the library's own HIP kernels (mibwt_mlp_train) and written to selector_model.bin next to the
 library (or --out).  Held-out texts report the chosen candidate's time against the best grid
 point and against fixed W=5,P=16.
 
-  python train_selector.py [--texts 300] [--holdout 24] [--out PATH] [--data runs.npz]
+  python train_selector.py [--texts 300] [--holdout 24] [--large 8 --large-holdout 2]
+                           [--out PATH] [--data runs.npz]
 """
 from __future__ import annotations
 
@@ -34,12 +35,12 @@ def _pkg():
     return mod
 
 
-def random_text(rng: np.random.Generator, max_len: int) -> tuple[np.ndarray, dict]:
+def random_text(rng: np.random.Generator, max_len: int, min_len: int = 4 << 20) -> tuple[np.ndarray, dict]:
     """A repetitive collection: random base (DNA, protein-like or bytes) x copies, mutated."""
     from mibwt import gen
     kind = rng.choice(["dna", "dna", "dna", "protein", "bytes"])
     copies = int(rng.integers(2, 65))
-    total = int(np.exp(rng.uniform(np.log(4 << 20), np.log(max_len))))
+    total = int(np.exp(rng.uniform(np.log(min_len), np.log(max_len))))
     base_len = max(4096, total // copies)
     sub = float(np.exp(rng.uniform(np.log(1e-5), np.log(5e-2))))
     indel = sub / float(rng.uniform(5, 20))
@@ -75,8 +76,6 @@ def run_grid(mibwt, st, T: np.ndarray, grid):
     Y = np.zeros((len(grid), 3), dtype=np.float32)
     ms = np.zeros(len(grid))
     st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)  # warm the buffers
-    st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)
-    fixed_ms = st.stats()["total_ms"]
     for c, (W, P) in enumerate(grid):
         st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
         s = st.stats()
@@ -84,7 +83,7 @@ def run_grid(mibwt, st, T: np.ndarray, grid):
         Y[c] = [np.log(s["dict_length"] / n), np.log(s["phrases"] / n), np.log(s["total_ms"] / n)]
     dT.free()
     dO.free()
-    return X, Y, ms, fixed_ms
+    return X, Y, ms, float(ms[grid.index((5, 16))])
 
 
 def main():
@@ -92,6 +91,9 @@ def main():
     ap.add_argument("--texts", type=int, default=300)
     ap.add_argument("--holdout", type=int, default=24)
     ap.add_argument("--max-mib", type=int, default=64)
+    ap.add_argument("--large", type=int, default=0, help="extra texts of 96 MiB .. --large-max-mib")
+    ap.add_argument("--large-holdout", type=int, default=0)
+    ap.add_argument("--large-max-mib", type=int, default=384)
     ap.add_argument("--epochs", type=int, default=300)
     ap.add_argument("--out", default=os.path.join(HERE, "selector_model.bin"))
     ap.add_argument("--data", default=None, help="write / reuse the collected runs (npz)")
@@ -101,12 +103,16 @@ def main():
     mibwt = _pkg()
     grid = mibwt.wp_grid()
     rng = np.random.default_rng(a.seed)
-    st = mibwt.Storage(a.max_mib << 20)
+    st = mibwt.Storage(max(a.max_mib, a.large_max_mib if a.large else 0) << 20)
     t0 = time.time()
     Xs, Ys, MS, FX, meta = [], [], [], [], []
-    total = a.texts + a.holdout
-    for i in range(total):
-        T, info = random_text(rng, a.max_mib << 20)
+    # order: regular training texts, large training texts, regular held-out, large held-out
+    sizes = ([(4 << 20, a.max_mib << 20)] * a.texts + [(96 << 20, a.large_max_mib << 20)] * a.large
+             + [(4 << 20, a.max_mib << 20)] * a.holdout
+             + [(96 << 20, a.large_max_mib << 20)] * a.large_holdout)
+    total = len(sizes)
+    for i, (lo, hi) in enumerate(sizes):
+        T, info = random_text(rng, hi, lo)
         X, Y, ms, fms = run_grid(mibwt, st, T, grid)
         Xs.append(X)
         Ys.append(Y)
@@ -123,13 +129,16 @@ def main():
     FX = np.array(FX)
     if a.data:
         np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX)
-    ntr = a.texts
+    ntr = a.texts + a.large
     loss = st.mlp_train(X[:ntr].reshape(-1, 64), Y[:ntr].reshape(-1, 3), epochs=a.epochs, batch=512,
                         lr=1e-3, seed=a.seed & 0xFFFF, out_path=a.out)
     print(f"[train_selector] trained on {ntr * len(grid)} runs, final loss {loss:.4f}, model {a.out}", flush=True)
     mibwt.selector_load(a.out)
     # held-out evaluation: predicted choice against the measured grid
     report = evaluate(mibwt, st, X[ntr:], MS[ntr:], FX[ntr:], grid, a.out)
+    if a.large_holdout:
+        k = a.large_holdout
+        report["large_holdout"] = evaluate(mibwt, st, X[-k:], MS[-k:], FX[-k:], grid, a.out)
     report["train_runs"] = int(ntr * len(grid))
     report["final_loss"] = loss
     print(json.dumps(report), flush=True)
@@ -140,7 +149,8 @@ def main():
 
 
 def evaluate(mibwt, st, X, MS, FX, grid, model_path):
-    """Choice of the trained model on held-out texts (features X[t][40][64], measured ms)."""
+    """Choice of the trained model on held-out texts (features X[t][candidates][64], measured
+    ms); the last candidate is the fixed default W=5,P=16."""
     with open(model_path, "rb") as f:
         raw = np.frombuffer(f.read(), dtype=np.float32)
     hdr = 4

@@ -136,3 +136,19 @@ def test_dna256m_roundtrip(mibwt):
         L, idx = st.bwt(T, "pfp", 5, 16)
     assert (np.bincount(L, minlength=256) == np.bincount(T, minlength=256)).all()
     assert (oracle.unbwt(L, idx) == T).all()
+
+
+def test_device_entry_learned_choice(mibwt):
+    # W = P = 0: the selector picks (W, P) on the resident text; the BWT is the same
+    from mibwt import gen
+    T = gen.repetitive_dna(1 << 17, 12, 0.001, 0.0001, 10, seed=13)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(len(T))
+    with mibwt.Storage(len(T)) as st:
+        idx = st.bwt_device(dT.address, dO.address, len(T), "pfp", 0, 0)
+        s = st.stats()
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (dO.download() == O).all()
+    assert (s["W"], s["P"]) in mibwt.wp_grid()
+    dT.free()
+    dO.free()

@@ -89,10 +89,53 @@ def test_selector_choice_keeps_bwt_exact(mibwt):
     dT = mibwt.DeviceBuffer.from_numpy(T)
     with mibwt.Storage(len(T)) as st:
         (W, P), trained, pred = st.select_wp(dT.address, len(T))
-        assert (W, P) in mibwt.wp_grid() or (W, P) == (5, 16)
+        assert (W, P) in mibwt.wp_grid() and mibwt.wp_grid()[-1] == (5, 16)
         X = st.wp_features(dT.address, len(T))
-        assert np.isfinite(X).all() and X.shape == (40, 64)
+        assert np.isfinite(X).all() and X.shape == (41, 64)
         L, idx = st.bwt(T, "cubwt")
     O, oidx = oracle.bwt(T)
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
     dT.free()
+
+
+def sample_features(T):
+    """CPU restatement of the sampled selector statistics: 64 chunks of 16384 bytes at stride
+    n/64; per grid candidate the fraction of sampled window ends p >= W whose exact window
+    value is divisible by P (the hash rule of PFPk:468-475); order-0 entropy of the sample."""
+    n = len(T)
+    chunk, chunks = 16384, 64
+    stride = n // chunks if n > chunk * chunks else chunk
+    pos = np.concatenate([np.arange(c * stride, min(c * stride + chunk, n)) for c in range(chunks)
+                          if c * stride < n])
+    t = T.astype(np.int64)
+    out = {}
+    for W in (5, 10, 20, 35):
+        ok = pos[pos >= W]
+        v = np.zeros(len(ok), dtype=np.int64)  # exact: 255 * (2^35 - 1) < 2^63
+        for k in range(W):
+            v = v * 2 + t[ok - W + 1 + k]
+        for P in (11, 13, 17, 23, 31, 41, 53, 61, 71, 81) + ((16,) if W == 5 else ()):
+            out[(W, P)] = int(np.sum(v % P == 0))
+    hist = np.bincount(T[pos], minlength=256)
+    p = hist[hist > 0] / len(pos)
+    return out, len(pos), float(-(p * np.log2(p)).sum())
+
+
+@pytest.mark.gpu
+def test_sampled_features_match_cpu_restatement(mibwt):
+    from mibwt import gen
+    T = gen.repetitive_dna(1 << 18, 12, 0.001, 0.0001, 10, seed=21)  # 3 MiB: 64 strided chunks
+    trig, npos, H0 = sample_features(T)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    with mibwt.Storage(len(T)) as st:
+        X = st.wp_features(dT.address, len(T))
+        X2 = st.wp_features(dT.address, len(T))  # deterministic, tables reused across calls
+    dT.free()
+    assert (X == X2).all()
+    grid = mibwt.wp_grid()
+    for c, (W, P) in enumerate(grid):
+        dens = (trig[(W, P)] + 1.0) / (npos + 1.0)
+        assert abs(X[c, 16] - dens) <= 1e-6 * dens, (W, P)
+    assert abs(X[0, 2] - H0 / 8.0) < 1e-5
+    # repetitiveness: 12 copies of one base leave few distinct 64-byte fingerprints
+    assert 0.0 < X[0, 5] < 0.5 and X[0, 4] <= X[0, 5] + 1e-6
