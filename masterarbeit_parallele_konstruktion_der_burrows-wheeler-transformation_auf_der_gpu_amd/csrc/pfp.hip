@@ -27,6 +27,8 @@
 //
 // The terminator is virtual: positions >= n of the extended text T.$^W compare below every
 // byte, so all 256 byte values are accepted (the reference needs bytes >= 2).
+#include <cstdlib>
+
 #include "doubling.hpp"
 #include "text.hpp"
 
@@ -446,26 +448,30 @@ __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
     V[k] = k;
 }
 
-// phrase fingerprints are sorted by their low 48 bits (6 radix passes)
+// Phrase fingerprints are first sorted by their low 48 bits (6 radix passes).  Neighbours
+// with equal sort keys must be equal phrases: different 64-bit fingerprints under one sort key
+// (bit 1 of *collision) repeat the sort on all 64 bits, equal fingerprints of different
+// phrases (bit 0) re-seed the hash.
 constexpr int kDictKeyBits = 48;
-constexpr uint64_t kDictKeyMask = (1ull << kDictKeyBits) - 1;
 
 __global__ void k_dict_mark(const uint8_t* __restrict__ T, uint64_t n, int W,
                             const uint32_t* __restrict__ E, const uint64_t* __restrict__ H,
-                            const uint32_t* __restrict__ V, uint32_t d, uint32_t* __restrict__ flag,
-                            uint32_t* __restrict__ collision)
+                            const uint32_t* __restrict__ V, uint32_t d, uint64_t key_mask,
+                            uint32_t* __restrict__ flag, uint32_t* __restrict__ collision)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d) return;
     uint32_t f = 1;
-    if (j > 0 && (H[j] & kDictKeyMask) == (H[j - 1] & kDictKeyMask)) {
-        // the sort looked at the low kDictKeyBits only: equal sort keys must be equal
-        // fingerprints of equal phrases, anything else re-seeds the hash
+    if (j > 0 && (H[j] & key_mask) == (H[j - 1] & key_mask)) {
         f = 0;
-        uint32_t s1, e1, s2, e2;
-        phrase_bounds(E, V[j], W, s1, e1);
-        phrase_bounds(E, V[j - 1], W, s2, e2);
-        if (H[j] != H[j - 1] || !phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
+        if (H[j] != H[j - 1]) {
+            atomicOr(collision, 2u);
+        } else {
+            uint32_t s1, e1, s2, e2;
+            phrase_bounds(E, V[j], W, s1, e1);
+            phrase_bounds(E, V[j - 1], W, s2, e2);
+            if (!phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
+        }
     }
     flag[j] = f;
 }
@@ -975,20 +981,30 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint32_t* flag = S.buf<uint32_t>(B_PH_FLAG, d);
     uint32_t du = 0;
     uint64_t seed = 0x6A09E667F3BCC909ull;
+    int key_bits = kDictKeyBits;
+    if (const char* kb = getenv("MIBWT_DICT_KEY_BITS")) {
+        // narrower first sorts exercise the widening path (tests)
+        const int v = atoi(kb);
+        if (v >= 8 && v <= 64) key_bits = v & ~7;
+    }
+    int dict_passes = 0;
     for (int attempt = 0;; attempt++) {
-        MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
+        MIBWT_REQUIRE(attempt < 6, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
         k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, K0, V0);
         MIBWT_LAUNCH_CHECK();
-        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, kDictKeyBits);
+        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, key_bits);
+        dict_passes += key_bits / 8;
         fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
-        k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, K0, V0, d, flag, sc + SC_COLLISION);
+        const uint64_t key_mask = (key_bits >= 64) ? ~0ull : ((1ull << key_bits) - 1);
+        k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, K0, V0, d, key_mask, flag, sc + SC_COLLISION);
         MIBWT_LAUNCH_CHECK();
         inclusive_sum<uint32_t>(S, flag, K1 ? reinterpret_cast<uint32_t*>(K1) : nullptr, d, sc + SC_DISTINCT);
         uint32_t hv[SC_COUNT_];
         S.read(hv, sc, SC_COUNT_);
         du = hv[SC_DISTINCT];
         if (hv[SC_COLLISION] == 0) break;
-        seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        if (hv[SC_COLLISION] & 1u) seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+        else key_bits = 64;
     }
     const uint32_t* ids = reinterpret_cast<const uint32_t*>(K1);
     uint32_t* pid = S.buf<uint32_t>(B_PH_PID, d);
@@ -1012,7 +1028,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     S.read(&sigma, codes + 256);
     MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "dictionary too large");
     const uint32_t L = (uint32_t)L64;
-    S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 8.0 * d + 24.0 * d * (kDictKeyBits / 8) + 12.0 * d);
+    S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 8.0 * d + 24.0 * d * dict_passes + 12.0 * d);
     S.stats.distinct_phrases = du;
     S.stats.dict_length = L;
 

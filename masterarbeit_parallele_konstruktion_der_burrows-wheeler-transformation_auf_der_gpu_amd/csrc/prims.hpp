@@ -398,19 +398,36 @@ struct EmitPair {
     __device__ __forceinline__ void val(uint64_t pos, V v) const { vals_out[pos] = v; }
 };
 
-// Exclusive prefix of digit d over the tiles before `tile` (one thread per digit).
+// Exclusive prefix of digit d over the tiles before `tile` (one thread per digit).  Status
+// words of 8 predecessors are requested at once: device-coherent loads take a round trip to
+// memory, so a one-tile-per-step walk would serialize on that latency.
+constexpr int kLookbackBatch = 8;
 __device__ __forceinline__ uint32_t chain_lookback_digit(const Chain& c, uint32_t tile, uint32_t d)
 {
     uint32_t excl = 0;
-    for (int64_t j = (int64_t)tile - 1; j >= 0; j--) {
-        const uint64_t* st = c.status + (uint64_t)j * kRadix + d;
-        uint64_t w = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        while ((uint32_t)(w >> 34) != c.epoch) {
-            __builtin_amdgcn_s_sleep(1);
-            w = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int64_t j = (int64_t)tile - 1;
+    while (j >= 0) {
+        uint64_t w[kLookbackBatch];
+#pragma unroll
+        for (int k = 0; k < kLookbackBatch; k++)
+            w[k] = (j - k >= 0) ? __hip_atomic_load(c.status + (uint64_t)(j - k) * kRadix + d, __ATOMIC_RELAXED,
+                                                    __HIP_MEMORY_SCOPE_AGENT)
+                                : 0ull;
+        bool done = false;
+#pragma unroll
+        for (int k = 0; k < kLookbackBatch; k++) {
+            if (done || j - k < 0) break;
+            const uint64_t* st = c.status + (uint64_t)(j - k) * kRadix + d;
+            uint64_t x = w[k];
+            while ((uint32_t)(x >> 34) != c.epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                x = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+            excl += (uint32_t)x;
+            done = ((x >> 32) & 3u) == 2u;
         }
-        excl += (uint32_t)w;
-        if (((w >> 32) & 3u) == 2u) break;
+        if (done) break;
+        j -= kLookbackBatch;
     }
     return excl;
 }

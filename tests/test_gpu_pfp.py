@@ -152,3 +152,15 @@ def test_device_entry_learned_choice(mibwt):
     assert (s["W"], s["P"]) in mibwt.wp_grid()
     dT.free()
     dO.free()
+
+
+def test_narrow_fingerprint_sort_widens_on_collision(mibwt, monkeypatch):
+    # an 8-bit first sort makes different phrases share sort keys: the dictionary must detect
+    # it and repeat the sort on all 64 bits, with the same exact result
+    rng = np.random.default_rng(17)
+    T = rng.integers(0, 256, size=200000).astype(np.uint8)
+    monkeypatch.setenv("MIBWT_DICT_KEY_BITS", "8")
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "pfp", 5, 11)
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
