@@ -65,6 +65,8 @@ def main():
     ap.add_argument("--W", type=int, default=5)
     ap.add_argument("--P", type=int, default=16)
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--verify", choices=["gpu", "host"], default="gpu",
+                    help="round trip by the library's inverse BWT (gpu) or the oracle's (host)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=32)
     ap.add_argument("--wp", choices=["auto", "fixed"], default="auto",
@@ -184,19 +186,28 @@ def main():
 
     verified = None
     if not a.no_verify:
-        sys.path.insert(0, ROOT)
-        import oracle  # checker only
         step()
         O = dO.download()
         if not (O == O_fixed).all():
             log("BWT differs between the fixed and the learned (W,P)")
             sys.exit(1)
-        L = mibwt.reference_to_cubwt(O, idx)
         t = time.perf_counter()
-        back = oracle.unbwt(L, idx)
+        if a.verify == "host":
+            sys.path.insert(0, ROOT)
+            import oracle  # checker only
+            back = oracle.unbwt(mibwt.reference_to_cubwt(O, idx), idx)
+            verified = "unbwt_roundtrip_ok_host"
+        else:
+            # inverse BWT on the device (libcubwt_unbwt's kernels), compared on the host
+            dU = mibwt.DeviceBuffer(n)
+            st.unbwt_device(dO.address, dU.address, n, idx)
+            back = dU.download()
+            dU.free()
+            verified = "unbwt_roundtrip_ok_gpu"
         ok = bool((back == T).all())
-        verified = "unbwt_roundtrip_ok" if ok else "MISMATCH"
-        log(f"verification (inverse BWT on the host): {verified} in {time.perf_counter() - t:.1f} s")
+        if not ok:
+            verified = "MISMATCH"
+        log(f"verification (inverse BWT, {a.verify}): {verified} in {time.perf_counter() - t:.1f} s")
         if not ok:
             sys.exit(1)
 

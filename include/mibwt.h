@@ -58,6 +58,21 @@ int64_t libcubwt_free_device_storage(void* device_storage);
  * from mibwt_select_wp (fixed W=5,P=16 when the selector is not trained). */
 int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t n);
 
+/* Sampled suffix-array inverse together with the BWT (libcubwt.cuh:79-89): L as libcubwt_bwt,
+ * I[j] = the row of the rotation starting at text position j*r, in the convention of the
+ * primary index (I[0] is the primary index); r is a power of two >= 2, I has (n-1)/r + 1
+ * entries.  Returns LIBCUBWT_NO_ERROR or an error code.  (The reference's libcubwt.cu binds
+ * it at :2433-2450.) */
+int64_t libcubwt_bwt_aux(void* device_storage, const uint8_t* T, uint8_t* L, int64_t n, int64_t r,
+                         uint32_t* I);
+
+/* Inverse BWT (libcubwt.cuh:91-101; declared but not defined by the reference's libcubwt.cu):
+ * T is a BWT in libcubwt format with primary index i (1 <= i <= n), U receives the text (U may
+ * alias T).  freq is accepted and not needed.  Returns LIBCUBWT_NO_ERROR, or
+ * LIBCUBWT_BAD_PARAMETER when T is not a BWT with primary index i. */
+int64_t libcubwt_unbwt(void* device_storage, const uint8_t* T, uint8_t* U, int64_t n, const int32_t* freq,
+                       int32_t i);
+
 /* ---------------------------------------------------------------- mibwt extensions */
 
 /* Prefix-free-parsing BWT with runtime window W (1..64) and modulus P (1..65535).
@@ -73,6 +88,12 @@ int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, in
  * choice is reported by mibwt_last_phase_stats).  Returns the primary index. */
 int64_t mibwt_pfp_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
                              int32_t W, int32_t P, void* stream);
+
+/* Inverse BWT of a device BWT in REFERENCE format (the output of the *_device entries):
+ * dO[0..n) with primary index -> dU[0..n).  Synchronous.  Returns LIBCUBWT_NO_ERROR or
+ * LIBCUBWT_BAD_PARAMETER (no reference counterpart: the on-device round-trip check). */
+int64_t mibwt_unbwt_device(void* storage, const uint8_t* dO, uint8_t* dU, int64_t n, int64_t index,
+                           void* stream);
 
 /* Lyndon-grammar BWT (Lyndon_Grammar_BWT.cu main :5547-5576: build_Lyndon_Grammar :2592,
  * Sort_Lyndon_Grammar :3621, Build_BWT :4628).  Host buffers, libcubwt format. */

@@ -37,6 +37,9 @@ EXPORTED = [
     "libcubwt_allocate_device_storage",
     "libcubwt_free_device_storage",
     "libcubwt_bwt",
+    "libcubwt_bwt_aux",
+    "libcubwt_unbwt",
+    "mibwt_unbwt_device",
     "mibwt_pfp_bwt",
     "mibwt_pfp_bwt_device",
     "mibwt_lyndon_bwt",
@@ -101,6 +104,9 @@ def lib():
         L.libcubwt_allocate_device_storage.argtypes = [ctypes.POINTER(vp), i64]
         L.libcubwt_free_device_storage.argtypes = [vp]
         L.libcubwt_bwt.argtypes = [vp, u8p, u8p, i64]
+        L.libcubwt_bwt_aux.argtypes = [vp, u8p, u8p, i64, i64, ctypes.POINTER(ctypes.c_uint32)]
+        L.libcubwt_unbwt.argtypes = [vp, u8p, u8p, i64, ctypes.POINTER(ctypes.c_int32), i32]
+        L.mibwt_unbwt_device.argtypes = [vp, vp, vp, i64, i64, vp]
         L.mibwt_pfp_bwt.argtypes = [vp, u8p, u8p, i64, i32, i32]
         L.mibwt_pfp_bwt_device.argtypes = [vp, vp, vp, i64, i32, i32, vp]
         L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
@@ -226,6 +232,32 @@ class Storage:
         if rc < 0:
             raise BwtError(rc, method + "_device")
         return int(rc)
+
+    def unbwt(self, L, index: int) -> np.ndarray:
+        """Inverse BWT (libcubwt_unbwt) of L in libcubwt format with its primary index."""
+        L = _as_text(L)
+        U = np.empty_like(L)
+        rc = lib().libcubwt_unbwt(self.handle, _u8(L), _u8(U), len(L), None, int(index))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "libcubwt_unbwt")
+        return U
+
+    def bwt_aux(self, T, r: int):
+        """(L, I): the libcubwt BWT and the sampled rows I[j] of text positions j*r."""
+        T = _as_text(T)
+        L = np.empty_like(T)
+        I = np.zeros((len(T) - 1) // r + 1, dtype=np.uint32)
+        rc = lib().libcubwt_bwt_aux(self.handle, _u8(T), _u8(L), len(T), int(r),
+                                    I.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "libcubwt_bwt_aux")
+        return L, I
+
+    def unbwt_device(self, dO_ptr: int, dU_ptr: int, n: int, index: int, stream: int | None = None):
+        """Inverse of a device BWT in reference format (the *_device entries' output)."""
+        rc = lib().mibwt_unbwt_device(self.handle, dO_ptr, dU_ptr, n, int(index), stream)
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_unbwt_device")
 
     def pfp_triggers(self, T, W: int = 5, P: int = 16) -> np.ndarray:
         """Trigger end positions (ascending) of the prefix-free parse of T."""

@@ -14,6 +14,8 @@ int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n, int W, int P
 double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
+int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n, int64_t index, bool ref_format,
+                  uint32_t* dI, int64_t rate);
 void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred);
 bool selector_load(const char* path);
 bool selector_has_model();
@@ -149,6 +151,68 @@ int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t
         const int64_t idx = pfp_bwt_run(*S, nullptr, false, dO, n, W, P);
         copy_out_cubwt(*S, dO, L, wrap, n, idx);
         return idx;
+    });
+}
+
+int64_t libcubwt_bwt_aux(void* device_storage, const uint8_t* T, uint8_t* L, int64_t n, int64_t r, uint32_t* I)
+{
+    Storage* S = reinterpret_cast<Storage*>(device_storage);
+    if (!S || !T || !L || !I || n < 16 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    if (r < 2 || (r & (r - 1)) != 0) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* dO = S->buf<uint8_t>(B_STAGE, (size_t)n);
+        const uint8_t wrap = T[n - 1];
+        uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
+        int W = 5, P = 16;
+        selector_choose_device(*S, dT, (uint64_t)n, &W, &P, nullptr);
+        const int64_t idx = pfp_bwt_run(*S, nullptr, false, dO, n, W, P);
+        // sampled rows from the LF walk over the BWT just built (still in HBM)
+        const int64_t cnt = (n - 1) / r + 1;
+        uint32_t* dI = S->buf<uint32_t>(B_UN_AUX, (size_t)cnt);
+        const int64_t rc = unbwt_run(*S, dO, nullptr, n, idx, true, dI, r);
+        if (rc != LIBCUBWT_NO_ERROR) return rc;
+        MIBWT_HIP(hipMemcpyAsync(I, dI, sizeof(uint32_t) * cnt, hipMemcpyDeviceToHost, S->stream));
+        copy_out_cubwt(*S, dO, L, wrap, n, idx);
+        return (int64_t)LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t libcubwt_unbwt(void* device_storage, const uint8_t* T, uint8_t* U, int64_t n, const int32_t* freq, int32_t i)
+{
+    (void)freq;  // symbol counts are taken on the device
+    Storage* S = reinterpret_cast<Storage*>(device_storage);
+    if (!S || !T || !U || n < 1 || n > S->max_length || i < 1 || i > n) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* dL = S->buf<uint8_t>(B_UN_IO, (size_t)n);
+        uint8_t* dU = S->buf<uint8_t>(B_STAGE, (size_t)n);
+        MIBWT_HIP(hipMemcpyAsync(dL, T, (size_t)n, hipMemcpyHostToDevice, S->stream));
+        const int64_t rc = unbwt_run(*S, dL, dU, n, i, false, nullptr, 0);
+        if (rc != LIBCUBWT_NO_ERROR) return rc;
+        MIBWT_HIP(hipMemcpyAsync(U, dU, (size_t)n, hipMemcpyDeviceToHost, S->stream));
+        MIBWT_HIP(hipStreamSynchronize(S->stream));
+        return (int64_t)LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_unbwt_device(void* storage, const uint8_t* dO, uint8_t* dU, int64_t n, int64_t index, void* stream)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !dO || !dU || n < 1 || n > S->max_length || index < 1 || index > n) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
+        if (caller) {
+            hipEvent_t ev;
+            MIBWT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            MIBWT_HIP(hipEventRecord(ev, caller));
+            MIBWT_HIP(hipStreamWaitEvent(S->stream, ev, 0));
+            MIBWT_HIP(hipEventDestroy(ev));
+        }
+        const int64_t rc = unbwt_run(*S, dO, dU, n, index, true, nullptr, 0);
+        MIBWT_HIP(hipStreamSynchronize(S->stream));
+        return rc;
     });
 }
 

@@ -260,6 +260,7 @@ enum BufId : int {
     B_DT_OWNER,
     B_DT_RANK,
     B_DT_SA,
+    B_DT_LIM,
     B_ACT_X,
     B_ACT_POS,
     B_ACT_X2,
@@ -277,12 +278,24 @@ enum BufId : int {
     B_MX_ENT,
     B_Q_CNT,
     B_Q_OFF,
+    B_Q_BLK,
+    B_UN_REC,
+    B_UN_NXT0,
+    B_UN_NXT1,
+    B_UN_W,
+    B_UN_D0,
+    B_UN_D1,
+    B_UN_IO,
+    B_UN_AUX,
     B_SCALARS,
     B_STAGE,
     B_DB_KEY2,
     B_DB_GID,
     B_DB_GOFS,
     B_DB_LARGE,
+    B_DB_NGZ,
+    B_DB_GOFS0,
+    B_LG_SZ,
     B_LG_K0,
     B_LG_K1,
     B_LG_V0,
@@ -308,6 +321,9 @@ enum BufId : int {
     B_LY_VL,
     B_LY_JMP0,
     B_LY_JMP1,
+    B_LY_LABH,
+    B_LY_GL,
+    B_LY_GF,
     B_AS_REC,
     B_AS_R,
     B_AS_HD,

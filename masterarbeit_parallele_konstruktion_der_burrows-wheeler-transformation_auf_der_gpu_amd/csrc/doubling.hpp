@@ -1,12 +1,14 @@
 // Group refinement ("prefix doubling") shared by the BWT pipelines.
 //
-// State: elements V[0..m) in an order in which equal keys K form contiguous groups, each
-// group occupying the global sorted positions G[...] of its members.  A round
-//   1. assigns rank[x] = 1 + global position of the group head and SA[G[i]] = V[i],
-//   2. keeps the groups that can still split (>= 2 members and Policy::open),
-//   3. orders each kept group by the rank of Policy::next(x) (0 when there is none):
-//      groups of up to kSmallGroup members in place (every member counts the members ordered
-//      before it), larger ones by one radix sort of (group, rank) keys.
+// State: the active elements V[0..m) in an order in which the members of a group are
+// contiguous; per element the group record (first slot Hs, global start Gs, size Gz).  A round
+//   1. takes the secondary key of every member: the rank of Policy::next(x) (0 if none);
+//   2. orders each group by that key: groups of up to kSmallGroup members in place (every
+//      member counts the members ordered before it), larger ones by one radix sort of
+//      (group start, key) pairs; the same kernels write the new rank (1 + global position of
+//      the sub-group's first member), the SA slot and the sub-group record of every member;
+//   3. keeps the sub-groups that can still split (>= 2 members and Policy::open) and compacts
+//      them, which also yields next round's counts in one host read-back.
 // The loop stops when no group can split.  Policies:
 //   - strings of a family (PFP dictionary / parse): next(x) = x + h inside the segment,
 //     open while the compared prefix length h has not run past the strings (then equal
@@ -21,54 +23,13 @@ namespace mibwt {
 constexpr uint32_t kNoNext = 0xFFFFFFFFu;
 constexpr uint32_t kSmallGroup = 256;
 
+// ---- initial groups: runs of equal keys in the initially sorted order
+
 static __global__ void k_dbl_headflag(const uint64_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    hf[i] = (i == 0 || K[i] != K[i - 1]) ? i : 0;
-}
-
-template <typename Policy>
-__global__ void k_dbl_update(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V,
-                             const uint32_t* __restrict__ G, const uint32_t* __restrict__ hidx,
-                             uint32_t m, uint32_t* __restrict__ rank, uint32_t* __restrict__ SA,
-                             uint32_t* __restrict__ keep, Policy pol)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t x = V[i];
-    const uint32_t hi = hidx[i];
-    rank[x] = G[hi] + 1;
-    SA[G[i]] = x;
-    const bool single = (hi == i) && (i + 1 == m || K[i + 1] != K[i]);
-    keep[i] = (!single && pol.open(V[hi])) ? 1u : 0u;
-}
-
-static __global__ void k_dbl_compact(const uint32_t* __restrict__ V, const uint32_t* __restrict__ G,
-                                     const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
-                                     uint32_t m, uint32_t* __restrict__ V2, uint32_t* __restrict__ G2)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    if (keep[i]) {
-        V2[pos[i]] = V[i];
-        G2[pos[i]] = G[i];
-    }
-}
-
-static __global__ void k_iota(uint32_t* __restrict__ p, uint32_t m)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) p[i] = i;
-}
-
-// group heads of the compacted active array: members of one group share their rank
-static __global__ void k_grp_heads(const uint32_t* __restrict__ V, const uint32_t* __restrict__ rank, uint32_t m,
-                                   uint32_t* __restrict__ hf)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    hf[i] = (i == 0 || rank[V[i]] != rank[V[i - 1]]) ? 1u : 0u;
+    hf[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
 }
 
 // gofs[g] = first index of group g (0-based), gofs[ngroups] = m
@@ -81,6 +42,31 @@ static __global__ void k_grp_offsets(const uint32_t* __restrict__ hf, const uint
     if (i == m - 1) gofs[gid1[i]] = m;
 }
 
+// Element i of the initial order: rank and SA slot, whether its group stays active, and the
+// group record (first slot, global start, size) carried to the compaction.
+template <typename Policy>
+__global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
+                           const uint32_t* __restrict__ gofs, uint32_t m, uint32_t* __restrict__ rank,
+                           uint32_t* __restrict__ SA, uint32_t* __restrict__ keep, uint32_t* __restrict__ NV,
+                           uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
+                           Policy pol)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t g = gid1[i] - 1;
+    const uint32_t s = gofs[g], z = gofs[g + 1] - s;
+    const uint32_t x = V[i];
+    rank[x] = s + 1;
+    SA[i] = x;
+    keep[i] = (z > 1 && pol.open(x)) ? 1u : 0u;
+    NV[i] = x;
+    NHs[i] = s;
+    NGs[i] = s;
+    NGz[i] = z;
+}
+
+// ---- one refinement round over the m active elements (groups contiguous)
+
 template <typename Policy>
 __global__ void k_key2(const uint32_t* __restrict__ V, uint32_t m, const uint32_t* __restrict__ rank,
                        Policy pol, uint32_t* __restrict__ key2)
@@ -91,56 +77,145 @@ __global__ void k_key2(const uint32_t* __restrict__ V, uint32_t m, const uint32_
     key2[i] = (y == kNoNext) ? 0u : rank[y];
 }
 
-// Small groups: each member counts the members ordered before it (key, then index) and moves
-// to that slot of its group; members of large groups are only flagged.
-static __global__ void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ key2,
-                                   const uint32_t* __restrict__ gid1, const uint32_t* __restrict__ gofs,
-                                   uint32_t m, uint32_t small_max, uint32_t* __restrict__ V2,
-                                   uint64_t* __restrict__ K2, uint32_t* __restrict__ large)
+// Small groups: each member counts the members ordered before it (key, then slot), moves to
+// that slot and records its new sub-group (members with an equal key): rank = global start of
+// the sub-group + 1, SA slot, and the sub-group record for the compaction.  The block's keys
+// are staged in LDS: groups that lie inside the block count there, groups that cross the
+// block's edge read the keys from memory.  Members of large groups are only flagged.
+template <typename Policy>
+__global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
+                                                     const uint32_t* __restrict__ Gs, const uint32_t* __restrict__ Gz,
+                                                     const uint32_t* __restrict__ key2, uint32_t m,
+                                                     uint32_t small_max, uint32_t* __restrict__ rank,
+                                                     uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                                                     uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs,
+                                                     uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
+                                                     uint32_t* __restrict__ large, Policy pol)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t lk[kBlock];
+    const uint32_t b0 = blockIdx.x * kBlock;
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t s = 0, z = 0, k = 0;
+    if (i < m) {
+        s = Hs[i];
+        z = Gz[i];
+        k = key2[i];
+    }
+    lk[threadIdx.x] = k;
+    __syncthreads();
     if (i >= m) return;
-    const uint32_t g = gid1[i] - 1;
-    const uint32_t s = gofs[g], e = gofs[g + 1];
-    if (e - s > small_max) {
+    if (z > small_max) {
         large[i] = 1;
         return;
     }
     large[i] = 0;
-    const uint32_t k = key2[i];
-    uint32_t r = 0;
-    for (uint32_t j = s; j < e; j++) {
-        const uint32_t kj = key2[j];
-        r += (kj < k) || (kj == k && j < i);
+    uint32_t lt = 0, eq_before = 0, eq = 0;
+    if (s >= b0 && s + z <= b0 + kBlock) {
+        const uint32_t ls = s - b0, li = i - b0;
+        for (uint32_t j = ls; j < ls + z; j++) {
+            const uint32_t kj = lk[j];
+            lt += (kj < k);
+            eq += (kj == k);
+            eq_before += (kj == k && j < li);
+        }
+    } else {
+        for (uint32_t j = s; j < s + z; j++) {
+            const uint32_t kj = key2[j];
+            lt += (kj < k);
+            eq += (kj == k);
+            eq_before += (kj == k && j < i);
+        }
     }
-    V2[s + r] = V[i];
-    K2[s + r] = ((uint64_t)(g + 1) << 32) | k;
+    const uint32_t x = V[i];
+    const uint32_t gs = Gs[i];
+    const uint32_t p = s + lt + eq_before;
+    rank[x] = gs + lt + 1;
+    SA[gs + lt + eq_before] = x;
+    keep[p] = (eq > 1 && pol.open(x)) ? 1u : 0u;
+    NV[p] = x;
+    NHs[p] = s + lt;
+    NGs[p] = gs + lt;
+    NGz[p] = eq;
 }
 
-static __global__ void k_seg_large_gather(const uint32_t* __restrict__ V, const uint32_t* __restrict__ key2,
-                                          const uint32_t* __restrict__ gid1, const uint32_t* __restrict__ large,
+// Large groups: (group start slot, key) pairs of the flagged members in slot order.
+static __global__ void k_seg_large_gather(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
+                                          const uint32_t* __restrict__ key2, const uint32_t* __restrict__ large,
                                           const uint32_t* __restrict__ lpos, uint32_t m, int B,
-                                          uint64_t* __restrict__ LK, uint32_t* __restrict__ LV,
-                                          uint32_t* __restrict__ LI)
+                                          uint64_t* __restrict__ LK, uint32_t* __restrict__ LV)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m || !large[i]) return;
-    const uint32_t p = lpos[i];
-    LK[p] = ((uint64_t)gid1[i] << B) | key2[i];
-    LV[p] = V[i];
-    LI[p] = i;
+    const uint32_t q = lpos[i];
+    LK[q] = ((uint64_t)Hs[i] << B) | key2[i];
+    LV[q] = V[i];
 }
 
-static __global__ void k_seg_large_scatter(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
-                                           const uint32_t* __restrict__ LI, uint32_t cnt, int B,
-                                           uint32_t* __restrict__ V2, uint64_t* __restrict__ K2)
+static __global__ void k_lg_heads(const uint64_t* __restrict__ LK, uint32_t nl, uint32_t* __restrict__ hq)
 {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= cnt) return;
-    const uint32_t i = LI[j];
-    const uint64_t k = LK[j];
-    V2[i] = LV[j];
-    K2[i] = ((k >> B) << 32) | (k & ((1ull << B) - 1));
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nl) return;
+    hq[q] = (q == 0 || LK[q] != LK[q - 1]) ? q : 0u;
+}
+
+// size of each sub-group, stored at its first index
+static __global__ void k_lg_sizes(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ qh, uint32_t nl,
+                                  uint32_t* __restrict__ sz)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nl) return;
+    if (q + 1 == nl || LK[q] != LK[q + 1]) sz[qh[q]] = q - qh[q] + 1;
+}
+
+template <typename Policy>
+__global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
+                            const uint32_t* __restrict__ qh, const uint32_t* __restrict__ sz,
+                            const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ Gs, uint32_t nl, int B,
+                            uint32_t* __restrict__ rank, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                            uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs,
+                            uint32_t* __restrict__ NGz, Policy pol)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nl) return;
+    const uint32_t x = LV[q];
+    const uint32_t s = (uint32_t)(LK[q] >> B);  // group start slot (a member of the group)
+    const uint32_t q0 = lpos[s];                // first index of the group among the large ones
+    const uint32_t lt = qh[q] - q0, c = sz[qh[q]];
+    const uint32_t gs = Gs[s];
+    const uint32_t p = s + (q - q0);
+    rank[x] = gs + lt + 1;
+    SA[gs + (q - q0)] = x;
+    keep[p] = (c > 1 && pol.open(x)) ? 1u : 0u;
+    NV[p] = x;
+    NHs[p] = s + lt;
+    NGs[p] = gs + lt;
+    NGz[p] = c;
+}
+
+// kept members that will be in large groups next round
+static __global__ void k_next_large(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ NGz, uint32_t m,
+                                    uint32_t small_max, uint32_t* __restrict__ f)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m) return;
+    f[p] = (keep[p] && NGz[p] > small_max) ? 1u : 0u;
+}
+
+// compaction of the kept members (whole sub-groups are kept or dropped, so a sub-group's
+// first slot is kept with it and maps through pos as well)
+static __global__ void k_dbl_compact(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                                     const uint32_t* __restrict__ NV, const uint32_t* __restrict__ NHs,
+                                     const uint32_t* __restrict__ NGs, const uint32_t* __restrict__ NGz, uint32_t m,
+                                     uint32_t* __restrict__ V, uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs,
+                                     uint32_t* __restrict__ Gz)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m || !keep[p]) return;
+    const uint32_t o = pos[p];
+    V[o] = NV[p];
+    Hs[o] = pos[NHs[p]];
+    Gs[o] = NGs[p];
+    Gz[o] = NGz[p];
 }
 
 struct DoublingResult {
@@ -152,81 +227,94 @@ struct DoublingResult {
 // Refine the groups of an initial order.  K0/V0 hold the initial keys/elements of N items,
 // already sorted (equal keys = one group); K1/V1 are scratch of N items.  `advance(pol, Vact,
 // m)` is called after each round's secondary keys were taken, with the still-active elements.
+// On return rank[x] = 1 + global position of x's group head and SA is the sorted order.
+//
+// Per round: secondary keys, one kernel that sorts every small group in place and directly
+// writes ranks, SA slots and the sub-group records, a radix sort for large groups, then two
+// scans and the compaction; one host read-back per round (active and large counts).
 template <typename Policy, typename Advance>
 DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V1, uint32_t N, Policy& pol,
                              Advance advance, uint32_t* rank, uint32_t* SA)
 {
     DoublingResult res;
-    uint32_t* G0 = S.buf<uint32_t>(B_ACT_POS, N);
-    uint32_t* G1 = S.buf<uint32_t>(B_ACT_POS2, N);
-    uint32_t* hidx = S.buf<uint32_t>(B_ACT_HEAD, N);
+    uint32_t* V = S.buf<uint32_t>(B_ACT_X2, N);
+    uint32_t* Hs = S.buf<uint32_t>(B_ACT_POS, N);
+    uint32_t* Gs = S.buf<uint32_t>(B_ACT_POS2, N);
+    uint32_t* Gz = S.buf<uint32_t>(B_ACT_HEAD, N);
+    uint32_t* NV = V1;
+    uint32_t* NHs = S.buf<uint32_t>(B_DB_GID, N);
+    uint32_t* NGs = S.buf<uint32_t>(B_DB_GOFS, N + 1);
+    uint32_t* NGz = S.buf<uint32_t>(B_DB_NGZ, N);
     uint32_t* keep = S.buf<uint32_t>(B_ACT_FLAG, N);
     uint32_t* pos = S.buf<uint32_t>(B_ACT_X, N);
-    uint32_t* Vc = S.buf<uint32_t>(B_ACT_X2, N);
     uint32_t* key2 = S.buf<uint32_t>(B_DB_KEY2, N);
-    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, N);
-    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, N + 1);
     uint32_t* large = S.buf<uint32_t>(B_DB_LARGE, N);
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-
     const int B = bits_for((uint64_t)N + 1);
+
+    // initial groups: runs of equal keys
+    {
+        uint32_t* hf = keep;
+        uint32_t* gid1 = pos;
+        uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS0, N + 1);
+        k_dbl_headflag<<<grid_for(N), kBlock, 0, S.stream>>>(K0, N, hf);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_sum<uint32_t>(S, hf, gid1, N);
+        k_grp_offsets<<<grid_for(N), kBlock, 0, S.stream>>>(hf, gid1, N, gofs);
+        MIBWT_LAUNCH_CHECK();
+        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NV, NHs, NGs,
+                                                                 NGz, pol);
+        MIBWT_LAUNCH_CHECK();
+        res.bytes += (double)N * (8 * 2 + 4 * 12);
+    }
     uint32_t m = N;
-    k_iota<<<grid_for(m), kBlock, 0, S.stream>>>(G0, m);
-    MIBWT_LAUNCH_CHECK();
-    uint64_t* K = K0;
-    uint32_t* V = V0;
     for (;;) {
-        // ---- ranks and SA slots of the current order; which groups stay active
-        k_dbl_headflag<<<grid_for(m), kBlock, 0, S.stream>>>(K, m, hidx);
+        // ---- compaction of the kept sub-groups; next round's counts
+        k_next_large<<<grid_for(m), kBlock, 0, S.stream>>>(keep, NGz, m, kSmallGroup, large);
         MIBWT_LAUNCH_CHECK();
-        inclusive_max<uint32_t>(S, hidx, hidx, m);
-        k_dbl_update<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(K, V, G0, hidx, m, rank, SA, keep, pol);
-        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, large, large, m, sc + SC_LARGE);
         exclusive_sum<uint32_t>(S, keep, pos, m, sc + SC_ACTIVE);
-        uint32_t m2 = 0;
-        S.read(&m2, sc + SC_ACTIVE);
-        res.rounds++;
-        res.bytes += (double)m * (8 * 2 + 4 * 8);
-        if (m2 == 0) break;
-        k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(V, G0, keep, pos, m, Vc, G1);
-        MIBWT_LAUNCH_CHECK();
-        std::swap(G0, G1);
-        m = m2;
-        // ---- group structure of the active elements
-        k_grp_heads<<<grid_for(m), kBlock, 0, S.stream>>>(Vc, rank, m, keep);
-        MIBWT_LAUNCH_CHECK();
-        inclusive_sum<uint32_t>(S, keep, gid1, m, sc + SC_GROUPS);
-        k_grp_offsets<<<grid_for(m), kBlock, 0, S.stream>>>(keep, gid1, m, gofs);
-        MIBWT_LAUNCH_CHECK();
-        // ---- order each group by the rank of the next position
-        k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(Vc, m, rank, pol, key2);
-        MIBWT_LAUNCH_CHECK();
-        advance(pol, Vc, m);
-        k_seg_small<<<grid_for(m), kBlock, 0, S.stream>>>(Vc, key2, gid1, gofs, m, kSmallGroup, V1, K0, large);
-        MIBWT_LAUNCH_CHECK();
-        exclusive_sum<uint32_t>(S, large, pos, m, sc + SC_LARGE);
         uint32_t hv[SC_COUNT_];
         S.read(hv, sc, SC_COUNT_);
-        const uint32_t nl = hv[SC_LARGE];
-        res.bytes += (double)m * (4 * 9 + 8);
+        const uint32_t m2 = hv[SC_ACTIVE], nl = hv[SC_LARGE];
+        res.rounds++;
+        if (m2 == 0) break;
+        k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, m, V, Hs, Gs, Gz);
+        MIBWT_LAUNCH_CHECK();
+        res.bytes += (double)m * (4 * 12) + (double)m2 * 16;
+        m = m2;
+        // ---- order each group by the rank of the next position
+        k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, m, rank, pol, key2);
+        MIBWT_LAUNCH_CHECK();
+        advance(pol, V, m);
+        k_seg_small<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, m, kSmallGroup, rank, SA, keep,
+                                                                  NV, NHs, NGs, NGz, large, pol);
+        MIBWT_LAUNCH_CHECK();
+        res.bytes += (double)m * (4 * 12);
         if (nl > 0) {
+            uint32_t* lpos = pos;
+            exclusive_sum<uint32_t>(S, large, lpos, m);
             uint64_t* LK0 = S.buf<uint64_t>(B_LG_K0, nl);
             uint64_t* LK1 = S.buf<uint64_t>(B_LG_K1, nl);
             uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
             uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
-            uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
-            k_seg_large_gather<<<grid_for(m), kBlock, 0, S.stream>>>(Vc, key2, gid1, large, pos, m, B, LK0, LV0, LI);
+            uint32_t* qh = S.buf<uint32_t>(B_LG_I, nl);
+            uint32_t* sz = S.buf<uint32_t>(B_LG_SZ, nl);
+            k_seg_large_gather<<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, key2, large, lpos, m, B, LK0, LV0);
             MIBWT_LAUNCH_CHECK();
-            const int kb = B + bits_for(hv[SC_GROUPS]);
+            const int kb = B + bits_for(m);
             radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, kb);
-            k_seg_large_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, B, V1, K0);
+            k_lg_heads<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, nl, qh);
+            MIBWT_LAUNCH_CHECK();
+            inclusive_max<uint32_t>(S, qh, qh, nl);
+            k_lg_sizes<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, qh, nl, sz);
+            MIBWT_LAUNCH_CHECK();
+            k_lg_update<Policy><<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, qh, sz, lpos, Gs, nl, B, rank, SA,
+                                                                       keep, NV, NHs, NGs, NGz, pol);
             MIBWT_LAUNCH_CHECK();
             res.large_items += nl;
-            res.bytes += (double)nl * (12 * 2 * ((kb + 7) / 8) + 32);
+            res.bytes += (double)nl * (12 * 2 * ((kb + 7) / 8) + 48);
         }
-        // the compaction reads V and writes Vc, the refinement reads Vc and writes V1
-        K = K0;
-        V = V1;
     }
     return res;
 }

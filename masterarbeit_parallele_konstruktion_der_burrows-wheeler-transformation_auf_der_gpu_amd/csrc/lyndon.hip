@@ -335,10 +335,19 @@ __global__ void k_lyn_initkey(const uint8_t* __restrict__ T, uint64_t n, const u
     V[i] = (uint32_t)i;
 }
 
+// fingerprint of every label P_i (compared before any longest-common-prefix search: most tie
+// groups of a collection are copies of one label)
+__global__ void k_label_hash(PrefixHash ph, const uint32_t* __restrict__ lam, uint64_t n, uint64_t* __restrict__ F)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) F[i] = ph.sub((uint32_t)i, lam[i]);
+}
+
 struct LabelCmp {
     const uint8_t* T;
     const uint32_t* lam;
     PrefixHash ph;
+    const uint64_t* F;  // label fingerprints
     // -1 / 0 / +1 for P_x against P_y, knowing their first `known` symbols agree
     __device__ int operator()(uint32_t x, uint32_t y, uint32_t known) const
     {
@@ -346,7 +355,7 @@ struct LabelCmp {
         const uint32_t lx = lam[x], ly = lam[y];
         const uint32_t m = lx < ly ? lx : ly;
         uint32_t lo = known < m ? known : m;
-        if (lx == ly && ph.sub(x, lx) == ph.sub(y, ly)) return 0;
+        if (lx == ly && F[x] == F[y]) return 0;
         if (ph.sub(x, m) == ph.sub(y, m)) {
             lo = m;
         } else {
@@ -361,34 +370,6 @@ struct LabelCmp {
         return lx < ly ? -1 : (lx > ly ? 1 : 0);
     }
 };
-
-// Groups of equal initial keys whose members reach past the packed prefix: every member
-// counts the members ordered before it with the comparator (small groups only).
-__global__ void k_label_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
-                              const uint32_t* __restrict__ gofs, uint32_t m, uint32_t known, LabelCmp cmp,
-                              uint32_t* __restrict__ V2, uint32_t* __restrict__ eqhead, uint32_t* __restrict__ large)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t g = gid1[i] - 1;
-    const uint32_t s = gofs[g], e = gofs[g + 1];
-    if (e - s > kSmallGroup) {
-        large[i] = 1;
-        return;
-    }
-    large[i] = 0;
-    const uint32_t x = V[i];
-    uint32_t before = 0, equal_before = 0;
-    for (uint32_t j = s; j < e; j++) {
-        if (j == i) continue;
-        const int c = cmp(V[j], x, known);
-        before += (c < 0) || (c == 0 && j < i);
-        equal_before += (c == 0 && j < i);
-    }
-    V2[s + before] = x;
-    // the first of a run of equal labels starts a new label group
-    eqhead[s + before] = (equal_before == 0) ? 1u : 0u;
-}
 
 // Groups of the current label order: gid1 (1-based, inclusive scan of heads) and gofs.
 __global__ void k_lab_heads_from_keys(const uint64_t* __restrict__ K, uint64_t n, uint32_t* __restrict__ hf)
@@ -416,9 +397,23 @@ __global__ void k_lab_classify(const uint32_t* __restrict__ V, const uint32_t* _
 
 // Small tie groups: comparator counting sort; settled groups are copied.  hf receives the
 // heads of the refined order (first of every run of equal labels).
+// label length and fingerprint of every position in the current order (the comparisons of a
+// tie group then read them coalesced; most members of a collection's groups are equal labels)
+__global__ void k_lab_gather(const uint32_t* __restrict__ V, const uint32_t* __restrict__ lam,
+                             const uint64_t* __restrict__ F, uint64_t n, uint32_t* __restrict__ GL,
+                             uint64_t* __restrict__ GF)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t x = V[i];
+    GL[i] = lam[x];
+    GF[i] = F[x];
+}
+
 __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                             const uint32_t* __restrict__ gofs, const uint8_t* __restrict__ cls, uint64_t n,
-                            uint32_t known, LabelCmp cmp, uint32_t* __restrict__ V2, uint32_t* __restrict__ hf)
+                            uint32_t known, LabelCmp cmp, const uint32_t* __restrict__ GL,
+                            const uint64_t* __restrict__ GF, uint32_t* __restrict__ V2, uint32_t* __restrict__ hf)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -431,10 +426,12 @@ __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __re
         hf[i] = (i == s) ? 1u : 0u;
         return;
     }
+    const uint32_t lx = GL[i];
+    const uint64_t fx = GF[i];
     uint32_t before = 0, equal_before = 0;
     for (uint32_t j = s; j < e; j++) {
         if (j == i) continue;
-        const int c = cmp(V[j], x, known);
+        const int c = (GL[j] == lx && GF[j] == fx) ? 0 : cmp(V[j], x, known);
         before += (c < 0) || (c == 0 && j < (uint32_t)i);
         equal_before += (c == 0 && j < (uint32_t)i);
     }
@@ -613,7 +610,10 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "label fingerprint collisions persist");
         if (!pre_fresh) pre = prefix_hashes(S, T, n, base);
         pre_fresh = false;
-        LabelCmp cmp{T, lam, PrefixHash{pre}};
+        uint64_t* LF = S.buf<uint64_t>(B_LY_LABH, n);
+        k_label_hash<<<grid_for(n), kBlock, 0, S.stream>>>(PrefixHash{pre}, lam, n, LF);
+        MIBWT_LAUNCH_CHECK();
+        LabelCmp cmp{T, lam, PrefixHash{pre}, LF};
         // working copies: V (order) and hf (heads) of the packed-key order
         uint32_t* V = Vl;
         MIBWT_HIP(hipMemcpyAsync(V, V0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
@@ -633,7 +633,11 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
             S.read(hv, sc, SC_COUNT_);
             const uint32_t nl = hv[SC_LARGE];
             uint32_t* V2 = (V == Vl) ? V1 : Vl;
-            k_lab_small<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, cls, n, known, cmp, V2, H);
+            uint32_t* GL = S.buf<uint32_t>(B_LY_GL, n);
+            uint64_t* GF = S.buf<uint64_t>(B_LY_GF, n);
+            k_lab_gather<<<grid_for(n), kBlock, 0, S.stream>>>(V, lam, LF, n, GL, GF);
+            MIBWT_LAUNCH_CHECK();
+            k_lab_small<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, cls, n, known, cmp, GL, GF, V2, H);
             MIBWT_LAUNCH_CHECK();
             if (nl > 0) {
                 uint64_t* LK0 = S.buf<uint64_t>(B_LG_K0, nl);

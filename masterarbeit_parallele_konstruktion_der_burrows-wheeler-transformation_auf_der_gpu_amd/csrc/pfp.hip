@@ -510,14 +510,8 @@ __global__ void k_dict_freq(const uint32_t* __restrict__ head, uint32_t du, uint
 // Segment limits of a string family.  Dictionary: the suffix at DT position x ends at the
 // end of its phrase.  Parse: one string of length d.
 struct LimDict {
-    const uint32_t* owner;
-    const uint64_t* off;
-    const uint32_t* len;
-    __device__ __forceinline__ uint32_t operator()(uint32_t x) const
-    {
-        const uint32_t u = owner[x];
-        return (uint32_t)off[u] + len[u];
-    }
+    const uint32_t* lim;  // end (exclusive) of the phrase of each DT position, precomputed
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return lim[x]; }
 };
 struct LimParse {
     uint32_t d;
@@ -540,7 +534,8 @@ __global__ void k_dt_initkey(const uint8_t* __restrict__ T, uint64_t n,
                              const uint32_t* __restrict__ owner, const uint64_t* __restrict__ off,
                              const uint32_t* __restrict__ len, const uint32_t* __restrict__ start,
                              const uint32_t* __restrict__ code, int bits, int per,
-                             uint32_t L, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
+                             uint32_t L, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
+                             uint32_t* __restrict__ lim)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
@@ -563,6 +558,7 @@ __global__ void k_dt_initkey(const uint8_t* __restrict__ T, uint64_t n,
     }
     K[x] = key;
     V[x] = x;
+    lim[x] = (uint32_t)off[u] + len[u];
 }
 
 // Initial parse key: `per` consecutive phrase ranks (+1; 0 past the end) of B bits each.
@@ -611,15 +607,6 @@ DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t*
 
 // --------------------------------------------------------------------- 5. phrase order
 
-__global__ void k_phrase_start_flag(const uint32_t* __restrict__ SA, const uint32_t* __restrict__ owner,
-                                    const uint64_t* __restrict__ off, uint32_t L, uint32_t* __restrict__ F)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= L) return;
-    const uint32_t x = SA[g];
-    F[g] = (x == (uint32_t)off[owner[x]]) ? 1u : 0u;
-}
-
 __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __restrict__ off,
                           const uint32_t* __restrict__ Fs, uint32_t du, uint32_t* __restrict__ lexrank)
 {
@@ -637,10 +624,17 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 //   bit  41      whole phrase (j == 0).
 constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
+// record and group head of dictionary position x: one 16-byte load gathers both
+struct AsmRec {
+    uint64_t r;     // packed record above
+    uint32_t head;  // sorted position of the group head (rank - 1)
+    uint32_t pad;
+};
+
 __global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq, uint32_t L, int W,
-                         uint64_t* __restrict__ rec)
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
+                         const uint32_t* __restrict__ rank, uint32_t L, int W, AsmRec* __restrict__ rec)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L) return;
@@ -651,19 +645,20 @@ __global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restri
     if (rem > (uint32_t)W) r |= kRecAlpha;
     if (j == 0) r |= kRecWhole;
     else if (rem > (uint32_t)W) r |= (uint64_t)T[start[u] + j - 1] << 32;
-    rec[x] = r;
+    rec[x] = AsmRec{r, rank[x] - 1, 0};
 }
 
-// Sorted order: gather each position's record and its group head once.
-__global__ void k_asm_gather(const uint32_t* __restrict__ SA, const uint32_t* __restrict__ rank,
-                             const uint64_t* __restrict__ rec, uint32_t L, uint64_t* __restrict__ R,
-                             uint32_t* __restrict__ Hd)
+// Sorted order: gather each position's record and its group head once (also gives the
+// whole-phrase flags in sorted order, i.e. the lexicographic phrase order).
+__global__ void k_asm_gather(const uint32_t* __restrict__ SA, const AsmRec* __restrict__ rec, uint32_t L,
+                             uint64_t* __restrict__ R, uint32_t* __restrict__ Hd, uint32_t* __restrict__ whole)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
-    const uint32_t x = SA[g];
-    R[g] = rec[x];
-    Hd[g] = rank[x] - 1;
+    const AsmRec a = rec[SA[g]];
+    R[g] = a.r;
+    Hd[g] = a.head;
+    whole[g] = (a.r & kRecWhole) ? 1u : 0u;
 }
 
 // a group is mixed when a member's preceding character differs from its head's
@@ -739,8 +734,8 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict_
 }
 
 // mixed-group members in dictionary order (grouped by phrase)
-__global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
-                          const uint8_t* __restrict__ gmix, uint32_t L, uint32_t* __restrict__ mxf)
+__global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint8_t* __restrict__ gmix, uint32_t L,
+                          uint32_t* __restrict__ mxf)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x > L) return;
@@ -748,17 +743,17 @@ __global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __re
         mxf[x] = 0;
         return;
     }
-    const uint64_t r = rec[x];
-    mxf[x] = ((r & kRecAlpha) && !(r & kRecWhole) && gmix[rank[x] - 1]) ? 1u : 0u;
+    const AsmRec a = rec[x];
+    mxf[x] = ((a.r & kRecAlpha) && !(a.r & kRecWhole) && gmix[a.head]) ? 1u : 0u;
 }
 
-__global__ void k_mx_write(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
-                           const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs, uint32_t L,
-                           uint64_t* __restrict__ mx)
+__global__ void k_mx_write(const AsmRec* __restrict__ rec, const uint32_t* __restrict__ mxf,
+                           const uint32_t* __restrict__ mxs, uint32_t L, uint64_t* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L || !mxf[x]) return;
-    mx[mxs[x]] = ((uint64_t)(rank[x] - 1) << 9) | ((rec[x] >> 32) & 0xFF);
+    const AsmRec a = rec[x];
+    mx[mxs[x]] = ((uint64_t)a.head << 9) | ((a.r >> 32) & 0xFF);
 }
 
 // per distinct phrase: first mixed entry, number of mixed entries, whole-phrase group
@@ -767,14 +762,14 @@ struct PhraseRec {
 };
 
 __global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                             const uint32_t* __restrict__ mxs, const uint32_t* __restrict__ rank, uint32_t du,
+                             const uint32_t* __restrict__ mxs, const AsmRec* __restrict__ rec, uint32_t du,
                              PhraseRec* __restrict__ pr)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= du) return;
     const uint32_t o = (uint32_t)off[u];
     const uint32_t a = mxs[o], b = mxs[o + len[u]];
-    pr[u] = PhraseRec{a, b - a, rank[o] - 1, 0};
+    pr[u] = PhraseRec{a, b - a, rec[o].head, 0};
 }
 
 // entries per parse-suffix rank r: the occurrence k = psa[r]-1 contributes its whole-phrase
@@ -797,20 +792,55 @@ __device__ __forceinline__ uint64_t whole_phrase_entry(const uint32_t* __restric
     return ((uint64_t)alpha0 << 9) | ((k == 0) ? 0x100ull : 0ull) | c;
 }
 
-__global__ void k_q_emit(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
-                         const PhraseRec* __restrict__ pr, const uint64_t* __restrict__ mx,
-                         const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
-                         const uint32_t* __restrict__ qo, uint64_t* __restrict__ ent)
+// Load-balanced generation of the entries: every block writes kQBlock consecutive entries.
+// k_q_blocks records, per block, the parse-suffix rank whose entries contain the block's first
+// output; the block then maps each output o to its rank r (bisection over the ranks' offsets
+// in LDS) and to j = o - qo[r]: j == 0 is the occurrence's whole-phrase entry, j >= 1 the
+// (j-1)-th mixed suffix of its phrase.  Writes are fully coalesced.
+constexpr int kQBlock = 1024;
+
+__global__ void k_q_blocks(const uint32_t* __restrict__ qo, const uint32_t* __restrict__ qc, uint32_t d,
+                           uint32_t* __restrict__ rb)
 {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= d) return;
-    const uint32_t k1 = psa[r];
-    if (k1 == 0) return;
-    const uint32_t k = k1 - 1;
-    const PhraseRec q = pr[pid[k]];
-    uint32_t o = qo[r];
-    ent[o++] = whole_phrase_entry(E, T, n, W, k, q.alpha0);
-    for (uint32_t i = 0; i < q.mx_count; i++) ent[o++] = mx[q.mx_begin + i];
+    if (r >= d || qc[r] == 0) return;
+    const uint32_t a = qo[r], e = a + qc[r];  // outputs [a, e)
+    for (uint32_t b = (a + kQBlock - 1) / kQBlock; (uint64_t)b * kQBlock < e; b++) rb[b] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict__ psa, uint32_t d,
+                                                    const uint32_t* __restrict__ pid,
+                                                    const PhraseRec* __restrict__ pr,
+                                                    const uint64_t* __restrict__ mx, const uint32_t* __restrict__ E,
+                                                    const uint8_t* __restrict__ T, uint64_t n, int W,
+                                                    const uint32_t* __restrict__ qo, const uint32_t* __restrict__ rb,
+                                                    uint32_t total, uint64_t* __restrict__ ent)
+{
+    __shared__ uint32_t lq[kQBlock + 2];
+    const uint32_t o0 = blockIdx.x * kQBlock;
+    const uint32_t r0 = rb[blockIdx.x];
+    // every rank but one (the suffix starting at phrase 0) has at least one entry, so
+    // kQBlock + 2 ranks cover the block's outputs
+    const uint32_t cnt = (d - r0 < (uint32_t)kQBlock + 2) ? d - r0 : (uint32_t)kQBlock + 2;
+    for (uint32_t t = threadIdx.x; t < cnt; t += kBlock) lq[t] = qo[r0 + t];
+    __syncthreads();
+    for (uint32_t t = threadIdx.x; t < (uint32_t)kQBlock; t += kBlock) {
+        const uint32_t o = o0 + t;
+        if (o >= total) break;
+        // last local rank with qo <= o
+        uint32_t a = 0, b = cnt - 1;
+        while (a < b) {
+            const uint32_t m = (a + b + 1) >> 1;
+            if (lq[m] <= o) a = m;
+            else b = m - 1;
+        }
+        // ranks without entries share their successor's offset: take the last of equal offsets
+        const uint32_t r = r0 + a;
+        const uint32_t j = o - lq[a];
+        const uint32_t k = psa[r] - 1;
+        const PhraseRec q = pr[pid[k]];
+        ent[o] = (j == 0) ? whole_phrase_entry(E, T, n, W, k, q.alpha0) : mx[q.mx_begin + j - 1];
+    }
 }
 
 // the last occurrence has no next parse suffix; its whole-phrase group is a singleton
@@ -848,6 +878,18 @@ __global__ void k_u32_to_u64(const uint32_t* __restrict__ a, uint64_t* __restric
 }
 
 // --------------------------------------------------------------------- driver
+
+// Bits of the packed initial keys of the dictionary and parse sorts (MIBWT_INIT_KEY_BITS
+// overrides it for tuning runs).
+static int init_key_bits()
+{
+    int v = 64;
+    if (const char* e = getenv("MIBWT_INIT_KEY_BITS")) {
+        const int x = atoi(e);
+        if (x >= 8 && x <= 64) v = x;
+    }
+    return v;
+}
 
 // Trigger end positions of the staged text into B_TRIG (room for one more entry, the last
 // phrase end); returns their number.  With `codes` the dense alphabet codes are computed
@@ -1047,11 +1089,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     // alphabet variant packs two symbols per radix digit, PFPs16:995-1002; here a 64-bit key
     // holds as many codes as fit, 21 for DNA)
     const int abits = bits_for((uint64_t)sigma + 1);
-    const int aper = 64 / abits;
+    // initial key width: whole radix passes trade against one more doubling round
+    const int key_cap = init_key_bits();
+    const int aper = std::max(1, key_cap / abits);
+    uint32_t* dlim = S.buf<uint32_t>(B_DT_LIM, L);
     k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, codes, abits, aper,
-                                                        L, K0, V0);
+                                                        L, K0, V0, dlim);
     MIBWT_LAUNCH_CHECK();
-    LimDict limd{owner, doff, dlen};
+    LimDict limd{dlim};
     DoublingResult rd = prefix_doubling<LimDict>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper, limd,
                                                  drank, dsa);
     S.phase_end(ph, rd.bytes);
@@ -1059,8 +1104,15 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
 
     // ---- 5. phrase order and parse suffix array
     ph = S.phase_begin("psort");
+    // per dictionary position a 16-byte record (frequency, preceding character, flags, group
+    // head), gathered once into sorted order; its whole-phrase flags give the phrase order
+    AsmRec* rec = S.buf<AsmRec>(B_AS_REC, L);
+    uint64_t* R = S.buf<uint64_t>(B_AS_R, L);
+    uint32_t* Hd = S.buf<uint32_t>(B_AS_HD, L);
     uint32_t* F = S.buf<uint32_t>(B_G_ECNT, L);
-    k_phrase_start_flag<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, owner, doff, L, F);
+    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, drank, L, W, rec);
+    MIBWT_LAUNCH_CHECK();
+    k_asm_gather<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, rec, L, R, Hd, F);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, F, F, L);
     uint32_t* lexrank = S.buf<uint32_t>(B_LEXRANK, du);
@@ -1071,7 +1123,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     V0 = S.buf<uint32_t>(B_SORT_V0, d);
     V1 = S.buf<uint32_t>(B_SORT_V1, d);
     const int B0 = bits_for((uint64_t)du + 1);
-    const int pper = 64 / B0;
+    const int pper = std::max(1, key_cap / B0);
     k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, pper, K0, V0);
     MIBWT_LAUNCH_CHECK();
     uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);
@@ -1083,13 +1135,6 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
 
     // ---- 6. BWT assembly
     ph = S.phase_begin("assemble");
-    uint64_t* rec = S.buf<uint64_t>(B_AS_REC, L);
-    uint64_t* R = S.buf<uint64_t>(B_AS_R, L);
-    uint32_t* Hd = S.buf<uint32_t>(B_AS_HD, L);
-    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, L, W, rec);
-    MIBWT_LAUNCH_CHECK();
-    k_asm_gather<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, drank, rec, L, R, Hd);
-    MIBWT_LAUNCH_CHECK();
     uint8_t* gmix = S.buf<uint8_t>(B_G_MIX, L);
     fill<uint8_t>(S, gmix, L, (uint8_t)0);
     k_asm_mix<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, L, gmix);
@@ -1108,16 +1153,16 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_LAUNCH_CHECK();
     uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, drank, gmix, L, mxf);
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, gmix, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
     S.read(&nmx, sc + SC_ENTRIES);
     uint64_t* mx = S.buf<uint64_t>(B_MX_ENT, nmx + 1);
-    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(rec, drank, mxf, mxs, L, mx);
+    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, L, mx);
     MIBWT_LAUNCH_CHECK();
     PhraseRec* prec = S.buf<PhraseRec>(B_AS_PREC, du);
-    k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, drank, du, prec);
+    k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, du, prec);
     MIBWT_LAUNCH_CHECK();
     uint32_t* qc = S.buf<uint32_t>(B_Q_CNT, d);
     uint32_t* qo = S.buf<uint32_t>(B_Q_OFF, d);
@@ -1129,8 +1174,15 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     const uint32_t ne = hv[SC_ENTRIES] + 1;
     uint64_t* ent0 = S.buf<uint64_t>(B_SORT_K0, ne);
     uint64_t* ent1 = S.buf<uint64_t>(B_SORT_K1, ne);
-    k_q_emit<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, ent0);
-    MIBWT_LAUNCH_CHECK();
+    const uint32_t nq = ne - 1;  // entries of the parse suffixes; the last occurrence adds one
+    if (nq > 0) {
+        const uint32_t qblocks = ceil_div(nq, (uint32_t)kQBlock);
+        uint32_t* rb = S.buf<uint32_t>(B_Q_BLK, qblocks);
+        k_q_blocks<<<grid_for(d), kBlock, 0, S.stream>>>(qo, qc, d, rb);
+        MIBWT_LAUNCH_CHECK();
+        k_q_expand<<<qblocks, kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, rb, nq, ent0);
+        MIBWT_LAUNCH_CHECK();
+    }
     k_q_last<<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
     MIBWT_LAUNCH_CHECK();
     fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);

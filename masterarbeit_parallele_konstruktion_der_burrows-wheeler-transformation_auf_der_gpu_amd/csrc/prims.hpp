@@ -439,13 +439,18 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
                                                       Chain c, Emit emit)
 {
     static_assert(sizeof(V) <= sizeof(K), "values are staged in the key buffer");
-    __shared__ uint32_t wcnt[kWaves][kRadix];
+    static_assert(kSortItems % 2 == 0, "slots are packed in pairs");
+    // per-wave digit counters; once the slots are known the same bytes hold the digit of
+    // every slot (4 KiB either way), which keeps the block at 38 KiB of LDS (4 blocks per CU)
+    __shared__ uint32_t wcnt_dig[kWaves * kRadix];
+    static_assert(kWaves * kRadix * 4 == kSortTile, "digit bytes alias the counters");
+    uint32_t(*wcnt)[kRadix] = reinterpret_cast<uint32_t(*)[kRadix]>(wcnt_dig);
+    uint8_t* sdig = reinterpret_cast<uint8_t*>(wcnt_dig);
     __shared__ uint32_t dstart[kRadix];
-    __shared__ uint32_t dglob[kRadix];
+    __shared__ uint32_t doff[kRadix];  // output position - tile slot, per digit (mod 2^32)
     __shared__ uint32_t scan_lds[kWaves];
     __shared__ uint32_t slot;
     __shared__ K stage[kSortTile];
-    __shared__ uint8_t sdig[kSortTile];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
@@ -454,15 +459,17 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     const uint64_t tile_base = (uint64_t)tile * kSortTile;
     const uint64_t base = tile_base + (uint64_t)w * kWaveTile + lane;
     K k[kSortItems];
-    V v[kSortItems];
-    uint16_t rank[kSortItems];
+    uint32_t rk[kSortItems / 2];  // two 16-bit ranks (later: tile slots) per register
     const uint64_t lt = lanemask_lt();
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
         const uint64_t idx = base + (uint64_t)i * 64;
         const bool valid = idx < n;
         k[i] = valid ? keys[idx] : K(0);
-        if (Emit::kValues) v[i] = (valid && vals) ? vals[idx] : V(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        const bool valid = base + (uint64_t)i * 64 < n;
         const uint32_t d = digit_of(k[i], shift, mask);
         uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -475,7 +482,9 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
         const int leader = valid ? (__ffsll((unsigned long long)peers) - 1) : 0;
         if (valid && lane == leader) old = atomicAdd(&wcnt[w][d], (uint32_t)__popcll(peers));
         old = __shfl(old, leader, 64);
-        rank[i] = valid ? (uint16_t)(old + (uint32_t)__popcll(peers & lt)) : (uint16_t)0;
+        const uint32_t r = valid ? (old + (uint32_t)__popcll(peers & lt)) : 0u;
+        if (i & 1) rk[i >> 1] |= r << 16;
+        else rk[i >> 1] = r;
     }
     __syncthreads();
     {
@@ -488,7 +497,8 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
             run += cq;
         }
         uint32_t tot;
-        dstart[d] = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
+        const uint32_t ds = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
+        dstart[d] = ds;
         const uint32_t gstart = block_exclusive_scan(hist_pass[d], 0u, OpSum{}, scan_lds, &tot);
         uint32_t excl = 0;
         if (tile == 0) {
@@ -498,38 +508,48 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
             excl = chain_lookback_digit(c, tile, d);
             chain_publish(c, tile * kRadix + d, 2, excl + run);
         }
-        dglob[d] = gstart + excl;
+        doff[d] = gstart + excl - ds;
     }
     __syncthreads();
-    // rank[i] becomes the item's slot in the tile
+    // ranks become the items' slots in the tile
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
         const uint64_t idx = base + (uint64_t)i * 64;
         if (idx < n) {
             const uint32_t d = digit_of(k[i], shift, mask);
-            const uint32_t p = dstart[d] + wcnt[w][d] + rank[i];
-            rank[i] = (uint16_t)p;
+            const uint32_t r = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
+            const uint32_t p = dstart[d] + wcnt[w][d] + r;
+            rk[i >> 1] = (i & 1) ? ((rk[i >> 1] & 0xFFFFu) | (p << 16)) : ((rk[i >> 1] & 0xFFFF0000u) | p);
+        }
+    }
+    __syncthreads();  // the counters are dead from here on: their bytes take the slot digits
+    // keys go through LDS in slot order
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        if (base + (uint64_t)i * 64 < n) {
+            const uint32_t p = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
             stage[p] = k[i];
-            sdig[p] = (uint8_t)d;
+            sdig[p] = (uint8_t)digit_of(k[i], shift, mask);
         }
     }
     __syncthreads();
     const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)kSortTile ? (n - tile_base) : kSortTile);
-    for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) {
-        const uint32_t d = sdig[p];
-        emit.key((uint64_t)dglob[d] + (p - dstart[d]), stage[p]);
-    }
+    for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.key(p + doff[sdig[p]], stage[p]);
     if (Emit::kValues && vals) {
+        // values are read only now, which keeps the ranking phase's register footprint small
+        V v[kSortItems];
+#pragma unroll
+        for (int i = 0; i < kSortItems; i++) {
+            const uint64_t idx = base + (uint64_t)i * 64;
+            v[i] = (idx < n) ? vals[idx] : V(0);
+        }
         __syncthreads();
         V* sv = reinterpret_cast<V*>(stage);
 #pragma unroll
         for (int i = 0; i < kSortItems; i++)
-            if (base + (uint64_t)i * 64 < n) sv[rank[i]] = v[i];
+            if (base + (uint64_t)i * 64 < n) sv[(i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu)] = v[i];
         __syncthreads();
-        for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) {
-            const uint32_t d = sdig[p];
-            emit.val((uint64_t)dglob[d] + (p - dstart[d]), sv[p]);
-        }
+        for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.val(p + doff[sdig[p]], sv[p]);
     }
 }
 

@@ -1,0 +1,181 @@
+// Inverse BWT on gfx950 (libcubwt_unbwt, libcubwt.cuh:91-101; the reference declares it but its
+// kernels exist only as an undefined remnant, libcubwt.cu:2452-2968).
+//
+// The n+1 rows of the BWT matrix of T$: B[index] = '$' and the other rows hold the text's bytes
+// (libcubwt format: B[r] = L[r] for r < index, L[r-1] for r > index; reference format:
+// B[0] = O[index-1], B[r] = O[r-1]).  LF maps a row to the row of the preceding rotation; the
+// text is T[n-1-k] = B[r_k] with r_0 = 0, r_{k+1} = LF(r_k), ending at the '$' row after n steps.
+//
+//  1. LF by one stable 8-bit radix pass over the n byte rows (the '$' row sorts first): the
+//     pass's emitter stores rec[r] = LF(r) | B[r] << 32, so a walk step is one 8-byte load.
+//  2. The LF cycle is ranked by sampling: every kUnSample-th row is a splitter; each splitter
+//     walks to the next splitter (or the '$' row) and records the number of rows and the next
+//     splitter; the splitter list is ranked by pointer jumping (suffix sums, a fixed number of
+//     rounds); a second walk writes the text.  An input that is not a BWT with this primary
+//     index (LF cycle from row 0 shorter than n) is reported, not written.
+#include "prims.hpp"
+
+namespace mibwt {
+
+constexpr uint32_t kUnSample = 64;
+constexpr uint32_t kUnEnd = 0xFFFFFFFFu;
+
+__device__ __forceinline__ uint8_t bwt_row(const uint8_t* __restrict__ L, uint64_t r, uint64_t index, bool ref)
+{
+    if (ref) return (r == 0) ? L[index - 1] : L[r - 1];
+    return (r < index) ? L[r] : L[r - 1];
+}
+
+// keys of the n byte rows: byte << 32 | row
+__global__ void k_un_keys(const uint8_t* __restrict__ L, uint64_t n, uint64_t index, int ref,
+                          uint64_t* __restrict__ K)
+{
+    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const uint64_t r = (q < index) ? q : q + 1;  // skip the '$' row
+    K[q] = ((uint64_t)bwt_row(L, r, index, ref != 0) << 32) | r;
+}
+
+// last pass of the row sort: row r lands at sorted position pos + 1 ('$' is position 0)
+struct EmitLF {
+    static constexpr double bytes_per_item = 8;
+    static constexpr bool kValues = false;
+    uint64_t* rec;
+    __device__ __forceinline__ void val(uint64_t, uint32_t) const {}
+    __device__ __forceinline__ void key(uint64_t pos, uint64_t key) const
+    {
+        rec[(uint32_t)key] = (pos + 1) | ((key >> 32) << 32);
+    }
+};
+
+// walk from splitter s to the next splitter: rows visited, next splitter (kUnEnd: the '$' row)
+__global__ void k_un_walk1(const uint64_t* __restrict__ rec, uint64_t m, uint64_t index, uint32_t S,
+                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ w, uint32_t* __restrict__ bad)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint64_t r0 = (uint64_t)s * kUnSample;
+    if (r0 == index) {  // the '$' row ends the text
+        nxt[s] = kUnEnd;
+        w[s] = 0;
+        return;
+    }
+    uint64_t r = r0, steps = 0;
+    uint32_t next = kUnEnd;
+    for (;;) {
+        r = (uint32_t)rec[r];
+        steps++;
+        if (r == index) break;
+        if (r % kUnSample == 0) {
+            next = (uint32_t)(r / kUnSample);
+            break;
+        }
+        if (steps > m) {  // cannot happen for a permutation; guards the loop
+            atomicOr(bad, 1u);
+            break;
+        }
+    }
+    nxt[s] = next;
+    w[s] = (uint32_t)steps;
+}
+
+// one pointer-jumping round on suffix sums: d(s) += d(nxt(s)), nxt(s) = nxt(nxt(s))
+__global__ void k_un_jump(const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ d, uint32_t S,
+                          uint32_t* __restrict__ nxt2, uint32_t* __restrict__ d2)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint32_t t = nxt[s];
+    if (t == kUnEnd) {
+        nxt2[s] = kUnEnd;
+        d2[s] = d[s];
+        return;
+    }
+    d2[s] = d[s] + d[t];
+    nxt2[s] = nxt[t];
+}
+
+// second walk: splitter s starts at step k = n - dist(s) (dist = rows from s to the end); the
+// row at step k is the rotation starting at text position n - k and is preceded by T[n-1-k].
+// With U the text is written; with I (sampling rate `rate`) the rows of the positions that are
+// multiples of the rate (libcubwt_bwt_aux, libcubwt.cuh:79-89: the same convention as the
+// primary index, which is I[0]).
+__global__ void k_un_walk2(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ w,
+                           const uint32_t* __restrict__ dist, uint64_t n, uint32_t S, uint8_t* __restrict__ U,
+                           uint32_t* __restrict__ I, uint64_t rate)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || w[s] == 0) return;
+    uint64_t k = n - dist[s];
+    uint64_t r = (uint64_t)s * kUnSample;
+    for (uint32_t i = 0; i < w[s]; i++) {
+        const uint64_t x = rec[r];
+        if (k < n) {
+            if (U) U[n - 1 - k] = (uint8_t)(x >> 32);
+            if (I && k > 0 && (n - k) % rate == 0) I[(n - k) / rate] = (uint32_t)r;
+        }
+        k++;
+        r = (uint32_t)x;
+    }
+}
+
+// Inverse of the BWT dL[0..n) with primary index `index` into dU[0..n) (device buffers), or,
+// with dI, the sampled rows dI[p / rate] of the text positions p = 0, rate, 2 rate, ...
+// Returns LIBCUBWT_NO_ERROR or LIBCUBWT_BAD_PARAMETER when the input is not a BWT.
+int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64_t index64, bool ref_format,
+                  uint32_t* dI, int64_t rate)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    MIBWT_REQUIRE(index64 >= 1 && index64 <= n64, LIBCUBWT_BAD_PARAMETER, "primary index out of range");
+    const uint64_t n = (uint64_t)n64, index = (uint64_t)index64, m = n + 1;
+    S.phases_reset();
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    int ph = S.phase_begin("lf");
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
+    uint64_t* rec = S.buf<uint64_t>(B_UN_REC, m);
+    k_un_keys<<<grid_for(n), kBlock, 0, S.stream>>>(dL, n, index, ref_format ? 1 : 0, K0);
+    MIBWT_LAUNCH_CHECK();
+    fill<uint64_t>(S, rec + index, 1, 0ull);  // LF('$' row) = row 0; its byte field is unused
+    EmitLF em{rec};
+    radix_sort<uint64_t, uint32_t, EmitLF>(S, K0, K1, nullptr, nullptr, n, 32, 40, &em);
+    S.phase_end(ph, 8.0 * n * 4);
+
+    ph = S.phase_begin("rank");
+    const uint32_t Sn = (uint32_t)ceil_div(m, (uint64_t)kUnSample);
+    uint32_t* nxt0 = S.buf<uint32_t>(B_UN_NXT0, Sn);
+    uint32_t* nxt1 = S.buf<uint32_t>(B_UN_NXT1, Sn);
+    uint32_t* w = S.buf<uint32_t>(B_UN_W, Sn);
+    uint32_t* d0 = S.buf<uint32_t>(B_UN_D0, Sn);
+    uint32_t* d1 = S.buf<uint32_t>(B_UN_D1, Sn);
+    fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+    k_un_walk1<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, m, index, Sn, nxt0, w, sc + SC_COLLISION);
+    MIBWT_LAUNCH_CHECK();
+    MIBWT_HIP(hipMemcpyAsync(d0, w, sizeof(uint32_t) * Sn, hipMemcpyDeviceToDevice, S.stream));
+    for (uint64_t span = 1; span < Sn; span *= 2) {
+        k_un_jump<<<grid_for(Sn), kBlock, 0, S.stream>>>(nxt0, d0, Sn, nxt1, d1);
+        MIBWT_LAUNCH_CHECK();
+        std::swap(nxt0, nxt1);
+        std::swap(d0, d1);
+    }
+    // the walk from row 0 must reach the '$' row after exactly n byte rows
+    uint32_t hv[3];
+    MIBWT_HIP(hipMemcpyAsync(sc + SC_TOTAL, d0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(sc + SC_GROUPS, nxt0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    S.read(hv, sc + SC_COLLISION, 1);
+    S.read(hv + 1, sc + SC_TOTAL, 1);
+    S.read(hv + 2, sc + SC_GROUPS, 1);
+    MIBWT_REQUIRE(hv[0] == 0 && hv[1] == (uint32_t)n && hv[2] == kUnEnd, LIBCUBWT_BAD_PARAMETER,
+                  "not a BWT with this primary index");
+    S.phase_end(ph, 16.0 * m);
+
+    ph = S.phase_begin("text");
+    if (dI) fill<uint32_t>(S, dI, 1, (uint32_t)index);  // position 0: the primary index
+    k_un_walk2<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, n, Sn, dU, dI, dI ? (uint64_t)rate : 1);
+    MIBWT_LAUNCH_CHECK();
+    S.phase_end(ph, 9.0 * m);
+    S.phases_collect();
+    return LIBCUBWT_NO_ERROR;
+}
+
+}  // namespace mibwt

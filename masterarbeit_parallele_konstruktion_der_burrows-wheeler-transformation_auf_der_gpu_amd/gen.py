@@ -51,26 +51,29 @@ def repetitive_dna(base_len: int, copies: int, sub_rate: float, indel_rate: floa
     return ACGT[np.concatenate(parts)]
 
 
-def markov2_printable(n: int, seed: int, alpha: float = 0.5) -> np.ndarray:
+def markov2_printable(n: int, seed: int, alpha: float = 0.5, chains: int = 1024) -> np.ndarray:
+    """Order-2 Markov text over the 96 printable bytes 32..127: transition rows drawn from
+    Dirichlet(alpha) per two-symbol context; `chains` independent chains of equal length are
+    advanced together and concatenated, each step sampled by a 1024-bucket inverse-CDF table
+    per context (the buckets quantize the row's CDF at multiples of 1/1024)."""
     rng = np.random.default_rng(seed)
     k = 96
     probs = rng.dirichlet(np.full(k, alpha), size=k * k)
     cdf = np.cumsum(probs, axis=1)
     cdf[:, -1] = 1.0
-    out = np.empty(n, dtype=np.uint8)
-    u = rng.random(n)
-    a, b = int(rng.integers(0, k)), int(rng.integers(0, k))
-    # sequential chain; vectorised per block would change the distribution, so loop in chunks
-    flat = cdf.reshape(k * k, k)
-    ctx = a * k + b
-    chunk = 1 << 16
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        for i in range(s, e):
-            c = int(np.searchsorted(flat[ctx], u[i]))
-            out[i] = c
-            ctx = (ctx % k) * k + c
-    return out + 32
+    q = (np.arange(1024) + 0.5) / 1024.0
+    table = np.empty((k * k, 1024), dtype=np.uint8)
+    for c in range(k * k):
+        table[c] = np.minimum(np.searchsorted(cdf[c], q), k - 1)
+    steps = -(-n // chains)
+    out = np.empty((steps, chains), dtype=np.uint8)
+    ctx = rng.integers(0, k * k, size=chains)
+    for t in range(steps):
+        u = rng.integers(0, 1024, size=chains)
+        c = table[ctx, u]
+        out[t] = c
+        ctx = (ctx % k) * k + c
+    return (out.T.reshape(-1)[:n] + 32).astype(np.uint8)
 
 
 # Workloads as named in the bench/test configuration.
@@ -89,6 +92,9 @@ CONFIGS = {
 
 def workload(name: str, scale: float = 1.0) -> np.ndarray:
     """Text of a named workload; `scale` < 1 shrinks the base length (for quick tests)."""
+    if name == "markov256m":
+        # low-repetition control of the 1 GiB collection: 256 MiB order-2 Markov text
+        return markov2_printable(max(1 << 16, int((256 << 20) * scale)), SEED_BASE + 3)
     cfg = dict(CONFIGS[name])
     cfg["base_len"] = max(1024, int(cfg["base_len"] * scale))
     return repetitive_dna(**cfg)

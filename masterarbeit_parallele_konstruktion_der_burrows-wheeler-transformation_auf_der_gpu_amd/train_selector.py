@@ -77,10 +77,14 @@ def run_grid(mibwt, st, T: np.ndarray, grid):
     ms = np.zeros(len(grid))
     st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)  # warm the buffers
     for c, (W, P) in enumerate(grid):
-        st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
-        s = st.stats()
-        ms[c] = s["total_ms"]
-        Y[c] = [np.log(s["dict_length"] / n), np.log(s["phrases"] / n), np.log(s["total_ms"] / n)]
+        best = None
+        for _ in range(2):  # the faster of two builds: less timing noise in the targets
+            st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
+            s = st.stats()
+            if best is None or s["total_ms"] < best["total_ms"]:
+                best = s
+        ms[c] = best["total_ms"]
+        Y[c] = [np.log(best["dict_length"] / n), np.log(best["phrases"] / n), np.log(best["total_ms"] / n)]
     dT.free()
     dO.free()
     return X, Y, ms, float(ms[grid.index((5, 16))])

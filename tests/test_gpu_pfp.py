@@ -164,3 +164,25 @@ def test_narrow_fingerprint_sort_widens_on_collision(mibwt, monkeypatch):
         L, idx = st.bwt(T, "pfp", 5, 11)
     O, oidx = oracle.bwt(T)
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+
+
+@pytest.mark.parametrize("name,W,P", [("dna1g", 5, 16), ("dna1g", 35, 81), ("markov256m", 5, 16),
+                                      ("markov256m", 35, 81)])
+def test_large_workloads_roundtrip(mibwt, name, W, P):
+    # the 1 GiB collection and its 256 MiB low-repetition control: inverse BWT on the device,
+    # symbol histogram, and the same BWT for both (W,P)
+    from mibwt import gen
+    T = gen.workload(name)
+    n = len(T)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(n)
+    with mibwt.Storage(n) as st:
+        idx = st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
+        dT.free()
+        dU = mibwt.DeviceBuffer(n)
+        st.unbwt_device(dO.address, dU.address, n, idx)
+    O = dO.download()
+    assert (np.bincount(O, minlength=256) == np.bincount(T, minlength=256)).all()
+    assert (dU.download() == T).all()
+    dO.free()
+    dU.free()
