@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--wp", choices=["auto", "fixed"], default="auto",
                     help="auto: learned (W,P) choice timed per step (value), fixed W/P reported beside it")
     ap.add_argument("--lyndon-steps", type=int, default=3, help="timed Lyndon-grammar builds (0: skip)")
+    ap.add_argument("--grid-steps", type=int, default=0,
+                    help="also time every (W,P) candidate with this many builds (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a plain 256 MiB device copy kernel (byte-counter reference)")
     a = ap.parse_args()
@@ -157,6 +159,23 @@ def main():
     def step():
         return st.bwt_device(dT.address, dO.address, n, "pfp", W_run, P_run)
 
+    # every candidate timed alone: how far the learned choice is from the best grid point
+    grid = None
+    if a.grid_steps > 0:
+        res = {}
+        for (W, P) in mibwt.wp_grid():
+            ms_c, idx_c = run(lambda W=W, P=P: st.bwt_device(dT.address, dO.address, n, "pfp", W, P),
+                              a.grid_steps, 1, f"grid W={W},P={P}")
+            assert idx_c == idx, "primary index differs between (W,P) choices"
+            res[(W, P)] = ms_c
+        best = min(res, key=res.get)
+        grid = {"candidates": len(res), "best_wp": list(best), "best_ms": round(res[best], 4),
+                "chosen_build_ms": round(res[(W_run, P_run)], 4),
+                "chosen_over_best": round(res[(W_run, P_run)] / res[best], 4),
+                "learned_step_over_best": round(ms_per_step / res[best], 4),
+                "fixed_over_best": round(ms_fixed / res[best], 4),
+                "ms": {f"{W},{P}": round(v, 3) for (W, P), v in res.items()}}
+
     # dominant kernel: the LSD radix sort's digit pass, every launch bracketed with HIP events
     # on the storage stream during one extra (untimed) build with the fixed (W,P), the
     # configuration the hardware-counter profile in profiles/ was taken on
@@ -239,6 +258,7 @@ def main():
                      "ms_per_step": round(ms_fixed, 4),
                      "phases": {p["name"]: round(p["ms"], 3) for p in stats_fixed["phases"]}},
         "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
+        "grid": grid,
         "lyndon": lyndon,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -33,6 +33,19 @@ namespace {
 
 constexpr int64_t kMaxLength = (int64_t)INT32_MAX - (1 << 20);
 
+// Build of the staged text with the selector's (W,P); a choice whose dictionary outgrows the
+// 32-bit index range falls back to the reference default W=5,P=16.
+int64_t pfp_bwt_selected(Storage& S, uint8_t* dO, int64_t n, int W, int P)
+{
+    try {
+        return pfp_bwt_run(S, nullptr, false, dO, n, W, P);
+    } catch (const Error& e) {
+        if (e.code != LIBCUBWT_GPU_NOT_ENOUGH_MEMORY || (W == 5 && P == 16)) throw;
+        (void)hipGetLastError();
+        return pfp_bwt_run(S, nullptr, false, dO, n, 5, 16);
+    }
+}
+
 template <typename F>
 int64_t guarded(F&& f)
 {
@@ -148,7 +161,7 @@ int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t
         uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
         int W = 5, P = 16;
         selector_choose_device(*S, dT, (uint64_t)n, &W, &P, nullptr);
-        const int64_t idx = pfp_bwt_run(*S, nullptr, false, dO, n, W, P);
+        const int64_t idx = pfp_bwt_selected(*S, dO, n, W, P);
         copy_out_cubwt(*S, dO, L, wrap, n, idx);
         return idx;
     });
@@ -166,7 +179,7 @@ int64_t libcubwt_bwt_aux(void* device_storage, const uint8_t* T, uint8_t* L, int
         uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
         int W = 5, P = 16;
         selector_choose_device(*S, dT, (uint64_t)n, &W, &P, nullptr);
-        const int64_t idx = pfp_bwt_run(*S, nullptr, false, dO, n, W, P);
+        const int64_t idx = pfp_bwt_selected(*S, dO, n, W, P);
         // sampled rows from the LF walk over the BWT just built (still in HBM)
         const int64_t cnt = (n - 1) / r + 1;
         uint32_t* dI = S->buf<uint32_t>(B_UN_AUX, (size_t)cnt);
@@ -250,7 +263,7 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
             uint8_t* T = stage_text(*S, dT, false, (uint64_t)n);
             int w = 5, p = 16;
             selector_choose_device(*S, T, (uint64_t)n, &w, &p, nullptr);
-            idx = pfp_bwt_run(*S, nullptr, false, dO, n, w, p);
+            idx = pfp_bwt_selected(*S, dO, n, w, p);
         } else if (method == MIBWT_METHOD_PFP) {
             idx = pfp_bwt_run(*S, dT, false, dO, n, W, P);
         } else {
@@ -379,7 +392,7 @@ int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* 
                     uint8_t* dT = stage_text(*sub, T[i], true, (uint64_t)n[i]);
                     int w = 5, p = 16;
                     selector_choose_device(*sub, dT, (uint64_t)n[i], &w, &p, nullptr);
-                    const int64_t idx = pfp_bwt_run(*sub, nullptr, false, dO, n[i], w, p);
+                    const int64_t idx = pfp_bwt_selected(*sub, dO, n[i], w, p);
                     copy_out_cubwt(*sub, dO, L[i], wrap, n[i], idx);
                     return idx;
                 });

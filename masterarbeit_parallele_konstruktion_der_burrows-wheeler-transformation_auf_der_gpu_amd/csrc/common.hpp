@@ -113,6 +113,7 @@ struct Storage {
     uint32_t chain_epoch = 0;
     // selector sketch tables: epoch of the last use
     uint32_t sketch_epoch = 0;
+    uint64_t sel_model_version = 0;  // selector model resident in B_SEL_MP / B_SEL_MPB / B_SEL_NORM
 
     // device buffer #id with capacity >= bytes; contents are not preserved on growth
     template <typename T>
@@ -279,6 +280,10 @@ enum BufId : int {
     B_Q_CNT,
     B_Q_OFF,
     B_Q_BLK,
+    B_Q_PART,
+    B_DB_LPART,
+    B_E_HIST,
+    B_E_PART,
     B_UN_REC,
     B_UN_NXT0,
     B_UN_NXT1,
@@ -351,6 +356,11 @@ enum BufId : int {
     B_SEL_H1B,
     B_SEL_H2B,
     B_SEL_YB,
+    B_SEL_XRAW,
+    B_SEL_MP,
+    B_SEL_MPB,
+    B_SEL_NORM,
+    B_SEL_PICK,
     B_COUNT_
 };
 
@@ -366,6 +376,9 @@ enum ScalarSlot : int {
     SC_GROUPS,
     SC_LARGE,
     SC_SIGMA,
+    SC_DICTLEN,        // two slots: 64-bit dictionary length
+    SC_SPECIAL = SC_DICTLEN + 2,  // two slots: group and entry index of the primary-index entry
+    SC_EHSUM = SC_SPECIAL + 2,    // entries counted by the entry-sort histogram
     SC_COUNT_ = 16
 };
 

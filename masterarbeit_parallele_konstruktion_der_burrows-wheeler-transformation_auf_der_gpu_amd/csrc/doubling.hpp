@@ -43,12 +43,11 @@ static __global__ void k_grp_offsets(const uint32_t* __restrict__ hf, const uint
 }
 
 // Element i of the initial order: rank and SA slot, whether its group stays active, and the
-// group record (first slot, global start, size) carried to the compaction.
+// group size (the first compaction reads the rest of the group record from gid1 / gofs).
 template <typename Policy>
 __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                            const uint32_t* __restrict__ gofs, uint32_t m, uint32_t* __restrict__ rank,
-                           uint32_t* __restrict__ SA, uint32_t* __restrict__ keep, uint32_t* __restrict__ NV,
-                           uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
+                           uint32_t* __restrict__ SA, uint32_t* __restrict__ keep, uint32_t* __restrict__ NGz,
                            Policy pol)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -59,10 +58,24 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
     rank[x] = s + 1;
     SA[i] = x;
     keep[i] = (z > 1 && pol.open(x)) ? 1u : 0u;
-    NV[i] = x;
-    NHs[i] = s;
-    NGs[i] = s;
     NGz[i] = z;
+}
+
+// first compaction: the group record of a kept element from the initial groups
+static __global__ void k_dbl_compact_init(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
+                                          const uint32_t* __restrict__ V0, const uint32_t* __restrict__ gid1,
+                                          const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ NGz,
+                                          uint32_t m, uint32_t* __restrict__ V, uint32_t* __restrict__ Hs,
+                                          uint32_t* __restrict__ Gs, uint32_t* __restrict__ Gz)
+{
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m || !keep[p]) return;
+    const uint32_t o = pos[p];
+    const uint32_t s = gofs[gid1[p] - 1];
+    V[o] = V0[p];
+    Hs[o] = pos[s];
+    Gs[o] = s;
+    Gz[o] = NGz[p];
 }
 
 // ---- one refinement round over the m active elements (groups contiguous)
@@ -79,9 +92,11 @@ __global__ void k_key2(const uint32_t* __restrict__ V, uint32_t m, const uint32_
 
 // Small groups: each member counts the members ordered before it (key, then slot), moves to
 // that slot and records its new sub-group (members with an equal key): rank = global start of
-// the sub-group + 1, SA slot, and the sub-group record for the compaction.  The block's keys
-// are staged in LDS: groups that lie inside the block count there, groups that cross the
-// block's edge read the keys from memory.  Members of large groups are only flagged.
+// the sub-group + 1, SA slot, and the sub-group record for the compaction.  A small group that
+// has a member in the block lies inside [b0 - kSmallGroup, b0 + kBlock + kSmallGroup), so the
+// keys of that window are staged in LDS and every count reads LDS (members of one group read
+// the same words: broadcasts).  Members of large groups are only flagged.
+static_assert(kSmallGroup <= kBlock, "the LDS key window assumes small groups of at most one block");
 template <typename Policy>
 __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
                                                      const uint32_t* __restrict__ Gs, const uint32_t* __restrict__ Gz,
@@ -92,39 +107,32 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
                                                      uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
                                                      uint32_t* __restrict__ large, Policy pol)
 {
-    __shared__ uint32_t lk[kBlock];
+    constexpr uint32_t kWin = kBlock + 2 * kSmallGroup;
+    __shared__ uint32_t lk[kWin];
     const uint32_t b0 = blockIdx.x * kBlock;
-    const uint32_t i = b0 + threadIdx.x;
-    uint32_t s = 0, z = 0, k = 0;
-    if (i < m) {
-        s = Hs[i];
-        z = Gz[i];
-        k = key2[i];
+    const int64_t w0 = (int64_t)b0 - kSmallGroup;  // slot of lk[0]
+    for (uint32_t t = threadIdx.x; t < kWin; t += kBlock) {
+        const int64_t j = w0 + t;
+        lk[t] = (j >= 0 && j < (int64_t)m) ? key2[j] : 0u;
     }
-    lk[threadIdx.x] = k;
     __syncthreads();
+    const uint32_t i = b0 + threadIdx.x;
     if (i >= m) return;
-    if (z > small_max) {
+    const uint32_t z = Gz[i];
+    if (z > small_max || z > kSmallGroup) {
         large[i] = 1;
         return;
     }
     large[i] = 0;
+    const uint32_t s = Hs[i];
+    const uint32_t k = lk[i - w0];
     uint32_t lt = 0, eq_before = 0, eq = 0;
-    if (s >= b0 && s + z <= b0 + kBlock) {
-        const uint32_t ls = s - b0, li = i - b0;
-        for (uint32_t j = ls; j < ls + z; j++) {
-            const uint32_t kj = lk[j];
-            lt += (kj < k);
-            eq += (kj == k);
-            eq_before += (kj == k && j < li);
-        }
-    } else {
-        for (uint32_t j = s; j < s + z; j++) {
-            const uint32_t kj = key2[j];
-            lt += (kj < k);
-            eq += (kj == k);
-            eq_before += (kj == k && j < i);
-        }
+    const uint32_t ls = (uint32_t)((int64_t)s - w0), li = (uint32_t)((int64_t)i - w0);
+    for (uint32_t j = ls; j < ls + z; j++) {
+        const uint32_t kj = lk[j];
+        lt += (kj < k);
+        eq += (kj == k);
+        eq_before += (kj == k && j < li);
     }
     const uint32_t x = V[i];
     const uint32_t gs = Gs[i];
@@ -192,13 +200,83 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
     NGz[p] = c;
 }
 
-// kept members that will be in large groups next round
-static __global__ void k_next_large(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ NGz, uint32_t m,
-                                    uint32_t small_max, uint32_t* __restrict__ f)
+// Compaction positions of the kept members: pos = exclusive scan of keep (chained tiles, the
+// total into *active); per tile the number of kept members whose sub-group is large next round
+// (lpart, summed by k_sum_tiles).
+static __global__ __launch_bounds__(kBlock) void k_dbl_keep_scan(const uint32_t* __restrict__ keep,
+                                                                const uint32_t* __restrict__ NGz, uint32_t m,
+                                                                uint32_t small_max, uint32_t* __restrict__ pos,
+                                                                uint32_t* __restrict__ active,
+                                                                uint32_t* __restrict__ lpart, Chain c, uint32_t tiles)
 {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m) return;
-    f[p] = (keep[p] && NGz[p] > small_max) ? 1u : 0u;
+    __shared__ uint32_t slot;
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t tile = chain_ticket(c, tiles, &slot);
+    const uint64_t base = (uint64_t)tile * kChainTile + (uint64_t)threadIdx.x * kChainItems;
+    uint32_t v[kChainItems];
+    uint32_t nl = 0;
+    if (base + kChainItems <= m) {
+#pragma unroll
+        for (int q = 0; q < kChainItems / 4; q++) {
+            const uint4 a = *reinterpret_cast<const uint4*>(keep + base + 4 * q);
+            const uint4 z = *reinterpret_cast<const uint4*>(NGz + base + 4 * q);
+            v[4 * q] = a.x;
+            v[4 * q + 1] = a.y;
+            v[4 * q + 2] = a.z;
+            v[4 * q + 3] = a.w;
+            nl += (a.x && z.x > small_max) + (a.y && z.y > small_max) + (a.z && z.z > small_max) +
+                  (a.w && z.w > small_max);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kChainItems; i++) {
+            const bool in = base + i < m;
+            v[i] = in ? keep[base + i] : 0u;
+            nl += (v[i] && NGz[base + i] > small_max) ? 1u : 0u;
+        }
+    }
+    uint32_t acc = 0;
+#pragma unroll
+    for (int i = 0; i < kChainItems; i++) acc += v[i];
+    uint32_t tot;
+    uint32_t run = block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
+    const uint32_t pre = chain_prefix(c, tile, tot, OpSum{}, 0u, &slot);
+    run += pre;
+    if (base + kChainItems <= m) {
+#pragma unroll
+        for (int q = 0; q < kChainItems / 4; q++) {
+            uint32_t o[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                o[j] = run;
+                run += v[4 * q + j];
+            }
+            *reinterpret_cast<uint4*>(pos + base + 4 * q) = make_uint4(o[0], o[1], o[2], o[3]);
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < kChainItems; i++) {
+            if (base + i < m) pos[base + i] = run;
+            run += v[i];
+        }
+    }
+    uint32_t ltot;
+    (void)block_exclusive_scan(nl, 0u, OpSum{}, lds, &ltot);
+    if (threadIdx.x == 0) {
+        lpart[tile] = ltot;
+        if (tile == tiles - 1) *active = pre + tot;
+    }
+}
+
+static __global__ __launch_bounds__(kBlock) void k_sum_tiles(const uint32_t* __restrict__ part, uint32_t tiles,
+                                                            uint32_t* __restrict__ out)
+{
+    __shared__ uint32_t lds[kWaves];
+    uint32_t acc = 0;
+    for (uint32_t t = threadIdx.x; t < tiles; t += kBlock) acc += part[t];
+    uint32_t tot;
+    (void)block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
+    if (threadIdx.x == 0) *out = tot;
 }
 
 // compaction of the kept members (whole sub-groups are kept or dropped, so a sub-group's
@@ -230,8 +308,9 @@ struct DoublingResult {
 // On return rank[x] = 1 + global position of x's group head and SA is the sorted order.
 //
 // Per round: secondary keys, one kernel that sorts every small group in place and directly
-// writes ranks, SA slots and the sub-group records, a radix sort for large groups, then two
-// scans and the compaction; one host read-back per round (active and large counts).
+// writes ranks, SA slots and the sub-group records, a radix sort for large groups, then one
+// chained scan of the kept flags (which also counts next round's large-group members) and the
+// compaction; one host read-back per round (active and large counts).
 template <typename Policy, typename Advance>
 DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V1, uint32_t N, Policy& pol,
                              Advance advance, uint32_t* rank, uint32_t* SA)
@@ -252,34 +331,45 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     const int B = bits_for((uint64_t)N + 1);
 
-    // initial groups: runs of equal keys
+    // initial groups: runs of equal keys (gid1 lives in `large` until the first compaction)
+    uint32_t* gid1 = large;
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS0, N + 1);
     {
         uint32_t* hf = keep;
-        uint32_t* gid1 = pos;
-        uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS0, N + 1);
         k_dbl_headflag<<<grid_for(N), kBlock, 0, S.stream>>>(K0, N, hf);
         MIBWT_LAUNCH_CHECK();
         inclusive_sum<uint32_t>(S, hf, gid1, N);
         k_grp_offsets<<<grid_for(N), kBlock, 0, S.stream>>>(hf, gid1, N, gofs);
         MIBWT_LAUNCH_CHECK();
-        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NV, NHs, NGs,
-                                                                 NGz, pol);
+        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NGz, pol);
         MIBWT_LAUNCH_CHECK();
-        res.bytes += (double)N * (8 * 2 + 4 * 12);
+        res.bytes += (double)N * (8 * 2 + 4 * 9);
     }
+    bool first = true;
     uint32_t m = N;
     for (;;) {
         // ---- compaction of the kept sub-groups; next round's counts
-        k_next_large<<<grid_for(m), kBlock, 0, S.stream>>>(keep, NGz, m, kSmallGroup, large);
-        MIBWT_LAUNCH_CHECK();
-        exclusive_sum<uint32_t>(S, large, large, m, sc + SC_LARGE);
-        exclusive_sum<uint32_t>(S, keep, pos, m, sc + SC_ACTIVE);
+        {
+            const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, (uint32_t)kChainTile));
+            uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
+            const Chain c = S.chain(tiles);
+            k_dbl_keep_scan<<<tiles, kBlock, 0, S.stream>>>(keep, NGz, m, kSmallGroup, pos, sc + SC_ACTIVE, lpart, c,
+                                                            tiles);
+            MIBWT_LAUNCH_CHECK();
+            k_sum_tiles<<<1, kBlock, 0, S.stream>>>(lpart, tiles, sc + SC_LARGE);
+            MIBWT_LAUNCH_CHECK();
+        }
         uint32_t hv[SC_COUNT_];
         S.read(hv, sc, SC_COUNT_);
         const uint32_t m2 = hv[SC_ACTIVE], nl = hv[SC_LARGE];
         res.rounds++;
         if (m2 == 0) break;
-        k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, m, V, Hs, Gs, Gz);
+        if (first) {
+            k_dbl_compact_init<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, V0, gid1, gofs, NGz, m, V, Hs, Gs, Gz);
+            first = false;
+        } else {
+            k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, m, V, Hs, Gs, Gz);
+        }
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)m * (4 * 12) + (double)m2 * 16;
         m = m2;

@@ -48,27 +48,44 @@ __device__ __forceinline__ void phrase_bounds(const uint32_t* __restrict__ E, ui
     end = E[k];
 }
 
-// Fingerprint of the phrase [s, e] (virtual terminator beyond n).
+// 8 text bytes at word index k of a stream starting sh/8 bytes into word w[0].
+__device__ __forceinline__ uint64_t funnel8(uint64_t lo, uint64_t hi, int sh)
+{
+    return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
+}
+
+// Fingerprint of the phrase [s, e] (virtual terminator beyond n): 8-byte chunks at offsets
+// c = 0, 8, ... of the bytes inside the text, each mixed with its offset, then the number of
+// terminator positions.  The chunks are streamed as aligned words, 32 bytes per step with the
+// four loads issued together (the padded text copy makes the over-read safe).
 __device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint64_t n, uint32_t s, uint32_t e,
                                 uint64_t seed)
 {
     const uint32_t len = e - s + 1;
     const uint32_t text_end = (e < n) ? e + 1 : (uint32_t)n;  // exclusive
+    const uint32_t L = text_end - s;
     uint64_t h = seed ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
-    for (uint32_t p = s; p < text_end; p += 8) {
-        uint64_t w = load8(T, p);
-        const uint32_t c = text_end - p;
-        if (c < 8) w &= (~0ull) >> (64 - 8 * c);
-        h = mix64(h ^ (w + 0x2545F4914F6CDD1Dull * (p - s + 1)));
+    const uint64_t* w = reinterpret_cast<const uint64_t*>(T + (s & ~7u));
+    const int sh = (int)(s & 7u) * 8;
+    uint64_t prev = w[0];
+    for (uint32_t i = 0; i < L; i += 32) {
+        const uint32_t q = i >> 3;
+        uint64_t a[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) a[j] = w[q + 1 + j];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const uint32_t c = i + 8 * j;
+            if (c < L) {
+                uint64_t u = funnel8(j ? a[j - 1] : prev, a[j], sh);
+                if (L - c < 8) u &= (~0ull) >> (64 - 8 * (L - c));
+                h = mix64(h ^ (u + 0x2545F4914F6CDD1Dull * (c + 1)));
+            }
+        }
+        prev = a[3];
     }
     if (e >= n) h = mix64(h ^ 0xA24BAED4963EE407ull ^ (uint64_t)(e - n + 1));
     return h;
-}
-
-// 8 text bytes at word index k of a stream starting sh/8 bytes into word w[0].
-__device__ __forceinline__ uint64_t funnel8(uint64_t lo, uint64_t hi, int sh)
-{
-    return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
 }
 
 // Byte equality of two phrases.  Both are streamed as aligned words, 32 bytes per step with
@@ -437,43 +454,53 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, 
 // --------------------------------------------------------------------- 2/3. dictionary
 
 __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
-                              const uint32_t* __restrict__ E, uint32_t d, uint64_t seed,
+                              const uint32_t* __restrict__ E, uint32_t d, uint64_t seed, uint64_t hmask,
                               uint64_t* __restrict__ H, uint32_t* __restrict__ V)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
     uint32_t s, e;
     phrase_bounds(E, k, W, s, e);
-    H[k] = phrase_hash(T, n, s, e, seed);
+    H[k] = phrase_hash(T, n, s, e, seed) & hmask;
     V[k] = k;
 }
 
 // Phrase fingerprints are first sorted by their low 48 bits (6 radix passes).  Neighbours
-// with equal sort keys must be equal phrases: different 64-bit fingerprints under one sort key
-// (bit 1 of *collision) repeat the sort on all 64 bits, equal fingerprints of different
-// phrases (bit 0) re-seed the hash.
+// with equal sort keys are taken as equal phrases: different 64-bit fingerprints under one sort
+// key (bit 1 of *collision) repeat the sort on all 64 bits; every occurrence is then compared
+// byte by byte with its phrase's representative (k_dict_verify), and equal fingerprints of
+// different phrases (bit 0) re-seed the hash.
 constexpr int kDictKeyBits = 48;
 
-__global__ void k_dict_mark(const uint8_t* __restrict__ T, uint64_t n, int W,
-                            const uint32_t* __restrict__ E, const uint64_t* __restrict__ H,
-                            const uint32_t* __restrict__ V, uint32_t d, uint64_t key_mask,
+__global__ void k_dict_mark(const uint64_t* __restrict__ H, uint32_t d, uint64_t key_mask,
                             uint32_t* __restrict__ flag, uint32_t* __restrict__ collision)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d) return;
     uint32_t f = 1;
-    if (j > 0 && (H[j] & key_mask) == (H[j - 1] & key_mask)) {
-        f = 0;
-        if (H[j] != H[j - 1]) {
-            atomicOr(collision, 2u);
-        } else {
-            uint32_t s1, e1, s2, e2;
-            phrase_bounds(E, V[j], W, s1, e1);
-            phrase_bounds(E, V[j - 1], W, s2, e2);
-            if (!phrase_equal(T, n, s1, e1, s2, e2)) atomicOr(collision, 1u);
+    if (j > 0) {
+        const uint64_t a = H[j], b = H[j - 1];
+        if ((a & key_mask) == (b & key_mask)) {
+            f = 0;
+            if (a != b) atomicOr(collision, 2u);
         }
     }
     flag[j] = f;
+}
+
+// Occurrence k (text order: its own bytes stream) against the representative of its phrase
+// (few distinct phrases, cache resident); a mismatch is a fingerprint collision (bit 0).
+__global__ void k_dict_verify(const uint8_t* __restrict__ T, uint64_t n, int W, const uint32_t* __restrict__ E,
+                              uint32_t d, const uint32_t* __restrict__ pid, const uint32_t* __restrict__ dstart,
+                              const uint32_t* __restrict__ dlen, uint32_t* __restrict__ collision)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    uint32_t s, e;
+    phrase_bounds(E, k, W, s, e);
+    const uint32_t u = pid[k];
+    const uint32_t s2 = dstart[u];
+    if (s2 != s && !phrase_equal(T, n, s, e, s2, s2 + dlen[u] - 1)) atomicOr(collision, 1u);
 }
 
 // ids[j] = inclusive count of heads (1-based id of the distinct phrase at sorted position j)
@@ -687,10 +714,70 @@ __global__ void k_asm_counts(const uint64_t* __restrict__ R, const uint32_t* __r
     ecnt[g] = (alpha && ((r & kRecWhole) || gmix[Hd[g]])) ? f : 0u;
 }
 
+// Digit histograms of the entry sort, weighted from the groups instead of read from the
+// entries: every sorted position g contributes ecnt[g] entries keyed by its group head Hd[g]
+// (nondecreasing in g).  Each thread walks kEhPer consecutive positions and adds runs of equal
+// digits at once; per-block counts go to part[block][kEhPasses * 256].
+constexpr int kEhPer = 64, kEhPasses = 4;
+__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ ecnt,
+                                                      uint32_t L, int passes, uint32_t* __restrict__ part)
+{
+    __shared__ uint32_t h[kEhPasses * kRadix];
+    for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) h[i] = 0;
+    __syncthreads();
+    const uint64_t g0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kEhPer;
+    uint32_t prev[kEhPasses], run[kEhPasses];
+#pragma unroll
+    for (int p = 0; p < kEhPasses; p++) {
+        prev[p] = 0xFFFFFFFFu;
+        run[p] = 0;
+    }
+    for (int i = 0; i < kEhPer; i++) {
+        const uint64_t g = g0 + i;
+        if (g >= L) break;
+        const uint32_t c = ecnt[g];
+        if (c == 0) continue;
+        const uint32_t hd = Hd[g];
+#pragma unroll
+        for (int p = 0; p < kEhPasses; p++) {
+            if (p >= passes) break;
+            const uint32_t d = (hd >> (8 * p)) & 0xFFu;
+            if (d == prev[p]) {
+                run[p] += c;
+            } else {
+                if (run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
+                prev[p] = d;
+                run[p] = c;
+            }
+        }
+    }
+#pragma unroll
+    for (int p = 0; p < kEhPasses; p++)
+        if (p < passes && run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
+    __syncthreads();
+    for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) part[(uint64_t)blockIdx.x * kEhPasses * kRadix + i] = h[i];
+}
+
+// hist[i] = sum over blocks of part[block][i]: one block per counter
+__global__ __launch_bounds__(kBlock) void k_entry_hist_sum(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                          uint32_t* __restrict__ hist)
+{
+    __shared__ uint32_t red[kWaves];
+    const uint32_t i = blockIdx.x;
+    uint32_t s = 0;
+    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) s += part[(uint64_t)b * kEhPasses * kRadix + i];
+    uint32_t tot;
+    (void)block_exclusive_scan(s, 0u, OpSum{}, red, &tot);
+    if (threadIdx.x == 0) hist[i] = tot;
+}
+
 // Direct fill of the single-character groups, one wavefront per 64 consecutive sorted
-// positions: their runs are consecutive in the output, so the lanes sweep the wave's span
-// together (coalesced byte stores) and look their run up in LDS.  Also writes delta = slot
-// offset minus entry offset for every position.
+// positions: their slot ranges are consecutive in the output, so the lanes sweep the wave's
+// span together in 16-byte granules and look the runs up in LDS (one bisection per granule,
+// then a forward walk).  A granule whose bytes are all fill bytes of this wave is one 16-byte
+// store; otherwise its fill bytes are stored one by one (neighbouring waves and the later
+// entry pass own the other bytes).  Also writes delta = slot offset minus entry offset for
+// every position.
 __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd,
                                                     const uint8_t* __restrict__ gmix, const uint32_t* __restrict__ goff,
                                                     const uint32_t* __restrict__ escan, uint32_t L, uint64_t n,
@@ -700,6 +787,7 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict_
     __shared__ uint8_t rc[kBlock];
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const int lane = threadIdx.x & 63, wb = threadIdx.x & ~63;
+    const bool aligned = ((uintptr_t)O & 15u) == 0;
     uint32_t st = 0, ln = 0;
     uint8_t ch = 0;
     if (g < L) {
@@ -720,16 +808,54 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict_
     const uint32_t nv = (g0 >= L) ? 0u : ((L - g0) < 64u ? (L - g0) : 64u);
     if (nv == 0) return;
     const uint32_t lo = rs[wb];
-    const uint32_t hi = rs[wb + nv - 1] + rl[wb + nv - 1];
-    for (uint32_t p = lo + lane; p < hi; p += 64) {
-        // last run start <= p (runs are ordered by start)
+    const uint32_t hi_run = rs[wb + nv - 1] + rl[wb + nv - 1];
+    const uint32_t hi = (hi_run < n) ? hi_run : (uint32_t)n;
+    if (hi <= lo) return;
+    for (uint32_t q = (lo >> 4) + lane; q <= ((hi - 1) >> 4); q += 64) {
+        const uint32_t p0 = q << 4;
+        const uint32_t first = (p0 > lo) ? p0 : lo;
+        // last run start <= first
         int a = 0, b = (int)nv - 1;
         while (a < b) {
             const int m = (a + b + 1) >> 1;
-            if (rs[wb + m] <= p) a = m;
+            if (rs[wb + m] <= first) a = m;
             else b = m - 1;
         }
-        if (p - rs[wb + a] < rl[wb + a] && p < n) O[p] = rc[wb + a];
+        uint32_t nxt = (a + 1 < (int)nv) ? rs[wb + a + 1] : 0xFFFFFFFFu;
+        {
+            // common case: the whole granule lies inside one fill run
+            const uint32_t re = rs[wb + a] + rl[wb + a];
+            if (aligned && first == p0 && p0 + 16 <= hi && p0 + 16 <= nxt && p0 + 16 <= re) {
+                const uint32_t c4 = (uint32_t)rc[wb + a] * 0x01010101u;
+                *reinterpret_cast<uint4*>(O + p0) = make_uint4(c4, c4, c4, c4);
+                continue;
+            }
+        }
+        uint32_t bytes[4] = {0, 0, 0, 0};
+        uint32_t mask = 0;
+#pragma unroll
+        for (int k = 0; k < 16; k++) {
+            const uint32_t p = p0 + k;
+            if (p >= first && p < hi) {
+                while (p >= nxt) {
+                    a++;
+                    nxt = (a + 1 < (int)nv) ? rs[wb + a + 1] : 0xFFFFFFFFu;
+                }
+                if (p - rs[wb + a] < rl[wb + a]) {
+                    bytes[k >> 2] |= (uint32_t)rc[wb + a] << (8 * (k & 3));
+                    mask |= 1u << k;
+                }
+            }
+        }
+        if (mask == 0xFFFFu && aligned) {
+            *reinterpret_cast<uint4*>(O + p0) = make_uint4(bytes[0], bytes[1], bytes[2], bytes[3]);
+        } else {
+            while (mask) {
+                const int k = __ffs(mask) - 1;
+                mask &= mask - 1;
+                O[p0 + k] = (uint8_t)(bytes[k >> 2] >> (8 * (k & 3)));
+            }
+        }
     }
 }
 
@@ -747,13 +873,27 @@ __global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint8_t* __restr
     mxf[x] = ((a.r & kRecAlpha) && !(a.r & kRecWhole) && gmix[a.head]) ? 1u : 0u;
 }
 
+// BWT entries: group (sorted position of the dictionary group head) above the output byte.
+// 64-bit entries keep bit 8 for the primary-index mark (group from bit 9); 32-bit entries
+// (groups below 2^24) hold group << 8 | byte and the primary index is counted separately.
+template <typename EntT>
+struct Ent {
+    static constexpr int kShift = sizeof(EntT) == 4 ? 8 : 9;
+    __device__ __forceinline__ static EntT make(uint32_t group, uint32_t c)
+    {
+        return ((EntT)group << kShift) | (EntT)c;
+    }
+    __device__ __forceinline__ static uint32_t group(EntT e) { return (uint32_t)(e >> kShift); }
+};
+
+template <typename EntT>
 __global__ void k_mx_write(const AsmRec* __restrict__ rec, const uint32_t* __restrict__ mxf,
-                           const uint32_t* __restrict__ mxs, uint32_t L, uint64_t* __restrict__ mx)
+                           const uint32_t* __restrict__ mxs, uint32_t L, EntT* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L || !mxf[x]) return;
     const AsmRec a = rec[x];
-    mx[mxs[x]] = ((uint64_t)a.head << 9) | ((a.r >> 32) & 0xFF);
+    mx[mxs[x]] = Ent<EntT>::make(a.head, (uint32_t)(a.r >> 32) & 0xFFu);
 }
 
 // per distinct phrase: first mixed entry, number of mixed entries, whole-phrase group
@@ -783,13 +923,26 @@ __global__ void k_q_count(const uint32_t* __restrict__ psa, uint32_t d, const ui
     qc[r] = (k1 == 0) ? 0u : 1u + pr[pid[k1 - 1]].mx_count;
 }
 
-__device__ __forceinline__ uint64_t whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
-                                                       uint64_t n, int W, uint32_t k, uint32_t alpha0)
+template <typename EntT>
+__device__ __forceinline__ EntT whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
+                                                   uint64_t n, int W, uint32_t k, uint32_t alpha0)
 {
     uint32_t s, e;
     phrase_bounds(E, k, W, s, e);
     const uint8_t c = (s > 0) ? T[s - 1] : T[n - 1];
-    return ((uint64_t)alpha0 << 9) | ((k == 0) ? 0x100ull : 0ull) | c;
+    EntT v = Ent<EntT>::make(alpha0, c);
+    if (sizeof(EntT) == 8 && k == 0) v |= (EntT)0x100;
+    return v;
+}
+
+// The primary-index entry (whole phrase of occurrence 0): its group and its index among the
+// entries (the first entry of parse-suffix rank prank[1] - 1, or the last entry if d == 1).
+__global__ void k_q_special(const uint32_t* __restrict__ prank, const uint32_t* __restrict__ pid, uint32_t d,
+                            const PhraseRec* __restrict__ pr, const uint32_t* __restrict__ qo, uint32_t last,
+                            uint32_t* __restrict__ special)
+{
+    special[0] = pr[pid[0]].alpha0;
+    special[1] = (d >= 2) ? qo[prank[1] - 1] : last;
 }
 
 // Load-balanced generation of the entries: every block writes kQBlock consecutive entries.
@@ -808,15 +961,18 @@ __global__ void k_q_blocks(const uint32_t* __restrict__ qo, const uint32_t* __re
     for (uint32_t b = (a + kQBlock - 1) / kQBlock; (uint64_t)b * kQBlock < e; b++) rb[b] = r;
 }
 
+template <typename EntT>
 __global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict__ psa, uint32_t d,
                                                     const uint32_t* __restrict__ pid,
                                                     const PhraseRec* __restrict__ pr,
-                                                    const uint64_t* __restrict__ mx, const uint32_t* __restrict__ E,
+                                                    const EntT* __restrict__ mx, const uint32_t* __restrict__ E,
                                                     const uint8_t* __restrict__ T, uint64_t n, int W,
                                                     const uint32_t* __restrict__ qo, const uint32_t* __restrict__ rb,
-                                                    uint32_t total, uint64_t* __restrict__ ent)
+                                                    uint32_t total, EntT* __restrict__ ent,
+                                                    const uint32_t* __restrict__ special, uint32_t* __restrict__ part)
 {
     __shared__ uint32_t lq[kQBlock + 2];
+    __shared__ uint32_t red[kWaves];
     const uint32_t o0 = blockIdx.x * kQBlock;
     const uint32_t r0 = rb[blockIdx.x];
     // every rank but one (the suffix starting at phrase 0) has at least one entry, so
@@ -824,6 +980,11 @@ __global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict_
     const uint32_t cnt = (d - r0 < (uint32_t)kQBlock + 2) ? d - r0 : (uint32_t)kQBlock + 2;
     for (uint32_t t = threadIdx.x; t < cnt; t += kBlock) lq[t] = qo[r0 + t];
     __syncthreads();
+    uint32_t g0 = 0, s0 = 0, before = 0;
+    if (part) {
+        g0 = special[0];
+        s0 = special[1];
+    }
     for (uint32_t t = threadIdx.x; t < (uint32_t)kQBlock; t += kBlock) {
         const uint32_t o = o0 + t;
         if (o >= total) break;
@@ -839,17 +1000,57 @@ __global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict_
         const uint32_t j = o - lq[a];
         const uint32_t k = psa[r] - 1;
         const PhraseRec q = pr[pid[k]];
-        ent[o] = (j == 0) ? whole_phrase_entry(E, T, n, W, k, q.alpha0) : mx[q.mx_begin + j - 1];
+        const EntT v = (j == 0) ? whole_phrase_entry<EntT>(E, T, n, W, k, q.alpha0) : mx[q.mx_begin + j - 1];
+        ent[o] = v;
+        if (part) {
+            // entries that the stable sort places before the primary-index entry
+            const uint32_t g = Ent<EntT>::group(v);
+            before += (g < g0 || (g == g0 && o < s0)) ? 1u : 0u;
+        }
+    }
+    if (part) {
+        uint32_t tot;
+        (void)block_exclusive_scan(before, 0u, OpSum{}, red, &tot);
+        if (threadIdx.x == 0) part[blockIdx.x] = tot;
     }
 }
 
 // the last occurrence has no next parse suffix; its whole-phrase group is a singleton
+template <typename EntT>
 __global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
                          const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
-                         uint32_t pos, uint64_t* __restrict__ ent)
+                         uint32_t pos, EntT* __restrict__ ent)
 {
     const uint32_t k = d - 1;
-    ent[pos] = whole_phrase_entry(E, T, n, W, k, pr[pid[k]].alpha0);
+    ent[pos] = whole_phrase_entry<EntT>(E, T, n, W, k, pr[pid[k]].alpha0);
+}
+
+// Primary index from 32-bit entries: 1 + delta(group) + number of entries sorted before the
+// primary-index entry (block counts of k_q_expand, summed by k_q_count_sum into *acc, plus the
+// last entry).
+constexpr int kPidxPer = 16;
+__global__ __launch_bounds__(kBlock) void k_q_count_sum(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                       uint32_t* __restrict__ acc)
+{
+    __shared__ uint32_t red[kWaves];
+    const uint32_t b0 = (blockIdx.x * kBlock + threadIdx.x) * kPidxPer;
+    uint32_t v = 0;
+#pragma unroll
+    for (int i = 0; i < kPidxPer; i++)
+        if (b0 + i < blocks) v += part[b0 + i];
+    uint32_t tot;
+    (void)block_exclusive_scan(v, 0u, OpSum{}, red, &tot);
+    if (threadIdx.x == 0 && tot) atomicAdd(acc, tot);
+}
+
+__global__ void k_q_pidx(const uint32_t* __restrict__ acc, const uint32_t* __restrict__ ent, uint32_t last,
+                         const uint32_t* __restrict__ special, const uint32_t* __restrict__ delta,
+                         uint32_t* __restrict__ pidx)
+{
+    const uint32_t g0 = special[0], s0 = special[1];
+    const uint32_t g = ent[last] >> 8;
+    const uint32_t tot = *acc + ((last != s0 && (g < g0 || (g == g0 && last < s0))) ? 1u : 0u);
+    *pidx = delta[g0] + tot + 1;
 }
 
 // last radix pass of the entry sort: slot = sorted position + delta(alpha)
@@ -867,6 +1068,22 @@ struct EmitBwt {
         if (slot >= n) return;  // guard: a slot past the output is a bug caught by the checks
         O[slot] = (uint8_t)(key & 0xFF);
         if (key & 0x100) *pidx = (uint32_t)slot + 1;
+    }
+};
+
+// the same for 32-bit entries (the primary index comes from k_q_pidx)
+struct EmitBwt32 {
+    static constexpr double bytes_per_item = 1 + 4;
+    static constexpr bool kValues = false;
+    const uint32_t* delta;
+    uint8_t* O;
+    uint64_t n;
+    __device__ __forceinline__ void val(uint64_t, uint32_t) const {}
+    __device__ __forceinline__ void key(uint64_t pos, uint32_t key) const
+    {
+        const uint64_t slot = pos + delta[key >> 8];
+        if (slot >= n) return;  // guard: a slot past the output is a bug caught by the checks
+        O[slot] = (uint8_t)(key & 0xFF);
     }
 };
 
@@ -1029,45 +1246,67 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         const int v = atoi(kb);
         if (v >= 8 && v <= 64) key_bits = v & ~7;
     }
+    int hash_bits = 64;
+    if (const char* hb = getenv("MIBWT_DICT_HASH_BITS")) {
+        // narrow first-attempt fingerprints collide and exercise the re-seeding path (tests)
+        const int v = atoi(hb);
+        if (v >= 1 && v <= 64) hash_bits = v;
+    }
     int dict_passes = 0;
+    uint32_t* pid = nullptr;
+    uint32_t *head = nullptr, *dstart = nullptr, *dlen = nullptr, *freq = nullptr;
+    uint64_t* doff = nullptr;
+    uint64_t L64 = 0;
     for (int attempt = 0;; attempt++) {
         MIBWT_REQUIRE(attempt < 6, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
-        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, K0, V0);
+        const uint64_t hmask = (attempt == 0 && hash_bits < 64) ? ((1ull << hash_bits) - 1) : ~0ull;
+        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, hmask, K0, V0);
         MIBWT_LAUNCH_CHECK();
         radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, key_bits);
         dict_passes += key_bits / 8;
         fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
         const uint64_t key_mask = (key_bits >= 64) ? ~0ull : ((1ull << key_bits) - 1);
-        k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, K0, V0, d, key_mask, flag, sc + SC_COLLISION);
+        k_dict_mark<<<grid_for(d), kBlock, 0, S.stream>>>(K0, d, key_mask, flag, sc + SC_COLLISION);
         MIBWT_LAUNCH_CHECK();
         inclusive_sum<uint32_t>(S, flag, K1 ? reinterpret_cast<uint32_t*>(K1) : nullptr, d, sc + SC_DISTINCT);
         uint32_t hv[SC_COUNT_];
         S.read(hv, sc, SC_COUNT_);
         du = hv[SC_DISTINCT];
-        if (hv[SC_COLLISION] == 0) break;
-        if (hv[SC_COLLISION] & 1u) seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
-        else key_bits = 64;
+        if (hv[SC_COLLISION] & 2u) {  // distinct fingerprints under one sort key: sort all 64 bits
+            key_bits = 64;
+            continue;
+        }
+        const uint32_t* ids = reinterpret_cast<const uint32_t*>(K1);
+        pid = S.buf<uint32_t>(B_PH_PID, d);
+        head = S.buf<uint32_t>(B_DU_HEAD, du);
+        dstart = S.buf<uint32_t>(B_DU_START, du);
+        dlen = S.buf<uint32_t>(B_DU_LEN, du);
+        freq = S.buf<uint32_t>(B_DU_FREQ, du);
+        k_dict_assign<<<grid_for(d), kBlock, 0, S.stream>>>(ids, V0, flag, E, d, W, pid, head, dstart, dlen);
+        MIBWT_LAUNCH_CHECK();
+        k_dict_verify<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, pid, dstart, dlen, sc + SC_COLLISION);
+        MIBWT_LAUNCH_CHECK();
+        k_dict_freq<<<grid_for(du), kBlock, 0, S.stream>>>(head, du, d, freq);
+        MIBWT_LAUNCH_CHECK();
+        uint64_t* dlen64 = S.buf<uint64_t>(B_DU_LEN64, du + 1);
+        doff = S.buf<uint64_t>(B_DU_OFF, du + 1);
+        k_u32_to_u64<<<grid_for(du + 1), kBlock, 0, S.stream>>>(dlen, dlen64, du);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint64_t>(S, dlen64, doff, du + 1);
+        // alphabet of the dictionary = alphabet of the text (the phrases cover it)
+        alphabet_codes(S, T, n, codes, dstart, dlen, du);
+        // one read-back: collision flag, dictionary length, alphabet size
+        MIBWT_HIP(hipMemcpyAsync(sc + SC_DICTLEN, doff + du, sizeof(uint64_t), hipMemcpyDeviceToDevice, S.stream));
+        MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+        S.read(hv, sc, SC_COUNT_);
+        if (hv[SC_COLLISION] & 1u) {  // equal fingerprints of different phrases: new seed
+            seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
+            continue;
+        }
+        L64 = (uint64_t)hv[SC_DICTLEN] | ((uint64_t)hv[SC_DICTLEN + 1] << 32);
+        sigma = hv[SC_SIGMA];
+        break;
     }
-    const uint32_t* ids = reinterpret_cast<const uint32_t*>(K1);
-    uint32_t* pid = S.buf<uint32_t>(B_PH_PID, d);
-    uint32_t* head = S.buf<uint32_t>(B_DU_HEAD, du);
-    uint32_t* dstart = S.buf<uint32_t>(B_DU_START, du);
-    uint32_t* dlen = S.buf<uint32_t>(B_DU_LEN, du);
-    uint32_t* freq = S.buf<uint32_t>(B_DU_FREQ, du);
-    k_dict_assign<<<grid_for(d), kBlock, 0, S.stream>>>(ids, V0, flag, E, d, W, pid, head, dstart, dlen);
-    MIBWT_LAUNCH_CHECK();
-    k_dict_freq<<<grid_for(du), kBlock, 0, S.stream>>>(head, du, d, freq);
-    MIBWT_LAUNCH_CHECK();
-    uint64_t* dlen64 = S.buf<uint64_t>(B_DU_LEN64, du + 1);
-    uint64_t* doff = S.buf<uint64_t>(B_DU_OFF, du + 1);
-    k_u32_to_u64<<<grid_for(du + 1), kBlock, 0, S.stream>>>(dlen, dlen64, du);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint64_t>(S, dlen64, doff, du + 1);
-    // alphabet of the dictionary = alphabet of the text (the phrases cover it)
-    alphabet_codes(S, T, n, codes, dstart, dlen, du);
-    uint64_t L64 = 0;
-    S.read(&L64, doff + du);
-    S.read(&sigma, codes + 256);
     MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "dictionary too large");
     const uint32_t L = (uint32_t)L64;
     S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 8.0 * d + 24.0 * d * dict_passes + 12.0 * d);
@@ -1143,6 +1382,21 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
     k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, L, goff, ecnt);
     MIBWT_LAUNCH_CHECK();
+    // digit histograms of the entry sort (bits of the group head), from the per-position counts
+    const int gbits = bits_for(L);
+    const int epasses = (gbits + kRadixBits - 1) / kRadixBits;
+    uint32_t* ehist = S.buf<uint32_t>(B_E_HIST, (size_t)kMaxPasses * kRadix);
+    const bool ehist_ok = epasses <= kEhPasses;
+    if (ehist_ok) {
+        const uint32_t eb = (uint32_t)ceil_div((uint64_t)L, (uint64_t)kBlock * kEhPer);
+        uint32_t* epart = S.buf<uint32_t>(B_E_PART, (size_t)eb * kEhPasses * kRadix);
+        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(Hd, ecnt, L, epasses, epart);
+        MIBWT_LAUNCH_CHECK();
+        k_entry_hist_sum<<<kEhPasses * kRadix, kBlock, 0, S.stream>>>(epart, eb, ehist);
+        MIBWT_LAUNCH_CHECK();
+        k_sum_tiles<<<1, kBlock, 0, S.stream>>>(ehist, kRadix, sc + SC_EHSUM);  // pass-0 total
+        MIBWT_LAUNCH_CHECK();
+    }
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
     exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t slots = 0;
@@ -1158,38 +1412,72 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
     S.read(&nmx, sc + SC_ENTRIES);
-    uint64_t* mx = S.buf<uint64_t>(B_MX_ENT, nmx + 1);
-    k_mx_write<<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, L, mx);
-    MIBWT_LAUNCH_CHECK();
     PhraseRec* prec = S.buf<PhraseRec>(B_AS_PREC, du);
-    k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, du, prec);
-    MIBWT_LAUNCH_CHECK();
     uint32_t* qc = S.buf<uint32_t>(B_Q_CNT, d);
     uint32_t* qo = S.buf<uint32_t>(B_Q_OFF, d);
-    k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, qc);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
-    uint32_t hv[SC_COUNT_];
-    S.read(hv, sc, SC_COUNT_);
-    const uint32_t ne = hv[SC_ENTRIES] + 1;
-    uint64_t* ent0 = S.buf<uint64_t>(B_SORT_K0, ne);
-    uint64_t* ent1 = S.buf<uint64_t>(B_SORT_K1, ne);
-    const uint32_t nq = ne - 1;  // entries of the parse suffixes; the last occurrence adds one
-    if (nq > 0) {
-        const uint32_t qblocks = ceil_div(nq, (uint32_t)kQBlock);
-        uint32_t* rb = S.buf<uint32_t>(B_Q_BLK, qblocks);
-        k_q_blocks<<<grid_for(d), kBlock, 0, S.stream>>>(qo, qc, d, rb);
+    uint32_t ne = 0;
+    // entries: (group, byte) in parse-suffix order, then one stable sort by group whose last
+    // pass writes the BWT bytes; 32-bit entries when the groups fit in 24 bits
+    auto assemble = [&](auto tag) {
+        using EntT = decltype(tag);
+        constexpr bool k32 = sizeof(EntT) == 4;
+        EntT* mx = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_MX_ENT, nmx + 1));
+        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, L, mx);
         MIBWT_LAUNCH_CHECK();
-        k_q_expand<<<qblocks, kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, rb, nq, ent0);
+        k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, du, prec);
         MIBWT_LAUNCH_CHECK();
-    }
-    k_q_last<<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
-    MIBWT_LAUNCH_CHECK();
-    fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
-    EmitBwt em{delta, dO, sc + SC_PIDX, n};
-    const int gbits = bits_for(L);
-    radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em);
-    S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + 8.0 * ne * 2 * ((gbits + 7) / 8) + 4.0 * d * 3);
+        k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, qc);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
+        uint32_t hv[SC_COUNT_];
+        S.read(hv, sc, SC_COUNT_);
+        ne = hv[SC_ENTRIES] + 1;
+        // the histograms from the group counts are used only if they count every entry
+        const uint32_t* eh = (ehist_ok && hv[SC_EHSUM] == ne) ? ehist : nullptr;
+        EntT* ent0 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K0, ne));
+        EntT* ent1 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K1, ne));
+        const uint32_t nq = ne - 1;  // entries of the parse suffixes; the last occurrence adds one
+        const uint32_t qblocks = nq > 0 ? ceil_div(nq, (uint32_t)kQBlock) : 0;
+        uint32_t* special = sc + SC_SPECIAL;
+        uint32_t* part = nullptr;
+        if (k32) {
+            part = S.buf<uint32_t>(B_Q_PART, qblocks + 1);
+            k_q_special<<<1, 1, 0, S.stream>>>(prank, pid, d, prec, qo, ne - 1, special);
+            MIBWT_LAUNCH_CHECK();
+        }
+        if (nq > 0) {
+            uint32_t* rb = S.buf<uint32_t>(B_Q_BLK, qblocks);
+            k_q_blocks<<<grid_for(d), kBlock, 0, S.stream>>>(qo, qc, d, rb);
+            MIBWT_LAUNCH_CHECK();
+            k_q_expand<EntT><<<qblocks, kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, rb, nq, ent0,
+                                                              special, part);
+            MIBWT_LAUNCH_CHECK();
+        }
+        k_q_last<EntT><<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
+        MIBWT_LAUNCH_CHECK();
+        if constexpr (k32) {
+            fill<uint32_t>(S, sc + SC_TOTAL, 1, 0u);
+            if (qblocks > 0) {
+                k_q_count_sum<<<ceil_div(qblocks, (uint32_t)(kBlock * kPidxPer)), kBlock, 0, S.stream>>>(part, qblocks,
+                                                                                                     sc + SC_TOTAL);
+                MIBWT_LAUNCH_CHECK();
+            }
+            k_q_pidx<<<1, 1, 0, S.stream>>>(sc + SC_TOTAL, ent0, ne - 1, special, delta, sc + SC_PIDX);
+            MIBWT_LAUNCH_CHECK();
+            EmitBwt32 em{delta, dO, n};
+            radix_sort<uint32_t, uint32_t, EmitBwt32>(S, ent0, ent1, nullptr, nullptr, ne, 8, 8 + gbits, &em, nullptr,
+                                                      eh);
+        } else {
+            fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
+            EmitBwt em{delta, dO, sc + SC_PIDX, n};
+            radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em, nullptr,
+                                                    eh);
+        }
+    };
+    if (gbits <= 24) assemble(uint32_t{});
+    else assemble(uint64_t{});
+    const double esize = (gbits <= 24) ? 4.0 : 8.0;
+    S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + esize * ne * 2 * ((gbits + 7) / 8) + 4.0 * d * 3);
     S.stats.alpha_groups = L;
     S.stats.mixed_entries = ne;
 

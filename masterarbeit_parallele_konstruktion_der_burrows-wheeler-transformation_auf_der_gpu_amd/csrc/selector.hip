@@ -55,13 +55,6 @@ __device__ __host__ inline float bf16_to_f32(uint16_t h)
 
 // ------------------------------------------------------------------------ features
 
-struct SampleCounts {
-    uint32_t hist1[256];
-    uint32_t trig[kNumCand];
-    uint32_t positions;
-    uint32_t sketch[3];  // selected positions, distinct 16-mer and 64-mer fingerprints
-};
-
 // Divisibility by the odd grid moduli without division: v % P == 0 <=> v * P^-1 <= lim
 // (mod 2^32 for W <= 20, mod 2^64 for W = 35).
 struct GridDiv {
@@ -69,81 +62,132 @@ struct GridDiv {
     uint64_t inv64[kNumP], lim64[kNumP];
 };
 
-constexpr int kSampleParts = 4;                          // blocks per chunk
-constexpr int kSamplePer = kChunk / (kBlock * kSampleParts);  // 16 consecutive positions per thread
+// Per-chunk partial counts of the sample: trigger counts of the 41 candidates, sampled
+// positions, and 1 + the last text byte when the chunk holds position n-1 (else 0).
+constexpr int kSpTrig = 0, kSpPos = kNumCand, kSpLast = kNumCand + 1, kSpWords = kNumCand + 2;
+constexpr int kSampleBlock = 1024;                   // threads per chunk
+constexpr int kSamplePer = kChunk / kSampleBlock;    // 16 consecutive positions per thread
+constexpr int kSampleHalo = 48;                      // bytes loaded before a thread's positions
 
-// Sample statistics over kSampleChunks chunks of kChunk bytes (kSampleParts blocks each): byte and
-// byte-pair histograms (the pair histogram lives in LDS as packed 16-bit counters), and the
-// trigger count of every grid candidate from rolled exact window values.
-__global__ __launch_bounds__(kBlock) void k_feat_sample(const uint8_t* __restrict__ T, uint64_t n,
-                                                       uint64_t stride, GridDiv gd, uint32_t* __restrict__ hist2,
-                                                       SampleCounts* __restrict__ out)
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Sample statistics over up to kSampleChunks chunks of kChunk bytes at a stride that is a
+// multiple of 16 (one block per chunk): the byte-pair histogram (LDS, packed 16-bit counters,
+// then added to hist2) and the trigger count of every candidate from rolled exact window values.
+// Bytes outside [0, n) read as 0; windows ending before position W-1 are not counted.
+__global__ __launch_bounds__(kSampleBlock) void k_feat_sample(const uint8_t* __restrict__ T, uint64_t n,
+                                                             uint64_t stride, GridDiv gd,
+                                                             uint32_t* __restrict__ hist2,
+                                                             uint32_t* __restrict__ part)
 {
     __shared__ uint32_t h2[65536 / 2];
-    __shared__ uint32_t h1[256];
-    __shared__ uint32_t tc[kNumCand];
-    for (int i = threadIdx.x; i < 65536 / 2; i += kBlock) h2[i] = 0;
-    h1[threadIdx.x] = 0;
-    if (threadIdx.x < kNumCand) tc[threadIdx.x] = 0;
+    __shared__ uint32_t tc[(kNumCand + 1) / 2 + 2];
+    {
+        uint4* z = reinterpret_cast<uint4*>(h2);
+        for (int i = threadIdx.x; i < 65536 / 8; i += kSampleBlock) z[i] = make_uint4(0, 0, 0, 0);
+        if (threadIdx.x < (kNumCand + 1) / 2 + 2) tc[threadIdx.x] = 0;
+    }
     __syncthreads();
-    const uint64_t cstart = (uint64_t)(blockIdx.x / kSampleParts) * stride;
-    const uint64_t p0 = cstart + (uint64_t)(blockIdx.x % kSampleParts) * (kChunk / kSampleParts) +
-                        (uint64_t)threadIdx.x * kSamplePer;
+    const uint64_t p0 = (uint64_t)blockIdx.x * stride + (uint64_t)threadIdx.x * kSamplePer;
+    // bytes T[p0-48 .. p0+15] in w[0..15], T[p0+16] in nx (0 outside the text)
+    uint32_t w[16];
+    uint32_t nx = 0;
+    if (p0 >= kSampleHalo && p0 + kSamplePer + 1 <= n) {
+        const uint4* v = reinterpret_cast<const uint4*>(T + p0 - kSampleHalo);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint4 x = v[i];
+            w[4 * i] = x.x;
+            w[4 * i + 1] = x.y;
+            w[4 * i + 2] = x.z;
+            w[4 * i + 3] = x.w;
+        }
+        nx = T[p0 + kSamplePer];
+    } else {
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            uint32_t x = 0;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint64_t p = p0 - kSampleHalo + 4 * i + b;  // wraps for p0 < 48: then >= n
+                if (p0 + 4 * i + b >= kSampleHalo && p < n) x |= (uint32_t)T[p] << (8 * b);
+            }
+            w[i] = x;
+        }
+        if (p0 + kSamplePer < n) nx = T[p0 + kSamplePer];
+    }
+    auto byte_at = [&](int i) -> uint32_t { return (w[i >> 2] >> (8 * (i & 3))) & 0xFFu; };
     uint32_t cnt[kNumCand];
 #pragma unroll
     for (int c = 0; c < kNumCand; c++) cnt[c] = 0;
     uint32_t v5 = 0, v10 = 0, v20 = 0;
     uint64_t v35 = 0;
-    // windows ending before p0 (bytes before the text read as 0, windows there are skipped)
-    for (int k = 35; k >= 1; k--) {
-        const uint32_t c = (p0 >= (uint64_t)k && p0 - k < n) ? T[p0 - k] : 0u;
+#pragma unroll
+    for (int k = 35; k >= 1; k--) {  // windows ending at p0 - 1
+        const uint32_t c = byte_at(kSampleHalo - k);
         if (k <= 5) v5 = (v5 << 1) + c;
         if (k <= 10) v10 = (v10 << 1) + c;
         if (k <= 20) v20 = (v20 << 1) + c;
         v35 = (v35 << 1) + c;
     }
-    uint32_t pos = 0;
+    uint32_t pos = 0, last = 0;
+#pragma unroll
     for (int q = 0; q < kSamplePer; q++) {
         const uint64_t p = p0 + q;
-        if (p >= n) break;
-        const uint32_t c = T[p];
-        const uint32_t c5 = p >= 5 ? T[p - 5] : 0u, c10 = p >= 10 ? T[p - 10] : 0u;
-        const uint32_t c20 = p >= 20 ? T[p - 20] : 0u, c35 = p >= 35 ? T[p - 35] : 0u;
-        v5 = (v5 << 1) + c - (c5 << 5);
-        v10 = (v10 << 1) + c - (c10 << 10);
-        v20 = (v20 << 1) + c - (c20 << 20);
-        v35 = (v35 << 1) + c - ((uint64_t)c35 << 35);
-        atomicAdd(&h1[c], 1u);
-        if (p + 1 < n) {
-            const uint32_t pr = (c << 8) | T[p + 1];
-            atomicAdd(&h2[pr >> 1], 1u << (16 * (pr & 1u)));
-        }
-        pos++;
+        if (p < n) {
+            const uint32_t c = byte_at(kSampleHalo + q);
+            v5 = (v5 << 1) + c - (byte_at(kSampleHalo + q - 5) << 5);
+            v10 = (v10 << 1) + c - (byte_at(kSampleHalo + q - 10) << 10);
+            v20 = (v20 << 1) + c - (byte_at(kSampleHalo + q - 20) << 20);
+            v35 = (v35 << 1) + c - ((uint64_t)byte_at(kSampleHalo + q - 35) << 35);
+            if (p + 1 < n) {
+                const uint32_t pr = (c << 8) | (q + 1 < kSamplePer ? byte_at(kSampleHalo + q + 1) : nx);
+                atomicAdd(&h2[pr >> 1], 1u << (16 * (pr & 1u)));
+            } else {
+                last = c + 1;
+            }
+            pos++;
 #pragma unroll
-        for (int pi = 0; pi < kNumP; pi++) {
-            if (p >= 5) cnt[0 * kNumP + pi] += (v5 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
-            if (p >= 10) cnt[1 * kNumP + pi] += (v10 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
-            if (p >= 20) cnt[2 * kNumP + pi] += (v20 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
-            if (p >= 35) cnt[3 * kNumP + pi] += (v35 * gd.inv64[pi] <= gd.lim64[pi]) ? 1u : 0u;
+            for (int pi = 0; pi < kNumP; pi++) {
+                if (p >= 5) cnt[0 * kNumP + pi] += (v5 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+                if (p >= 10) cnt[1 * kNumP + pi] += (v10 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+                if (p >= 20) cnt[2 * kNumP + pi] += (v20 * gd.inv32[pi] <= gd.lim32[pi]) ? 1u : 0u;
+                if (p >= 35) cnt[3 * kNumP + pi] += (v35 * gd.inv64[pi] <= gd.lim64[pi]) ? 1u : 0u;
+            }
+            if (p >= 5) cnt[kDefaultCand] += ((v5 & 15u) == 0) ? 1u : 0u;
         }
-        if (p >= 5) cnt[kDefaultCand] += ((v5 & 15u) == 0) ? 1u : 0u;
     }
+    // counts are at most 16 per thread: two per word in 16-bit halves, summed over the wave
+    // (at most 1024), then over the block's waves (at most 16384)
 #pragma unroll
-    for (int c = 0; c < kNumCand; c++)
-        if (cnt[c]) atomicAdd(&tc[c], cnt[c]);
-    if (pos) atomicAdd(&out->positions, pos);
+    for (int c = 0; c < kNumCand + 1; c += 2) {
+        const uint32_t lo = (c < kNumCand) ? cnt[c] : pos;
+        const uint32_t hi = (c + 1 < kNumCand) ? cnt[c + 1] : (c + 1 == kNumCand ? pos : 0u);
+        const uint32_t s = wave_sum(lo | (hi << 16));
+        if ((threadIdx.x & 63) == 0 && s) atomicAdd(&tc[c / 2], s);
+    }
+    if (last) tc[(kNumCand + 1) / 2] = last;  // one thread of the grid at most
     __syncthreads();
-    if (h1[threadIdx.x]) atomicAdd(&out->hist1[threadIdx.x], h1[threadIdx.x]);
-    if (threadIdx.x < kNumCand && tc[threadIdx.x]) atomicAdd(&out->trig[threadIdx.x], tc[threadIdx.x]);
-    for (int i = threadIdx.x; i < 65536 / 2; i += kBlock) {
-        const uint32_t w = h2[i];
-        if (w & 0xFFFFu) atomicAdd(&hist2[2 * i], w & 0xFFFFu);
-        if (w >> 16) atomicAdd(&hist2[2 * i + 1], w >> 16);
+    if (threadIdx.x < kNumCand + 1) {
+        const uint32_t word = tc[threadIdx.x / 2];
+        part[(uint64_t)blockIdx.x * kSpWords + threadIdx.x] = (threadIdx.x & 1) ? (word >> 16) : (word & 0xFFFFu);
+    }
+    if (threadIdx.x == 0) part[(uint64_t)blockIdx.x * kSpWords + kSpLast] = tc[(kNumCand + 1) / 2];
+    for (int i = threadIdx.x; i < 65536 / 2; i += kSampleBlock) {
+        const uint32_t x = h2[i];
+        if (x & 0xFFFFu) atomicAdd(&hist2[2 * i], x & 0xFFFFu);
+        if (x >> 16) atomicAdd(&hist2[2 * i + 1], x >> 16);
     }
 }
 
 constexpr int kSketchPer = 64;                              // positions per thread
 constexpr uint64_t kSketchBlock = (uint64_t)kBlock * kSketchPer;
+constexpr int kSketchQueue = 512;                           // selected positions per block in LDS
 
 // insert a 32-bit fingerprint into an epoch-tagged open-addressing table (load factor at
 // most 1/4); true if new.  The probe sequence is bounded: a fingerprint that finds no slot
@@ -171,90 +215,248 @@ __device__ __forceinline__ bool sketch_insert(uint64_t* __restrict__ tab, uint32
     return true;
 }
 
-// Content-defined sketch of the whole text: positions whose 8-byte window hashes into the
-// top 1/1024 contribute 32-bit fingerprints of the 16 and 64 bytes that start there; the
-// distinct ones are counted with two hash sets.  Per-block totals, no global counter.
+// selection hash of the 8 bytes at a position (lo = bytes 0..3, hi = bytes 4..7, little endian):
+// the position is selected when the top 10 bits are zero (about 1 in 1024 windows)
+__device__ __forceinline__ bool sketch_selected(uint32_t lo, uint32_t hi)
+{
+    return ((lo ^ (hi * 0x85EBCA6Bu)) * 0x9E3779B1u) >> 22 == 0;
+}
+
+// 32-bit fingerprints of the 16 and 64 bytes at p (p + 64 <= n)
+__device__ __forceinline__ void sketch_fingerprints(const uint8_t* __restrict__ T, uint64_t p, uint32_t& f16,
+                                                    uint32_t& f64)
+{
+    uint64_t h = 0x9E3779B97F4A7C15ull;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        h = mix64(h ^ load8(T, p + 8 * k));
+        if (k == 1) f16 = (uint32_t)(h >> 32);
+    }
+    f64 = (uint32_t)(h >> 32);
+}
+
+// Content-defined sketch of the whole text: the selected positions p (p + 64 <= n) contribute
+// 32-bit fingerprints of the 16 and 64 bytes that start there; the distinct ones are counted
+// with two hash sets.  A block first collects its selected positions in LDS, then inserts them
+// with one lane each.  Per-block totals (selected, distinct 16, distinct 64), no global counter.
 __global__ __launch_bounds__(kBlock) void k_feat_sketch(const uint8_t* __restrict__ T, uint64_t n,
                                                        uint64_t* __restrict__ tab16, uint64_t* __restrict__ tab64,
                                                        uint32_t mask, uint32_t epoch, uint32_t* __restrict__ part)
 {
-    __shared__ uint32_t red[3];
+    __shared__ uint32_t qn, red[3];
+    __shared__ uint32_t queue[kSketchQueue];
     if (threadIdx.x < 3) red[threadIdx.x] = 0;
+    if (threadIdx.x == 0) qn = 0;
     __syncthreads();
-    const uint64_t p0 = (uint64_t)blockIdx.x * kSketchBlock + (uint64_t)threadIdx.x * kSketchPer;
-    uint32_t sel = 0, d16 = 0, d64 = 0;
-    if (p0 < n) {
-        const uint64_t* w = reinterpret_cast<const uint64_t*>(T + p0);  // p0 is a multiple of 64
-        uint64_t cur = w[0];
-        for (int q = 0; q < kSketchPer; q += 8) {
-            const uint64_t nxt = w[q / 8 + 1];
+    const uint64_t base = (uint64_t)blockIdx.x * kSketchBlock;
+    const uint64_t p0 = base + (uint64_t)threadIdx.x * kSketchPer;
+    uint32_t d16 = 0, d64 = 0;
+    auto take = [&](uint64_t p) {
+        const uint32_t slot = atomicAdd(&qn, 1u);
+        if (slot < kSketchQueue) {
+            queue[slot] = (uint32_t)(p - base);
+        } else {  // queue full: insert directly
+            uint32_t f16, f64;
+            sketch_fingerprints(T, p, f16, f64);
+            d16 += sketch_insert(tab16, mask, epoch, f16) ? 1u : 0u;
+            d64 += sketch_insert(tab64, mask, epoch, f64) ? 1u : 0u;
+        }
+    };
+    if (p0 + kSketchPer + 8 <= n) {
+        // words x[0..17] = T[p0 .. p0+71]; p0 is a multiple of 64
+        uint32_t x[18];
+        const uint4* v = reinterpret_cast<const uint4*>(T + p0);
 #pragma unroll
-            for (int r = 0; r < 8; r++) {
-                const uint64_t p = p0 + q + r;
-                const uint64_t win = r ? ((cur >> (8 * r)) | (nxt << (64 - 8 * r))) : cur;
-                if (((win * 0x9E3779B97F4A7C15ull) >> 54) == 0 && p + 64 <= n) {
-                    uint64_t h = 0x9E3779B97F4A7C15ull, a = 0;
-                    for (int k = 0; k < 8; k++) {
-                        h = mix64(h ^ load8(T, p + 8 * k));
-                        if (k == 1) a = h;
-                    }
-                    sel++;
-                    d16 += sketch_insert(tab16, mask, epoch, (uint32_t)(a >> 32)) ? 1u : 0u;
-                    d64 += sketch_insert(tab64, mask, epoch, (uint32_t)(h >> 32)) ? 1u : 0u;
-                }
+        for (int i = 0; i < 4; i++) {
+            const uint4 y = v[i];
+            x[4 * i] = y.x;
+            x[4 * i + 1] = y.y;
+            x[4 * i + 2] = y.z;
+            x[4 * i + 3] = y.w;
+        }
+        const uint2 y = *reinterpret_cast<const uint2*>(T + p0 + kSketchPer);
+        x[16] = y.x;
+        x[17] = y.y;
+        uint64_t selmask = 0;
+#pragma unroll
+        for (int j = 0; j < kSketchPer; j++) {
+            const int i = j >> 2, s = j & 3;
+            const uint32_t lo = s ? __builtin_amdgcn_alignbyte(x[i + 1], x[i], s) : x[i];
+            const uint32_t hi = s ? __builtin_amdgcn_alignbyte(x[i + 2], x[i + 1], s) : x[i + 1];
+            selmask |= (uint64_t)(sketch_selected(lo, hi) ? 1u : 0u) << j;
+        }
+        while (selmask) {
+            const int j = __builtin_ctzll(selmask);
+            selmask &= selmask - 1;
+            if (p0 + j + 64 <= n) take(p0 + j);
+        }
+    } else if (p0 < n) {  // tail: byte loads inside the text
+        for (int j = 0; j < kSketchPer && p0 + j + 64 <= n; j++) {
+            const uint64_t p = p0 + j;
+            uint32_t lo = 0, hi = 0;
+            for (int b = 0; b < 4; b++) {
+                lo |= (uint32_t)T[p + b] << (8 * b);
+                hi |= (uint32_t)T[p + 4 + b] << (8 * b);
             }
-            cur = nxt;
+            if (sketch_selected(lo, hi)) take(p);
         }
     }
-    if (sel) {
-        atomicAdd(&red[0], sel);
-        atomicAdd(&red[1], d16);
-        atomicAdd(&red[2], d64);
-    }
     __syncthreads();
+    const uint32_t queued = min(qn, (uint32_t)kSketchQueue);
+    for (uint32_t i = threadIdx.x; i < queued; i += kBlock) {
+        uint32_t f16, f64;
+        sketch_fingerprints(T, base + queue[i], f16, f64);
+        d16 += sketch_insert(tab16, mask, epoch, f16) ? 1u : 0u;
+        d64 += sketch_insert(tab64, mask, epoch, f64) ? 1u : 0u;
+    }
+    if (d16) atomicAdd(&red[1], d16);
+    if (d64) atomicAdd(&red[2], d64);
+    __syncthreads();
+    if (threadIdx.x == 0) red[0] = qn;
     if (threadIdx.x < 3) part[(uint64_t)blockIdx.x * 3 + threadIdx.x] = red[threadIdx.x];
 }
 
-__global__ __launch_bounds__(kBlock) void k_sum3(const uint32_t* __restrict__ part, uint32_t blocks,
-                                                uint32_t* __restrict__ out)
+// block-wide sum of a double over kSampleBlock threads (all threads get the result)
+__device__ double block_sum_f64(double v, double* lds)
 {
-    __shared__ uint32_t lds[kWaves];
-    for (int c = 0; c < 3; c++) {
-        uint32_t acc = 0;
-        for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) acc += part[(uint64_t)b * 3 + c];
-        uint32_t tot;
-        (void)block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
-        if (threadIdx.x == 0) out[c] = tot;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < kSampleBlock / 64; i++) s += lds[i];
+    return s;
+}
+
+// Norm of the features: x_k -> (x_k - mean_k) / std_k; null for raw features.
+struct FeatNorm {
+    const float* mean;
+    const float* std;
+};
+
+// One block: totals of the sample and sketch partials, order-0 and order-1 entropies of the
+// sample, and the feature vector of every candidate (raw into X; normalised into Xn if given).
+__global__ __launch_bounds__(kSampleBlock) void k_feat_finish(const uint32_t* __restrict__ spart, uint32_t chunks,
+                                                             const uint32_t* __restrict__ hist2,
+                                                             const uint32_t* __restrict__ kpart, uint32_t kblocks,
+                                                             uint64_t n, FeatNorm norm, float* __restrict__ X,
+                                                             float* __restrict__ Xn)
+{
+    __shared__ uint32_t tot[kSpWords];
+    __shared__ uint32_t h1[256];
+    __shared__ uint32_t sk[3];
+    __shared__ double red[kSampleBlock / 64];
+    const int t = threadIdx.x;
+    if (t < kSpWords) {
+        uint32_t s = 0;
+        for (uint32_t b = 0; b < chunks; b++) s += spart[(uint64_t)b * kSpWords + t];
+        tot[t] = s;
+    }
+    if (t < 256) {
+        uint32_t s = 0;
+        for (int b = 0; b < 256; b++) s += hist2[t * 256 + b];
+        h1[t] = s;
+    }
+    if (t < 3) sk[t] = 0;
+    __syncthreads();
+    if (t == 0 && tot[kSpLast]) h1[(tot[kSpLast] - 1) & 0xFFu]++;  // position n-1 has no pair
+    uint32_t a0 = 0, a1 = 0, a2 = 0;
+    for (uint32_t b = t; b < kblocks; b += kSampleBlock) {
+        a0 += kpart[(uint64_t)b * 3];
+        a1 += kpart[(uint64_t)b * 3 + 1];
+        a2 += kpart[(uint64_t)b * 3 + 2];
+    }
+    a0 = wave_sum(a0);
+    a1 = wave_sum(a1);
+    a2 = wave_sum(a2);
+    if ((t & 63) == 0) {
+        atomicAdd(&sk[0], a0);
+        atomicAdd(&sk[1], a1);
+        atomicAdd(&sk[2], a2);
+    }
+    __syncthreads();
+    // entropies (bits): H0 of the bytes, Hpair of the byte pairs
+    const double tot1 = (double)tot[kSpPos];
+    const double tot2 = tot1 - (tot[kSpLast] ? 1.0 : 0.0);
+    double e0 = 0, e2 = 0, s1 = 0;
+    if (t < 256 && h1[t]) {
+        const double p = h1[t] / tot1;
+        e0 = -p * log2(p);
+        s1 = 1;
+    }
+    for (int i = t; i < 65536; i += kSampleBlock) {
+        const uint32_t v = hist2[i];
+        if (v) {
+            const double p = v / tot2;
+            e2 -= p * log2(p);
+        }
+    }
+    const double H0 = block_sum_f64(e0, red);
+    const double Hpair = block_sum_f64(e2, red);
+    const double sigma = block_sum_f64(s1, red);
+    if (t >= kNumCand) return;
+    const double H1 = fmax(0.0, Hpair - H0);
+    const uint32_t m = sk[0];
+    const double rep16 = m ? (double)sk[1] / m : 1.0;
+    const double rep64 = m ? (double)sk[2] / m : 1.0;
+    const double pos = fmax(1.0, tot1);
+    float x[kIn];
+#pragma unroll
+    for (int k = 0; k < kIn; k++) x[k] = 0.f;
+    const int c = t;
+    const int W = c < kNumGrid ? kGridW[c / kNumP] : 5;
+    const int P = c < kNumGrid ? kGridP[c % kNumP] : 16;
+    const int wi = c < kNumGrid ? c / kNumP : 0;
+    const double dens = (tot[kSpTrig + c] + 1.0) / (pos + 1.0);
+    x[0] = (float)(log2((double)n) / 32.0);
+    x[1] = (float)(log2(fmax(1.0, sigma)) / 8.0);
+    x[2] = (float)(H0 / 8.0);
+    x[3] = (float)(H1 / 8.0);
+    x[4] = (float)rep16;
+    x[5] = (float)rep64;
+    x[6] = (float)log(rep16 + 1e-6);
+    x[7] = (float)log(rep64 + 1e-6);
+#pragma unroll
+    for (int k = 0; k < kNumW; k++) x[8 + k] = (k == wi) ? 1.f : 0.f;
+    x[12] = (float)(P / 100.0);
+    x[13] = (float)(log((double)P) / 5.0);
+    x[14] = (float)(1.0 / P);
+    x[15] = (float)(W / 35.0);
+    x[16] = (float)dens;
+    x[17] = (float)log(dens);
+    x[18] = (float)(log(1.0 / dens + W) / 5.0);          // mean phrase length
+    x[19] = (float)(log((double)n * dens + 1.0) / 32.0);  // parse length
+    x[20] = (float)(rep64 / dens / 100.0);                // distinct content x phrase length
+    x[21] = (float)(log(rep64 + 1e-6) + log(1.0 / dens + W));
+#pragma unroll
+    for (int k = 0; k < kIn; k++) {
+        X[c * kIn + k] = x[k];
+        if (Xn) Xn[c * kIn + k] = (x[k] - norm.mean[k]) / norm.std[k];
     }
 }
 
-static uint32_t inv_odd32(uint32_t q)
-{
-    uint32_t x = q;
-    for (int i = 0; i < 5; i++) x *= 2 - q * x;
-    return x;
-}
-
-// Features of all candidates (host side assembly after one read-back).
-void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X /* [kNumCand][kIn] */)
+// Features of all candidates into the device array X[kNumCand][kIn] (and normalised into Xn).
+void wp_features_device(Storage& S, const uint8_t* T, uint64_t n, float* X, FeatNorm norm, float* Xn)
 {
     uint32_t* hist2 = S.buf<uint32_t>(B_SEL_HIST2, 65536);
-    SampleCounts* sc = S.buf<SampleCounts>(B_SEL_COUNTS, 1);
     fill<uint32_t>(S, hist2, 65536, 0u);
-    fill<uint32_t>(S, reinterpret_cast<uint32_t*>(sc), sizeof(SampleCounts) / 4, 0u);
     GridDiv gd;
     for (int pi = 0; pi < kNumP; pi++) {
         const uint32_t P = (uint32_t)kGridP[pi];
-        gd.inv32[pi] = inv_odd32(P);
+        uint32_t x32 = P;
+        for (int i = 0; i < 5; i++) x32 *= 2 - P * x32;
+        gd.inv32[pi] = x32;
         gd.lim32[pi] = 0xFFFFFFFFu / P;
         uint64_t x = P;
         for (int i = 0; i < 6; i++) x *= 2 - (uint64_t)P * x;
         gd.inv64[pi] = x;
         gd.lim64[pi] = ~0ull / P;
     }
-    const uint64_t stride = (n > (uint64_t)kChunk * kSampleChunks) ? n / kSampleChunks : kChunk;
+    const uint64_t stride = (n > (uint64_t)kChunk * kSampleChunks) ? (n / kSampleChunks) & ~15ull : kChunk;
     const uint32_t chunks = (uint32_t)std::min<uint64_t>(kSampleChunks, ceil_div(n, (uint64_t)kChunk));
-    k_feat_sample<<<chunks * kSampleParts, kBlock, 0, S.stream>>>(T, n, stride, gd, hist2, sc);
+    uint32_t* spart = S.buf<uint32_t>(B_SEL_COUNTS, (size_t)kSampleChunks * kSpWords);
+    k_feat_sample<<<chunks, kSampleBlock, 0, S.stream>>>(T, n, stride, gd, hist2, spart);
     MIBWT_LAUNCH_CHECK();
     // sketch of the whole text: two epoch-tagged hash sets of 2^k >= n/256 slots (about n/1024
     // positions are selected, so the load factor stays below 1/4)
@@ -269,68 +471,21 @@ void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X /* [kNumCand
     }
     S.sketch_epoch++;
     const uint32_t blocks = (uint32_t)ceil_div(n, kSketchBlock);
-    uint32_t* part = S.buf<uint32_t>(B_SEL_PART, (size_t)blocks * 3);
+    uint32_t* kpart = S.buf<uint32_t>(B_SEL_PART, (size_t)blocks * 3);
     k_feat_sketch<<<blocks, kBlock, 0, S.stream>>>(T, n, tab, tab + slots, (uint32_t)(slots - 1), S.sketch_epoch,
-                                                   part);
+                                                   kpart);
     MIBWT_LAUNCH_CHECK();
-    k_sum3<<<1, kBlock, 0, S.stream>>>(part, blocks, sc->sketch);
+    k_feat_finish<<<1, kSampleBlock, 0, S.stream>>>(spart, chunks, hist2, kpart, blocks, n, norm, X, Xn);
     MIBWT_LAUNCH_CHECK();
-    SampleCounts h;
-    S.read(&h, sc);
-    const uint32_t m = h.sketch[0];
-    uint32_t dist[2] = {h.sketch[1], h.sketch[2]};
+}
 
-    std::vector<uint32_t> h2(65536);
-    S.read(h2.data(), hist2, 65536);
-    // order-0 / order-1 entropies
-    double tot = 0, H0 = 0, sigma = 0;
-    for (int c = 0; c < 256; c++) tot += h.hist1[c];
-    for (int c = 0; c < 256; c++)
-        if (h.hist1[c]) {
-            sigma += 1;
-            const double p = h.hist1[c] / tot;
-            H0 -= p * std::log2(p);
-        }
-    double tot2 = 0, Hpair = 0;
-    for (uint32_t v : h2) tot2 += v;
-    for (uint32_t v : h2)
-        if (v) {
-            const double p = v / tot2;
-            Hpair -= p * std::log2(p);
-        }
-    const double H1 = std::max(0.0, Hpair - H0);
-    const double rep16 = m ? (double)dist[0] / m : 1.0;
-    const double rep64 = m ? (double)dist[1] / m : 1.0;
-    const double pos = std::max(1u, h.positions);
-    for (int c = 0; c < kNumCand; c++) {
-        {
-            float* x = X + c * kIn;
-            for (int k = 0; k < kIn; k++) x[k] = 0.f;
-            const int W = cand_W(c), P = cand_P(c);
-            int wi = 0;
-            while (kGridW[wi] != W) wi++;
-            const double dens = (h.trig[c] + 1.0) / (pos + 1.0);
-            x[0] = (float)(std::log2((double)n) / 32.0);
-            x[1] = (float)(std::log2(std::max(1.0, sigma)) / 8.0);
-            x[2] = (float)(H0 / 8.0);
-            x[3] = (float)(H1 / 8.0);
-            x[4] = (float)rep16;
-            x[5] = (float)rep64;
-            x[6] = (float)std::log(rep16 + 1e-6);
-            x[7] = (float)std::log(rep64 + 1e-6);
-            x[8 + wi] = 1.f;
-            x[12] = (float)(P / 100.0);
-            x[13] = (float)(std::log((double)P) / 5.0);
-            x[14] = (float)(1.0 / P);
-            x[15] = (float)(W / 35.0);
-            x[16] = (float)dens;
-            x[17] = (float)(std::log(dens));
-            x[18] = (float)(std::log(1.0 / dens + W) / 5.0);          // mean phrase length
-            x[19] = (float)(std::log((double)n * dens + 1.0) / 32.0);  // parse length
-            x[20] = (float)(rep64 / dens / 100.0);                     // distinct content x phrase length
-            x[21] = (float)(std::log(rep64 + 1e-6) + std::log(1.0 / dens + W));
-        }
-    }
+// Raw features of all candidates into host memory X[kNumCand][kIn].
+void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X)
+{
+    float* dX = S.buf<float>(B_SEL_XRAW, (size_t)kNumCand * kIn);
+    wp_features_device(S, T, n, dX, FeatNorm{nullptr, nullptr}, nullptr);
+    MIBWT_HIP(hipMemcpyAsync(X, dX, sizeof(float) * kNumCand * kIn, hipMemcpyDeviceToHost, S.stream));
+    MIBWT_HIP(hipStreamSynchronize(S.stream));
 }
 
 // ------------------------------------------------------------------------ MLP kernels
@@ -428,12 +583,14 @@ constexpr int kParams = kW1 + kHid + kW2 + kHid + kW3 + kOut;  // fp32 master la
 
 struct Model {
     bool loaded = false;
+    uint64_t version = 0;  // distinct for every model loaded or trained in this process
     float xmean[kIn], xstd[kIn], ymean[kOut], ystd[kOut];
     std::vector<float> params;  // fp32 master layout (weights bf16-representable)
 };
 
 static Model g_model;
 static std::mutex g_model_mu;
+static uint64_t g_model_version = 0;  // guarded by g_model_mu
 
 static const uint32_t kMagic = 0x4D424C50;  // "PLBM"
 
@@ -667,6 +824,7 @@ float mlp_train(Storage& S, const float* Xraw, const float* Yraw, uint32_t B, in
     M.loaded = true;
     if (path && !model_save(M, path)) throw Error(LIBCUBWT_BAD_PARAMETER, "cannot write the selector model");
     std::lock_guard<std::mutex> lk(g_model_mu);
+    M.version = ++g_model_version;
     g_model = M;
     return loss;
 }
@@ -676,6 +834,7 @@ bool selector_load(const char* path)
     Model M;
     if (!model_load(M, path)) return false;
     std::lock_guard<std::mutex> lk(g_model_mu);
+    M.version = ++g_model_version;
     g_model = M;
     return true;
 }
@@ -699,7 +858,70 @@ static void ensure_default_model()
     });
 }
 
+// Result of one device-side choice: the chosen candidate and the predicted log(ms per byte)
+// of every candidate.
+struct SelPick {
+    uint32_t best;
+    float pred[kNumCand];
+};
+
+// one wave: de-normalised predicted time of each candidate, argmin (lowest index on ties,
+// NaN never chosen)
+__global__ void k_sel_pick(const float* __restrict__ Y, const float* __restrict__ ynorm /* mean[3] std[3] */,
+                           SelPick* __restrict__ out)
+{
+    const uint32_t c = threadIdx.x;
+    float t = INFINITY;
+    if (c < kNumCand) {
+        t = Y[c * kOut + 2] * ynorm[kOut + 2] + ynorm[2];
+        out->pred[c] = t;
+        if (!(t == t)) t = INFINITY;
+    }
+    uint32_t bi = c;
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const float to = __shfl_xor(t, off);
+        const uint32_t io = __shfl_xor(bi, off);
+        if (to < t || (to == t && io < bi)) {
+            t = to;
+            bi = io;
+        }
+    }
+    if (c == 0) out->best = (t == INFINITY) ? 0u : bi;
+}
+
+// The loaded model's parameters, bf16 weights and normalisation on this Storage's device;
+// uploaded again only when another model was loaded or trained since.
+struct ModelDev {
+    MlpDev dev;
+    const float* xnorm;  // mean[kIn] std[kIn]
+    const float* ynorm;  // mean[kOut] std[kOut]
+};
+
+static ModelDev model_on_device(Storage& S, const Model& M)
+{
+    float* p = S.buf<float>(B_SEL_MP, kParams);
+    uint16_t* pb = S.buf<uint16_t>(B_SEL_MPB, kParams);
+    float* nrm = S.buf<float>(B_SEL_NORM, 2 * kIn + 2 * kOut);
+    if (S.sel_model_version != M.version) {
+        std::vector<float> h(2 * kIn + 2 * kOut);
+        std::copy(M.xmean, M.xmean + kIn, h.begin());
+        std::copy(M.xstd, M.xstd + kIn, h.begin() + kIn);
+        std::copy(M.ymean, M.ymean + kOut, h.begin() + 2 * kIn);
+        std::copy(M.ystd, M.ystd + kOut, h.begin() + 2 * kIn + kOut);
+        MIBWT_HIP(hipMemcpyAsync(p, M.params.data(), sizeof(float) * kParams, hipMemcpyHostToDevice, S.stream));
+        MIBWT_HIP(hipMemcpyAsync(nrm, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, S.stream));
+        k_to_bf16<<<grid_for(kParams), kBlock, 0, S.stream>>>(p, pb, kParams);
+        MIBWT_LAUNCH_CHECK();
+        MIBWT_HIP(hipStreamSynchronize(S.stream));  // the host copies above are temporaries
+        S.sel_model_version = M.version;
+    }
+    MlpBufs m{p, pb, nullptr};
+    return ModelDev{m.dev(), nrm, nrm + 2 * kIn};
+}
+
 // Choose (W,P) for the text resident at T (padded device copy); fixed W=5,P=16 without a model.
+// Features, normalisation, the MLP and the argmin run on the device; one read-back.
 void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred_out)
 {
     ensure_default_model();
@@ -711,22 +933,19 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
     *W = 5;
     *P = 16;
     if (!M.loaded) return;
-    std::vector<float> X((size_t)kNumCand * kIn);
-    wp_features(S, T, n, X.data());
-    for (int c = 0; c < kNumCand; c++)
-        for (int k = 0; k < kIn; k++) X[(size_t)c * kIn + k] = (X[(size_t)c * kIn + k] - M.xmean[k]) / M.xstd[k];
-    std::vector<float> Y((size_t)kNumCand * kOut);
-    mlp_predict_raw(S, M.params.data(), X.data(), kNumCand, Y.data());
-    int best = 0;
-    float bt = 1e30f;
-    for (int c = 0; c < kNumCand; c++) {
-        const float t = Y[(size_t)c * kOut + 2] * M.ystd[2] + M.ymean[2];
-        if (pred_out) pred_out[c] = t;
-        if (t < bt) {
-            bt = t;
-            best = c;
-        }
-    }
+    const ModelDev md = model_on_device(S, M);
+    MlpWork w = mlp_work(S, kNumCand);
+    float* Xraw = S.buf<float>(B_SEL_XRAW, (size_t)kNumCand * kIn);
+    wp_features_device(S, T, n, Xraw, FeatNorm{md.xnorm, md.xnorm + kIn}, w.X);
+    mlp_forward(S, md.dev, w.X, kNumCand, w.H1, w.H2, w.Y);
+    SelPick* pick = S.buf<SelPick>(B_SEL_PICK, 1);
+    k_sel_pick<<<1, 64, 0, S.stream>>>(w.Y, md.ynorm, pick);
+    MIBWT_LAUNCH_CHECK();
+    SelPick h;
+    S.read(&h, pick);
+    if (pred_out)
+        for (int c = 0; c < kNumCand; c++) pred_out[c] = h.pred[c];
+    const int best = (int)std::min<uint32_t>(h.best, kNumCand - 1);
     *W = cand_W(best);
     *P = cand_P(best);
 }

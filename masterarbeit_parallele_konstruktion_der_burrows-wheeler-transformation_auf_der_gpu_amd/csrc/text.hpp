@@ -34,62 +34,82 @@ __device__ __forceinline__ uint64_t mix64(uint64_t h)
 // (one compare and select per word, no dynamic register indexing), the wavefront ORs the
 // bitmaps with lane shuffles, and each block writes one partial bitmap (no global atomics:
 // same-address device atomics serialize); k_alpha_merge ORs the partials.
-__device__ __forceinline__ void bitmap_add(uint32_t (&bm)[8], uint32_t c)
+// Presence flags of the byte values in LDS: a byte is looked up first and written only when
+// not yet seen (reads of one address broadcast; concurrent writes all store 1).  Sixteen
+// lookups are issued before any write.
+__device__ __forceinline__ void seen_add16(uint8_t* seen, const uint4& v, uint32_t valid /* bit i: byte i */)
 {
-    const uint32_t w = c >> 5, bit = 1u << (c & 31);
+    const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    uint32_t f[16];
 #pragma unroll
-    for (int k = 0; k < 8; k++) bm[k] |= (w == (uint32_t)k) ? bit : 0u;
+    for (int i = 0; i < 16; i++) f[i] = seen[(w[i >> 2] >> (8 * (i & 3))) & 0xFFu];
+#pragma unroll
+    for (int i = 0; i < 16; i++)
+        if (((valid >> i) & 1u) && !f[i]) seen[(w[i >> 2] >> (8 * (i & 3))) & 0xFFu] = 1;
 }
 
-__device__ __forceinline__ void bitmap_block_store(uint32_t (&bm)[8], uint32_t* __restrict__ part)
+// the block's flags as a 256-bit map: part[block][8]
+__device__ __forceinline__ void seen_block_store(const uint8_t* seen, uint32_t* __restrict__ part)
 {
-    __shared__ uint32_t acc[8];
-    if (threadIdx.x < 8) acc[threadIdx.x] = 0;
     __syncthreads();
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-        uint32_t x = bm[k];
-        for (int o = 32; o > 0; o >>= 1) x |= __shfl_xor(x, o, 64);
-        if ((threadIdx.x & 63) == 0 && x) atomicOr(&acc[k], x);
+    const uint64_t b = __ballot(seen[threadIdx.x] != 0);  // kBlock == 256: wave w holds bytes 64w..64w+63
+    if ((threadIdx.x & 63) == 0) {
+        const uint32_t w = threadIdx.x >> 6;
+        part[(uint64_t)blockIdx.x * 8 + 2 * w] = (uint32_t)b;
+        part[(uint64_t)blockIdx.x * 8 + 2 * w + 1] = (uint32_t)(b >> 32);
     }
-    __syncthreads();
-    if (threadIdx.x < 8) part[(uint64_t)blockIdx.x * 8 + threadIdx.x] = acc[threadIdx.x];
 }
 
 // bytes of T[0..n), 64 per thread
 static __global__ __launch_bounds__(kBlock) void k_alpha_text(const uint8_t* __restrict__ T, uint64_t n,
                                                              uint32_t* __restrict__ part)
 {
-    uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    static_assert(kBlock == 256, "one flag per thread");
+    __shared__ uint8_t seen[256];
+    seen[threadIdx.x] = 0;
+    __syncthreads();
     const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * 64;
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         const uint64_t p = base + q * 16;
         if (p >= n) break;
         const uint4 v = *reinterpret_cast<const uint4*>(T + p);
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-        const int cnt = (n - p) < 16 ? (int)(n - p) : 16;
-#pragma unroll
-        for (int i = 0; i < 16; i++)
-            if (i < cnt) bitmap_add(bm, (w[i >> 2] >> (8 * (i & 3))) & 0xFF);
+        const uint32_t cnt = (n - p) < 16 ? (uint32_t)(n - p) : 16u;
+        seen_add16(seen, v, cnt >= 16 ? 0xFFFFu : ((1u << cnt) - 1u));
     }
-    bitmap_block_store(bm, part);
+    seen_block_store(seen, part);
 }
 
-// bytes of the ranges T[start[u] .. start[u] + len[u]) clipped to n, one range per thread
+// bytes of the ranges T[start[u] .. start[u] + len[u]) clipped to n, one range per thread,
+// read as aligned 16-byte words, four per step issued together (T is the padded copy)
 static __global__ __launch_bounds__(kBlock) void k_alpha_ranges(const uint8_t* __restrict__ T, uint64_t n,
                                                                const uint32_t* __restrict__ start,
                                                                const uint32_t* __restrict__ len, uint32_t m,
                                                                uint32_t* __restrict__ part)
 {
-    uint32_t bm[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    __shared__ uint8_t seen[256];
+    seen[threadIdx.x] = 0;
+    __syncthreads();
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u < m) {
         const uint64_t s = start[u];
         const uint64_t e = (s + len[u] < n) ? s + len[u] : n;
-        for (uint64_t p = s; p < e; p++) bitmap_add(bm, T[p]);
+        for (uint64_t p = s & ~15ull; p < e; p += 64) {
+            uint4 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) v[j] = *reinterpret_cast<const uint4*>(T + p + 16 * j);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                // bytes q = p + 16 j + i with s <= q < e
+                const uint64_t q0 = p + 16 * j;
+                uint32_t valid = 0xFFFFu;
+                if (q0 < s) valid &= 0xFFFFu << (uint32_t)(s - q0);
+                if (q0 + 16 > e) valid &= (q0 >= e) ? 0u : ((1u << (uint32_t)(e - q0)) - 1u);
+                seen_add16(seen, v[j], valid);
+            }
+        }
     }
-    bitmap_block_store(bm, part);
+    seen_block_store(seen, part);
 }
 
 static __global__ __launch_bounds__(kBlock) void k_alpha_merge(const uint32_t* __restrict__ part, uint32_t blocks,

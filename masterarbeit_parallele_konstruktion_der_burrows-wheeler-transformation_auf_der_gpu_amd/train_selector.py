@@ -38,8 +38,11 @@ def _pkg():
 def random_text(rng: np.random.Generator, max_len: int, min_len: int = 4 << 20) -> tuple[np.ndarray, dict]:
     """A repetitive collection: random base (DNA, protein-like or bytes) x copies, mutated."""
     from mibwt import gen
-    kind = rng.choice(["dna", "dna", "dna", "protein", "bytes"])
-    copies = int(rng.integers(2, 65))
+    # large texts lean to DNA collections (the application); small ones mix the three kinds
+    kinds = ["dna", "dna", "dna", "protein", "bytes"] if min_len < (64 << 20) else \
+        ["dna"] * 7 + ["protein"] * 2 + ["bytes"]
+    kind = rng.choice(kinds)
+    copies = int(np.exp(rng.uniform(np.log(2), np.log(192))))
     total = int(np.exp(rng.uniform(np.log(min_len), np.log(max_len))))
     base_len = max(4096, total // copies)
     sub = float(np.exp(rng.uniform(np.log(1e-5), np.log(5e-2))))
@@ -67,8 +70,15 @@ def random_text(rng: np.random.Generator, max_len: int, min_len: int = 4 << 20) 
     return idx, {"kind": str(kind), "copies": copies, "base_len": base_len, "sub": sub}
 
 
-def run_grid(mibwt, st, T: np.ndarray, grid):
-    """features, targets and ms of every grid candidate, and the ms of fixed W=5,P=16"""
+def run_grid(mibwt, T: np.ndarray, grid):
+    """features, targets and ms of every grid candidate, and the ms of fixed W=5,P=16.  A fresh
+    storage per text: its grow-only buffers would otherwise keep the largest dictionary of every
+    text seen so far (a poor (W,P) on a low-repetition text can need many times n)."""
+    with mibwt.Storage(len(T)) as st:
+        return _run_grid(mibwt, st, T, grid)
+
+
+def _run_grid(mibwt, st, T: np.ndarray, grid):
     n = len(T)
     dT = mibwt.DeviceBuffer.from_numpy(T)
     dO = mibwt.DeviceBuffer(n)
@@ -79,10 +89,20 @@ def run_grid(mibwt, st, T: np.ndarray, grid):
     for c, (W, P) in enumerate(grid):
         best = None
         for _ in range(2):  # the faster of two builds: less timing noise in the targets
-            st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
+            try:
+                st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
+            except mibwt.BwtError as e:
+                if e.code != -9:  # only "dictionary beyond the 32-bit index range" is expected
+                    raise
+                best = None
+                break
             s = st.stats()
             if best is None or s["total_ms"] < best["total_ms"]:
                 best = s
+        if best is None:  # infeasible candidate: no training sample, never the best
+            ms[c] = np.inf
+            Y[c] = np.nan
+            continue
         ms[c] = best["total_ms"]
         Y[c] = [np.log(best["dict_length"] / n), np.log(best["phrases"] / n), np.log(best["total_ms"] / n)]
     dT.free()
@@ -103,11 +123,13 @@ def main():
     ap.add_argument("--data", default=None, help="write / reuse the collected runs (npz)")
     ap.add_argument("--report", default=None, help="write the held-out evaluation (json)")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--eval-workloads", default="",
+                    help="comma-separated named workloads (gen.workload) evaluated, never trained on")
     a = ap.parse_args()
     mibwt = _pkg()
     grid = mibwt.wp_grid()
     rng = np.random.default_rng(a.seed)
-    st = mibwt.Storage(max(a.max_mib, a.large_max_mib if a.large else 0) << 20)
+    st = mibwt.Storage(1 << 20)  # model training and inference only
     t0 = time.time()
     Xs, Ys, MS, FX, meta = [], [], [], [], []
     # order: regular training texts, large training texts, regular held-out, large held-out
@@ -117,7 +139,9 @@ def main():
     total = len(sizes)
     for i, (lo, hi) in enumerate(sizes):
         T, info = random_text(rng, hi, lo)
-        X, Y, ms, fms = run_grid(mibwt, st, T, grid)
+        X, Y, ms, fms = run_grid(mibwt, T, grid)
+        if not np.isfinite(fms):
+            raise RuntimeError("W=5,P=16 itself is infeasible for a generated text")
         Xs.append(X)
         Ys.append(Y)
         MS.append(ms)
@@ -134,16 +158,33 @@ def main():
     if a.data:
         np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX)
     ntr = a.texts + a.large
-    loss = st.mlp_train(X[:ntr].reshape(-1, 64), Y[:ntr].reshape(-1, 3), epochs=a.epochs, batch=512,
-                        lr=1e-3, seed=a.seed & 0xFFFF, out_path=a.out)
-    print(f"[train_selector] trained on {ntr * len(grid)} runs, final loss {loss:.4f}, model {a.out}", flush=True)
+    Xtr, Ytr = X[:ntr].reshape(-1, 64), Y[:ntr].reshape(-1, 3)
+    ok = np.isfinite(Ytr).all(axis=1)
+    loss = st.mlp_train(Xtr[ok], Ytr[ok], epochs=a.epochs, batch=512, lr=1e-3, seed=a.seed & 0xFFFF,
+                        out_path=a.out)
+    print(f"[train_selector] trained on {int(ok.sum())} runs ({int((~ok).sum())} infeasible skipped), "
+          f"final loss {loss:.4f}, model {a.out}", flush=True)
     mibwt.selector_load(a.out)
     # held-out evaluation: predicted choice against the measured grid
     report = evaluate(mibwt, st, X[ntr:], MS[ntr:], FX[ntr:], grid, a.out)
     if a.large_holdout:
         k = a.large_holdout
         report["large_holdout"] = evaluate(mibwt, st, X[-k:], MS[-k:], FX[-k:], grid, a.out)
-    report["train_runs"] = int(ntr * len(grid))
+    if a.eval_workloads:
+        from mibwt import gen
+        named = {}
+        for name in a.eval_workloads.split(","):
+            T = gen.workload(name)
+            Xw, _, msw, fmsw = run_grid(mibwt, T, grid)
+            ev = evaluate(mibwt, st, Xw[None], msw[None], np.array([fmsw]), grid, a.out)
+            c = int(np.argmin(np.where(np.isfinite(msw), msw, np.inf)))
+            named[name] = {"n": len(T), "chosen_wp": ev["chosen_wp"][0], "best_wp": list(grid[c]),
+                           "best_ms": float(msw[c]), "w5p16_ms": float(fmsw),
+                           "chosen_over_best": ev["chosen_over_best_mean"],
+                           "speedup_over_w5p16": ev["speedup_over_w5p16_median"]}
+            print(f"[train_selector] workload {name}: {named[name]}", flush=True)
+        report["named_workloads"] = named
+    report["train_runs"] = int(ok.sum())
     report["final_loss"] = loss
     print(json.dumps(report), flush=True)
     if a.report:
@@ -154,7 +195,9 @@ def main():
 
 def evaluate(mibwt, st, X, MS, FX, grid, model_path):
     """Choice of the trained model on held-out texts (features X[t][candidates][64], measured
-    ms); the last candidate is the fixed default W=5,P=16."""
+    ms, inf for a candidate whose dictionary outgrows the index range); the last candidate is
+    the fixed default W=5,P=16.  An infeasible choice is charged the library's fallback: a
+    failed attempt (counted as one more W=5,P=16 build) plus the W=5,P=16 build."""
     with open(model_path, "rb") as f:
         raw = np.frombuffer(f.read(), dtype=np.float32)
     hdr = 4
@@ -163,14 +206,16 @@ def evaluate(mibwt, st, X, MS, FX, grid, model_path):
     ymean = raw[hdr + 128:hdr + 131]
     ystd = raw[hdr + 131:hdr + 134]
     params = raw[hdr + 134:]
-    ratios_best, ratios_fixed, ratios_oracle = [], [], []
+    ratios_best, ratios_fixed, ratios_oracle, picks = [], [], [], []
     for t in range(len(X)):
         Xn = (X[t] - xmean) / xstd
         y = st.mlp_predict(params, Xn)
         pred_ms = y[:, 2] * ystd[2] + ymean[2]
         c = int(np.argmin(pred_ms))
-        ratios_best.append(MS[t][c] / MS[t].min())
-        ratios_fixed.append(FX[t] / MS[t][c])
+        picks.append(list(grid[c]))
+        chosen = MS[t][c] if np.isfinite(MS[t][c]) else 2.0 * FX[t]
+        ratios_best.append(chosen / MS[t].min())
+        ratios_fixed.append(FX[t] / chosen)
         ratios_oracle.append(FX[t] / MS[t].min())
     out = {"holdout_texts": len(X),
            "chosen_over_best_median": float(np.median(ratios_best)),
@@ -179,6 +224,7 @@ def evaluate(mibwt, st, X, MS, FX, grid, model_path):
     out["speedup_over_w5p16_median"] = float(np.median(ratios_fixed))
     out["speedup_over_w5p16_geomean"] = float(np.exp(np.mean(np.log(ratios_fixed))))
     out["best_grid_over_w5p16_geomean"] = float(np.exp(np.mean(np.log(ratios_oracle))))
+    out["chosen_wp"] = picks
     return out
 
 

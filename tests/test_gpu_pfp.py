@@ -166,6 +166,18 @@ def test_narrow_fingerprint_sort_widens_on_collision(mibwt, monkeypatch):
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
 
 
+def test_colliding_fingerprints_reseed(mibwt, monkeypatch):
+    # 12-bit first-attempt fingerprints make different phrases collide: the byte comparison
+    # of every occurrence with its representative must catch it and re-seed, same exact result
+    rng = np.random.default_rng(23)
+    T = rng.integers(0, 256, size=200000).astype(np.uint8)
+    monkeypatch.setenv("MIBWT_DICT_HASH_BITS", "12")
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "pfp", 5, 11)
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+
+
 @pytest.mark.parametrize("name,W,P", [("dna1g", 5, 16), ("dna1g", 35, 81), ("markov256m", 5, 16),
                                       ("markov256m", 35, 81)])
 def test_large_workloads_roundtrip(mibwt, name, W, P):

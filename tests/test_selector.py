@@ -85,13 +85,29 @@ def test_training_fits_a_smooth_target(storage, tmp_path):
 @pytest.mark.gpu
 def test_selector_choice_keeps_bwt_exact(mibwt):
     from mibwt import gen
+    import os
     T = gen.repetitive_dna(1 << 18, 8, 0.001, 0.0001, 10, seed=9)
     dT = mibwt.DeviceBuffer.from_numpy(T)
+    # the shipped model (a model trained earlier in this process would otherwise be active)
+    path = os.environ.get("MIBWT_SELECTOR") or os.path.join(os.path.dirname(mibwt.__file__),
+                                                            "selector_model.bin")
+    if os.path.exists(path):
+        mibwt.selector_load(path)
     with mibwt.Storage(len(T)) as st:
         (W, P), trained, pred = st.select_wp(dT.address, len(T))
         assert (W, P) in mibwt.wp_grid() and mibwt.wp_grid()[-1] == (5, 16)
         X = st.wp_features(dT.address, len(T))
         assert np.isfinite(X).all() and X.shape == (41, 64)
+        if trained:
+            # the device-side choice equals the host path: raw features, normalisation from
+            # the model file, plain MLP inference, argmin
+            raw = np.fromfile(path, dtype=np.float32)
+            xm, xs = raw[4:68], raw[68:132]
+            ym, ys = raw[132:135], raw[135:138]
+            Y = st.mlp_predict(raw[138:], (X - xm) / xs)
+            t = Y[:, 2] * ys[2] + ym[2]
+            assert np.allclose(t, pred, rtol=0, atol=1e-4)
+            assert mibwt.wp_grid()[int(np.argmin(pred))] == (W, P)
         L, idx = st.bwt(T, "cubwt")
     O, oidx = oracle.bwt(T)
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
@@ -100,11 +116,11 @@ def test_selector_choice_keeps_bwt_exact(mibwt):
 
 def sample_features(T):
     """CPU restatement of the sampled selector statistics: 64 chunks of 16384 bytes at stride
-    n/64; per grid candidate the fraction of sampled window ends p >= W whose exact window
+    n/64 rounded down to a multiple of 16; per grid candidate the fraction of sampled window ends p >= W whose exact window
     value is divisible by P (the hash rule of PFPk:468-475); order-0 entropy of the sample."""
     n = len(T)
     chunk, chunks = 16384, 64
-    stride = n // chunks if n > chunk * chunks else chunk
+    stride = (n // chunks) & ~15 if n > chunk * chunks else chunk
     pos = np.concatenate([np.arange(c * stride, min(c * stride + chunk, n)) for c in range(chunks)
                           if c * stride < n])
     t = T.astype(np.int64)
@@ -119,6 +135,57 @@ def sample_features(T):
     hist = np.bincount(T[pos], minlength=256)
     p = hist[hist > 0] / len(pos)
     return out, len(pos), float(-(p * np.log2(p)).sum())
+
+
+def _mix64(h):
+    h = h ^ (h >> np.uint64(33))
+    h = h * np.uint64(0xFF51AFD7ED558CCD)
+    h = h ^ (h >> np.uint64(33))
+    h = h * np.uint64(0xC4CEB9FE1A85EC53)
+    return h ^ (h >> np.uint64(33))
+
+
+def sketch_counts(T):
+    """CPU restatement of the whole-text sketch: positions p with p + 64 <= n whose 8-byte window
+    (lo = bytes 0..3, hi = bytes 4..7, little endian) has ((lo ^ hi * 0x85EBCA6B) * 0x9E3779B1)
+    >> 22 == 0 (32-bit arithmetic); each contributes the top 32 bits of a chained mix of its
+    16 and 64 bytes; returns (selected, distinct 16-byte, distinct 64-byte fingerprints)."""
+    n = len(T)
+    t = T.astype(np.uint64)
+
+    def word(off, nbytes, pos):
+        v = np.zeros(len(pos), dtype=np.uint64)
+        for b in range(nbytes):
+            v |= t[pos + off + b] << np.uint64(8 * b)
+        return v
+
+    pos = np.arange(0, n - 63)
+    lo = word(0, 4, pos).astype(np.uint32)
+    hi = word(4, 4, pos).astype(np.uint32)
+    with np.errstate(over="ignore"):
+        h = (lo ^ (hi * np.uint32(0x85EBCA6B))) * np.uint32(0x9E3779B1)
+        sel = pos[(h >> np.uint32(22)) == 0]
+        f = np.full(len(sel), 0x9E3779B97F4A7C15, dtype=np.uint64)
+        f16 = None
+        for k in range(8):
+            f = _mix64(f ^ word(8 * k, 8, sel))
+            if k == 1:
+                f16 = f >> np.uint64(32)
+    f64 = f >> np.uint64(32)
+    return len(sel), len(np.unique(f16)), len(np.unique(f64))
+
+
+def test_sketch_restatement_separates_repetitive_from_random():
+    # the restatement itself: a text made of 16 copies of one 64 KiB block has about 1/16 as
+    # many distinct fingerprints as selected positions; a random text has all distinct
+    rng = np.random.default_rng(5)
+    block = rng.integers(0, 256, size=1 << 16, dtype=np.uint8)
+    m, d16, d64 = sketch_counts(np.tile(block, 16))
+    assert m > 500 and abs(d64 / m - 1 / 16) < 0.01 and d16 <= d64
+    m, d16, d64 = sketch_counts(rng.integers(0, 256, size=1 << 20, dtype=np.uint8))
+    assert m > 500 and d16 == m and d64 == m
+    # selection rate is about 1 / 1024 on random bytes
+    assert 0.7 < m * 1024 / (1 << 20) < 1.3
 
 
 @pytest.mark.gpu
@@ -139,3 +206,5 @@ def test_sampled_features_match_cpu_restatement(mibwt):
     assert abs(X[0, 2] - H0 / 8.0) < 1e-5
     # repetitiveness: 12 copies of one base leave few distinct 64-byte fingerprints
     assert 0.0 < X[0, 5] < 0.5 and X[0, 4] <= X[0, 5] + 1e-6
+    m, d16, d64 = sketch_counts(T)
+    assert abs(X[0, 4] - d16 / m) <= 1e-6 and abs(X[0, 5] - d64 / m) <= 1e-6, (m, d16, d64)
