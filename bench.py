@@ -6,12 +6,14 @@ collection (4 MiB uniform-ACGT base x 64 copies, 0.05% substitutions + 0.005% in
 length 1..50), W=5, P=16.  Prints one JSON line.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna256m] [--W 5 --P 16]
-                  [--wp auto|fixed] [--lyndon-steps 3] [--no-verify] [--no-cpu-baseline]
+                  [--wp auto|fixed] [--lyndon-steps 3] [--batch-texts 256] [--grid-steps 0]
+                  [--no-verify] [--no-cpu-baseline]
 
 With --wp auto (default) every timed step runs the learned (W,P) selector on the resident text
 and then the build with its choice; `value` is that rate.  The fixed W/P build is timed
-beside it ("fixed_wp", "speedup_vs_fixed"), and the Lyndon-grammar BWT of the same text is
-timed and compared byte for byte ("lyndon").
+beside it ("fixed_wp", "speedup_vs_fixed"), the Lyndon-grammar BWT of the same text is
+timed and compared byte for byte ("lyndon"), and a batch of independent texts of uneven length
+is timed through the batch entry with host buffers ("batch").
 """
 from __future__ import annotations
 
@@ -43,6 +45,25 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def batch_workload(count, gen, seed=0xB17C0DE + 4):
+    """`count` texts shaped like the batched configuration: lengths log-uniform in
+    [64 KiB, 16 MiB], each a random base replicated 2..64 times with substitutions at a
+    log-uniform rate in [1e-5, 1e-2] (indels at a tenth of it, length 1..10); every other text
+    is mapped onto 256 byte values."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n_total = int(np.exp(rng.uniform(np.log(64 << 10), np.log(16 << 20))))
+        copies = int(rng.integers(2, 65))
+        rate = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-2))))
+        T = gen.repetitive_dna(max(1024, n_total // copies), copies, rate, rate / 10, 10, seed=int(rng.integers(1 << 31)))
+        if i % 2:
+            T = (T.astype(np.uint32) * 97 + np.arange(len(T)) % 7 * 31).astype(np.uint8)
+        out.append(T)
+    return out
+
+
 def cpu_baseline(T, W, P, sample_bytes):
     """The oracle's restatement of the reference PFP pipeline (single thread) on a prefix."""
     sys.path.insert(0, ROOT)
@@ -65,8 +86,8 @@ def main():
     ap.add_argument("--W", type=int, default=5)
     ap.add_argument("--P", type=int, default=16)
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--verify", choices=["gpu", "host"], default="gpu",
-                    help="round trip by the library's inverse BWT (gpu) or the oracle's (host)")
+    ap.add_argument("--batch-texts", type=int, default=256,
+                    help="batched leg: this many independent texts of 64 KiB-16 MiB (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-mib", type=int, default=32)
     ap.add_argument("--wp", choices=["auto", "fixed"], default="auto",
@@ -211,24 +232,43 @@ def main():
             log("BWT differs between the fixed and the learned (W,P)")
             sys.exit(1)
         t = time.perf_counter()
-        if a.verify == "host":
-            sys.path.insert(0, ROOT)
-            import oracle  # checker only
-            back = oracle.unbwt(mibwt.reference_to_cubwt(O, idx), idx)
-            verified = "unbwt_roundtrip_ok_host"
-        else:
-            # inverse BWT on the device (libcubwt_unbwt's kernels), compared on the host
-            dU = mibwt.DeviceBuffer(n)
-            st.unbwt_device(dO.address, dU.address, n, idx)
-            back = dU.download()
-            dU.free()
-            verified = "unbwt_roundtrip_ok_gpu"
+        # inverse BWT on the device (libcubwt_unbwt's kernels), compared on the host
+        dU = mibwt.DeviceBuffer(n)
+        st.unbwt_device(dO.address, dU.address, n, idx)
+        back = dU.download()
+        dU.free()
+        verified = "unbwt_roundtrip_ok_gpu"
         ok = bool((back == T).all())
         if not ok:
             verified = "MISMATCH"
-        log(f"verification (inverse BWT, {a.verify}): {verified} in {time.perf_counter() - t:.1f} s")
+        log(f"verification (inverse BWT on the GPU): {verified} in {time.perf_counter() - t:.1f} s")
         if not ok:
             sys.exit(1)
+
+    # batched leg: independent texts of uneven length (64 KiB-16 MiB, log-uniform; a random base
+    # x 2..64 copies, mutation 0.001%-1%; alphabet 4 or 256), host buffers in and out,
+    # (W,P) chosen per text by the selector; a sample is checked by the GPU inverse BWT
+    batch = None
+    if a.batch_texts > 0:
+        texts = batch_workload(a.batch_texts, gen)
+        tot = sum(len(t) for t in texts)
+        res = st.batch_bwt(texts[:8], "pfp", 0, 0)  # warm the worker storages
+        mibwt.synchronize()
+        t0 = time.perf_counter()
+        res = st.batch_bwt(texts, "pfp", 0, 0)
+        mibwt.synchronize()
+        bms = (time.perf_counter() - t0) * 1000.0
+        checked = 0
+        for i in range(0, len(texts), max(1, len(texts) // 8)):
+            L, idx_b = res[i]
+            if not (st.unbwt(L, idx_b) == texts[i]).all():
+                log(f"batched text {i}: inverse BWT differs")
+                sys.exit(1)
+            checked += 1
+        batch = {"texts": len(texts), "bytes": tot, "ms": round(bms, 2), "value": round(tot / 1e6 / (bms / 1000.0), 1),
+                 "unit": "MB/s", "wp_choice": "learned per text", "io": "host buffers in and out (PCIe included)",
+                 "verified_texts": checked}
+        log(f"batch: {len(texts)} texts, {tot / 1e6:.0f} MB in {bms:.1f} ms = {batch['value']} MB/s")
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -260,6 +300,7 @@ def main():
         "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
         "grid": grid,
         "lyndon": lyndon,
+        "batch": batch,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "verified": verified,

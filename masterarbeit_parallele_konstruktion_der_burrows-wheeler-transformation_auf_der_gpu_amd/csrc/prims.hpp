@@ -336,7 +336,6 @@ constexpr int kSortItems = 16;                    // items per thread
 constexpr int kSortTile = kBlock * kSortItems;    // 4096 items per tile
 constexpr int kWaveTile = 64 * kSortItems;        // 1024 items per wavefront
 constexpr int kMaxPasses = 8;
-constexpr int kTilesPerTicket = 2;                // consecutive tiles per block
 constexpr int kHistItems = 16;                    // consecutive keys per thread in the histogram
 
 template <typename K>
@@ -466,118 +465,109 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     __shared__ K stage[kSortTile];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    // one ticket covers kTilesPerTicket consecutive tiles, done in order by this block: the
-    // device-wide ticket counter (same-address atomics serialize) is hit half as often
-    const uint32_t tickets = (tiles + kTilesPerTicket - 1) / kTilesPerTicket;
-    const uint32_t ticket = chain_ticket(c, tickets, &slot);
-    for (int sub = 0; sub < kTilesPerTicket; sub++) {
-        const uint32_t tile = ticket * kTilesPerTicket + sub;
-        if (tile >= tiles) break;
-        if (sub) __syncthreads();  // the previous tile's reads of the counters / digits / stage are done
-        for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
-        __syncthreads();
+    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
+    const uint32_t tile = chain_ticket(c, tiles, &slot);  // also orders the zeroing above
 
-        const uint64_t tile_base = (uint64_t)tile * kSortTile;
-        const uint64_t base = tile_base + (uint64_t)w * kWaveTile + lane;
-        K k[kSortItems];
-        uint32_t rk[kSortItems / 2];  // two 16-bit ranks (later: tile slots) per register
+    const uint64_t tile_base = (uint64_t)tile * kSortTile;
+    const uint64_t base = tile_base + (uint64_t)w * kWaveTile + lane;
+    K k[kSortItems];
+    uint32_t rk[kSortItems / 2];  // two 16-bit ranks (later: tile slots) per register
 #pragma unroll
-        for (int i = 0; i < kSortItems; i++) {
-            const uint64_t idx = base + (uint64_t)i * 64;
-            const bool valid = idx < n;
-            k[i] = valid ? keys[idx] : K(0);
+    for (int i = 0; i < kSortItems; i++) {
+        const uint64_t idx = base + (uint64_t)i * 64;
+        const bool valid = idx < n;
+        k[i] = valid ? keys[idx] : K(0);
+    }
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        const bool valid = base + (uint64_t)i * 64 < n;
+        const uint32_t d = digit_of(k[i], shift, mask);
+        // lanes with the same digit: per bit, the lanes that agree with this lane's bit.  The
+        // bit is sign-extended to a 0 / all-ones word so that each half of the mask is one
+        // three-input boolean op: peers &= ~(ballot ^ bitmask).
+        const uint64_t vm = __ballot(valid);
+        uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
+#pragma unroll
+        for (int b = 0; b < kRadixBits; b++) {
+            const uint32_t mk = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+            const uint64_t m = __ballot(mk != 0u);
+            plo &= ~((uint32_t)m ^ mk);
+            phi &= ~((uint32_t)(m >> 32) ^ mk);
         }
+        uint32_t old = 0;
+        const int leader = plo ? (__ffs(plo) - 1) : (31 + __ffs(phi));
+        if (valid && lane == leader) old = atomicAdd(&wcnt[w][d], (uint32_t)(__popc(plo) + __popc(phi)));
+        old = __shfl(old, valid ? leader : 0, 64);
+        const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
+        const uint32_t r = valid ? (old + below) : 0u;
+        if (i & 1) rk[i >> 1] |= r << 16;
+        else rk[i >> 1] = r;
+    }
+    __syncthreads();
+    {
+        const int d = threadIdx.x;  // kBlock == kRadix
+        uint32_t run = 0;
 #pragma unroll
-        for (int i = 0; i < kSortItems; i++) {
-            const bool valid = base + (uint64_t)i * 64 < n;
+        for (int q = 0; q < kWaves; q++) {
+            const uint32_t cq = wcnt[q][d];
+            wcnt[q][d] = run;
+            run += cq;
+        }
+        uint32_t tot;
+        const uint32_t ds = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
+        dstart[d] = ds;
+        const uint32_t gstart = block_exclusive_scan(hist_pass[d], 0u, OpSum{}, scan_lds, &tot);
+        uint32_t excl = 0;
+        if (tile == 0) {
+            chain_publish(c, d, 2, run);
+        } else {
+            chain_publish(c, tile * kRadix + d, 1, run);
+            excl = chain_lookback_digit(c, tile, d);
+            chain_publish(c, tile * kRadix + d, 2, excl + run);
+        }
+        doff[d] = gstart + excl - ds;
+    }
+    __syncthreads();
+    // ranks become the items' slots in the tile
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        const uint64_t idx = base + (uint64_t)i * 64;
+        if (idx < n) {
             const uint32_t d = digit_of(k[i], shift, mask);
-            // lanes with the same digit: per bit, the lanes that agree with this lane's bit.  The
-            // bit is sign-extended to a 0 / all-ones word so that each half of the mask is one
-            // three-input boolean op: peers &= ~(ballot ^ bitmask).
-            const uint64_t vm = __ballot(valid);
-            uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
-#pragma unroll
-            for (int b = 0; b < kRadixBits; b++) {
-                const uint32_t mk = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
-                const uint64_t m = __ballot(mk != 0u);
-                plo &= ~((uint32_t)m ^ mk);
-                phi &= ~((uint32_t)(m >> 32) ^ mk);
-            }
-            uint32_t old = 0;
-            const int leader = plo ? (__ffs(plo) - 1) : (31 + __ffs(phi));
-            if (valid && lane == leader) old = atomicAdd(&wcnt[w][d], (uint32_t)(__popc(plo) + __popc(phi)));
-            old = __shfl(old, valid ? leader : 0, 64);
-            const uint32_t below = __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
-            const uint32_t r = valid ? (old + below) : 0u;
-            if (i & 1) rk[i >> 1] |= r << 16;
-            else rk[i >> 1] = r;
+            const uint32_t r = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
+            const uint32_t p = dstart[d] + wcnt[w][d] + r;
+            rk[i >> 1] = (i & 1) ? ((rk[i >> 1] & 0xFFFFu) | (p << 16)) : ((rk[i >> 1] & 0xFFFF0000u) | p);
         }
-        __syncthreads();
-        {
-            const int d = threadIdx.x;  // kBlock == kRadix
-            uint32_t run = 0;
+    }
+    __syncthreads();  // the counters are dead from here on: their bytes take the slot digits
+    // keys go through LDS in slot order
 #pragma unroll
-            for (int q = 0; q < kWaves; q++) {
-                const uint32_t cq = wcnt[q][d];
-                wcnt[q][d] = run;
-                run += cq;
-            }
-            uint32_t tot;
-            const uint32_t ds = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
-            dstart[d] = ds;
-            const uint32_t gstart = block_exclusive_scan(hist_pass[d], 0u, OpSum{}, scan_lds, &tot);
-            uint32_t excl = 0;
-            if (tile == 0) {
-                chain_publish(c, d, 2, run);
-            } else {
-                chain_publish(c, tile * kRadix + d, 1, run);
-                excl = chain_lookback_digit(c, tile, d);
-                chain_publish(c, tile * kRadix + d, 2, excl + run);
-            }
-            doff[d] = gstart + excl - ds;
+    for (int i = 0; i < kSortItems; i++) {
+        if (base + (uint64_t)i * 64 < n) {
+            const uint32_t p = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
+            stage[p] = k[i];
+            sdig[p] = (uint8_t)digit_of(k[i], shift, mask);
         }
-        __syncthreads();
-        // ranks become the items' slots in the tile
+    }
+    __syncthreads();
+    const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)kSortTile ? (n - tile_base) : kSortTile);
+    for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.key(p + doff[sdig[p]], stage[p]);
+    if (Emit::kValues && vals) {
+        // values are read only now, which keeps the ranking phase's register footprint small
+        V v[kSortItems];
 #pragma unroll
         for (int i = 0; i < kSortItems; i++) {
             const uint64_t idx = base + (uint64_t)i * 64;
-            if (idx < n) {
-                const uint32_t d = digit_of(k[i], shift, mask);
-                const uint32_t r = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
-                const uint32_t p = dstart[d] + wcnt[w][d] + r;
-                rk[i >> 1] = (i & 1) ? ((rk[i >> 1] & 0xFFFFu) | (p << 16)) : ((rk[i >> 1] & 0xFFFF0000u) | p);
-            }
-        }
-        __syncthreads();  // the counters are dead from here on: their bytes take the slot digits
-        // keys go through LDS in slot order
-#pragma unroll
-        for (int i = 0; i < kSortItems; i++) {
-            if (base + (uint64_t)i * 64 < n) {
-                const uint32_t p = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
-                stage[p] = k[i];
-                sdig[p] = (uint8_t)digit_of(k[i], shift, mask);
-            }
+            v[i] = (idx < n) ? vals[idx] : V(0);
         }
         __syncthreads();
-        const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)kSortTile ? (n - tile_base) : kSortTile);
-        for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.key(p + doff[sdig[p]], stage[p]);
-        if (Emit::kValues && vals) {
-            // values are read only now, which keeps the ranking phase's register footprint small
-            V v[kSortItems];
+        V* sv = reinterpret_cast<V*>(stage);
 #pragma unroll
-            for (int i = 0; i < kSortItems; i++) {
-                const uint64_t idx = base + (uint64_t)i * 64;
-                v[i] = (idx < n) ? vals[idx] : V(0);
-            }
-            __syncthreads();
-            V* sv = reinterpret_cast<V*>(stage);
-#pragma unroll
-            for (int i = 0; i < kSortItems; i++)
-                if (base + (uint64_t)i * 64 < n) sv[(i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu)] = v[i];
-            __syncthreads();
-            for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.val(p + doff[sdig[p]], sv[p]);
-        }
-    }  // tiles of this ticket
+        for (int i = 0; i < kSortItems; i++)
+            if (base + (uint64_t)i * 64 < n) sv[(i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu)] = v[i];
+        __syncthreads();
+        for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.val(p + doff[sdig[p]], sv[p]);
+    }
 }
 
 template <typename T>
@@ -611,7 +601,6 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
     }
     MIBWT_REQUIRE(n < (1ull << 32), LIBCUBWT_BAD_PARAMETER, "radix_sort: too many items");
     const uint32_t tiles = ceil_div(n, kSortTile);
-    const uint32_t rgrid = ceil_div(tiles, (uint32_t)kTilesPerTicket);
     const int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
     MIBWT_REQUIRE(passes <= kMaxPasses, LIBCUBWT_BAD_PARAMETER, "radix_sort: key too wide");
     const uint32_t* hist = hist_in;
@@ -637,13 +626,13 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         const Chain c = S.chain(tiles * kRadix);
         if (is_last && last) {
             const int kt = S.kt_begin(1, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
-            k_radix_pass<K, V, LastEmit><<<rgrid, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, hist + p * kRadix,
+            k_radix_pass<K, V, LastEmit><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, hist + p * kRadix,
                                                                         tiles, c, *last);
             S.kt_end(kt);
         } else {
             const int kt = S.kt_begin(1, 2.0 * (double)n * (sizeof(K) + vbytes));
             EmitPair<K, V> e{kout, vout};
-            k_radix_pass<K, V, EmitPair<K, V>><<<rgrid, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+            k_radix_pass<K, V, EmitPair<K, V>><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
                                                                               hist + p * kRadix, tiles, c, e);
             S.kt_end(kt);
         }
