@@ -164,7 +164,8 @@ int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double*
  * 64->128->128->3 (bf16 weights, fp32 accumulation, hand-written HIP kernels) predicts
  * log(dictionary length/n), log(parse length/n) and log(build ms per byte) for each of 41
  * candidates: the grid W in {5,10,20,35} x P in {11,13,17,23,31,41,53,61,71,81} and the
- * reference default W=5,P=16 (last); the fastest predicted candidate is chosen.  The model file is `selector_model.bin` next to lib/ (or $MIBWT_SELECTOR). */
+ * reference default W=5,P=16 (last); the fastest predicted candidate is chosen (ensemble mean,
+ * smoothed over each grid candidate's P-neighbours).  The model file is `selector_model.bin` next to lib/ (or $MIBWT_SELECTOR). */
 
 /* The 41 candidates (W[i], P[i]) (arrays of at least 64); returns 41. */
 int64_t mibwt_wp_grid(int32_t* W, int32_t* P);
@@ -192,6 +193,14 @@ int64_t mibwt_mlp_predict(void* storage, const float* params, const float* X, in
  * be NULL) and installs it. */
 int64_t mibwt_mlp_train(void* storage, const float* X, const float* Y, int64_t B, int32_t epochs,
                         int32_t batch, float lr, uint32_t seed, const char* out_path, float* final_loss);
+
+/* The same for an ensemble of `members` (1..16) networks trained from seeds seed, seed+1, ...
+ * with one shared normalisation; the selector averages their predicted times and smooths them
+ * over each grid candidate's P-neighbours of the same W before choosing.  final_loss = the
+ * members' mean. */
+int64_t mibwt_mlp_train_ensemble(void* storage, const float* X, const float* Y, int64_t B, int32_t epochs,
+                                 int32_t batch, float lr, uint32_t seed, int32_t members, const char* out_path,
+                                 float* final_loss);
 
 /* Device-memory plumbing for callers that keep texts resident in HBM without another HIP
  * runtime in the process (no counterpart in the reference, whose programs allocate inside

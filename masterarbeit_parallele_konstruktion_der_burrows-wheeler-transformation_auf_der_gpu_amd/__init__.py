@@ -57,6 +57,7 @@ EXPORTED = [
     "mibwt_mlp_grad",
     "mibwt_mlp_predict",
     "mibwt_mlp_train",
+    "mibwt_mlp_train_ensemble",
     "mibwt_device_alloc",
     "mibwt_device_free",
     "mibwt_copy",
@@ -130,6 +131,8 @@ def lib():
         L.mibwt_mlp_predict.argtypes = [vp, fp, fp, i64, fp]
         L.mibwt_mlp_train.argtypes = [vp, fp, fp, i64, i32, i32, ctypes.c_float, ctypes.c_uint32,
                                       ctypes.c_char_p, fp]
+        L.mibwt_mlp_train_ensemble.argtypes = [vp, fp, fp, i64, i32, i32, ctypes.c_float, ctypes.c_uint32, i32,
+                                               ctypes.c_char_p, fp]
         L.mibwt_device_alloc.argtypes = [ctypes.POINTER(vp), i64]
         L.mibwt_device_free.argtypes = [vp]
         L.mibwt_copy.argtypes = [vp, vp, i64, i32]
@@ -158,6 +161,31 @@ def wp_grid():
     count = lib().mibwt_wp_grid(W.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
                                 P.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
     return list(zip(W[:count].tolist(), P[:count].tolist()))
+
+
+def read_selector_model(path: str) -> dict:
+    """The selector model file: normalisation and the members' fp32 parameter vectors (a single
+    network, magic "PLBM", or an ensemble, magic "PLBE" followed by the member count)."""
+    raw = np.fromfile(path, dtype=np.uint32)
+    magic = int(raw[0])
+    members, hdr = (1, 4) if magic == 0x4D424C50 else (int(raw[4]), 5)
+    f = raw.view(np.float32)
+    xm, xs = f[hdr:hdr + 64], f[hdr + 64:hdr + 128]
+    ym, ys = f[hdr + 128:hdr + 131], f[hdr + 131:hdr + 134]
+    params = f[hdr + 134:].reshape(members, -1)
+    return {"members": members, "xmean": xm, "xstd": xs, "ymean": ym, "ystd": ys, "params": params}
+
+
+def smooth_along_p(t: np.ndarray, num_p: int = 10) -> np.ndarray:
+    """Mean of each grid candidate's predicted time with its P-neighbours of the same W (the
+    grid is W-major with num_p values of P; candidates after the grid are left as they are)."""
+    s = t.astype(np.float64).copy()
+    grid = (len(t) // num_p) * num_p
+    for c in range(grid):
+        pi = c % num_p
+        nb = [t[c]] + ([t[c - 1]] if pi > 0 else []) + ([t[c + 1]] if pi + 1 < num_p else [])
+        s[c] = np.mean(nb)
+    return s
 
 
 def selector_load(path: str):
@@ -332,12 +360,19 @@ class Storage:
         return Y
 
     def mlp_train(self, X: np.ndarray, Y: np.ndarray, epochs: int = 200, batch: int = 512,
-                  lr: float = 1e-3, seed: int = 1, out_path: str | None = None) -> float:
+                  lr: float = 1e-3, seed: int = 1, out_path: str | None = None, members: int = 1) -> float:
+        """Train the selector network (an ensemble of `members` networks when > 1), install it and
+        write it to out_path; returns the final training loss."""
         X = np.ascontiguousarray(X, dtype=np.float32)
         Y = np.ascontiguousarray(Y, dtype=np.float32)
         loss = ctypes.c_float()
-        rc = lib().mibwt_mlp_train(self.handle, _fp(X), _fp(Y), len(X), epochs, batch, lr, seed,
-                                   out_path.encode() if out_path else None, ctypes.byref(loss))
+        path = out_path.encode() if out_path else None
+        if members == 1:
+            rc = lib().mibwt_mlp_train(self.handle, _fp(X), _fp(Y), len(X), epochs, batch, lr, seed, path,
+                                       ctypes.byref(loss))
+        else:
+            rc = lib().mibwt_mlp_train_ensemble(self.handle, _fp(X), _fp(Y), len(X), epochs, batch, lr, seed,
+                                                members, path, ctypes.byref(loss))
         if rc != NO_ERROR:
             raise BwtError(rc, "mibwt_mlp_train")
         return float(loss.value)

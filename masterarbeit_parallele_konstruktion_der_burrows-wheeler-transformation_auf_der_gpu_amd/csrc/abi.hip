@@ -24,7 +24,7 @@ int wp_grid(int* W, int* P);
 float mlp_grad(Storage& S, const float* params, const float* X, const float* Y, uint32_t B, float* grads);
 void mlp_predict_raw(Storage& S, const float* params, const float* X, uint32_t B, float* Y);
 float mlp_train(Storage& S, const float* X, const float* Y, uint32_t B, int epochs, uint32_t mb, float lr,
-                uint32_t seed, const char* path);
+                uint32_t seed, const char* path, int members);
 }  // namespace mibwt
 
 using namespace mibwt;
@@ -482,7 +482,23 @@ int64_t mibwt_mlp_train(void* storage, const float* X, const float* Y, int64_t B
     if (!S || !X || !Y || B < 2 || B > (1 << 24) || epochs < 1 || batch < 0 || !(lr > 0)) return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
-        const float l = mlp_train(*S, X, Y, (uint32_t)B, epochs, (uint32_t)batch, lr, seed, out_path);
+        const float l = mlp_train(*S, X, Y, (uint32_t)B, epochs, (uint32_t)batch, lr, seed, out_path, 1);
+        if (final_loss) *final_loss = l;
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_mlp_train_ensemble(void* storage, const float* X, const float* Y, int64_t B, int32_t epochs,
+                                 int32_t batch, float lr, uint32_t seed, int32_t members, const char* out_path,
+                                 float* final_loss)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !X || !Y || B < 2 || B > (1 << 24) || epochs < 1 || batch < 0 || !(lr > 0) || members < 1 ||
+        members > 16)
+        return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        const float l = mlp_train(*S, X, Y, (uint32_t)B, epochs, (uint32_t)batch, lr, seed, out_path, members);
         if (final_loss) *final_loss = l;
         return LIBCUBWT_NO_ERROR;
     });

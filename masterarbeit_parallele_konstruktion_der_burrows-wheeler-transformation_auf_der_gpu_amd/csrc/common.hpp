@@ -284,6 +284,9 @@ enum BufId : int {
     B_DB_LPART,
     B_E_HIST,
     B_E_PART,
+    B_E_GFLAG,
+    B_E_CID,
+    B_E_DELTAC,
     B_UN_REC,
     B_UN_NXT0,
     B_UN_NXT1,
@@ -361,6 +364,7 @@ enum BufId : int {
     B_SEL_MPB,
     B_SEL_NORM,
     B_SEL_PICK,
+    B_SEL_YALL,
     B_COUNT_
 };
 
@@ -379,6 +383,7 @@ enum ScalarSlot : int {
     SC_DICTLEN,        // two slots: 64-bit dictionary length
     SC_SPECIAL = SC_DICTLEN + 2,  // two slots: group and entry index of the primary-index entry
     SC_EHSUM = SC_SPECIAL + 2,    // entries counted by the entry-sort histogram
+    SC_NGROUPS,                   // groups that receive BWT entries
     SC_COUNT_ = 16
 };
 

@@ -126,14 +126,23 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
     large[i] = 0;
     const uint32_t s = Hs[i];
     const uint32_t k = lk[i - w0];
-    uint32_t lt = 0, eq_before = 0, eq = 0;
     const uint32_t ls = (uint32_t)((int64_t)s - w0), li = (uint32_t)((int64_t)i - w0);
-    for (uint32_t j = ls; j < ls + z; j++) {
+    // members before this one and from it on, each counting keys < k and <= k (two compares and
+    // two carry-adds per key): lt = all smaller keys, eq_before = equal keys before it
+    uint32_t lt1 = 0, le1 = 0, lt2 = 0, le2 = 0;
+#pragma unroll 4
+    for (uint32_t j = ls; j < li; j++) {
         const uint32_t kj = lk[j];
-        lt += (kj < k);
-        eq += (kj == k);
-        eq_before += (kj == k && j < li);
+        lt1 += (kj < k) ? 1u : 0u;
+        le1 += (kj <= k) ? 1u : 0u;
     }
+#pragma unroll 4
+    for (uint32_t j = li; j < ls + z; j++) {
+        const uint32_t kj = lk[j];
+        lt2 += (kj < k) ? 1u : 0u;
+        le2 += (kj <= k) ? 1u : 0u;
+    }
+    const uint32_t lt = lt1 + lt2, eq_before = le1 - lt1, eq = eq_before + (le2 - lt2);
     const uint32_t x = V[i];
     const uint32_t gs = Gs[i];
     const uint32_t p = s + lt + eq_before;

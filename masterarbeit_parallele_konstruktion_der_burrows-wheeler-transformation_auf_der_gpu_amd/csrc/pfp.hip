@@ -715,12 +715,13 @@ __global__ void k_asm_counts(const uint64_t* __restrict__ R, const uint32_t* __r
 }
 
 // Digit histograms of the entry sort, weighted from the groups instead of read from the
-// entries: every sorted position g contributes ecnt[g] entries keyed by its group head Hd[g]
-// (nondecreasing in g).  Each thread walks kEhPer consecutive positions and adds runs of equal
+// entries: every sorted position g contributes ecnt[g] entries keyed by the dense id of its
+// group head, cid[Hd[g]] (nondecreasing in g).  Each thread walks kEhPer consecutive positions and adds runs of equal
 // digits at once; per-block counts go to part[block][kEhPasses * 256].
 constexpr int kEhPer = 64, kEhPasses = 4;
-__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ ecnt,
-                                                      uint32_t L, int passes, uint32_t* __restrict__ part)
+__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ cid,
+                                                      const uint32_t* __restrict__ ecnt, uint32_t L, int passes,
+                                                      uint32_t* __restrict__ part)
 {
     __shared__ uint32_t h[kEhPasses * kRadix];
     for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) h[i] = 0;
@@ -737,7 +738,7 @@ __global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restric
         if (g >= L) break;
         const uint32_t c = ecnt[g];
         if (c == 0) continue;
-        const uint32_t hd = Hd[g];
+        const uint32_t hd = cid[Hd[g]];
 #pragma unroll
         for (int p = 0; p < kEhPasses; p++) {
             if (p >= passes) break;
@@ -888,12 +889,31 @@ struct Ent {
 
 template <typename EntT>
 __global__ void k_mx_write(const AsmRec* __restrict__ rec, const uint32_t* __restrict__ mxf,
-                           const uint32_t* __restrict__ mxs, uint32_t L, EntT* __restrict__ mx)
+                           const uint32_t* __restrict__ mxs, const uint32_t* __restrict__ cid, uint32_t L,
+                           EntT* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L || !mxf[x]) return;
     const AsmRec a = rec[x];
-    mx[mxs[x]] = Ent<EntT>::make(a.head, (uint32_t)(a.r >> 32) & 0xFFu);
+    mx[mxs[x]] = Ent<EntT>::make(cid[a.head], (uint32_t)(a.r >> 32) & 0xFFu);
+}
+
+// Entry groups are numbered densely: the heads of the groups that receive entries (mixed
+// groups and whole phrases) get consecutive ids in sorted order, so the entry keys need only
+// bits_for(number of such groups) bits.
+__global__ void k_ent_gflag(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ ecnt, uint32_t L,
+                            uint32_t* __restrict__ gflag)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < L && ecnt[g]) gflag[Hd[g]] = 1;  // every writer stores 1
+}
+
+// slot offset minus entry offset of each entry group, by its dense id
+__global__ void k_ent_delta(const uint32_t* __restrict__ gflag, const uint32_t* __restrict__ cid,
+                            const uint32_t* __restrict__ delta, uint32_t L, uint32_t* __restrict__ deltac)
+{
+    const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
+    if (h < L && gflag[h]) deltac[cid[h]] = delta[h];
 }
 
 // per distinct phrase: first mixed entry, number of mixed entries, whole-phrase group
@@ -902,14 +922,14 @@ struct PhraseRec {
 };
 
 __global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                             const uint32_t* __restrict__ mxs, const AsmRec* __restrict__ rec, uint32_t du,
-                             PhraseRec* __restrict__ pr)
+                             const uint32_t* __restrict__ mxs, const AsmRec* __restrict__ rec,
+                             const uint32_t* __restrict__ cid, uint32_t du, PhraseRec* __restrict__ pr)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= du) return;
     const uint32_t o = (uint32_t)off[u];
     const uint32_t a = mxs[o], b = mxs[o + len[u]];
-    pr[u] = PhraseRec{a, b - a, rec[o].head, 0};
+    pr[u] = PhraseRec{a, b - a, cid[rec[o].head], 0};
 }
 
 // entries per parse-suffix rank r: the occurrence k = psa[r]-1 contributes its whole-phrase
@@ -1382,15 +1402,20 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
     k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, L, goff, ecnt);
     MIBWT_LAUNCH_CHECK();
-    // digit histograms of the entry sort (bits of the group head), from the per-position counts
-    const int gbits = bits_for(L);
-    const int epasses = (gbits + kRadixBits - 1) / kRadixBits;
+    // dense ids of the entry groups
+    uint32_t* gflag = S.buf<uint32_t>(B_E_GFLAG, L);
+    uint32_t* cid = S.buf<uint32_t>(B_E_CID, L);
+    fill<uint32_t>(S, gflag, L, 0u);
+    k_ent_gflag<<<grid_for(L), kBlock, 0, S.stream>>>(Hd, ecnt, L, gflag);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, gflag, cid, L, sc + SC_NGROUPS);
+    // digit histograms of the entry sort (bits of the dense group id), from the per-position
+    // counts; kEhPasses passes cover any id width the 32-bit entries can hold
     uint32_t* ehist = S.buf<uint32_t>(B_E_HIST, (size_t)kMaxPasses * kRadix);
-    const bool ehist_ok = epasses <= kEhPasses;
-    if (ehist_ok) {
+    {
         const uint32_t eb = (uint32_t)ceil_div((uint64_t)L, (uint64_t)kBlock * kEhPer);
         uint32_t* epart = S.buf<uint32_t>(B_E_PART, (size_t)eb * kEhPasses * kRadix);
-        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(Hd, ecnt, L, epasses, epart);
+        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(Hd, cid, ecnt, L, kEhPasses, epart);
         MIBWT_LAUNCH_CHECK();
         k_entry_hist_sum<<<kEhPasses * kRadix, kBlock, 0, S.stream>>>(epart, eb, ehist);
         MIBWT_LAUNCH_CHECK();
@@ -1399,11 +1424,16 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     }
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
     exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
-    uint32_t slots = 0;
-    S.read(&slots, sc + SC_TOTAL);
-    MIBWT_REQUIRE(slots == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
+    uint32_t hv0[SC_COUNT_];
+    S.read(hv0, sc, SC_COUNT_);
+    MIBWT_REQUIRE(hv0[SC_TOTAL] == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
+    const uint32_t ngroups = hv0[SC_NGROUPS];
+    const int gbits = std::max(1, bits_for(ngroups > 0 ? ngroups - 1 : 0));
     uint32_t* delta = S.buf<uint32_t>(B_G_DELTA, L);
     k_asm_fill<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, goff, ecnt, L, n, dO, delta);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* deltac = S.buf<uint32_t>(B_E_DELTAC, std::max<uint32_t>(1, ngroups));
+    k_ent_delta<<<grid_for(L), kBlock, 0, S.stream>>>(gflag, cid, delta, L, deltac);
     MIBWT_LAUNCH_CHECK();
     uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
@@ -1422,9 +1452,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         using EntT = decltype(tag);
         constexpr bool k32 = sizeof(EntT) == 4;
         EntT* mx = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_MX_ENT, nmx + 1));
-        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, L, mx);
+        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, cid, L, mx);
         MIBWT_LAUNCH_CHECK();
-        k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, du, prec);
+        k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, cid, du, prec);
         MIBWT_LAUNCH_CHECK();
         k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, qc);
         MIBWT_LAUNCH_CHECK();
@@ -1433,7 +1463,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         S.read(hv, sc, SC_COUNT_);
         ne = hv[SC_ENTRIES] + 1;
         // the histograms from the group counts are used only if they count every entry
-        const uint32_t* eh = (ehist_ok && hv[SC_EHSUM] == ne) ? ehist : nullptr;
+        const uint32_t* eh = (hv[SC_EHSUM] == ne) ? ehist : nullptr;
         EntT* ent0 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K0, ne));
         EntT* ent1 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K1, ne));
         const uint32_t nq = ne - 1;  // entries of the parse suffixes; the last occurrence adds one
@@ -1462,14 +1492,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
                                                                                                      sc + SC_TOTAL);
                 MIBWT_LAUNCH_CHECK();
             }
-            k_q_pidx<<<1, 1, 0, S.stream>>>(sc + SC_TOTAL, ent0, ne - 1, special, delta, sc + SC_PIDX);
+            k_q_pidx<<<1, 1, 0, S.stream>>>(sc + SC_TOTAL, ent0, ne - 1, special, deltac, sc + SC_PIDX);
             MIBWT_LAUNCH_CHECK();
-            EmitBwt32 em{delta, dO, n};
+            EmitBwt32 em{deltac, dO, n};
             radix_sort<uint32_t, uint32_t, EmitBwt32>(S, ent0, ent1, nullptr, nullptr, ne, 8, 8 + gbits, &em, nullptr,
                                                       eh);
         } else {
             fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
-            EmitBwt em{delta, dO, sc + SC_PIDX, n};
+            EmitBwt em{deltac, dO, sc + SC_PIDX, n};
             radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em, nullptr,
                                                     eh);
         }

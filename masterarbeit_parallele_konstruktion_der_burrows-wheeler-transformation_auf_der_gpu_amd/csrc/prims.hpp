@@ -411,9 +411,10 @@ struct EmitPair {
 };
 
 // Exclusive prefix of digit d over the tiles before `tile` (one thread per digit).  Status
-// words of 8 predecessors are requested at once: device-coherent loads take a round trip to
-// memory, so a one-tile-per-step walk would serialize on that latency.
-constexpr int kLookbackBatch = 8;
+// words of kLookbackBatch predecessors are requested at once: device-coherent loads take a round
+// trip to memory, so a one-tile-per-step walk would serialize on that latency, while larger
+// batches cost registers (measured on dna256m: 4 beats 8 by 5% and 16 by 9% of the pass time).
+constexpr int kLookbackBatch = 4;
 __device__ __forceinline__ uint32_t chain_lookback_digit(const Chain& c, uint32_t tile, uint32_t d)
 {
     uint32_t excl = 0;

@@ -581,28 +581,34 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
 constexpr int kW1 = kHid * kIn, kW2 = kHid * kHid, kW3 = kOut * kHid;
 constexpr int kParams = kW1 + kHid + kW2 + kHid + kW3 + kOut;  // fp32 master layout: W1 b1 W2 b2 W3 b3
 
+// An ensemble of `members` networks sharing one input / output normalisation; the selector
+// averages their predictions.
+constexpr int kMaxMembers = 16;
 struct Model {
     bool loaded = false;
     uint64_t version = 0;  // distinct for every model loaded or trained in this process
+    int members = 1;
     float xmean[kIn], xstd[kIn], ymean[kOut], ystd[kOut];
-    std::vector<float> params;  // fp32 master layout (weights bf16-representable)
+    std::vector<float> params;  // members x fp32 master layout (weights bf16-representable)
 };
 
 static Model g_model;
 static std::mutex g_model_mu;
 static uint64_t g_model_version = 0;  // guarded by g_model_mu
 
-static const uint32_t kMagic = 0x4D424C50;  // "PLBM"
+static const uint32_t kMagic = 0x4D424C50;   // "PLBM": one network
+static const uint32_t kMagicE = 0x45424C50;  // "PLBE": ensemble, member count after the sizes
 
 bool model_save(const Model& M, const char* path)
 {
     FILE* f = fopen(path, "wb");
     if (!f) return false;
-    const uint32_t hdr[4] = {kMagic, (uint32_t)kIn, (uint32_t)kHid, (uint32_t)kOut};
-    bool ok = fwrite(hdr, 4, 4, f) == 4;
+    const uint32_t hdr[5] = {kMagicE, (uint32_t)kIn, (uint32_t)kHid, (uint32_t)kOut, (uint32_t)M.members};
+    bool ok = fwrite(hdr, 4, 5, f) == 5;
     ok = ok && fwrite(M.xmean, 4, kIn, f) == (size_t)kIn && fwrite(M.xstd, 4, kIn, f) == (size_t)kIn;
     ok = ok && fwrite(M.ymean, 4, kOut, f) == (size_t)kOut && fwrite(M.ystd, 4, kOut, f) == (size_t)kOut;
-    ok = ok && fwrite(M.params.data(), 4, kParams, f) == (size_t)kParams;
+    const size_t np = (size_t)kParams * M.members;
+    ok = ok && M.params.size() == np && fwrite(M.params.data(), 4, np, f) == np;
     return (fclose(f) == 0) && ok;
 }
 
@@ -611,12 +617,15 @@ bool model_load(Model& M, const char* path)
     FILE* f = fopen(path, "rb");
     if (!f) return false;
     uint32_t hdr[4];
-    bool ok = fread(hdr, 4, 4, f) == 4 && hdr[0] == kMagic && hdr[1] == (uint32_t)kIn && hdr[2] == (uint32_t)kHid &&
-              hdr[3] == (uint32_t)kOut;
-    M.params.assign(kParams, 0.f);
+    bool ok = fread(hdr, 4, 4, f) == 4 && (hdr[0] == kMagic || hdr[0] == kMagicE) && hdr[1] == (uint32_t)kIn &&
+              hdr[2] == (uint32_t)kHid && hdr[3] == (uint32_t)kOut;
+    uint32_t members = 1;
+    if (ok && hdr[0] == kMagicE) ok = fread(&members, 4, 1, f) == 1 && members >= 1 && members <= kMaxMembers;
+    M.members = (int)members;
+    M.params.assign((size_t)kParams * members, 0.f);
     ok = ok && fread(M.xmean, 4, kIn, f) == (size_t)kIn && fread(M.xstd, 4, kIn, f) == (size_t)kIn;
     ok = ok && fread(M.ymean, 4, kOut, f) == (size_t)kOut && fread(M.ystd, 4, kOut, f) == (size_t)kOut;
-    ok = ok && fread(M.params.data(), 4, kParams, f) == (size_t)kParams;
+    ok = ok && fread(M.params.data(), 4, M.params.size(), f) == M.params.size();
     fclose(f);
     M.loaded = ok;
     return ok;
@@ -732,58 +741,35 @@ static uint64_t splitmix(uint64_t& s)
     return z ^ (z >> 31);
 }
 
-// Train on B samples (raw features / targets); writes the model file; returns the final
-// training loss (normalised targets).
-float mlp_train(Storage& S, const float* Xraw, const float* Yraw, uint32_t B, int epochs, uint32_t mb, float lr,
-                uint32_t seed, const char* path)
+// Train one network on the normalised samples X, Y (B rows, shuffled by `seed`) from a He
+// initialisation drawn from `seed`; returns the final full-batch loss, params[kParams] out.
+static float mlp_train_member(Storage& S, const std::vector<float>& Xn, const std::vector<float>& Yn, uint32_t B,
+                              int epochs, uint32_t mb, float lr, uint32_t seed, float* params)
 {
-    Model M;
-    // normalisation
-    for (int k = 0; k < kIn; k++) {
-        double s1 = 0, s2 = 0;
-        for (uint32_t i = 0; i < B; i++) {
-            s1 += Xraw[(size_t)i * kIn + k];
-            s2 += (double)Xraw[(size_t)i * kIn + k] * Xraw[(size_t)i * kIn + k];
-        }
-        const double mu = s1 / B, var = s2 / B - mu * mu;
-        M.xmean[k] = (float)mu;
-        M.xstd[k] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
-    }
-    for (int k = 0; k < kOut; k++) {
-        double s1 = 0, s2 = 0;
-        for (uint32_t i = 0; i < B; i++) {
-            s1 += Yraw[(size_t)i * kOut + k];
-            s2 += (double)Yraw[(size_t)i * kOut + k] * Yraw[(size_t)i * kOut + k];
-        }
-        const double mu = s1 / B, var = s2 / B - mu * mu;
-        M.ymean[k] = (float)mu;
-        M.ystd[k] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
-    }
-    // shuffled, normalised copy
     std::vector<uint32_t> perm(B);
     for (uint32_t i = 0; i < B; i++) perm[i] = i;
     uint64_t st = seed * 0x2545F4914F6CDD1Dull + 1;
     for (uint32_t i = B; i > 1; i--) std::swap(perm[i - 1], perm[splitmix(st) % i]);
     std::vector<float> X((size_t)B * kIn), Y((size_t)B * kOut);
     for (uint32_t i = 0; i < B; i++) {
-        for (int k = 0; k < kIn; k++) X[(size_t)i * kIn + k] = (Xraw[(size_t)perm[i] * kIn + k] - M.xmean[k]) / M.xstd[k];
-        for (int k = 0; k < kOut; k++) Y[(size_t)i * kOut + k] = (Yraw[(size_t)perm[i] * kOut + k] - M.ymean[k]) / M.ystd[k];
+        for (int k = 0; k < kIn; k++) X[(size_t)i * kIn + k] = Xn[(size_t)perm[i] * kIn + k];
+        for (int k = 0; k < kOut; k++) Y[(size_t)i * kOut + k] = Yn[(size_t)perm[i] * kOut + k];
     }
     // He initialisation, weights rounded to bf16
-    M.params.assign(kParams, 0.f);
+    std::vector<float> P0(kParams, 0.f);
     auto init = [&](size_t off, int count, int fan_in) {
         const double sd = std::sqrt(2.0 / fan_in);
         for (int i = 0; i < count; i++) {
             const double u1 = (splitmix(st) >> 11) * (1.0 / 9007199254740992.0) + 1e-12;
             const double u2 = (splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
             const double z = std::sqrt(-2.0 * std::log(u1)) * std::cos(6.283185307179586 * u2);
-            M.params[off + i] = bf16_to_f32(f32_to_bf16((float)(z * sd)));
+            P0[off + i] = bf16_to_f32(f32_to_bf16((float)(z * sd)));
         }
     };
     init(0, kW1, kIn);
     init(kW1 + kHid, kW2, kHid);
     init(kW1 + kHid + kW2 + kHid, kW3, kHid);
-    MlpBufs m = mlp_params(S, M.params.data());
+    MlpBufs m = mlp_params(S, P0.data());
     float* m1 = S.buf<float>(B_SEL_M1, kParams);
     float* m2 = S.buf<float>(B_SEL_M2, kParams);
     fill<float>(S, m1, kParams, 0.f);
@@ -818,15 +804,57 @@ float mlp_train(Storage& S, const float* Xraw, const float* Yraw, uint32_t B, in
     float loss = 0;
     S.read(&loss, w.loss);
     // the model file stores bf16-representable weights
-    MIBWT_HIP(hipMemcpyAsync(M.params.data(), m.p, sizeof(float) * kParams, hipMemcpyDeviceToHost, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(params, m.p, sizeof(float) * kParams, hipMemcpyDeviceToHost, S.stream));
     MIBWT_HIP(hipStreamSynchronize(S.stream));
-    for (int i = 0; i < kParams; i++) M.params[i] = bf16_to_f32(f32_to_bf16(M.params[i]));
+    for (int i = 0; i < kParams; i++) params[i] = bf16_to_f32(f32_to_bf16(params[i]));
+    return loss;
+}
+
+// Train an ensemble of `members` networks (seeds seed, seed+1, ...) on B samples (raw features /
+// targets) with one shared normalisation; writes the model file; returns the members' mean
+// final training loss (normalised targets).
+float mlp_train(Storage& S, const float* Xraw, const float* Yraw, uint32_t B, int epochs, uint32_t mb, float lr,
+                uint32_t seed, const char* path, int members)
+{
+    MIBWT_REQUIRE(members >= 1 && members <= kMaxMembers, LIBCUBWT_BAD_PARAMETER, "ensemble size out of range");
+    Model M;
+    M.members = members;
+    // normalisation
+    for (int k = 0; k < kIn; k++) {
+        double s1 = 0, s2 = 0;
+        for (uint32_t i = 0; i < B; i++) {
+            s1 += Xraw[(size_t)i * kIn + k];
+            s2 += (double)Xraw[(size_t)i * kIn + k] * Xraw[(size_t)i * kIn + k];
+        }
+        const double mu = s1 / B, var = s2 / B - mu * mu;
+        M.xmean[k] = (float)mu;
+        M.xstd[k] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
+    }
+    for (int k = 0; k < kOut; k++) {
+        double s1 = 0, s2 = 0;
+        for (uint32_t i = 0; i < B; i++) {
+            s1 += Yraw[(size_t)i * kOut + k];
+            s2 += (double)Yraw[(size_t)i * kOut + k] * Yraw[(size_t)i * kOut + k];
+        }
+        const double mu = s1 / B, var = s2 / B - mu * mu;
+        M.ymean[k] = (float)mu;
+        M.ystd[k] = (float)(var > 1e-12 ? std::sqrt(var) : 1.0);
+    }
+    std::vector<float> Xn((size_t)B * kIn), Yn((size_t)B * kOut);
+    for (uint32_t i = 0; i < B; i++) {
+        for (int k = 0; k < kIn; k++) Xn[(size_t)i * kIn + k] = (Xraw[(size_t)i * kIn + k] - M.xmean[k]) / M.xstd[k];
+        for (int k = 0; k < kOut; k++) Yn[(size_t)i * kOut + k] = (Yraw[(size_t)i * kOut + k] - M.ymean[k]) / M.ystd[k];
+    }
+    M.params.assign((size_t)kParams * members, 0.f);
+    double loss = 0;
+    for (int e = 0; e < members; e++)
+        loss += mlp_train_member(S, Xn, Yn, B, epochs, mb, lr, seed + (uint32_t)e, M.params.data() + (size_t)e * kParams);
     M.loaded = true;
     if (path && !model_save(M, path)) throw Error(LIBCUBWT_BAD_PARAMETER, "cannot write the selector model");
     std::lock_guard<std::mutex> lk(g_model_mu);
     M.version = ++g_model_version;
     g_model = M;
-    return loss;
+    return (float)(loss / members);
 }
 
 bool selector_load(const char* path)
@@ -859,21 +887,45 @@ static void ensure_default_model()
 }
 
 // Result of one device-side choice: the chosen candidate and the predicted log(ms per byte)
-// of every candidate.
+// of every candidate (ensemble mean, smoothed along P).
 struct SelPick {
     uint32_t best;
     float pred[kNumCand];
 };
 
-// one wave: de-normalised predicted time of each candidate, argmin (lowest index on ties,
-// NaN never chosen)
-__global__ void k_sel_pick(const float* __restrict__ Y, const float* __restrict__ ynorm /* mean[3] std[3] */,
+// One wave: the ensemble's mean de-normalised predicted time of each candidate, smoothed over
+// each grid candidate and its P-neighbours of the same W (a choice in a flat stretch of the
+// grid is robust to the error of any single prediction), argmin (lowest index on ties, NaN
+// never chosen).  Y holds `members` blocks of kNumCand x kOut outputs.
+__global__ void k_sel_pick(const float* __restrict__ Y, int members, const float* __restrict__ ynorm /* mean[3] std[3] */,
                            SelPick* __restrict__ out)
 {
+    __shared__ float v[64];
     const uint32_t c = threadIdx.x;
+    float m = 0.f;
+    if (c < kNumCand) {
+        for (int e = 0; e < members; e++) m += Y[((size_t)e * kNumCand + c) * kOut + 2];
+        m = (m / members) * ynorm[kOut + 2] + ynorm[2];
+    }
+    v[c] = m;
+    __syncthreads();
     float t = INFINITY;
     if (c < kNumCand) {
-        t = Y[c * kOut + 2] * ynorm[kOut + 2] + ynorm[2];
+        t = m;
+        if (c < kNumGrid) {
+            const uint32_t pi = c % kNumP;
+            float acc = m;
+            int cnt = 1;
+            if (pi > 0) {
+                acc += v[c - 1];
+                cnt++;
+            }
+            if (pi + 1 < kNumP) {
+                acc += v[c + 1];
+                cnt++;
+            }
+            t = acc / cnt;
+        }
         out->pred[c] = t;
         if (!(t == t)) t = INFINITY;
     }
@@ -893,15 +945,17 @@ __global__ void k_sel_pick(const float* __restrict__ Y, const float* __restrict_
 // The loaded model's parameters, bf16 weights and normalisation on this Storage's device;
 // uploaded again only when another model was loaded or trained since.
 struct ModelDev {
-    MlpDev dev;
+    MlpDev dev[kMaxMembers];
+    int members;
     const float* xnorm;  // mean[kIn] std[kIn]
     const float* ynorm;  // mean[kOut] std[kOut]
 };
 
 static ModelDev model_on_device(Storage& S, const Model& M)
 {
-    float* p = S.buf<float>(B_SEL_MP, kParams);
-    uint16_t* pb = S.buf<uint16_t>(B_SEL_MPB, kParams);
+    const size_t np = (size_t)kParams * M.members;
+    float* p = S.buf<float>(B_SEL_MP, np);
+    uint16_t* pb = S.buf<uint16_t>(B_SEL_MPB, np);
     float* nrm = S.buf<float>(B_SEL_NORM, 2 * kIn + 2 * kOut);
     if (S.sel_model_version != M.version) {
         std::vector<float> h(2 * kIn + 2 * kOut);
@@ -909,19 +963,27 @@ static ModelDev model_on_device(Storage& S, const Model& M)
         std::copy(M.xstd, M.xstd + kIn, h.begin() + kIn);
         std::copy(M.ymean, M.ymean + kOut, h.begin() + 2 * kIn);
         std::copy(M.ystd, M.ystd + kOut, h.begin() + 2 * kIn + kOut);
-        MIBWT_HIP(hipMemcpyAsync(p, M.params.data(), sizeof(float) * kParams, hipMemcpyHostToDevice, S.stream));
+        MIBWT_HIP(hipMemcpyAsync(p, M.params.data(), sizeof(float) * np, hipMemcpyHostToDevice, S.stream));
         MIBWT_HIP(hipMemcpyAsync(nrm, h.data(), sizeof(float) * h.size(), hipMemcpyHostToDevice, S.stream));
-        k_to_bf16<<<grid_for(kParams), kBlock, 0, S.stream>>>(p, pb, kParams);
+        k_to_bf16<<<grid_for(np), kBlock, 0, S.stream>>>(p, pb, (uint32_t)np);
         MIBWT_LAUNCH_CHECK();
         MIBWT_HIP(hipStreamSynchronize(S.stream));  // the host copies above are temporaries
         S.sel_model_version = M.version;
     }
-    MlpBufs m{p, pb, nullptr};
-    return ModelDev{m.dev(), nrm, nrm + 2 * kIn};
+    ModelDev md{};
+    md.members = M.members;
+    for (int e = 0; e < M.members; e++) {
+        MlpBufs m{p + (size_t)e * kParams, pb + (size_t)e * kParams, nullptr};
+        md.dev[e] = m.dev();
+    }
+    md.xnorm = nrm;
+    md.ynorm = nrm + 2 * kIn;
+    return md;
 }
 
 // Choose (W,P) for the text resident at T (padded device copy); fixed W=5,P=16 without a model.
-// Features, normalisation, the MLP and the argmin run on the device; one read-back.
+// Features, normalisation, the ensemble's networks and the choice run on the device; one
+// read-back.
 void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred_out)
 {
     ensure_default_model();
@@ -936,10 +998,12 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
     const ModelDev md = model_on_device(S, M);
     MlpWork w = mlp_work(S, kNumCand);
     float* Xraw = S.buf<float>(B_SEL_XRAW, (size_t)kNumCand * kIn);
+    float* Yall = S.buf<float>(B_SEL_YALL, (size_t)kNumCand * kOut * md.members);
     wp_features_device(S, T, n, Xraw, FeatNorm{md.xnorm, md.xnorm + kIn}, w.X);
-    mlp_forward(S, md.dev, w.X, kNumCand, w.H1, w.H2, w.Y);
+    for (int e = 0; e < md.members; e++)
+        mlp_forward(S, md.dev[e], w.X, kNumCand, w.H1, w.H2, Yall + (size_t)e * kNumCand * kOut);
     SelPick* pick = S.buf<SelPick>(B_SEL_PICK, 1);
-    k_sel_pick<<<1, 64, 0, S.stream>>>(w.Y, md.ynorm, pick);
+    k_sel_pick<<<1, 64, 0, S.stream>>>(Yall, md.members, md.ynorm, pick);
     MIBWT_LAUNCH_CHECK();
     SelPick h;
     S.read(&h, pick);

@@ -70,6 +70,29 @@ def random_text(rng: np.random.Generator, max_len: int, min_len: int = 4 << 20) 
     return idx, {"kind": str(kind), "copies": copies, "base_len": base_len, "sub": sub}
 
 
+def collect(mibwt, a, grid, rng, t0):
+    """Runs of every candidate on the generated texts, in the order: regular training texts,
+    large training texts, regular held-out, large held-out."""
+    Xs, Ys, MS, FX = [], [], [], []
+    sizes = ([(4 << 20, a.max_mib << 20)] * a.texts + [(96 << 20, a.large_max_mib << 20)] * a.large
+             + [(4 << 20, a.max_mib << 20)] * a.holdout
+             + [(96 << 20, a.large_max_mib << 20)] * a.large_holdout)
+    total = len(sizes)
+    for i, (lo, hi) in enumerate(sizes):
+        T, info = random_text(rng, hi, lo)
+        X, Y, ms, fms = run_grid(mibwt, T, grid)
+        if not np.isfinite(fms):
+            raise RuntimeError("W=5,P=16 itself is infeasible for a generated text")
+        Xs.append(X)
+        Ys.append(Y)
+        MS.append(ms)
+        FX.append(fms)
+        print(f"[train_selector] text {i + 1}/{total} n={len(T)} {info['kind']} best "
+              f"{grid[int(ms.argmin())]} {ms.min():.2f} ms, W5P16 {fms:.2f} ms "
+              f"({time.time() - t0:.0f} s)", flush=True)
+    return np.stack(Xs), np.stack(Ys), np.stack(MS), np.array(FX)
+
+
 def run_grid(mibwt, T: np.ndarray, grid):
     """features, targets and ms of every grid candidate, and the ms of fixed W=5,P=16.  A fresh
     storage per text: its grow-only buffers would otherwise keep the largest dictionary of every
@@ -119,10 +142,13 @@ def main():
     ap.add_argument("--large-holdout", type=int, default=0)
     ap.add_argument("--large-max-mib", type=int, default=384)
     ap.add_argument("--epochs", type=int, default=300)
+    ap.add_argument("--members", type=int, default=5, help="networks in the ensemble")
     ap.add_argument("--out", default=os.path.join(HERE, "selector_model.bin"))
     ap.add_argument("--data", default=None, help="write / reuse the collected runs (npz)")
     ap.add_argument("--report", default=None, help="write the held-out evaluation (json)")
     ap.add_argument("--seed", type=int, default=20261203)
+    ap.add_argument("--reuse", default=None,
+                    help="train on the runs of an earlier --data file instead of collecting new ones")
     ap.add_argument("--eval-workloads", default="",
                     help="comma-separated named workloads (gen.workload) evaluated, never trained on")
     a = ap.parse_args()
@@ -131,37 +157,24 @@ def main():
     rng = np.random.default_rng(a.seed)
     st = mibwt.Storage(1 << 20)  # model training and inference only
     t0 = time.time()
-    Xs, Ys, MS, FX, meta = [], [], [], [], []
-    # order: regular training texts, large training texts, regular held-out, large held-out
-    sizes = ([(4 << 20, a.max_mib << 20)] * a.texts + [(96 << 20, a.large_max_mib << 20)] * a.large
-             + [(4 << 20, a.max_mib << 20)] * a.holdout
-             + [(96 << 20, a.large_max_mib << 20)] * a.large_holdout)
-    total = len(sizes)
-    for i, (lo, hi) in enumerate(sizes):
-        T, info = random_text(rng, hi, lo)
-        X, Y, ms, fms = run_grid(mibwt, T, grid)
-        if not np.isfinite(fms):
-            raise RuntimeError("W=5,P=16 itself is infeasible for a generated text")
-        Xs.append(X)
-        Ys.append(Y)
-        MS.append(ms)
-        FX.append(fms)
-        info["n"] = len(T)
-        meta.append(info)
-        print(f"[train_selector] text {i + 1}/{total} n={len(T)} {info['kind']} best "
-              f"{grid[int(ms.argmin())]} {ms.min():.2f} ms, W5P16 {fms:.2f} ms "
-              f"({time.time() - t0:.0f} s)", flush=True)
-    X = np.stack(Xs)
-    Y = np.stack(Ys)
-    MS = np.stack(MS)
-    FX = np.array(FX)
-    if a.data:
-        np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX)
+    named_data = {}
+    if a.reuse:
+        d = np.load(a.reuse)
+        X, Y, MS, FX = d["X"], d["Y"], d["MS"], d["FX"]
+        for k in d.files:
+            if k.endswith("_X"):
+                name = k[:-2]
+                named_data[name] = (d[name + "_X"], d[name + "_MS"], float(d[name + "_FX"][0]))
+        assert len(X) == a.texts + a.large + a.holdout + a.large_holdout, "counts differ from the reused runs"
+    else:
+        X, Y, MS, FX = collect(mibwt, a, grid, rng, t0)
+        if a.data:
+            np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX)
     ntr = a.texts + a.large
     Xtr, Ytr = X[:ntr].reshape(-1, 64), Y[:ntr].reshape(-1, 3)
     ok = np.isfinite(Ytr).all(axis=1)
     loss = st.mlp_train(Xtr[ok], Ytr[ok], epochs=a.epochs, batch=512, lr=1e-3, seed=a.seed & 0xFFFF,
-                        out_path=a.out)
+                        out_path=a.out, members=a.members)
     print(f"[train_selector] trained on {int(ok.sum())} runs ({int((~ok).sum())} infeasible skipped), "
           f"final loss {loss:.4f}, model {a.out}", flush=True)
     mibwt.selector_load(a.out)
@@ -174,16 +187,25 @@ def main():
         from mibwt import gen
         named = {}
         for name in a.eval_workloads.split(","):
-            T = gen.workload(name)
-            Xw, _, msw, fmsw = run_grid(mibwt, T, grid)
+            if name in named_data:
+                Xw, msw, fmsw = named_data[name]
+            else:
+                T = gen.workload(name)
+                Xw, _, msw, fmsw = run_grid(mibwt, T, grid)
+                named_data[name] = (Xw, msw, fmsw)
+            n_w = int(round(2 ** (float(Xw[0, 0]) * 32)))  # feature 0 is log2(n) / 32
             ev = evaluate(mibwt, st, Xw[None], msw[None], np.array([fmsw]), grid, a.out)
             c = int(np.argmin(np.where(np.isfinite(msw), msw, np.inf)))
-            named[name] = {"n": len(T), "chosen_wp": ev["chosen_wp"][0], "best_wp": list(grid[c]),
+            named[name] = {"n": n_w, "chosen_wp": ev["chosen_wp"][0], "best_wp": list(grid[c]),
                            "best_ms": float(msw[c]), "w5p16_ms": float(fmsw),
                            "chosen_over_best": ev["chosen_over_best_mean"],
                            "speedup_over_w5p16": ev["speedup_over_w5p16_median"]}
             print(f"[train_selector] workload {name}: {named[name]}", flush=True)
         report["named_workloads"] = named
+        if a.data:  # the named workloads' grids beside the training runs
+            extra = {f"{k}_{f}": v for k, (Xw, msw, fmsw) in named_data.items()
+                     for f, v in (("X", Xw), ("MS", msw), ("FX", np.array([fmsw])))}
+            np.savez(a.data, X=X, Y=Y, MS=MS, FX=FX, **extra)
     report["train_runs"] = int(ok.sum())
     report["final_loss"] = loss
     print(json.dumps(report), flush=True)
@@ -196,22 +218,20 @@ def main():
 def evaluate(mibwt, st, X, MS, FX, grid, model_path):
     """Choice of the trained model on held-out texts (features X[t][candidates][64], measured
     ms, inf for a candidate whose dictionary outgrows the index range); the last candidate is
-    the fixed default W=5,P=16.  An infeasible choice is charged the library's fallback: a
-    failed attempt (counted as one more W=5,P=16 build) plus the W=5,P=16 build."""
-    with open(model_path, "rb") as f:
-        raw = np.frombuffer(f.read(), dtype=np.float32)
-    hdr = 4
-    xmean = raw[hdr:hdr + 64]
-    xstd = raw[hdr + 64:hdr + 128]
-    ymean = raw[hdr + 128:hdr + 131]
-    ystd = raw[hdr + 131:hdr + 134]
-    params = raw[hdr + 134:]
+    the fixed default W=5,P=16.  The choice is the library's: mean predicted time of the
+    ensemble members, smoothed over each grid candidate's P-neighbours, argmin.  An infeasible
+    choice is charged the library's fallback: a failed attempt (counted as one more W=5,P=16
+    build) plus the W=5,P=16 build."""
+    m = mibwt.read_selector_model(model_path)
+
+    def predicted(Xt):
+        Xn = (Xt - m["xmean"]) / m["xstd"]
+        t = np.mean([st.mlp_predict(p, Xn)[:, 2] for p in m["params"]], axis=0)
+        return mibwt.smooth_along_p(t * m["ystd"][2] + m["ymean"][2])
+
     ratios_best, ratios_fixed, ratios_oracle, picks = [], [], [], []
     for t in range(len(X)):
-        Xn = (X[t] - xmean) / xstd
-        y = st.mlp_predict(params, Xn)
-        pred_ms = y[:, 2] * ystd[2] + ymean[2]
-        c = int(np.argmin(pred_ms))
+        c = int(np.argmin(predicted(X[t])))
         picks.append(list(grid[c]))
         chosen = MS[t][c] if np.isfinite(MS[t][c]) else 2.0 * FX[t]
         ratios_best.append(chosen / MS[t].min())

@@ -83,6 +83,25 @@ def test_training_fits_a_smooth_target(storage, tmp_path):
 
 
 @pytest.mark.gpu
+def test_ensemble_training_writes_members(mibwt, storage, tmp_path):
+    # three members from consecutive seeds share one normalisation; each fits the target and
+    # they differ (different initialisations and shuffles)
+    rng = np.random.default_rng(4)
+    X = rng.normal(0, 1, (3000, 64)).astype(np.float32)
+    Y = np.stack([np.sin(X[:, 0]), X[:, 2] * X[:, 3], np.abs(X[:, 4])], 1).astype(np.float32)
+    path = str(tmp_path / "e.bin")
+    loss = storage.mlp_train(X, Y, epochs=40, batch=256, lr=2e-3, seed=5, out_path=path, members=3)
+    assert loss < 0.3
+    m = mibwt.read_selector_model(path)
+    assert m["members"] == 3 and m["params"].shape == (3, mlp.COUNT)
+    assert not np.array_equal(m["params"][0], m["params"][1])
+    Xn = (X[:200] - m["xmean"]) / m["xstd"]
+    for p in m["params"]:
+        y = storage.mlp_predict(p, Xn) * m["ystd"] + m["ymean"]
+        assert np.corrcoef(y[:, 0], Y[:200, 0])[0, 1] > 0.9
+
+
+@pytest.mark.gpu
 def test_selector_choice_keeps_bwt_exact(mibwt):
     from mibwt import gen
     import os
@@ -100,12 +119,12 @@ def test_selector_choice_keeps_bwt_exact(mibwt):
         assert np.isfinite(X).all() and X.shape == (41, 64)
         if trained:
             # the device-side choice equals the host path: raw features, normalisation from
-            # the model file, plain MLP inference, argmin
-            raw = np.fromfile(path, dtype=np.float32)
-            xm, xs = raw[4:68], raw[68:132]
-            ym, ys = raw[132:135], raw[135:138]
-            Y = st.mlp_predict(raw[138:], (X - xm) / xs)
-            t = Y[:, 2] * ys[2] + ym[2]
+            # the model file, plain MLP inference of every member, mean, smoothing along P,
+            # argmin
+            m = mibwt.read_selector_model(path)
+            Xn = (X - m["xmean"]) / m["xstd"]
+            t = np.mean([st.mlp_predict(p, Xn)[:, 2] for p in m["params"]], axis=0)
+            t = mibwt.smooth_along_p(t * m["ystd"][2] + m["ymean"][2])
             assert np.allclose(t, pred, rtol=0, atol=1e-4)
             assert mibwt.wp_grid()[int(np.argmin(pred))] == (W, P)
         L, idx = st.bwt(T, "cubwt")
@@ -173,6 +192,15 @@ def sketch_counts(T):
                 f16 = f >> np.uint64(32)
     f64 = f >> np.uint64(32)
     return len(sel), len(np.unique(f16)), len(np.unique(f64))
+
+
+def test_smoothing_along_p(mibwt):
+    # every grid candidate averages with its P-neighbours of the same W; the default candidate
+    # after the grid is left alone
+    t = np.arange(41, dtype=np.float64) ** 2
+    s = mibwt.smooth_along_p(t)
+    assert s[0] == (0 + 1) / 2 and s[9] == (64 + 81) / 2 and s[10] == (100 + 121) / 2
+    assert s[5] == (16 + 25 + 36) / 3 and s[40] == 1600
 
 
 def test_sketch_restatement_separates_repetitive_from_random():
