@@ -130,13 +130,11 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
     // members before this one and from it on, each counting keys < k and <= k (two compares and
     // two carry-adds per key): lt = all smaller keys, eq_before = equal keys before it
     uint32_t lt1 = 0, le1 = 0, lt2 = 0, le2 = 0;
-#pragma unroll 4
     for (uint32_t j = ls; j < li; j++) {
         const uint32_t kj = lk[j];
         lt1 += (kj < k) ? 1u : 0u;
         le1 += (kj <= k) ? 1u : 0u;
     }
-#pragma unroll 4
     for (uint32_t j = li; j < ls + z; j++) {
         const uint32_t kj = lk[j];
         lt2 += (kj < k) ? 1u : 0u;
