@@ -272,6 +272,7 @@ enum BufId : int {
     B_P_SA,
     B_G_OFF,
     B_G_MIX,
+    B_G_MIXBITS,
     B_G_ECNT,
     B_G_DELTA,
     B_MX_FLAG,
@@ -281,6 +282,8 @@ enum BufId : int {
     B_Q_OFF,
     B_Q_BLK,
     B_Q_PART,
+    B_Q_OCC,
+    B_Q_RREC,
     B_DB_LPART,
     B_E_HIST,
     B_E_PART,
@@ -302,6 +305,8 @@ enum BufId : int {
     B_DB_GOFS,
     B_DB_LARGE,
     B_DB_NGZ,
+    B_DB_R,
+    B_DB_NR,
     B_DB_GOFS0,
     B_LG_SZ,
     B_LG_K0,

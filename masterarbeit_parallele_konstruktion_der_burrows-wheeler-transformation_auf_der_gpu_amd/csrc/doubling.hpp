@@ -9,11 +9,16 @@
 //      the sub-group's first member), the SA slot and the sub-group record of every member;
 //   3. keeps the sub-groups that can still split (>= 2 members and Policy::open) and compacts
 //      them, which also yields next round's counts in one host read-back.
-// The loop stops when no group can split.  Policies:
+// The loop stops when no group can split.  A policy gives next(x, r) and open(x, r), where r
+// is a per-element value that travels with the element through the rounds when
+// Policy::kCarry (taken once from rem(x) at the start, so that no round gathers it):
 //   - strings of a family (PFP dictionary / parse): next(x) = x + h inside the segment,
 //     open while the compared prefix length h has not run past the strings (then equal
-//     strings stay in one group); h doubles every round;
+//     strings stay in one group); h doubles every round; for the dictionary r is the length
+//     of the rest of the phrase;
 //   - chains (Lyndon): next(x) = jump pointer, always open; pointers are squared per round.
+// SA receives an element's slot once, in the round in which its group stops (kept elements
+// are placed again later).
 #pragma once
 
 #include "prims.hpp"
@@ -48,25 +53,33 @@ template <typename Policy>
 __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                            const uint32_t* __restrict__ gofs, uint32_t m, uint32_t* __restrict__ rank,
                            uint32_t* __restrict__ SA, uint32_t* __restrict__ keep, uint32_t* __restrict__ NGz,
-                           Policy pol)
+                           uint32_t* __restrict__ NR, Policy pol)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t s = gofs[g], z = gofs[g + 1] - s;
     const uint32_t x = V[i];
+    uint32_t r = 0;
+    if constexpr (Policy::kCarry) {
+        r = pol.rem(x);
+        NR[i] = r;
+    }
     rank[x] = s + 1;
-    SA[i] = x;
-    keep[i] = (z > 1 && pol.open(x)) ? 1u : 0u;
+    const bool kp = z > 1 && pol.open(x, r);
+    if (!kp) SA[i] = x;
+    keep[i] = kp ? 1u : 0u;
     NGz[i] = z;
 }
 
-// first compaction: the group record of a kept element from the initial groups
+// first compaction: the group record of a kept element from the initial groups (NR is
+// null unless the policy carries a value)
 static __global__ void k_dbl_compact_init(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
                                           const uint32_t* __restrict__ V0, const uint32_t* __restrict__ gid1,
                                           const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ NGz,
-                                          uint32_t m, uint32_t* __restrict__ V, uint32_t* __restrict__ Hs,
-                                          uint32_t* __restrict__ Gs, uint32_t* __restrict__ Gz)
+                                          const uint32_t* __restrict__ NR, uint32_t m, uint32_t* __restrict__ V,
+                                          uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs,
+                                          uint32_t* __restrict__ Gz, uint32_t* __restrict__ R)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= m || !keep[p]) return;
@@ -76,94 +89,208 @@ static __global__ void k_dbl_compact_init(const uint32_t* __restrict__ keep, con
     Hs[o] = pos[s];
     Gs[o] = s;
     Gz[o] = NGz[p];
+    if (NR) R[o] = NR[p];
 }
 
 // ---- one refinement round over the m active elements (groups contiguous)
 
 template <typename Policy>
-__global__ void k_key2(const uint32_t* __restrict__ V, uint32_t m, const uint32_t* __restrict__ rank,
-                       Policy pol, uint32_t* __restrict__ key2)
+__global__ void k_key2(const uint32_t* __restrict__ V, const uint32_t* __restrict__ R, uint32_t m,
+                       const uint32_t* __restrict__ rank, Policy pol, uint32_t* __restrict__ key2)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
-    const uint32_t y = pol.next(V[i]);
+    uint32_t r = 0;
+    if constexpr (Policy::kCarry) r = R[i];
+    const uint32_t y = pol.next(V[i], r);
     key2[i] = (y == kNoNext) ? 0u : rank[y];
 }
 
-// Small groups: each member counts the members ordered before it (key, then slot), moves to
-// that slot and records its new sub-group (members with an equal key): rank = global start of
-// the sub-group + 1, SA slot, and the sub-group record for the compaction.  A small group that
-// has a member in the block lies inside [b0 - kSmallGroup, b0 + kBlock + kSmallGroup), so the
-// keys of that window are staged in LDS and every count reads LDS (members of one group read
-// the same words: broadcasts).  Members of large groups are only flagged.
-static_assert(kSmallGroup <= kBlock, "the LDS key window assumes small groups of at most one block");
+// Small groups (at most kSmallGroup members) are sorted by the block that holds their first
+// slot: such a group lies inside [b0, b0 + 2 * kBlock).  Every member stages (key << 32 | slot)
+// in LDS and counts the members of its group with a smaller value (one 64-bit compare per
+// member, 16-byte LDS reads, broadcast within a group), which is its place in the sorted
+// group; sub-group starts are then a max-scan of the heads of equal-key runs, sub-group ends
+// are recorded by the last member of each run.  Outputs are written in sorted slot order:
+// rank = global start of the sub-group + 1, the SA slot, the sub-group record and the keep
+// flag.  Members of larger groups are only flagged (per slot of the block's own range).
+static_assert(kSmallGroup <= kBlock, "a small group must fit in the block's extended range");
+
+// number of staged values in wk[a, a + z) below c; wk has at least a + z + 1 entries when
+// a + z is odd (the pair-aligned reads cover one neighbour on each side, subtracted again)
+__device__ __forceinline__ uint32_t count_below(const uint64_t* wk, uint32_t a, uint32_t z, uint64_t c)
+{
+    uint32_t j = a & ~1u;
+    const uint32_t e = (a + z + 1) & ~1u;
+    uint32_t n = 0;
+    for (; j + 8 <= e; j += 8) {
+        const ulonglong2 q0 = *reinterpret_cast<const ulonglong2*>(wk + j);
+        const ulonglong2 q1 = *reinterpret_cast<const ulonglong2*>(wk + j + 2);
+        const ulonglong2 q2 = *reinterpret_cast<const ulonglong2*>(wk + j + 4);
+        const ulonglong2 q3 = *reinterpret_cast<const ulonglong2*>(wk + j + 6);
+        n += (q0.x < c) + (q0.y < c) + (q1.x < c) + (q1.y < c) + (q2.x < c) + (q2.y < c) + (q3.x < c) +
+             (q3.y < c);
+    }
+    for (; j < e; j += 2) {
+        const ulonglong2 q = *reinterpret_cast<const ulonglong2*>(wk + j);
+        n += (q.x < c) + (q.y < c);
+    }
+    if (a & 1u) n -= (wk[a - 1] < c) ? 1u : 0u;
+    if ((a + z) & 1u) n -= (wk[a + z] < c) ? 1u : 0u;
+    return n;
+}
+
 template <typename Policy>
 __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
                                                      const uint32_t* __restrict__ Gs, const uint32_t* __restrict__ Gz,
-                                                     const uint32_t* __restrict__ key2, uint32_t m,
-                                                     uint32_t small_max, uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ key2, const uint32_t* __restrict__ R,
+                                                     uint32_t m, uint32_t small_max, uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
                                                      uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs,
                                                      uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
-                                                     uint32_t* __restrict__ large, Policy pol)
+                                                     uint32_t* __restrict__ NR, uint32_t* __restrict__ large, Policy pol)
 {
-    constexpr uint32_t kWin = kBlock + 2 * kSmallGroup;
-    __shared__ uint32_t lk[kWin];
+    constexpr uint32_t kSpan = 2 * kBlock;
+    constexpr uint32_t kSr = Policy::kCarry ? kSpan : 1;
+    __shared__ __align__(16) uint64_t wk[kSpan];
+    __shared__ uint32_t sk[kSpan];    // key at sorted slot
+    __shared__ uint32_t sx[kSpan];    // element at sorted slot
+    __shared__ uint32_t send[kSpan];  // end of the sub-group that starts at a slot
+    __shared__ uint32_t sr[kSr];      // carried value at sorted slot
+    __shared__ uint32_t tail[4];      // group crossing b0 + kBlock: start, size, global start, end
+    __shared__ uint32_t red[2 * kWaves];
     const uint32_t b0 = blockIdx.x * kBlock;
-    const int64_t w0 = (int64_t)b0 - kSmallGroup;  // slot of lk[0]
-    for (uint32_t t = threadIdx.x; t < kWin; t += kBlock) {
-        const int64_t j = w0 + t;
-        lk[t] = (j >= 0 && j < (int64_t)m) ? key2[j] : 0u;
+    const uint32_t t = threadIdx.x;
+
+    // ---- own slots: classification and staging
+    const uint32_t i0 = b0 + t;
+    bool own0 = false;
+    uint32_t a0 = 0, z0 = 0, gs0 = 0, x0 = 0, k0 = 0, r0 = 0;
+    if (i0 < m) {
+        const uint32_t s = Hs[i0];
+        z0 = Gz[i0];
+        const bool lg = z0 > small_max || z0 > kSmallGroup;
+        large[i0] = lg ? 1u : 0u;
+        own0 = !lg && s >= b0;
+        if (own0) {
+            a0 = s - b0;
+            gs0 = Gs[i0];
+            x0 = V[i0];
+            k0 = key2[i0];
+            if constexpr (Policy::kCarry) r0 = R[i0];
+            wk[t] = ((uint64_t)k0 << 32) | t;
+        }
+    }
+    if (t == kBlock - 1) {
+        const bool cross = own0 && a0 + z0 > kBlock;
+        tail[0] = a0;
+        tail[1] = z0;
+        tail[2] = gs0;
+        tail[3] = cross ? a0 + z0 : kBlock;
     }
     __syncthreads();
-    const uint32_t i = b0 + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t z = Gz[i];
-    if (z > small_max || z > kSmallGroup) {
-        large[i] = 1;
-        return;
+
+    // ---- the rest of the group that crosses b0 + kBlock (inside the grid's range)
+    const uint32_t p1 = kBlock + t;
+    const bool own1 = p1 < tail[3];
+    uint32_t a1 = 0, z1 = 0, gs1 = 0, x1 = 0, k1 = 0, r1 = 0;
+    if (own1) {
+        a1 = tail[0];
+        z1 = tail[1];
+        gs1 = tail[2];
+        x1 = V[b0 + p1];
+        k1 = key2[b0 + p1];
+        if constexpr (Policy::kCarry) r1 = R[b0 + p1];
+        wk[p1] = ((uint64_t)k1 << 32) | p1;
     }
-    large[i] = 0;
-    const uint32_t s = Hs[i];
-    const uint32_t k = lk[i - w0];
-    const uint32_t ls = (uint32_t)((int64_t)s - w0), li = (uint32_t)((int64_t)i - w0);
-    // members before this one and from it on, each counting keys < k and <= k (two compares and
-    // two carry-adds per key): lt = all smaller keys, eq_before = equal keys before it
-    uint32_t lt1 = 0, le1 = 0, lt2 = 0, le2 = 0;
-    for (uint32_t j = ls; j < li; j++) {
-        const uint32_t kj = lk[j];
-        lt1 += (kj < k) ? 1u : 0u;
-        le1 += (kj <= k) ? 1u : 0u;
+    __syncthreads();
+
+    // ---- place in the sorted group
+    if (own0) {
+        const uint32_t q = a0 + count_below(wk, a0, z0, ((uint64_t)k0 << 32) | t);
+        sk[q] = k0;
+        sx[q] = x0;
+        if constexpr (Policy::kCarry) sr[q] = r0;
     }
-    for (uint32_t j = li; j < ls + z; j++) {
-        const uint32_t kj = lk[j];
-        lt2 += (kj < k) ? 1u : 0u;
-        le2 += (kj <= k) ? 1u : 0u;
+    if (own1) {
+        const uint32_t q = a1 + count_below(wk, a1, z1, ((uint64_t)k1 << 32) | p1);
+        sk[q] = k1;
+        sx[q] = x1;
+        if constexpr (Policy::kCarry) sr[q] = r1;
     }
-    const uint32_t lt = lt1 + lt2, eq_before = le1 - lt1, eq = eq_before + (le2 - lt2);
-    const uint32_t x = V[i];
-    const uint32_t gs = Gs[i];
-    const uint32_t p = s + lt + eq_before;
-    rank[x] = gs + lt + 1;
-    SA[gs + lt + eq_before] = x;
-    keep[p] = (eq > 1 && pol.open(x)) ? 1u : 0u;
-    NV[p] = x;
-    NHs[p] = s + lt;
-    NGs[p] = gs + lt;
-    NGz[p] = eq;
+    __syncthreads();
+
+    // ---- sub-groups: heads of equal-key runs, their starts by a max-scan over both halves
+    const uint32_t p0 = t;
+    uint32_t kk0 = 0, kk1 = 0;
+    uint32_t h0 = 0, h1 = 0;
+    if (own0) {
+        kk0 = sk[p0];
+        h0 = (p0 == a0 || sk[p0 - 1] != kk0) ? p0 : 0u;
+    }
+    if (own1) {
+        kk1 = sk[p1];
+        h1 = (p1 == a1 || sk[p1 - 1] != kk1) ? p1 : 0u;
+    }
+    const uint32_t lane = t & 63, wid = t >> 6;
+    uint32_t c0 = wave_inclusive_scan(h0, OpMax{});
+    uint32_t c1 = wave_inclusive_scan(h1, OpMax{});
+    if (lane == 63) {
+        red[wid] = c0;
+        red[kWaves + wid] = c1;
+    }
+    __syncthreads();
+    uint32_t pre0 = 0, pre1 = 0;
+#pragma unroll
+    for (int w = 0; w < kWaves; w++) {
+        const uint32_t r0 = red[w], r1 = red[kWaves + w];
+        pre1 = r0 > pre1 ? r0 : pre1;  // the whole first half precedes the second
+        if (w < (int)wid) {
+            pre0 = r0 > pre0 ? r0 : pre0;
+            pre1 = r1 > pre1 ? r1 : pre1;
+        }
+    }
+    const uint32_t ss0 = c0 > pre0 ? c0 : pre0;
+    const uint32_t ss1 = c1 > pre1 ? c1 : pre1;
+    if (own0 && (p0 + 1 == a0 + z0 || sk[p0 + 1] != kk0)) send[ss0] = p0 + 1;
+    if (own1 && (p1 + 1 == a1 + z1 || sk[p1 + 1] != kk1)) send[ss1] = p1 + 1;
+    __syncthreads();
+
+    // ---- outputs in sorted slot order
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+        const bool own = h ? own1 : own0;
+        if (!own) continue;
+        const uint32_t p = h ? p1 : p0, a = h ? a1 : a0, gs = h ? gs1 : gs0, ss = h ? ss1 : ss0;
+        const uint32_t eq = send[ss] - ss, lt = ss - a;
+        const uint32_t x = sx[p], g = b0 + p;
+        uint32_t r = 0;
+        if constexpr (Policy::kCarry) {
+            r = sr[p];
+            NR[g] = r;
+        }
+        rank[x] = gs + lt + 1;
+        const bool kp = eq > 1 && pol.open(x, r);
+        if (!kp) SA[gs + (p - a)] = x;
+        keep[g] = kp ? 1u : 0u;
+        NV[g] = x;
+        NHs[g] = b0 + ss;
+        NGs[g] = gs + lt;
+        NGz[g] = eq;
+    }
 }
 
-// Large groups: (group start slot, key) pairs of the flagged members in slot order.
-static __global__ void k_seg_large_gather(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
-                                          const uint32_t* __restrict__ key2, const uint32_t* __restrict__ large,
-                                          const uint32_t* __restrict__ lpos, uint32_t m, int B,
-                                          uint64_t* __restrict__ LK, uint32_t* __restrict__ LV)
+// Large groups: (group start slot, key) pairs of the flagged members in slot order, with the
+// member's active slot as the value.
+static __global__ void k_seg_large_gather(const uint32_t* __restrict__ Hs, const uint32_t* __restrict__ key2,
+                                          const uint32_t* __restrict__ large, const uint32_t* __restrict__ lpos,
+                                          uint32_t m, int B, uint64_t* __restrict__ LK, uint32_t* __restrict__ LV)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m || !large[i]) return;
     const uint32_t q = lpos[i];
     LK[q] = ((uint64_t)Hs[i] << B) | key2[i];
-    LV[q] = V[i];
+    LV[q] = i;
 }
 
 static __global__ void k_lg_heads(const uint64_t* __restrict__ LK, uint32_t nl, uint32_t* __restrict__ hq)
@@ -185,22 +312,30 @@ static __global__ void k_lg_sizes(const uint64_t* __restrict__ LK, const uint32_
 template <typename Policy>
 __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
                             const uint32_t* __restrict__ qh, const uint32_t* __restrict__ sz,
-                            const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ Gs, uint32_t nl, int B,
+                            const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ V,
+                            const uint32_t* __restrict__ R, const uint32_t* __restrict__ Gs, uint32_t nl, int B,
                             uint32_t* __restrict__ rank, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
                             uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs,
-                            uint32_t* __restrict__ NGz, Policy pol)
+                            uint32_t* __restrict__ NGz, uint32_t* __restrict__ NR, Policy pol)
 {
     const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nl) return;
-    const uint32_t x = LV[q];
+    const uint32_t i = LV[q];
+    const uint32_t x = V[i];
+    uint32_t r = 0;
     const uint32_t s = (uint32_t)(LK[q] >> B);  // group start slot (a member of the group)
     const uint32_t q0 = lpos[s];                // first index of the group among the large ones
     const uint32_t lt = qh[q] - q0, c = sz[qh[q]];
     const uint32_t gs = Gs[s];
     const uint32_t p = s + (q - q0);
+    if constexpr (Policy::kCarry) {
+        r = R[i];
+        NR[p] = r;
+    }
     rank[x] = gs + lt + 1;
-    SA[gs + (q - q0)] = x;
-    keep[p] = (c > 1 && pol.open(x)) ? 1u : 0u;
+    const bool kp = c > 1 && pol.open(x, r);
+    if (!kp) SA[gs + (q - q0)] = x;
+    keep[p] = kp ? 1u : 0u;
     NV[p] = x;
     NHs[p] = s + lt;
     NGs[p] = gs + lt;
@@ -290,9 +425,10 @@ static __global__ __launch_bounds__(kBlock) void k_sum_tiles(const uint32_t* __r
 // first slot is kept with it and maps through pos as well)
 static __global__ void k_dbl_compact(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
                                      const uint32_t* __restrict__ NV, const uint32_t* __restrict__ NHs,
-                                     const uint32_t* __restrict__ NGs, const uint32_t* __restrict__ NGz, uint32_t m,
-                                     uint32_t* __restrict__ V, uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs,
-                                     uint32_t* __restrict__ Gz)
+                                     const uint32_t* __restrict__ NGs, const uint32_t* __restrict__ NGz,
+                                     const uint32_t* __restrict__ NR, uint32_t m, uint32_t* __restrict__ V,
+                                     uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs, uint32_t* __restrict__ Gz,
+                                     uint32_t* __restrict__ R)
 {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= m || !keep[p]) return;
@@ -301,6 +437,7 @@ static __global__ void k_dbl_compact(const uint32_t* __restrict__ keep, const ui
     Hs[o] = pos[NHs[p]];
     Gs[o] = NGs[p];
     Gz[o] = NGz[p];
+    if (NR) R[o] = NR[p];
 }
 
 struct DoublingResult {
@@ -336,6 +473,12 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t* key2 = S.buf<uint32_t>(B_DB_KEY2, N);
     uint32_t* large = S.buf<uint32_t>(B_DB_LARGE, N);
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    uint32_t* R = nullptr;
+    uint32_t* NR = nullptr;
+    if constexpr (Policy::kCarry) {
+        R = S.buf<uint32_t>(B_DB_R, N);
+        NR = S.buf<uint32_t>(B_DB_NR, N);
+    }
     const int B = bits_for((uint64_t)N + 1);
 
     // initial groups: runs of equal keys (gid1 lives in `large` until the first compaction)
@@ -348,7 +491,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         inclusive_sum<uint32_t>(S, hf, gid1, N);
         k_grp_offsets<<<grid_for(N), kBlock, 0, S.stream>>>(hf, gid1, N, gofs);
         MIBWT_LAUNCH_CHECK();
-        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NGz, pol);
+        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NGz, NR, pol);
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)N * (8 * 2 + 4 * 9);
     }
@@ -372,20 +515,22 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         res.rounds++;
         if (m2 == 0) break;
         if (first) {
-            k_dbl_compact_init<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, V0, gid1, gofs, NGz, m, V, Hs, Gs, Gz);
+            k_dbl_compact_init<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, V0, gid1, gofs, NGz, NR, m, V, Hs,
+                                                                     Gs, Gz, R);
             first = false;
         } else {
-            k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, m, V, Hs, Gs, Gz);
+            k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, NR, m, V, Hs, Gs, Gz,
+                                                                R);
         }
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)m * (4 * 12) + (double)m2 * 16;
         m = m2;
         // ---- order each group by the rank of the next position
-        k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, m, rank, pol, key2);
+        k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, R, m, rank, pol, key2);
         MIBWT_LAUNCH_CHECK();
         advance(pol, V, m);
-        k_seg_small<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, m, kSmallGroup, rank, SA, keep,
-                                                                  NV, NHs, NGs, NGz, large, pol);
+        k_seg_small<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, m, kSmallGroup, rank, SA,
+                                                                  keep, NV, NHs, NGs, NGz, NR, large, pol);
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)m * (4 * 12);
         if (nl > 0) {
@@ -397,7 +542,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
             uint32_t* qh = S.buf<uint32_t>(B_LG_I, nl);
             uint32_t* sz = S.buf<uint32_t>(B_LG_SZ, nl);
-            k_seg_large_gather<<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, key2, large, lpos, m, B, LK0, LV0);
+            k_seg_large_gather<<<grid_for(m), kBlock, 0, S.stream>>>(Hs, key2, large, lpos, m, B, LK0, LV0);
             MIBWT_LAUNCH_CHECK();
             const int kb = B + bits_for(m);
             radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, kb);
@@ -406,8 +551,8 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             inclusive_max<uint32_t>(S, qh, qh, nl);
             k_lg_sizes<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, qh, nl, sz);
             MIBWT_LAUNCH_CHECK();
-            k_lg_update<Policy><<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, qh, sz, lpos, Gs, nl, B, rank, SA,
-                                                                       keep, NV, NHs, NGs, NGz, pol);
+            k_lg_update<Policy><<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, qh, sz, lpos, V, R, Gs, nl, B, rank,
+                                                                       SA, keep, NV, NHs, NGs, NGz, NR, pol);
             MIBWT_LAUNCH_CHECK();
             res.large_items += nl;
             res.bytes += (double)nl * (12 * 2 * ((kb + 7) / 8) + 48);

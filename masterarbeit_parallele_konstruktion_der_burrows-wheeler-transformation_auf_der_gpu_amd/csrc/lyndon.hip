@@ -502,14 +502,16 @@ __global__ void k_u32_to_key(const uint32_t* __restrict__ a, uint64_t n, uint64_
 // ------------------------------------------------------------------- 4. induction
 
 struct ChainPolicy {
+    static constexpr bool kCarry = false;
     const uint32_t* jmp;
     uint32_t n;
-    __device__ __forceinline__ uint32_t next(uint32_t x) const
+    __device__ __forceinline__ uint32_t rem(uint32_t) const { return 0; }
+    __device__ __forceinline__ uint32_t next(uint32_t x, uint32_t) const
     {
         const uint32_t y = jmp[x];
         return y < n ? y : kNoNext;
     }
-    __device__ __forceinline__ bool open(uint32_t) const { return true; }
+    __device__ __forceinline__ bool open(uint32_t, uint32_t) const { return true; }
 };
 
 __global__ void k_jump_init(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ a,

@@ -537,10 +537,12 @@ __global__ void k_dict_freq(const uint32_t* __restrict__ head, uint32_t du, uint
 // Segment limits of a string family.  Dictionary: the suffix at DT position x ends at the
 // end of its phrase.  Parse: one string of length d.
 struct LimDict {
+    static constexpr bool kCarry = true;  // the rest length travels with the element
     const uint32_t* lim;  // end (exclusive) of the phrase of each DT position, precomputed
     __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return lim[x]; }
 };
 struct LimParse {
+    static constexpr bool kCarry = false;
     uint32_t d;
     __device__ __forceinline__ uint32_t operator()(uint32_t) const { return d; }
 };
@@ -606,15 +608,24 @@ __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint
 // Family of strings compared h symbols at a time: suffix x continues at x + h inside its
 // segment [.., lim(x)); a group of equal h-prefixes is final once its strings are shorter
 // than h (they are then equal).
+// With Lim::kCarry the element carries r = lim(x) - x (the rest of its string) and no round
+// reads lim.
 template <typename Lim>
 struct StringPolicy {
+    static constexpr bool kCarry = Lim::kCarry;
     Lim lim;
     uint32_t h;
-    __device__ __forceinline__ uint32_t next(uint32_t x) const
+    __device__ __forceinline__ uint32_t rem(uint32_t x) const { return lim(x) - x; }
+    __device__ __forceinline__ uint32_t next(uint32_t x, uint32_t r) const
     {
-        return ((uint64_t)x + h < lim(x)) ? x + h : kNoNext;
+        if constexpr (kCarry) return h < r ? x + h : kNoNext;
+        else return ((uint64_t)x + h < lim(x)) ? x + h : kNoNext;
     }
-    __device__ __forceinline__ bool open(uint32_t xh) const { return lim(xh) - xh >= h; }
+    __device__ __forceinline__ bool open(uint32_t x, uint32_t r) const
+    {
+        if constexpr (kCarry) return r >= h;
+        else return lim(x) - x >= h;
+    }
 };
 
 // Prefix doubling over N suffix positions: K0/V0 hold the initial keys (bits [0, key_bits),
@@ -716,45 +727,66 @@ __global__ void k_asm_counts(const uint64_t* __restrict__ R, const uint32_t* __r
 
 // Digit histograms of the entry sort, weighted from the groups instead of read from the
 // entries: every sorted position g contributes ecnt[g] entries keyed by the dense id of its
-// group head, cid[Hd[g]] (nondecreasing in g).  Each thread walks kEhPer consecutive positions and adds runs of equal
-// digits at once; per-block counts go to part[block][kEhPasses * 256].
-constexpr int kEhPer = 64, kEhPasses = 4;
-__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ cid,
+// group head, cid[Hd[g]], which is cid[g] + gflag[g] - 1 (groups are contiguous and only
+// heads carry flags), nondecreasing in g.  A block stages tiles of kEhTile positions in LDS
+// with coalesced loads, then every thread walks kEhPer consecutive positions and adds runs of
+// equal digits at once; blocks stride over the tiles and write their counts to
+// part[block][kEhPasses * 256].
+constexpr int kEhPer = 16, kEhPasses = 4, kEhTile = kBlock * kEhPer, kEhBlocks = 2048;
+__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ gflag,
+                                                      const uint32_t* __restrict__ cid,
                                                       const uint32_t* __restrict__ ecnt, uint32_t L, int passes,
                                                       uint32_t* __restrict__ part)
 {
+    constexpr int kPad = kEhTile + kEhTile / kEhPer;  // one pad word per kEhPer: conflict-free walks
     __shared__ uint32_t h[kEhPasses * kRadix];
+    __shared__ uint32_t lc[kPad], le[kPad];
     for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) h[i] = 0;
-    __syncthreads();
-    const uint64_t g0 = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kEhPer;
-    uint32_t prev[kEhPasses], run[kEhPasses];
-#pragma unroll
-    for (int p = 0; p < kEhPasses; p++) {
-        prev[p] = 0xFFFFFFFFu;
-        run[p] = 0;
-    }
-    for (int i = 0; i < kEhPer; i++) {
-        const uint64_t g = g0 + i;
-        if (g >= L) break;
-        const uint32_t c = ecnt[g];
-        if (c == 0) continue;
-        const uint32_t hd = cid[Hd[g]];
+    const uint32_t tiles = (uint32_t)(((uint64_t)L + kEhTile - 1) / kEhTile);
+    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
+        const uint64_t base = (uint64_t)tile * kEhTile;
+        __syncthreads();  // the previous tile's walks are done
+#pragma unroll 4
+        for (int i = 0; i < kEhPer; i++) {
+            const uint32_t j = i * kBlock + threadIdx.x;
+            const uint64_t g = base + j;
+            uint32_t c = 0, e = 0;
+            if (g < L) {
+                c = ecnt[g];
+                if (c) e = cid[g] + gflag[g] - 1;
+            }
+            lc[j + j / kEhPer] = c;
+            le[j + j / kEhPer] = e;
+        }
+        __syncthreads();
+        uint32_t prev[kEhPasses], run[kEhPasses];
 #pragma unroll
         for (int p = 0; p < kEhPasses; p++) {
-            if (p >= passes) break;
-            const uint32_t d = (hd >> (8 * p)) & 0xFFu;
-            if (d == prev[p]) {
-                run[p] += c;
-            } else {
-                if (run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
-                prev[p] = d;
-                run[p] = c;
+            prev[p] = 0xFFFFFFFFu;
+            run[p] = 0;
+        }
+        const uint32_t w0 = threadIdx.x * (kEhPer + 1);
+        for (int i = 0; i < kEhPer; i++) {
+            const uint32_t c = lc[w0 + i];
+            if (c == 0) continue;
+            const uint32_t e = le[w0 + i];
+#pragma unroll
+            for (int p = 0; p < kEhPasses; p++) {
+                if (p >= passes) break;
+                const uint32_t d = (e >> (8 * p)) & 0xFFu;
+                if (d == prev[p]) {
+                    run[p] += c;
+                } else {
+                    if (run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
+                    prev[p] = d;
+                    run[p] = c;
+                }
             }
         }
-    }
 #pragma unroll
-    for (int p = 0; p < kEhPasses; p++)
-        if (p < passes && run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
+        for (int p = 0; p < kEhPasses; p++)
+            if (p < passes && run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) part[(uint64_t)blockIdx.x * kEhPasses * kRadix + i] = h[i];
 }
@@ -860,8 +892,18 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict_
     }
 }
 
+// mixed-group flags as a bitmap (bit g of word g / 64), small enough to stay cached for the
+// lookups in dictionary order
+__global__ void k_mix_bits(const uint8_t* __restrict__ gmix, uint32_t L, uint64_t* __restrict__ bits)
+{
+    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool b = g < L && gmix[g];
+    const uint64_t w = __ballot(b);
+    if ((threadIdx.x & 63) == 0 && g < L) bits[g >> 6] = w;
+}
+
 // mixed-group members in dictionary order (grouped by phrase)
-__global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint8_t* __restrict__ gmix, uint32_t L,
+__global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint64_t* __restrict__ mixbits, uint32_t L,
                           uint32_t* __restrict__ mxf)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -871,7 +913,8 @@ __global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint8_t* __restr
         return;
     }
     const AsmRec a = rec[x];
-    mxf[x] = ((a.r & kRecAlpha) && !(a.r & kRecWhole) && gmix[a.head]) ? 1u : 0u;
+    const bool cand = (a.r & kRecAlpha) && !(a.r & kRecWhole);
+    mxf[x] = (cand && ((mixbits[a.head >> 6] >> (a.head & 63)) & 1u)) ? 1u : 0u;
 }
 
 // BWT entries: group (sorted position of the dictionary group head) above the output byte.
@@ -932,17 +975,6 @@ __global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* _
     pr[u] = PhraseRec{a, b - a, cid[rec[o].head], 0};
 }
 
-// entries per parse-suffix rank r: the occurrence k = psa[r]-1 contributes its whole-phrase
-// entry and one entry per mixed suffix of its phrase
-__global__ void k_q_count(const uint32_t* __restrict__ psa, uint32_t d, const uint32_t* __restrict__ pid,
-                          const PhraseRec* __restrict__ pr, uint32_t* __restrict__ qc)
-{
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= d) return;
-    const uint32_t k1 = psa[r];
-    qc[r] = (k1 == 0) ? 0u : 1u + pr[pid[k1 - 1]].mx_count;
-}
-
 template <typename EntT>
 __device__ __forceinline__ EntT whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
                                                    uint64_t n, int W, uint32_t k, uint32_t alpha0)
@@ -953,6 +985,51 @@ __device__ __forceinline__ EntT whole_phrase_entry(const uint32_t* __restrict__ 
     EntT v = Ent<EntT>::make(alpha0, c);
     if (sizeof(EntT) == 8 && k == 0) v |= (EntT)0x100;
     return v;
+}
+
+// Per occurrence k (parse order, sequential reads): its whole-phrase entry and its phrase's
+// run of mixed entries.
+template <typename EntT>
+struct alignas(16) OccRec {
+    EntT whole;
+    uint32_t mx_begin, mx_count;
+};
+
+template <typename EntT>
+__global__ void k_occ_rec(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
+                          const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
+                          OccRec<EntT>* __restrict__ occ)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= d) return;
+    const PhraseRec q = pr[pid[k]];
+    occ[k] = OccRec<EntT>{whole_phrase_entry<EntT>(E, T, n, W, k, q.alpha0), q.mx_begin, q.mx_count};
+}
+
+// Per parse-suffix rank r (one gather of the occurrence k = psa[r]-1): the number of entries
+// (its whole-phrase entry and one per mixed suffix of its phrase; none for the suffix that
+// starts at phrase 0, psa[r] == 0) and, in rank order, its whole-phrase entry and first
+// mixed entry.
+template <typename EntT>
+struct alignas(sizeof(EntT) == 4 ? 8 : 16) RankRec {
+    EntT whole;
+    uint32_t mx_begin;
+};
+
+template <typename EntT>
+__global__ void k_q_count(const uint32_t* __restrict__ psa, uint32_t d, const OccRec<EntT>* __restrict__ occ,
+                          uint32_t* __restrict__ qc, RankRec<EntT>* __restrict__ qrec)
+{
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= d) return;
+    const uint32_t k1 = psa[r];
+    if (k1 == 0) {
+        qc[r] = 0;
+        return;
+    }
+    const OccRec<EntT> o = occ[k1 - 1];
+    qc[r] = 1u + o.mx_count;
+    qrec[r] = RankRec<EntT>{o.whole, o.mx_begin};
 }
 
 // The primary-index entry (whole phrase of occurrence 0): its group and its index among the
@@ -967,9 +1044,11 @@ __global__ void k_q_special(const uint32_t* __restrict__ prank, const uint32_t* 
 
 // Load-balanced generation of the entries: every block writes kQBlock consecutive entries.
 // k_q_blocks records, per block, the parse-suffix rank whose entries contain the block's first
-// output; the block then maps each output o to its rank r (bisection over the ranks' offsets
-// in LDS) and to j = o - qo[r]: j == 0 is the occurrence's whole-phrase entry, j >= 1 the
-// (j-1)-th mixed suffix of its phrase.  Writes are fully coalesced.
+// output.  The block stages the offsets of the following ranks and, for the ranks whose
+// entries reach into the block, their whole-phrase entry and first mixed entry (coalesced
+// reads in rank order); each output o then finds its rank r by bisection over the offsets and takes
+// j = o - qo[r]: j == 0 is the whole-phrase entry, j >= 1 the (j-1)-th mixed suffix of the
+// phrase.  Writes are fully coalesced.
 constexpr int kQBlock = 1024;
 
 __global__ void k_q_blocks(const uint32_t* __restrict__ qo, const uint32_t* __restrict__ qc, uint32_t d,
@@ -982,23 +1061,33 @@ __global__ void k_q_blocks(const uint32_t* __restrict__ qo, const uint32_t* __re
 }
 
 template <typename EntT>
-__global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict__ psa, uint32_t d,
-                                                    const uint32_t* __restrict__ pid,
-                                                    const PhraseRec* __restrict__ pr,
-                                                    const EntT* __restrict__ mx, const uint32_t* __restrict__ E,
-                                                    const uint8_t* __restrict__ T, uint64_t n, int W,
+__global__ __launch_bounds__(kBlock) void k_q_expand(const RankRec<EntT>* __restrict__ qrec, uint32_t d,
+                                                    const EntT* __restrict__ mx,
                                                     const uint32_t* __restrict__ qo, const uint32_t* __restrict__ rb,
                                                     uint32_t total, EntT* __restrict__ ent,
                                                     const uint32_t* __restrict__ special, uint32_t* __restrict__ part)
 {
-    __shared__ uint32_t lq[kQBlock + 2];
+    constexpr uint32_t kCap = kQBlock + 2;
+    __shared__ uint32_t lq[kCap];
+    __shared__ uint32_t lmx[kCap];
+    __shared__ EntT lwe[kCap];
     __shared__ uint32_t red[kWaves];
     const uint32_t o0 = blockIdx.x * kQBlock;
+    const uint32_t oend = (total - o0 < (uint32_t)kQBlock) ? total : o0 + kQBlock;
     const uint32_t r0 = rb[blockIdx.x];
     // every rank but one (the suffix starting at phrase 0) has at least one entry, so
     // kQBlock + 2 ranks cover the block's outputs
-    const uint32_t cnt = (d - r0 < (uint32_t)kQBlock + 2) ? d - r0 : (uint32_t)kQBlock + 2;
-    for (uint32_t t = threadIdx.x; t < cnt; t += kBlock) lq[t] = qo[r0 + t];
+    const uint32_t cnt = (d - r0 < kCap) ? d - r0 : kCap;
+    for (uint32_t t = threadIdx.x; t < cnt; t += kBlock) {
+        const uint32_t q = qo[r0 + t];
+        lq[t] = q;
+        if (q < oend) {
+            // ranks without entries carry no record; they are never chosen below
+            const RankRec<EntT> rr = qrec[r0 + t];
+            lmx[t] = rr.mx_begin;
+            lwe[t] = rr.whole;
+        }
+    }
     __syncthreads();
     uint32_t g0 = 0, s0 = 0, before = 0;
     if (part) {
@@ -1015,12 +1104,10 @@ __global__ __launch_bounds__(kBlock) void k_q_expand(const uint32_t* __restrict_
             if (lq[m] <= o) a = m;
             else b = m - 1;
         }
-        // ranks without entries share their successor's offset: take the last of equal offsets
-        const uint32_t r = r0 + a;
+        // ranks without entries share their successor's offset: the bisection takes the last
+        // of equal offsets, a rank with entries
         const uint32_t j = o - lq[a];
-        const uint32_t k = psa[r] - 1;
-        const PhraseRec q = pr[pid[k]];
-        const EntT v = (j == 0) ? whole_phrase_entry<EntT>(E, T, n, W, k, q.alpha0) : mx[q.mx_begin + j - 1];
+        const EntT v = (j == 0) ? lwe[a] : mx[lmx[a] + j - 1];
         ent[o] = v;
         if (part) {
             // entries that the stable sort places before the primary-index entry
@@ -1117,11 +1204,13 @@ __global__ void k_u32_to_u64(const uint32_t* __restrict__ a, uint64_t* __restric
 // --------------------------------------------------------------------- driver
 
 // Bits of the packed initial keys of the dictionary and parse sorts (MIBWT_INIT_KEY_BITS
-// overrides it for tuning runs).
-static int init_key_bits()
+// overrides both, MIBWT_PARSE_KEY_BITS the parse sort alone, for tuning runs).
+static int init_key_bits(bool parse = false)
 {
     int v = 64;
-    if (const char* e = getenv("MIBWT_INIT_KEY_BITS")) {
+    const char* e = getenv("MIBWT_INIT_KEY_BITS");
+    if (parse && getenv("MIBWT_PARSE_KEY_BITS")) e = getenv("MIBWT_PARSE_KEY_BITS");
+    if (e) {
         const int x = atoi(e);
         if (x >= 8 && x <= 64) v = x;
     }
@@ -1382,7 +1471,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     V0 = S.buf<uint32_t>(B_SORT_V0, d);
     V1 = S.buf<uint32_t>(B_SORT_V1, d);
     const int B0 = bits_for((uint64_t)du + 1);
-    const int pper = std::max(1, key_cap / B0);
+    const int pper = std::max(1, init_key_bits(true) / B0);
     k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, pper, K0, V0);
     MIBWT_LAUNCH_CHECK();
     uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);
@@ -1413,9 +1502,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     // counts; kEhPasses passes cover any id width the 32-bit entries can hold
     uint32_t* ehist = S.buf<uint32_t>(B_E_HIST, (size_t)kMaxPasses * kRadix);
     {
-        const uint32_t eb = (uint32_t)ceil_div((uint64_t)L, (uint64_t)kBlock * kEhPer);
+        const uint32_t eb = (uint32_t)std::min<uint64_t>(kEhBlocks, ceil_div((uint64_t)L, (uint64_t)kEhTile));
         uint32_t* epart = S.buf<uint32_t>(B_E_PART, (size_t)eb * kEhPasses * kRadix);
-        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(Hd, cid, ecnt, L, kEhPasses, epart);
+        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(gflag, cid, ecnt, L, kEhPasses, epart);
         MIBWT_LAUNCH_CHECK();
         k_entry_hist_sum<<<kEhPasses * kRadix, kBlock, 0, S.stream>>>(epart, eb, ehist);
         MIBWT_LAUNCH_CHECK();
@@ -1437,7 +1526,10 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_LAUNCH_CHECK();
     uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, gmix, L, mxf);
+    uint64_t* mixbits = S.buf<uint64_t>(B_G_MIXBITS, ceil_div((uint64_t)L, (uint64_t)64));
+    k_mix_bits<<<grid_for(L), kBlock, 0, S.stream>>>(gmix, L, mixbits);
+    MIBWT_LAUNCH_CHECK();
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, mixbits, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
@@ -1456,7 +1548,11 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, cid, du, prec);
         MIBWT_LAUNCH_CHECK();
-        k_q_count<<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, pid, prec, qc);
+        OccRec<EntT>* occ = reinterpret_cast<OccRec<EntT>*>(S.buf<uint4>(B_Q_OCC, d));
+        RankRec<EntT>* qrec = reinterpret_cast<RankRec<EntT>*>(S.buf<uint4>(B_Q_RREC, k32 ? (d + 1) / 2 : d));
+        k_occ_rec<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(pid, d, prec, E, T, n, W, occ);
+        MIBWT_LAUNCH_CHECK();
+        k_q_count<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, occ, qc, qrec);
         MIBWT_LAUNCH_CHECK();
         exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
         uint32_t hv[SC_COUNT_];
@@ -1479,8 +1575,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
             uint32_t* rb = S.buf<uint32_t>(B_Q_BLK, qblocks);
             k_q_blocks<<<grid_for(d), kBlock, 0, S.stream>>>(qo, qc, d, rb);
             MIBWT_LAUNCH_CHECK();
-            k_q_expand<EntT><<<qblocks, kBlock, 0, S.stream>>>(psa, d, pid, prec, mx, E, T, n, W, qo, rb, nq, ent0,
-                                                              special, part);
+            k_q_expand<EntT><<<qblocks, kBlock, 0, S.stream>>>(qrec, d, mx, qo, rb, nq, ent0, special, part);
             MIBWT_LAUNCH_CHECK();
         }
         k_q_last<EntT><<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
@@ -1504,9 +1599,11 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
                                                     eh);
         }
     };
-    if (gbits <= 24) assemble(uint32_t{});
+    // MIBWT_WIDE_ENTRIES=1 takes the 64-bit entries (with the in-sort primary index) at any size
+    const bool wide = gbits > 24 || (getenv("MIBWT_WIDE_ENTRIES") && atoi(getenv("MIBWT_WIDE_ENTRIES")) == 1);
+    if (!wide) assemble(uint32_t{});
     else assemble(uint64_t{});
-    const double esize = (gbits <= 24) ? 4.0 : 8.0;
+    const double esize = wide ? 8.0 : 4.0;
     S.phase_end(ph, 2.0 * n + 8.0 * L * 6 + esize * ne * 2 * ((gbits + 7) / 8) + 4.0 * d * 3);
     S.stats.alpha_groups = L;
     S.stats.mixed_entries = ne;

@@ -178,6 +178,24 @@ def test_colliding_fingerprints_reseed(mibwt, monkeypatch):
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
 
 
+@pytest.mark.parametrize("kind", ["random", "repetitive"])
+def test_wide_entries(mibwt, monkeypatch, kind):
+    # 64-bit BWT entries (taken when the entry groups exceed 24 bits) carry the primary index
+    # through the sort instead of counting it: forced on small texts, same exact result
+    rng = np.random.default_rng(29)
+    if kind == "random":
+        T = rng.integers(0, 256, size=300000).astype(np.uint8)
+    else:
+        base = rng.integers(0, 4, size=20000).astype(np.uint8)
+        T = np.frombuffer(b"ACGT", dtype=np.uint8)[np.tile(base, 12)].copy()
+        T[rng.integers(0, len(T), size=200)] = ord("T")
+    monkeypatch.setenv("MIBWT_WIDE_ENTRIES", "1")
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "pfp", 5, 11)
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+
+
 @pytest.mark.parametrize("name,W,P", [("dna1g", 5, 16), ("dna1g", 35, 81), ("markov256m", 5, 16),
                                       ("markov256m", 35, 81)])
 def test_large_workloads_roundtrip(mibwt, name, W, P):
