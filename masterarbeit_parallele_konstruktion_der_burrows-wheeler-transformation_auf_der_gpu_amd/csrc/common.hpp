@@ -251,6 +251,7 @@ enum BufId : int {
     B_SORT_HIST,
     B_PH_FLAG,      // dictionary head flags / ids
     B_PH_PID,       // phrase id per occurrence
+    B_PH_CHAR,      // character before each phrase
     B_DU_HEAD,
     B_DU_START,
     B_DU_LEN,

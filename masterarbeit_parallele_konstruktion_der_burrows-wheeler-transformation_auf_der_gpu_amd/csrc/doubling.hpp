@@ -72,26 +72,6 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
     NGz[i] = z;
 }
 
-// first compaction: the group record of a kept element from the initial groups (NR is
-// null unless the policy carries a value)
-static __global__ void k_dbl_compact_init(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
-                                          const uint32_t* __restrict__ V0, const uint32_t* __restrict__ gid1,
-                                          const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ NGz,
-                                          const uint32_t* __restrict__ NR, uint32_t m, uint32_t* __restrict__ V,
-                                          uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs,
-                                          uint32_t* __restrict__ Gz, uint32_t* __restrict__ R)
-{
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m || !keep[p]) return;
-    const uint32_t o = pos[p];
-    const uint32_t s = gofs[gid1[p] - 1];
-    V[o] = V0[p];
-    Hs[o] = pos[s];
-    Gs[o] = s;
-    Gz[o] = NGz[p];
-    if (NR) R[o] = NR[p];
-}
-
 // ---- one refinement round over the m active elements (groups contiguous)
 
 template <typename Policy>
@@ -342,64 +322,98 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
     NGz[p] = c;
 }
 
-// Compaction positions of the kept members: pos = exclusive scan of keep (chained tiles, the
-// total into *active); per tile the number of kept members whose sub-group is large next round
-// (lpart, summed by k_sum_tiles).
-static __global__ __launch_bounds__(kBlock) void k_dbl_keep_scan(const uint32_t* __restrict__ keep,
-                                                                const uint32_t* __restrict__ NGz, uint32_t m,
-                                                                uint32_t small_max, uint32_t* __restrict__ pos,
-                                                                uint32_t* __restrict__ active,
-                                                                uint32_t* __restrict__ lpart, Chain c, uint32_t tiles)
+// Compaction of the kept members in one pass over chained tiles of kChainTile slots: the
+// block takes kChainItems rounds of kBlock consecutive slots, records each wavefront's kept
+// mask, scans the (round, wavefront) counts, obtains the tile's offset by decoupled look-back
+// and moves every kept member to its place with coalesced reads.  A kept sub-group is kept
+// whole and stays contiguous, so its first slot moves to o - (p - first slot) for a member that
+// moves from p to o.  The first compaction reads the initial groups (kInit: NV = initial
+// order, NHs = 1-based group id, NGs = group offsets).  Also counts the kept members whose
+// sub-group is large next round (lpart per tile, summed by k_sum_tiles) and the total (*active).
+template <bool kInit>
+__global__ __launch_bounds__(kBlock) void k_dbl_compact(const uint32_t* __restrict__ keep, uint32_t m,
+                                                       uint32_t small_max, const uint32_t* __restrict__ NV,
+                                                       const uint32_t* __restrict__ NHs,
+                                                       const uint32_t* __restrict__ NGs,
+                                                       const uint32_t* __restrict__ NGz,
+                                                       const uint32_t* __restrict__ NR, uint32_t* __restrict__ V,
+                                                       uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs,
+                                                       uint32_t* __restrict__ Gz, uint32_t* __restrict__ R,
+                                                       uint32_t* __restrict__ active, uint32_t* __restrict__ lpart,
+                                                       Chain c, uint32_t tiles)
 {
-    __shared__ uint32_t slot;
+    constexpr int kRounds = kChainItems;
+    static_assert(kRounds * kWaves == kBlock, "one (round, wavefront) count per thread");
+    __shared__ uint64_t bal[kRounds * kWaves];
+    __shared__ uint32_t off[kRounds * kWaves];
     __shared__ uint32_t lds[kWaves];
+    __shared__ uint32_t slot;
     const uint32_t tile = chain_ticket(c, tiles, &slot);
-    const uint64_t base = (uint64_t)tile * kChainTile + (uint64_t)threadIdx.x * kChainItems;
-    uint32_t v[kChainItems];
-    uint32_t nl = 0;
-    if (base + kChainItems <= m) {
+    const uint64_t base = (uint64_t)tile * kChainTile;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    // all flag loads of a batch are issued before the first ballot
+    constexpr int kBatch = 16;
+    for (int it0 = 0; it0 < kRounds; it0 += kBatch) {
+        uint32_t kv[kBatch];
 #pragma unroll
-        for (int q = 0; q < kChainItems / 4; q++) {
-            const uint4 a = *reinterpret_cast<const uint4*>(keep + base + 4 * q);
-            const uint4 z = *reinterpret_cast<const uint4*>(NGz + base + 4 * q);
-            v[4 * q] = a.x;
-            v[4 * q + 1] = a.y;
-            v[4 * q + 2] = a.z;
-            v[4 * q + 3] = a.w;
-            nl += (a.x && z.x > small_max) + (a.y && z.y > small_max) + (a.z && z.z > small_max) +
-                  (a.w && z.w > small_max);
+        for (int j = 0; j < kBatch; j++) {
+            const uint64_t p = base + (uint64_t)(it0 + j) * kBlock + threadIdx.x;
+            kv[j] = (p < m) ? keep[p] : 0u;
         }
-    } else {
 #pragma unroll
-        for (int i = 0; i < kChainItems; i++) {
-            const bool in = base + i < m;
-            v[i] = in ? keep[base + i] : 0u;
-            nl += (v[i] && NGz[base + i] > small_max) ? 1u : 0u;
+        for (int j = 0; j < kBatch; j++) {
+            const uint64_t b = __ballot(kv[j] != 0);
+            if (lane == 0) {
+                bal[(it0 + j) * kWaves + w] = b;
+                off[(it0 + j) * kWaves + w] = (uint32_t)__popcll(b);
+            }
         }
     }
-    uint32_t acc = 0;
-#pragma unroll
-    for (int i = 0; i < kChainItems; i++) acc += v[i];
+    __syncthreads();
+    const uint32_t cnt = off[threadIdx.x];
     uint32_t tot;
-    uint32_t run = block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
+    const uint32_t ex = block_exclusive_scan(cnt, 0u, OpSum{}, lds, &tot);
+    off[threadIdx.x] = ex;  // each thread rewrites only its own word; chain_prefix synchronizes
     const uint32_t pre = chain_prefix(c, tile, tot, OpSum{}, 0u, &slot);
-    run += pre;
-    if (base + kChainItems <= m) {
+    const uint64_t lt = lanemask_lt();
+    uint32_t nl = 0;
+    // moves in batches: the loads of a batch are issued before its stores
+    constexpr int kMove = 8;
+    for (int it0 = 0; it0 < kRounds; it0 += kMove) {
+        bool kk[kMove];
+        uint32_t xv[kMove], zv[kMove], hv[kMove], gv[kMove], rv[kMove];
 #pragma unroll
-        for (int q = 0; q < kChainItems / 4; q++) {
-            uint32_t o[4];
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                o[j] = run;
-                run += v[4 * q + j];
+        for (int j = 0; j < kMove; j++) {
+            const uint64_t b = bal[(it0 + j) * kWaves + w];
+            kk[j] = (b >> lane) & 1u;
+            const uint32_t p = (uint32_t)(base + (uint64_t)(it0 + j) * kBlock + threadIdx.x);
+            xv[j] = zv[j] = hv[j] = gv[j] = rv[j] = 0;
+            if (kk[j]) {
+                xv[j] = NV[p];
+                zv[j] = NGz[p];
+                hv[j] = NHs[p];
+                if constexpr (!kInit) gv[j] = NGs[p];
+                if (NR) rv[j] = NR[p];
             }
-            *reinterpret_cast<uint4*>(pos + base + 4 * q) = make_uint4(o[0], o[1], o[2], o[3]);
         }
-    } else {
 #pragma unroll
-        for (int i = 0; i < kChainItems; i++) {
-            if (base + i < m) pos[base + i] = run;
-            run += v[i];
+        for (int j = 0; j < kMove; j++) {
+            if (!kk[j]) continue;
+            const uint64_t b = bal[(it0 + j) * kWaves + w];
+            const uint32_t p = (uint32_t)(base + (uint64_t)(it0 + j) * kBlock + threadIdx.x);
+            const uint32_t o = pre + off[(it0 + j) * kWaves + w] + (uint32_t)__popcll(b & lt);
+            V[o] = xv[j];
+            Gz[o] = zv[j];
+            nl += (zv[j] > small_max) ? 1u : 0u;
+            if constexpr (kInit) {
+                const uint32_t s = NGs[hv[j] - 1];
+                Hs[o] = o - (p - s);
+                Gs[o] = s;
+            } else {
+                Hs[o] = o - (p - hv[j]);
+                Gs[o] = gv[j];
+            }
+            if (NR) R[o] = rv[j];
         }
     }
     uint32_t ltot;
@@ -419,25 +433,6 @@ static __global__ __launch_bounds__(kBlock) void k_sum_tiles(const uint32_t* __r
     uint32_t tot;
     (void)block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
     if (threadIdx.x == 0) *out = tot;
-}
-
-// compaction of the kept members (whole sub-groups are kept or dropped, so a sub-group's
-// first slot is kept with it and maps through pos as well)
-static __global__ void k_dbl_compact(const uint32_t* __restrict__ keep, const uint32_t* __restrict__ pos,
-                                     const uint32_t* __restrict__ NV, const uint32_t* __restrict__ NHs,
-                                     const uint32_t* __restrict__ NGs, const uint32_t* __restrict__ NGz,
-                                     const uint32_t* __restrict__ NR, uint32_t m, uint32_t* __restrict__ V,
-                                     uint32_t* __restrict__ Hs, uint32_t* __restrict__ Gs, uint32_t* __restrict__ Gz,
-                                     uint32_t* __restrict__ R)
-{
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= m || !keep[p]) return;
-    const uint32_t o = pos[p];
-    V[o] = NV[p];
-    Hs[o] = pos[NHs[p]];
-    Gs[o] = NGs[p];
-    Gz[o] = NGz[p];
-    if (NR) R[o] = NR[p];
 }
 
 struct DoublingResult {
@@ -498,13 +493,19 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     bool first = true;
     uint32_t m = N;
     for (;;) {
-        // ---- compaction of the kept sub-groups; next round's counts
+        // ---- compaction of the kept sub-groups with next round's counts
         {
             const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, (uint32_t)kChainTile));
             uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
             const Chain c = S.chain(tiles);
-            k_dbl_keep_scan<<<tiles, kBlock, 0, S.stream>>>(keep, NGz, m, kSmallGroup, pos, sc + SC_ACTIVE, lpart, c,
-                                                            tiles);
+            if (first) {
+                k_dbl_compact<true><<<tiles, kBlock, 0, S.stream>>>(keep, m, kSmallGroup, V0, gid1, gofs, NGz, NR, V,
+                                                                    Hs, Gs, Gz, R, sc + SC_ACTIVE, lpart, c, tiles);
+                first = false;
+            } else {
+                k_dbl_compact<false><<<tiles, kBlock, 0, S.stream>>>(keep, m, kSmallGroup, NV, NHs, NGs, NGz, NR, V,
+                                                                     Hs, Gs, Gz, R, sc + SC_ACTIVE, lpart, c, tiles);
+            }
             MIBWT_LAUNCH_CHECK();
             k_sum_tiles<<<1, kBlock, 0, S.stream>>>(lpart, tiles, sc + SC_LARGE);
             MIBWT_LAUNCH_CHECK();
@@ -513,17 +514,8 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         S.read(hv, sc, SC_COUNT_);
         const uint32_t m2 = hv[SC_ACTIVE], nl = hv[SC_LARGE];
         res.rounds++;
+        res.bytes += (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4);
         if (m2 == 0) break;
-        if (first) {
-            k_dbl_compact_init<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, V0, gid1, gofs, NGz, NR, m, V, Hs,
-                                                                     Gs, Gz, R);
-            first = false;
-        } else {
-            k_dbl_compact<<<grid_for(m), kBlock, 0, S.stream>>>(keep, pos, NV, NHs, NGs, NGz, NR, m, V, Hs, Gs, Gz,
-                                                                R);
-        }
-        MIBWT_LAUNCH_CHECK();
-        res.bytes += (double)m * (4 * 12) + (double)m2 * 16;
         m = m2;
         // ---- order each group by the rank of the next position
         k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, R, m, rank, pol, key2);

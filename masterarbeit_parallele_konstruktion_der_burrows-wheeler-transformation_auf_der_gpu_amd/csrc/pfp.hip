@@ -453,9 +453,12 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, 
 
 // --------------------------------------------------------------------- 2/3. dictionary
 
+// Also records the character before each phrase (the last text character for phrase 0),
+// which the whole-phrase BWT entries need later: it lies in the previous phrase's bytes,
+// which a neighbouring lane reads at the same time.
 __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
                               const uint32_t* __restrict__ E, uint32_t d, uint64_t seed, uint64_t hmask,
-                              uint64_t* __restrict__ H, uint32_t* __restrict__ V)
+                              uint64_t* __restrict__ H, uint32_t* __restrict__ V, uint8_t* __restrict__ pc)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
@@ -463,6 +466,7 @@ __global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
     phrase_bounds(E, k, W, s, e);
     H[k] = phrase_hash(T, n, s, e, seed) & hmask;
     V[k] = k;
+    pc[k] = (s > 0) ? T[s - 1] : T[n - 1];
 }
 
 // Phrase fingerprints are first sorted by their low 48 bits (6 radix passes).  Neighbours
@@ -997,13 +1001,14 @@ struct alignas(16) OccRec {
 
 template <typename EntT>
 __global__ void k_occ_rec(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
-                          const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
-                          OccRec<EntT>* __restrict__ occ)
+                          const uint8_t* __restrict__ pchar, OccRec<EntT>* __restrict__ occ)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
     const PhraseRec q = pr[pid[k]];
-    occ[k] = OccRec<EntT>{whole_phrase_entry<EntT>(E, T, n, W, k, q.alpha0), q.mx_begin, q.mx_count};
+    EntT v = Ent<EntT>::make(q.alpha0, pchar[k]);
+    if (sizeof(EntT) == 8 && k == 0) v |= (EntT)0x100;  // the primary-index entry
+    occ[k] = OccRec<EntT>{v, q.mx_begin, q.mx_count};
 }
 
 // Per parse-suffix rank r (one gather of the occurrence k = psa[r]-1): the number of entries
@@ -1366,10 +1371,11 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint32_t *head = nullptr, *dstart = nullptr, *dlen = nullptr, *freq = nullptr;
     uint64_t* doff = nullptr;
     uint64_t L64 = 0;
+    uint8_t* pchar = S.buf<uint8_t>(B_PH_CHAR, d);
     for (int attempt = 0;; attempt++) {
         MIBWT_REQUIRE(attempt < 6, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
         const uint64_t hmask = (attempt == 0 && hash_bits < 64) ? ((1ull << hash_bits) - 1) : ~0ull;
-        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, hmask, K0, V0);
+        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, hmask, K0, V0, pchar);
         MIBWT_LAUNCH_CHECK();
         radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, key_bits);
         dict_passes += key_bits / 8;
@@ -1550,7 +1556,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         OccRec<EntT>* occ = reinterpret_cast<OccRec<EntT>*>(S.buf<uint4>(B_Q_OCC, d));
         RankRec<EntT>* qrec = reinterpret_cast<RankRec<EntT>*>(S.buf<uint4>(B_Q_RREC, k32 ? (d + 1) / 2 : d));
-        k_occ_rec<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(pid, d, prec, E, T, n, W, occ);
+        k_occ_rec<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(pid, d, prec, pchar, occ);
         MIBWT_LAUNCH_CHECK();
         k_q_count<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, occ, qc, qrec);
         MIBWT_LAUNCH_CHECK();

@@ -325,16 +325,15 @@ void inclusive_max(Storage& S, const T* in, T* out, uint64_t n)
 //
 // Stable LSD radix sort, 8-bit digits, one kernel per digit pass ("onesweep" shape):
 //   - one histogram kernel reads the keys once and counts the digits of every pass;
-//   - each pass kernel ranks its 4096-item tile with wavefront multi-split (8 ballots per
-//     item), publishes its per-digit counts, obtains the counts of all earlier tiles by
-//     decoupled look-back (one thread per digit), and scatters keys, then values, through
-//     LDS so that every digit run leaves the tile as one contiguous write.
+//   - each pass kernel ranks its tile (4096 items of 8-byte keys, 8192 of 4-byte keys) with
+//     wavefront multi-split (8 ballots per item), publishes its per-digit counts, obtains the
+//     counts of all earlier tiles by decoupled look-back (one thread per digit), and scatters
+//     keys, then values, through LDS so that every digit run leaves the tile as one
+//     contiguous write.
 
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
-constexpr int kSortItems = 16;                    // items per thread
-constexpr int kSortTile = kBlock * kSortItems;    // 4096 items per tile
-constexpr int kWaveTile = 64 * kSortItems;        // 1024 items per wavefront
+constexpr int kSortItems = 16;                    // items per thread (8-byte keys)
 constexpr int kMaxPasses = 8;
 constexpr int kHistItems = 16;                    // consecutive keys per thread in the histogram
 
@@ -445,42 +444,49 @@ __device__ __forceinline__ uint32_t chain_lookback_digit(const Chain& c, uint32_
     return excl;
 }
 
-template <typename K, typename V, typename Emit>
+// Items keys per thread: 16 for 8-byte keys, 32 for 4-byte keys (the same 32 KiB of staging).
+template <typename K>
+constexpr int sort_items() { return sizeof(K) <= 4 ? 2 * kSortItems : kSortItems; }
+
+template <typename K, typename V, typename Emit, int Items = sort_items<K>()>
 __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ keys, const V* __restrict__ vals,
                                                       uint64_t n, int shift, uint32_t mask,
                                                       const uint32_t* __restrict__ hist_pass, uint32_t tiles,
                                                       Chain c, Emit emit)
 {
     static_assert(sizeof(V) <= sizeof(K), "values are staged in the key buffer");
-    static_assert(kSortItems % 2 == 0, "slots are packed in pairs");
+    static_assert(Items % 2 == 0, "slots are packed in pairs");
+    constexpr int Tile = kBlock * Items, WaveTile = 64 * Items;
+    static_assert(Tile <= 65536, "16-bit slots");
     // per-wave digit counters; once the slots are known the same bytes hold the digit of
-    // every slot (4 KiB either way), which keeps the block at 38 KiB of LDS (4 blocks per CU)
-    __shared__ uint32_t wcnt_dig[kWaves * kRadix];
-    static_assert(kWaves * kRadix * 4 == kSortTile, "digit bytes alias the counters");
+    // every slot (4 KiB of counters, Tile bytes of digits), next to Tile keys of staging:
+    // 38 KiB of LDS for 8-byte keys (4 blocks per CU)
+    constexpr int kCntWords = kWaves * kRadix, kDigWords = Tile / 4;
+    __shared__ uint32_t wcnt_dig[kCntWords > kDigWords ? kCntWords : kDigWords];
     uint32_t(*wcnt)[kRadix] = reinterpret_cast<uint32_t(*)[kRadix]>(wcnt_dig);
     uint8_t* sdig = reinterpret_cast<uint8_t*>(wcnt_dig);
     __shared__ uint32_t dstart[kRadix];
     __shared__ uint32_t doff[kRadix];  // output position - tile slot, per digit (mod 2^32)
     __shared__ uint32_t scan_lds[kWaves];
     __shared__ uint32_t slot;
-    __shared__ K stage[kSortTile];
+    __shared__ K stage[Tile];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
     const uint32_t tile = chain_ticket(c, tiles, &slot);  // also orders the zeroing above
 
-    const uint64_t tile_base = (uint64_t)tile * kSortTile;
-    const uint64_t base = tile_base + (uint64_t)w * kWaveTile + lane;
-    K k[kSortItems];
-    uint32_t rk[kSortItems / 2];  // two 16-bit ranks (later: tile slots) per register
+    const uint64_t tile_base = (uint64_t)tile * Tile;
+    const uint64_t base = tile_base + (uint64_t)w * WaveTile + lane;
+    K k[Items];
+    uint32_t rk[Items / 2];  // two 16-bit ranks (later: tile slots) per register
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < Items; i++) {
         const uint64_t idx = base + (uint64_t)i * 64;
         const bool valid = idx < n;
         k[i] = valid ? keys[idx] : K(0);
     }
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < Items; i++) {
         const bool valid = base + (uint64_t)i * 64 < n;
         const uint32_t d = digit_of(k[i], shift, mask);
         // lanes with the same digit: per bit, the lanes that agree with this lane's bit.  The
@@ -531,7 +537,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     __syncthreads();
     // ranks become the items' slots in the tile
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < Items; i++) {
         const uint64_t idx = base + (uint64_t)i * 64;
         if (idx < n) {
             const uint32_t d = digit_of(k[i], shift, mask);
@@ -543,7 +549,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     __syncthreads();  // the counters are dead from here on: their bytes take the slot digits
     // keys go through LDS in slot order
 #pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
+    for (int i = 0; i < Items; i++) {
         if (base + (uint64_t)i * 64 < n) {
             const uint32_t p = (i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu);
             stage[p] = k[i];
@@ -551,20 +557,20 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
         }
     }
     __syncthreads();
-    const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)kSortTile ? (n - tile_base) : kSortTile);
+    const uint32_t valid_in_tile = (uint32_t)((n - tile_base) < (uint64_t)Tile ? (n - tile_base) : Tile);
     for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.key(p + doff[sdig[p]], stage[p]);
     if (Emit::kValues && vals) {
         // values are read only now, which keeps the ranking phase's register footprint small
-        V v[kSortItems];
+        V v[Items];
 #pragma unroll
-        for (int i = 0; i < kSortItems; i++) {
+        for (int i = 0; i < Items; i++) {
             const uint64_t idx = base + (uint64_t)i * 64;
             v[i] = (idx < n) ? vals[idx] : V(0);
         }
         __syncthreads();
         V* sv = reinterpret_cast<V*>(stage);
 #pragma unroll
-        for (int i = 0; i < kSortItems; i++)
+        for (int i = 0; i < Items; i++)
             if (base + (uint64_t)i * 64 < n) sv[(i & 1) ? (rk[i >> 1] >> 16) : (rk[i >> 1] & 0xFFFFu)] = v[i];
         __syncthreads();
         for (uint32_t p = threadIdx.x; p < valid_in_tile; p += kBlock) emit.val(p + doff[sdig[p]], sv[p]);
@@ -601,7 +607,7 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         }
     }
     MIBWT_REQUIRE(n < (1ull << 32), LIBCUBWT_BAD_PARAMETER, "radix_sort: too many items");
-    const uint32_t tiles = ceil_div(n, kSortTile);
+    const uint32_t tiles = ceil_div(n, (uint64_t)kBlock * sort_items<K>());
     const int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
     MIBWT_REQUIRE(passes <= kMaxPasses, LIBCUBWT_BAD_PARAMETER, "radix_sort: key too wide");
     const uint32_t* hist = hist_in;
