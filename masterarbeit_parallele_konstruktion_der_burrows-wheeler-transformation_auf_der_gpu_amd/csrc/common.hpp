@@ -275,7 +275,7 @@ enum BufId : int {
     B_G_MIX,
     B_G_MIXBITS,
     B_G_ECNT,
-    B_G_DELTA,
+    B_G_FILLC,
     B_MX_FLAG,
     B_MX_SCAN,
     B_MX_ENT,
@@ -340,7 +340,8 @@ enum BufId : int {
     B_LY_GF,
     B_AS_REC,
     B_AS_R,
-    B_AS_HD,
+    B_AS_WBITS,
+    B_AS_WSCAN,
     B_AS_PREC,
     B_CHAIN_STATUS,
     B_CHAIN_TICKET,

@@ -17,8 +17,9 @@
 //     strings stay in one group); h doubles every round; for the dictionary r is the length
 //     of the rest of the phrase;
 //   - chains (Lyndon): next(x) = jump pointer, always open; pointers are squared per round.
-// SA receives an element's slot once, in the round in which its group stops (kept elements
-// are placed again later).
+// An element is placed once, in the round in which its group stops (kept elements are placed
+// again later): pol.place(SA, slot, x, group head) writes SA[slot] = x, or whatever the
+// caller wants in sorted order instead.
 #pragma once
 
 #include "prims.hpp"
@@ -26,6 +27,14 @@
 namespace mibwt {
 
 constexpr uint32_t kNoNext = 0xFFFFFFFFu;
+
+// default placement: the suffix array
+struct PlaceSA {
+    __device__ __forceinline__ void operator()(uint32_t* SA, uint32_t pos, uint32_t x, uint32_t) const
+    {
+        SA[pos] = x;
+    }
+};
 constexpr uint32_t kSmallGroup = 256;
 
 // ---- initial groups: runs of equal keys in the initially sorted order
@@ -67,7 +76,7 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
     }
     rank[x] = s + 1;
     const bool kp = z > 1 && pol.open(x, r);
-    if (!kp) SA[i] = x;
+    if (!kp) pol.place(SA, i, x, s);
     keep[i] = kp ? 1u : 0u;
     NGz[i] = z;
 }
@@ -251,7 +260,7 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
         }
         rank[x] = gs + lt + 1;
         const bool kp = eq > 1 && pol.open(x, r);
-        if (!kp) SA[gs + (p - a)] = x;
+        if (!kp) pol.place(SA, gs + (p - a), x, gs + lt);
         keep[g] = kp ? 1u : 0u;
         NV[g] = x;
         NHs[g] = b0 + ss;
@@ -314,7 +323,7 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
     }
     rank[x] = gs + lt + 1;
     const bool kp = c > 1 && pol.open(x, r);
-    if (!kp) SA[gs + (q - q0)] = x;
+    if (!kp) pol.place(SA, gs + (q - q0), x, gs + lt);
     keep[p] = kp ? 1u : 0u;
     NV[p] = x;
     NHs[p] = s + lt;

@@ -512,6 +512,7 @@ struct ChainPolicy {
         return y < n ? y : kNoNext;
     }
     __device__ __forceinline__ bool open(uint32_t, uint32_t) const { return true; }
+    __device__ __forceinline__ void place(uint32_t* SA, uint32_t pos, uint32_t x, uint32_t) const { SA[pos] = x; }
 };
 
 __global__ void k_jump_init(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ a,

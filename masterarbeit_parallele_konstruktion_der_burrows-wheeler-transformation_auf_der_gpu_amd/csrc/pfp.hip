@@ -614,11 +614,16 @@ __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint
 // than h (they are then equal).
 // With Lim::kCarry the element carries r = lim(x) - x (the rest of its string) and no round
 // reads lim.
-template <typename Lim>
+template <typename Lim, typename Place = PlaceSA>
 struct StringPolicy {
     static constexpr bool kCarry = Lim::kCarry;
     Lim lim;
     uint32_t h;
+    Place place_fn;
+    __device__ __forceinline__ void place(uint32_t* SA, uint32_t pos, uint32_t x, uint32_t head) const
+    {
+        place_fn(SA, pos, x, head);
+    }
     __device__ __forceinline__ uint32_t rem(uint32_t x) const { return lim(x) - x; }
     __device__ __forceinline__ uint32_t next(uint32_t x, uint32_t r) const
     {
@@ -635,26 +640,39 @@ struct StringPolicy {
 // Prefix doubling over N suffix positions: K0/V0 hold the initial keys (bits [0, key_bits),
 // the first h0 symbols with a terminator padding) and positions; on exit rank[x] = 1 +
 // position of x's group head and SA = the sorted order (equal strings share a group).
-template <typename Lim>
+template <typename Lim, typename Place = PlaceSA>
 DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t* V0, uint32_t* V1, uint32_t N,
-                               int key_bits, uint32_t h0, Lim lim, uint32_t* rank, uint32_t* SA)
+                               int key_bits, uint32_t h0, Lim lim, uint32_t* rank, uint32_t* SA,
+                               Place place = Place{})
 {
     radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, N, 0, key_bits);
-    StringPolicy<Lim> pol{lim, h0};
+    StringPolicy<Lim, Place> pol{lim, h0, place};
     DoublingResult r = refine_groups(
-        S, K0, V0, V1, N, pol, [](StringPolicy<Lim>& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA);
+        S, K0, V0, V1, N, pol, [](StringPolicy<Lim, Place>& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank,
+        SA);
     r.bytes += (double)N * 24 * ((key_bits + 7) / 8);
     return r;
 }
 
 // --------------------------------------------------------------------- 5. phrase order
 
+// Lexicographic rank of phrase u: the number of whole phrases placed before its group head h
+// (whole phrases are distinct strings, so no other whole phrase shares its group): word
+// prefix counts of the placement bitmap plus the bits of h's word below h.
+__global__ void k_popc_words(const uint32_t* __restrict__ bits, uint32_t words, uint32_t* __restrict__ cnt)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < words) cnt[i] = (uint32_t)__popc(bits[i]);
+}
+
 __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __restrict__ off,
-                          const uint32_t* __restrict__ Fs, uint32_t du, uint32_t* __restrict__ lexrank)
+                          const uint32_t* __restrict__ bits, const uint32_t* __restrict__ wscan, uint32_t du,
+                          uint32_t* __restrict__ lexrank)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= du) return;
-    lexrank[u] = Fs[rank[(uint32_t)off[u]] - 1];
+    const uint32_t h = rank[(uint32_t)off[u]] - 1;
+    lexrank[u] = wscan[h >> 5] + (uint32_t)__popc(bits[h >> 5] & ((1u << (h & 31)) - 1u));
 }
 
 // --------------------------------------------------------------------- 6. BWT assembly
@@ -666,17 +684,10 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 //   bit  41      whole phrase (j == 0).
 constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
-// record and group head of dictionary position x: one 16-byte load gathers both
-struct AsmRec {
-    uint64_t r;     // packed record above
-    uint32_t head;  // sorted position of the group head (rank - 1)
-    uint32_t pad;
-};
-
 __global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
-                         const uint32_t* __restrict__ rank, uint32_t L, int W, AsmRec* __restrict__ rec)
+                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq, uint32_t L, int W,
+                         uint64_t* __restrict__ rec)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L) return;
@@ -687,51 +698,66 @@ __global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restri
     if (rem > (uint32_t)W) r |= kRecAlpha;
     if (j == 0) r |= kRecWhole;
     else if (rem > (uint32_t)W) r |= (uint64_t)T[start[u] + j - 1] << 32;
-    rec[x] = AsmRec{r, rank[x] - 1, 0};
+    rec[x] = r;
 }
 
-// Sorted order: gather each position's record and its group head once (also gives the
-// whole-phrase flags in sorted order, i.e. the lexicographic phrase order).
-__global__ void k_asm_gather(const uint32_t* __restrict__ SA, const AsmRec* __restrict__ rec, uint32_t L,
-                             uint64_t* __restrict__ R, uint32_t* __restrict__ Hd, uint32_t* __restrict__ whole)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= L) return;
-    const AsmRec a = rec[SA[g]];
-    R[g] = a.r;
-    Hd[g] = a.head;
-    whole[g] = (a.r & kRecWhole) ? 1u : 0u;
-}
+// Final placement of a dictionary position by the suffix sort: its record, group head and
+// whole-phrase flag go straight to its sorted slot (no suffix array to gather through later);
+// the whole-phrase flags in sorted order give the lexicographic phrase order.
+struct alignas(16) SortedRec {
+    uint64_t r;     // record of the dictionary position placed here
+    uint32_t head;  // sorted position of its group head
+    uint32_t pad;
+};
+
+struct PlaceRec {
+    const uint64_t* rec;
+    SortedRec* out;
+    uint32_t* wbits;  // bit per sorted position: a whole phrase is placed there
+    __device__ __forceinline__ void operator()(uint32_t*, uint32_t pos, uint32_t x, uint32_t head) const
+    {
+        const uint64_t r = rec[x];
+        out[pos] = SortedRec{r, head, 0};
+        if (r & kRecWhole) atomicOr(wbits + (pos >> 5), 1u << (pos & 31));
+    }
+};
 
 // a group is mixed when a member's preceding character differs from its head's
-__global__ void k_asm_mix(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd, uint32_t L,
-                          uint8_t* __restrict__ gmix)
+__global__ void k_asm_mix(const SortedRec* __restrict__ RH, uint32_t L, uint8_t* __restrict__ gmix)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
-    const uint64_t r = R[g];
-    const uint32_t h = Hd[g];
+    const SortedRec a = RH[g];
+    const uint64_t r = a.r;
+    const uint32_t h = a.head;
     if (!(r & kRecAlpha) || (r & kRecWhole) || h == g) return;
-    if (((r >> 32) & 0xFF) != ((R[h] >> 32) & 0xFF)) gmix[h] = 1;
+    if (((r >> 32) & 0xFF) != ((RH[h].r >> 32) & 0xFF)) gmix[h] = 1;
 }
 
-// slots per sorted position (cnt) and slots written through ordered entries (ecnt)
-__global__ void k_asm_counts(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd,
-                             const uint8_t* __restrict__ gmix, uint32_t L, uint32_t* __restrict__ cnt,
-                             uint32_t* __restrict__ ecnt)
+// slots per sorted position (cnt), slots written through ordered entries (ecnt) and, for the
+// positions whose slots are filled directly (single-character groups), 0x100 | character (fc).
+// The heads of groups that receive entries are flagged (gflag, zeroed by the caller; every
+// writer stores 1): numbered densely they key the entry sort with few bits.
+__global__ void k_asm_counts(const SortedRec* __restrict__ RH, const uint8_t* __restrict__ gmix, uint32_t L,
+                             uint32_t* __restrict__ cnt, uint32_t* __restrict__ ecnt, uint16_t* __restrict__ fc,
+                             uint32_t* __restrict__ gflag)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
-    const uint64_t r = R[g];
+    const SortedRec a = RH[g];
+    const uint64_t r = a.r;
     const uint32_t f = (uint32_t)r;
     const bool alpha = (r & kRecAlpha) != 0;
+    const bool ordered = alpha && ((r & kRecWhole) || gmix[a.head]);
     cnt[g] = alpha ? f : 0u;
-    ecnt[g] = (alpha && ((r & kRecWhole) || gmix[Hd[g]])) ? f : 0u;
+    ecnt[g] = ordered ? f : 0u;
+    fc[g] = (alpha && !ordered) ? (uint16_t)(0x100u | ((uint32_t)(r >> 32) & 0xFFu)) : (uint16_t)0;
+    if (ordered && f) gflag[a.head] = 1;
 }
 
 // Digit histograms of the entry sort, weighted from the groups instead of read from the
 // entries: every sorted position g contributes ecnt[g] entries keyed by the dense id of its
-// group head, cid[Hd[g]], which is cid[g] + gflag[g] - 1 (groups are contiguous and only
+// group head, cid[head(g)], which is cid[g] + gflag[g] - 1 (groups are contiguous and only
 // heads carry flags), nondecreasing in g.  A block stages tiles of kEhTile positions in LDS
 // with coalesced loads, then every thread walks kEhPer consecutive positions and adds runs of
 // equal digits at once; blocks stride over the tiles and write their counts to
@@ -808,41 +834,45 @@ __global__ __launch_bounds__(kBlock) void k_entry_hist_sum(const uint32_t* __res
     if (threadIdx.x == 0) hist[i] = tot;
 }
 
-// Direct fill of the single-character groups, one wavefront per 64 consecutive sorted
-// positions: their slot ranges are consecutive in the output, so the lanes sweep the wave's
+// Direct fill of the single-character groups, one wavefront per 64 * kFillPer consecutive
+// sorted positions: their slot ranges are consecutive in the output, so the lanes sweep the wave's
 // span together in 16-byte granules and look the runs up in LDS (one bisection per granule,
 // then a forward walk).  A granule whose bytes are all fill bytes of this wave is one 16-byte
 // store; otherwise its fill bytes are stored one by one (neighbouring waves and the later
-// entry pass own the other bytes).  Also writes delta = slot offset minus entry offset for
-// every position.
-__global__ __launch_bounds__(kBlock) void k_asm_fill(const uint64_t* __restrict__ R, const uint32_t* __restrict__ Hd,
-                                                    const uint8_t* __restrict__ gmix, const uint32_t* __restrict__ goff,
-                                                    const uint32_t* __restrict__ escan, uint32_t L, uint64_t n,
-                                                    uint8_t* __restrict__ O, uint32_t* __restrict__ delta)
+// entry pass own the other bytes).  A run is [goff[g], goff[g + 1]) for the positions that
+// fc marks.
+constexpr int kFillPer = 4;  // sorted positions per lane: a wavefront covers 256 runs
+__global__ __launch_bounds__(kBlock) void k_asm_fill(const uint32_t* __restrict__ goff,
+                                                    const uint16_t* __restrict__ fc, uint32_t L, uint64_t n,
+                                                    uint8_t* __restrict__ O)
 {
-    __shared__ uint32_t rs[kBlock], rl[kBlock];
-    __shared__ uint8_t rc[kBlock];
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const int lane = threadIdx.x & 63, wb = threadIdx.x & ~63;
+    constexpr int kWavePos = 64 * kFillPer;
+    __shared__ uint32_t rs[kBlock * kFillPer], rl[kBlock * kFillPer];
+    __shared__ uint8_t rc[kBlock * kFillPer];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint32_t wb = (uint32_t)w * kWavePos;  // this wave's runs in LDS
+    const uint64_t g0 = (uint64_t)blockIdx.x * kBlock * kFillPer + wb;
     const bool aligned = ((uintptr_t)O & 15u) == 0;
-    uint32_t st = 0, ln = 0;
-    uint8_t ch = 0;
-    if (g < L) {
-        const uint64_t r = R[g];
-        st = goff[g];
-        delta[g] = st - escan[g];
-        if ((r & kRecAlpha) && !(r & kRecWhole) && !gmix[Hd[g]]) {
-            ln = (uint32_t)r;
-            ch = (uint8_t)(r >> 32);
+#pragma unroll
+    for (int i = 0; i < kFillPer; i++) {
+        const uint64_t g = g0 + (uint64_t)i * 64 + lane;
+        uint32_t st = 0, ln = 0;
+        uint8_t ch = 0;
+        if (g < L) {
+            st = goff[g];
+            const uint32_t f = fc[g];
+            if (f) {
+                ln = ((g + 1 < L) ? goff[g + 1] : (uint32_t)n) - st;
+                ch = (uint8_t)f;
+            }
         }
+        rs[wb + i * 64 + lane] = st;
+        rl[wb + i * 64 + lane] = ln;
+        rc[wb + i * 64 + lane] = ch;
     }
-    rs[threadIdx.x] = st;
-    rl[threadIdx.x] = ln;
-    rc[threadIdx.x] = ch;
     __syncthreads();
     // span of this wave: from its first run start to the end of its last run
-    const uint32_t g0 = blockIdx.x * blockDim.x + wb;
-    const uint32_t nv = (g0 >= L) ? 0u : ((L - g0) < 64u ? (L - g0) : 64u);
+    const uint32_t nv = (g0 >= L) ? 0u : (uint32_t)((L - g0) < (uint64_t)kWavePos ? (L - g0) : kWavePos);
     if (nv == 0) return;
     const uint32_t lo = rs[wb];
     const uint32_t hi_run = rs[wb + nv - 1] + rl[wb + nv - 1];
@@ -906,9 +936,10 @@ __global__ void k_mix_bits(const uint8_t* __restrict__ gmix, uint32_t L, uint64_
     if ((threadIdx.x & 63) == 0 && g < L) bits[g >> 6] = w;
 }
 
-// mixed-group members in dictionary order (grouped by phrase)
-__global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint64_t* __restrict__ mixbits, uint32_t L,
-                          uint32_t* __restrict__ mxf)
+// mixed-group members in dictionary order (grouped by phrase); the group head of x is
+// rank[x] - 1
+__global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+                          const uint64_t* __restrict__ mixbits, uint32_t L, uint32_t* __restrict__ mxf)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x > L) return;
@@ -916,9 +947,10 @@ __global__ void k_mx_flag(const AsmRec* __restrict__ rec, const uint64_t* __rest
         mxf[x] = 0;
         return;
     }
-    const AsmRec a = rec[x];
-    const bool cand = (a.r & kRecAlpha) && !(a.r & kRecWhole);
-    mxf[x] = (cand && ((mixbits[a.head >> 6] >> (a.head & 63)) & 1u)) ? 1u : 0u;
+    const uint64_t r = rec[x];
+    const bool cand = (r & kRecAlpha) && !(r & kRecWhole);
+    const uint32_t head = rank[x] - 1;
+    mxf[x] = (cand && ((mixbits[head >> 6] >> (head & 63)) & 1u)) ? 1u : 0u;
 }
 
 // BWT entries: group (sorted position of the dictionary group head) above the output byte.
@@ -935,32 +967,22 @@ struct Ent {
 };
 
 template <typename EntT>
-__global__ void k_mx_write(const AsmRec* __restrict__ rec, const uint32_t* __restrict__ mxf,
-                           const uint32_t* __restrict__ mxs, const uint32_t* __restrict__ cid, uint32_t L,
-                           EntT* __restrict__ mx)
+__global__ void k_mx_write(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+                           const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
+                           const uint32_t* __restrict__ cid, uint32_t L, EntT* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L || !mxf[x]) return;
-    const AsmRec a = rec[x];
-    mx[mxs[x]] = Ent<EntT>::make(cid[a.head], (uint32_t)(a.r >> 32) & 0xFFu);
-}
-
-// Entry groups are numbered densely: the heads of the groups that receive entries (mixed
-// groups and whole phrases) get consecutive ids in sorted order, so the entry keys need only
-// bits_for(number of such groups) bits.
-__global__ void k_ent_gflag(const uint32_t* __restrict__ Hd, const uint32_t* __restrict__ ecnt, uint32_t L,
-                            uint32_t* __restrict__ gflag)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g < L && ecnt[g]) gflag[Hd[g]] = 1;  // every writer stores 1
+    mx[mxs[x]] = Ent<EntT>::make(cid[rank[x] - 1], (uint32_t)(rec[x] >> 32) & 0xFFu);
 }
 
 // slot offset minus entry offset of each entry group, by its dense id
 __global__ void k_ent_delta(const uint32_t* __restrict__ gflag, const uint32_t* __restrict__ cid,
-                            const uint32_t* __restrict__ delta, uint32_t L, uint32_t* __restrict__ deltac)
+                            const uint32_t* __restrict__ goff, const uint32_t* __restrict__ escan, uint32_t L,
+                            uint32_t* __restrict__ deltac)
 {
     const uint32_t h = blockIdx.x * blockDim.x + threadIdx.x;
-    if (h < L && gflag[h]) deltac[cid[h]] = delta[h];
+    if (h < L && gflag[h]) deltac[cid[h]] = goff[h] - escan[h];
 }
 
 // per distinct phrase: first mixed entry, number of mixed entries, whole-phrase group
@@ -969,14 +991,14 @@ struct PhraseRec {
 };
 
 __global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                             const uint32_t* __restrict__ mxs, const AsmRec* __restrict__ rec,
+                             const uint32_t* __restrict__ mxs, const uint32_t* __restrict__ rank,
                              const uint32_t* __restrict__ cid, uint32_t du, PhraseRec* __restrict__ pr)
 {
     const uint32_t u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= du) return;
     const uint32_t o = (uint32_t)off[u];
     const uint32_t a = mxs[o], b = mxs[o + len[u]];
-    pr[u] = PhraseRec{a, b - a, cid[rec[o].head], 0};
+    pr[u] = PhraseRec{a, b - a, cid[rank[o] - 1], 0};
 }
 
 template <typename EntT>
@@ -1432,7 +1454,6 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     ph = S.phase_begin("dsort");
     uint32_t* owner = S.buf<uint32_t>(B_DT_OWNER, L);
     uint32_t* drank = S.buf<uint32_t>(B_DT_RANK, L);
-    uint32_t* dsa = S.buf<uint32_t>(B_DT_SA, L);
     k_dt_owner<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, du, owner);
     MIBWT_LAUNCH_CHECK();
     K0 = S.buf<uint64_t>(B_SORT_K0, L);
@@ -1450,27 +1471,29 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, codes, abits, aper,
                                                         L, K0, V0, dlim);
     MIBWT_LAUNCH_CHECK();
+    // per dictionary position a record (frequency, preceding character, flags); the sort
+    // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
+    uint64_t* rec = S.buf<uint64_t>(B_AS_REC, L);
+    SortedRec* RH = reinterpret_cast<SortedRec*>(S.buf<uint4>(B_AS_R, L));
+    const uint32_t wwords = (uint32_t)ceil_div((uint64_t)L, (uint64_t)32);
+    uint32_t* wbits = S.buf<uint32_t>(B_AS_WBITS, wwords);
+    fill<uint32_t>(S, wbits, wwords, 0u);
+    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, L, W, rec);
+    MIBWT_LAUNCH_CHECK();
     LimDict limd{dlim};
-    DoublingResult rd = prefix_doubling<LimDict>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper, limd,
-                                                 drank, dsa);
+    DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper,
+                                                           limd, drank, nullptr, PlaceRec{rec, RH, wbits});
     S.phase_end(ph, rd.bytes);
     S.stats.doubling_rounds_dict = rd.rounds;
 
     // ---- 5. phrase order and parse suffix array
     ph = S.phase_begin("psort");
-    // per dictionary position a 16-byte record (frequency, preceding character, flags, group
-    // head), gathered once into sorted order; its whole-phrase flags give the phrase order
-    AsmRec* rec = S.buf<AsmRec>(B_AS_REC, L);
-    uint64_t* R = S.buf<uint64_t>(B_AS_R, L);
-    uint32_t* Hd = S.buf<uint32_t>(B_AS_HD, L);
-    uint32_t* F = S.buf<uint32_t>(B_G_ECNT, L);
-    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, drank, L, W, rec);
+    uint32_t* wscan = S.buf<uint32_t>(B_AS_WSCAN, wwords);
+    k_popc_words<<<grid_for(wwords), kBlock, 0, S.stream>>>(wbits, wwords, wscan);
     MIBWT_LAUNCH_CHECK();
-    k_asm_gather<<<grid_for(L), kBlock, 0, S.stream>>>(dsa, rec, L, R, Hd, F);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, F, F, L);
+    exclusive_sum<uint32_t>(S, wscan, wscan, wwords);
     uint32_t* lexrank = S.buf<uint32_t>(B_LEXRANK, du);
-    k_lexrank<<<grid_for(du), kBlock, 0, S.stream>>>(drank, doff, F, du, lexrank);
+    k_lexrank<<<grid_for(du), kBlock, 0, S.stream>>>(drank, doff, wbits, wscan, du, lexrank);
     MIBWT_LAUNCH_CHECK();
     K0 = S.buf<uint64_t>(B_SORT_K0, d);
     K1 = S.buf<uint64_t>(B_SORT_K1, d);
@@ -1491,18 +1514,17 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     ph = S.phase_begin("assemble");
     uint8_t* gmix = S.buf<uint8_t>(B_G_MIX, L);
     fill<uint8_t>(S, gmix, L, (uint8_t)0);
-    k_asm_mix<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, L, gmix);
+    k_asm_mix<<<grid_for(L), kBlock, 0, S.stream>>>(RH, L, gmix);
     MIBWT_LAUNCH_CHECK();
     uint32_t* goff = S.buf<uint32_t>(B_G_OFF, L);
     uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
-    k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, L, goff, ecnt);
-    MIBWT_LAUNCH_CHECK();
-    // dense ids of the entry groups
+    uint16_t* fc = S.buf<uint16_t>(B_G_FILLC, L);
     uint32_t* gflag = S.buf<uint32_t>(B_E_GFLAG, L);
     uint32_t* cid = S.buf<uint32_t>(B_E_CID, L);
     fill<uint32_t>(S, gflag, L, 0u);
-    k_ent_gflag<<<grid_for(L), kBlock, 0, S.stream>>>(Hd, ecnt, L, gflag);
+    k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(RH, gmix, L, goff, ecnt, fc, gflag);
     MIBWT_LAUNCH_CHECK();
+    // dense ids of the entry groups
     exclusive_sum<uint32_t>(S, gflag, cid, L, sc + SC_NGROUPS);
     // digit histograms of the entry sort (bits of the dense group id), from the per-position
     // counts; kEhPasses passes cover any id width the 32-bit entries can hold
@@ -1524,18 +1546,17 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_REQUIRE(hv0[SC_TOTAL] == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
     const uint32_t ngroups = hv0[SC_NGROUPS];
     const int gbits = std::max(1, bits_for(ngroups > 0 ? ngroups - 1 : 0));
-    uint32_t* delta = S.buf<uint32_t>(B_G_DELTA, L);
-    k_asm_fill<<<grid_for(L), kBlock, 0, S.stream>>>(R, Hd, gmix, goff, ecnt, L, n, dO, delta);
+    k_asm_fill<<<ceil_div((uint64_t)L, (uint64_t)kBlock * kFillPer), kBlock, 0, S.stream>>>(goff, fc, L, n, dO);
     MIBWT_LAUNCH_CHECK();
     uint32_t* deltac = S.buf<uint32_t>(B_E_DELTAC, std::max<uint32_t>(1, ngroups));
-    k_ent_delta<<<grid_for(L), kBlock, 0, S.stream>>>(gflag, cid, delta, L, deltac);
+    k_ent_delta<<<grid_for(L), kBlock, 0, S.stream>>>(gflag, cid, goff, ecnt, L, deltac);
     MIBWT_LAUNCH_CHECK();
     uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
     uint64_t* mixbits = S.buf<uint64_t>(B_G_MIXBITS, ceil_div((uint64_t)L, (uint64_t)64));
     k_mix_bits<<<grid_for(L), kBlock, 0, S.stream>>>(gmix, L, mixbits);
     MIBWT_LAUNCH_CHECK();
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, mixbits, L, mxf);
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, drank, mixbits, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
@@ -1550,9 +1571,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         using EntT = decltype(tag);
         constexpr bool k32 = sizeof(EntT) == 4;
         EntT* mx = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_MX_ENT, nmx + 1));
-        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, mxf, mxs, cid, L, mx);
+        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, drank, mxf, mxs, cid, L, mx);
         MIBWT_LAUNCH_CHECK();
-        k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, rec, cid, du, prec);
+        k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, drank, cid, du, prec);
         MIBWT_LAUNCH_CHECK();
         OccRec<EntT>* occ = reinterpret_cast<OccRec<EntT>*>(S.buf<uint4>(B_Q_OCC, d));
         RankRec<EntT>* qrec = reinterpret_cast<RankRec<EntT>*>(S.buf<uint4>(B_Q_RREC, k32 ? (d + 1) / 2 : d));
