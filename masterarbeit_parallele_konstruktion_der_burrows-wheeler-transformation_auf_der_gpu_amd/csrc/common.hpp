@@ -347,6 +347,8 @@ enum BufId : int {
     B_CHAIN_TICKET,
     B_SEL_SKETCH,
     B_SEL_PART,
+    B_SEL_EPART,
+    B_SEL_ROWS,
     B_SEL_HIST2,
     B_SEL_COUNTS,
     B_SEL_X,

@@ -329,6 +329,34 @@ __device__ double block_sum_f64(double v, double* lds)
     return s;
 }
 
+// Pair counts, spread over kPairBlocks blocks of kSampleBlock / 256 rows each: per block the
+// sums of v and v * log2(v) over its counts (then H = (V log2 T - sum v log2 v) / T for T
+// pairs), and the row sums (first-byte counts), each summed in a fixed order.
+constexpr int kPairBlocks = 65536 / kSampleBlock;
+__global__ __launch_bounds__(kSampleBlock) void k_feat_pairent(const uint32_t* __restrict__ hist2,
+                                                              double* __restrict__ epart,
+                                                              uint32_t* __restrict__ rows)
+{
+    constexpr int kRows = kSampleBlock / 256, kWavesPerRow = 256 / 64;
+    __shared__ double red[kSampleBlock / 64];
+    __shared__ uint32_t ws[kSampleBlock / 64];
+    const uint32_t v = hist2[blockIdx.x * kSampleBlock + threadIdx.x];
+    const uint32_t rv = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) ws[threadIdx.x >> 6] = rv;
+    const double dv = (double)v;
+    const double a = block_sum_f64(dv, red);
+    const double b = block_sum_f64(v ? dv * log2(dv) : 0.0, red);
+    if (threadIdx.x == 0) {
+        epart[2 * blockIdx.x] = a;
+        epart[2 * blockIdx.x + 1] = b;
+    }
+    if (threadIdx.x < kRows) {
+        uint32_t r = 0;
+        for (int w = 0; w < kWavesPerRow; w++) r += ws[threadIdx.x * kWavesPerRow + w];
+        rows[blockIdx.x * kRows + threadIdx.x] = r;
+    }
+}
+
 // Norm of the features: x_k -> (x_k - mean_k) / std_k; null for raw features.
 struct FeatNorm {
     const float* mean;
@@ -340,7 +368,9 @@ struct FeatNorm {
 __global__ __launch_bounds__(kSampleBlock) void k_feat_finish(const uint32_t* __restrict__ spart, uint32_t chunks,
                                                              const uint32_t* __restrict__ hist2,
                                                              const uint32_t* __restrict__ kpart, uint32_t kblocks,
-                                                             uint64_t n, FeatNorm norm, float* __restrict__ X,
+                                                             const double* __restrict__ epart,
+                                                             const uint32_t* __restrict__ rows, uint64_t n,
+                                                             FeatNorm norm, float* __restrict__ X,
                                                              float* __restrict__ Xn)
 {
     __shared__ uint32_t tot[kSpWords];
@@ -348,20 +378,16 @@ __global__ __launch_bounds__(kSampleBlock) void k_feat_finish(const uint32_t* __
     __shared__ uint32_t sk[3];
     __shared__ double red[kSampleBlock / 64];
     const int t = threadIdx.x;
-    if (t < kSpWords) {
-        uint32_t s = 0;
-        for (uint32_t b = 0; b < chunks; b++) s += spart[(uint64_t)b * kSpWords + t];
-        tot[t] = s;
-    }
-    if (t < 256) {
-        uint32_t s = 0;
-        for (int b = 0; b < 256; b++) s += hist2[t * 256 + b];
-        h1[t] = s;
-    }
+    if (t < kSpWords) tot[t] = 0;
+    if (t < 256) h1[t] = rows[t];
     if (t < 3) sk[t] = 0;
+    __syncthreads();
+    // chunk partials: all threads load at once (integer sums, any order)
+    for (uint32_t i = t; i < chunks * (uint32_t)kSpWords; i += kSampleBlock) atomicAdd(&tot[i % kSpWords], spart[i]);
     __syncthreads();
     if (t == 0 && tot[kSpLast]) h1[(tot[kSpLast] - 1) & 0xFFu]++;  // position n-1 has no pair
     uint32_t a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll 8
     for (uint32_t b = t; b < kblocks; b += kSampleBlock) {
         a0 += kpart[(uint64_t)b * 3];
         a1 += kpart[(uint64_t)b * 3 + 1];
@@ -379,22 +405,21 @@ __global__ __launch_bounds__(kSampleBlock) void k_feat_finish(const uint32_t* __
     // entropies (bits): H0 of the bytes, Hpair of the byte pairs
     const double tot1 = (double)tot[kSpPos];
     const double tot2 = tot1 - (tot[kSpLast] ? 1.0 : 0.0);
-    double e0 = 0, e2 = 0, s1 = 0;
+    double e0 = 0, s1 = 0;
     if (t < 256 && h1[t]) {
         const double p = h1[t] / tot1;
         e0 = -p * log2(p);
         s1 = 1;
     }
-    for (int i = t; i < 65536; i += kSampleBlock) {
-        const uint32_t v = hist2[i];
-        if (v) {
-            const double p = v / tot2;
-            e2 -= p * log2(p);
-        }
-    }
     const double H0 = block_sum_f64(e0, red);
-    const double Hpair = block_sum_f64(e2, red);
     const double sigma = block_sum_f64(s1, red);
+    // pair entropy from the block partials, summed in block order
+    double sv = 0, svl = 0;
+    for (int b = 0; b < kPairBlocks; b++) {
+        sv += epart[2 * b];
+        svl += epart[2 * b + 1];
+    }
+    const double Hpair = (tot2 > 0) ? (sv * log2(tot2) - svl) / tot2 : 0.0;
     if (t >= kNumCand) return;
     const double H1 = fmax(0.0, Hpair - H0);
     const uint32_t m = sk[0];
@@ -475,7 +500,12 @@ void wp_features_device(Storage& S, const uint8_t* T, uint64_t n, float* X, Feat
     k_feat_sketch<<<blocks, kBlock, 0, S.stream>>>(T, n, tab, tab + slots, (uint32_t)(slots - 1), S.sketch_epoch,
                                                    kpart);
     MIBWT_LAUNCH_CHECK();
-    k_feat_finish<<<1, kSampleBlock, 0, S.stream>>>(spart, chunks, hist2, kpart, blocks, n, norm, X, Xn);
+    double* epart = S.buf<double>(B_SEL_EPART, 2 * kPairBlocks);
+    uint32_t* rows = S.buf<uint32_t>(B_SEL_ROWS, 256);
+    k_feat_pairent<<<kPairBlocks, kSampleBlock, 0, S.stream>>>(hist2, epart, rows);
+    MIBWT_LAUNCH_CHECK();
+    k_feat_finish<<<1, kSampleBlock, 0, S.stream>>>(spart, chunks, hist2, kpart, blocks, epart, rows, n, norm, X,
+                                                    Xn);
     MIBWT_LAUNCH_CHECK();
 }
 
@@ -942,6 +972,59 @@ __global__ void k_sel_pick(const float* __restrict__ Y, int members, const float
     if (c == 0) out->best = (t == INFINITY) ? 0u : bi;
 }
 
+// Inference of every member of the ensemble in one launch: block (member, kFwdRows samples)
+// keeps the samples' activations in LDS and evaluates each unit with the same fmaf chain as
+// k_layer_fwd, so the outputs equal the layer-by-layer forward bit for bit.
+constexpr int kFwdRows = 8;
+struct Nets {
+    MlpDev dev[kMaxMembers];
+};
+
+__global__ __launch_bounds__(kBlock) void k_mlp_forward_all(const float* __restrict__ X, uint32_t B, Nets nets,
+                                                           float* __restrict__ Y)
+{
+    __shared__ float xs[kFwdRows][kIn];
+    __shared__ float h1[kFwdRows][kHid];
+    __shared__ float h2[kFwdRows][kHid];
+    // hidden-layer weights transposed in LDS ([input][unit], rows padded to an odd number of
+    // words so that the transposing stores do not collide): the lanes of a wavefront read
+    // consecutive units of one input
+    constexpr int kRow = kHid + 2;
+    __shared__ uint16_t w1t[kIn * kRow];
+    __shared__ uint16_t w2t[kHid * kRow];
+    const MlpDev m = nets.dev[blockIdx.y];
+    const uint32_t s0 = blockIdx.x * kFwdRows;
+    const uint32_t rows = (B - s0 < (uint32_t)kFwdRows) ? B - s0 : (uint32_t)kFwdRows;
+    for (uint32_t i = threadIdx.x; i < rows * kIn; i += kBlock) xs[i / kIn][i % kIn] = X[(uint64_t)s0 * kIn + i];
+    for (uint32_t i = threadIdx.x; i < kHid * kIn; i += kBlock) w1t[(i % kIn) * kRow + i / kIn] = m.W1[i];
+    for (uint32_t i = threadIdx.x; i < kHid * kHid; i += kBlock) w2t[(i % kHid) * kRow + i / kHid] = m.W2[i];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < rows * kHid; i += kBlock) {
+        const uint32_t r = i / kHid, o = i % kHid;
+        float acc = m.b1[o];
+#pragma unroll 8
+        for (int k = 0; k < kIn; k++) acc = fmaf(bf16_to_f32(w1t[k * kRow + o]), xs[r][k], acc);
+        h1[r][o] = fmaxf(acc, 0.f);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < rows * kHid; i += kBlock) {
+        const uint32_t r = i / kHid, o = i % kHid;
+        float acc = m.b2[o];
+#pragma unroll 8
+        for (int k = 0; k < kHid; k++) acc = fmaf(bf16_to_f32(w2t[k * kRow + o]), h1[r][k], acc);
+        h2[r][o] = fmaxf(acc, 0.f);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < rows * kOut; i += kBlock) {
+        const uint32_t r = i / kOut, o = i % kOut;
+        float acc = m.b3[o];
+        const uint16_t* w = m.W3 + (uint64_t)o * kHid;
+#pragma unroll 8
+        for (int k = 0; k < kHid; k++) acc = fmaf(bf16_to_f32(w[k]), h2[r][k], acc);
+        Y[((uint64_t)blockIdx.y * B + s0 + r) * kOut + o] = acc;
+    }
+}
+
 // The loaded model's parameters, bf16 weights and normalisation on this Storage's device;
 // uploaded again only when another model was loaded or trained since.
 struct ModelDev {
@@ -1000,8 +1083,11 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
     float* Xraw = S.buf<float>(B_SEL_XRAW, (size_t)kNumCand * kIn);
     float* Yall = S.buf<float>(B_SEL_YALL, (size_t)kNumCand * kOut * md.members);
     wp_features_device(S, T, n, Xraw, FeatNorm{md.xnorm, md.xnorm + kIn}, w.X);
-    for (int e = 0; e < md.members; e++)
-        mlp_forward(S, md.dev[e], w.X, kNumCand, w.H1, w.H2, Yall + (size_t)e * kNumCand * kOut);
+    Nets nets{};
+    for (int e = 0; e < md.members; e++) nets.dev[e] = md.dev[e];
+    k_mlp_forward_all<<<dim3(ceil_div((uint32_t)kNumCand, (uint32_t)kFwdRows), md.members), kBlock, 0, S.stream>>>(
+        w.X, kNumCand, nets, Yall);
+    MIBWT_LAUNCH_CHECK();
     SelPick* pick = S.buf<SelPick>(B_SEL_PICK, 1);
     k_sel_pick<<<1, 64, 0, S.stream>>>(Yall, md.members, md.ynorm, pick);
     MIBWT_LAUNCH_CHECK();
