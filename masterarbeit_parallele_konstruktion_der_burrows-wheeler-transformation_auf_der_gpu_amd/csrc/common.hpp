@@ -262,7 +262,7 @@ enum BufId : int {
     B_DT_OWNER,
     B_DT_RANK,
     B_DT_SA,
-    B_DT_LIM,
+    B_DT_POS,
     B_ACT_X,
     B_ACT_POS,
     B_ACT_X2,
@@ -338,7 +338,6 @@ enum BufId : int {
     B_LY_LABH,
     B_LY_GL,
     B_LY_GF,
-    B_AS_REC,
     B_AS_R,
     B_AS_WBITS,
     B_AS_WSCAN,

@@ -540,10 +540,18 @@ __global__ void k_dict_freq(const uint32_t* __restrict__ head, uint32_t du, uint
 
 // Segment limits of a string family.  Dictionary: the suffix at DT position x ends at the
 // end of its phrase.  Parse: one string of length d.
+// Per dictionary position x: its assembly record (see kRecAlpha) and the end (exclusive) of
+// its phrase, side by side so that one gather by x serves both.
+struct alignas(16) DictPos {
+    uint64_t rec;
+    uint32_t lim;
+    uint32_t pad;
+};
+
 struct LimDict {
     static constexpr bool kCarry = true;  // the rest length travels with the element
-    const uint32_t* lim;  // end (exclusive) of the phrase of each DT position, precomputed
-    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return lim[x]; }
+    const DictPos* pos;
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return pos[x].lim; }
 };
 struct LimParse {
     static constexpr bool kCarry = false;
@@ -563,36 +571,6 @@ __global__ void k_dt_owner(const uint64_t* __restrict__ off, const uint32_t* __r
 // Initial key of DT position x: `per` symbols of `bits` bits each, most significant first;
 // symbol code 0 = past the phrase end, 1 = terminator, 2.. = dense code of the byte (codes
 // follow byte order, so key order is string order).
-__global__ void k_dt_initkey(const uint8_t* __restrict__ T, uint64_t n,
-                             const uint32_t* __restrict__ owner, const uint64_t* __restrict__ off,
-                             const uint32_t* __restrict__ len, const uint32_t* __restrict__ start,
-                             const uint32_t* __restrict__ code, int bits, int per,
-                             uint32_t L, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
-                             uint32_t* __restrict__ lim)
-{
-    __shared__ uint32_t cs[256];
-    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
-    __syncthreads();
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= L) return;
-    const uint32_t u = owner[x];
-    const uint32_t j = x - (uint32_t)off[u];
-    const uint32_t rem = len[u] - j;
-    const uint64_t tp = (uint64_t)start[u] + j;
-    uint64_t key = 0;
-    uint64_t w = 0;
-    for (int i = 0; i < per; i++) {
-        if ((i & 7) == 0) w = (tp + i < n) ? load8(T, tp + i) : 0ull;
-        uint64_t s;
-        if ((uint32_t)i >= rem) s = 0;
-        else if (tp + i >= n) s = 1;
-        else s = cs[(w >> (8 * (i & 7))) & 0xFF];
-        key = (key << bits) | s;
-    }
-    K[x] = key;
-    V[x] = x;
-    lim[x] = (uint32_t)off[u] + len[u];
-}
 
 // Initial parse key: `per` consecutive phrase ranks (+1; 0 past the end) of B bits each.
 __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid,
@@ -684,21 +662,42 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 //   bit  41      whole phrase (j == 0).
 constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
-__global__ void k_dt_rec(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
-                         const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
-                         const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq, uint32_t L, int W,
-                         uint64_t* __restrict__ rec)
+// Per dictionary position x (dictionary order): the initial sort key (the first `per` symbols
+// of its suffix as dense codes of `bits` bits, 0 past the phrase end, 1 past the text end),
+// V = x, and its DictPos (record: frequency, preceding character, flags; phrase end).
+__global__ void k_dt_init(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ owner,
+                          const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                          const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
+                          const uint32_t* __restrict__ code, int bits, int per, uint32_t L, int W,
+                          uint64_t* __restrict__ K, uint32_t* __restrict__ V, DictPos* __restrict__ dp)
 {
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    __syncthreads();
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L) return;
     const uint32_t u = owner[x];
-    const uint32_t j = x - (uint32_t)off[u];
+    const uint32_t o = (uint32_t)off[u];
+    const uint32_t j = x - o;
     const uint32_t rem = len[u] - j;
+    const uint64_t tp = (uint64_t)start[u] + j;
+    uint64_t key = 0;
+    uint64_t w = 0;
+    for (int i = 0; i < per; i++) {
+        if ((i & 7) == 0) w = (tp + i < n) ? load8(T, tp + i) : 0ull;
+        uint64_t s;
+        if ((uint32_t)i >= rem) s = 0;
+        else if (tp + i >= n) s = 1;
+        else s = cs[(w >> (8 * (i & 7))) & 0xFF];
+        key = (key << bits) | s;
+    }
+    K[x] = key;
+    V[x] = x;
     uint64_t r = freq[u];
     if (rem > (uint32_t)W) r |= kRecAlpha;
     if (j == 0) r |= kRecWhole;
-    else if (rem > (uint32_t)W) r |= (uint64_t)T[start[u] + j - 1] << 32;
-    rec[x] = r;
+    else if (rem > (uint32_t)W) r |= (uint64_t)T[tp - 1] << 32;
+    dp[x] = DictPos{r, o + len[u], 0};
 }
 
 // Final placement of a dictionary position by the suffix sort: its record, group head and
@@ -711,12 +710,12 @@ struct alignas(16) SortedRec {
 };
 
 struct PlaceRec {
-    const uint64_t* rec;
+    const DictPos* dp;
     SortedRec* out;
     uint32_t* wbits;  // bit per sorted position: a whole phrase is placed there
     __device__ __forceinline__ void operator()(uint32_t*, uint32_t pos, uint32_t x, uint32_t head) const
     {
-        const uint64_t r = rec[x];
+        const uint64_t r = dp[x].rec;
         out[pos] = SortedRec{r, head, 0};
         if (r & kRecWhole) atomicOr(wbits + (pos >> 5), 1u << (pos & 31));
     }
@@ -938,7 +937,7 @@ __global__ void k_mix_bits(const uint8_t* __restrict__ gmix, uint32_t L, uint64_
 
 // mixed-group members in dictionary order (grouped by phrase); the group head of x is
 // rank[x] - 1
-__global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+__global__ void k_mx_flag(const DictPos* __restrict__ dp, const uint32_t* __restrict__ rank,
                           const uint64_t* __restrict__ mixbits, uint32_t L, uint32_t* __restrict__ mxf)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -947,7 +946,7 @@ __global__ void k_mx_flag(const uint64_t* __restrict__ rec, const uint32_t* __re
         mxf[x] = 0;
         return;
     }
-    const uint64_t r = rec[x];
+    const uint64_t r = dp[x].rec;
     const bool cand = (r & kRecAlpha) && !(r & kRecWhole);
     const uint32_t head = rank[x] - 1;
     mxf[x] = (cand && ((mixbits[head >> 6] >> (head & 63)) & 1u)) ? 1u : 0u;
@@ -967,13 +966,13 @@ struct Ent {
 };
 
 template <typename EntT>
-__global__ void k_mx_write(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ rank,
+__global__ void k_mx_write(const DictPos* __restrict__ dp, const uint32_t* __restrict__ rank,
                            const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
                            const uint32_t* __restrict__ cid, uint32_t L, EntT* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
     if (x >= L || !mxf[x]) return;
-    mx[mxs[x]] = Ent<EntT>::make(cid[rank[x] - 1], (uint32_t)(rec[x] >> 32) & 0xFFu);
+    mx[mxs[x]] = Ent<EntT>::make(cid[rank[x] - 1], (uint32_t)(dp[x].rec >> 32) & 0xFFu);
 }
 
 // slot offset minus entry offset of each entry group, by its dense id
@@ -1467,22 +1466,19 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     // initial key width: whole radix passes trade against one more doubling round
     const int key_cap = init_key_bits();
     const int aper = std::max(1, key_cap / abits);
-    uint32_t* dlim = S.buf<uint32_t>(B_DT_LIM, L);
-    k_dt_initkey<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, codes, abits, aper,
-                                                        L, K0, V0, dlim);
+    DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
+    k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, freq, codes, abits, aper, L,
+                                                     W, K0, V0, dp);
     MIBWT_LAUNCH_CHECK();
     // per dictionary position a record (frequency, preceding character, flags); the sort
     // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
-    uint64_t* rec = S.buf<uint64_t>(B_AS_REC, L);
     SortedRec* RH = reinterpret_cast<SortedRec*>(S.buf<uint4>(B_AS_R, L));
     const uint32_t wwords = (uint32_t)ceil_div((uint64_t)L, (uint64_t)32);
     uint32_t* wbits = S.buf<uint32_t>(B_AS_WBITS, wwords);
     fill<uint32_t>(S, wbits, wwords, 0u);
-    k_dt_rec<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dstart, freq, L, W, rec);
-    MIBWT_LAUNCH_CHECK();
-    LimDict limd{dlim};
+    LimDict limd{dp};
     DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper,
-                                                           limd, drank, nullptr, PlaceRec{rec, RH, wbits});
+                                                           limd, drank, nullptr, PlaceRec{dp, RH, wbits});
     S.phase_end(ph, rd.bytes);
     S.stats.doubling_rounds_dict = rd.rounds;
 
@@ -1556,7 +1552,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint64_t* mixbits = S.buf<uint64_t>(B_G_MIXBITS, ceil_div((uint64_t)L, (uint64_t)64));
     k_mix_bits<<<grid_for(L), kBlock, 0, S.stream>>>(gmix, L, mixbits);
     MIBWT_LAUNCH_CHECK();
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(rec, drank, mixbits, L, mxf);
+    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(dp, drank, mixbits, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
@@ -1571,7 +1567,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         using EntT = decltype(tag);
         constexpr bool k32 = sizeof(EntT) == 4;
         EntT* mx = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_MX_ENT, nmx + 1));
-        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(rec, drank, mxf, mxs, cid, L, mx);
+        k_mx_write<EntT><<<grid_for(L), kBlock, 0, S.stream>>>(dp, drank, mxf, mxs, cid, L, mx);
         MIBWT_LAUNCH_CHECK();
         k_phrase_rec<<<grid_for(du), kBlock, 0, S.stream>>>(doff, dlen, mxs, drank, cid, du, prec);
         MIBWT_LAUNCH_CHECK();
