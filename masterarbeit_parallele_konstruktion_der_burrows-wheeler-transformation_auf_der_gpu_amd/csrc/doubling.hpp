@@ -331,14 +331,16 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
     NGz[p] = c;
 }
 
-// Compaction of the kept members in one pass over chained tiles of kChainTile slots: the
-// block takes kChainItems rounds of kBlock consecutive slots, records each wavefront's kept
+// Compaction of the kept members in one pass over chained tiles of kCompactTile slots: the
+// block takes kCompactRounds rounds of kBlock consecutive slots, records each wavefront's kept
 // mask, scans the (round, wavefront) counts, obtains the tile's offset by decoupled look-back
 // and moves every kept member to its place with coalesced reads.  A kept sub-group is kept
 // whole and stays contiguous, so its first slot moves to o - (p - first slot) for a member that
 // moves from p to o.  The first compaction reads the initial groups (kInit: NV = initial
 // order, NHs = 1-based group id, NGs = group offsets).  Also counts the kept members whose
 // sub-group is large next round (lpart per tile, summed by k_sum_tiles) and the total (*active).
+constexpr int kCompactRounds = 16;
+constexpr uint32_t kCompactTile = kCompactRounds * kBlock;  // 4096 slots: many tiles fill the chip
 template <bool kInit>
 __global__ __launch_bounds__(kBlock) void k_dbl_compact(const uint32_t* __restrict__ keep, uint32_t m,
                                                        uint32_t small_max, const uint32_t* __restrict__ NV,
@@ -351,17 +353,18 @@ __global__ __launch_bounds__(kBlock) void k_dbl_compact(const uint32_t* __restri
                                                        uint32_t* __restrict__ active, uint32_t* __restrict__ lpart,
                                                        Chain c, uint32_t tiles)
 {
-    constexpr int kRounds = kChainItems;
-    static_assert(kRounds * kWaves == kBlock, "one (round, wavefront) count per thread");
+    constexpr int kRounds = kCompactRounds;
+    static_assert(kRounds * kWaves == 64, "the (round, wavefront) counts are scanned by one wavefront");
     __shared__ uint64_t bal[kRounds * kWaves];
     __shared__ uint32_t off[kRounds * kWaves];
     __shared__ uint32_t lds[kWaves];
     __shared__ uint32_t slot;
+    __shared__ uint32_t tile_total;
     const uint32_t tile = chain_ticket(c, tiles, &slot);
-    const uint64_t base = (uint64_t)tile * kChainTile;
+    const uint64_t base = (uint64_t)tile * kCompactTile;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     // all flag loads of a batch are issued before the first ballot
-    constexpr int kBatch = 16;
+    constexpr int kBatch = kRounds;
     for (int it0 = 0; it0 < kRounds; it0 += kBatch) {
         uint32_t kv[kBatch];
 #pragma unroll
@@ -379,10 +382,14 @@ __global__ __launch_bounds__(kBlock) void k_dbl_compact(const uint32_t* __restri
         }
     }
     __syncthreads();
-    const uint32_t cnt = off[threadIdx.x];
-    uint32_t tot;
-    const uint32_t ex = block_exclusive_scan(cnt, 0u, OpSum{}, lds, &tot);
-    off[threadIdx.x] = ex;  // each thread rewrites only its own word; chain_prefix synchronizes
+    if (w == 0) {
+        const uint32_t cnt = off[lane];
+        const uint32_t inc = wave_inclusive_scan(cnt, OpSum{});
+        off[lane] = inc - cnt;
+        if (lane == 63) tile_total = inc;
+    }
+    __syncthreads();
+    const uint32_t tot = tile_total;
     const uint32_t pre = chain_prefix(c, tile, tot, OpSum{}, 0u, &slot);
     const uint64_t lt = lanemask_lt();
     uint32_t nl = 0;
@@ -504,7 +511,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     for (;;) {
         // ---- compaction of the kept sub-groups with next round's counts
         {
-            const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, (uint32_t)kChainTile));
+            const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, kCompactTile));
             uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
             const Chain c = S.chain(tiles);
             if (first) {
