@@ -136,6 +136,7 @@ int64_t libcubwt_free_device_storage(void* device_storage)
             if (b.ptr) MIBWT_HIP(hipFree(b.ptr));
         S->bufs.clear();
         if (S->pinned) MIBWT_HIP(hipHostFree(S->pinned));
+        if (S->sig) MIBWT_HIP(hipHostFree(S->sig));
         for (auto& p : S->phases) {
             if (p.start) MIBWT_HIP(hipEventDestroy(p.start));
             if (p.stop) MIBWT_HIP(hipEventDestroy(p.stop));

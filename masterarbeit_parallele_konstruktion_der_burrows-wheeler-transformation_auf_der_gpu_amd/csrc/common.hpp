@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,13 @@ struct Storage {
     std::vector<DevBuf> bufs;
     void* pinned = nullptr;          // small pinned host buffer for scalar read-backs
     size_t pinned_cap = 0;
+    // Device-to-host signal: a kernel stores a few words into mapped pinned memory and then a
+    // sequence number (system scope, release); the host polls the sequence number instead of
+    // synchronising the stream, which answers a per-round count in a few microseconds.
+    static constexpr int kSigWords = 63, kSigSeq = 63;
+    uint32_t* sig = nullptr;      // host view
+    uint32_t* sig_dev = nullptr;  // device view
+    uint32_t sig_seq = 0;
     // phase timing
     std::vector<Phase> phases;
     int phase_used = 0;
@@ -156,6 +164,47 @@ struct Storage {
         }
         return reinterpret_cast<T*>(pinned);
     }
+
+    uint32_t* signal_dev()
+    {
+        if (!sig) {
+            void* p = nullptr;
+            MIBWT_HIP(hipHostMalloc(&p, 4 * (kSigSeq + 1), hipHostMallocMapped | hipHostMallocCoherent));
+            sig = reinterpret_cast<uint32_t*>(p);
+            memset(sig, 0, 4 * (kSigSeq + 1));
+            MIBWT_HIP(hipHostGetDevicePointer(reinterpret_cast<void**>(&sig_dev), p, 0));
+        }
+        return sig_dev;
+    }
+    // next sequence number for a signal that is about to be enqueued (never 0)
+    uint32_t signal_next()
+    {
+        sig_seq = (sig_seq == 0xFFFFFFFFu) ? 1u : sig_seq + 1;
+        return sig_seq;
+    }
+    // Wait until the device published `seq`, then copy `count` signal words.  After 200 ms of
+    // polling the stream is synchronised instead (which also surfaces a failed kernel); the
+    // signal must be there once the stream is idle.
+    void wait_signal(uint32_t seq, uint32_t* dst, int count)
+    {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (;;) {
+            if (__atomic_load_n(sig + kSigSeq, __ATOMIC_ACQUIRE) == seq) break;
+            if (std::chrono::steady_clock::now() - t0 > std::chrono::milliseconds(200)) {
+                MIBWT_HIP(hipStreamSynchronize(stream));
+                if (__atomic_load_n(sig + kSigSeq, __ATOMIC_ACQUIRE) != seq)
+                    throw Error(LIBCUBWT_GPU_ERROR, "device signal missing after the stream completed");
+                break;
+            }
+            __builtin_ia32_pause();
+        }
+        for (int i = 0; i < count; i++) dst[i] = __atomic_load_n(sig + i, __ATOMIC_RELAXED);
+    }
+
+    // read `count` device values of type T through the signal words (no stream
+    // synchronisation; for read-backs inside a build, at most kSigWords words)
+    template <typename T>
+    void read_fast(T* dst, const T* dev, size_t count = 1);
 
     // read `count` device values of type T (synchronises the stream)
     template <typename T>
@@ -394,6 +443,33 @@ enum ScalarSlot : int {
     SC_NGROUPS,                   // groups that receive BWT entries
     SC_COUNT_ = 16
 };
+
+// One thread stores `words` words to the mapped signal memory, then the sequence number.
+static __global__ void k_signal_copy(const uint32_t* __restrict__ src, uint32_t words, uint32_t* __restrict__ sig,
+                                     uint32_t seq)
+{
+    if (threadIdx.x != 0) return;
+    for (uint32_t i = 0; i < words; i++) __hip_atomic_store(sig + i, src[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    __hip_atomic_store(sig + Storage::kSigSeq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+template <typename T>
+void Storage::read_fast(T* dst, const T* dev, size_t count)
+{
+    static_assert(sizeof(T) % 4 == 0, "whole words");
+    const uint32_t words = (uint32_t)(count * sizeof(T) / 4);
+    if (words > (uint32_t)kSigWords) {
+        read(dst, dev, count);
+        return;
+    }
+    uint32_t* sd = signal_dev();
+    const uint32_t seq = signal_next();
+    k_signal_copy<<<1, 64, 0, stream>>>(reinterpret_cast<const uint32_t*>(dev), words, sd, seq);
+    MIBWT_HIP(hipGetLastError());
+    uint32_t tmp[kSigWords];
+    wait_signal(seq, tmp, (int)words);
+    memcpy(dst, tmp, 4 * words);
+}
 
 inline Chain Storage::chain(uint32_t tiles)
 {

@@ -83,12 +83,15 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
 
 // ---- one refinement round over the m active elements (groups contiguous)
 
+// The active count is read from the device (*mdev, written by the compaction): the round can be
+// enqueued before the host knows the count, with a grid sized for an upper bound.
 template <typename Policy>
-__global__ void k_key2(const uint32_t* __restrict__ V, const uint32_t* __restrict__ R, uint32_t m,
-                       const uint32_t* __restrict__ rank, Policy pol, uint32_t* __restrict__ key2)
+__global__ void k_key2(const uint32_t* __restrict__ V, const uint32_t* __restrict__ R,
+                       const uint32_t* __restrict__ mdev, const uint32_t* __restrict__ rank, Policy pol,
+                       uint32_t* __restrict__ key2)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
+    if (i >= *mdev) return;
     uint32_t r = 0;
     if constexpr (Policy::kCarry) r = R[i];
     const uint32_t y = pol.next(V[i], r);
@@ -133,7 +136,8 @@ template <typename Policy>
 __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
                                                      const uint32_t* __restrict__ Gs, const uint32_t* __restrict__ Gz,
                                                      const uint32_t* __restrict__ key2, const uint32_t* __restrict__ R,
-                                                     uint32_t m, uint32_t small_max, uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ mdev, uint32_t small_max,
+                                                     uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
                                                      uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs,
                                                      uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
@@ -150,6 +154,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
     __shared__ uint32_t red[2 * kWaves];
     const uint32_t b0 = blockIdx.x * kBlock;
     const uint32_t t = threadIdx.x;
+    const uint32_t m = *mdev;
+    if (b0 >= m) return;  // the whole block (grid sized for an upper bound)
 
     // ---- own slots: classification and staging
     const uint32_t i0 = b0 + t;
@@ -440,15 +446,28 @@ __global__ __launch_bounds__(kBlock) void k_dbl_compact(const uint32_t* __restri
     }
 }
 
+// Also publishes (*active, total) to the host signal words with sequence number `seq` when
+// `sig` is given (stores to mapped host memory, then the sequence number with release
+// semantics at system scope).
 static __global__ __launch_bounds__(kBlock) void k_sum_tiles(const uint32_t* __restrict__ part, uint32_t tiles,
-                                                            uint32_t* __restrict__ out)
+                                                            uint32_t* __restrict__ out,
+                                                            const uint32_t* __restrict__ active = nullptr,
+                                                            uint32_t* __restrict__ sig = nullptr,
+                                                            uint32_t seq = 0)
 {
     __shared__ uint32_t lds[kWaves];
     uint32_t acc = 0;
     for (uint32_t t = threadIdx.x; t < tiles; t += kBlock) acc += part[t];
     uint32_t tot;
     (void)block_exclusive_scan(acc, 0u, OpSum{}, lds, &tot);
-    if (threadIdx.x == 0) *out = tot;
+    if (threadIdx.x == 0) {
+        *out = tot;
+        if (sig) {
+            __hip_atomic_store(sig, *active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(sig + 1, tot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            __hip_atomic_store(sig + Storage::kSigSeq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        }
+    }
 }
 
 struct DoublingResult {
@@ -510,6 +529,8 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t m = N;
     for (;;) {
         // ---- compaction of the kept sub-groups with next round's counts
+        uint32_t* sigd = S.signal_dev();
+        const uint32_t seq = S.signal_next();
         {
             const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, kCompactTile));
             uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
@@ -523,23 +544,31 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
                                                                      Hs, Gs, Gz, R, sc + SC_ACTIVE, lpart, c, tiles);
             }
             MIBWT_LAUNCH_CHECK();
-            k_sum_tiles<<<1, kBlock, 0, S.stream>>>(lpart, tiles, sc + SC_LARGE);
+            k_sum_tiles<<<1, kBlock, 0, S.stream>>>(lpart, tiles, sc + SC_LARGE, sc + SC_ACTIVE, sigd, seq);
             MIBWT_LAUNCH_CHECK();
         }
-        uint32_t hv[SC_COUNT_];
-        S.read(hv, sc, SC_COUNT_);
-        const uint32_t m2 = hv[SC_ACTIVE], nl = hv[SC_LARGE];
+        // ---- order each group by the rank of the next position: secondary keys and the sort of
+        // the small groups; with Policy::kSpeculate enqueued before the count comes back (the
+        // kernels read it on the device, grids sized for the previous count), so the device
+        // does not idle while the host waits
+        const uint32_t* mdev = sc + SC_ACTIVE;
+        auto sort_round = [&](uint32_t mb) {
+            k_key2<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, R, mdev, rank, pol, key2);
+            MIBWT_LAUNCH_CHECK();
+            advance(pol, V, mb);
+            k_seg_small<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, mdev, kSmallGroup, rank,
+                                                                       SA, keep, NV, NHs, NGs, NGz, NR, large, pol);
+            MIBWT_LAUNCH_CHECK();
+        };
+        if constexpr (Policy::kSpeculate) sort_round(m);
+        uint32_t hv[2];
+        S.wait_signal(seq, hv, 2);
+        const uint32_t m2 = hv[0], nl = hv[1];
         res.rounds++;
         res.bytes += (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4);
         if (m2 == 0) break;
         m = m2;
-        // ---- order each group by the rank of the next position
-        k_key2<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, R, m, rank, pol, key2);
-        MIBWT_LAUNCH_CHECK();
-        advance(pol, V, m);
-        k_seg_small<Policy><<<grid_for(m), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, m, kSmallGroup, rank, SA,
-                                                                  keep, NV, NHs, NGs, NGz, NR, large, pol);
-        MIBWT_LAUNCH_CHECK();
+        if constexpr (!Policy::kSpeculate) sort_round(m);
         res.bytes += (double)m * (4 * 12);
         if (nl > 0) {
             uint32_t* lpos = pos;

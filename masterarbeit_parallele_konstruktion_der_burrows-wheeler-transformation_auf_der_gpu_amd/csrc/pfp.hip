@@ -595,6 +595,7 @@ __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint
 template <typename Lim, typename Place = PlaceSA>
 struct StringPolicy {
     static constexpr bool kCarry = Lim::kCarry;
+    static constexpr bool kSpeculate = true;  // advancing is host-only (h doubles)
     Lim lim;
     uint32_t h;
     Place place_fn;
@@ -1268,7 +1269,7 @@ uint32_t pfp_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, u
                                          S.stream));
             }
             uint32_t hv[SC_COUNT_];
-            S.read(hv, sc, SC_COUNT_);
+            S.read_fast(hv, sc, SC_COUNT_);
             t = hv[SC_TRIGGERS];
             if (codes) *sigma = hv[SC_SIGMA];
             if (kt >= 0) S.kt_bytes[kt] = (double)n + 4.0 * std::min(t, cap);
@@ -1290,7 +1291,7 @@ uint32_t pfp_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, u
             MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
         }
         uint32_t hv[SC_COUNT_];
-        S.read(hv, sc, SC_COUNT_);
+        S.read_fast(hv, sc, SC_COUNT_);
         t = hv[SC_TRIGGERS];
         if (codes) *sigma = hv[SC_SIGMA];
         E = S.buf<uint32_t>(B_TRIG, (size_t)t + extra);
@@ -1406,7 +1407,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         inclusive_sum<uint32_t>(S, flag, K1 ? reinterpret_cast<uint32_t*>(K1) : nullptr, d, sc + SC_DISTINCT);
         uint32_t hv[SC_COUNT_];
-        S.read(hv, sc, SC_COUNT_);
+        S.read_fast(hv, sc, SC_COUNT_);
         du = hv[SC_DISTINCT];
         if (hv[SC_COLLISION] & 2u) {  // distinct fingerprints under one sort key: sort all 64 bits
             key_bits = 64;
@@ -1434,7 +1435,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         // one read-back: collision flag, dictionary length, alphabet size
         MIBWT_HIP(hipMemcpyAsync(sc + SC_DICTLEN, doff + du, sizeof(uint64_t), hipMemcpyDeviceToDevice, S.stream));
         MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
-        S.read(hv, sc, SC_COUNT_);
+        S.read_fast(hv, sc, SC_COUNT_);
         if (hv[SC_COLLISION] & 1u) {  // equal fingerprints of different phrases: new seed
             seed = seed * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull;
             continue;
@@ -1538,7 +1539,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
     exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t hv0[SC_COUNT_];
-    S.read(hv0, sc, SC_COUNT_);
+    S.read_fast(hv0, sc, SC_COUNT_);
     MIBWT_REQUIRE(hv0[SC_TOTAL] == (uint32_t)n, LIBCUBWT_GPU_ERROR, "BWT slot count mismatch");
     const uint32_t ngroups = hv0[SC_NGROUPS];
     const int gbits = std::max(1, bits_for(ngroups > 0 ? ngroups - 1 : 0));
@@ -1556,7 +1557,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
-    S.read(&nmx, sc + SC_ENTRIES);
+    S.read_fast(&nmx, sc + SC_ENTRIES);
     PhraseRec* prec = S.buf<PhraseRec>(B_AS_PREC, du);
     uint32_t* qc = S.buf<uint32_t>(B_Q_CNT, d);
     uint32_t* qo = S.buf<uint32_t>(B_Q_OFF, d);
@@ -1579,7 +1580,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         exclusive_sum<uint32_t>(S, qc, qo, d, sc + SC_ENTRIES);
         uint32_t hv[SC_COUNT_];
-        S.read(hv, sc, SC_COUNT_);
+        S.read_fast(hv, sc, SC_COUNT_);
         ne = hv[SC_ENTRIES] + 1;
         // the histograms from the group counts are used only if they count every entry
         const uint32_t* eh = (hv[SC_EHSUM] == ne) ? ehist : nullptr;

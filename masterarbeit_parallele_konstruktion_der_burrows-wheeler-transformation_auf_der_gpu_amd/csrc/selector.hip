@@ -1092,7 +1092,7 @@ void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, in
     k_sel_pick<<<1, 64, 0, S.stream>>>(Yall, md.members, md.ynorm, pick);
     MIBWT_LAUNCH_CHECK();
     SelPick h;
-    S.read(&h, pick);
+    S.read_fast(&h, pick);
     if (pred_out)
         for (int c = 0; c < kNumCand; c++) pred_out[c] = h.pred[c];
     const int best = (int)std::min<uint32_t>(h.best, kNumCand - 1);
