@@ -75,10 +75,11 @@ int64_t libcubwt_unbwt(void* device_storage, const uint8_t* T, uint8_t* U, int64
 
 /* ---------------------------------------------------------------- mibwt extensions */
 
-/* Prefix-free-parsing BWT with runtime window W (1..64) and modulus P (1..65535).
- * Replaces the compile-time variants Prefix_Free_Parsing_W=<W>_P=<P>.cu (main at
- * Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:4793-4815, Compute_D :2415, Compute_S :4330).
- * Host buffers, libcubwt format; 1 <= n <= max_length. */
+/* Prefix-free-parsing BWT with runtime window W (1..64) and modulus P (1..65535), or
+ * W = P = 0 for the learned selector's choice.  Replaces the compile-time variants
+ * Prefix_Free_Parsing_W=<W>_P=<P>.cu (main at Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:
+ * 4793-4815, Compute_D :2415, Compute_S :4330) and the hand-written per-file (W,P) table of
+ * Benchmark_Skript_Linux.sh:14-73.  Host buffers, libcubwt format; 1 <= n <= max_length. */
 int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int32_t W,
                       int32_t P);
 
@@ -172,6 +173,11 @@ int64_t mibwt_wp_grid(int32_t* W, int32_t* P);
 
 /* Features of the 41 candidates for the device text dT[0..n): X[41][64]. */
 int64_t mibwt_wp_features(void* storage, const uint8_t* dT, int64_t n, float* X);
+
+/* (W,P) for the host text T[0..n) (a temporary storage of n bytes is allocated and freed;
+ * callers that build afterwards use mibwt_select_wp_device on their storage instead).
+ * Returns 1 when a trained model chose, 0 for the fixed W=5,P=16 fallback, or an error code. */
+int64_t mibwt_select_wp(const uint8_t* T, int64_t n, int32_t* W, int32_t* P);
 
 /* (W,P) for the device text dT[0..n); pred (may be NULL) receives the 41 predicted
  * log(ms per byte).  Returns 1 when a trained model chose, 0 for the fixed W=5,P=16 fallback. */

@@ -52,6 +52,7 @@ EXPORTED = [
     "mibwt_batch_bwt",
     "mibwt_wp_grid",
     "mibwt_wp_features",
+    "mibwt_select_wp",
     "mibwt_select_wp_device",
     "mibwt_selector_load",
     "mibwt_mlp_grad",
@@ -125,6 +126,7 @@ def lib():
                                       i32, i32, i32, i32]
         L.mibwt_wp_grid.argtypes = [i32p, i32p]
         L.mibwt_wp_features.argtypes = [vp, vp, i64, fp]
+        L.mibwt_select_wp.argtypes = [vp, i64, i32p, i32p]
         L.mibwt_select_wp_device.argtypes = [vp, vp, i64, i32p, i32p, fp]
         L.mibwt_selector_load.argtypes = [ctypes.c_char_p]
         L.mibwt_mlp_grad.argtypes = [vp, fp, fp, fp, i64, fp, fp]
@@ -186,6 +188,17 @@ def smooth_along_p(t: np.ndarray, num_p: int = 10) -> np.ndarray:
         nb = [t[c]] + ([t[c - 1]] if pi > 0 else []) + ([t[c + 1]] if pi + 1 < num_p else [])
         s[c] = np.mean(nb)
     return s
+
+
+def select_wp(T):
+    """(W, P) the selector chooses for the host text T, and whether a trained model chose
+    (mibwt_select_wp)."""
+    T = _as_text(T)
+    W, P = ctypes.c_int32(), ctypes.c_int32()
+    rc = lib().mibwt_select_wp(_u8(T), len(T), ctypes.byref(W), ctypes.byref(P))
+    if rc < 0:
+        raise BwtError(rc, "mibwt_select_wp")
+    return (W.value, P.value), bool(rc)
 
 
 def selector_load(path: str):

@@ -79,13 +79,24 @@ int64_t host_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int met
 {
     Storage* S = reinterpret_cast<Storage*>(storage);
     if (!S || !T || !L || n < min_n || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
-    if (method == MIBWT_METHOD_PFP && (W < 1 || W > 64 || P < 1 || P > 65535)) return LIBCUBWT_BAD_PARAMETER;
+    const bool learned = (method == MIBWT_METHOD_PFP && W == 0 && P == 0);
+    if (method == MIBWT_METHOD_PFP && !learned && (W < 1 || W > 64 || P < 1 || P > 65535))
+        return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         uint8_t* dO = S->buf<uint8_t>(B_STAGE, (size_t)n);
         const uint8_t wrap = T[n - 1];
-        int64_t idx = (method == MIBWT_METHOD_PFP) ? pfp_bwt_run(*S, T, true, dO, n, W, P)
-                                                   : lyndon_bwt_run(*S, T, true, dO, n);
+        int64_t idx;
+        if (learned) {
+            // upload once; the (W,P) choice reads the staged text, the build reuses it
+            uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
+            int w = 5, p = 16;
+            selector_choose_device(*S, dT, (uint64_t)n, &w, &p, nullptr);
+            idx = pfp_bwt_selected(*S, dO, n, w, p);
+        } else {
+            idx = (method == MIBWT_METHOD_PFP) ? pfp_bwt_run(*S, T, true, dO, n, W, P)
+                                               : lyndon_bwt_run(*S, T, true, dO, n);
+        }
         if (idx < 0) return idx;
         copy_out_cubwt(*S, dO, L, wrap, n, idx);  // wrap read before L (may alias T) is written
         return idx;
@@ -421,6 +432,26 @@ int64_t mibwt_select_wp_device(void* storage, const uint8_t* dT, int64_t n, int3
         *P = p;
         return selector_has_model() ? 1 : 0;
     });
+}
+
+int64_t mibwt_select_wp(const uint8_t* T, int64_t n, int32_t* W, int32_t* P)
+{
+    if (!T || !W || !P || n < 1) return LIBCUBWT_BAD_PARAMETER;
+    void* s = nullptr;
+    int64_t rc = libcubwt_allocate_device_storage(&s, n);
+    if (rc != LIBCUBWT_NO_ERROR) return rc;
+    Storage* S = reinterpret_cast<Storage*>(s);
+    rc = guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        uint8_t* dT = stage_text(*S, T, true, (uint64_t)n);
+        int w = 5, p = 16;
+        selector_choose_device(*S, dT, (uint64_t)n, &w, &p, nullptr);
+        *W = w;
+        *P = p;
+        return selector_has_model() ? 1 : 0;
+    });
+    libcubwt_free_device_storage(s);
+    return rc;
 }
 
 int64_t mibwt_wp_features(void* storage, const uint8_t* dT, int64_t n, float* X)

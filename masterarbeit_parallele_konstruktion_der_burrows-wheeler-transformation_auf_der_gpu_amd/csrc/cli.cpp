@@ -86,7 +86,8 @@ bool write_reference_text(const char* path, const uint8_t* O, int64_t count, int
 
 int usage(const char* prog)
 {
-    fprintf(stderr, "usage: %s [-W w] [-P p] [-i input.txt] [-o out.txt] [--raw] [--driver-quirk]\n", prog);
+    fprintf(stderr, "usage: %s [-W w] [-P p | --learned] [-i input.txt] [-o out.txt] [--raw] [--driver-quirk]\n",
+            prog);
     return 2;
 }
 
@@ -104,6 +105,7 @@ int main(int argc, char** argv)
         else if (a == "-P" && i + 1 < argc) P = atoi(argv[++i]);
         else if (a == "-i" && i + 1 < argc) in = argv[++i];
         else if (a == "-o" && i + 1 < argc) outp = argv[++i];
+        else if (a == "--learned") W = P = 0;  // the learned selector chooses (W,P)
         else if (a == "--raw") raw = true;
         else if (a == "--driver-quirk") quirk = true;
         else return usage(argv[0]);

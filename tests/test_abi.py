@@ -58,6 +58,9 @@ def test_bad_parameters_rejected_without_device(mibwt):
     assert lib.mibwt_last_phase_stats(None, None) == mibwt.BAD_PARAMETER
     assert lib.mibwt_set_kernel_timing(None, None) == mibwt.BAD_PARAMETER
     assert lib.mibwt_time_kernel(None, b"scan", 1, None, None) == mibwt.BAD_PARAMETER
+    W, P = ctypes.c_int32(), ctypes.c_int32()
+    assert lib.mibwt_select_wp(None, 32, ctypes.byref(W), ctypes.byref(P)) == mibwt.BAD_PARAMETER
+    assert lib.mibwt_select_wp(u8, 0, ctypes.byref(W), ctypes.byref(P)) == mibwt.BAD_PARAMETER
 
 
 def test_reference_format_conversion(mibwt):

@@ -40,6 +40,8 @@ def test_pfp_cli_matches_oracle_format(tmp_path):
     assert not out.endswith(b"\n") and out.endswith(b" ")
     out35 = run(tmp_path, "pfp_bwt", "-W", "35", "-P", "81")
     assert out35 == out
+    # the learned (W,P) choice replaces the benchmark script's per-file table
+    assert run(tmp_path, "pfp_bwt", "--learned") == out
 
 
 def test_compare_script_semantics(tmp_path):

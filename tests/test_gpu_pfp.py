@@ -152,6 +152,12 @@ def test_device_entry_learned_choice(mibwt):
     assert (s["W"], s["P"]) in mibwt.wp_grid()
     dT.free()
     dO.free()
+    # the host entry with W = P = 0 makes the same choice and the same BWT
+    with mibwt.Storage(len(T)) as st:
+        L, hidx = st.bwt(T, "pfp", 0, 0)
+        hs = st.stats()
+    assert hidx == oidx and (mibwt.cubwt_to_reference(L, hidx) == O).all()
+    assert (hs["W"], hs["P"]) == (s["W"], s["P"])
 
 
 def test_narrow_fingerprint_sort_widens_on_collision(mibwt, monkeypatch):

@@ -128,6 +128,8 @@ def test_selector_choice_keeps_bwt_exact(mibwt):
             assert np.allclose(t, pred, rtol=0, atol=1e-4)
             assert mibwt.wp_grid()[int(np.argmin(pred))] == (W, P)
         L, idx = st.bwt(T, "cubwt")
+    # the host-text entry makes the same choice
+    assert mibwt.select_wp(T) == ((W, P), trained)
     O, oidx = oracle.bwt(T)
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
     dT.free()
