@@ -553,9 +553,15 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         // does not idle while the host waits
         const uint32_t* mdev = sc + SC_ACTIVE;
         auto sort_round = [&](uint32_t mb) {
+            // Policy::kAdvanceFirst: the step to this round's positions is taken at the start of
+            // the round (from the second on) over the compacted active set only.  For chains this
+            // is enough: an element that stays active after a round points (before the step) at
+            // an element that was active in that round, so it was stepped at that round's start.
+            if constexpr (Policy::kAdvanceFirst)
+                if (res.rounds > 1) advance(pol, V, mb);
             k_key2<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, R, mdev, rank, pol, key2);
             MIBWT_LAUNCH_CHECK();
-            advance(pol, V, mb);
+            if constexpr (!Policy::kAdvanceFirst) advance(pol, V, mb);
             k_seg_small<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, mdev, kSmallGroup, rank,
                                                                        SA, keep, NV, NHs, NGs, NGz, NR, large, pol);
             MIBWT_LAUNCH_CHECK();

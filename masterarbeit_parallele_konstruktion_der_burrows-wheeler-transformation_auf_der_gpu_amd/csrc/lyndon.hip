@@ -503,7 +503,8 @@ __global__ void k_u32_to_key(const uint32_t* __restrict__ a, uint64_t n, uint64_
 
 struct ChainPolicy {
     static constexpr bool kCarry = false;
-    static constexpr bool kSpeculate = false;  // advancing launches pointer jumping over the count
+    static constexpr bool kSpeculate = false;   // advancing launches pointer jumping over the count
+    static constexpr bool kAdvanceFirst = true;  // step only the elements that stay active
     const uint32_t* jmp;
     uint32_t n;
     __device__ __forceinline__ uint32_t rem(uint32_t) const { return 0; }

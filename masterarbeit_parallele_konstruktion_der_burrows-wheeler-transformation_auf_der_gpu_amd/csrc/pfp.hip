@@ -595,7 +595,8 @@ __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint
 template <typename Lim, typename Place = PlaceSA>
 struct StringPolicy {
     static constexpr bool kCarry = Lim::kCarry;
-    static constexpr bool kSpeculate = true;  // advancing is host-only (h doubles)
+    static constexpr bool kSpeculate = true;      // advancing is host-only (h doubles)
+    static constexpr bool kAdvanceFirst = false;  // open() of this round uses the doubled h
     Lim lim;
     uint32_t h;
     Place place_fn;
