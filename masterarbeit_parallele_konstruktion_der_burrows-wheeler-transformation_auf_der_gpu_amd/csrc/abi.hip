@@ -16,7 +16,8 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
 int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n, int64_t index, bool ref_format,
                   uint32_t* dI, int64_t rate);
-void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred);
+void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred,
+                            uint8_t* copy = nullptr);
 bool selector_load(const char* path);
 bool selector_has_model();
 void wp_features(Storage& S, const uint8_t* T, uint64_t n, float* X);
@@ -271,10 +272,13 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
         }
         int64_t idx;
         if (learned) {
-            // stage once: the (W,P) choice reads the staged text, the build reuses it
-            uint8_t* T = stage_text(*S, dT, false, (uint64_t)n);
+            // the selector's pass over the caller's text also fills the private padded copy
+            // that the build reads
+            const uint64_t bytes = staged_text_bytes((uint64_t)n);
+            uint8_t* T = S->buf<uint8_t>(B_TEXT, bytes);
+            MIBWT_HIP(hipMemsetAsync(T + n, 0, bytes - (uint64_t)n, S->stream));
             int w = 5, p = 16;
-            selector_choose_device(*S, T, (uint64_t)n, &w, &p, nullptr);
+            selector_choose_device(*S, dT, (uint64_t)n, &w, &p, nullptr, T);
             idx = pfp_bwt_selected(*S, dO, n, w, p);
         } else if (method == MIBWT_METHOD_PFP) {
             idx = pfp_bwt_run(*S, dT, false, dO, n, W, P);

@@ -239,9 +239,12 @@ __device__ __forceinline__ void sketch_fingerprints(const uint8_t* __restrict__ 
 // 32-bit fingerprints of the 16 and 64 bytes that start there; the distinct ones are counted
 // with two hash sets.  A block first collects its selected positions in LDS, then inserts them
 // with one lane each.  Per-block totals (selected, distinct 16, distinct 64), no global counter.
+// With `copy`, the block also writes the bytes it reads to copy[0..n) (the build's private
+// text copy is filled by the same pass over the text).
 __global__ __launch_bounds__(kBlock) void k_feat_sketch(const uint8_t* __restrict__ T, uint64_t n,
                                                        uint64_t* __restrict__ tab16, uint64_t* __restrict__ tab64,
-                                                       uint32_t mask, uint32_t epoch, uint32_t* __restrict__ part)
+                                                       uint32_t mask, uint32_t epoch, uint32_t* __restrict__ part,
+                                                       uint8_t* __restrict__ copy)
 {
     __shared__ uint32_t qn, red[3];
     __shared__ uint32_t queue[kSketchQueue];
@@ -277,6 +280,11 @@ __global__ __launch_bounds__(kBlock) void k_feat_sketch(const uint8_t* __restric
         const uint2 y = *reinterpret_cast<const uint2*>(T + p0 + kSketchPer);
         x[16] = y.x;
         x[17] = y.y;
+        if (copy) {
+            uint4* c = reinterpret_cast<uint4*>(copy + p0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) c[i] = make_uint4(x[4 * i], x[4 * i + 1], x[4 * i + 2], x[4 * i + 3]);
+        }
         uint64_t selmask = 0;
 #pragma unroll
         for (int j = 0; j < kSketchPer; j++) {
@@ -291,6 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_feat_sketch(const uint8_t* __restric
             if (p0 + j + 64 <= n) take(p0 + j);
         }
     } else if (p0 < n) {  // tail: byte loads inside the text
+        if (copy)
+            for (uint64_t q = p0; q < p0 + kSketchPer && q < n; q++) copy[q] = T[q];
         for (int j = 0; j < kSketchPer && p0 + j + 64 <= n; j++) {
             const uint64_t p = p0 + j;
             uint32_t lo = 0, hi = 0;
@@ -462,7 +472,8 @@ __global__ __launch_bounds__(kSampleBlock) void k_feat_finish(const uint32_t* __
 }
 
 // Features of all candidates into the device array X[kNumCand][kIn] (and normalised into Xn).
-void wp_features_device(Storage& S, const uint8_t* T, uint64_t n, float* X, FeatNorm norm, float* Xn)
+void wp_features_device(Storage& S, const uint8_t* T, uint64_t n, float* X, FeatNorm norm, float* Xn,
+                        uint8_t* copy = nullptr)
 {
     uint32_t* hist2 = S.buf<uint32_t>(B_SEL_HIST2, 65536);
     fill<uint32_t>(S, hist2, 65536, 0u);
@@ -498,7 +509,7 @@ void wp_features_device(Storage& S, const uint8_t* T, uint64_t n, float* X, Feat
     const uint32_t blocks = (uint32_t)ceil_div(n, kSketchBlock);
     uint32_t* kpart = S.buf<uint32_t>(B_SEL_PART, (size_t)blocks * 3);
     k_feat_sketch<<<blocks, kBlock, 0, S.stream>>>(T, n, tab, tab + slots, (uint32_t)(slots - 1), S.sketch_epoch,
-                                                   kpart);
+                                                   kpart, copy);
     MIBWT_LAUNCH_CHECK();
     double* epart = S.buf<double>(B_SEL_EPART, 2 * kPairBlocks);
     uint32_t* rows = S.buf<uint32_t>(B_SEL_ROWS, 256);
@@ -1040,7 +1051,7 @@ static ModelDev model_on_device(Storage& S, const Model& M)
     float* p = S.buf<float>(B_SEL_MP, np);
     uint16_t* pb = S.buf<uint16_t>(B_SEL_MPB, np);
     float* nrm = S.buf<float>(B_SEL_NORM, 2 * kIn + 2 * kOut);
-    if (S.sel_model_version != M.version) {
+    if (S.sel_model_version != M.version && !M.params.empty()) {
         std::vector<float> h(2 * kIn + 2 * kOut);
         std::copy(M.xmean, M.xmean + kIn, h.begin());
         std::copy(M.xstd, M.xstd + kIn, h.begin() + kIn);
@@ -1067,22 +1078,33 @@ static ModelDev model_on_device(Storage& S, const Model& M)
 // Choose (W,P) for the text resident at T (padded device copy); fixed W=5,P=16 without a model.
 // Features, normalisation, the ensemble's networks and the choice run on the device; one
 // read-back.
-void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred_out)
+// With `copy`, the text is also copied to copy[0..n) (by the sketch pass, or by a plain copy
+// when no model is loaded).
+void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred_out,
+                            uint8_t* copy)
 {
     ensure_default_model();
+    // the model's header only (its parameters are uploaded under the lock when this storage
+    // does not hold them yet): no per-call copy of the parameters
     Model M;
+    ModelDev md{};
     {
         std::lock_guard<std::mutex> lk(g_model_mu);
-        M = g_model;
+        M.loaded = g_model.loaded;
+        M.version = g_model.version;
+        M.members = g_model.members;
+        if (M.loaded) md = model_on_device(S, g_model);
     }
     *W = 5;
     *P = 16;
-    if (!M.loaded) return;
-    const ModelDev md = model_on_device(S, M);
+    if (!M.loaded) {
+        if (copy) MIBWT_HIP(hipMemcpyAsync(copy, T, n, hipMemcpyDeviceToDevice, S.stream));
+        return;
+    }
     MlpWork w = mlp_work(S, kNumCand);
     float* Xraw = S.buf<float>(B_SEL_XRAW, (size_t)kNumCand * kIn);
     float* Yall = S.buf<float>(B_SEL_YALL, (size_t)kNumCand * kOut * md.members);
-    wp_features_device(S, T, n, Xraw, FeatNorm{md.xnorm, md.xnorm + kIn}, w.X);
+    wp_features_device(S, T, n, Xraw, FeatNorm{md.xnorm, md.xnorm + kIn}, w.X, copy);
     Nets nets{};
     for (int e = 0; e < md.members; e++) nets.dev[e] = md.dev[e];
     k_mlp_forward_all<<<dim3(ceil_div((uint32_t)kNumCand, (uint32_t)kFwdRows), md.members), kBlock, 0, S.stream>>>(
