@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--W", type=int, default=5)
     ap.add_argument("--P", type=int, default=16)
     ap.add_argument("--no-verify", action="store_true")
+    ap.add_argument("--batch-workers", type=int, default=0,
+                    help="worker storages of the batched leg (0: the library default)")
     ap.add_argument("--batch-texts", type=int, default=256,
                     help="batched leg: this many independent texts of 64 KiB-16 MiB (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -252,10 +254,10 @@ def main():
     if a.batch_texts > 0:
         texts = batch_workload(a.batch_texts, gen)
         tot = sum(len(t) for t in texts)
-        res = st.batch_bwt(texts[:8], "pfp", 0, 0)  # warm the worker storages
+        res = st.batch_bwt(texts[:8], "pfp", 0, 0, workers=a.batch_workers)  # warm the worker storages
         mibwt.synchronize()
         t0 = time.perf_counter()
-        res = st.batch_bwt(texts, "pfp", 0, 0)
+        res = st.batch_bwt(texts, "pfp", 0, 0, workers=a.batch_workers)
         mibwt.synchronize()
         bms = (time.perf_counter() - t0) * 1000.0
         checked = 0

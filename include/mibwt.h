@@ -121,7 +121,7 @@ int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t*
 
 /* Batched BWT of k independent texts (host buffers, libcubwt format each; the reference has
  * no batch mode, its benchmark loops over files, Benchmark_Skript_Linux.sh:7-138).  Texts are
- * handed out longest first from one shared queue to `workers` (0 = 4, at most 16) worker
+ * handed out longest first from one shared queue to `workers` (0 = 8, at most 16) worker
  * storages, each with its own HIP stream and host thread.  method: MIBWT_METHOD_PFP (W = 0:
  * learned (W,P) per text) or MIBWT_METHOD_LYNDON.  index[i] receives the primary index of
  * text i or its error code; the return value is 0 or the first error. */

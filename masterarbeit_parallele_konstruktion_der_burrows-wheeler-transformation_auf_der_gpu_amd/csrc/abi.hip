@@ -373,7 +373,9 @@ int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* 
     for (int64_t i = 0; i < k; i++)
         if (!T[i] || !L[i] || n[i] < 1 || n[i] > S->max_length) return LIBCUBWT_BAD_PARAMETER;
     if (k == 0) return LIBCUBWT_NO_ERROR;
-    if (workers == 0) workers = 4;
+    // default 8: with 256 mixed texts of 64 KiB-16 MiB in host buffers, 4 workers reach
+    // 2.2 GB/s, 8 reach 2.55 GB/s and 16 reach 2.57 GB/s
+    if (workers == 0) workers = 8;
     // one storage (stream) per worker, created on first use and kept by the parent storage
     int64_t rc = guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
