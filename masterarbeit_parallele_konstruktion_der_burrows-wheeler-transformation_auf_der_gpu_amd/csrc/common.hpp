@@ -322,7 +322,6 @@ enum BufId : int {
     B_P_SA,
     B_G_OFF,
     B_G_MIX,
-    B_G_MIXBITS,
     B_G_ECNT,
     B_G_FILLC,
     B_MX_FLAG,

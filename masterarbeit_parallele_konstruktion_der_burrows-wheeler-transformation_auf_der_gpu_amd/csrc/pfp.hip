@@ -708,7 +708,7 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, uint64_t n, const uint3
 struct alignas(16) SortedRec {
     uint64_t r;     // record of the dictionary position placed here
     uint32_t head;  // sorted position of its group head
-    uint32_t pad;
+    uint32_t x;     // the dictionary position
 };
 
 struct PlaceRec {
@@ -718,7 +718,7 @@ struct PlaceRec {
     __device__ __forceinline__ void operator()(uint32_t*, uint32_t pos, uint32_t x, uint32_t head) const
     {
         const uint64_t r = dp[x].rec;
-        out[pos] = SortedRec{r, head, 0};
+        out[pos] = SortedRec{r, head, x};
         if (r & kRecWhole) atomicOr(wbits + (pos >> 5), 1u << (pos & 31));
     }
 };
@@ -738,10 +738,12 @@ __global__ void k_asm_mix(const SortedRec* __restrict__ RH, uint32_t L, uint8_t*
 // slots per sorted position (cnt), slots written through ordered entries (ecnt) and, for the
 // positions whose slots are filled directly (single-character groups), 0x100 | character (fc).
 // The heads of groups that receive entries are flagged (gflag, zeroed by the caller; every
-// writer stores 1): numbered densely they key the entry sort with few bits.
+// writer stores 1): numbered densely they key the entry sort with few bits.  The members of
+// mixed groups are flagged at their dictionary positions (mxf, zeroed by the caller), which
+// groups them by phrase for the entry generation.
 __global__ void k_asm_counts(const SortedRec* __restrict__ RH, const uint8_t* __restrict__ gmix, uint32_t L,
                              uint32_t* __restrict__ cnt, uint32_t* __restrict__ ecnt, uint16_t* __restrict__ fc,
-                             uint32_t* __restrict__ gflag)
+                             uint32_t* __restrict__ gflag, uint32_t* __restrict__ mxf)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
@@ -749,7 +751,9 @@ __global__ void k_asm_counts(const SortedRec* __restrict__ RH, const uint8_t* __
     const uint64_t r = a.r;
     const uint32_t f = (uint32_t)r;
     const bool alpha = (r & kRecAlpha) != 0;
-    const bool ordered = alpha && ((r & kRecWhole) || gmix[a.head]);
+    const bool mixed = alpha && !(r & kRecWhole) && gmix[a.head];
+    const bool ordered = mixed || (alpha && (r & kRecWhole));
+    if (mixed) mxf[a.x] = 1;
     cnt[g] = alpha ? f : 0u;
     ecnt[g] = ordered ? f : 0u;
     fc[g] = (alpha && !ordered) ? (uint16_t)(0x100u | ((uint32_t)(r >> 32) & 0xFFu)) : (uint16_t)0;
@@ -925,33 +929,6 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint32_t* __restrict_
             }
         }
     }
-}
-
-// mixed-group flags as a bitmap (bit g of word g / 64), small enough to stay cached for the
-// lookups in dictionary order
-__global__ void k_mix_bits(const uint8_t* __restrict__ gmix, uint32_t L, uint64_t* __restrict__ bits)
-{
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool b = g < L && gmix[g];
-    const uint64_t w = __ballot(b);
-    if ((threadIdx.x & 63) == 0 && g < L) bits[g >> 6] = w;
-}
-
-// mixed-group members in dictionary order (grouped by phrase); the group head of x is
-// rank[x] - 1
-__global__ void k_mx_flag(const DictPos* __restrict__ dp, const uint32_t* __restrict__ rank,
-                          const uint64_t* __restrict__ mixbits, uint32_t L, uint32_t* __restrict__ mxf)
-{
-    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x > L) return;
-    if (x == L) {
-        mxf[x] = 0;
-        return;
-    }
-    const uint64_t r = dp[x].rec;
-    const bool cand = (r & kRecAlpha) && !(r & kRecWhole);
-    const uint32_t head = rank[x] - 1;
-    mxf[x] = (cand && ((mixbits[head >> 6] >> (head & 63)) & 1u)) ? 1u : 0u;
 }
 
 // BWT entries: group (sorted position of the dictionary group head) above the output byte.
@@ -1519,8 +1496,11 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint16_t* fc = S.buf<uint16_t>(B_G_FILLC, L);
     uint32_t* gflag = S.buf<uint32_t>(B_E_GFLAG, L);
     uint32_t* cid = S.buf<uint32_t>(B_E_CID, L);
+    uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
+    uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
     fill<uint32_t>(S, gflag, L, 0u);
-    k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(RH, gmix, L, goff, ecnt, fc, gflag);
+    fill<uint32_t>(S, mxf, L + 1, 0u);
+    k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(RH, gmix, L, goff, ecnt, fc, gflag, mxf);
     MIBWT_LAUNCH_CHECK();
     // dense ids of the entry groups
     exclusive_sum<uint32_t>(S, gflag, cid, L, sc + SC_NGROUPS);
@@ -1548,13 +1528,6 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     MIBWT_LAUNCH_CHECK();
     uint32_t* deltac = S.buf<uint32_t>(B_E_DELTAC, std::max<uint32_t>(1, ngroups));
     k_ent_delta<<<grid_for(L), kBlock, 0, S.stream>>>(gflag, cid, goff, ecnt, L, deltac);
-    MIBWT_LAUNCH_CHECK();
-    uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
-    uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
-    uint64_t* mixbits = S.buf<uint64_t>(B_G_MIXBITS, ceil_div((uint64_t)L, (uint64_t)64));
-    k_mix_bits<<<grid_for(L), kBlock, 0, S.stream>>>(gmix, L, mixbits);
-    MIBWT_LAUNCH_CHECK();
-    k_mx_flag<<<grid_for(L + 1), kBlock, 0, S.stream>>>(dp, drank, mixbits, L, mxf);
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
     uint32_t nmx = 0;
