@@ -220,7 +220,10 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "traffic_radix_pass.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
+            tj = json.load(f)
+        # the counters were collected on one workload at W=5,P=16; other workloads report null
+        if str(tj.get("workload", "")).split(",")[0] == a.workload and (a.W, a.P) == (5, 16):
+            traffic = tj.get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "kernel": "k_radix_pass", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launches": ks["kt_launches"], "avg_launch_us": round(avg_ms * 1000, 2),
