@@ -37,6 +37,14 @@ struct PlaceSA {
 };
 constexpr uint32_t kSmallGroup = 256;
 
+// A policy may define first(SA, slot, x, group start, group size, r&): the start-of-refinement
+// work of one element (its carried value r, its placement when it is not kept; returns whether
+// it is kept), for policies whose carried value and placement come from one read.
+template <typename P, typename = void>
+struct HasFirst : std::false_type {};
+template <typename P>
+struct HasFirst<P, std::void_t<decltype(&P::first)>> : std::true_type {};
+
 // ---- initial groups: runs of equal keys in the initially sorted order
 
 static __global__ void k_dbl_headflag(const uint64_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
@@ -70,13 +78,16 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
     const uint32_t s = gofs[g], z = gofs[g + 1] - s;
     const uint32_t x = V[i];
     uint32_t r = 0;
-    if constexpr (Policy::kCarry) {
-        r = pol.rem(x);
-        NR[i] = r;
+    bool kp;
+    if constexpr (HasFirst<Policy>::value) {
+        kp = pol.first(SA, i, x, s, z, r);
+    } else {
+        if constexpr (Policy::kCarry) r = pol.rem(x);
+        kp = z > 1 && pol.open(x, r);
+        if (!kp) pol.place(SA, i, x, s);
     }
+    if constexpr (Policy::kCarry) NR[i] = r;
     rank[x] = s + 1;
-    const bool kp = z > 1 && pol.open(x, r);
-    if (!kp) pol.place(SA, i, x, s);
     keep[i] = kp ? 1u : 0u;
     NGz[i] = z;
 }

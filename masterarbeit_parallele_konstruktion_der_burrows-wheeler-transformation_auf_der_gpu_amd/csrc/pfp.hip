@@ -617,6 +617,11 @@ struct StringPolicy {
     }
 };
 
+template <typename Lim, typename Place>
+struct PolicyFor {
+    using type = StringPolicy<Lim, Place>;
+};
+
 // Prefix doubling over N suffix positions: K0/V0 hold the initial keys (bits [0, key_bits),
 // the first h0 symbols with a terminator padding) and positions; on exit rank[x] = 1 +
 // position of x's group head and SA = the sorted order (equal strings share a group).
@@ -626,10 +631,13 @@ DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t*
                                Place place = Place{})
 {
     radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, N, 0, key_bits);
-    StringPolicy<Lim, Place> pol{lim, h0, place};
-    DoublingResult r = refine_groups(
-        S, K0, V0, V1, N, pol, [](StringPolicy<Lim, Place>& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank,
-        SA);
+    using Pol = typename PolicyFor<Lim, Place>::type;
+    Pol pol{};
+    pol.lim = lim;
+    pol.h = h0;
+    pol.place_fn = place;
+    DoublingResult r =
+        refine_groups(S, K0, V0, V1, N, pol, [](Pol& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA);
     r.bytes += (double)N * 24 * ((key_bits + 7) / 8);
     return r;
 }
@@ -715,12 +723,36 @@ struct PlaceRec {
     const DictPos* dp;
     SortedRec* out;
     uint32_t* wbits;  // bit per sorted position: a whole phrase is placed there
-    __device__ __forceinline__ void operator()(uint32_t*, uint32_t pos, uint32_t x, uint32_t head) const
+    __device__ __forceinline__ void put(uint32_t pos, uint32_t x, uint32_t head, uint64_t r) const
     {
-        const uint64_t r = dp[x].rec;
         out[pos] = SortedRec{r, head, x};
         if (r & kRecWhole) atomicOr(wbits + (pos >> 5), 1u << (pos & 31));
     }
+    __device__ __forceinline__ void operator()(uint32_t*, uint32_t pos, uint32_t x, uint32_t head) const
+    {
+        put(pos, x, head, dp[x].rec);
+    }
+};
+
+// Dictionary suffixes: at the start one 16-byte read of a position's DictPos gives both its
+// rest length and, when it is placed at once, its record.
+struct DictPolicy : StringPolicy<LimDict, PlaceRec> {
+    __device__ __forceinline__ bool first(uint32_t*, uint32_t pos, uint32_t x, uint32_t s, uint32_t z,
+                                          uint32_t& r) const
+    {
+        typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+        // rec lo, rec hi, lim, pad in one streaming read (no reuse: the next read of this
+        // position comes rounds later)
+        const u32x4 d = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(lim.pos + x));
+        r = d.z - x;
+        const bool kp = z > 1 && open(x, r);
+        if (!kp) place_fn.put(pos, x, s, ((uint64_t)d.y << 32) | d.x);
+        return kp;
+    }
+};
+template <>
+struct PolicyFor<LimDict, PlaceRec> {
+    using type = DictPolicy;
 };
 
 // a group is mixed when a member's preceding character differs from its head's
