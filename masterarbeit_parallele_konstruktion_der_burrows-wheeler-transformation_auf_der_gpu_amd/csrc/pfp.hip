@@ -251,7 +251,7 @@ constexpr int kScanPer = 32;
 constexpr int kScanTileW = kBlock * kScanPer;  // window ends per tile
 constexpr int kScanHalo = 64;                   // bytes staged before the tile
 constexpr int kScanMaxW = 56;
-constexpr int kScanSub = 16;                    // sub-tiles per chained tile
+constexpr int kScanSub = 8;                     // sub-tiles per chained tile
 
 struct TrigScan {
     const uint8_t* T;
@@ -328,8 +328,9 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
 
 // One tile = kScanSub sub-tiles of kScanTileW window ends.  Each sub-tile is staged in LDS
 // (own bytes by two 16-byte loads per thread plus a 64-byte halo) and turned into one 32-bit
-// mask per thread; the masks stay in registers until the tile's offset is known.  Large tiles
-// keep the number of chained tiles (one ticket each) small.
+// mask per thread; the masks wait in LDS until the tile's offset is known.  Eight sub-tiles
+// per tile balance the number of chained tiles (one ticket each) against the LDS that limits
+// the resident blocks (measured faster than 2, 4, 16 or 32 on 256 MiB and 1 GiB texts).
 template <typename V, bool kPow2>
 __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 {
