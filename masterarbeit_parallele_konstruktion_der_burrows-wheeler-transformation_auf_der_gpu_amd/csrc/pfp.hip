@@ -330,7 +330,7 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
 // (own bytes by two 16-byte loads per thread plus a 64-byte halo) and turned into one 32-bit
 // mask per thread; the masks wait in LDS until the tile's offset is known.  Eight sub-tiles
 // per tile balance the number of chained tiles (one ticket each) against the LDS that limits
-// the resident blocks (measured faster than 2, 4, 16 or 32 on 256 MiB and 1 GiB texts).
+// the resident blocks (measured faster than 2, 4, 6, 12, 16 or 32 on 256 MiB and 1 GiB texts).
 template <typename V, bool kPow2>
 __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 {
