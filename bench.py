@@ -1,19 +1,22 @@
 """Benchmark: BWT build MB/s on one MI355X (prefix-free parsing, bit-exact).
 
 One step = one complete BWT build of the workload text, already resident in HBM, into a
-device output buffer (mibwt_pfp_bwt_device).  Default workload: the 256 MiB repetitive DNA
-collection (4 MiB uniform-ACGT base x 64 copies, 0.05% substitutions + 0.005% indels of
-length 1..50), W=5, P=16.  Prints one JSON line.
+device output buffer (mibwt_pfp_bwt_device).  Default workload: the 1 GiB repetitive DNA
+collection (8 MiB uniform-ACGT base x 128 copies, 0.02% mutations split 10:1 substitutions to
+indels of length 1..10).  Prints one JSON line.
 
-  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna256m] [--W 5 --P 16]
-                  [--wp auto|fixed] [--lyndon-steps 3] [--batch-texts 256] [--grid-steps 0]
-                  [--no-verify] [--no-cpu-baseline]
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna1g] [--W 5 --P 16]
+                  [--wp auto|fixed] [--lyndon-steps 1] [--batch-texts 256] [--grid-steps 0]
+                  [--control markov256m] [--control-steps 3] [--no-verify] [--no-cpu-baseline]
 
 With --wp auto (default) every timed step runs the learned (W,P) selector on the resident text
-and then the build with its choice; `value` is that rate.  The fixed W/P build is timed
-beside it ("fixed_wp", "speedup_vs_fixed"), the Lyndon-grammar BWT of the same text is
-timed and compared byte for byte ("lyndon"), and a batch of independent texts of uneven length
-is timed through the batch entry with host buffers ("batch").
+and then the build with its choice; `value` is that rate.  Beside it: the fixed W/P build
+("fixed_wp", "speedup_vs_fixed"), the W=35,P=81 build ("w35_p81"), the low-repetition control
+text with fixed and learned (W,P) ("control"), the Lyndon-grammar BWT of the same text, compared
+byte for byte ("lyndon"), the host-buffer entry with its PCIe copies ("pcie_inclusive"), a
+batch of independent texts of uneven length ("batch"), the peak device memory of the storage
+("device_memory_bytes"), and the multi-threaded C++ restatement of the reference pipeline on the
+host cores ("cpu_baseline").
 """
 from __future__ import annotations
 
@@ -45,36 +48,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def batch_workload(count, gen, seed=0xB17C0DE + 4):
-    """`count` texts shaped like the batched configuration: lengths log-uniform in
-    [64 KiB, 16 MiB], each a random base replicated 2..64 times with substitutions at a
-    log-uniform rate in [1e-5, 1e-2] (indels at a tenth of it, length 1..10); every other text
-    is mapped onto 256 byte values."""
-    import numpy as np
-    rng = np.random.default_rng(seed)
-    out = []
-    for i in range(count):
-        n_total = int(np.exp(rng.uniform(np.log(64 << 10), np.log(16 << 20))))
-        copies = int(rng.integers(2, 65))
-        rate = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-2))))
-        T = gen.repetitive_dna(max(1024, n_total // copies), copies, rate, rate / 10, 10, seed=int(rng.integers(1 << 31)))
-        if i % 2:
-            T = (T.astype(np.uint32) * 97 + np.arange(len(T)) % 7 * 31).astype(np.uint8)
-        out.append(T)
-    return out
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
-def cpu_baseline(T, W, P, sample_bytes):
-    """The oracle's restatement of the reference PFP pipeline (single thread) on a prefix."""
+def cpu_baseline(T, W, P, sais_sample_bytes):
+    """The multi-threaded C++ restatement of the reference PFP pipeline (oracle/cpu_pfp.cpp) on
+    the whole workload with every host core, and the single-threaded SA-IS oracle on a prefix."""
     sys.path.insert(0, ROOT)
     import oracle  # checker / CPU baseline only
-    sample = T[:sample_bytes]
+    threads = len(os.sched_getaffinity(0))
     t0 = time.perf_counter()
-    oracle.pfp_bwt(sample, W, P)
+    oracle.cpu_pfp_bwt(T, W, P, threads)
     dt = time.perf_counter() - t0
-    return {"value": round(len(sample) / 1e6 / dt, 3), "unit": "MB/s", "cores": 1, "kind": "port",
-            "sample": f"first {len(sample) >> 20} MiB of the workload, PFP W={W},P={P}, "
-                      f"single-threaded C restatement of the reference pipeline, {dt:.2f} s"}
+    sample = T[:sais_sample_bytes]
+    t1 = time.perf_counter()
+    oracle.bwt(sample)
+    ds = time.perf_counter() - t1
+    return {"value": round(len(T) / 1e6 / dt, 3), "unit": "MB/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "cpu_model": cpu_model(),
+            "sample": f"whole workload ({len(T)} B), PFP W={W},P={P}, std::thread restatement of the "
+                      f"reference pipeline on {threads} threads, {dt:.2f} s",
+            "sais_1thread": {"value": round(len(sample) / 1e6 / ds, 3), "unit": "MB/s",
+                             "sample": f"first {len(sample) >> 20} MiB, SA-IS oracle, 1 thread, {ds:.2f} s"}}
 
 
 def main():
@@ -82,19 +84,21 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="dna256m")
+    ap.add_argument("--workload", default="dna1g")
     ap.add_argument("--W", type=int, default=5)
     ap.add_argument("--P", type=int, default=16)
     ap.add_argument("--no-verify", action="store_true")
-    ap.add_argument("--batch-workers", type=int, default=0,
-                    help="worker storages of the batched leg (0: the library default)")
     ap.add_argument("--batch-texts", type=int, default=256,
                     help="batched leg: this many independent texts of 64 KiB-16 MiB (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-sample-mib", type=int, default=32)
+    ap.add_argument("--sais-sample-mib", type=int, default=32)
     ap.add_argument("--wp", choices=["auto", "fixed"], default="auto",
                     help="auto: learned (W,P) choice timed per step (value), fixed W/P reported beside it")
-    ap.add_argument("--lyndon-steps", type=int, default=3, help="timed Lyndon-grammar builds (0: skip)")
+    ap.add_argument("--w35-steps", type=int, default=5, help="timed W=35,P=81 builds (0: skip)")
+    ap.add_argument("--control", default="markov256m", help="low-repetition control text ('' to skip)")
+    ap.add_argument("--control-steps", type=int, default=3)
+    ap.add_argument("--lyndon-steps", type=int, default=1, help="timed Lyndon-grammar builds (0: skip)")
+    ap.add_argument("--pcie-steps", type=int, default=2, help="timed host-buffer builds (0: skip)")
     ap.add_argument("--grid-steps", type=int, default=0,
                     help="also time every (W,P) candidate with this many builds (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -102,8 +106,8 @@ def main():
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: this build has no multi-GPU path. Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     mibwt = load_pkg()
@@ -119,7 +123,7 @@ def main():
     mibwt.synchronize()
     st = mibwt.Storage(n)
 
-    def run(step, steps, warmup, label):
+    def run(step, steps, warmup, label, size=n):
         idx = None
         for i in range(warmup):
             idx = step()
@@ -130,7 +134,7 @@ def main():
             idx = step()
         mibwt.synchronize()
         ms = (time.perf_counter() - t0) * 1000.0 / steps
-        log(f"{label}: {ms:.3f} ms/step, {n / 1e3 / ms:.1f} MB/s")
+        log(f"{label}: {ms:.3f} ms/step, {size / 1e3 / ms:.1f} MB/s")
         return ms, idx
 
     # fixed (W, P)
@@ -162,6 +166,24 @@ def main():
         W_run, P_run = a.W, a.P
         ms_per_step = ms_fixed
     value = n / 1e6 / (ms_per_step / 1000.0)
+    O_run = dO.download() if not a.no_verify else None
+    if O_run is not None and not (O_run == O_fixed).all():
+        log("BWT differs between the fixed and the learned (W,P)")
+        sys.exit(1)
+
+    # the other fixed (W, P) of the configuration
+    w35 = None
+    if a.w35_steps > 0:
+        ms_35, idx_35 = run(lambda: st.bwt_device(dT.address, dO.address, n, "pfp", 35, 81), a.w35_steps, 1,
+                            "fixed W=35,P=81")
+        assert idx_35 == idx, "primary index differs between (W,P) choices"
+        s35 = st.stats()
+        w35 = {"value": round(n / 1e6 / (ms_35 / 1000.0), 2), "ms_per_step": round(ms_35, 4),
+               "phases": {p["name"]: round(p["ms"], 3) for p in s35["phases"]},
+               "phrases": s35["phrases"], "distinct_phrases": s35["distinct_phrases"]}
+        if O_fixed is not None and not (dO.download() == O_fixed).all():
+            log("BWT differs between W=5,P=16 and W=35,P=81")
+            sys.exit(1)
 
     # Lyndon-grammar BWT of the same text
     lyndon = None
@@ -169,18 +191,16 @@ def main():
         def step_lyndon():
             return st.bwt_device(dT.address, dO.address, n, "lyndon")
         ms_l, idx_l = run(step_lyndon, a.lyndon_steps, 1, "lyndon")
+        sl = st.stats()
         lyndon = {"value": round(n / 1e6 / (ms_l / 1000.0), 2), "ms_per_step": round(ms_l, 4),
-                  "steps": a.lyndon_steps,
-                  "phases": {p["name"]: round(p["ms"], 3) for p in st.stats()["phases"]}}
+                  "steps": a.lyndon_steps, "retries": sl["retries"],
+                  "phases": {p["name"]: round(p["ms"], 3) for p in sl["phases"]}}
         if O_fixed is not None:
             same = idx_l == idx and bool((dO.download() == O_fixed).all())
             lyndon["identical_to_pfp"] = same
             if not same:
                 log("Lyndon BWT differs from the PFP BWT")
                 sys.exit(1)
-
-    def step():
-        return st.bwt_device(dT.address, dO.address, n, "pfp", W_run, P_run)
 
     # every candidate timed alone: how far the learned choice is from the best grid point
     grid = None
@@ -209,6 +229,7 @@ def main():
     step_fixed()
     kscan = st.stats()
     st.set_kernel_timing(None)
+    device_mem = int(st.stats()["device_bytes"])
     if a.calibrate_copy:
         cms, cbytes = st.time_kernel("copy", reps=3, nbytes=float(256 << 20))
         log(f"copy kernel: {cbytes / (cms / 1000) / 1e9 * 2:.1f} GB/s (read+write) over {cbytes:.0f} B")
@@ -231,36 +252,74 @@ def main():
 
     verified = None
     if not a.no_verify:
-        step()
-        O = dO.download()
-        if not (O == O_fixed).all():
-            log("BWT differs between the fixed and the learned (W,P)")
-            sys.exit(1)
         t = time.perf_counter()
         # inverse BWT on the device (libcubwt_unbwt's kernels), compared on the host
+        st.bwt_device(dT.address, dO.address, n, "pfp", a.W, a.P)
         dU = mibwt.DeviceBuffer(n)
         st.unbwt_device(dO.address, dU.address, n, idx)
         back = dU.download()
         dU.free()
-        verified = "unbwt_roundtrip_ok_gpu"
         ok = bool((back == T).all())
-        if not ok:
-            verified = "MISMATCH"
+        verified = "unbwt_roundtrip_ok_gpu" if ok else "MISMATCH"
         log(f"verification (inverse BWT on the GPU): {verified} in {time.perf_counter() - t:.1f} s")
         if not ok:
             sys.exit(1)
+
+    # the host-buffer entry (libcubwt-style: host text in, host BWT out, learned (W,P)): the
+    # PCIe copies are inside this rate, which is therefore not the metric
+    pcie = None
+    if a.pcie_steps > 0:
+        def step_host():
+            L, i = st.bwt(T, "pfp", 0, 0)
+            return i
+        ms_h, idx_h = run(step_host, a.pcie_steps, 1, "host buffers (PCIe included)")
+        assert idx_h == idx
+        pcie = {"value": round(n / 1e6 / (ms_h / 1000.0), 2), "ms_per_step": round(ms_h, 4),
+                "entry": "mibwt_pfp_bwt (W=P=0), host text in, host BWT out"}
+    dT.free()
+    dO.free()
+
+    # low-repetition control: the same fixed and learned (W,P) on the order-2 Markov text
+    control = None
+    if a.control:
+        log(f"generating control {a.control}")
+        C = gen.workload(a.control)
+        nc = len(C)
+        dC = mibwt.DeviceBuffer.from_numpy(C)
+        dCO = mibwt.DeviceBuffer(nc)
+        stc = mibwt.Storage(nc)
+        res = {}
+        for label, (W, P) in (("fixed", (a.W, a.P)), ("fixed_w35_p81", (35, 81)), ("learned", (0, 0))):
+            def step_c(W=W, P=P):
+                return stc.bwt_device(dC.address, dCO.address, nc, "pfp", W, P)
+            step_c()
+            mibwt.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(a.control_steps):
+                step_c()
+            mibwt.synchronize()
+            ms_c = (time.perf_counter() - t0) * 1000.0 / a.control_steps
+            sc = stc.stats()
+            res[label] = {"W": int(sc["W"]), "P": int(sc["P"]), "ms_per_step": round(ms_c, 4),
+                          "value": round(nc / 1e6 / (ms_c / 1000.0), 2)}
+            log(f"control {label}: {ms_c:.3f} ms/step")
+        control = {"workload": a.control, "n_bytes": nc, **res,
+                   "speedup_learned_vs_fixed": round(res["fixed"]["ms_per_step"] / res["learned"]["ms_per_step"], 4)}
+        stc.close()
+        dC.free()
+        dCO.free()
 
     # batched leg: independent texts of uneven length (64 KiB-16 MiB, log-uniform; a random base
     # x 2..64 copies, mutation 0.001%-1%; alphabet 4 or 256), host buffers in and out,
     # (W,P) chosen per text by the selector; a sample is checked by the GPU inverse BWT
     batch = None
     if a.batch_texts > 0:
-        texts = batch_workload(a.batch_texts, gen)
+        texts = gen.batch_collection(a.batch_texts)
         tot = sum(len(t) for t in texts)
-        res = st.batch_bwt(texts[:8], "pfp", 0, 0, workers=a.batch_workers)  # warm the worker storages
+        res = st.batch_bwt(texts[:8], "pfp", 0, 0)  # warm the worker storages
         mibwt.synchronize()
         t0 = time.perf_counter()
-        res = st.batch_bwt(texts, "pfp", 0, 0, workers=a.batch_workers)
+        res = st.batch_bwt(texts, "pfp", 0, 0)
         mibwt.synchronize()
         bms = (time.perf_counter() - t0) * 1000.0
         checked = 0
@@ -277,9 +336,9 @@ def main():
 
     cpu = None
     if not a.no_cpu_baseline:
-        cpu = cpu_baseline(T, W_run, P_run, min(n, a.cpu_sample_mib << 20))
-        cpu["cores"] = 1
-        log(f"cpu baseline: {cpu['value']} MB/s")
+        cpu = cpu_baseline(T, a.W, a.P, min(n, a.sais_sample_mib << 20))
+        log(f"cpu baseline: {cpu['value']} MB/s on {cpu['cores']} threads; SA-IS 1 thread "
+            f"{cpu['sais_1thread']['value']} MB/s")
 
     phases = {p["name"]: {"ms": round(p["ms"], 3),
                           "GBps": round(p["bytes"] / max(p["ms"], 1e-9) / 1e6, 1)} for p in stats["phases"]}
@@ -303,9 +362,13 @@ def main():
                      "ms_per_step": round(ms_fixed, 4),
                      "phases": {p["name"]: round(p["ms"], 3) for p in stats_fixed["phases"]}},
         "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
+        "w35_p81": w35,
+        "control": control,
         "grid": grid,
         "lyndon": lyndon,
+        "pcie_inclusive": pcie,
         "batch": batch,
+        "device_memory_bytes": device_mem,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "verified": verified,

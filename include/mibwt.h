@@ -32,6 +32,9 @@
 #define LIBCUBWT_GPU_NOT_ENOUGH_MEMORY -9
 /* mibwt-specific: a 64-bit phrase fingerprint collision survived every re-seed. */
 #define MIBWT_HASH_COLLISION -20
+/* mibwt-specific: the text's dictionary (sum of the distinct phrase lengths for the chosen W,P)
+ * or its entry count exceeds the 32-bit index range of the device pipeline. */
+#define MIBWT_INDEX_RANGE -21
 
 /* BWT methods for mibwt_bwt / mibwt_batch_bwt. */
 #define MIBWT_METHOD_PFP 0
@@ -60,9 +63,10 @@ int64_t libcubwt_bwt(void* device_storage, const uint8_t* T, uint8_t* L, int64_t
 
 /* Sampled suffix-array inverse together with the BWT (libcubwt.cuh:79-89): L as libcubwt_bwt,
  * I[j] = the row of the rotation starting at text position j*r, in the convention of the
- * primary index (I[0] is the primary index); r is a power of two >= 2, I has (n-1)/r + 1
- * entries.  Returns LIBCUBWT_NO_ERROR or an error code.  (The reference's libcubwt.cu binds
- * it at :2433-2450.) */
+ * primary index (I[0] is the primary index); r is a power of two >= 4 (BAD_PARAMETER
+ * otherwise, as libcubwt.cu:2437), I has (n-1)/r + 1 entries.  Returns LIBCUBWT_NO_ERROR or an
+ * error code.  (The reference's libcubwt.cu binds it at :2433-2450.)  Deliberate difference:
+ * libcubwt.cu:2437 also rejects (n + r - 1) / r > 255 samples; here any count is accepted. */
 int64_t libcubwt_bwt_aux(void* device_storage, const uint8_t* T, uint8_t* L, int64_t n, int64_t r,
                          uint32_t* I);
 
@@ -112,6 +116,14 @@ int64_t mibwt_lyndon_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, i
 int64_t mibwt_pfp_triggers(void* storage, const uint8_t* T, int64_t n, int32_t W, int32_t P,
                            uint32_t* E, int64_t cap);
 
+/* Intermediate results of the prefix-free parse of host T[0..n) for (W, P): rank[k] = dense
+ * lexicographic rank of the phrase of occurrence k (the reference's phrase order,
+ * Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:1990-2164 and :616-691) and psa = the suffix array
+ * of the parse (:2342-2412), first min(d, cap) entries each; *distinct = number of distinct
+ * phrases.  Returns the parse length d or an error code. */
+int64_t mibwt_pfp_parse(void* storage, const uint8_t* T, int64_t n, int32_t W, int32_t P, uint32_t* rank,
+                        uint32_t* psa, int64_t cap, int64_t* distinct);
+
 /* Lyndon array of host T[0..n): lam[i] = length of the longest Lyndon word starting at i,
  * computed by the block-local passes and the merge tree of the Lyndon pipeline (the forest
  * the reference builds in Compute_Lyndon_Grammar_Local1/2, Lyndon_Grammar_BWT.cu:647-1868,
@@ -143,6 +155,9 @@ struct mibwt_phase_stats {
     int64_t kt_launches;
     double kt_ms_total;                  /* sum of the launches' HIP-event durations */
     double kt_bytes_total;               /* sum of their algorithmic HBM bytes */
+    int64_t retries;                     /* Lyndon: builds repeated after a fingerprint collision */
+    int64_t device_bytes;                /* device memory the storage holds after the call (its peak:
+                                            buffers only grow) */
 };
 int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out);
 
@@ -216,6 +231,11 @@ int64_t mibwt_device_alloc(void** ptr, int64_t bytes);
 int64_t mibwt_device_free(void* ptr);
 int64_t mibwt_copy(void* dst, const void* src, int64_t bytes, int32_t kind);
 int64_t mibwt_synchronize(void);
+/* A non-blocking HIP stream of the library's runtime (for the `stream` arguments above), its
+ * destruction (after it drained), and an asynchronous copy on it. */
+int64_t mibwt_stream_create(void** stream);
+int64_t mibwt_stream_destroy(void* stream);
+int64_t mibwt_copy_async(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream);
 
 #ifdef __cplusplus
 }

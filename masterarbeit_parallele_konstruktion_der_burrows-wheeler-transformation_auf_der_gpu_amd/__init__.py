@@ -29,6 +29,7 @@ GPU_ERROR = -7
 GPU_NOT_SUPPORTED = -8
 GPU_NOT_ENOUGH_MEMORY = -9
 HASH_COLLISION = -20
+INDEX_RANGE = -21
 
 METHOD_PFP = 0
 METHOD_LYNDON = 1
@@ -45,6 +46,7 @@ EXPORTED = [
     "mibwt_lyndon_bwt",
     "mibwt_lyndon_bwt_device",
     "mibwt_lyndon_array",
+    "mibwt_pfp_parse",
     "mibwt_pfp_triggers",
     "mibwt_last_phase_stats",
     "mibwt_set_kernel_timing",
@@ -63,6 +65,9 @@ EXPORTED = [
     "mibwt_device_free",
     "mibwt_copy",
     "mibwt_synchronize",
+    "mibwt_stream_create",
+    "mibwt_stream_destroy",
+    "mibwt_copy_async",
 ]
 
 _lib = None
@@ -91,6 +96,8 @@ class PhaseStats(ctypes.Structure):
         ("total_ms", ctypes.c_double),
         ("kt_launches", ctypes.c_int64), ("kt_ms_total", ctypes.c_double),
         ("kt_bytes_total", ctypes.c_double),
+        ("retries", ctypes.c_int64),
+        ("device_bytes", ctypes.c_int64),
     ]
 
 
@@ -114,6 +121,8 @@ def lib():
         L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
         L.mibwt_lyndon_bwt_device.argtypes = [vp, vp, vp, i64, vp]
         L.mibwt_lyndon_array.argtypes = [vp, u8p, i64, ctypes.POINTER(ctypes.c_uint32)]
+        L.mibwt_pfp_parse.argtypes = [vp, u8p, i64, i32, i32, ctypes.POINTER(ctypes.c_uint32),
+                                      ctypes.POINTER(ctypes.c_uint32), i64, ctypes.POINTER(ctypes.c_int64)]
         L.mibwt_pfp_triggers.argtypes = [vp, u8p, i64, i32, i32, ctypes.POINTER(ctypes.c_uint32), i64]
         L.mibwt_last_phase_stats.argtypes = [vp, ctypes.POINTER(PhaseStats)]
         L.mibwt_set_kernel_timing.argtypes = [vp, ctypes.c_char_p]
@@ -139,6 +148,9 @@ def lib():
         L.mibwt_device_free.argtypes = [vp]
         L.mibwt_copy.argtypes = [vp, vp, i64, i32]
         L.mibwt_synchronize.argtypes = []
+        L.mibwt_stream_create.argtypes = [ctypes.POINTER(vp)]
+        L.mibwt_stream_destroy.argtypes = [vp]
+        L.mibwt_copy_async.argtypes = [vp, vp, i64, i32, vp]
         for name in EXPORTED:
             getattr(L, name).restype = i64
         _lib = L
@@ -287,7 +299,7 @@ class Storage:
         """(L, I): the libcubwt BWT and the sampled rows I[j] of text positions j*r."""
         T = _as_text(T)
         L = np.empty_like(T)
-        I = np.zeros((len(T) - 1) // r + 1, dtype=np.uint32)
+        I = np.zeros((len(T) - 1) // r + 1 if r > 0 else 1, dtype=np.uint32)  # r < 4 is rejected
         rc = lib().libcubwt_bwt_aux(self.handle, _u8(T), _u8(L), len(T), int(r),
                                     I.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         if rc != NO_ERROR:
@@ -310,6 +322,20 @@ class Storage:
         if rc < 0:
             raise BwtError(rc, "mibwt_pfp_triggers")
         return E[:rc].copy()
+
+    def pfp_parse(self, T, W: int = 5, P: int = 16):
+        """(lexicographic phrase rank per occurrence, parse suffix array, #distinct phrases)."""
+        T = _as_text(T)
+        cap = len(T) + 1
+        rank = np.empty(cap, dtype=np.uint32)
+        psa = np.empty(cap, dtype=np.uint32)
+        du = ctypes.c_int64()
+        d = lib().mibwt_pfp_parse(self.handle, _u8(T), len(T), W, P,
+                                  rank.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+                                  psa.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), cap, ctypes.byref(du))
+        if d < 0:
+            raise BwtError(d, "mibwt_pfp_parse")
+        return rank[:d].copy(), psa[:d].copy(), int(du.value)
 
     def lyndon_array(self, T) -> np.ndarray:
         T = _as_text(T)
@@ -457,6 +483,31 @@ class DeviceBuffer:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream of the library's runtime (for the ``stream`` arguments)."""
+
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        rc = lib().mibwt_stream_create(ctypes.byref(self.handle))
+        if rc != NO_ERROR:
+            raise BwtError(rc, "mibwt_stream_create")
+
+    @property
+    def address(self) -> int:
+        return int(self.handle.value)
+
+    def close(self):
+        if self.handle:
+            lib().mibwt_stream_destroy(self.handle)
+            self.handle = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass
 

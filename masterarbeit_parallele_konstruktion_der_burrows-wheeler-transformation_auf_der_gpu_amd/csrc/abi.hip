@@ -8,14 +8,15 @@
 #include <thread>
 
 namespace mibwt {
-int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P);
+int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P,
+                    PfpParseOut* parse_out = nullptr);
 double pfp_time_scan(Storage& S, int reps, double* bytes);
 int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n, int W, int P, uint32_t* E, int64_t cap);
 double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
 int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n, int64_t index, bool ref_format,
-                  uint32_t* dI, int64_t rate);
+                  uint32_t* dI, int64_t rate, bool own_phases = true);
 void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred,
                             uint8_t* copy = nullptr);
 bool selector_load(const char* path);
@@ -35,17 +36,34 @@ namespace {
 constexpr int64_t kMaxLength = (int64_t)INT32_MAX - (1 << 20);
 
 // Build of the staged text with the selector's (W,P); a choice whose dictionary outgrows the
-// 32-bit index range falls back to the reference default W=5,P=16.
+// 32-bit index range or the device memory falls back to the reference default W=5,P=16.
 int64_t pfp_bwt_selected(Storage& S, uint8_t* dO, int64_t n, int W, int P)
 {
     try {
         return pfp_bwt_run(S, nullptr, false, dO, n, W, P);
     } catch (const Error& e) {
-        if (e.code != LIBCUBWT_GPU_NOT_ENOUGH_MEMORY || (W == 5 && P == 16)) throw;
+        if ((e.code != MIBWT_INDEX_RANGE && e.code != LIBCUBWT_GPU_NOT_ENOUGH_MEMORY) || (W == 5 && P == 16)) throw;
         (void)hipGetLastError();
         return pfp_bwt_run(S, nullptr, false, dO, n, 5, 16);
     }
 }
+
+// Orders the storage stream after the work already enqueued on a caller's stream (nothing for
+// NULL).  The event is destroyed as soon as the wait is enqueued, also when a later step of
+// the entry point throws.
+struct StreamOrder {
+    explicit StreamOrder(Storage& S, void* stream)
+    {
+        hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
+        if (!caller) return;
+        hipEvent_t ev = nullptr;
+        MIBWT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+        const hipError_t e1 = hipEventRecord(ev, caller);
+        const hipError_t e2 = (e1 == hipSuccess) ? hipStreamWaitEvent(S.stream, ev, 0) : e1;
+        (void)hipEventDestroy(ev);
+        MIBWT_HIP(e2);
+    }
+};
 
 template <typename F>
 int64_t guarded(F&& f)
@@ -184,7 +202,9 @@ int64_t libcubwt_bwt_aux(void* device_storage, const uint8_t* T, uint8_t* L, int
 {
     Storage* S = reinterpret_cast<Storage*>(device_storage);
     if (!S || !T || !L || !I || n < 16 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
-    if (r < 2 || (r & (r - 1)) != 0) return LIBCUBWT_BAD_PARAMETER;
+    // libcubwt.cu:2437: r is a power of two >= 4.  Its further limit (n + r - 1) / r <= 255 is
+    // not applied: any number of samples fits the uint32 rows of I here.
+    if (r < 4 || (r & (r - 1)) != 0) return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         uint8_t* dO = S->buf<uint8_t>(B_STAGE, (size_t)n);
@@ -228,14 +248,7 @@ int64_t mibwt_unbwt_device(void* storage, const uint8_t* dO, uint8_t* dU, int64_
     if (!S || !dO || !dU || n < 1 || n > S->max_length || index < 1 || index > n) return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
-        hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
-        if (caller) {
-            hipEvent_t ev;
-            MIBWT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            MIBWT_HIP(hipEventRecord(ev, caller));
-            MIBWT_HIP(hipStreamWaitEvent(S->stream, ev, 0));
-            MIBWT_HIP(hipEventDestroy(ev));
-        }
+        StreamOrder order(*S, stream);
         const int64_t rc = unbwt_run(*S, dO, dU, n, index, true, nullptr, 0);
         MIBWT_HIP(hipStreamSynchronize(S->stream));
         return rc;
@@ -262,14 +275,7 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
         return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
-        hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
-        hipEvent_t ev = nullptr;
-        if (caller) {
-            // order the storage stream after the caller's pending work
-            MIBWT_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
-            MIBWT_HIP(hipEventRecord(ev, caller));
-            MIBWT_HIP(hipStreamWaitEvent(S->stream, ev, 0));
-        }
+        StreamOrder order(*S, stream);  // after the caller's pending work
         int64_t idx;
         if (learned) {
             // the selector's pass over the caller's text also fills the private padded copy
@@ -285,7 +291,6 @@ static int64_t device_bwt(void* storage, const uint8_t* dT, uint8_t* dO, int64_t
         } else {
             idx = lyndon_bwt_run(*S, dT, false, dO, n);
         }
-        if (ev) MIBWT_HIP(hipEventDestroy(ev));
         return idx;
     });
 }
@@ -309,6 +314,22 @@ int64_t mibwt_pfp_triggers(void* storage, const uint8_t* T, int64_t n, int32_t W
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         return pfp_triggers_run(*S, T, n, W, P, E, cap);
+    });
+}
+
+int64_t mibwt_pfp_parse(void* storage, const uint8_t* T, int64_t n, int32_t W, int32_t P, uint32_t* rank,
+                        uint32_t* psa, int64_t cap, int64_t* distinct)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !T || ((!rank || !psa) && cap > 0) || cap < 0 || n < 1 || n > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    if (W < 1 || W > 64 || P < 1 || P > 65535) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        PfpParseOut out{rank, psa, cap, 0};
+        uint8_t* dO = S->buf<uint8_t>(B_STAGE, (size_t)n);
+        const int64_t d = pfp_bwt_run(*S, T, true, dO, n, W, P, &out);
+        if (distinct) *distinct = out.distinct;
+        return d;
     });
 }
 
@@ -569,6 +590,40 @@ int64_t mibwt_copy(void* dst, const void* src, int64_t bytes, int32_t kind)
                                                                         : hipMemcpyDeviceToDevice;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipMemcpy(dst, src, (size_t)bytes, k));
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_stream_create(void** stream)
+{
+    if (!stream) return LIBCUBWT_BAD_PARAMETER;
+    *stream = nullptr;
+    return guarded([&]() -> int64_t {
+        hipStream_t s = nullptr;
+        MIBWT_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        *stream = s;
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_stream_destroy(void* stream)
+{
+    if (!stream) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipStreamSynchronize(reinterpret_cast<hipStream_t>(stream)));
+        MIBWT_HIP(hipStreamDestroy(reinterpret_cast<hipStream_t>(stream)));
+        return LIBCUBWT_NO_ERROR;
+    });
+}
+
+int64_t mibwt_copy_async(void* dst, const void* src, int64_t bytes, int32_t kind, void* stream)
+{
+    if (!dst || !src || bytes < 0 || kind < 0 || kind > 2) return LIBCUBWT_BAD_PARAMETER;
+    if (bytes == 0) return LIBCUBWT_NO_ERROR;
+    const hipMemcpyKind k = kind == 0 ? hipMemcpyHostToDevice : kind == 1 ? hipMemcpyDeviceToHost
+                                                                        : hipMemcpyDeviceToDevice;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipMemcpyAsync(dst, src, (size_t)bytes, k, reinterpret_cast<hipStream_t>(stream)));
         return LIBCUBWT_NO_ERROR;
     });
 }

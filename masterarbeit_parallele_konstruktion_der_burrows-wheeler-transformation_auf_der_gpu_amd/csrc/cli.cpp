@@ -11,11 +11,13 @@
 //                symbols, libcubwt.cu:3003-3015).
 // --raw reads the input as raw bytes and writes the BWT as raw bytes (reference order).
 #include <cerrno>
+#include <climits>
 #include <cstdint>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <new>
 #include <vector>
 
 #include "mibwt.h"
@@ -49,12 +51,17 @@ bool parse_reference_text(const std::vector<char>& buf, std::vector<uint8_t>& T)
         if (*p == '-') { neg = true; p++; }
         if (p >= e || *p < '0' || *p > '9') return false;
         long long x = 0;
-        while (p < e && *p >= '0' && *p <= '9') x = x * 10 + (*p++ - '0');
+        while (p < e && *p >= '0' && *p <= '9') {
+            if (x > (LLONG_MAX - 9) / 10) return false;  // beyond the 64-bit range: malformed
+            x = x * 10 + (*p++ - '0');
+        }
         v = neg ? -x : x;
         return true;
     };
     long long n;
     if (!next(n) || n < 0) return false;
+    // every symbol takes at least two characters ("v "): a larger count cannot be in the file
+    if ((unsigned long long)n > buf.size() / 2) return false;
     T.resize((size_t)n);
     for (long long i = 0; i < n; i++) {
         long long v;
@@ -93,7 +100,7 @@ int usage(const char* prog)
 
 }  // namespace
 
-int main(int argc, char** argv)
+int run(int argc, char** argv)
 {
     int W = 5, P = 16;
     const char* in = "input.txt";
@@ -169,4 +176,14 @@ int main(int argc, char** argv)
         return ok ? 0 : 1;
     }
     return write_reference_text(outp, O.data(), count, n) ? 0 : 1;
+}
+
+int main(int argc, char** argv)
+{
+    try {
+        return run(argc, argv);
+    } catch (const std::bad_alloc&) {
+        fprintf(stderr, "out of host memory (input too large or malformed)\n");
+        return 1;
+    }
 }

@@ -121,6 +121,8 @@ struct Storage {
     uint32_t chain_epoch = 0;
     // selector sketch tables: epoch of the last use
     uint32_t sketch_epoch = 0;
+    // Lyndon builds on this storage (seeds the per-call fingerprint base)
+    uint64_t lyn_calls = 0;
     uint64_t sel_model_version = 0;  // selector model resident in B_SEL_MP / B_SEL_MPB / B_SEL_NORM
 
     // device buffer #id with capacity >= bytes; contents are not preserved on growth
@@ -270,6 +272,8 @@ struct Storage {
             stats.ms[i] = ms;
             stats.bytes[i] = phases[i].bytes;
         }
+        stats.device_bytes = 0;
+        for (const DevBuf& b : bufs) stats.device_bytes += (int64_t)b.cap;
         stats.kt_launches = kt_used;
         stats.kt_ms_total = 0;
         stats.kt_bytes_total = 0;
@@ -280,6 +284,15 @@ struct Storage {
             stats.kt_bytes_total += kt_bytes[i];
         }
     }
+};
+
+// Host destination of the intermediate PFP results (phrase ranks per occurrence, parse
+// suffix array), first `cap` entries each.
+struct PfpParseOut {
+    uint32_t* rank;
+    uint32_t* psa;
+    int64_t cap;
+    int64_t distinct;
 };
 
 // Buffer ids (one grow-only allocation each).  Buffers that are live at the same time have
@@ -386,6 +399,7 @@ enum BufId : int {
     B_LY_LABH,
     B_LY_GL,
     B_LY_GF,
+    B_LY_SETTLED,
     B_AS_R,
     B_AS_WBITS,
     B_AS_WSCAN,

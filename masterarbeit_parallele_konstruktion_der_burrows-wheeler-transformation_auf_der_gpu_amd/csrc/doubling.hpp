@@ -496,9 +496,11 @@ struct DoublingResult {
 // writes ranks, SA slots and the sub-group records, a radix sort for large groups, then one
 // chained scan of the kept flags (which also counts next round's large-group members) and the
 // compaction; one host read-back per round (active and large counts).
+// `max_rounds` bounds the loop: a caller whose keys can be inconsistent (elements that never
+// separate) gets MIBWT_HASH_COLLISION instead of an endless refinement.
 template <typename Policy, typename Advance>
 DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V1, uint32_t N, Policy& pol,
-                             Advance advance, uint32_t* rank, uint32_t* SA)
+                             Advance advance, uint32_t* rank, uint32_t* SA, int max_rounds = 1 << 30)
 {
     DoublingResult res;
     uint32_t* V = S.buf<uint32_t>(B_ACT_X2, N);
@@ -582,6 +584,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         S.wait_signal(seq, hv, 2);
         const uint32_t m2 = hv[0], nl = hv[1];
         res.rounds++;
+        MIBWT_REQUIRE(m2 == 0 || res.rounds < max_rounds, MIBWT_HASH_COLLISION, "groups do not separate");
         res.bytes += (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4);
         if (m2 == 0) break;
         m = m2;

@@ -25,10 +25,15 @@
 //              the next-smaller-suffix chain with pointer jumping (segmented group refinement
 //              shared with the prefix-free-parsing pipeline).
 //  5. output   O[j] = T[SA[j]-1] with the wrap to T[n-1].
+#include <chrono>
+#include <cstdlib>
+
 #include "doubling.hpp"
 #include "text.hpp"
 
 namespace mibwt {
+
+bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT);
 
 constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass
 constexpr uint64_t kM61 = (1ull << 61) - 1;
@@ -98,16 +103,17 @@ __global__ void k_hash_items(const uint8_t* __restrict__ T, uint64_t n, uint64_t
 
 struct PrefixHash {
     const HashPair* pre;  // pre[k] = (H[k+1], base^(k+1))
+    uint64_t mask;        // bits of the fingerprints compared (all 61 but in collision tests)
     // fingerprint of T[a..a+len)
     __device__ __forceinline__ uint64_t sub(uint32_t a, uint32_t len) const
     {
         if (len == 0) return 0;
         const uint64_t hb = pre[a + len - 1].h;
-        if (a == 0) return hb;
+        if (a == 0) return hb & mask;
         const uint64_t ha = pre[a - 1].h;
         const uint64_t pw = pre[len - 1].p;
         const uint64_t x = mulmod61(ha, pw);
-        return hb >= x ? hb - x : hb + kM61 - x;
+        return (hb >= x ? hb - x : hb + kM61 - x) & mask;
     }
 };
 
@@ -123,7 +129,11 @@ struct SuffixCmp {
     PrefixHash ph;
     __device__ uint32_t lcp(uint32_t a, uint32_t b) const
     {
-        const uint32_t m = (uint32_t)(n - (a > b ? a : b));  // comparable length
+        return lcp_upto(a, b, (uint32_t)(n - (a > b ? a : b)));  // the comparable length
+    }
+    // longest common prefix of the suffixes at a and b, at most m (m <= n - max(a, b))
+    __device__ uint32_t lcp_upto(uint32_t a, uint32_t b, uint32_t m) const
+    {
         uint32_t k = 0;
         while (k < 32 && k + 8 <= m) {
             const uint64_t x = load8(T, a + k) ^ load8(T, b + k);
@@ -343,6 +353,22 @@ __global__ void k_label_hash(PrefixHash ph, const uint32_t* __restrict__ lam, ui
     if (i < n) F[i] = ph.sub((uint32_t)i, lam[i]);
 }
 
+// Common prefix of T[x..x+m) and T[y..y+m) by fingerprints, knowing that the first `known`
+// symbols agree: the whole range first, then bisection over the candidate lengths.
+__device__ __forceinline__ uint32_t label_lcp(const PrefixHash& ph, uint32_t x, uint32_t y, uint32_t known,
+                                              uint32_t m)
+{
+    uint32_t lo = known < m ? known : m;
+    if (ph.sub(x, m) == ph.sub(y, m)) return m;
+    uint32_t hi = m - 1;  // largest candidate LCP
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo + 1) / 2;
+        if (ph.sub(x, mid) == ph.sub(y, mid)) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 struct LabelCmp {
     const uint8_t* T;
     const uint32_t* lam;
@@ -354,18 +380,8 @@ struct LabelCmp {
         if (x == y) return 0;
         const uint32_t lx = lam[x], ly = lam[y];
         const uint32_t m = lx < ly ? lx : ly;
-        uint32_t lo = known < m ? known : m;
         if (lx == ly && F[x] == F[y]) return 0;
-        if (ph.sub(x, m) == ph.sub(y, m)) {
-            lo = m;
-        } else {
-            uint32_t hi = m - 1;  // largest candidate LCP
-            while (lo < hi) {
-                const uint32_t mid = lo + (hi - lo + 1) / 2;
-                if (ph.sub(x, mid) == ph.sub(y, mid)) lo = mid;
-                else hi = mid - 1;
-            }
-        }
+        const uint32_t lo = label_lcp(ph, x, y, known, m);
         if (lo < m) return T[x + lo] < T[y + lo] ? -1 : 1;
         return lx < ly ? -1 : (lx > ly ? 1 : 0);
     }
@@ -383,14 +399,15 @@ __global__ void k_lab_heads_from_keys(const uint64_t* __restrict__ K, uint64_t n
 // symbols), and is that group small or large?  cls: 0 = settled, 1 = small tie, 2 = large tie.
 __global__ void k_lab_classify(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                                const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ lam, uint64_t n,
-                               uint32_t known, uint8_t* __restrict__ cls, uint32_t* __restrict__ large)
+                               uint32_t known, const uint8_t* __restrict__ settled, uint8_t* __restrict__ cls,
+                               uint32_t* __restrict__ large)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t s = gofs[g], e = gofs[g + 1];
     uint8_t c = 0;
-    if (e - s >= 2 && lam[V[s]] >= known) c = (e - s > kSmallGroup) ? 2 : 1;
+    if (e - s >= 2 && lam[V[s]] >= known && !settled[s]) c = (e - s > kSmallGroup) ? 2 : 1;
     cls[i] = c;
     large[i] = (c == 2) ? 1u : 0u;
 }
@@ -439,53 +456,98 @@ __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __re
     hf[s + before] = (equal_before == 0) ? 1u : 0u;
 }
 
-// Large tie groups: key = (group, next `per` symbols after `known`).
-__global__ void k_lab_chunk_gather(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ V,
-                                   const uint32_t* __restrict__ gid1, const uint32_t* __restrict__ large,
-                                   const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ lam,
-                                   const uint32_t* __restrict__ code, int bits, int per, uint32_t known, int cb,
-                                   uint64_t* __restrict__ LK, uint32_t* __restrict__ LV, uint32_t* __restrict__ LI)
+// Large tie groups are split around a pivot, their first member p: a member x is compared
+// with p through the fingerprint LCP l of the two labels (both agree on the first `known`
+// symbols) and falls into one of three classes, LESS, EQUAL (the same label) or GREATER.
+// Ordered inside its group, LESS members rise with l (they agree longer with p) and GREATER
+// members fall with l; members with the same class, l and next symbol tie and form the
+// sub-groups of the next round, EQUAL members form a settled group.  Key (low 11 + lbits
+// bits): class (2) | l - known, clamped to lbits bits | next symbol + 1 (9, 0 = the label ends
+// at l); the dense id of the large group goes above it (k_lab_pivot_ids).  Every round removes
+// at least the pivot's class from each group, so a group of g labels needs at most g rounds,
+// and a run such as a^k b (labels a^j b that all share long prefixes) is sorted in one.
+
+__global__ void k_lab_pivot_gather(PrefixHash ph, const uint8_t* __restrict__ T, uint64_t n,
+                                   const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
+                                   const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ large,
+                                   const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ lam, uint32_t known,
+                                   int lbits, uint64_t* __restrict__ LK, uint32_t* __restrict__ LV, uint32_t* __restrict__ LI,
+                                   uint32_t* __restrict__ LG)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !large[i]) return;
-    const uint32_t x = V[i];
-    const uint32_t l = lam[x];
-    uint64_t key = 0;
-    for (int k = 0; k < per; k++) {
-        const uint64_t q = (uint64_t)known + k;
-        const uint64_t s = (q < l) ? code[T[x + q]] : 0ull;
-        key = (key << bits) | s;
+    const uint32_t g = gid1[i] - 1;
+    const uint32_t x = V[i], p = V[gofs[g]];
+    const uint32_t lx = lam[x], lp = lam[p];
+    const uint32_t m = lx < lp ? lx : lp;
+    const uint32_t l = (x == p) ? m : label_lcp(ph, x, p, known, m);
+    uint64_t cls, lpart = 0, ch = 0;
+    if (l == m && lx == lp) {
+        cls = 1;  // EQUAL
+    } else {
+        const uint32_t cx = (l < lx) ? (uint32_t)T[x + l] + 1u : 0u;  // 0: x ends at l
+        const uint32_t cp = (l < lp) ? (uint32_t)T[p + l] + 1u : 0u;
+        const uint32_t lmax = (uint32_t)((1ull << lbits) - 1);
+        const uint32_t d = (l > known) ? l - known : 0u;
+        const uint32_t lc = d < lmax ? d : lmax;
+        if (cx < cp) {
+            cls = 0;  // LESS: rises with l
+            lpart = lc;
+        } else {
+            cls = 2;  // GREATER: falls with l
+            lpart = lmax - lc;
+        }
+        ch = cx;
     }
-    const uint32_t p = lpos[i];
-    LK[p] = ((uint64_t)gid1[i] << cb) | key;
-    LV[p] = x;
-    LI[p] = (uint32_t)i;
+    const uint32_t q = lpos[i];
+    LK[q] = (cls << (9 + lbits)) | (lpart << 9) | ch;
+    LV[q] = x;
+    LI[q] = (uint32_t)i;
+    LG[q] = g;
 }
 
-__global__ void k_lab_chunk_scatter(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
-                                    const uint32_t* __restrict__ LI, uint32_t cnt, uint32_t* __restrict__ V2,
-                                    uint32_t* __restrict__ hf)
+// dense id of every large group, above the pivot key: flag the first member of each group
+__global__ void k_lab_pivot_heads(const uint32_t* __restrict__ LG, uint32_t nl, uint32_t* __restrict__ hq)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nl) hq[q] = (q == 0 || LG[q] != LG[q - 1]) ? 1u : 0u;
+}
+
+__global__ void k_lab_pivot_ids(const uint32_t* __restrict__ id1, uint32_t nl, int low_bits, uint64_t* __restrict__ LK)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q < nl) LK[q] |= (uint64_t)(id1[q] - 1) << low_bits;
+}
+
+// Sorted large members back to the group slots (a group occupies the same slot range before and
+// after), heads where the key changes, EQUAL sub-groups marked settled.
+__global__ void k_lab_pivot_scatter(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
+                                    const uint32_t* __restrict__ LI, uint32_t cnt, int lbits, uint32_t* __restrict__ V2,
+                                    uint32_t* __restrict__ hf, uint8_t* __restrict__ settled)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= cnt) return;
     const uint32_t i = LI[j];
     V2[i] = LV[j];
     hf[i] = (j == 0 || LK[j] != LK[j - 1]) ? 1u : 0u;
+    if (((LK[j] >> (9 + lbits)) & 3u) == 1u) settled[i] = 1;
 }
 
-// exact check of the final label order: neighbours must be non-decreasing, and equal exactly
-// when they share a label group
-__global__ void k_lab_verify(const uint8_t* __restrict__ T, const uint32_t* __restrict__ V,
-                             const uint32_t* __restrict__ hf, const uint32_t* __restrict__ lam, uint64_t n,
-                             uint32_t* __restrict__ bad)
+// check of the final label order: neighbours must be non-decreasing, and equal exactly when
+// they share a label group.  The common prefix of two labels is found through the fingerprints
+// (O(log lcp), so long runs of equal labels cost no more than short ones); a fingerprint
+// collision that slips through is caught by the exact inversion check of the finished BWT.
+__global__ void k_lab_verify(SuffixCmp sc, const uint32_t* __restrict__ V, const uint32_t* __restrict__ hf,
+                             const uint32_t* __restrict__ lam, uint32_t* __restrict__ bad)
 {
+    const uint64_t n = sc.n;
+    const uint8_t* T = sc.T;
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0 || i >= n) return;
     const uint32_t a = V[i - 1], b = V[i];
     const uint32_t la = lam[a], lb = lam[b];
     const uint32_t m = la < lb ? la : lb;
-    const uint32_t lo = a < b ? a : b, hi = a < b ? b : a;
-    const uint32_t k = lcp_text(T, n, lo, hi, m);
+    const uint32_t k = (a == b) ? m : sc.lcp_upto(a, b, m);
     int c;
     if (k < m) c = T[a + k] < T[b + k] ? -1 : 1;
     else c = la < lb ? -1 : (la > lb ? 1 : 0);
@@ -546,47 +608,44 @@ __global__ void k_bwt_from_sa(const uint32_t* __restrict__ SA, const uint8_t* __
     const uint64_t g = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const uint32_t p = SA[g];
+    if (p >= n) return;  // never for a permutation; an unwritten byte fails the inversion check
     O[g] = T[p == 0 ? n - 1 : p - 1];
     if (p == 0) *pidx = (uint32_t)g + 1;
 }
 
 // ------------------------------------------------------------------- driver
 
-int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64)
+// Fingerprint base in [2^32, 2^61 - 1) drawn from a seed.
+static uint64_t lyndon_base(uint64_t seed)
 {
-    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
-    const uint64_t n = (uint64_t)n64;
-    S.phases_reset();
-    S.stats.n = n64;
-    if (!S.t_begin) {
-        MIBWT_HIP(hipEventCreate(&S.t_begin));
-        MIBWT_HIP(hipEventCreate(&S.t_end));
-    }
-    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    uint8_t* T;
-    if (src_host) {
-        T = stage_text(S, src, true, n);
-        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
-    } else {
-        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
-        T = stage_text(S, src, false, n);
-    }
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (1ull << 32) + z % (kM61 - (1ull << 32));
+}
 
+// One build attempt with fingerprint base `base` (fingerprints compared under `hmask`): the
+// phases rules .. output.  Returns the primary index, or 0 when the label order check found a
+// fingerprint collision.
+static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_t base, uint64_t hmask,
+                               uint32_t* codes, uint32_t* sigma_io, uint8_t* dO)
+{
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     // ---- 1. prefix fingerprints
     int ph = S.phase_begin("rules");
-    uint64_t base = 0x1F3D5B79A2C4E681ull % kM61;
     HashPair* pre = prefix_hashes(S, T, n, base);
-    bool pre_fresh = true;
+    const PrefixHash phh{pre, hmask};
     S.phase_end(ph, (double)n * (1 + 16 * 4));
 
     // ---- 2. Lyndon array
     ph = S.phase_begin("lyndon");
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
-    const int levels = lyndon_array(S, T, n, PrefixHash{pre}, lam);
-    uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
+    const int levels = lyndon_array(S, T, n, phh, lam);
     alphabet_codes(S, T, n, codes);
     uint32_t sigma = 0;
     S.read(&sigma, codes + 256);
+    *sigma_io = sigma;
     S.phase_end(ph, (double)n * (1 + 4 + 4 + 1) * 2);
     S.stats.doubling_rounds_dict = levels;
 
@@ -611,27 +670,25 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
     k_lab_heads_from_keys<<<grid_for(n), kBlock, 0, S.stream>>>(K0, n, hf);
     MIBWT_LAUNCH_CHECK();
     double rbytes = (double)n * 24 * ((bits * per + 7) / 8);
-    for (int attempt = 0;; attempt++) {
-        MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "label fingerprint collisions persist");
-        if (!pre_fresh) pre = prefix_hashes(S, T, n, base);
-        pre_fresh = false;
+    {
         uint64_t* LF = S.buf<uint64_t>(B_LY_LABH, n);
-        k_label_hash<<<grid_for(n), kBlock, 0, S.stream>>>(PrefixHash{pre}, lam, n, LF);
+        k_label_hash<<<grid_for(n), kBlock, 0, S.stream>>>(phh, lam, n, LF);
         MIBWT_LAUNCH_CHECK();
-        LabelCmp cmp{T, lam, PrefixHash{pre}, LF};
+        LabelCmp cmp{T, lam, phh, LF};
         // working copies: V (order) and hf (heads) of the packed-key order
         uint32_t* V = Vl;
         MIBWT_HIP(hipMemcpyAsync(V, V0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
         uint32_t* H = S.buf<uint32_t>(B_LY_HF2, n);
         MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-        uint32_t known = (uint32_t)per;
-        int cper = per;
+        const uint32_t known = (uint32_t)per;  // every tie group agrees on its first `per` symbols
+        uint8_t* settled = S.buf<uint8_t>(B_LY_SETTLED, n);
+        fill<uint8_t>(S, settled, n, (uint8_t)0);
         for (int round = 0;; round++) {
             MIBWT_REQUIRE(round < 1 << 20, LIBCUBWT_GPU_ERROR, "label refinement does not converge");
             inclusive_sum<uint32_t>(S, H, gid1, n, sc + SC_GROUPS);
             k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(H, gid1, (uint32_t)n, gofs);
             MIBWT_LAUNCH_CHECK();
-            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, cls, large);
+            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, settled, cls, large);
             MIBWT_LAUNCH_CHECK();
             exclusive_sum<uint32_t>(S, large, lpos, n, sc + SC_LARGE);
             uint32_t hv[SC_COUNT_];
@@ -650,35 +707,40 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
                 uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
                 uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
                 uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
-                // (group, next symbols) must fit one 64-bit key
-                const int gb = bits_for(hv[SC_GROUPS]);
-                cper = (64 - gb) / bits;
-                MIBWT_REQUIRE(cper >= 1, LIBCUBWT_GPU_ERROR, "chunk key too wide");
-                const int cb = bits * cper;
-                k_lab_chunk_gather<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, V, gid1, large, lpos, lam, codes, bits,
-                                                                         cper, known, cb, LK0, LV0, LI);
+                uint32_t* LG = S.buf<uint32_t>(B_LG_SZ, nl);
+                uint32_t* hq = S.buf<uint32_t>(B_LY_SCAN0, nl);
+                // large groups hold more than kSmallGroup members each, so their dense ids take
+                // gb bits; the l part takes what is left of the 64-bit key (at most 31 bits)
+                const int gb = bits_for((uint64_t)nl / (kSmallGroup + 1));
+                const int lbits = std::min(31, 64 - 11 - gb);
+                k_lab_pivot_gather<<<grid_for(n), kBlock, 0, S.stream>>>(phh, T, n, V, gid1, gofs, large, lpos, lam,
+                                                                         known, lbits, LK0, LV0, LI, LG);
                 MIBWT_LAUNCH_CHECK();
-                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, cb + gb);
-                k_lab_chunk_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, V2, H);
+                k_lab_pivot_heads<<<grid_for(nl), kBlock, 0, S.stream>>>(LG, nl, hq);
+                MIBWT_LAUNCH_CHECK();
+                inclusive_sum<uint32_t>(S, hq, hq, nl, sc + SC_GROUPS);
+                k_lab_pivot_ids<<<grid_for(nl), kBlock, 0, S.stream>>>(hq, nl, 11 + lbits, LK0);
+                MIBWT_LAUNCH_CHECK();
+                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, 11 + lbits + gb);
+                k_lab_pivot_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, lbits, V2, H, settled);
                 MIBWT_LAUNCH_CHECK();
                 rbytes += (double)nl * 24 * 8;
             }
             V = V2;
             if (nl == 0) break;
-            known += (uint32_t)cper;
         }
-        // exact verification of the label order
+        // check of the label order
         fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
-        k_lab_verify<<<grid_for(n), kBlock, 0, S.stream>>>(T, V, H, lam, n, sc + SC_COLLISION);
+        k_lab_verify<<<grid_for(n), kBlock, 0, S.stream>>>(SuffixCmp{T, n, phh}, V, H, lam, sc + SC_COLLISION);
         MIBWT_LAUNCH_CHECK();
         uint32_t bad = 0;
         S.read(&bad, sc + SC_COLLISION);
-        if (bad == 0) {
-            if (V != Vl) MIBWT_HIP(hipMemcpyAsync(Vl, V, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-            MIBWT_HIP(hipMemcpyAsync(hf, H, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-            break;
+        if (bad != 0) {
+            S.phase_end(ph, rbytes);
+            return 0;
         }
-        base = (base * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull) % kM61;
+        if (V != Vl) MIBWT_HIP(hipMemcpyAsync(Vl, V, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+        MIBWT_HIP(hipMemcpyAsync(hf, H, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
     }
     // label group ids as the initial keys of the induction
     inclusive_sum<uint32_t>(S, hf, gid1, n, sc + SC_GROUPS);
@@ -706,7 +768,18 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         std::swap(jcur, jnext);
         p.jmp = jcur;
     };
-    DoublingResult rr = refine_groups(S, K0, Vl, V1, (uint32_t)n, pol, advance, rank, SA);
+    // pointers double per round, so after 34 rounds every chain has left the text: a group that
+    // is still active then holds equal elements, which only inconsistent (colliding) label
+    // fingerprints produce
+    DoublingResult rr;
+    try {
+        rr = refine_groups(S, K0, Vl, V1, (uint32_t)n, pol, advance, rank, SA, 40);
+    } catch (const Error& e) {
+        if (e.code != MIBWT_HASH_COLLISION) throw;
+        MIBWT_HIP(hipStreamSynchronize(S.stream));
+        S.phase_end(ph, 0);
+        return 0;
+    }
     S.phase_end(ph, rr.bytes);
     S.stats.doubling_rounds_parse = rr.rounds;
 
@@ -716,14 +789,62 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
     k_bwt_from_sa<<<grid_for(n), kBlock, 0, S.stream>>>(SA, T, n, dO, sc + SC_PIDX);
     MIBWT_LAUNCH_CHECK();
     S.phase_end(ph, (double)n * 6);
-    MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
     uint32_t pidx = 0;
     S.read(&pidx, sc + SC_PIDX);
+    return (pidx >= 1 && pidx <= n) ? pidx : 0xFFFFFFFFu;  // an invalid index fails the inversion
+}
+
+
+int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    const uint64_t n = (uint64_t)n64;
+    S.phases_reset();
+    S.stats.n = n64;
+    if (!S.t_begin) {
+        MIBWT_HIP(hipEventCreate(&S.t_begin));
+        MIBWT_HIP(hipEventCreate(&S.t_end));
+    }
+    uint8_t* T;
+    if (src_host) {
+        T = stage_text(S, src, true, n);
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    } else {
+        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        T = stage_text(S, src, false, n);
+    }
+
+    // The suffix comparisons behind the Lyndon array and the label order trust Karp-Rabin
+    // fingerprints; the finished BWT is therefore inverted on the device and compared with the
+    // text, and a mismatch (a fingerprint collision) repeats the build with a new base.  The base
+    // is drawn per call.  MIBWT_LYN_HASH_BITS narrows the first attempt's fingerprints (tests).
+    uint64_t mask0 = ~0ull;
+    if (const char* hb = getenv("MIBWT_LYN_HASH_BITS")) {
+        const int v = atoi(hb);
+        if (v >= 1 && v < 64) mask0 = (1ull << v) - 1;
+    }
+    S.lyn_calls++;
+    uint64_t base = lyndon_base(S.lyn_calls ^ (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count());
+    uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
+    uint32_t sigma = 0;
+    uint32_t pidx = 0;
+    for (int attempt = 0;; attempt++) {
+        MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "Lyndon fingerprint collisions persist");
+        if (attempt > 0) base = lyndon_base(base * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
+        const uint64_t hmask = (attempt == 0) ? mask0 : ~0ull;
+        S.stats.retries = attempt;  // the timing covers every attempt
+        pidx = lyndon_attempt(S, T, n, base, hmask, codes, &sigma, dO);
+        if (pidx == 0) continue;  // the label order check failed: new base
+        const int ph = S.phase_begin("verify");
+        const bool ok = bwt_inverts_to(S, dO, (int64_t)n, pidx, T);
+        S.phase_end(ph, (double)n * 40);
+        if (ok) break;
+    }
+    MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
     S.phases_collect();
     float tot = 0;
     MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
     S.stats.total_ms = tot;
-    MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
     return (int64_t)pidx;
 }
 
@@ -735,7 +856,7 @@ int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n64, uint32_t* 
     uint8_t* T = stage_text(S, src, true, n);
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
     HashPair* pre = prefix_hashes(S, T, n, 0x1F3D5B79A2C4E681ull % kM61);
-    const int levels = lyndon_array(S, T, n, PrefixHash{pre}, lam);
+    const int levels = lyndon_array(S, T, n, PrefixHash{pre, ~0ull}, lam);
     MIBWT_HIP(hipMemcpyAsync(lam_host, lam, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, S.stream));
     MIBWT_HIP(hipStreamSynchronize(S.stream));
     return levels;

@@ -1337,7 +1337,16 @@ int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n64, int W, int
 }
 
 
-int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P)
+// rank[k] = lexicographic rank of the phrase of occurrence k
+__global__ void k_occ_lexrank(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid, uint32_t d,
+                              uint32_t* __restrict__ out)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < d) out[k] = lexrank[pid[k]];
+}
+
+int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P,
+                    PfpParseOut* parse_out)
 {
     MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
     MIBWT_REQUIRE(W >= 1 && W <= kMaxW && P >= 1 && P <= 65535, LIBCUBWT_BAD_PARAMETER, "bad W/P");
@@ -1455,7 +1464,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         sigma = hv[SC_SIGMA];
         break;
     }
-    MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "dictionary too large");
+    MIBWT_REQUIRE(L64 < (uint64_t)INT32_MAX - 2, MIBWT_INDEX_RANGE, "dictionary exceeds the 32-bit index range");
     const uint32_t L = (uint32_t)L64;
     S.phase_end(ph, (double)n + (double)W * d + 16.0 * d + 8.0 * d + 24.0 * d * dict_passes + 12.0 * d);
     S.stats.distinct_phrases = du;
@@ -1517,6 +1526,20 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
                                                   prank, psa);
     S.phase_end(ph, rp.bytes);
     S.stats.doubling_rounds_parse = rp.rounds;
+    if (parse_out) {
+        // the intermediate results of the phrase order and the parse suffix sort, then stop
+        const uint64_t m = std::min<uint64_t>(d, (uint64_t)parse_out->cap);
+        uint32_t* occ = S.buf<uint32_t>(B_Q_CNT, d);
+        k_occ_lexrank<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, occ);
+        MIBWT_LAUNCH_CHECK();
+        if (m > 0) {
+            MIBWT_HIP(hipMemcpyAsync(parse_out->rank, occ, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, S.stream));
+            MIBWT_HIP(hipMemcpyAsync(parse_out->psa, psa, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, S.stream));
+        }
+        MIBWT_HIP(hipStreamSynchronize(S.stream));
+        parse_out->distinct = du;
+        return (int64_t)d;
+    }
 
     // ---- 6. BWT assembly
     ph = S.phase_begin("assemble");

@@ -17,6 +17,9 @@
 
 namespace mibwt {
 
+int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64_t index64, bool ref_format,
+                  uint32_t* dI, int64_t rate, bool own_phases = true);
+
 constexpr uint32_t kUnSample = 64;
 constexpr uint32_t kUnEnd = 0xFFFFFFFFu;
 
@@ -119,18 +122,61 @@ __global__ void k_un_walk2(const uint64_t* __restrict__ rec, const uint32_t* __r
     }
 }
 
+// bad |= 1 where a[i] != b[i] (16 bytes per thread; both buffers padded to 16 bytes)
+__global__ void k_un_compare(const uint8_t* __restrict__ a, const uint8_t* __restrict__ b, uint64_t n,
+                             uint32_t* __restrict__ bad)
+{
+    const uint64_t p = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;
+    if (p >= n) return;
+    bool diff = false;
+    if (p + 16 <= n && ((uintptr_t)a & 15u) == 0 && ((uintptr_t)b & 15u) == 0) {
+        const uint4 x = *reinterpret_cast<const uint4*>(a + p), y = *reinterpret_cast<const uint4*>(b + p);
+        diff = (x.x != y.x) | (x.y != y.y) | (x.z != y.z) | (x.w != y.w);
+    } else {
+        for (uint64_t i = p; i < n && i < p + 16; i++) diff |= a[i] != b[i];
+    }
+    if (diff) atomicOr(bad, 1u);
+}
+
+// Whether the device BWT dO[0..n) (reference format, primary index `index`) inverts to the
+// device text dT[0..n): the exact check of a build whose intermediate steps trust fingerprints.
+// Runs on the storage stream inside the caller's phases (none of its own).
+bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT)
+{
+    if (index < 1 || index > n) return false;
+    uint8_t* dU = S.buf<uint8_t>(B_UN_IO, (size_t)n + 16);
+    try {
+        if (unbwt_run(S, dO, dU, n, index, true, nullptr, 0, false) != LIBCUBWT_NO_ERROR) return false;
+    } catch (const Error& e) {
+        if (e.code != LIBCUBWT_BAD_PARAMETER) throw;  // not a BWT with this index: a mismatch
+        return false;
+    }
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+    k_un_compare<<<grid_for(ceil_div((uint64_t)n, 16)), kBlock, 0, S.stream>>>(dU, dT, (uint64_t)n, sc + SC_COLLISION);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t bad = 1;
+    S.read(&bad, sc + SC_COLLISION);
+    return bad == 0;
+}
+
 // Inverse of the BWT dL[0..n) with primary index `index` into dU[0..n) (device buffers), or,
 // with dI, the sampled rows dI[p / rate] of the text positions p = 0, rate, 2 rate, ...
-// Returns LIBCUBWT_NO_ERROR or LIBCUBWT_BAD_PARAMETER when the input is not a BWT.
+// Returns LIBCUBWT_NO_ERROR or LIBCUBWT_BAD_PARAMETER when the input is not a BWT.  With
+// `own_phases` the call records and collects its own phase timing (a standalone entry).
 int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64_t index64, bool ref_format,
-                  uint32_t* dI, int64_t rate)
+                  uint32_t* dI, int64_t rate, bool own_phases)
 {
     MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
     MIBWT_REQUIRE(index64 >= 1 && index64 <= n64, LIBCUBWT_BAD_PARAMETER, "primary index out of range");
     const uint64_t n = (uint64_t)n64, index = (uint64_t)index64, m = n + 1;
-    S.phases_reset();
+    if (own_phases) S.phases_reset();
+    auto phase_begin = [&](const char* name) { return own_phases ? S.phase_begin(name) : -1; };
+    auto phase_end = [&](int id, double bytes) {
+        if (id >= 0) S.phase_end(id, bytes);
+    };
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    int ph = S.phase_begin("lf");
+    int ph = phase_begin("lf");
     uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
     uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
     uint64_t* rec = S.buf<uint64_t>(B_UN_REC, m);
@@ -139,9 +185,9 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     fill<uint64_t>(S, rec + index, 1, 0ull);  // LF('$' row) = row 0; its byte field is unused
     EmitLF em{rec};
     radix_sort<uint64_t, uint32_t, EmitLF>(S, K0, K1, nullptr, nullptr, n, 32, 40, &em);
-    S.phase_end(ph, 8.0 * n * 4);
+    phase_end(ph, 8.0 * n * 4);
 
-    ph = S.phase_begin("rank");
+    ph = phase_begin("rank");
     const uint32_t Sn = (uint32_t)ceil_div(m, (uint64_t)kUnSample);
     uint32_t* nxt0 = S.buf<uint32_t>(B_UN_NXT0, Sn);
     uint32_t* nxt1 = S.buf<uint32_t>(B_UN_NXT1, Sn);
@@ -167,14 +213,14 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     S.read(hv + 2, sc + SC_GROUPS, 1);
     MIBWT_REQUIRE(hv[0] == 0 && hv[1] == (uint32_t)n && hv[2] == kUnEnd, LIBCUBWT_BAD_PARAMETER,
                   "not a BWT with this primary index");
-    S.phase_end(ph, 16.0 * m);
+    phase_end(ph, 16.0 * m);
 
-    ph = S.phase_begin("text");
+    ph = phase_begin("text");
     if (dI) fill<uint32_t>(S, dI, 1, (uint32_t)index);  // position 0: the primary index
     k_un_walk2<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, n, Sn, dU, dI, dI ? (uint64_t)rate : 1);
     MIBWT_LAUNCH_CHECK();
-    S.phase_end(ph, 9.0 * m);
-    S.phases_collect();
+    phase_end(ph, 9.0 * m);
+    if (own_phases) S.phases_collect();
     return LIBCUBWT_NO_ERROR;
 }
 

@@ -90,6 +90,33 @@ CONFIGS = {
 }
 
 
+def batch_collection(count: int, seed: int = SEED_BASE + 4, threads: int = 16, min_len: int = 64 << 10,
+                     max_len: int = 16 << 20, max_copies: int = 64):
+    """`count` independent texts shaped like the batched configuration: lengths log-uniform in
+    [min_len, max_len], each a random base replicated 2..max_copies times with substitutions at a
+    log-uniform rate in [1e-5, 1e-2] (indels at a tenth of it, length 1..10); every other text is
+    mapped onto 256 byte values.  The parameters of text i are drawn from one seeded stream, the
+    text itself from its own seed, so the result does not depend on `threads`."""
+    from concurrent.futures import ThreadPoolExecutor
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n_total = int(np.exp(rng.uniform(np.log(min_len), np.log(max_len))))
+        copies = int(rng.integers(2, max_copies + 1))
+        rate = float(np.exp(rng.uniform(np.log(1e-5), np.log(1e-2))))
+        specs.append((i, n_total, copies, rate, int(rng.integers(1 << 31))))
+
+    def make(spec):
+        i, n_total, copies, rate, s = spec
+        T = repetitive_dna(max(1024, n_total // copies), copies, rate, rate / 10, 10, seed=s)
+        if i % 2:
+            T = (T.astype(np.uint32) * 97 + np.arange(len(T)) % 7 * 31).astype(np.uint8)
+        return T
+
+    with ThreadPoolExecutor(max(1, threads)) as ex:
+        return list(ex.map(make, specs))
+
+
 def workload(name: str, scale: float = 1.0) -> np.ndarray:
     """Text of a named workload; `scale` < 1 shrinks the base length (for quick tests)."""
     if name == "markov256m":

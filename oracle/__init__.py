@@ -19,7 +19,9 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_CPU_PFP_PATH = os.path.join(_HERE, "libcpupfp.so")
 _lib = None
+_cpu = None
 
 
 def build() -> str:
@@ -47,6 +49,8 @@ def lib():
         L.oracle_cubwt_to_ref.restype = None
         L.oracle_unbwt.argtypes = [u8p, i64, i64, u8p]
         L.oracle_unbwt.restype = ctypes.c_int
+        L.oracle_unbwt_mt.argtypes = [u8p, i64, i64, u8p, ctypes.c_int]
+        L.oracle_unbwt_mt.restype = ctypes.c_int
         L.oracle_pfp_triggers.argtypes = [u8p, i64, ctypes.c_int, ctypes.c_int, i32p]
         L.oracle_pfp_triggers.restype = i64
         L.oracle_pfp_parse.argtypes = [u8p, i64, ctypes.c_int, ctypes.c_int, i32p, i32p, i64p]
@@ -128,6 +132,16 @@ def unbwt(L: np.ndarray, index: int) -> np.ndarray:
     return T
 
 
+def unbwt_mt(L: np.ndarray, index: int, threads: int = 0) -> np.ndarray:
+    """oracle_unbwt with `threads` threads (0: all online CPUs)."""
+    L = _as_text(L)
+    T = np.empty_like(L)
+    rc = lib().oracle_unbwt_mt(_u8(L), len(L), index, _u8(T), threads)
+    if rc != 0:
+        raise ValueError("not a valid BWT / primary index")
+    return T
+
+
 def pfp_triggers(T, W: int, P: int) -> np.ndarray:
     T = _as_text(T)
     t = lib().oracle_pfp_triggers(_u8(T), len(T), W, P, None)
@@ -157,6 +171,28 @@ def pfp_bwt(T, W: int, P: int):
     if idx < 0:
         raise RuntimeError(f"oracle_pfp_bwt failed ({idx})")
     return O, int(idx), {k: getattr(info, k) for k, _ in PfpInfo._fields_}
+
+
+def cpu_pfp_bwt(T, W: int, P: int, threads: int = 0):
+    """Reference-format BWT and primary index from the multi-threaded C++ restatement of the
+    reference PFP pipeline (oracle/cpu_pfp.cpp; the CPU baseline), with `threads` threads
+    (0: all hardware threads).  Returns (O, index, parse length, distinct phrases)."""
+    global _cpu
+    if _cpu is None:
+        if not os.path.exists(_CPU_PFP_PATH):
+            build()
+        c = ctypes.CDLL(_CPU_PFP_PATH)
+        c.cpu_pfp_bwt.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                  ctypes.c_int, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64)]
+        c.cpu_pfp_bwt.restype = ctypes.c_int64
+        _cpu = c
+    T = _as_text(T)
+    O = np.empty(len(T), dtype=np.uint8)
+    d, du = ctypes.c_int64(), ctypes.c_int64()
+    idx = _cpu.cpu_pfp_bwt(T.ctypes.data, len(T), W, P, O.ctypes.data, threads, ctypes.byref(d), ctypes.byref(du))
+    if idx < 1:
+        raise RuntimeError(f"cpu_pfp_bwt failed ({idx})")
+    return O, int(idx), int(d.value), int(du.value)
 
 
 def lyndon_factors(T) -> np.ndarray:

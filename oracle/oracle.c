@@ -22,8 +22,10 @@
 #define _GNU_SOURCE
 #include "oracle.h"
 
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 /* ------------------------------------------------------------------ 1. SA-IS */
 
@@ -194,6 +196,132 @@ int oracle_unbwt(const uint8_t* L, int64_t n, int64_t index, uint8_t* T)
     }
     free(lf);
     return (r == index) ? 0 : -1;
+}
+
+/* The same inverse with `threads` POSIX threads (0: one per online CPU): LF by per-chunk symbol
+ * counts, then the LF cycle is cut at every row that is a multiple of a power-of-two stride;
+ * the segments between cut rows are walked in parallel twice (lengths, then the text), with
+ * their order taken from the cycle starting at row 0.  Same result and error convention as
+ * oracle_unbwt (the walk visits each row once; a cycle that misses the '$' row after n
+ * symbols is an error). */
+typedef struct {
+    const uint8_t* L;
+    int64_t n, index, m;
+    uint32_t* lf;
+    int64_t* occ;      /* per chunk: first LF slot of each symbol */
+    int64_t chunk;     /* rows per chunk */
+    int64_t stride;    /* power of two */
+    int64_t nseg;
+    int64_t* seg_len;  /* rows walked from each cut row */
+    int64_t* seg_next; /* next cut row index, -1 = the '$' row */
+    int64_t* seg_off;  /* output offset (from the end of the text) of each segment */
+    uint8_t* T;
+    int tid, threads, pass;
+} unbwt_job;
+
+static inline uint8_t un_row(const unbwt_job* j, int64_t r) { return (r < j->index) ? j->L[r] : j->L[r - 1]; }
+
+static void* unbwt_worker(void* arg)
+{
+    unbwt_job* j = (unbwt_job*)arg;
+    if (j->pass == 0) {  /* symbol counts of this thread's chunk */
+        int64_t* cnt = j->occ + (int64_t)j->tid * 256;
+        memset(cnt, 0, sizeof(int64_t) * 256);
+        int64_t a = j->tid * j->chunk, b = a + j->chunk < j->m ? a + j->chunk : j->m;
+        for (int64_t r = a; r < b; r++)
+            if (r != j->index) cnt[un_row(j, r)]++;
+    } else if (j->pass == 1) {  /* LF of this chunk's rows */
+        int64_t* nx = j->occ + (int64_t)j->tid * 256;
+        int64_t a = j->tid * j->chunk, b = a + j->chunk < j->m ? a + j->chunk : j->m;
+        for (int64_t r = a; r < b; r++) {
+            if (r == j->index) { j->lf[r] = 0; continue; }
+            j->lf[r] = (uint32_t)(nx[un_row(j, r)]++);
+        }
+    } else {  /* pass 2: segment lengths; pass 3: the text */
+        for (int64_t s = j->tid; s < j->nseg; s += j->threads) {
+            int64_t r = s * j->stride;
+            if (r == j->index) { j->seg_len[s] = 0; j->seg_next[s] = -1; continue; }
+            int64_t k = (j->pass == 3) ? j->seg_off[s] : 0, steps = 0;
+            if (j->pass == 3 && k < 0) continue;  /* not on the cycle from row 0 */
+            for (;;) {
+                if (j->pass == 3) {
+                    if (k >= j->n) break;
+                    j->T[j->n - 1 - k] = un_row(j, r);
+                    k++;
+                }
+                r = j->lf[r];
+                steps++;
+                if (r == j->index || (r & (j->stride - 1)) == 0 || steps > j->m) break;
+            }
+            if (j->pass == 2) {
+                j->seg_len[s] = steps;
+                j->seg_next[s] = (r == j->index || steps > j->m) ? -1 : r / j->stride;
+            }
+        }
+    }
+    return NULL;
+}
+
+int oracle_unbwt_mt(const uint8_t* L, int64_t n, int64_t index, uint8_t* T, int threads)
+{
+    if (n <= 0) return 0;
+    if (index < 1 || index > n) return -1;
+    if (threads <= 0) threads = (int)sysconf(_SC_NPROCESSORS_ONLN);
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    int64_t m = n + 1;
+    unbwt_job base;
+    memset(&base, 0, sizeof(base));
+    base.L = L; base.n = n; base.index = index; base.m = m; base.T = T; base.threads = threads;
+    base.lf = (uint32_t*)malloc(sizeof(uint32_t) * (size_t)m);
+    base.occ = (int64_t*)malloc(sizeof(int64_t) * 256 * (size_t)threads);
+    base.chunk = (m + threads - 1) / threads;
+    int64_t stride = 1;
+    while (stride * threads * 256 < m) stride <<= 1;
+    base.stride = stride;
+    base.nseg = (m + stride - 1) / stride;
+    base.seg_len = (int64_t*)malloc(sizeof(int64_t) * (size_t)base.nseg);
+    base.seg_next = (int64_t*)malloc(sizeof(int64_t) * (size_t)base.nseg);
+    base.seg_off = (int64_t*)malloc(sizeof(int64_t) * (size_t)base.nseg);
+    unbwt_job* jobs = (unbwt_job*)malloc(sizeof(unbwt_job) * (size_t)threads);
+    pthread_t* th = (pthread_t*)malloc(sizeof(pthread_t) * (size_t)threads);
+    int rc = 0;
+    if (!base.lf || !base.occ || !base.seg_len || !base.seg_next || !base.seg_off || !jobs || !th) { rc = -2; goto done; }
+    for (int pass = 0; pass < 4; pass++) {
+        for (int t = 0; t < threads; t++) {
+            jobs[t] = base;
+            jobs[t].tid = t;
+            jobs[t].pass = pass;
+            pthread_create(&th[t], NULL, unbwt_worker, &jobs[t]);
+        }
+        for (int t = 0; t < threads; t++) pthread_join(th[t], NULL);
+        if (pass == 0) {
+            /* first LF slot of symbol c in chunk t: 1 + #symbols < c + #c in earlier chunks */
+            int64_t run = 1;
+            for (int c = 0; c < 256; c++)
+                for (int t = 0; t < threads; t++) {
+                    int64_t v = base.occ[(int64_t)t * 256 + c];
+                    base.occ[(int64_t)t * 256 + c] = run;
+                    run += v;
+                }
+        } else if (pass == 2) {
+            /* segment order along the cycle from row 0 */
+            for (int64_t s = 0; s < base.nseg; s++) base.seg_off[s] = -1;
+            int64_t pos = 0, s = 0, guard = 0;
+            for (;;) {
+                if (base.seg_off[s] >= 0 || guard++ > base.nseg) { rc = -1; goto done; }
+                base.seg_off[s] = pos;
+                pos += base.seg_len[s];
+                if (base.seg_next[s] < 0) break;
+                s = base.seg_next[s];
+            }
+            if (pos != n) { rc = -1; goto done; }
+        }
+    }
+done:
+    free(base.lf); free(base.occ); free(base.seg_len); free(base.seg_next); free(base.seg_off);
+    free(jobs); free(th);
+    return rc;
 }
 
 /* ------------------------------------------------------------------ 2. PFP restatement */

@@ -37,6 +37,9 @@ void oracle_cubwt_to_ref(const uint8_t* L, int64_t n, int64_t index, uint8_t* O)
 /* Inverse BWT of the libcubwt format (L, index) -> T.  0 on success. */
 int oracle_unbwt(const uint8_t* L, int64_t n, int64_t index, uint8_t* T);
 
+/* The same with `threads` POSIX threads (0: all online CPUs); same result and return codes. */
+int oracle_unbwt_mt(const uint8_t* L, int64_t n, int64_t index, uint8_t* T, int threads);
+
 /* --- prefix-free parsing restatement (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu) --- */
 
 /* Trigger window end positions e (W <= e <= n-1), ascending; E may be NULL to count only.

@@ -10,7 +10,6 @@ import pytest
 import oracle
 from conftest import PKG_DIR
 
-pytestmark = pytest.mark.gpu
 BIN = os.path.join(PKG_DIR, "bin")
 
 
@@ -29,6 +28,7 @@ def run(d, prog, *args):
     return open(os.path.join(d, "out.txt"), "rb").read()
 
 
+@pytest.mark.gpu
 def test_pfp_cli_matches_oracle_format(tmp_path):
     rng = np.random.default_rng(1)
     T = rng.integers(2, 255, size=20000).astype(np.uint8)
@@ -44,6 +44,7 @@ def test_pfp_cli_matches_oracle_format(tmp_path):
     assert run(tmp_path, "pfp_bwt", "--learned") == out
 
 
+@pytest.mark.gpu
 def test_compare_script_semantics(tmp_path):
     # cmp output_<i>.txt output_libcubwt_<i>.txt must report equality (compare_script.sh:10)
     rng = np.random.default_rng(2)
@@ -55,6 +56,7 @@ def test_compare_script_semantics(tmp_path):
     assert a == b
 
 
+@pytest.mark.gpu
 def test_cubwt_driver_quirk(tmp_path):
     # primary index == n happens when suffix 0 is the largest suffix, e.g. "zyx...a"-like text
     T = np.array([200] + [3] * 40, dtype=np.uint8)
@@ -68,6 +70,7 @@ def test_cubwt_driver_quirk(tmp_path):
     assert quirk == (f"{len(T)}\n" + "".join(f"{v} " for v in L[1:])).encode()
 
 
+@pytest.mark.gpu
 def test_raw_mode(tmp_path):
     rng = np.random.default_rng(3)
     T = rng.integers(0, 256, size=50000).astype(np.uint8)
@@ -83,3 +86,13 @@ def test_malformed_input_rejected(tmp_path):
     (tmp_path / "input.txt").write_bytes(b"5\n1 2 3")
     r = subprocess.run([os.path.join(BIN, "pfp_bwt")], cwd=tmp_path, capture_output=True, timeout=60)
     assert r.returncode != 0
+
+
+def test_malformed_counts_rejected(tmp_path):
+    # a declared count beyond what the file can hold, and a count beyond the 64-bit range, are
+    # malformed input (exit 1 with a message), not an allocation abort
+    for text in (b"99999999999999\n1 2 3 ", b"123456789012345678901234567890\n1 "):
+        (tmp_path / "input.txt").write_bytes(text)
+        r = subprocess.run([os.path.join(BIN, "pfp_bwt")], cwd=tmp_path, capture_output=True, timeout=60)
+        assert r.returncode == 1, r.stderr.decode()
+        assert b"malformed" in r.stderr

@@ -52,3 +52,63 @@ def test_lyndon_dna32m(mibwt):
         assert (lam.astype(np.int64) == oracle.lyndon_array(T[: 1 << 22])).all()
         L, idx = st.bwt(T, "lyndon")
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+
+
+@pytest.mark.parametrize("bits", [2, 6])
+def test_fingerprint_collisions_are_caught(mibwt, monkeypatch, bits):
+    # narrow first-attempt fingerprints make the suffix comparisons of the Lyndon array and the
+    # label order collide: the inversion check of the finished BWT must catch it and rebuild
+    # with full fingerprints, giving the exact BWT (and reporting the retry)
+    monkeypatch.setenv("MIBWT_LYN_HASH_BITS", str(bits))
+    rng = np.random.default_rng(bits)
+    retried = 0
+    with mibwt.Storage(1 << 16) as st:
+        for k in range(6):
+            base = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=2000)
+            T = np.concatenate([base, base[7:], base[:900], base]) if k % 2 else rng.integers(0, 3, size=3000).astype(np.uint8)
+            L, idx = st.bwt(T, "lyndon")
+            O, oidx = oracle.bwt(T)
+            assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+            retried += st.stats()["retries"] > 0
+    assert retried > 0
+
+
+@pytest.mark.timeout(120)
+@pytest.mark.parametrize("kind", ["a_run_b", "dna_with_n_run"])
+def test_long_runs_are_fast(mibwt, kind):
+    # a long run of one symbol followed by a larger one gives Lyndon labels a^j b that all share
+    # long prefixes: the label order must not take a round per few symbols
+    import time
+    rng = np.random.default_rng(5)
+    if kind == "a_run_b":
+        T = np.frombuffer(b"a" * 300000 + b"b", dtype=np.uint8)
+    else:
+        dna = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=400000)
+        T = np.concatenate([dna[:200000], np.full(1 << 20, ord("N"), np.uint8), dna[200000:]])
+    with mibwt.Storage(len(T)) as st:
+        t0 = time.perf_counter()
+        L, idx = st.bwt(T, "lyndon")
+        dt = time.perf_counter() - t0
+    O, oidx = oracle.bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+    assert dt < 10.0, dt
+
+
+@pytest.mark.timeout(600)
+def test_lyndon_dna256m_matches_pfp_and_inverts(mibwt):
+    # the 256 MiB collection: the Lyndon-grammar BWT is byte for byte the PFP one, and the host
+    # oracle's inverse gives the text back
+    from mibwt import gen
+    T = gen.workload("dna256m")
+    n = len(T)
+    dT = mibwt.DeviceBuffer.from_numpy(T)
+    dO = mibwt.DeviceBuffer(n)
+    with mibwt.Storage(n) as st:
+        idx_l = st.bwt_device(dT.address, dO.address, n, "lyndon")
+        O_l = dO.download()
+        idx_p = st.bwt_device(dT.address, dO.address, n, "pfp", 5, 16)
+        O_p = dO.download()
+    dT.free()
+    dO.free()
+    assert idx_l == idx_p and (O_l == O_p).all()
+    assert (oracle.unbwt_mt(oracle.ref_to_cubwt(O_l, idx_l), idx_l, 16) == T).all()

@@ -60,7 +60,16 @@ def test_unbwt_of_gpu_bwt_32mib(mibwt):
         b.free()
 
 
-@pytest.mark.parametrize("r", [2, 4, 64])
+def test_bwt_aux_rejects_rates_libcubwt_rejects(storage, mibwt):
+    # libcubwt.cu:2437: the sampling rate is a power of two >= 4
+    T = np.frombuffer(b"ACGT" * 100, dtype=np.uint8)
+    for r in (0, 1, 2, 3, 6, 12):
+        with pytest.raises(mibwt.BwtError) as e:
+            storage.bwt_aux(T, r)
+        assert e.value.code == mibwt.BAD_PARAMETER
+
+
+@pytest.mark.parametrize("r", [4, 8, 64])
 def test_bwt_aux_indexes(storage, r):
     rng = np.random.default_rng(r)
     for n in (16, 17, 1000, 5003):

@@ -281,3 +281,41 @@ def test_lyndon_bwt_restatement_repetitive():
     assert (O == Ob).all() and idx == ib
     r, g = oracle.lyndon_ranks(T)
     assert 0 < g < len(T)
+
+
+@pytest.mark.parametrize("threads", [1, 2, 5, 16])
+def test_unbwt_mt_matches_sequential_inverse(threads):
+    # the multi-threaded inverse (segments of the LF cycle walked in parallel) gives the text
+    # back exactly like the sequential LF walk, and rejects the same non-BWT inputs
+    rng = np.random.default_rng(threads)
+    for n in [1, 2, 17, 1000, 65537, 300000]:
+        T = (rng.integers(0, 3, size=n) * 40).astype(np.uint8)
+        O, idx = oracle.bwt(T)
+        L = oracle.ref_to_cubwt(O, idx)
+        assert (oracle.unbwt_mt(L, idx, threads) == T).all()
+        assert (oracle.unbwt(L, idx) == T).all()
+        if n > 2:
+            bad = idx % n + 1  # a wrong primary index is not a BWT of this text
+            for f in (oracle.unbwt, lambda a, b: oracle.unbwt_mt(a, b, threads)):
+                try:
+                    U = f(L, bad)
+                except ValueError:
+                    continue
+                assert not (U == T).all() or bad == idx
+
+
+@pytest.mark.parametrize("W,P", [(5, 16), (1, 2), (3, 4), (10, 31), (35, 81)])
+def test_cpu_pfp_baseline_matches_sais(W, P):
+    # the multi-threaded C++ restatement (the CPU baseline) gives the SA-IS BWT exactly, for any
+    # number of threads, also with bytes 0/1/255 and texts shorter than W
+    rng = np.random.default_rng(W * 100 + P)
+    for it in range(60):
+        n = int(rng.integers(1, 4000))
+        k = int(rng.integers(1, 5))
+        T = (rng.integers(0, k, size=n) * int(rng.integers(1, 86))).astype(np.uint8)
+        if it % 5 == 0:
+            T = np.concatenate([T, T, T[: n // 2]])
+        O, idx = oracle.bwt(T)
+        O2, idx2, d, du = oracle.cpu_pfp_bwt(T, W, P, threads=1 + it % 5)
+        assert idx2 == idx and (O2 == O).all(), (n, W, P)
+        assert d == len(oracle.pfp_triggers(T, W, P)) + 1
