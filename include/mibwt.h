@@ -94,6 +94,18 @@ int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, in
 int64_t mibwt_pfp_bwt_device(void* storage, const uint8_t* dT, uint8_t* dO, int64_t n,
                              int32_t W, int32_t P, void* stream);
 
+/* mibwt_pfp_bwt_device with fixed (W, P) that allocates nothing: no counterpart in the
+ * reference; for callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  No storage: every buffer comes from
+ * one file-scope device array of 320 MiB that each GPU running the library holds until the
+ * process ends; texts of 1 <= n <= 1 MiB (BAD_PARAMETER beyond), W in 1..64, P in 1..65535
+ * (no learned choice).  A (W, P) whose dictionary outgrows the array returns
+ * LIBCUBWT_GPU_NOT_ENOUGH_MEMORY.  Calls on one GPU share the array and are serialised within a
+ * process; the host side uses no pinned or mapped memory.  Runs on `stream` (NULL: the null
+ * stream), synchronous; dT[0..n) -> dO[0..n) in reference format; returns the primary index. */
+int64_t mibwt_pfp_bwt_device_noalloc(const uint8_t* dT, uint8_t* dO, int64_t n, int32_t W, int32_t P,
+                                     void* stream);
+
 /* Inverse BWT of a device BWT in REFERENCE format (the output of the *_device entries):
  * dO[0..n) with primary index -> dU[0..n).  Synchronous.  Returns LIBCUBWT_NO_ERROR or
  * LIBCUBWT_BAD_PARAMETER (no reference counterpart: the on-device round-trip check). */

@@ -43,6 +43,7 @@ EXPORTED = [
     "mibwt_unbwt_device",
     "mibwt_pfp_bwt",
     "mibwt_pfp_bwt_device",
+    "mibwt_pfp_bwt_device_noalloc",
     "mibwt_lyndon_bwt",
     "mibwt_lyndon_bwt_device",
     "mibwt_lyndon_array",
@@ -118,6 +119,7 @@ def lib():
         L.mibwt_unbwt_device.argtypes = [vp, vp, vp, i64, i64, vp]
         L.mibwt_pfp_bwt.argtypes = [vp, u8p, u8p, i64, i32, i32]
         L.mibwt_pfp_bwt_device.argtypes = [vp, vp, vp, i64, i32, i32, vp]
+        L.mibwt_pfp_bwt_device_noalloc.argtypes = [vp, vp, i64, i32, i32, vp]
         L.mibwt_lyndon_bwt.argtypes = [vp, u8p, u8p, i64]
         L.mibwt_lyndon_bwt_device.argtypes = [vp, vp, vp, i64, vp]
         L.mibwt_lyndon_array.argtypes = [vp, u8p, i64, ctypes.POINTER(ctypes.c_uint32)]
@@ -510,6 +512,18 @@ class Stream:
             self.close()
         except Exception:
             pass
+
+
+NOALLOC_MAX_LENGTH = 1 << 20
+
+
+def bwt_device_noalloc(dT_ptr: int, dO_ptr: int, n: int, W: int = 5, P: int = 16, stream: int | None = None) -> int:
+    """mibwt_pfp_bwt_device_noalloc: the device build with fixed (W, P) that allocates nothing
+    (reference format, n <= NOALLOC_MAX_LENGTH); returns the primary index."""
+    rc = lib().mibwt_pfp_bwt_device_noalloc(dT_ptr, dO_ptr, n, W, P, stream)
+    if rc < 0:
+        raise BwtError(rc, "mibwt_pfp_bwt_device_noalloc")
+    return int(rc)
 
 
 def synchronize():

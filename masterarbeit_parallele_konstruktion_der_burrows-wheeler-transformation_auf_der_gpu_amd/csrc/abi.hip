@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <mutex>
 #include <thread>
 
 namespace mibwt {
@@ -121,6 +122,28 @@ int64_t host_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int met
         return idx;
     });
 }
+
+// ---- the allocation-free entry point
+//
+// mibwt_pfp_bwt_device_noalloc carves every buffer of the PFP pipeline from `scratch`, a
+// file-scope device array that each GPU running the library holds until the process ends.  In
+// host code the array's name only stands for the symbol: its device address on the current
+// device comes from hipGetSymbolAddress, once per device at the first call, and only that
+// address is handed to kernels and copies.  Calls on one GPU are serialised by a mutex (they
+// share the array, which keeps the last call's contents).
+constexpr int64_t kNoAllocMaxLength = 1 << 20;          // texts up to 1 MiB
+constexpr size_t kScratchBytes = (size_t)320 << 20;     // 320 MiB per GPU
+constexpr int kNoAllocDevices = 64;
+
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+struct NoAllocDevice {
+    std::mutex mu;
+    bool ready = false;
+    uint8_t* ptr = nullptr;  // device address of `scratch` on this device
+    Storage S;               // arena-mode state (host side), set up at the first call
+};
+NoAllocDevice g_noalloc[kNoAllocDevices];
 
 }  // namespace
 
@@ -253,6 +276,41 @@ int64_t mibwt_unbwt_device(void* storage, const uint8_t* dO, uint8_t* dU, int64_
         MIBWT_HIP(hipStreamSynchronize(S->stream));
         return rc;
     });
+}
+
+int64_t mibwt_pfp_bwt_device_noalloc(const uint8_t* dT, uint8_t* dO, int64_t n, int32_t W, int32_t P, void* stream)
+{
+    if (!dT || !dO || n < 1 || W < 1 || W > 64 || P < 1 || P > 65535) return LIBCUBWT_BAD_PARAMETER;
+    if (n > kNoAllocMaxLength) {
+        if (getenv("MIBWT_VERBOSE"))
+            fprintf(stderr, "mibwt: n = %lld exceeds the %lld bytes of the allocation-free entry\n", (long long)n,
+                    (long long)kNoAllocMaxLength);
+        return LIBCUBWT_BAD_PARAMETER;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kNoAllocDevices) return LIBCUBWT_GPU_NOT_SUPPORTED;
+    NoAllocDevice& D = g_noalloc[dev];
+    std::lock_guard<std::mutex> lock(D.mu);
+    const int64_t rc = guarded([&]() -> int64_t {
+        if (!D.ready) {
+            // set-up on this device, with it current: the device address of its copy of `scratch`
+            void* p = nullptr;
+            MIBWT_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+            MIBWT_REQUIRE(p != nullptr, LIBCUBWT_GPU_ERROR, "no device address for the scratch array");
+            D.ptr = reinterpret_cast<uint8_t*>(p);
+            D.S.device = dev;
+            D.S.bufs.resize(B_COUNT_);
+            D.ready = true;
+        }
+        D.S.stream = reinterpret_cast<hipStream_t>(stream);
+        D.S.max_length = kNoAllocMaxLength;
+        D.S.arena_reset(D.ptr, kScratchBytes);
+        return pfp_bwt_run(D.S, dT, false, dO, n, W, P);
+    });
+    if (getenv("MIBWT_VERBOSE"))
+        fprintf(stderr, "mibwt: allocation-free build n = %lld: %zu of %zu scratch bytes\n", (long long)n,
+                D.S.arena_peak, kScratchBytes);
+    return rc;
 }
 
 int64_t mibwt_pfp_bwt(void* storage, const uint8_t* T, uint8_t* L, int64_t n, int32_t W, int32_t P)

@@ -124,6 +124,23 @@ struct Storage {
     // Lyndon builds on this storage (seeds the per-call fingerprint base)
     uint64_t lyn_calls = 0;
     uint64_t sel_model_version = 0;  // selector model resident in B_SEL_MP / B_SEL_MPB / B_SEL_NORM
+    // Arena mode (the allocation-free entry point): every buffer is carved from one device region
+    // that the caller owns, host read-backs go through `arena_host` (no pinned or mapped memory),
+    // and no phase is timed (no events).  arena_peak = high-water mark of the region.
+    uint8_t* arena = nullptr;
+    size_t arena_cap = 0, arena_used = 0, arena_peak = 0;
+    alignas(16) unsigned char arena_host[4096];
+
+    // start a call in arena mode on [base, base + cap): every buffer is carved anew
+    void arena_reset(uint8_t* base, size_t cap)
+    {
+        arena = base;
+        arena_cap = cap;
+        arena_used = 0;
+        arena_peak = 0;
+        for (DevBuf& b : bufs) b = DevBuf{};
+        chain_epoch = 0;  // the chain status words are cleared again on first use
+    }
 
     // device buffer #id with capacity >= bytes; contents are not preserved on growth
     template <typename T>
@@ -132,6 +149,18 @@ struct Storage {
         size_t bytes = count * sizeof(T) + 256;
         if ((size_t)id >= bufs.size()) bufs.resize(id + 1);
         DevBuf& b = bufs[id];
+        if (arena) {
+            // bump allocation; a buffer that grows leaves its old region behind until the call ends
+            if (b.cap >= bytes) return reinterpret_cast<T*>(b.ptr);
+            const size_t off = (arena_used + 255) & ~(size_t)255;
+            if (off > arena_cap || bytes > arena_cap - off)
+                throw Error(LIBCUBWT_GPU_NOT_ENOUGH_MEMORY, "the scratch array is too small for this text");
+            b.ptr = arena + off;
+            b.cap = bytes;
+            arena_used = off + bytes;
+            if (arena_used > arena_peak) arena_peak = arena_used;
+            return reinterpret_cast<T*>(b.ptr);
+        }
         if (b.cap < bytes) {
             if (b.ptr) {
                 MIBWT_HIP(hipStreamSynchronize(stream));
@@ -158,6 +187,10 @@ struct Storage {
     T* host(size_t count)
     {
         size_t bytes = count * sizeof(T);
+        if (arena) {
+            MIBWT_REQUIRE(bytes <= sizeof(arena_host), LIBCUBWT_GPU_ERROR, "read-back too large");
+            return reinterpret_cast<T*>(arena_host);
+        }
         if (pinned_cap < bytes) {
             if (pinned) MIBWT_HIP(hipHostFree(pinned));
             pinned = nullptr;
@@ -245,6 +278,7 @@ struct Storage {
     }
     int phase_begin(const char* name)
     {
+        if (arena) return -1;
         if ((int)phases.size() <= phase_used) {
             phases.emplace_back();
             MIBWT_HIP(hipEventCreate(&phases.back().start));
@@ -258,12 +292,18 @@ struct Storage {
     }
     void phase_end(int id, double bytes)
     {
+        if (id < 0) return;
         phases[id].bytes = bytes;
         MIBWT_HIP(hipEventRecord(phases[id].stop, stream));
     }
     void phases_collect()
     {
         MIBWT_HIP(hipStreamSynchronize(stream));
+        if (arena) {
+            stats.num_phases = 0;
+            stats.device_bytes = (int64_t)arena_peak;
+            return;
+        }
         stats.num_phases = phase_used < MIBWT_MAX_PHASES ? phase_used : MIBWT_MAX_PHASES;
         for (int i = 0; i < stats.num_phases; i++) {
             float ms = 0;
@@ -471,7 +511,7 @@ void Storage::read_fast(T* dst, const T* dev, size_t count)
 {
     static_assert(sizeof(T) % 4 == 0, "whole words");
     const uint32_t words = (uint32_t)(count * sizeof(T) / 4);
-    if (words > (uint32_t)kSigWords) {
+    if (words > (uint32_t)kSigWords || arena) {
         read(dst, dev, count);
         return;
     }

@@ -542,8 +542,10 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t m = N;
     for (;;) {
         // ---- compaction of the kept sub-groups with next round's counts
-        uint32_t* sigd = S.signal_dev();
-        const uint32_t seq = S.signal_next();
+        // the counts come back through the mapped signal words, or, in arena mode (no mapped
+        // memory), by a plain read of the scalar slots
+        uint32_t* sigd = S.arena ? nullptr : S.signal_dev();
+        const uint32_t seq = S.arena ? 0u : S.signal_next();
         {
             const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, kCompactTile));
             uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
@@ -581,7 +583,14 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         };
         if constexpr (Policy::kSpeculate) sort_round(m);
         uint32_t hv[2];
-        S.wait_signal(seq, hv, 2);
+        if (S.arena) {
+            uint32_t all[SC_COUNT_];
+            S.read(all, sc, SC_COUNT_);
+            hv[0] = all[SC_ACTIVE];
+            hv[1] = all[SC_LARGE];
+        } else {
+            S.wait_signal(seq, hv, 2);
+        }
         const uint32_t m2 = hv[0], nl = hv[1];
         res.rounds++;
         MIBWT_REQUIRE(m2 == 0 || res.rounds < max_rounds, MIBWT_HASH_COLLISION, "groups do not separate");

@@ -1355,22 +1355,26 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     S.stats.n = n64;
     S.stats.W = W;
     S.stats.P = P;
-    if (!S.t_begin) {
+    const bool timed = !S.arena;  // arena mode creates and records no events
+    if (timed && !S.t_begin) {
         MIBWT_HIP(hipEventCreate(&S.t_begin));
         MIBWT_HIP(hipEventCreate(&S.t_end));
     }
+    auto mark_begin = [&]() {
+        if (timed) MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    };
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     // the text: a padded private copy (src == nullptr: already staged by the caller); a copy
     // from device memory is part of the timed build, an upload from the host is not
     uint8_t* T;
     if (src == nullptr) {
         T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
-        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        mark_begin();
     } else if (src_host) {
         T = stage_text(S, src, true, n);
-        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        mark_begin();
     } else {
-        MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+        mark_begin();
         T = stage_text(S, src, false, n);
     }
 
@@ -1662,12 +1666,12 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     S.stats.alpha_groups = L;
     S.stats.mixed_entries = ne;
 
-    MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
+    if (timed) MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
     uint32_t pidx = 0;
     S.read(&pidx, sc + SC_PIDX);
     S.phases_collect();
     float tot = 0;
-    MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
+    if (timed) MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
     S.stats.total_ms = tot;
     MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
     S.last_text = src_host ? nullptr : src;
