@@ -144,13 +144,27 @@ int64_t mibwt_pfp_parse(void* storage, const uint8_t* T, int64_t n, int32_t W, i
 int64_t mibwt_lyndon_array(void* storage, const uint8_t* T, int64_t n, uint32_t* lam);
 
 /* Batched BWT of k independent texts (host buffers, libcubwt format each; the reference has
- * no batch mode, its benchmark loops over files, Benchmark_Skript_Linux.sh:7-138).  Texts are
- * handed out longest first from one shared queue to `workers` (0 = 8, at most 16) worker
- * storages, each with its own HIP stream and host thread.  method: MIBWT_METHOD_PFP (W = 0:
- * learned (W,P) per text) or MIBWT_METHOD_LYNDON.  index[i] receives the primary index of
- * text i or its error code; the return value is 0 or the first error. */
+ * no batch mode, its benchmark loops over files, Benchmark_Skript_Linux.sh:7-138).
+ * MIBWT_METHOD_PFP: consecutive texts are combined into builds of about 1 GiB
+ * ($MIBWT_BATCH_CHUNK bytes), each uploaded as one array and built by the batched pipeline of
+ * mibwt_batch_bwt_device (W = P = 0: the learned (W,P) per build); `workers` is not used.
+ * MIBWT_METHOD_LYNDON: texts are handed out longest first to `workers` (0 = 8, at most 16)
+ * worker storages, each with its own HIP stream and host thread.  index[i] receives the
+ * primary index of text i or its error code; the return value is 0 or the first error. */
 int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* L, const int64_t* n,
                         int64_t k, int64_t* index, int32_t method, int32_t W, int32_t P, int32_t workers);
+
+/* Batched PFP BWT of k texts resident in HBM (no counterpart in the reference, which runs one
+ * process per file, Benchmark_Skript_Linux.sh:7-138): text i is dT[offsets[i] .. offsets[i+1])
+ * (host offsets, nondecreasing, every text 1..max_length bytes), its BWT in REFERENCE format
+ * goes to dO[offsets[i] .. offsets[i+1]) and its primary index to index[i].  Consecutive texts
+ * are built together: one staged layout of up to about 1 GiB ($MIBWT_BATCH_CHUNK), every
+ * kernel of the pipeline launched once per build over all of its texts (each text keeps its
+ * own terminator, first-window rule and primary index; dictionary groups never mix texts).
+ * W = P = 0: the learned selector chooses (W,P) per build.  On `stream` (NULL: the storage's
+ * own), synchronous.  Returns LIBCUBWT_NO_ERROR or an error code. */
+int64_t mibwt_batch_bwt_device(void* storage, const uint8_t* dT, const int64_t* offsets, int64_t k, uint8_t* dO,
+                               int64_t* index, int32_t W, int32_t P, void* stream);
 
 /* Per-phase statistics of the last call on this storage. */
 #define MIBWT_MAX_PHASES 16

@@ -53,6 +53,7 @@ EXPORTED = [
     "mibwt_set_kernel_timing",
     "mibwt_time_kernel",
     "mibwt_batch_bwt",
+    "mibwt_batch_bwt_device",
     "mibwt_wp_grid",
     "mibwt_wp_features",
     "mibwt_select_wp",
@@ -135,6 +136,8 @@ def lib():
         L.mibwt_batch_bwt.argtypes = [vp, ctypes.POINTER(ctypes.c_void_p), ctypes.POINTER(ctypes.c_void_p),
                                       ctypes.POINTER(ctypes.c_int64), i64, ctypes.POINTER(ctypes.c_int64),
                                       i32, i32, i32, i32]
+        L.mibwt_batch_bwt_device.argtypes = [vp, vp, ctypes.POINTER(ctypes.c_int64), i64, vp,
+                                             ctypes.POINTER(ctypes.c_int64), i32, i32, vp]
         L.mibwt_wp_grid.argtypes = [i32p, i32p]
         L.mibwt_wp_features.argtypes = [vp, vp, i64, fp]
         L.mibwt_select_wp.argtypes = [vp, i64, i32p, i32p]
@@ -362,6 +365,19 @@ class Storage:
         if rc < 0:
             raise BwtError(rc, "mibwt_batch_bwt")
         return [(outs[i], int(idx[i])) for i in range(k)]
+
+    def batch_bwt_device(self, dT_ptr: int, offsets, dO_ptr: int, W: int = 5, P: int = 16,
+                         stream: int | None = None) -> np.ndarray:
+        """Primary indices of the texts dT[offsets[i]:offsets[i+1]] (device); their BWTs in
+        reference format go to dO at the same offsets (mibwt_batch_bwt_device)."""
+        offs = np.ascontiguousarray(offsets, dtype=np.int64)
+        k = len(offs) - 1
+        idx = np.zeros(max(k, 1), dtype=np.int64)
+        rc = lib().mibwt_batch_bwt_device(self.handle, dT_ptr, offs.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), k,
+                                          dO_ptr, idx.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), W, P, stream)
+        if rc < 0:
+            raise BwtError(rc, "mibwt_batch_bwt_device")
+        return idx[:k]
 
     # ---- learned (W,P) selector
     def wp_features(self, dT_ptr: int, n: int) -> np.ndarray:

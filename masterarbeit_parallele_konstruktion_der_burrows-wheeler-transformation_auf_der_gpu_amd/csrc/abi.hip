@@ -12,6 +12,8 @@ namespace mibwt {
 int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n, int W, int P,
                     PfpParseOut* parse_out = nullptr);
 double pfp_time_scan(Storage& S, int reps, double* bytes);
+int64_t pfp_batch_run(Storage& S, const uint8_t* dT, const int64_t* offs, int64_t k, uint8_t* dO, int W, int P,
+                      int64_t* index, uint64_t chunk_bytes);
 int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n, int W, int P, uint32_t* E, int64_t cap);
 double time_copy(Storage& S, uint64_t bytes, int reps);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
@@ -65,6 +67,15 @@ struct StreamOrder {
         MIBWT_HIP(e2);
     }
 };
+
+// Staged bytes per batched build: texts are combined up to this much of the staged layout
+// (MIBWT_BATCH_CHUNK overrides it, in bytes; 0 builds every text alone).
+uint64_t batch_chunk_bytes()
+{
+    uint64_t v = (uint64_t)1 << 30;
+    if (const char* e = getenv("MIBWT_BATCH_CHUNK")) v = strtoull(e, nullptr, 10);
+    return v;
+}
 
 template <typename F>
 int64_t guarded(F&& f)
@@ -164,7 +175,7 @@ int64_t libcubwt_allocate_device_storage(void** device_storage, int64_t max_leng
         S->max_length = max_length;
         // text and output staging up front, like libcubwt's arena; the phase buffers grow
         // on first use and are then reused
-        (void)S->buf<uint8_t>(B_TEXT, (size_t)max_length + 8192);
+        (void)S->buf<uint8_t>(B_TEXT, staged_text_bytes((uint64_t)max_length));
         (void)S->buf<uint8_t>(B_STAGE, (size_t)max_length + 64);
         (void)S->host<uint64_t>(64);
         return LIBCUBWT_NO_ERROR;
@@ -452,8 +463,48 @@ int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* 
     for (int64_t i = 0; i < k; i++)
         if (!T[i] || !L[i] || n[i] < 1 || n[i] > S->max_length) return LIBCUBWT_BAD_PARAMETER;
     if (k == 0) return LIBCUBWT_NO_ERROR;
-    // default 8: with 256 mixed texts of 64 KiB-16 MiB in host buffers, 4 workers reach
-    // 2.2 GB/s, 8 reach 2.55 GB/s and 16 reach 2.57 GB/s
+    if (method == MIBWT_METHOD_PFP) {
+        // PFP: consecutive texts are combined into builds of up to batch_chunk_bytes() staged
+        // bytes; per build the texts go up as one dense device array, the batched pipeline runs
+        // on it, and every BWT comes back in libcubwt format
+        for (int64_t i = 0; i < k; i++) index[i] = LIBCUBWT_GPU_ERROR;
+        return guarded([&]() -> int64_t {
+            MIBWT_HIP(hipSetDevice(S->device));
+            const uint64_t chunk = (W > 56) ? 0 : batch_chunk_bytes();  // wider windows: one text per build
+            int64_t a = 0;
+            while (a < k) {
+                int64_t b = a;
+                uint64_t bytes = 0;
+                while (b < k && (b == a || bytes + (uint64_t)n[b] <= chunk)) bytes += (uint64_t)n[b++];
+                std::vector<int64_t> offs((size_t)(b - a + 1), 0);
+                for (int64_t i = a; i < b; i++) offs[(size_t)(i - a + 1)] = offs[(size_t)(i - a)] + n[i];
+                uint8_t* din = S->buf<uint8_t>(B_BATCH_IN, (size_t)bytes);
+                uint8_t* dout = S->buf<uint8_t>(B_BATCH_OUT, (size_t)bytes);
+                for (int64_t i = a; i < b; i++)
+                    MIBWT_HIP(hipMemcpyAsync(din + offs[(size_t)(i - a)], T[i], (size_t)n[i], hipMemcpyHostToDevice,
+                                             S->stream));
+                std::vector<uint8_t> wrap((size_t)(b - a));
+                for (int64_t i = a; i < b; i++) wrap[(size_t)(i - a)] = T[i][n[i] - 1];  // before L (may alias T)
+                pfp_batch_run(*S, din, offs.data(), b - a, dout, W, P, index + a, chunk);
+                for (int64_t i = a; i < b; i++) {
+                    // libcubwt format: L[0] = T[n-1], L[1..i-1] = O[0..i-2], L[i..n-1] = O[i..n-1]
+                    const int64_t idx = index[i];
+                    const uint8_t* O = dout + offs[(size_t)(i - a)];
+                    if (idx > 1)
+                        MIBWT_HIP(hipMemcpyAsync(L[i] + 1, O, (size_t)(idx - 1), hipMemcpyDeviceToHost, S->stream));
+                    if (n[i] > idx)
+                        MIBWT_HIP(hipMemcpyAsync(L[i] + idx, O + idx, (size_t)(n[i] - idx), hipMemcpyDeviceToHost,
+                                                 S->stream));
+                }
+                MIBWT_HIP(hipStreamSynchronize(S->stream));
+                for (int64_t i = a; i < b; i++) L[i][0] = wrap[(size_t)(i - a)];
+                a = b;
+            }
+            return LIBCUBWT_NO_ERROR;
+        });
+    }
+    // Lyndon: a pool of worker storages; default 8: with 256 mixed texts of 64 KiB-16 MiB in
+    // host buffers, 4 workers reach 2.2 GB/s, 8 reach 2.55 GB/s and 16 reach 2.57 GB/s
     if (workers == 0) workers = 8;
     // one storage (stream) per worker, created on first use and kept by the parent storage
     int64_t rc = guarded([&]() -> int64_t {
@@ -502,6 +553,28 @@ int64_t mibwt_batch_bwt(void* storage, const uint8_t* const* T, uint8_t* const* 
     for (int64_t i = 0; i < k; i++)
         if (index[i] < 0) return index[i];
     return LIBCUBWT_NO_ERROR;
+}
+
+int64_t mibwt_batch_bwt_device(void* storage, const uint8_t* dT, const int64_t* offsets, int64_t k, uint8_t* dO,
+                               int64_t* index, int32_t W, int32_t P, void* stream)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || !dT || !dO || !offsets || !index || k < 0) return LIBCUBWT_BAD_PARAMETER;
+    if (!(W == 0 && P == 0) && (W < 1 || W > 64 || P < 1 || P > 65535)) return LIBCUBWT_BAD_PARAMETER;
+    for (int64_t i = 0; i < k; i++) {
+        const int64_t len = offsets[i + 1] - offsets[i];
+        if (offsets[i] < 0 || len < 1 || len > S->max_length) return LIBCUBWT_BAD_PARAMETER;
+    }
+    if (k == 0) return LIBCUBWT_NO_ERROR;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        StreamOrder order(*S, stream);
+        // windows wider than 56 bytes build every text alone
+        const uint64_t chunk = (W > 56) ? 0 : batch_chunk_bytes();
+        pfp_batch_run(*S, dT, offsets, k, dO, W, P, index, chunk);
+        MIBWT_HIP(hipStreamSynchronize(S->stream));
+        return LIBCUBWT_NO_ERROR;
+    });
 }
 
 int64_t mibwt_select_wp_device(void* storage, const uint8_t* dT, int64_t n, int32_t* W, int32_t* P, float* pred)

@@ -357,6 +357,13 @@ enum BufId : int {
     B_DU_HEAD,
     B_DU_START,
     B_DU_LEN,
+    B_DU_TL,        // text bytes of each distinct phrase
+    B_DU_TEXT,      // text of each distinct phrase (batches)
+    B_PTEXT,        // text of each phrase occurrence (batches)
+    B_BT_TABLES,    // batch layout tables
+    B_BT_PIDX,      // batch primary indices
+    B_BATCH_IN,     // host batches: the texts of one build, back to back
+    B_BATCH_OUT,    // host batches: their BWTs
     B_DU_LEN64,
     B_DU_FREQ,
     B_DU_OFF,

@@ -40,12 +40,53 @@ constexpr int kMaxW = 64;
 
 // --------------------------------------------------------------------- helpers
 
-// Phrase k of the parse: [start, end] in the extended text, end >= n means sentinels.
-__device__ __forceinline__ void phrase_bounds(const uint32_t* __restrict__ E, uint32_t k, int W,
-                                              uint32_t& start, uint32_t& end)
+// The texts of one build.  One text or many (a batch): text t lies at base[t] of the staged
+// buffer, n_t = len[t] bytes followed by zero bytes up to the next text, which starts at the
+// next multiple of kTextAlign at least kTextGap bytes later (text 0 at 0).  Each text has its
+// own virtual terminator $^W: its last phrase ends at base + n_t + W - 1, and that end carries
+// kSentinel in the phrase-end array E.  The BWT of text t lands at obase[t] of the output (the
+// texts' BWTs back to back).
+constexpr uint32_t kSentinel = 0x80000000u;
+constexpr uint32_t kTextAlign = 8192;  // = the trigger scan's sub-tile (kScanTileW)
+constexpr uint32_t kTextGap = 64;      // >= W: the sentinel end stays inside the text's range
+
+struct TextSet {
+    uint32_t count;
+    const uint32_t* base;   // [count]
+    const uint32_t* len;    // [count]
+    const uint32_t* obase;  // [count + 1]
+    const uint32_t* ptext;  // [d] text of every phrase occurrence (nullptr for one text)
+    uint32_t n1;            // one text: its length (base == nullptr)
+    __device__ __forceinline__ uint32_t text_of(uint32_t k) const { return ptext ? ptext[k] : 0u; }
+};
+
+// start of the text that follows the text whose last phrase ends at sentinel end `se`
+__device__ __host__ __forceinline__ uint32_t next_text_base(uint32_t se, int W)
 {
-    start = (k == 0) ? 0u : E[k - 1] - (uint32_t)W + 1u;
-    end = E[k];
+    const uint32_t text_end = se - (uint32_t)W + 1u;
+    return (text_end + kTextGap + kTextAlign - 1) / kTextAlign * kTextAlign;
+}
+
+// Phrase k of the parse: [start, end] in the extended text; tl = its bytes inside the text
+// (the last phrase of a text has W terminator symbols after them).
+__device__ __forceinline__ void phrase_bounds(const uint32_t* __restrict__ E, uint32_t k, int W,
+                                              uint32_t& start, uint32_t& end, uint32_t& tl)
+{
+    const uint32_t ek = E[k];
+    end = ek & ~kSentinel;
+    if (k == 0) {
+        start = 0;
+    } else {
+        const uint32_t ep = E[k - 1];
+        start = (ep & kSentinel) ? next_text_base(ep & ~kSentinel, W) : ep - (uint32_t)W + 1u;
+    }
+    tl = (ek & kSentinel) ? end - (uint32_t)W + 1u - start : end - start + 1u;
+}
+
+// whether occurrence k is the first phrase of its text
+__device__ __forceinline__ bool first_of_text(const uint32_t* __restrict__ E, uint32_t k)
+{
+    return k == 0 || (E[k - 1] & kSentinel);
 }
 
 // 8 text bytes at word index k of a stream starting sh/8 bytes into word w[0].
@@ -54,16 +95,14 @@ __device__ __forceinline__ uint64_t funnel8(uint64_t lo, uint64_t hi, int sh)
     return sh ? ((lo >> sh) | (hi << (64 - sh))) : lo;
 }
 
-// Fingerprint of the phrase [s, e] (virtual terminator beyond n): 8-byte chunks at offsets
-// c = 0, 8, ... of the bytes inside the text, each mixed with its offset, then the number of
-// terminator positions.  The chunks are streamed as aligned words, 32 bytes per step with the
-// four loads issued together (the padded text copy makes the over-read safe).
-__device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint64_t n, uint32_t s, uint32_t e,
-                                uint64_t seed)
+// Fingerprint of the phrase at s of `len` symbols, the first `tl` of them text bytes, the rest
+// terminators: 8-byte chunks at offsets c = 0, 8, ... of the text bytes, each mixed with its
+// offset, then the number of terminator symbols.  The chunks are streamed as aligned words, 32
+// bytes per step with the four loads issued together (the padded text copy makes the
+// over-read safe).
+__device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint32_t s, uint32_t len, uint32_t tl, uint64_t seed)
 {
-    const uint32_t len = e - s + 1;
-    const uint32_t text_end = (e < n) ? e + 1 : (uint32_t)n;  // exclusive
-    const uint32_t L = text_end - s;
+    const uint32_t L = tl;
     uint64_t h = seed ^ ((uint64_t)len * 0x9E3779B97F4A7C15ull);
     const uint64_t* w = reinterpret_cast<const uint64_t*>(T + (s & ~7u));
     const int sh = (int)(s & 7u) * 8;
@@ -84,20 +123,18 @@ __device__ uint64_t phrase_hash(const uint8_t* __restrict__ T, uint64_t n, uint3
         }
         prev = a[3];
     }
-    if (e >= n) h = mix64(h ^ 0xA24BAED4963EE407ull ^ (uint64_t)(e - n + 1));
+    if (len > tl) h = mix64(h ^ 0xA24BAED4963EE407ull ^ (uint64_t)(len - tl));
     return h;
 }
 
-// Byte equality of two phrases.  Both are streamed as aligned words, 32 bytes per step with
-// all eight loads issued together (the padded text copy makes the over-read safe).
-__device__ bool phrase_equal(const uint8_t* __restrict__ T, uint64_t n, uint32_t s1, uint32_t e1,
-                             uint32_t s2, uint32_t e2)
+// Byte equality of two phrases (start, length, text bytes).  Both are streamed as aligned
+// words, 32 bytes per step with all eight loads issued together (the padded text copy makes the
+// over-read safe).
+__device__ bool phrase_equal(const uint8_t* __restrict__ T, uint32_t s1, uint32_t len1, uint32_t tl1, uint32_t s2,
+                             uint32_t len2, uint32_t tl2)
 {
-    if (e1 - s1 != e2 - s2) return false;
-    const bool x1 = e1 >= n, x2 = e2 >= n;
-    if (x1 != x2) return false;  // only the last phrase reaches into the terminator
-    const uint32_t len = (x1 ? (uint32_t)n - s1 : e1 - s1 + 1);
-    if (x1 && (uint32_t)n - s2 != len) return false;
+    if (len1 != len2 || tl1 != tl2) return false;
+    const uint32_t len = tl1;
     const uint64_t* w1 = reinterpret_cast<const uint64_t*>(T + (s1 & ~7u));
     const uint64_t* w2 = reinterpret_cast<const uint64_t*>(T + (s2 & ~7u));
     const int sh1 = (int)(s1 & 7u) * 8, sh2 = (int)(s2 & 7u) * 8;
@@ -255,7 +292,11 @@ constexpr int kScanSub = 8;                     // sub-tiles per chained tile
 
 struct TrigScan {
     const uint8_t* T;
-    uint64_t n;
+    uint64_t n;       // staged length (all texts with their gaps, a multiple of kTextAlign)
+    uint32_t n1;      // one text: its length (sub_text == nullptr)
+    const uint32_t* sub_text;  // text of every sub-tile (nullptr: one text at 0)
+    const uint32_t* tbase;     // per text: start and length
+    const uint32_t* tlen;
     int W;
     uint32_t P;
     uint32_t pshift;  // P = 2^pshift * q, q odd
@@ -281,10 +322,12 @@ __device__ __forceinline__ bool trig_divisible(V v, const TrigScan& p)
 
 // Trigger mask of the 32 window ends [e0, e0+32) of one thread; `txt` holds the current
 // sub-tile's bytes from kScanHalo before its first window end (LDS), a/b the thread's own 32
-// bytes.
+// bytes; window ends lo <= e < hi of the sub-tile's text count, and its sentinel end `sent` is
+// always set.
 template <typename V, bool kPow2>
 __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc, const uint4& a, const uint4& b,
-                                                const TrigScan& p, V v0, const uint8_t* first, uint64_t e0)
+                                                const TrigScan& p, V v0, const uint8_t* first, uint64_t e0,
+                                                uint64_t lo, uint64_t hi, uint64_t sent)
 {
     const int W = p.W;
     // value of the window ending at e0 - 1
@@ -320,9 +363,12 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
         for (int k = 0; k < W && eq; k++) eq = (txt[loc + i - W + 1 + k] == first[k]);
         if (eq) mask |= 1u << i;
     }
-    // window ends W <= e < n
-    if (e0 + kScanPer > p.n) mask &= (e0 >= p.n) ? 0u : ((1u << (uint32_t)(p.n - e0)) - 1u);
-    if (e0 < (uint64_t)W) mask &= ~((W - e0 >= 32) ? ~0u : ((1u << (uint32_t)(W - e0)) - 1u));
+    // window ends lo <= e < hi (W <= e - base < n_t), then the sentinel end
+    const uint64_t ia = lo > e0 ? lo - e0 : 0, ib = hi > e0 ? hi - e0 : 0;
+    const uint32_t below_b = ib >= 32 ? ~0u : ((1u << (uint32_t)ib) - 1u);
+    const uint32_t below_a = ia >= 32 ? ~0u : ((1u << (uint32_t)ia) - 1u);
+    mask &= below_b & ~below_a;
+    if (sent >= e0 && sent < e0 + kScanPer) mask |= 1u << (uint32_t)(sent - e0);
     return mask;
 }
 
@@ -337,36 +383,49 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
     __shared__ __align__(16) uint8_t txt[kScanHalo + kScanTileW + 16];
     __shared__ uint8_t first[64];
     __shared__ uint32_t masks[kScanSub][kBlock];
+    __shared__ uint32_t sub_sent[kScanSub];  // sentinel end of each sub-tile's text
     __shared__ V v0s;
     __shared__ uint32_t slot;
     __shared__ uint32_t wsum[kWaves];
     const int W = p.W;
     const uint32_t tile = chain_ticket(p.chain, p.tiles, &slot);
-    if (threadIdx.x < 64) {
-        // value of the first window T[0..W-1], reduced over wavefront 0
-        const int k = threadIdx.x;
-        const uint8_t c = (k < W && (uint64_t)k < p.n) ? p.T[k] : 0;
-        first[k] = c;
-        V x = (k < W) ? ((V)c << (W - 1 - k)) : (V)0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            if (sizeof(V) == 4) x += (V)__shfl_xor((uint32_t)x, o, 64);
-            else x += (V)__shfl_xor((unsigned long long)x, o, 64);
-        }
-        if (k == 0) v0s = x;
-    }
     const uint32_t loc = kScanHalo + threadIdx.x * kScanPer;
     uint32_t cnt = 0;
+    uint32_t cur_text = 0xFFFFFFFFu;
 #pragma unroll 1
     for (int sub = 0; sub < kScanSub; sub++) {
-        const uint64_t sb = ((uint64_t)tile * kScanSub + sub) * kScanTileW;
+        const uint64_t gsub = (uint64_t)tile * kScanSub + sub;
+        const uint64_t sb = gsub * kScanTileW;
         const uint64_t e0 = sb + (uint64_t)threadIdx.x * kScanPer;
         uint4 a = make_uint4(0, 0, 0, 0), b = make_uint4(0, 0, 0, 0);
         if (e0 < p.n) {
             a = *reinterpret_cast<const uint4*>(p.T + e0);
             b = *reinterpret_cast<const uint4*>(p.T + e0 + 16);
         }
+        // the sub-tile's text (uniform across the block)
+        uint32_t t = 0, base = 0, nt = p.n1;
+        if (p.sub_text && sb < p.n) {
+            t = p.sub_text[gsub];
+            base = p.tbase[t];
+            nt = p.tlen[t];
+        }
         if (sub) __syncthreads();  // the previous sub-tile's LDS reads are done
+        if (t != cur_text && threadIdx.x < 64) {
+            // value of the text's first window T[base .. base+W-1], reduced over wavefront 0
+            const int k = threadIdx.x;
+            const uint8_t c = (k < W && (uint32_t)k < nt) ? p.T[base + k] : 0;
+            first[k] = c;
+            V x = (k < W) ? ((V)c << (W - 1 - k)) : (V)0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                if (sizeof(V) == 4) x += (V)__shfl_xor((uint32_t)x, o, 64);
+                else x += (V)__shfl_xor((unsigned long long)x, o, 64);
+            }
+            if (k == 0) v0s = x;
+        }
+        cur_text = t;
+        const uint64_t sent = (uint64_t)base + nt + W - 1;
+        if (threadIdx.x == 0) sub_sent[sub] = (uint32_t)sent;
         *reinterpret_cast<uint4*>(txt + loc) = a;
         *reinterpret_cast<uint4*>(txt + loc + 16) = b;
         if (threadIdx.x < kScanHalo / 16) {
@@ -378,7 +437,9 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
             *reinterpret_cast<uint4*>(txt + kScanHalo + kScanTileW) = make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        const uint32_t m = (sb < p.n) ? trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0) : 0u;
+        const uint32_t m = (sb < p.n) ? trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0, (uint64_t)base + W,
+                                                               (uint64_t)base + nt, sent)
+                                      : 0u;
         masks[sub][threadIdx.x] = m;
         cnt += (uint32_t)__popc(m);
     }
@@ -400,10 +461,12 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
             uint32_t o = run + (h ? tot0 + (ex >> 16) : (ex & 0xFFFFu));
             uint32_t mm = h ? m1 : m0;
             const uint64_t e0 = ((uint64_t)tile * kScanSub + sub + h) * kScanTileW + (uint64_t)threadIdx.x * kScanPer;
+            const uint32_t sent = sub_sent[sub + h];
             while (mm) {
                 const int i = __ffs(mm) - 1;
                 mm &= mm - 1;
-                if (o < p.cap) p.E[o] = (uint32_t)(e0 + i);
+                const uint32_t e = (uint32_t)(e0 + i);
+                if (o < p.cap) p.E[o] = e | (e == sent ? kSentinel : 0u);
                 o++;
             }
         }
@@ -419,14 +482,21 @@ inline uint64_t inverse_mod_2_64(uint64_t q)
     return x;
 }
 
-// Trigger end positions in order into E (capacity cap); returns the number of triggers,
-// which may exceed cap (then E holds the first cap of them).  Requires W <= kScanMaxW.
-uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, uint32_t* E, uint32_t cap,
+// Phrase ends in order into E (capacity cap): the trigger ends of every text and its sentinel
+// end (flagged kSentinel); *total_dev receives their number, which may exceed cap (then E holds
+// the first cap of them).  n = staged length (a multiple of kTextAlign); one text of n1 bytes
+// (sub_text == nullptr) or many (sub_text/tbase/tlen on the device).  Requires W <= kScanMaxW.
+uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, uint32_t n1, const uint32_t* sub_text,
+                   const uint32_t* tbase, const uint32_t* tlen, int W, uint32_t P, uint32_t* E, uint32_t cap,
                    uint32_t* total_dev)
 {
     TrigScan p{};
     p.T = T;
     p.n = n;
+    p.n1 = n1;
+    p.sub_text = sub_text;
+    p.tbase = tbase;
+    p.tlen = tlen;
     p.W = W;
     p.P = P;
     uint32_t s = 0, q = P;
@@ -454,20 +524,28 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, 
 
 // --------------------------------------------------------------------- 2/3. dictionary
 
-// Also records the character before each phrase (the last text character for phrase 0),
-// which the whole-phrase BWT entries need later: it lies in the previous phrase's bytes,
-// which a neighbouring lane reads at the same time.
-__global__ void k_phrase_hash(const uint8_t* __restrict__ T, uint64_t n, int W,
-                              const uint32_t* __restrict__ E, uint32_t d, uint64_t seed, uint64_t hmask,
-                              uint64_t* __restrict__ H, uint32_t* __restrict__ V, uint8_t* __restrict__ pc)
+// Also records the character before each phrase (the last character of its text for the
+// first phrase of a text), which the whole-phrase BWT entries need later: it lies in the previous
+// phrase's bytes, which a neighbouring lane reads at the same time.  With many texts the text id
+// goes into the fingerprint: equal strings of different texts are different phrases.
+__global__ void k_phrase_hash(const uint8_t* __restrict__ T, TextSet ts, int W, const uint32_t* __restrict__ E,
+                              uint32_t d, uint64_t seed, uint64_t hmask, uint64_t* __restrict__ H,
+                              uint32_t* __restrict__ V, uint8_t* __restrict__ pc)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
-    uint32_t s, e;
-    phrase_bounds(E, k, W, s, e);
-    H[k] = phrase_hash(T, n, s, e, seed) & hmask;
+    uint32_t s, e, tl;
+    phrase_bounds(E, k, W, s, e, tl);
+    uint64_t sd = seed;
+    if (ts.ptext) sd ^= mix64(0x51ED270B27A1F3ADull + ts.ptext[k]);
+    H[k] = phrase_hash(T, s, e - s + 1, tl, sd) & hmask;
     V[k] = k;
-    pc[k] = (s > 0) ? T[s - 1] : T[n - 1];
+    if (first_of_text(E, k)) {
+        const uint32_t t = ts.text_of(k);
+        pc[k] = T[ts.base ? ts.base[t] + ts.len[t] - 1 : ts.n1 - 1];
+    } else {
+        pc[k] = T[s - 1];
+    }
 }
 
 // Phrase fingerprints are first sorted by their low 48 bits (6 radix passes).  Neighbours
@@ -494,26 +572,29 @@ __global__ void k_dict_mark(const uint64_t* __restrict__ H, uint32_t d, uint64_t
 }
 
 // Occurrence k (text order: its own bytes stream) against the representative of its phrase
-// (few distinct phrases, cache resident); a mismatch is a fingerprint collision (bit 0).
-__global__ void k_dict_verify(const uint8_t* __restrict__ T, uint64_t n, int W, const uint32_t* __restrict__ E,
+// (few distinct phrases, cache resident); a mismatch, or a representative of another text, is
+// a fingerprint collision (bit 0).
+__global__ void k_dict_verify(const uint8_t* __restrict__ T, TextSet ts, int W, const uint32_t* __restrict__ E,
                               uint32_t d, const uint32_t* __restrict__ pid, const uint32_t* __restrict__ dstart,
-                              const uint32_t* __restrict__ dlen, uint32_t* __restrict__ collision)
+                              const uint32_t* __restrict__ dlen, const uint32_t* __restrict__ dtl,
+                              const uint32_t* __restrict__ dtext, uint32_t* __restrict__ collision)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
-    uint32_t s, e;
-    phrase_bounds(E, k, W, s, e);
+    uint32_t s, e, tl;
+    phrase_bounds(E, k, W, s, e, tl);
     const uint32_t u = pid[k];
     const uint32_t s2 = dstart[u];
-    if (s2 != s && !phrase_equal(T, n, s, e, s2, s2 + dlen[u] - 1)) atomicOr(collision, 1u);
+    if (ts.ptext && ts.ptext[k] != dtext[u]) atomicOr(collision, 1u);
+    else if (s2 != s && !phrase_equal(T, s, e - s + 1, tl, s2, dlen[u], dtl[u])) atomicOr(collision, 1u);
 }
 
 // ids[j] = inclusive count of heads (1-based id of the distinct phrase at sorted position j)
 __global__ void k_dict_assign(const uint32_t* __restrict__ ids, const uint32_t* __restrict__ V,
-                              const uint32_t* __restrict__ flag, const uint32_t* __restrict__ E,
-                              uint32_t d, int W, uint32_t* __restrict__ pid,
-                              uint32_t* __restrict__ head, uint32_t* __restrict__ du_start,
-                              uint32_t* __restrict__ du_len)
+                              const uint32_t* __restrict__ flag, const uint32_t* __restrict__ E, TextSet ts,
+                              uint32_t d, int W, uint32_t* __restrict__ pid, uint32_t* __restrict__ head,
+                              uint32_t* __restrict__ du_start, uint32_t* __restrict__ du_len,
+                              uint32_t* __restrict__ du_tl, uint32_t* __restrict__ du_text)
 {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= d) return;
@@ -521,11 +602,13 @@ __global__ void k_dict_assign(const uint32_t* __restrict__ ids, const uint32_t* 
     const uint32_t k = V[j];
     pid[k] = u;
     if (flag[j]) {
-        uint32_t s, e;
-        phrase_bounds(E, k, W, s, e);
+        uint32_t s, e, tl;
+        phrase_bounds(E, k, W, s, e, tl);
         head[u] = j;
         du_start[u] = s;
         du_len[u] = e - s + 1;
+        du_tl[u] = tl;
+        du_text[u] = ts.text_of(k);
     }
 }
 
@@ -673,13 +756,16 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 //   bit  41      whole phrase (j == 0).
 constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
-// Per dictionary position x (dictionary order): the initial sort key (the first `per` symbols
-// of its suffix as dense codes of `bits` bits, 0 past the phrase end, 1 past the text end),
-// V = x, and its DictPos (record: frequency, preceding character, flags; phrase end).
-__global__ void k_dt_init(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ owner,
+// Per dictionary position x (dictionary order): the initial sort key (the text id of its
+// phrase in the `tbits` bits above the first `per` symbols of its suffix, which are dense codes
+// of `bits` bits, 0 past the phrase end, 1 terminator), V = x, and its DictPos (record: frequency,
+// preceding character, flags; phrase end).  The text id keeps the suffixes of different texts
+// in different groups.
+__global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
                           const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
+                          const uint32_t* __restrict__ tlen, const uint32_t* __restrict__ dtext,
                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
-                          const uint32_t* __restrict__ code, int bits, int per, uint32_t L, int W,
+                          const uint32_t* __restrict__ code, int bits, int per, int tbits, uint32_t L, int W,
                           uint64_t* __restrict__ K, uint32_t* __restrict__ V, DictPos* __restrict__ dp)
 {
     __shared__ uint32_t cs[256];
@@ -691,17 +777,19 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, uint64_t n, const uint3
     const uint32_t o = (uint32_t)off[u];
     const uint32_t j = x - o;
     const uint32_t rem = len[u] - j;
+    const uint32_t tr = (tlen[u] > j) ? tlen[u] - j : 0u;  // text bytes left in the suffix
     const uint64_t tp = (uint64_t)start[u] + j;
     uint64_t key = 0;
     uint64_t w = 0;
     for (int i = 0; i < per; i++) {
-        if ((i & 7) == 0) w = (tp + i < n) ? load8(T, tp + i) : 0ull;
+        if ((i & 7) == 0) w = ((uint32_t)i < tr) ? load8(T, tp + i) : 0ull;
         uint64_t s;
         if ((uint32_t)i >= rem) s = 0;
-        else if (tp + i >= n) s = 1;
+        else if ((uint32_t)i >= tr) s = 1;
         else s = cs[(w >> (8 * (i & 7))) & 0xFF];
         key = (key << bits) | s;
     }
+    if (tbits) key |= (uint64_t)dtext[u] << (bits * per);  // bits * per <= 64 - tbits
     K[x] = key;
     V[x] = x;
     uint64_t r = freq[u];
@@ -1012,18 +1100,6 @@ __global__ void k_phrase_rec(const uint64_t* __restrict__ off, const uint32_t* _
     pr[u] = PhraseRec{a, b - a, cid[rank[o] - 1], 0};
 }
 
-template <typename EntT>
-__device__ __forceinline__ EntT whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ T,
-                                                   uint64_t n, int W, uint32_t k, uint32_t alpha0)
-{
-    uint32_t s, e;
-    phrase_bounds(E, k, W, s, e);
-    const uint8_t c = (s > 0) ? T[s - 1] : T[n - 1];
-    EntT v = Ent<EntT>::make(alpha0, c);
-    if (sizeof(EntT) == 8 && k == 0) v |= (EntT)0x100;
-    return v;
-}
-
 // Per occurrence k (parse order, sequential reads): its whole-phrase entry and its phrase's
 // run of mixed entries.
 template <typename EntT>
@@ -1032,16 +1108,26 @@ struct alignas(16) OccRec {
     uint32_t mx_begin, mx_count;
 };
 
+// whole-phrase entry of occurrence k: its phrase's group above the character before it; with
+// 64-bit entries the first phrase of every text carries the primary-index mark (bit 8)
+template <typename EntT>
+__device__ __forceinline__ EntT whole_phrase_entry(const uint32_t* __restrict__ E, const uint8_t* __restrict__ pchar,
+                                                   uint32_t k, uint32_t alpha0)
+{
+    EntT v = Ent<EntT>::make(alpha0, pchar[k]);
+    if (sizeof(EntT) == 8 && first_of_text(E, k)) v |= (EntT)0x100;
+    return v;
+}
+
 template <typename EntT>
 __global__ void k_occ_rec(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
-                          const uint8_t* __restrict__ pchar, OccRec<EntT>* __restrict__ occ)
+                          const uint8_t* __restrict__ pchar, const uint32_t* __restrict__ E,
+                          OccRec<EntT>* __restrict__ occ)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
     const PhraseRec q = pr[pid[k]];
-    EntT v = Ent<EntT>::make(q.alpha0, pchar[k]);
-    if (sizeof(EntT) == 8 && k == 0) v |= (EntT)0x100;  // the primary-index entry
-    occ[k] = OccRec<EntT>{v, q.mx_begin, q.mx_count};
+    occ[k] = OccRec<EntT>{whole_phrase_entry<EntT>(E, pchar, k, q.alpha0), q.mx_begin, q.mx_count};
 }
 
 // Per parse-suffix rank r (one gather of the occurrence k = psa[r]-1): the number of entries
@@ -1160,14 +1246,16 @@ __global__ __launch_bounds__(kBlock) void k_q_expand(const RankRec<EntT>* __rest
     }
 }
 
-// the last occurrence has no next parse suffix; its whole-phrase group is a singleton
+// The last occurrence has no next parse suffix; its whole-phrase group is a singleton.  (With
+// many texts the last phrase of every other text is reached from the first parse suffix of the
+// next text; its group is a singleton as well, so its place among the entries does not matter.)
 template <typename EntT>
 __global__ void k_q_last(const uint32_t* __restrict__ pid, uint32_t d, const PhraseRec* __restrict__ pr,
-                         const uint32_t* __restrict__ E, const uint8_t* __restrict__ T, uint64_t n, int W,
-                         uint32_t pos, EntT* __restrict__ ent)
+                         const uint32_t* __restrict__ E, const uint8_t* __restrict__ pchar, uint32_t pos,
+                         EntT* __restrict__ ent)
 {
     const uint32_t k = d - 1;
-    ent[pos] = whole_phrase_entry<EntT>(E, T, n, W, k, pr[pid[k]].alpha0);
+    ent[pos] = whole_phrase_entry<EntT>(E, pchar, k, pr[pid[k]].alpha0);
 }
 
 // Primary index from 32-bit entries: 1 + delta(group) + number of entries sorted before the
@@ -1198,21 +1286,37 @@ __global__ void k_q_pidx(const uint32_t* __restrict__ acc, const uint32_t* __res
     *pidx = delta[g0] + tot + 1;
 }
 
-// last radix pass of the entry sort: slot = sorted position + delta(alpha)
+// last radix pass of the entry sort: slot = sorted position + delta(alpha); a marked entry (the
+// first phrase of a text) gives that text's primary index: 1 + its slot inside the text's BWT,
+// the text found by bisection over the output offsets obase[0..texts]
 struct EmitBwt {
     static constexpr double bytes_per_item = 1 + 4;  // BWT byte written, delta read
     static constexpr bool kValues = false;
     const uint32_t* delta;
     uint8_t* O;
-    uint32_t* pidx;
+    uint32_t* pidx;  // [texts]
     uint64_t n;
+    const uint32_t* obase;  // [texts + 1]; nullptr for one text
+    uint32_t texts;
     __device__ __forceinline__ void val(uint64_t, uint32_t) const {}
     __device__ __forceinline__ void key(uint64_t pos, uint64_t key) const
     {
         const uint64_t slot = pos + delta[(uint32_t)(key >> 9)];
         if (slot >= n) return;  // guard: a slot past the output is a bug caught by the checks
         O[slot] = (uint8_t)(key & 0xFF);
-        if (key & 0x100) *pidx = (uint32_t)slot + 1;
+        if (key & 0x100) {
+            if (!obase) {  // one text: its BWT starts at 0
+                pidx[0] = (uint32_t)slot + 1;
+                return;
+            }
+            uint32_t a = 0, b = texts - 1;  // last text with obase <= slot
+            while (a < b) {
+                const uint32_t m = (a + b + 1) >> 1;
+                if (obase[m] <= slot) a = m;
+                else b = m - 1;
+            }
+            pidx[a] = (uint32_t)(slot - obase[a]) + 1;
+        }
     }
 };
 
@@ -1255,13 +1359,36 @@ static int init_key_bits(bool parse = false)
     return v;
 }
 
-// Trigger end positions of the staged text into B_TRIG (room for one more entry, the last
-// phrase end); returns their number.  With `codes` the dense alphabet codes are computed
-// in the same round trip (codes[256] = sigma).
-uint32_t pfp_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, uint32_t extra, uint32_t** E_out,
-                  double* bytes, uint32_t* codes, uint32_t* sigma)
+// Host description of the texts of one build: one text of n1 bytes at 0, or `count` texts laid
+// out as TextSet describes (device arrays tbase, tlen, obase, sub_text).
+struct BuildTexts {
+    uint32_t count = 1;
+    uint64_t staged = 0;  // staged length (a multiple of kTextAlign, >= every text end + kTextGap)
+    uint64_t total = 0;   // sum of the text lengths = output length
+    uint32_t n1 = 0;
+    const uint32_t* tbase = nullptr;
+    const uint32_t* tlen = nullptr;
+    const uint32_t* obase = nullptr;
+    const uint32_t* sub_text = nullptr;
+    uint32_t* pidx = nullptr;  // [count] primary indices (device, count > 1)
+};
+
+inline uint64_t staged_length_one(uint64_t n) { return (n + kTextGap + kTextAlign - 1) / kTextAlign * kTextAlign; }
+
+// flags[k] = 1 where phrase k ends a text (its end carries kSentinel)
+__global__ void k_sent_flags(const uint32_t* __restrict__ E, uint32_t d, uint32_t* __restrict__ f)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < d) f[k] = E[k] >> 31;
+}
+
+// Phrase ends of the staged texts into B_TRIG: every trigger end and every text's sentinel end
+// (flagged), in text order; returns their number d (the parse length).
+uint32_t pfp_scan(Storage& S, const uint8_t* T, const BuildTexts& bt, int W, uint32_t P, uint32_t** E_out,
+                  double* bytes)
 {
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    const uint64_t n = bt.staged;
     uint32_t t = 0;
     uint32_t* E = nullptr;
     double by = 0;
@@ -1270,45 +1397,39 @@ uint32_t pfp_scan(Storage& S, const uint8_t* T, uint64_t n, int W, uint32_t P, u
         // with the exact size when a text has more triggers
         uint32_t cap = (uint32_t)std::min<uint64_t>(n, n / 8 + 65536);
         for (int pass = 0;; pass++) {
-            E = S.buf<uint32_t>(B_TRIG, (size_t)cap + extra);
+            E = S.buf<uint32_t>(B_TRIG, (size_t)cap + 1);
             const int kt = S.kt_begin(2, (double)n);
-            trig_scan(S, T, n, W, P, E, cap, sc + SC_TRIGGERS);
+            trig_scan(S, T, n, bt.n1, bt.sub_text, bt.tbase, bt.tlen, W, P, E, cap, sc + SC_TRIGGERS);
             S.kt_end(kt);
-            if (pass == 0 && codes) {
-                alphabet_codes(S, T, n, codes);
-                MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice,
-                                         S.stream));
-            }
             uint32_t hv[SC_COUNT_];
             S.read_fast(hv, sc, SC_COUNT_);
             t = hv[SC_TRIGGERS];
-            if (codes) *sigma = hv[SC_SIGMA];
-            if (kt >= 0) S.kt_bytes[kt] = (double)n + 4.0 * std::min(t, cap);
-            by += (double)n + 4.0 * std::min(t, cap);
+            if (kt >= 0) S.kt_bytes[kt] = (double)bt.total + 4.0 * std::min(t, cap);
+            by += (double)bt.total + 4.0 * std::min(t, cap);
             if (t <= cap) break;
             MIBWT_REQUIRE(pass == 0, LIBCUBWT_GPU_ERROR, "trigger count changed between scans");
             cap = t;
         }
     } else {
-        // windows wider than the exact-value range: two passes over per-thread masks
-        const uint32_t tiles = ceil_div(n + 1, kTrigTile);
+        // windows wider than the exact-value range (one text only): two passes over per-thread
+        // masks, then the sentinel end
+        MIBWT_REQUIRE(bt.count == 1, LIBCUBWT_BAD_PARAMETER, "batched builds need W <= 56");
+        const uint64_t n1 = bt.n1;
+        const uint32_t tiles = ceil_div(n1 + 1, kTrigTile);
         uint16_t* masks = S.buf<uint16_t>(B_TRIG_MASK, (size_t)tiles * kBlock);
         uint32_t* tcnt = S.buf<uint32_t>(B_TILE_CNT, tiles + 1);
-        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n, W, P, masks, tcnt);
+        k_trig_mask<<<tiles, kBlock, 0, S.stream>>>(T, n1, W, P, masks, tcnt);
         MIBWT_LAUNCH_CHECK();
         exclusive_sum<uint32_t>(S, tcnt, tcnt, tiles, sc + SC_TRIGGERS);
-        if (codes) {
-            alphabet_codes(S, T, n, codes);
-            MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
-        }
         uint32_t hv[SC_COUNT_];
         S.read_fast(hv, sc, SC_COUNT_);
         t = hv[SC_TRIGGERS];
-        if (codes) *sigma = hv[SC_SIGMA];
-        E = S.buf<uint32_t>(B_TRIG, (size_t)t + extra);
+        E = S.buf<uint32_t>(B_TRIG, (size_t)t + 1);
         k_trig_emit<<<tiles, kBlock, 0, S.stream>>>(masks, tcnt, E);
         MIBWT_LAUNCH_CHECK();
-        by = (double)n + 2.0 * tiles * kBlock * 2 + 4.0 * t;
+        fill<uint32_t>(S, E + t, 1, (uint32_t)(n1 + W - 1) | kSentinel);
+        t += 1;
+        by = (double)n1 + 2.0 * tiles * kBlock * 2 + 4.0 * t;
     }
     *E_out = E;
     *bytes = by;
@@ -1324,9 +1445,14 @@ int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n64, int W, int
     const uint64_t n = (uint64_t)n64;
     S.phases_reset();
     uint8_t* T = stage_text(S, src, true, n);
+    BuildTexts bt;
+    bt.count = 1;
+    bt.staged = staged_length_one(n);
+    bt.total = n;
+    bt.n1 = (uint32_t)n;
     uint32_t* E = nullptr;
     double bytes = 0;
-    const uint32_t t = pfp_scan(S, T, n, W, (uint32_t)P, 1, &E, &bytes, nullptr, nullptr);
+    const uint32_t t = pfp_scan(S, T, bt, W, (uint32_t)P, &E, &bytes) - 1;  // without the sentinel end
     const int64_t m = std::min<int64_t>((int64_t)t, cap);
     if (m > 0) {
         MIBWT_HIP(hipMemcpyAsync(Eh, E, sizeof(uint32_t) * m, hipMemcpyDeviceToHost, S.stream));
@@ -1336,7 +1462,6 @@ int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n64, int W, int
     return (int64_t)t;
 }
 
-
 // rank[k] = lexicographic rank of the phrase of occurrence k
 __global__ void k_occ_lexrank(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid, uint32_t d,
                               uint32_t* __restrict__ out)
@@ -1345,50 +1470,33 @@ __global__ void k_occ_lexrank(const uint32_t* __restrict__ lexrank, const uint32
     if (k < d) out[k] = lexrank[pid[k]];
 }
 
-int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P,
-                    PfpParseOut* parse_out)
+// The build of the staged texts T (B_TEXT) into dO: one text (bt.count == 1, returns its
+// primary index) or a batch (returns 0; the primary indices land in pidx_out[count]).
+int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int W, int P, PfpParseOut* parse_out,
+                  int64_t* pidx_out)
 {
-    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
-    MIBWT_REQUIRE(W >= 1 && W <= kMaxW && P >= 1 && P <= 65535, LIBCUBWT_BAD_PARAMETER, "bad W/P");
-    const uint64_t n = (uint64_t)n64;
-    S.phases_reset();
-    S.stats.n = n64;
-    S.stats.W = W;
-    S.stats.P = P;
-    const bool timed = !S.arena;  // arena mode creates and records no events
-    if (timed && !S.t_begin) {
-        MIBWT_HIP(hipEventCreate(&S.t_begin));
-        MIBWT_HIP(hipEventCreate(&S.t_end));
-    }
-    auto mark_begin = [&]() {
-        if (timed) MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
-    };
+    const uint64_t n = bt.total;
+    const bool multi = bt.count > 1;
+    const bool timed = !S.arena;  // arena mode records no events
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    // the text: a padded private copy (src == nullptr: already staged by the caller); a copy
-    // from device memory is part of the timed build, an upload from the host is not
-    uint8_t* T;
-    if (src == nullptr) {
-        T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
-        mark_begin();
-    } else if (src_host) {
-        T = stage_text(S, src, true, n);
-        mark_begin();
-    } else {
-        mark_begin();
-        T = stage_text(S, src, false, n);
-    }
-
     // ---- 1. trigger scan
     int ph = S.phase_begin("scan");
     uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
     uint32_t sigma = 0;
     double scan_bytes = 0;
     uint32_t* E = nullptr;
-    const uint32_t t = pfp_scan(S, T, n, W, (uint32_t)P, 1, &E, &scan_bytes, nullptr, nullptr);
-    const uint32_t d = t + 1;
-    fill<uint32_t>(S, E + t, 1, (uint32_t)(n + W - 1));
-    S.phase_end(ph, scan_bytes + 4.0);
-    S.stats.triggers = t;
+    const uint32_t d = pfp_scan(S, T, bt, W, (uint32_t)P, &E, &scan_bytes);
+    TextSet ts{bt.count, bt.tbase, bt.tlen, bt.obase, nullptr, bt.n1};
+    if (multi) {
+        // text of every phrase occurrence: the number of sentinel ends before it
+        uint32_t* ptext = S.buf<uint32_t>(B_PTEXT, d);
+        k_sent_flags<<<grid_for(d), kBlock, 0, S.stream>>>(E, d, ptext);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, ptext, ptext, d);
+        ts.ptext = ptext;
+    }
+    S.phase_end(ph, scan_bytes);
+    S.stats.triggers = d - bt.count;
     S.stats.phrases = d;
 
     // ---- 2/3. phrase fingerprints and the dictionary
@@ -1414,14 +1522,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     }
     int dict_passes = 0;
     uint32_t* pid = nullptr;
-    uint32_t *head = nullptr, *dstart = nullptr, *dlen = nullptr, *freq = nullptr;
+    uint32_t *head = nullptr, *dstart = nullptr, *dlen = nullptr, *dtl = nullptr, *dtext = nullptr, *freq = nullptr;
     uint64_t* doff = nullptr;
     uint64_t L64 = 0;
     uint8_t* pchar = S.buf<uint8_t>(B_PH_CHAR, d);
     for (int attempt = 0;; attempt++) {
         MIBWT_REQUIRE(attempt < 6, MIBWT_HASH_COLLISION, "phrase fingerprint collisions persist");
         const uint64_t hmask = (attempt == 0 && hash_bits < 64) ? ((1ull << hash_bits) - 1) : ~0ull;
-        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, seed, hmask, K0, V0, pchar);
+        k_phrase_hash<<<grid_for(d), kBlock, 0, S.stream>>>(T, ts, W, E, d, seed, hmask, K0, V0, pchar);
         MIBWT_LAUNCH_CHECK();
         radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, d, 0, key_bits);
         dict_passes += key_bits / 8;
@@ -1442,10 +1550,14 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         head = S.buf<uint32_t>(B_DU_HEAD, du);
         dstart = S.buf<uint32_t>(B_DU_START, du);
         dlen = S.buf<uint32_t>(B_DU_LEN, du);
+        dtl = S.buf<uint32_t>(B_DU_TL, du);
+        dtext = S.buf<uint32_t>(B_DU_TEXT, du);
         freq = S.buf<uint32_t>(B_DU_FREQ, du);
-        k_dict_assign<<<grid_for(d), kBlock, 0, S.stream>>>(ids, V0, flag, E, d, W, pid, head, dstart, dlen);
+        k_dict_assign<<<grid_for(d), kBlock, 0, S.stream>>>(ids, V0, flag, E, ts, d, W, pid, head, dstart, dlen, dtl,
+                                                             dtext);
         MIBWT_LAUNCH_CHECK();
-        k_dict_verify<<<grid_for(d), kBlock, 0, S.stream>>>(T, n, W, E, d, pid, dstart, dlen, sc + SC_COLLISION);
+        k_dict_verify<<<grid_for(d), kBlock, 0, S.stream>>>(T, ts, W, E, d, pid, dstart, dlen, dtl, dtext,
+                                                             sc + SC_COLLISION);
         MIBWT_LAUNCH_CHECK();
         k_dict_freq<<<grid_for(du), kBlock, 0, S.stream>>>(head, du, d, freq);
         MIBWT_LAUNCH_CHECK();
@@ -1455,7 +1567,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         exclusive_sum<uint64_t>(S, dlen64, doff, du + 1);
         // alphabet of the dictionary = alphabet of the text (the phrases cover it)
-        alphabet_codes(S, T, n, codes, dstart, dlen, du);
+        alphabet_codes(S, T, bt.staged, codes, dstart, dtl, du);
         // one read-back: collision flag, dictionary length, alphabet size
         MIBWT_HIP(hipMemcpyAsync(sc + SC_DICTLEN, doff + du, sizeof(uint64_t), hipMemcpyDeviceToDevice, S.stream));
         MIBWT_HIP(hipMemcpyAsync(sc + SC_SIGMA, codes + 256, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
@@ -1488,12 +1600,15 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     // alphabet variant packs two symbols per radix digit, PFPs16:995-1002; here a 64-bit key
     // holds as many codes as fit, 21 for DNA)
     const int abits = bits_for((uint64_t)sigma + 1);
-    // initial key width: whole radix passes trade against one more doubling round
+    // initial key width: whole radix passes trade against one more doubling round; with many
+    // texts the text id takes the bits above the symbols
+    const int tbits = multi ? bits_for(bt.count - 1) : 0;
     const int key_cap = init_key_bits();
-    const int aper = std::max(1, key_cap / abits);
+    const int aper = std::max(1, (key_cap - tbits) / abits);
+    MIBWT_REQUIRE(tbits + abits * aper <= 64, LIBCUBWT_BAD_PARAMETER, "too many texts for one build");
     DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
-    k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, n, owner, doff, dlen, dstart, freq, codes, abits, aper, L,
-                                                     W, K0, V0, dp);
+    k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dtl, dtext, dstart, freq, codes, abits, aper,
+                                                     tbits, L, W, K0, V0, dp);
     MIBWT_LAUNCH_CHECK();
     // per dictionary position a record (frequency, preceding character, flags); the sort
     // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
@@ -1502,8 +1617,9 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
     uint32_t* wbits = S.buf<uint32_t>(B_AS_WBITS, wwords);
     fill<uint32_t>(S, wbits, wwords, 0u);
     LimDict limd{dp};
-    DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, abits * aper, (uint32_t)aper,
-                                                           limd, drank, nullptr, PlaceRec{dp, RH, wbits});
+    DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, tbits + abits * aper,
+                                                           (uint32_t)aper, limd, drank, nullptr,
+                                                           PlaceRec{dp, RH, wbits});
     S.phase_end(ph, rd.bytes);
     S.stats.doubling_rounds_dict = rd.rounds;
 
@@ -1608,7 +1724,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
         MIBWT_LAUNCH_CHECK();
         OccRec<EntT>* occ = reinterpret_cast<OccRec<EntT>*>(S.buf<uint4>(B_Q_OCC, d));
         RankRec<EntT>* qrec = reinterpret_cast<RankRec<EntT>*>(S.buf<uint4>(B_Q_RREC, k32 ? (d + 1) / 2 : d));
-        k_occ_rec<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(pid, d, prec, pchar, occ);
+        k_occ_rec<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(pid, d, prec, pchar, E, occ);
         MIBWT_LAUNCH_CHECK();
         k_q_count<EntT><<<grid_for(d), kBlock, 0, S.stream>>>(psa, d, occ, qc, qrec);
         MIBWT_LAUNCH_CHECK();
@@ -1636,7 +1752,7 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
             k_q_expand<EntT><<<qblocks, kBlock, 0, S.stream>>>(qrec, d, mx, qo, rb, nq, ent0, special, part);
             MIBWT_LAUNCH_CHECK();
         }
-        k_q_last<EntT><<<1, 1, 0, S.stream>>>(pid, d, prec, E, T, n, W, ne - 1, ent0);
+        k_q_last<EntT><<<1, 1, 0, S.stream>>>(pid, d, prec, E, pchar, ne - 1, ent0);
         MIBWT_LAUNCH_CHECK();
         if constexpr (k32) {
             fill<uint32_t>(S, sc + SC_TOTAL, 1, 0u);
@@ -1651,14 +1767,16 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
             radix_sort<uint32_t, uint32_t, EmitBwt32>(S, ent0, ent1, nullptr, nullptr, ne, 8, 8 + gbits, &em, nullptr,
                                                       eh);
         } else {
-            fill<uint32_t>(S, sc + SC_PIDX, 1, 0u);
-            EmitBwt em{deltac, dO, sc + SC_PIDX, n};
+            uint32_t* pidx_dev = multi ? bt.pidx : sc + SC_PIDX;
+            fill<uint32_t>(S, pidx_dev, bt.count, 0u);
+            EmitBwt em{deltac, dO, pidx_dev, n, multi ? bt.obase : nullptr, bt.count};
             radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em, nullptr,
                                                     eh);
         }
     };
     // MIBWT_WIDE_ENTRIES=1 takes the 64-bit entries (with the in-sort primary index) at any size
-    const bool wide = gbits > 24 || (getenv("MIBWT_WIDE_ENTRIES") && atoi(getenv("MIBWT_WIDE_ENTRIES")) == 1);
+    // (batches always: the primary-index marks go through the sort)
+    const bool wide = multi || gbits > 24 || (getenv("MIBWT_WIDE_ENTRIES") && atoi(getenv("MIBWT_WIDE_ENTRIES")) == 1);
     if (!wide) assemble(uint32_t{});
     else assemble(uint64_t{});
     const double esize = wide ? 8.0 : 4.0;
@@ -1668,18 +1786,207 @@ int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, 
 
     if (timed) MIBWT_HIP(hipEventRecord(S.t_end, S.stream));
     uint32_t pidx = 0;
-    S.read(&pidx, sc + SC_PIDX);
+    if (multi) {
+        std::vector<uint32_t> h(bt.count);
+        MIBWT_HIP(hipMemcpyAsync(h.data(), bt.pidx, sizeof(uint32_t) * bt.count, hipMemcpyDeviceToHost, S.stream));
+        MIBWT_HIP(hipStreamSynchronize(S.stream));
+        for (uint32_t t = 0; t < bt.count; t++) pidx_out[t] = h[t];
+    } else {
+        S.read(&pidx, sc + SC_PIDX);
+    }
     S.phases_collect();
     float tot = 0;
     if (timed) MIBWT_HIP(hipEventElapsedTime(&tot, S.t_begin, S.t_end));
     S.stats.total_ms = tot;
-    MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
-    S.last_text = src_host ? nullptr : src;
-    S.last_n = n64;
     S.last_W = W;
     S.last_P = P;
     S.last_sort_items = d;
+    if (multi) return 0;
+    MIBWT_REQUIRE(pidx >= 1 && pidx <= n, LIBCUBWT_GPU_ERROR, "primary index not produced");
     return (int64_t)pidx;
+}
+
+int64_t pfp_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64, int W, int P,
+                    PfpParseOut* parse_out)
+{
+    MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
+    MIBWT_REQUIRE(W >= 1 && W <= kMaxW && P >= 1 && P <= 65535, LIBCUBWT_BAD_PARAMETER, "bad W/P");
+    const uint64_t n = (uint64_t)n64;
+    S.phases_reset();
+    S.stats.n = n64;
+    S.stats.W = W;
+    S.stats.P = P;
+    const bool timed = !S.arena;  // arena mode creates and records no events
+    if (timed && !S.t_begin) {
+        MIBWT_HIP(hipEventCreate(&S.t_begin));
+        MIBWT_HIP(hipEventCreate(&S.t_end));
+    }
+    auto mark_begin = [&]() {
+        if (timed) MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    };
+    // the text: a padded private copy (src == nullptr: already staged by the caller); a copy
+    // from device memory is part of the timed build, an upload from the host is not
+    uint8_t* T;
+    if (src == nullptr) {
+        T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
+        mark_begin();
+    } else if (src_host) {
+        T = stage_text(S, src, true, n);
+        mark_begin();
+    } else {
+        mark_begin();
+        T = stage_text(S, src, false, n);
+    }
+    BuildTexts bt;
+    bt.count = 1;
+    bt.staged = staged_length_one(n);
+    bt.total = n;
+    bt.n1 = (uint32_t)n;
+    const int64_t idx = pfp_build(S, T, bt, dO, W, P, parse_out, nullptr);
+    S.last_text = src_host ? nullptr : src;
+    S.last_n = n64;
+    return idx;
+}
+
+// Staged layout of a batch: text t at base[t] (see TextSet), sub_text[] = text of every
+// kTextAlign-byte sub-tile.
+__global__ void k_stage_texts(const uint8_t* __restrict__ src, const uint32_t* __restrict__ sub_text,
+                              const uint32_t* __restrict__ tbase, const uint32_t* __restrict__ tlen,
+                              const uint32_t* __restrict__ obase, uint64_t staged, uint8_t* __restrict__ T)
+{
+    const uint64_t q = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16;  // 16 staged bytes
+    if (q >= staged) return;
+    const uint32_t t = sub_text[q / kTextAlign];
+    const uint64_t b = tbase[t], len = tlen[t], so = obase[t];
+    uint32_t w[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const uint64_t l = q + 4 * j + i - b;  // q >= b: a sub-tile never starts before its text
+            if (l < len) v |= (uint32_t)src[so + l] << (8 * i);
+        }
+        w[j] = v;
+    }
+    *reinterpret_cast<uint4*>(T + q) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// PFP BWT of k device texts at dT + offs[i] (offs[k] = end of the last) into dO + offs[i]
+// (reference format), in builds of consecutive texts whose staged layout stays below
+// `chunk_bytes`; index[i] receives text i's primary index.  W = P = 0: the selector chooses per
+// build on the staged texts.  A build whose dictionary outgrows the index range or the memory
+// is split in two.
+void pfp_batch_range(Storage& S, const uint8_t* dT, const int64_t* offs, int64_t a, int64_t b, uint8_t* dO, int W,
+                     int P, int64_t* index);
+void selector_choose_device(Storage& S, const uint8_t* T, uint64_t n, int* W, int* P, float* pred, uint8_t* copy);
+
+int64_t pfp_batch_run(Storage& S, const uint8_t* dT, const int64_t* offs, int64_t k, uint8_t* dO, int W, int P,
+                      int64_t* index, uint64_t chunk_bytes)
+{
+    int64_t a = 0;
+    while (a < k) {
+        // consecutive texts up to the chunk budget (at least one)
+        uint64_t staged = 0;
+        int64_t b = a;
+        while (b < k) {
+            const uint64_t nt = (uint64_t)(offs[b + 1] - offs[b]);
+            const uint64_t next = (staged + nt + kTextGap + kTextAlign - 1) / kTextAlign * kTextAlign;
+            if (b > a && next > chunk_bytes) break;
+            staged = next;
+            b++;
+        }
+        pfp_batch_range(S, dT, offs, a, b, dO, W, P, index);
+        a = b;
+    }
+    return LIBCUBWT_NO_ERROR;
+}
+
+void pfp_batch_range(Storage& S, const uint8_t* dT, const int64_t* offs, int64_t a, int64_t b, uint8_t* dO, int W,
+                     int P, int64_t* index)
+{
+    const int64_t k = b - a;
+    if (k == 1) {
+        // one text: the single-text build (no batch layout)
+        int w = W, p = P;
+        const uint64_t n = (uint64_t)(offs[a + 1] - offs[a]);
+        if (W == 0) {
+            const uint64_t bytes = staged_text_bytes(n);
+            uint8_t* T = S.buf<uint8_t>(B_TEXT, bytes);
+            MIBWT_HIP(hipMemsetAsync(T + n, 0, bytes - n, S.stream));
+            selector_choose_device(S, dT + offs[a], n, &w, &p, nullptr, T);
+            index[a] = pfp_bwt_run(S, nullptr, false, dO + offs[a], (int64_t)n, w, p, nullptr);
+        } else {
+            index[a] = pfp_bwt_run(S, dT + offs[a], false, dO + offs[a], (int64_t)n, w, p, nullptr);
+        }
+        return;
+    }
+    // host tables of the layout
+    std::vector<uint32_t> tb(k), tn(k), ob(k + 1);
+    uint64_t staged = 0;
+    for (int64_t t = 0; t < k; t++) {
+        const uint64_t nt = (uint64_t)(offs[a + t + 1] - offs[a + t]);
+        tb[t] = (uint32_t)staged;
+        tn[t] = (uint32_t)nt;
+        ob[t] = (uint32_t)(offs[a + t] - offs[a]);
+        staged = (staged + nt + kTextGap + kTextAlign - 1) / kTextAlign * kTextAlign;
+    }
+    ob[k] = (uint32_t)(offs[b] - offs[a]);
+    MIBWT_REQUIRE(staged < (uint64_t)INT32_MAX - (1u << 20), LIBCUBWT_BAD_PARAMETER, "batch build too large");
+    const uint64_t subs = staged / kTextAlign;
+    std::vector<uint32_t> st(subs);
+    for (int64_t t = 0; t < k; t++) {
+        const uint64_t s0 = tb[t] / kTextAlign;
+        const uint64_t s1 = (t + 1 < k) ? tb[t + 1] / kTextAlign : subs;
+        for (uint64_t q = s0; q < s1; q++) st[q] = (uint32_t)t;
+    }
+    uint32_t* dtab = S.buf<uint32_t>(B_BT_TABLES, 3 * (size_t)k + 1 + subs);
+    uint32_t* d_tbase = dtab;
+    uint32_t* d_tlen = dtab + k;
+    uint32_t* d_obase = dtab + 2 * k;
+    uint32_t* d_sub = dtab + 3 * k + 1;
+    MIBWT_HIP(hipMemcpyAsync(d_tbase, tb.data(), 4 * k, hipMemcpyHostToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(d_tlen, tn.data(), 4 * k, hipMemcpyHostToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(d_obase, ob.data(), 4 * (k + 1), hipMemcpyHostToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(d_sub, st.data(), 4 * subs, hipMemcpyHostToDevice, S.stream));
+    uint32_t* d_pidx = S.buf<uint32_t>(B_BT_PIDX, (size_t)k);
+    S.phases_reset();
+    S.stats.n = (int64_t)ob[k];
+    const bool timed = !S.arena;
+    if (timed && !S.t_begin) {
+        MIBWT_HIP(hipEventCreate(&S.t_begin));
+        MIBWT_HIP(hipEventCreate(&S.t_end));
+    }
+    if (timed) MIBWT_HIP(hipEventRecord(S.t_begin, S.stream));
+    uint8_t* T = S.buf<uint8_t>(B_TEXT, staged + 8192);
+    MIBWT_HIP(hipMemsetAsync(T + staged, 0, 8192, S.stream));
+    k_stage_texts<<<grid_for(ceil_div(staged, 16)), kBlock, 0, S.stream>>>(dT + offs[a], d_sub, d_tbase, d_tlen,
+                                                                           d_obase, staged, T);
+    MIBWT_LAUNCH_CHECK();
+    int w = W, p = P;
+    if (W == 0) selector_choose_device(S, T, staged, &w, &p, nullptr, nullptr);
+    S.stats.W = w;
+    S.stats.P = p;
+    BuildTexts bt;
+    bt.count = (uint32_t)k;
+    bt.staged = staged;
+    bt.total = ob[k];
+    bt.tbase = d_tbase;
+    bt.tlen = d_tlen;
+    bt.obase = d_obase;
+    bt.sub_text = d_sub;
+    bt.pidx = d_pidx;
+    try {
+        pfp_build(S, T, bt, dO + offs[a], w, p, nullptr, index + a);
+    } catch (const Error& e) {
+        // a dictionary beyond the 32-bit index range or the memory: two smaller builds
+        if (e.code != MIBWT_INDEX_RANGE && e.code != LIBCUBWT_GPU_NOT_ENOUGH_MEMORY) throw;
+        (void)hipGetLastError();
+        MIBWT_HIP(hipStreamSynchronize(S.stream));
+        const int64_t m = a + k / 2;
+        pfp_batch_range(S, dT, offs, a, m, dO, W, P, index);
+        pfp_batch_range(S, dT, offs, m, b, dO, W, P, index);
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy16(const uint4* __restrict__ a, uint4* __restrict__ b, uint64_t m)
@@ -1722,9 +2029,13 @@ double pfp_time_scan(Storage& S, int reps, double* bytes)
     hipEvent_t a, b;
     MIBWT_HIP(hipEventCreate(&a));
     MIBWT_HIP(hipEventCreate(&b));
-    trig_scan(S, T, n, S.last_W, (uint32_t)S.last_P, E, cap, sc + SC_TRIGGERS);
+    const uint64_t staged = staged_length_one(n);
+    trig_scan(S, T, staged, (uint32_t)n, nullptr, nullptr, nullptr, S.last_W, (uint32_t)S.last_P, E, cap,
+              sc + SC_TRIGGERS);
     MIBWT_HIP(hipEventRecord(a, S.stream));
-    for (int i = 0; i < reps; i++) trig_scan(S, T, n, S.last_W, (uint32_t)S.last_P, E, cap, sc + SC_TRIGGERS);
+    for (int i = 0; i < reps; i++)
+        trig_scan(S, T, staged, (uint32_t)n, nullptr, nullptr, nullptr, S.last_W, (uint32_t)S.last_P, E, cap,
+                  sc + SC_TRIGGERS);
     MIBWT_HIP(hipEventRecord(b, S.stream));
     MIBWT_HIP(hipEventSynchronize(b));
     float ms = 0;

@@ -159,9 +159,10 @@ inline void alphabet_codes(Storage& S, const uint8_t* T, uint64_t n, uint32_t* c
     MIBWT_LAUNCH_CHECK();
 }
 
-// Private padded copy of the text in B_TEXT: T[0..n) followed by zero bytes up to a 4096-byte
-// multiple plus kPad (tile loads and unaligned word loads stay inside the allocation).
-inline uint64_t staged_text_bytes(uint64_t n) { return ((n + 1 + 4095) / 4096) * 4096 + kPad + 4096; }
+// Private padded copy of the text in B_TEXT: T[0..n) followed by zero bytes to the next
+// 8192-byte multiple at least 64 bytes past n (the trigger scan's staged length), then 8192
+// more (tile loads and unaligned word loads stay inside the allocation).
+inline uint64_t staged_text_bytes(uint64_t n) { return ((n + 64 + 8191) / 8192) * 8192 + 8192; }
 
 inline uint8_t* stage_text(Storage& S, const uint8_t* src, bool src_host, uint64_t n)
 {
