@@ -99,6 +99,8 @@ def main():
     ap.add_argument("--control-steps", type=int, default=3)
     ap.add_argument("--lyndon-steps", type=int, default=1, help="timed Lyndon-grammar builds (0: skip)")
     ap.add_argument("--pcie-steps", type=int, default=2, help="timed host-buffer builds (0: skip)")
+    ap.add_argument("--tuning-steps", type=int, default=5,
+                    help="timed fixed builds with the default launch shapes (0: skip)")
     ap.add_argument("--grid-steps", type=int, default=0,
                     help="also time every (W,P) candidate with this many builds (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -184,6 +186,29 @@ def main():
         if O_fixed is not None and not (dO.download() == O_fixed).all():
             log("BWT differs between W=5,P=16 and W=35,P=81")
             sys.exit(1)
+
+    # launch shapes: the autotuner's table (tuning_table.txt, when present) against the defaults
+    tuning = {"table": os.path.exists(mibwt.TUNING_PATH), "shape": mibwt.tuning_shape(n)}
+    if a.tuning_steps > 0:
+        mibwt.tuning_use(False)
+        tuning["default_shape"] = mibwt.tuning_shape(n)
+        ms_def, idx_def = run(step_fixed, a.tuning_steps, 1, f"fixed W={a.W},P={a.P}, default shapes")
+        mibwt.tuning_use(True)
+        assert idx_def == idx
+        tuning.update(default_ms_per_step=round(ms_def, 4), tuned_ms_per_step=round(ms_fixed, 4),
+                      speedup=round(ms_def / ms_fixed, 4))
+        # where the shapes matter most: a short text (1 MiB of the workload), tuned against default
+        m = 1 << 20
+        small = {}
+        for label, on in (("default", False), ("tuned", True)):
+            mibwt.tuning_use(on)
+            small[label + "_shape"] = mibwt.tuning_shape(m)
+            ms_s, _ = run(lambda: st.bwt_device(dT.address, dO.address, m, "pfp", a.W, a.P), 20, 2,
+                          f"1 MiB prefix, {label} shapes", size=m)
+            small[label + "_ms_per_step"] = round(ms_s, 4)
+        mibwt.tuning_use(True)
+        small["speedup"] = round(small["default_ms_per_step"] / small["tuned_ms_per_step"], 4)
+        tuning["short_text_1mib"] = small
 
     # Lyndon-grammar BWT of the same text
     lyndon = None
@@ -389,6 +414,7 @@ def main():
                      "phases": {p["name"]: round(p["ms"], 3) for p in stats_fixed["phases"]}},
         "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
         "w35_p81": w35,
+        "tuning": tuning,
         "control": control,
         "grid": grid,
         "lyndon": lyndon,

@@ -249,6 +249,27 @@ int64_t mibwt_mlp_train_ensemble(void* storage, const float* X, const float* Y, 
                                  int32_t batch, float lr, uint32_t seed, int32_t members, const char* out_path,
                                  float* final_loss);
 
+/* ---------------------------------------------------------------- launch-shape autotuner
+ * The reference fixes its tiles at compile time for every input size
+ * (Prefix_Free_Parsing_W=5_P=16_kommentiert.cu:16-23).  Here every build takes its launch shapes
+ * (trigger-scan sub-tiles per tile, radix items per thread, chained-scan items per thread,
+ * width of the packed initial sort keys) from a table with one row per length bucket
+ * 2^16 .. 2^31 bytes, read at the first build from tuning_table.txt next to lib/ (or
+ * $MIBWT_TUNING; "off": the defaults; $MIBWT_TUNE_FORCE="sub,items,chain,keybits" forces one
+ * shape).  The BWT does not depend on the shapes. */
+
+/* Measure the table on the device for every bucket up to max_bytes (<= the storage's
+ * max_length): synthetic repetitive DNA of 3/4 of each bucket built with W=5,P=16 under each
+ * candidate shape (coordinate descent, best of `reps` HIP-event timings); installs the table and
+ * writes it to out_path (may be NULL).  Returns the number of buckets tuned. */
+int64_t mibwt_autotune(void* storage, int64_t max_bytes, int32_t reps, const char* out_path);
+/* Load a table file; use the table (on != 0) or the default shapes. */
+int64_t mibwt_tuning_load(const char* path);
+int64_t mibwt_tuning_use(int32_t on);
+/* The shape a build of `size` bytes uses: shape[0..3] = scan sub-tiles, radix items, chained-scan
+ * items, initial key bits. */
+int64_t mibwt_tuning_get(int64_t size, int32_t* shape);
+
 /* Device-memory plumbing for callers that keep texts resident in HBM without another HIP
  * runtime in the process (no counterpart in the reference, whose programs allocate inside
  * main).  Current device; synchronous copies; kind: 0 host->device, 1 device->host,

@@ -63,6 +63,10 @@ EXPORTED = [
     "mibwt_mlp_predict",
     "mibwt_mlp_train",
     "mibwt_mlp_train_ensemble",
+    "mibwt_autotune",
+    "mibwt_tuning_load",
+    "mibwt_tuning_use",
+    "mibwt_tuning_get",
     "mibwt_device_alloc",
     "mibwt_device_free",
     "mibwt_copy",
@@ -149,6 +153,10 @@ def lib():
                                       ctypes.c_char_p, fp]
         L.mibwt_mlp_train_ensemble.argtypes = [vp, fp, fp, i64, i32, i32, ctypes.c_float, ctypes.c_uint32, i32,
                                                ctypes.c_char_p, fp]
+        L.mibwt_autotune.argtypes = [vp, i64, i32, ctypes.c_char_p]
+        L.mibwt_tuning_load.argtypes = [ctypes.c_char_p]
+        L.mibwt_tuning_use.argtypes = [i32]
+        L.mibwt_tuning_get.argtypes = [i64, i32p]
         L.mibwt_device_alloc.argtypes = [ctypes.POINTER(vp), i64]
         L.mibwt_device_free.argtypes = [vp]
         L.mibwt_copy.argtypes = [vp, vp, i64, i32]
@@ -379,6 +387,13 @@ class Storage:
             raise BwtError(rc, "mibwt_batch_bwt_device")
         return idx[:k]
 
+    def autotune(self, max_bytes: int, reps: int = 2, out_path: str | None = None) -> int:
+        """Measure the launch-shape table up to max_bytes on the device (mibwt_autotune)."""
+        rc = lib().mibwt_autotune(self.handle, int(max_bytes), int(reps), out_path.encode() if out_path else None)
+        if rc < 0:
+            raise BwtError(rc, "mibwt_autotune")
+        return int(rc)
+
     # ---- learned (W,P) selector
     def wp_features(self, dT_ptr: int, n: int) -> np.ndarray:
         X = np.zeros((NUM_CAND, 64), dtype=np.float32)
@@ -540,6 +555,22 @@ def bwt_device_noalloc(dT_ptr: int, dO_ptr: int, n: int, W: int = 5, P: int = 16
     if rc < 0:
         raise BwtError(rc, "mibwt_pfp_bwt_device_noalloc")
     return int(rc)
+
+
+TUNING_PATH = os.path.join(_HERE, "tuning_table.txt")
+
+
+def tuning_shape(size: int) -> dict:
+    """Launch shapes a build of `size` bytes uses (the autotuner's table or the defaults)."""
+    sh = np.zeros(4, dtype=np.int32)
+    rc = lib().mibwt_tuning_get(int(size), sh.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    if rc != NO_ERROR:
+        raise BwtError(rc, "mibwt_tuning_get")
+    return {"scan_sub": int(sh[0]), "sort_items": int(sh[1]), "chain_items": int(sh[2]), "key_bits": int(sh[3])}
+
+
+def tuning_use(on: bool):
+    lib().mibwt_tuning_use(1 if on else 0)
 
 
 def synchronize():

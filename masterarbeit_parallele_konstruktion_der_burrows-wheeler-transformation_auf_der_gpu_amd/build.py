@@ -26,7 +26,7 @@ DEVICE_FLAGS = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang
 COMMON = ["-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-I" + CSRC,
           "-I" + os.path.join(HERE, "..", "include")]
 
-LIB_SOURCES = ["abi.hip", "pfp.hip", "lyndon.hip", "selector.hip", "unbwt.hip"]
+LIB_SOURCES = ["abi.hip", "pfp.hip", "lyndon.hip", "selector.hip", "unbwt.hip", "tune.hip"]
 CLI_SOURCES = {"pfp_bwt": "cli.cpp", "lyndon_bwt": "cli.cpp", "cubwt_compat": "cli.cpp"}
 
 

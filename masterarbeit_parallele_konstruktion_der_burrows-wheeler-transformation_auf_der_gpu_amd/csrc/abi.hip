@@ -30,6 +30,10 @@ float mlp_grad(Storage& S, const float* params, const float* X, const float* Y, 
 void mlp_predict_raw(Storage& S, const float* params, const float* X, uint32_t B, float* Y);
 float mlp_train(Storage& S, const float* X, const float* Y, uint32_t B, int epochs, uint32_t mb, float lr,
                 uint32_t seed, const char* path, int members);
+int autotune_run(Storage& S, uint64_t max_bytes, int reps, const char* path);
+bool tuning_load(const char* path);
+void tuning_use_table(bool on);
+Tuning tuning_for(uint64_t size);
 }  // namespace mibwt
 
 using namespace mibwt;
@@ -692,6 +696,39 @@ int64_t mibwt_mlp_train_ensemble(void* storage, const float* X, const float* Y, 
         if (final_loss) *final_loss = l;
         return LIBCUBWT_NO_ERROR;
     });
+}
+
+int64_t mibwt_autotune(void* storage, int64_t max_bytes, int32_t reps, const char* out_path)
+{
+    Storage* S = reinterpret_cast<Storage*>(storage);
+    if (!S || max_bytes < (1 << 16) || max_bytes > S->max_length || reps < 1 || reps > 16) return LIBCUBWT_BAD_PARAMETER;
+    return guarded([&]() -> int64_t {
+        MIBWT_HIP(hipSetDevice(S->device));
+        return autotune_run(*S, (uint64_t)max_bytes, reps, out_path);
+    });
+}
+
+int64_t mibwt_tuning_load(const char* path)
+{
+    if (!path) return LIBCUBWT_BAD_PARAMETER;
+    return tuning_load(path) ? LIBCUBWT_NO_ERROR : LIBCUBWT_BAD_PARAMETER;
+}
+
+int64_t mibwt_tuning_use(int32_t on)
+{
+    tuning_use_table(on != 0);
+    return LIBCUBWT_NO_ERROR;
+}
+
+int64_t mibwt_tuning_get(int64_t size, int32_t* shape)
+{
+    if (!shape || size < 1) return LIBCUBWT_BAD_PARAMETER;
+    const Tuning t = tuning_for((uint64_t)size);
+    shape[0] = t.scan_sub;
+    shape[1] = t.sort_items;
+    shape[2] = t.chain_items;
+    shape[3] = t.key_bits;
+    return LIBCUBWT_NO_ERROR;
 }
 
 int64_t mibwt_device_alloc(void** ptr, int64_t bytes)

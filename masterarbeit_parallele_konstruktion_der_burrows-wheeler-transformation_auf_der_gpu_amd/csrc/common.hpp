@@ -85,6 +85,23 @@ struct Chain {
     uint32_t epoch;
 };
 
+// Launch shapes chosen per build (by the autotuner's table for the build's size bucket, or the
+// defaults): trigger-scan sub-tiles per chained tile (1, 2, 4, 8), radix items per thread for
+// 8-byte keys (8 or 16; 4-byte keys take twice as many), chained-scan items per thread (16 or
+// 64), and the width of the packed initial keys of the suffix sorts (32..64 bits: fewer radix
+// passes against more doubling rounds).
+struct Tuning {
+    int scan_sub = 8;
+    int sort_items = 16;
+    int chain_items = 64;
+    int key_bits = 64;
+};
+
+struct Storage;
+// Sets S.tune for a build of `size` bytes from the autotuner's table (tune.hip); kept while
+// S.tune_pinned.
+void apply_tuning(Storage& S, uint64_t size);
+
 // Everything one libcubwt-style storage owns.  One stream; not safe for concurrent calls.
 struct Storage {
     int device = 0;
@@ -123,6 +140,9 @@ struct Storage {
     uint32_t sketch_epoch = 0;
     // Lyndon builds on this storage (seeds the per-call fingerprint base)
     uint64_t lyn_calls = 0;
+    // launch shapes of the current build; `tune_pinned` keeps them (the autotuner's trials)
+    Tuning tune;
+    bool tune_pinned = false;
     uint64_t sel_model_version = 0;  // selector model resident in B_SEL_MP / B_SEL_MPB / B_SEL_NORM
     // Arena mode (the allocation-free entry point): every buffer is carved from one device region
     // that the caller owns, host read-backs go through `arena_host` (no pinned or mapped memory),

@@ -801,6 +801,7 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
     const uint64_t n = (uint64_t)n64;
     S.phases_reset();
     S.stats.n = n64;
+    apply_tuning(S, n);
     if (!S.t_begin) {
         MIBWT_HIP(hipEventCreate(&S.t_begin));
         MIBWT_HIP(hipEventCreate(&S.t_end));

@@ -288,7 +288,8 @@ constexpr int kScanPer = 32;
 constexpr int kScanTileW = kBlock * kScanPer;  // window ends per tile
 constexpr int kScanHalo = 64;                   // bytes staged before the tile
 constexpr int kScanMaxW = 56;
-constexpr int kScanSub = 8;                     // sub-tiles per chained tile
+// Sub-tiles per chained tile: a launch shape (Storage::tune.scan_sub, 8 by default; smaller for
+// short texts, whose few tiles would otherwise leave most of the chip idle).
 
 struct TrigScan {
     const uint8_t* T;
@@ -377,7 +378,7 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
 // mask per thread; the masks wait in LDS until the tile's offset is known.  Eight sub-tiles
 // per tile balance the number of chained tiles (one ticket each) against the LDS that limits
 // the resident blocks (measured faster than 2, 4, 6, 12, 16 or 32 on 256 MiB and 1 GiB texts).
-template <typename V, bool kPow2>
+template <typename V, bool kPow2, int kScanSub>
 __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 {
     __shared__ __align__(16) uint8_t txt[kScanHalo + kScanTileW + 16];
@@ -452,7 +453,7 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 #pragma unroll 1
     for (int sub = 0; sub < kScanSub; sub += 2) {
         // two sub-tiles per scan: counts are at most 32 * kBlock < 2^16
-        const uint32_t m0 = masks[sub][threadIdx.x], m1 = masks[sub + 1][threadIdx.x];
+        const uint32_t m0 = masks[sub][threadIdx.x], m1 = (sub + 1 < kScanSub) ? masks[sub + 1][threadIdx.x] : 0u;
         const uint32_t pair = (uint32_t)__popc(m0) | ((uint32_t)__popc(m1) << 16);
         uint32_t tot;
         const uint32_t ex = block_exclusive_scan(pair, 0u, OpSum{}, wsum, &tot);
@@ -461,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
             uint32_t o = run + (h ? tot0 + (ex >> 16) : (ex & 0xFFFFu));
             uint32_t mm = h ? m1 : m0;
             const uint64_t e0 = ((uint64_t)tile * kScanSub + sub + h) * kScanTileW + (uint64_t)threadIdx.x * kScanPer;
-            const uint32_t sent = sub_sent[sub + h];
+            const uint32_t sent = (sub + h < kScanSub) ? sub_sent[sub + h] : 0u;
             while (mm) {
                 const int i = __ffs(mm) - 1;
                 mm &= mm - 1;
@@ -508,16 +509,24 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, uint32_t n1, const 
     p.qinv = inverse_mod_2_64(q);
     const bool narrow = W <= 24;
     p.plim = narrow ? (uint64_t)(0xFFFFFFFFull / P) : (~0ull) / P;
-    p.tiles = (uint32_t)ceil_div(n, (uint64_t)kScanTileW * kScanSub);
+    const int sub = S.tune.scan_sub;
+    p.tiles = (uint32_t)ceil_div(n, (uint64_t)kScanTileW * sub);
     p.cap = cap;
     p.E = E;
     p.total = total_dev;
     p.chain = S.chain(p.tiles);
     const bool pow2 = (q == 1);
-    if (narrow && pow2) k_trig_scan<uint32_t, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
-    else if (narrow) k_trig_scan<uint32_t, false><<<p.tiles, kBlock, 0, S.stream>>>(p);
-    else if (pow2) k_trig_scan<uint64_t, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
-    else k_trig_scan<uint64_t, false><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    auto launch = [&](auto sub_tag) {
+        constexpr int kSub = decltype(sub_tag)::value;
+        if (narrow && pow2) k_trig_scan<uint32_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (narrow) k_trig_scan<uint32_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (pow2) k_trig_scan<uint64_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else k_trig_scan<uint64_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+    };
+    if (sub == 1) launch(std::integral_constant<int, 1>{});
+    else if (sub == 2) launch(std::integral_constant<int, 2>{});
+    else if (sub == 4) launch(std::integral_constant<int, 4>{});
+    else launch(std::integral_constant<int, 8>{});
     MIBWT_LAUNCH_CHECK();
     return p.tiles;
 }
@@ -1347,9 +1356,9 @@ __global__ void k_u32_to_u64(const uint32_t* __restrict__ a, uint64_t* __restric
 
 // Bits of the packed initial keys of the dictionary and parse sorts (MIBWT_INIT_KEY_BITS
 // overrides both, MIBWT_PARSE_KEY_BITS the parse sort alone, for tuning runs).
-static int init_key_bits(bool parse = false)
+static int init_key_bits(const Storage& S, bool parse = false)
 {
-    int v = 64;
+    int v = S.tune.key_bits;
     const char* e = getenv("MIBWT_INIT_KEY_BITS");
     if (parse && getenv("MIBWT_PARSE_KEY_BITS")) e = getenv("MIBWT_PARSE_KEY_BITS");
     if (e) {
@@ -1478,6 +1487,7 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     const uint64_t n = bt.total;
     const bool multi = bt.count > 1;
     const bool timed = !S.arena;  // arena mode records no events
+    apply_tuning(S, bt.staged);
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     // ---- 1. trigger scan
     int ph = S.phase_begin("scan");
@@ -1603,7 +1613,7 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     // initial key width: whole radix passes trade against one more doubling round; with many
     // texts the text id takes the bits above the symbols
     const int tbits = multi ? bits_for(bt.count - 1) : 0;
-    const int key_cap = init_key_bits();
+    const int key_cap = init_key_bits(S);
     const int aper = std::max(1, (key_cap - tbits) / abits);
     MIBWT_REQUIRE(tbits + abits * aper <= 64, LIBCUBWT_BAD_PARAMETER, "too many texts for one build");
     DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
@@ -1637,7 +1647,7 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     V0 = S.buf<uint32_t>(B_SORT_V0, d);
     V1 = S.buf<uint32_t>(B_SORT_V1, d);
     const int B0 = bits_for((uint64_t)du + 1);
-    const int pper = std::max(1, init_key_bits(true) / B0);
+    const int pper = std::max(1, init_key_bits(S, true) / B0);
     k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, pper, K0, V0);
     MIBWT_LAUNCH_CHECK();
     uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);

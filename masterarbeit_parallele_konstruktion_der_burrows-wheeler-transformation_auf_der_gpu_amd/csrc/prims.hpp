@@ -209,14 +209,14 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, 
 // Chained tiles are large (kChainItems per thread, 16384 per tile): every tile draws one
 // ticket from a single device-wide counter, and such atomics serialize at roughly 11 ns each
 // on MI355X, so a tile must carry far more than 11 ns of memory traffic.
-constexpr int kChainItems = 64;
-constexpr int kChainTile = kBlock * kChainItems;
-
-template <typename Op, bool kVec>
+// The items per thread are a launch shape (Storage::tune.chain_items: 64 by default, 16 for
+// short arrays, whose tiles would otherwise be too few to fill the chip).
+template <typename Op, bool kVec, int kChainItems>
 __global__ __launch_bounds__(kBlock) void k_scan_chain(const uint32_t* in, uint32_t* out, uint64_t n,
                                                       uint32_t identity, Op op, int inclusive,
                                                       uint32_t* __restrict__ total_out, Chain c, uint32_t tiles)
 {
+    constexpr int kChainTile = kBlock * kChainItems;
     __shared__ uint32_t slot;
     __shared__ uint32_t lds[kWaves];
     const uint32_t tile = chain_ticket(c, tiles, &slot);
@@ -279,18 +279,22 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
         return;
     }
     if constexpr (std::is_same<T, uint32_t>::value) {
-        tiles = ceil_div(n, (uint64_t)kChainTile);
+        const int items = (S.tune.chain_items == 16) ? 16 : 64;
+        tiles = ceil_div(n, (uint64_t)kBlock * items);
         const Chain c = S.chain(tiles);
         const uint32_t* in32 = reinterpret_cast<const uint32_t*>(in);
         uint32_t* out32 = reinterpret_cast<uint32_t*>(out);
         uint32_t* tot32 = reinterpret_cast<uint32_t*>(total_dev);
         const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
-        if (vec)
-            k_scan_chain<Op, true><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, (uint32_t)identity, op,
-                                                                   inclusive ? 1 : 0, tot32, c, tiles);
-        else
-            k_scan_chain<Op, false><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, (uint32_t)identity, op,
-                                                                    inclusive ? 1 : 0, tot32, c, tiles);
+        const int inc = inclusive ? 1 : 0;
+        const uint32_t id = (uint32_t)identity;
+        if (items == 16) {
+            if (vec) k_scan_chain<Op, true, 16><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+            else k_scan_chain<Op, false, 16><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+        } else {
+            if (vec) k_scan_chain<Op, true, 64><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+            else k_scan_chain<Op, false, 64><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+        }
         MIBWT_LAUNCH_CHECK();
         return;
     }
@@ -607,7 +611,10 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         }
     }
     MIBWT_REQUIRE(n < (1ull << 32), LIBCUBWT_BAD_PARAMETER, "radix_sort: too many items");
-    const uint32_t tiles = ceil_div(n, (uint64_t)kBlock * sort_items<K>());
+    // items per thread: the launch shape of this build (8-byte keys 8 or 16, 4-byte keys twice)
+    const bool half = S.tune.sort_items == 8;
+    constexpr int kFull = sort_items<K>(), kHalf = sort_items<K>() / 2;
+    const uint32_t tiles = ceil_div(n, (uint64_t)kBlock * (half ? kHalf : kFull));
     const int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
     MIBWT_REQUIRE(passes <= kMaxPasses, LIBCUBWT_BAD_PARAMETER, "radix_sort: key too wide");
     const uint32_t* hist = hist_in;
@@ -633,14 +640,22 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         const Chain c = S.chain(tiles * kRadix);
         if (is_last && last) {
             const int kt = S.kt_begin(1, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
-            k_radix_pass<K, V, LastEmit><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, hist + p * kRadix,
-                                                                        tiles, c, *last);
+            if (half)
+                k_radix_pass<K, V, LastEmit, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+                                                                                   hist + p * kRadix, tiles, c, *last);
+            else
+                k_radix_pass<K, V, LastEmit, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+                                                                                   hist + p * kRadix, tiles, c, *last);
             S.kt_end(kt);
         } else {
             const int kt = S.kt_begin(1, 2.0 * (double)n * (sizeof(K) + vbytes));
             EmitPair<K, V> e{kout, vout};
-            k_radix_pass<K, V, EmitPair<K, V>><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                              hist + p * kRadix, tiles, c, e);
+            if (half)
+                k_radix_pass<K, V, EmitPair<K, V>, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+                                                                                         hist + p * kRadix, tiles, c, e);
+            else
+                k_radix_pass<K, V, EmitPair<K, V>, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
+                                                                                         hist + p * kRadix, tiles, c, e);
             S.kt_end(kt);
         }
         MIBWT_LAUNCH_CHECK();

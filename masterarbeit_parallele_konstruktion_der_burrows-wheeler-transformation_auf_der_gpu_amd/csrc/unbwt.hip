@@ -170,7 +170,10 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     MIBWT_REQUIRE(n64 >= 1 && n64 < (int64_t)INT32_MAX - (1 << 20), LIBCUBWT_BAD_PARAMETER, "n out of range");
     MIBWT_REQUIRE(index64 >= 1 && index64 <= n64, LIBCUBWT_BAD_PARAMETER, "primary index out of range");
     const uint64_t n = (uint64_t)n64, index = (uint64_t)index64, m = n + 1;
-    if (own_phases) S.phases_reset();
+    if (own_phases) {
+        S.phases_reset();
+        apply_tuning(S, n);
+    }
     auto phase_begin = [&](const char* name) { return own_phases ? S.phase_begin(name) : -1; };
     auto phase_end = [&](int id, double bytes) {
         if (id >= 0) S.phase_end(id, bytes);
