@@ -169,6 +169,13 @@ def main():
         ms_per_step = ms_fixed
     value = n / 1e6 / (ms_per_step / 1000.0)
     O_run = dO.download() if not a.no_verify else None
+    # the learned choice as a fixed build: the difference is the selection's own cost
+    selection = None
+    if a.wp == "auto" and (W_run, P_run) != (a.W, a.P):
+        ms_chosen, _ = run(lambda: st.bwt_device(dT.address, dO.address, n, "pfp", W_run, P_run), a.steps, 1,
+                           f"fixed W={W_run},P={P_run} (the learned choice)")
+        selection = {"chosen_fixed_ms_per_step": round(ms_chosen, 4),
+                     "selection_ms": round(ms_per_step - ms_chosen, 4)}
     if O_run is not None and not (O_run == O_fixed).all():
         log("BWT differs between the fixed and the learned (W,P)")
         sys.exit(1)
@@ -186,6 +193,9 @@ def main():
         if O_fixed is not None and not (dO.download() == O_fixed).all():
             log("BWT differs between W=5,P=16 and W=35,P=81")
             sys.exit(1)
+
+    # device memory the storage holds after the PFP builds (its buffers only grow)
+    device_mem = int(st.stats()["device_bytes"])
 
     # launch shapes: the autotuner's table (tuning_table.txt, when present) against the defaults
     tuning = {"table": os.path.exists(mibwt.TUNING_PATH), "shape": mibwt.tuning_shape(n)}
@@ -219,6 +229,7 @@ def main():
         sl = st.stats()
         lyndon = {"value": round(n / 1e6 / (ms_l / 1000.0), 2), "ms_per_step": round(ms_l, 4),
                   "steps": a.lyndon_steps, "retries": sl["retries"],
+                  "device_memory_bytes": int(sl["device_bytes"]),
                   "phases": {p["name"]: round(p["ms"], 3) for p in sl["phases"]}}
         if O_fixed is not None:
             same = idx_l == idx and bool((dO.download() == O_fixed).all())
@@ -254,7 +265,6 @@ def main():
     step_fixed()
     kscan = st.stats()
     st.set_kernel_timing(None)
-    device_mem = int(st.stats()["device_bytes"])
     if a.calibrate_copy:
         cms, cbytes = st.time_kernel("copy", reps=3, nbytes=float(256 << 20))
         log(f"copy kernel: {cbytes / (cms / 1000) / 1e9 * 2:.1f} GB/s (read+write) over {cbytes:.0f} B")
@@ -413,6 +423,7 @@ def main():
                      "ms_per_step": round(ms_fixed, 4),
                      "phases": {p["name"]: round(p["ms"], 3) for p in stats_fixed["phases"]}},
         "speedup_vs_fixed": round(ms_fixed / ms_per_step, 4),
+        "selection": selection,
         "w35_p81": w35,
         "tuning": tuning,
         "control": control,
@@ -420,7 +431,7 @@ def main():
         "lyndon": lyndon,
         "pcie_inclusive": pcie,
         "batch": batch,
-        "device_memory_bytes": device_mem,
+        "device_memory_bytes": {"pfp": device_mem, "bytes_per_text_byte": round(device_mem / n, 2)},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "verified": verified,

@@ -373,12 +373,77 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
     return mask;
 }
 
+// Lanes of a 4-byte word that are zero, as 4 bits (exact per lane: no borrow between lanes).
+__device__ __forceinline__ uint32_t zero_lanes4(uint32_t x)
+{
+    const uint32_t z = ~(((x & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | x) & 0x80808080u;
+    return ((z >> 7) * 0x01020408u) >> 24;  // bits 7, 15, 23, 31 -> bits 0..3
+}
+
+// The same mask for P = 2^s with s <= 5, four window ends per 32-bit word and no window value:
+// mod 2^s only the last min(W, s) bytes count, byte T[e-j] with its low s-j bits and weight 2^j,
+// so the lanes of sum_j (w_j & low_{s-j}) << j (w_j = the word of the bytes j positions back) are
+// the hash values; each lane stays below 5 * 32 < 256, so no carry leaves its byte.  First-window
+// candidates are the lanes whose last min(W, 4) bytes equal the first window's (then confirmed
+// byte by byte).
+__device__ __forceinline__ uint32_t trig_mask32_swar(const uint8_t* txt, uint32_t loc, const uint4& a, const uint4& b,
+                                                     const TrigScan& p, const uint8_t* first, uint64_t e0,
+                                                     uint64_t lo, uint64_t hi, uint64_t sent)
+{
+    const int W = p.W;
+    const uint32_t sh = p.pshift;
+    const int nt = W < (int)sh ? W : (int)sh;  // hash terms
+    const int nf = W < 4 ? W : 4;              // bytes compared with the first window
+    uint32_t keep[5], fb[4];
+#pragma unroll
+    for (int j = 0; j < 5; j++) keep[j] = (j < nt) ? 0x01010101u * ((1u << (sh - j)) - 1u) : 0u;
+#pragma unroll
+    for (int j = 0; j < 4; j++) fb[j] = (j < nf) ? 0x01010101u * first[W - 1 - j] : 0u;
+    const uint32_t low = 0x01010101u * ((1u << sh) - 1u);
+    const uint32_t in[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+    uint32_t prev = *reinterpret_cast<const uint32_t*>(txt + loc - 4);  // bytes e0-4 .. e0-1
+    uint32_t mh = 0, mc = 0;
+#pragma unroll
+    for (int m = 0; m < 8; m++) {
+        const uint32_t w0 = in[m];
+        const uint32_t w1 = __builtin_amdgcn_alignbyte(w0, prev, 3);  // lane i: byte e-1
+        const uint32_t w2 = __builtin_amdgcn_alignbyte(w0, prev, 2);
+        const uint32_t w3 = __builtin_amdgcn_alignbyte(w0, prev, 1);
+        const uint32_t ws[5] = {w0, w1, w2, w3, prev};  // prev: lane i = byte e-4
+        uint32_t h = 0;
+#pragma unroll
+        for (int j = 0; j < 5; j++) h += (ws[j] & keep[j]) << j;
+        h &= low;
+        uint32_t ne = 0;
+#pragma unroll
+        for (int j = 0; j < 4; j++) ne |= (j < nf) ? (ws[j] ^ fb[j]) : 0u;
+        mh |= zero_lanes4(h) << (4 * m);
+        mc |= zero_lanes4(ne) << (4 * m);
+        prev = w0;
+    }
+    uint32_t mask = mh;
+    uint32_t cand = mc & ~mask;
+    while (cand) {
+        const int i = __ffs(cand) - 1;
+        cand &= cand - 1;
+        bool eq = true;
+        for (int k = 0; k < W && eq; k++) eq = (txt[loc + i - W + 1 + k] == first[k]);
+        if (eq) mask |= 1u << i;
+    }
+    const uint64_t ia = lo > e0 ? lo - e0 : 0, ib = hi > e0 ? hi - e0 : 0;
+    const uint32_t below_b = ib >= 32 ? ~0u : ((1u << (uint32_t)ib) - 1u);
+    const uint32_t below_a = ia >= 32 ? ~0u : ((1u << (uint32_t)ia) - 1u);
+    mask &= below_b & ~below_a;
+    if (sent >= e0 && sent < e0 + kScanPer) mask |= 1u << (uint32_t)(sent - e0);
+    return mask;
+}
+
 // One tile = kScanSub sub-tiles of kScanTileW window ends.  Each sub-tile is staged in LDS
 // (own bytes by two 16-byte loads per thread plus a 64-byte halo) and turned into one 32-bit
 // mask per thread; the masks wait in LDS until the tile's offset is known.  Eight sub-tiles
 // per tile balance the number of chained tiles (one ticket each) against the LDS that limits
 // the resident blocks (measured faster than 2, 4, 6, 12, 16 or 32 on 256 MiB and 1 GiB texts).
-template <typename V, bool kPow2, int kScanSub>
+template <typename V, bool kPow2, int kScanSub, bool kSwar = false>
 __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 {
     __shared__ __align__(16) uint8_t txt[kScanHalo + kScanTileW + 16];
@@ -438,9 +503,14 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
             *reinterpret_cast<uint4*>(txt + kScanHalo + kScanTileW) = make_uint4(0, 0, 0, 0);
         }
         __syncthreads();
-        const uint32_t m = (sb < p.n) ? trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0, (uint64_t)base + W,
-                                                               (uint64_t)base + nt, sent)
-                                      : 0u;
+        uint32_t m = 0;
+        if (sb < p.n) {
+            if constexpr (kSwar)
+                m = trig_mask32_swar(txt, loc, a, b, p, first, e0, (uint64_t)base + W, (uint64_t)base + nt, sent);
+            else
+                m = trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0, (uint64_t)base + W, (uint64_t)base + nt,
+                                          sent);
+        }
         masks[sub][threadIdx.x] = m;
         cnt += (uint32_t)__popc(m);
     }
@@ -516,9 +586,13 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, uint32_t n1, const 
     p.total = total_dev;
     p.chain = S.chain(p.tiles);
     const bool pow2 = (q == 1);
+    // P = 2^s with s <= 5: four window ends per word (MIBWT_SCAN_SWAR=0 takes the rolling value)
+    const char* sw = getenv("MIBWT_SCAN_SWAR");
+    const bool swar = pow2 && s <= 5 && !(sw && atoi(sw) == 0);
     auto launch = [&](auto sub_tag) {
         constexpr int kSub = decltype(sub_tag)::value;
-        if (narrow && pow2) k_trig_scan<uint32_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        if (swar) k_trig_scan<uint32_t, true, kSub, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (narrow && pow2) k_trig_scan<uint32_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
         else if (narrow) k_trig_scan<uint32_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
         else if (pow2) k_trig_scan<uint64_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
         else k_trig_scan<uint64_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);

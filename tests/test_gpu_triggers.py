@@ -14,7 +14,7 @@ import oracle
 pytestmark = pytest.mark.gpu
 
 WS = [1, 2, 3, 5, 10, 16, 20, 24, 25, 35, 56, 57, 64]
-PS = [1, 2, 16, 11, 13, 81, 96, 1024, 65535]
+PS = [1, 2, 4, 8, 16, 32, 11, 13, 81, 96, 1024, 65535]
 
 
 def texts(seed):
