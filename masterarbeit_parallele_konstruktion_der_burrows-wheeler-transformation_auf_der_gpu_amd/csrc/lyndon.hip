@@ -36,6 +36,12 @@ namespace mibwt {
 bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT);
 
 constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass
+// Label tie groups of more members go to the pivot partition (linear work per round; a group of
+// equal labels settles in one round) instead of the comparator counting sort (quadratic in the
+// group size, but one pass); MIBWT_LAB_SMALL overrides it (2..256).  Measured on the 256 MiB DNA
+// collection: 256 -> rsort 83 ms, 64 -> 97 ms, 16 -> 148 ms, 4 -> 174 ms (more pivot rounds,
+// each a radix sort, outweigh the cheaper comparisons).
+constexpr uint32_t kLabSmall = 256;
 constexpr uint64_t kM61 = (1ull << 61) - 1;
 
 // ------------------------------------------------------------------- 1. Lyndon array
@@ -399,15 +405,15 @@ __global__ void k_lab_heads_from_keys(const uint64_t* __restrict__ K, uint64_t n
 // symbols), and is that group small or large?  cls: 0 = settled, 1 = small tie, 2 = large tie.
 __global__ void k_lab_classify(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                                const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ lam, uint64_t n,
-                               uint32_t known, const uint8_t* __restrict__ settled, uint8_t* __restrict__ cls,
-                               uint32_t* __restrict__ large)
+                               uint32_t known, uint32_t small, const uint8_t* __restrict__ settled,
+                               uint8_t* __restrict__ cls, uint32_t* __restrict__ large)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t s = gofs[g], e = gofs[g + 1];
     uint8_t c = 0;
-    if (e - s >= 2 && lam[V[s]] >= known && !settled[s]) c = (e - s > kSmallGroup) ? 2 : 1;
+    if (e - s >= 2 && lam[V[s]] >= known && !settled[s]) c = (e - s > small) ? 2 : 1;
     cls[i] = c;
     large[i] = (c == 2) ? 1u : 0u;
 }
@@ -681,6 +687,11 @@ static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_
         uint32_t* H = S.buf<uint32_t>(B_LY_HF2, n);
         MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
         const uint32_t known = (uint32_t)per;  // every tie group agrees on its first `per` symbols
+        uint32_t small = kLabSmall;
+        if (const char* e = getenv("MIBWT_LAB_SMALL")) {
+            const int v = atoi(e);
+            if (v >= 2 && v <= 256) small = (uint32_t)v;
+        }
         uint8_t* settled = S.buf<uint8_t>(B_LY_SETTLED, n);
         fill<uint8_t>(S, settled, n, (uint8_t)0);
         for (int round = 0;; round++) {
@@ -688,7 +699,8 @@ static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_
             inclusive_sum<uint32_t>(S, H, gid1, n, sc + SC_GROUPS);
             k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(H, gid1, (uint32_t)n, gofs);
             MIBWT_LAUNCH_CHECK();
-            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, settled, cls, large);
+            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, small, settled, cls,
+                                                                 large);
             MIBWT_LAUNCH_CHECK();
             exclusive_sum<uint32_t>(S, large, lpos, n, sc + SC_LARGE);
             uint32_t hv[SC_COUNT_];
@@ -709,9 +721,9 @@ static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_
                 uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
                 uint32_t* LG = S.buf<uint32_t>(B_LG_SZ, nl);
                 uint32_t* hq = S.buf<uint32_t>(B_LY_SCAN0, nl);
-                // large groups hold more than kSmallGroup members each, so their dense ids take
-                // gb bits; the l part takes what is left of the 64-bit key (at most 31 bits)
-                const int gb = bits_for((uint64_t)nl / (kSmallGroup + 1));
+                // large groups hold more than `small` members each, so their dense ids take gb
+                // bits; the l part takes what is left of the 64-bit key (at most 31 bits)
+                const int gb = bits_for((uint64_t)nl / (small + 1));
                 const int lbits = std::min(31, 64 - 11 - gb);
                 k_lab_pivot_gather<<<grid_for(n), kBlock, 0, S.stream>>>(phh, T, n, V, gid1, gofs, large, lpos, lam,
                                                                          known, lbits, LK0, LV0, LI, LG);
