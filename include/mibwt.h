@@ -196,7 +196,10 @@ int64_t mibwt_set_kernel_timing(void* storage, const char* kernel);
 /* Diagnostics: re-run a kernel `reps` times on the storage stream and return its mean
  * duration (ms) and algorithmic bytes per launch.  "scan": the trigger-scan kernel on the
  * text of the last PFP call; "copy": a plain 16-byte-per-thread device copy of
- * bytes_per_launch bytes (input), used to calibrate hardware byte counters. */
+ * bytes_per_launch bytes (input), used to calibrate hardware byte counters; "radix": the LSD
+ * radix sort of bytes_per_launch (input: an item count) uniformly random 64-bit keys with
+ * 32-bit values, its mean duration per digit pass (8 per sort) and 24 bytes per item
+ * (returns LIBCUBWT_GPU_ERROR if a sorted result is wrong). */
 int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms,
                           double* bytes_per_launch);
 

@@ -16,6 +16,7 @@ int64_t pfp_batch_run(Storage& S, const uint8_t* dT, const int64_t* offs, int64_
                       int64_t* index, uint64_t chunk_bytes);
 int64_t pfp_triggers_run(Storage& S, const uint8_t* src, int64_t n, int W, int P, uint32_t* E, int64_t cap);
 double time_copy(Storage& S, uint64_t bytes, int reps);
+double time_radix(Storage& S, uint64_t m, int reps, double* bytes);
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n);
 int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n, uint32_t* lam_host);
 int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n, int64_t index, bool ref_format,
@@ -439,7 +440,7 @@ int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double*
 {
     Storage* S = reinterpret_cast<Storage*>(storage);
     if (!S || !name || reps < 1 || !ms || !bytes_per_launch) return LIBCUBWT_BAD_PARAMETER;
-    if (strcmp(name, "copy") != 0 && S->last_n < 1) return LIBCUBWT_BAD_PARAMETER;
+    if (strcmp(name, "copy") != 0 && strcmp(name, "radix") != 0 && S->last_n < 1) return LIBCUBWT_BAD_PARAMETER;
     return guarded([&]() -> int64_t {
         MIBWT_HIP(hipSetDevice(S->device));
         if (strcmp(name, "scan") == 0) {
@@ -451,6 +452,12 @@ int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double*
             if (bytes < 16) return (int64_t)LIBCUBWT_BAD_PARAMETER;
             *ms = time_copy(*S, bytes, reps);
             *bytes_per_launch = (double)bytes;
+            return LIBCUBWT_NO_ERROR;
+        }
+        if (strcmp(name, "radix") == 0) {
+            const double items = *bytes_per_launch;
+            if (!(items >= 1 && items < 2147483648.0)) return (int64_t)LIBCUBWT_BAD_PARAMETER;
+            *ms = time_radix(*S, (uint64_t)items, reps, bytes_per_launch);
             return LIBCUBWT_NO_ERROR;
         }
         return (int64_t)LIBCUBWT_BAD_PARAMETER;

@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <cstdint>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <stdexcept>
@@ -370,7 +371,7 @@ enum BufId : int {
     B_SORT_K1,
     B_SORT_V0,
     B_SORT_V1,
-    B_SORT_HIST,
+    B_SORT_TCNT,
     B_PH_FLAG,      // dictionary head flags / ids
     B_PH_PID,       // phrase id per occurrence
     B_PH_CHAR,      // character before each phrase
@@ -414,8 +415,6 @@ enum BufId : int {
     B_Q_OCC,
     B_Q_RREC,
     B_DB_LPART,
-    B_E_HIST,
-    B_E_PART,
     B_E_GFLAG,
     B_E_CID,
     B_E_DELTAC,
@@ -519,8 +518,7 @@ enum ScalarSlot : int {
     SC_SIGMA,
     SC_DICTLEN,        // two slots: 64-bit dictionary length
     SC_SPECIAL = SC_DICTLEN + 2,  // two slots: group and entry index of the primary-index entry
-    SC_EHSUM = SC_SPECIAL + 2,    // entries counted by the entry-sort histogram
-    SC_NGROUPS,                   // groups that receive BWT entries
+    SC_NGROUPS = SC_SPECIAL + 2,  // groups that receive BWT entries
     SC_COUNT_ = 16
 };
 

@@ -964,85 +964,6 @@ __global__ void k_asm_counts(const SortedRec* __restrict__ RH, const uint8_t* __
     if (ordered && f) gflag[a.head] = 1;
 }
 
-// Digit histograms of the entry sort, weighted from the groups instead of read from the
-// entries: every sorted position g contributes ecnt[g] entries keyed by the dense id of its
-// group head, cid[head(g)], which is cid[g] + gflag[g] - 1 (groups are contiguous and only
-// heads carry flags), nondecreasing in g.  A block stages tiles of kEhTile positions in LDS
-// with coalesced loads, then every thread walks kEhPer consecutive positions and adds runs of
-// equal digits at once; blocks stride over the tiles and write their counts to
-// part[block][kEhPasses * 256].
-constexpr int kEhPer = 16, kEhPasses = 4, kEhTile = kBlock * kEhPer, kEhBlocks = 2048;
-__global__ __launch_bounds__(kBlock) void k_entry_hist(const uint32_t* __restrict__ gflag,
-                                                      const uint32_t* __restrict__ cid,
-                                                      const uint32_t* __restrict__ ecnt, uint32_t L, int passes,
-                                                      uint32_t* __restrict__ part)
-{
-    constexpr int kPad = kEhTile + kEhTile / kEhPer;  // one pad word per kEhPer: conflict-free walks
-    __shared__ uint32_t h[kEhPasses * kRadix];
-    __shared__ uint32_t lc[kPad], le[kPad];
-    for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) h[i] = 0;
-    const uint32_t tiles = (uint32_t)(((uint64_t)L + kEhTile - 1) / kEhTile);
-    for (uint32_t tile = blockIdx.x; tile < tiles; tile += gridDim.x) {
-        const uint64_t base = (uint64_t)tile * kEhTile;
-        __syncthreads();  // the previous tile's walks are done
-#pragma unroll 4
-        for (int i = 0; i < kEhPer; i++) {
-            const uint32_t j = i * kBlock + threadIdx.x;
-            const uint64_t g = base + j;
-            uint32_t c = 0, e = 0;
-            if (g < L) {
-                c = ecnt[g];
-                if (c) e = cid[g] + gflag[g] - 1;
-            }
-            lc[j + j / kEhPer] = c;
-            le[j + j / kEhPer] = e;
-        }
-        __syncthreads();
-        uint32_t prev[kEhPasses], run[kEhPasses];
-#pragma unroll
-        for (int p = 0; p < kEhPasses; p++) {
-            prev[p] = 0xFFFFFFFFu;
-            run[p] = 0;
-        }
-        const uint32_t w0 = threadIdx.x * (kEhPer + 1);
-        for (int i = 0; i < kEhPer; i++) {
-            const uint32_t c = lc[w0 + i];
-            if (c == 0) continue;
-            const uint32_t e = le[w0 + i];
-#pragma unroll
-            for (int p = 0; p < kEhPasses; p++) {
-                if (p >= passes) break;
-                const uint32_t d = (e >> (8 * p)) & 0xFFu;
-                if (d == prev[p]) {
-                    run[p] += c;
-                } else {
-                    if (run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
-                    prev[p] = d;
-                    run[p] = c;
-                }
-            }
-        }
-#pragma unroll
-        for (int p = 0; p < kEhPasses; p++)
-            if (p < passes && run[p]) atomicAdd(&h[p * kRadix + prev[p]], run[p]);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < kEhPasses * kRadix; i += kBlock) part[(uint64_t)blockIdx.x * kEhPasses * kRadix + i] = h[i];
-}
-
-// hist[i] = sum over blocks of part[block][i]: one block per counter
-__global__ __launch_bounds__(kBlock) void k_entry_hist_sum(const uint32_t* __restrict__ part, uint32_t blocks,
-                                                          uint32_t* __restrict__ hist)
-{
-    __shared__ uint32_t red[kWaves];
-    const uint32_t i = blockIdx.x;
-    uint32_t s = 0;
-    for (uint32_t b = threadIdx.x; b < blocks; b += kBlock) s += part[(uint64_t)b * kEhPasses * kRadix + i];
-    uint32_t tot;
-    (void)block_exclusive_scan(s, 0u, OpSum{}, red, &tot);
-    if (threadIdx.x == 0) hist[i] = tot;
-}
-
 // Direct fill of the single-character groups, one wavefront per 64 * kFillPer consecutive
 // sorted positions: their slot ranges are consecutive in the output, so the lanes sweep the wave's
 // span together in 16-byte granules and look the runs up in LDS (one bisection per granule,
@@ -1764,19 +1685,6 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     MIBWT_LAUNCH_CHECK();
     // dense ids of the entry groups
     exclusive_sum<uint32_t>(S, gflag, cid, L, sc + SC_NGROUPS);
-    // digit histograms of the entry sort (bits of the dense group id), from the per-position
-    // counts; kEhPasses passes cover any id width the 32-bit entries can hold
-    uint32_t* ehist = S.buf<uint32_t>(B_E_HIST, (size_t)kMaxPasses * kRadix);
-    {
-        const uint32_t eb = (uint32_t)std::min<uint64_t>(kEhBlocks, ceil_div((uint64_t)L, (uint64_t)kEhTile));
-        uint32_t* epart = S.buf<uint32_t>(B_E_PART, (size_t)eb * kEhPasses * kRadix);
-        k_entry_hist<<<eb, kBlock, 0, S.stream>>>(gflag, cid, ecnt, L, kEhPasses, epart);
-        MIBWT_LAUNCH_CHECK();
-        k_entry_hist_sum<<<kEhPasses * kRadix, kBlock, 0, S.stream>>>(epart, eb, ehist);
-        MIBWT_LAUNCH_CHECK();
-        k_sum_tiles<<<1, kBlock, 0, S.stream>>>(ehist, kRadix, sc + SC_EHSUM);  // pass-0 total
-        MIBWT_LAUNCH_CHECK();
-    }
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
     exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t hv0[SC_COUNT_];
@@ -1817,7 +1725,6 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
         S.read_fast(hv, sc, SC_COUNT_);
         ne = hv[SC_ENTRIES] + 1;
         // the histograms from the group counts are used only if they count every entry
-        const uint32_t* eh = (hv[SC_EHSUM] == ne) ? ehist : nullptr;
         EntT* ent0 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K0, ne));
         EntT* ent1 = reinterpret_cast<EntT*>(S.buf<uint64_t>(B_SORT_K1, ne));
         const uint32_t nq = ne - 1;  // entries of the parse suffixes; the last occurrence adds one
@@ -1848,14 +1755,12 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
             k_q_pidx<<<1, 1, 0, S.stream>>>(sc + SC_TOTAL, ent0, ne - 1, special, deltac, sc + SC_PIDX);
             MIBWT_LAUNCH_CHECK();
             EmitBwt32 em{deltac, dO, n};
-            radix_sort<uint32_t, uint32_t, EmitBwt32>(S, ent0, ent1, nullptr, nullptr, ne, 8, 8 + gbits, &em, nullptr,
-                                                      eh);
+            radix_sort<uint32_t, uint32_t, EmitBwt32>(S, ent0, ent1, nullptr, nullptr, ne, 8, 8 + gbits, &em);
         } else {
             uint32_t* pidx_dev = multi ? bt.pidx : sc + SC_PIDX;
             fill<uint32_t>(S, pidx_dev, bt.count, 0u);
             EmitBwt em{deltac, dO, pidx_dev, n, multi ? bt.obase : nullptr, bt.count};
-            radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em, nullptr,
-                                                    eh);
+            radix_sort<uint64_t, uint32_t, EmitBwt>(S, ent0, ent1, nullptr, nullptr, ne, 9, 9 + gbits, &em);
         }
     };
     // MIBWT_WIDE_ENTRIES=1 takes the 64-bit entries (with the in-sort primary index) at any size
@@ -2101,6 +2006,66 @@ double time_copy(Storage& S, uint64_t bytes, int reps)
     return ms / reps;
 }
 
+__global__ void k_rand_pairs(uint64_t* __restrict__ K, uint32_t* __restrict__ V, uint64_t m, uint64_t seed)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    K[i] = mix64(seed + 0x9E3779B97F4A7C15ull * (i + 1));
+    V[i] = (uint32_t)i;
+}
+
+// out-of-order neighbours (keys) and values that did not travel with their key
+__global__ void k_sorted_check(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V, uint64_t m,
+                               uint64_t seed, uint32_t* __restrict__ bad)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const bool b = (i + 1 < m && K[i] > K[i + 1]) || K[i] != mix64(seed + 0x9E3779B97F4A7C15ull * ((uint64_t)V[i] + 1));
+    if (b) atomicAdd(bad, 1u);
+}
+
+// LSD radix sort of m uniformly random (64-bit key, 32-bit value) pairs, `reps` times: mean
+// duration of one digit pass (8 per sort, all of its kernels) and its algorithmic bytes (24 per
+// pair).  The result of every sort is checked (sorted keys, values with their keys).
+double time_radix(Storage& S, uint64_t m, int reps, double* bytes)
+{
+    MIBWT_REQUIRE(m >= 1 && m < (1ull << 31), LIBCUBWT_BAD_PARAMETER, "radix timing: item count out of range");
+    apply_tuning(S, m);  // the launch shape of a build of m bytes
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, m);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, m);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, m);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, m);
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    hipEvent_t e0, e1;
+    MIBWT_HIP(hipEventCreate(&e0));
+    MIBWT_HIP(hipEventCreate(&e1));
+    double total = 0;
+    uint32_t bad = 0;
+    for (int r = 0; r <= reps; r++) {  // the first sort warms the buffers and is not timed
+        const uint64_t seed = 0x5EED0000ull + (uint64_t)r;
+        k_rand_pairs<<<grid_for(m), kBlock, 0, S.stream>>>(K0, V0, m, seed);
+        MIBWT_LAUNCH_CHECK();
+        MIBWT_HIP(hipEventRecord(e0, S.stream));
+        radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, m, 0, 64);
+        MIBWT_HIP(hipEventRecord(e1, S.stream));
+        fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+        k_sorted_check<<<grid_for(m), kBlock, 0, S.stream>>>(K0, V0, m, seed, sc + SC_COLLISION);
+        MIBWT_LAUNCH_CHECK();
+        MIBWT_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MIBWT_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (r > 0) total += ms;
+        uint32_t b = 0;
+        S.read(&b, sc + SC_COLLISION);
+        bad += b;
+    }
+    MIBWT_HIP(hipEventDestroy(e0));
+    MIBWT_HIP(hipEventDestroy(e1));
+    MIBWT_REQUIRE(bad == 0, LIBCUBWT_GPU_ERROR, "radix timing: the sort's output is not sorted");
+    *bytes = 24.0 * (double)m;
+    return total / reps / 8.0;
+}
+
 // Re-run the trigger scan kernel `reps` times on the text of the last call.
 double pfp_time_scan(Storage& S, int reps, double* bytes)
 {
@@ -2108,7 +2073,7 @@ double pfp_time_scan(Storage& S, int reps, double* bytes)
     MIBWT_REQUIRE(S.last_W >= 1 && S.last_W <= kScanMaxW, LIBCUBWT_BAD_PARAMETER, "no single-pass scan for this W");
     uint8_t* T = S.buf<uint8_t>(B_TEXT, staged_text_bytes(n));
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    const uint32_t cap = (uint32_t)S.stats.triggers + 1;
+    const uint32_t cap = (uint32_t)S.stats.triggers + 1;  // the triggers and the text's sentinel end
     uint32_t* E = S.buf<uint32_t>(B_TRIG, cap);
     hipEvent_t a, b;
     MIBWT_HIP(hipEventCreate(&a));
@@ -2128,7 +2093,7 @@ double pfp_time_scan(Storage& S, int reps, double* bytes)
     MIBWT_HIP(hipEventDestroy(b));
     uint32_t t = 0;
     S.read(&t, sc + SC_TRIGGERS);
-    MIBWT_REQUIRE((int64_t)t == S.stats.triggers, LIBCUBWT_GPU_ERROR, "trigger count differs between runs");
+    MIBWT_REQUIRE((int64_t)t == S.stats.triggers + 1, LIBCUBWT_GPU_ERROR, "trigger count differs between runs");
     *bytes = (double)n + 4.0 * t;
     return ms / reps;
 }

@@ -327,79 +327,27 @@ void inclusive_max(Storage& S, const T* in, T* out, uint64_t n)
 
 // ----------------------------------------------------------------------------- radix sort
 //
-// Stable LSD radix sort, 8-bit digits, one kernel per digit pass ("onesweep" shape):
-//   - one histogram kernel reads the keys once and counts the digits of every pass;
-//   - each pass kernel ranks its tile (4096 items of 8-byte keys, 8192 of 4-byte keys) with
-//     wavefront multi-split (8 ballots per item), publishes its per-digit counts, obtains the
-//     counts of all earlier tiles by decoupled look-back (one thread per digit), and scatters
-//     keys, then values, through LDS so that every digit run leaves the tile as one
-//     contiguous write.
+// Stable LSD radix sort, 8-bit digits, three kernels per digit pass:
+//   - k_radix_up counts the digits of every tile (4096 items of 8-byte keys, 8192 of 4-byte
+//     keys), digit-major, so that one flat exclusive scan (k_scan_chain) of the counts gives
+//     every (digit, tile) its output start;
+//   - k_radix_pass ranks its tile with wavefront multi-split (8 ballots per item) and scatters
+//     keys, then values, through LDS so that every digit run leaves the tile as one contiguous
+//     write.
+// The tiles need no look-back: a single kernel per pass whose tiles learn the counts of earlier
+// tiles by decoupled look-back spent about half of its time waiting for them (device-coherent
+// status reads cross the XCDs), while the extra read of the keys by k_radix_up costs a third of
+// a pass's algorithmic bytes.
 
 constexpr int kRadixBits = 8;
 constexpr int kRadix = 1 << kRadixBits;
 constexpr int kSortItems = 16;                    // items per thread (8-byte keys)
 constexpr int kMaxPasses = 8;
-constexpr int kHistItems = 16;                    // consecutive keys per thread in the histogram
 
 template <typename K>
 __device__ __forceinline__ uint32_t digit_of(K key, int shift, uint32_t mask)
 {
     return (uint32_t)(key >> shift) & mask;
-}
-
-// Digit counts of all passes: hist[p * kRadix + d] (zeroed by the caller).  Each thread walks
-// kHistItems consecutive keys and adds runs of equal digits at once, which keeps the LDS
-// atomics cheap on presorted high digits.
-template <typename K>
-__global__ __launch_bounds__(kBlock) void k_radix_hist_all(const K* __restrict__ keys, uint64_t n, int lo, int hi,
-                                                          int passes, uint32_t* __restrict__ hist)
-{
-    __shared__ uint32_t h[kMaxPasses * kRadix];
-    for (int i = threadIdx.x; i < passes * kRadix; i += kBlock) h[i] = 0;
-    __syncthreads();
-    const uint64_t base = ((uint64_t)blockIdx.x * kBlock + threadIdx.x) * kHistItems;
-    K k[kHistItems];
-    int cnt_valid = 0;
-    static_assert((kHistItems * sizeof(K)) % 16 == 0, "whole 16-byte words per thread");
-    if (base + kHistItems <= n && ((uintptr_t)keys & 15u) == 0) {
-        // the thread's consecutive keys as 16-byte words (a quarter of the load instructions)
-        const uint4* v = reinterpret_cast<const uint4*>(keys + base);
-#pragma unroll
-        for (int q = 0; q < (int)(kHistItems * sizeof(K) / 16); q++) {
-            const uint4 x = v[q];
-            memcpy(reinterpret_cast<char*>(k) + 16 * q, &x, 16);
-        }
-        cnt_valid = kHistItems;
-    } else {
-#pragma unroll
-        for (int i = 0; i < kHistItems; i++) {
-            k[i] = (base + i < n) ? keys[base + i] : K(0);
-            cnt_valid += (base + i < n) ? 1 : 0;
-        }
-    }
-    for (int p = 0; p < passes; p++) {
-        const int shift = lo + p * kRadixBits;
-        const int nb = (hi - shift) < kRadixBits ? (hi - shift) : kRadixBits;
-        const uint32_t mask = (1u << nb) - 1u;
-        uint32_t prev = 0xFFFFFFFFu, run = 0;
-#pragma unroll
-        for (int i = 0; i < kHistItems; i++) {
-            if (i < cnt_valid) {
-                const uint32_t d = digit_of(k[i], shift, mask);
-                if (d == prev) {
-                    run++;
-                } else {
-                    if (run) atomicAdd(&h[p * kRadix + prev], run);
-                    prev = d;
-                    run = 1;
-                }
-            }
-        }
-        if (run) atomicAdd(&h[p * kRadix + prev], run);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < passes * kRadix; i += kBlock)
-        if (h[i]) atomicAdd(&hist[i], h[i]);
 }
 
 // Default emitter: writes the sorted key/value pair.
@@ -413,50 +361,67 @@ struct EmitPair {
     __device__ __forceinline__ void val(uint64_t pos, V v) const { vals_out[pos] = v; }
 };
 
-// Exclusive prefix of digit d over the tiles before `tile` (one thread per digit).  Status
-// words of kLookbackBatch predecessors are requested at once: device-coherent loads take a round
-// trip to memory, so a one-tile-per-step walk would serialize on that latency, while larger
-// batches cost registers (measured on dna256m: 4 beats 8 by 5% and 16 by 9% of the pass time).
-constexpr int kLookbackBatch = 4;
-__device__ __forceinline__ uint32_t chain_lookback_digit(const Chain& c, uint32_t tile, uint32_t d)
-{
-    uint32_t excl = 0;
-    int64_t j = (int64_t)tile - 1;
-    while (j >= 0) {
-        uint64_t w[kLookbackBatch];
-#pragma unroll
-        for (int k = 0; k < kLookbackBatch; k++)
-            w[k] = (j - k >= 0) ? __hip_atomic_load(c.status + (uint64_t)(j - k) * kRadix + d, __ATOMIC_RELAXED,
-                                                    __HIP_MEMORY_SCOPE_AGENT)
-                                : 0ull;
-        bool done = false;
-#pragma unroll
-        for (int k = 0; k < kLookbackBatch; k++) {
-            if (done || j - k < 0) break;
-            const uint64_t* st = c.status + (uint64_t)(j - k) * kRadix + d;
-            uint64_t x = w[k];
-            while ((uint32_t)(x >> 34) != c.epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                x = __hip_atomic_load(st, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            }
-            excl += (uint32_t)x;
-            done = ((x >> 32) & 3u) == 2u;
-        }
-        if (done) break;
-        j -= kLookbackBatch;
-    }
-    return excl;
-}
-
 // Items keys per thread: 16 for 8-byte keys, 32 for 4-byte keys (the same 32 KiB of staging).
 template <typename K>
 constexpr int sort_items() { return sizeof(K) <= 4 ? 2 * kSortItems : kSortItems; }
 
+// Tile of block b in launches without tickets.  The blocks of a launch are dealt to the 8 XCDs
+// in turn (block b to XCD b mod 8); XCD x is given the contiguous tiles
+// [x q + min(x, r), ...) with q = tiles / 8, r = tiles mod 8, so that neighbouring tiles, whose
+// digit runs meet in the same output lines, run behind one L2.  (Any other dispatch order only
+// changes which XCD runs a tile.)
+__device__ __forceinline__ uint32_t xcd_tile(uint32_t b, uint32_t tiles)
+{
+    constexpr uint32_t kXcd = 8;
+    const uint32_t x = b % kXcd, j = b / kXcd, q = tiles / kXcd, r = tiles % kXcd;
+    return x * q + (x < r ? x : r) + j;
+}
+
+// Digit pass, first half: per tile the count of every digit, stored digit-major
+// (cnt[d * tiles + tile]) so that one flat exclusive scan of cnt gives every (digit, tile) the
+// output position of its first item.  A wavefront whose valid items share one digit adds them
+// with one LDS atomic (presorted high digits), otherwise every item adds itself.
+template <typename K, int Items>
+__global__ __launch_bounds__(kBlock) void k_radix_up(const K* __restrict__ keys, uint64_t n, int shift, uint32_t mask,
+                                                    uint32_t tiles, uint32_t* __restrict__ cnt)
+{
+    constexpr int Tile = kBlock * Items, WaveTile = 64 * Items;
+    __shared__ uint32_t h[kWaves][kRadix];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&h[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = xcd_tile(blockIdx.x, tiles);
+    const uint64_t base = (uint64_t)tile * Tile + (uint64_t)w * WaveTile + lane;
+    K k[Items];
+#pragma unroll
+    for (int i = 0; i < Items; i++) {
+        const uint64_t idx = base + (uint64_t)i * 64;
+        k[i] = (idx < n) ? keys[idx] : K(0);
+    }
+#pragma unroll
+    for (int i = 0; i < Items; i++) {
+        const bool valid = base + (uint64_t)i * 64 < n;
+        const uint32_t d = digit_of(k[i], shift, mask);
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+        const uint64_t vm = __ballot(valid);
+        const uint64_t other = __ballot(valid && d != d0);
+        if (other == 0) {
+            if (lane == 0 && vm) atomicAdd(&h[w][d0], (uint32_t)__popcll(vm));
+        } else if (valid) {
+            atomicAdd(&h[w][d], 1u);
+        }
+    }
+    __syncthreads();
+    const uint32_t d = threadIdx.x;  // kBlock == kRadix
+    cnt[(uint64_t)d * tiles + tile] = h[0][d] + h[1][d] + h[2][d] + h[3][d];
+}
+
+// Digit pass, second half: `start` holds the scanned tile counts of k_radix_up
+// (start[d * tiles + tile] = output position of the tile's first item with digit d).
 template <typename K, typename V, typename Emit, int Items = sort_items<K>()>
 __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ keys, const V* __restrict__ vals,
                                                       uint64_t n, int shift, uint32_t mask,
-                                                      const uint32_t* __restrict__ hist_pass, uint32_t tiles,
-                                                      Chain c, Emit emit)
+                                                      const uint32_t* __restrict__ start, uint32_t tiles, Emit emit)
 {
     static_assert(sizeof(V) <= sizeof(K), "values are staged in the key buffer");
     static_assert(Items % 2 == 0, "slots are packed in pairs");
@@ -472,12 +437,12 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
     __shared__ uint32_t dstart[kRadix];
     __shared__ uint32_t doff[kRadix];  // output position - tile slot, per digit (mod 2^32)
     __shared__ uint32_t scan_lds[kWaves];
-    __shared__ uint32_t slot;
     __shared__ K stage[Tile];
 
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < kWaves * kRadix; i += kBlock) (&wcnt[0][0])[i] = 0;
-    const uint32_t tile = chain_ticket(c, tiles, &slot);  // also orders the zeroing above
+    const uint32_t tile = xcd_tile(blockIdx.x, tiles);
+    __syncthreads();
 
     const uint64_t tile_base = (uint64_t)tile * Tile;
     const uint64_t base = tile_base + (uint64_t)w * WaveTile + lane;
@@ -527,16 +492,7 @@ __global__ __launch_bounds__(kBlock) void k_radix_pass(const K* __restrict__ key
         uint32_t tot;
         const uint32_t ds = block_exclusive_scan(run, 0u, OpSum{}, scan_lds, &tot);
         dstart[d] = ds;
-        const uint32_t gstart = block_exclusive_scan(hist_pass[d], 0u, OpSum{}, scan_lds, &tot);
-        uint32_t excl = 0;
-        if (tile == 0) {
-            chain_publish(c, d, 2, run);
-        } else {
-            chain_publish(c, tile * kRadix + d, 1, run);
-            excl = chain_lookback_digit(c, tile, d);
-            chain_publish(c, tile * kRadix + d, 2, excl + run);
-        }
-        doff[d] = gstart + excl - ds;
+        doff[d] = start[(uint64_t)d * tiles + tile] - ds;
     }
     __syncthreads();
     // ranks become the items' slots in the tile
@@ -592,16 +548,13 @@ __global__ void k_fill(T* p, uint64_t n, T v)
 // input in buffer 0 and ping-pong through buffer 1.  Without `where` the sorted data ends in
 // buffer 0 (copied back after an odd number of passes); with `where` it stays where the last
 // pass wrote it and *where is set to 0 or 1.  vals0/vals1 may be nullptr (keys only).  If
-// `last` is given, the final pass calls it instead of writing a buffer.  `hist_in` (device,
-// kMaxPasses x 256) supplies the digit counts of every pass when the caller already knows them.
+// `last` is given, the final pass calls it instead of writing a buffer.
 template <typename K, typename V, typename LastEmit = EmitPair<K, V>>
 void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, int lo, int hi,
-                const LastEmit* last = nullptr, int* where = nullptr, const uint32_t* hist_in = nullptr)
+                const LastEmit* last = nullptr, int* where = nullptr)
 {
     if (where) *where = 0;
-    bool trivial = false;
     if (n == 0 || hi <= lo) {
-        trivial = true;
         if (last && n) {
             // nothing to sort: still route the data through the emitter with one trivial pass
             lo = 0;
@@ -617,16 +570,7 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
     const uint32_t tiles = ceil_div(n, (uint64_t)kBlock * (half ? kHalf : kFull));
     const int passes = (hi - lo + kRadixBits - 1) / kRadixBits;
     MIBWT_REQUIRE(passes <= kMaxPasses, LIBCUBWT_BAD_PARAMETER, "radix_sort: key too wide");
-    const uint32_t* hist = hist_in;
-    if (!hist_in || trivial) {
-        uint32_t* h = S.buf<uint32_t>(B_SORT_HIST, (size_t)kMaxPasses * kRadix);
-        k_fill<uint32_t><<<ceil_div(kMaxPasses * kRadix, kBlock), kBlock, 0, S.stream>>>(h, kMaxPasses * kRadix, 0u);
-        MIBWT_LAUNCH_CHECK();
-        k_radix_hist_all<K><<<ceil_div(n, (uint64_t)kBlock * kHistItems), kBlock, 0, S.stream>>>(keys0, n, lo, hi,
-                                                                                               passes, h);
-        MIBWT_LAUNCH_CHECK();
-        hist = h;
-    }
+    uint32_t* start = S.buf<uint32_t>(B_SORT_TCNT, (size_t)tiles * kRadix);
     K* kin = keys0;
     K* kout = keys1;
     V* vin = vals0;
@@ -635,30 +579,29 @@ void radix_sort(Storage& S, K* keys0, K* keys1, V* vals0, V* vals1, uint64_t n, 
         const int shift = lo + p * kRadixBits;
         const int nb = (hi - shift) < kRadixBits ? (hi - shift) : kRadixBits;
         const uint32_t mask = (1u << nb) - 1u;
-        const bool is_last = (p == passes - 1);
         const double vbytes = vin ? (double)sizeof(V) : 0.0;
-        const Chain c = S.chain(tiles * kRadix);
-        if (is_last && last) {
-            const int kt = S.kt_begin(1, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
-            if (half)
-                k_radix_pass<K, V, LastEmit, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                                   hist + p * kRadix, tiles, c, *last);
-            else
-                k_radix_pass<K, V, LastEmit, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                                   hist + p * kRadix, tiles, c, *last);
-            S.kt_end(kt);
-        } else {
-            const int kt = S.kt_begin(1, 2.0 * (double)n * (sizeof(K) + vbytes));
-            EmitPair<K, V> e{kout, vout};
-            if (half)
-                k_radix_pass<K, V, EmitPair<K, V>, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                                         hist + p * kRadix, tiles, c, e);
-            else
-                k_radix_pass<K, V, EmitPair<K, V>, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask,
-                                                                                         hist + p * kRadix, tiles, c, e);
-            S.kt_end(kt);
-        }
+        if (half)
+            k_radix_up<K, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, n, shift, mask, tiles, start);
+        else
+            k_radix_up<K, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, n, shift, mask, tiles, start);
         MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, start, start, (uint64_t)tiles * kRadix);
+        auto launch = [&](auto emit, double bytes) {
+            using E = decltype(emit);
+            const int kt = S.kt_begin(1, bytes);
+            if (half)
+                k_radix_pass<K, V, E, kHalf><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, start, tiles,
+                                                                            emit);
+            else
+                k_radix_pass<K, V, E, kFull><<<tiles, kBlock, 0, S.stream>>>(kin, vin, n, shift, mask, start, tiles,
+                                                                            emit);
+            S.kt_end(kt);
+            MIBWT_LAUNCH_CHECK();
+        };
+        if (p == passes - 1 && last)
+            launch(*last, (double)n * (sizeof(K) + vbytes) + LastEmit::bytes_per_item * (double)n);
+        else
+            launch(EmitPair<K, V>{kout, vout}, 2.0 * (double)n * (sizeof(K) + vbytes));
         std::swap(kin, kout);
         std::swap(vin, vout);
     }

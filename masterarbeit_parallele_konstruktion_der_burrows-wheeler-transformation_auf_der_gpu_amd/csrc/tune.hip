@@ -174,6 +174,7 @@ Tuning tuning_for(uint64_t size)
 
 bool tuning_load(const char* path)
 {
+    ensure_default_table();  // the default file is read once; an explicit load then replaces it
     Table t;
     if (!table_load(path, t)) return false;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -265,6 +266,7 @@ int autotune_run(Storage& S, uint64_t max_bytes, int reps, const char* path)
         tuned++;
     }
     if (path && *path) MIBWT_REQUIRE(table_save(path, t), LIBCUBWT_BAD_PARAMETER, "cannot write the tuning table");
+    ensure_default_table();
     std::lock_guard<std::mutex> lk(g_mu);
     g_table = t;
     g_use_table = true;

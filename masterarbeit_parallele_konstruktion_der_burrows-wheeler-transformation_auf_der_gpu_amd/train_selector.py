@@ -115,7 +115,9 @@ def _run_grid(mibwt, st, T: np.ndarray, grid):
             try:
                 st.bwt_device(dT.address, dO.address, n, "pfp", W, P)
             except mibwt.BwtError as e:
-                if e.code != -9:  # only "dictionary beyond the 32-bit index range" is expected
+                # expected only for a dictionary beyond the 32-bit index range, or one that does
+                # not fit the device
+                if e.code not in (mibwt.INDEX_RANGE, mibwt.GPU_NOT_ENOUGH_MEMORY):
                     raise
                 best = None
                 break

@@ -54,18 +54,20 @@ def test_lyndon_dna32m(mibwt):
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
 
 
-@pytest.mark.parametrize("bits", [2, 6])
+@pytest.mark.parametrize("bits", [1, 2])
 def test_fingerprint_collisions_are_caught(mibwt, monkeypatch, bits):
     # narrow first-attempt fingerprints make the suffix comparisons of the Lyndon array and the
     # label order collide: the inversion check of the finished BWT must catch it and rebuild
-    # with full fingerprints, giving the exact BWT (and reporting the retry)
+    # with full fingerprints, giving the exact BWT (and reporting the retry).  Fingerprints are
+    # only consulted past 32 equal bytes, so every text repeats long stretches; with 1 or 2
+    # bits nearly every such comparison is decided by a collision.
     monkeypatch.setenv("MIBWT_LYN_HASH_BITS", str(bits))
     rng = np.random.default_rng(bits)
     retried = 0
     with mibwt.Storage(1 << 16) as st:
         for k in range(6):
             base = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=2000)
-            T = np.concatenate([base, base[7:], base[:900], base]) if k % 2 else rng.integers(0, 3, size=3000).astype(np.uint8)
+            T = np.concatenate([base, base[7:], base[:900], base]) if k % 2 else np.tile(base[:300], 10)
             L, idx = st.bwt(T, "lyndon")
             O, oidx = oracle.bwt(T)
             assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
