@@ -268,6 +268,13 @@ def main():
     if a.calibrate_copy:
         cms, cbytes = st.time_kernel("copy", reps=3, nbytes=float(256 << 20))
         log(f"copy kernel: {cbytes / (cms / 1000) / 1e9 * 2:.1f} GB/s (read+write) over {cbytes:.0f} B")
+    # the whole digit pass (digit counts, their scan, downsweep) on uniformly random pairs
+    rp_items = 1 << 25
+    rp_ms, rp_bytes = st.time_kernel("radix", 3, float(rp_items))
+    radix_whole = {"items": rp_items, "ms_per_pass": round(rp_ms, 4),
+                   "GBps": round(rp_bytes / (rp_ms / 1e3) / 1e9, 1),
+                   "frac": round(rp_bytes / (rp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                   "kernels": "k_radix_up + k_scan_chain + k_radix_pass"}
     avg_ms = ks["kt_ms_total"] / max(1, ks["kt_launches"])
     avg_bytes = ks["kt_bytes_total"] / max(1, ks["kt_launches"])
     achieved = avg_bytes / (avg_ms / 1000.0) / 1e9
@@ -283,7 +290,8 @@ def main():
     roofline = {"bound": "hbm", "kernel": "k_radix_pass", "achieved": round(achieved, 1),
                 "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": traffic, "launches": ks["kt_launches"], "avg_launch_us": round(avg_ms * 1000, 2),
-                "algorithmic_bytes_per_launch": round(avg_bytes), "wp": [a.W, a.P]}
+                "algorithmic_bytes_per_launch": round(avg_bytes), "wp": [a.W, a.P],
+                "radix_whole_pass": radix_whole}
 
     verified = None
     if not a.no_verify:

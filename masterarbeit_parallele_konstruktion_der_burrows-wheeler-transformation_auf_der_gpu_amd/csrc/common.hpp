@@ -132,6 +132,7 @@ struct Storage {
     std::vector<Storage*> pool;
     // state kept for mibwt_time_kernel
     const uint8_t* last_text = nullptr;
+    bool verbose = getenv("MIBWT_VERBOSE") != nullptr;  // per-round counts on stderr
     int64_t last_n = 0;
     int last_W = 0, last_P = 0;
     int64_t last_sort_items = 0;

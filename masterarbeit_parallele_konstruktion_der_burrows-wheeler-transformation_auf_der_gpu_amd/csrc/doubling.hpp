@@ -6,7 +6,9 @@
 //   2. orders each group by that key: groups of up to kSmallGroup members in place (every
 //      member counts the members ordered before it), larger ones by one radix sort of
 //      (group start, key) pairs; the same kernels write the new rank (1 + global position of
-//      the sub-group's first member), the SA slot and the sub-group record of every member;
+//      the sub-group's first member; every member of a group holds 1 + the group's start, so
+//      the members of its first sub-group keep their rank and are not written), the SA slot
+//      and the sub-group record of every member;
 //   3. keeps the sub-groups that can still split (>= 2 members and Policy::open) and compacts
 //      them, which also yields next round's counts in one host read-back.
 // The loop stops when no group can split.  A policy gives next(x, r) and open(x, r), where r
@@ -275,7 +277,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
             r = sr[p];
             NR[g] = r;
         }
-        rank[x] = gs + lt + 1;
+        // members of the group's first sub-group keep their rank (gs + 1): no write
+        if (lt != 0) rank[x] = gs + lt + 1;
         const bool kp = eq > 1 && pol.open(x, r);
         if (!kp) pol.place(SA, gs + (p - a), x, gs + lt);
         keep[g] = kp ? 1u : 0u;
@@ -338,7 +341,7 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
         r = R[i];
         NR[p] = r;
     }
-    rank[x] = gs + lt + 1;
+    if (lt != 0) rank[x] = gs + lt + 1;  // the first sub-group keeps its rank
     const bool kp = c > 1 && pol.open(x, r);
     if (!kp) pol.place(SA, gs + (q - q0), x, gs + lt);
     keep[p] = kp ? 1u : 0u;
@@ -593,6 +596,9 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         }
         const uint32_t m2 = hv[0], nl = hv[1];
         res.rounds++;
+        if (S.verbose)
+            fprintf(stderr, "mibwt: refinement round %d: %u of %u elements active, %u in large groups\n", res.rounds,
+                    m2, N, nl);
         MIBWT_REQUIRE(m2 == 0 || res.rounds < max_rounds, MIBWT_HASH_COLLISION, "groups do not separate");
         res.bytes += (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4);
         if (m2 == 0) break;
