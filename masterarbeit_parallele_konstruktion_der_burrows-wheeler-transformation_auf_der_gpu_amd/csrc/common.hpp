@@ -416,6 +416,8 @@ enum BufId : int {
     B_Q_OCC,
     B_Q_RREC,
     B_DB_LPART,
+    B_DB_PART,
+    B_DB_EXTRA,
     B_E_GFLAG,
     B_E_CID,
     B_E_DELTAC,

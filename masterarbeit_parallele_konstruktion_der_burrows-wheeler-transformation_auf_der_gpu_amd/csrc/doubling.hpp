@@ -96,18 +96,29 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
 
 // ---- one refinement round over the m active elements (groups contiguous)
 
-// The active count is read from the device (*mdev, written by the compaction): the round can be
-// enqueued before the host knows the count, with a grid sized for an upper bound.
+// The slot count is read from the device (*mdev, written by the compaction: the round can be
+// enqueued before the host knows the count, with a grid sized for an upper bound) or, for a round
+// over an uncompacted layout, given by the host (mdev == nullptr, m_host).  In such a layout the
+// slots of finished elements are holes: group size 0 (`live` = the group sizes, else nullptr).
+__device__ __forceinline__ uint32_t slot_count(const uint32_t* mdev, uint32_t m_host)
+{
+    return mdev ? *mdev : m_host;
+}
+
 template <typename Policy>
 __global__ void k_key2(const uint32_t* __restrict__ V, const uint32_t* __restrict__ R,
-                       const uint32_t* __restrict__ mdev, const uint32_t* __restrict__ rank, Policy pol,
-                       uint32_t* __restrict__ key2)
+                       const uint32_t* __restrict__ mdev, uint32_t m_host, const uint32_t* __restrict__ live,
+                       const uint32_t* __restrict__ rank, Policy pol, uint32_t* __restrict__ key2)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= *mdev) return;
+    if (i >= slot_count(mdev, m_host)) return;
+    // all loads of the slot at once (a hole's element and rest are stale but in bounds)
+    const uint32_t z = live ? live[i] : 1u;
     uint32_t r = 0;
     if constexpr (Policy::kCarry) r = R[i];
-    const uint32_t y = pol.next(V[i], r);
+    const uint32_t x = V[i];
+    if (z == 0) return;  // a hole: no key
+    const uint32_t y = pol.next(x, r);
     key2[i] = (y == kNoNext) ? 0u : rank[y];
 }
 
@@ -149,12 +160,13 @@ template <typename Policy>
 __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ Hs,
                                                      const uint32_t* __restrict__ Gs, const uint32_t* __restrict__ Gz,
                                                      const uint32_t* __restrict__ key2, const uint32_t* __restrict__ R,
-                                                     const uint32_t* __restrict__ mdev, uint32_t small_max,
-                                                     uint32_t* __restrict__ rank,
+                                                     const uint32_t* __restrict__ mdev, uint32_t m_host,
+                                                     uint32_t small_max, uint32_t* __restrict__ rank,
                                                      uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
                                                      uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs,
                                                      uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
-                                                     uint32_t* __restrict__ NR, uint32_t* __restrict__ large, Policy pol)
+                                                     uint32_t* __restrict__ NR, uint32_t* __restrict__ large,
+                                                     uint32_t* __restrict__ part, Policy pol)
 {
     constexpr uint32_t kSpan = 2 * kBlock;
     constexpr uint32_t kSr = Policy::kCarry ? kSpan : 1;
@@ -167,19 +179,26 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
     __shared__ uint32_t red[2 * kWaves];
     const uint32_t b0 = blockIdx.x * kBlock;
     const uint32_t t = threadIdx.x;
-    const uint32_t m = *mdev;
-    if (b0 >= m) return;  // the whole block (grid sized for an upper bound)
+    const uint32_t m = slot_count(mdev, m_host);
+    if (b0 >= m) {  // the whole block (grid sized for an upper bound)
+        if (t == 0) part[blockIdx.x] = 0;
+        return;
+    }
 
-    // ---- own slots: classification and staging
+    // ---- own slots: classification and staging (a hole, group size 0, only clears its flags)
     const uint32_t i0 = b0 + t;
     bool own0 = false;
     uint32_t a0 = 0, z0 = 0, gs0 = 0, x0 = 0, k0 = 0, r0 = 0;
     if (i0 < m) {
-        const uint32_t s = Hs[i0];
+        const uint32_t s = Hs[i0];  // loaded with the size (a hole's is stale, unused)
         z0 = Gz[i0];
         const bool lg = z0 > small_max || z0 > kSmallGroup;
         large[i0] = lg ? 1u : 0u;
-        own0 = !lg && s >= b0;
+        if (z0 == 0) {
+            keep[i0] = 0;
+            NGz[i0] = 0;
+        }
+        own0 = z0 != 0 && !lg && s >= b0;
         if (own0) {
             a0 = s - b0;
             gs0 = Gs[i0];
@@ -264,7 +283,8 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
     if (own1 && (p1 + 1 == a1 + z1 || sk[p1 + 1] != kk1)) send[ss1] = p1 + 1;
     __syncthreads();
 
-    // ---- outputs in sorted slot order
+    // ---- outputs in sorted slot order; a finished member leaves a hole (group size 0)
+    uint32_t kept = 0;  // kept members | kept members of large sub-groups << 16
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         const bool own = h ? own1 : own0;
@@ -285,8 +305,13 @@ __global__ __launch_bounds__(kBlock) void k_seg_small(const uint32_t* __restrict
         NV[g] = x;
         NHs[g] = b0 + ss;
         NGs[g] = gs + lt;
-        NGz[g] = eq;
+        NGz[g] = kp ? eq : 0u;
+        if (kp) kept += 1u + ((eq > small_max) ? (1u << 16) : 0u);
     }
+    // this block's kept members (at most 2 * kBlock per field): the next round's counts
+    uint32_t tot;
+    (void)block_exclusive_scan(kept, 0u, OpSum{}, red, &tot);
+    if (t == 0) part[blockIdx.x] = tot;
 }
 
 // Large groups: (group start slot, key) pairs of the flagged members in slot order, with the
@@ -318,17 +343,18 @@ static __global__ void k_lg_sizes(const uint64_t* __restrict__ LK, const uint32_
     if (q + 1 == nl || LK[q] != LK[q + 1]) sz[qh[q]] = q - qh[q] + 1;
 }
 
+// One member of a large group, sorted position q: its new rank, placement or sub-group record.
 template <typename Policy>
-__global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
-                            const uint32_t* __restrict__ qh, const uint32_t* __restrict__ sz,
-                            const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ V,
-                            const uint32_t* __restrict__ R, const uint32_t* __restrict__ Gs, uint32_t nl, int B,
-                            uint32_t* __restrict__ rank, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
-                            uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs,
-                            uint32_t* __restrict__ NGz, uint32_t* __restrict__ NR, Policy pol)
+__device__ __forceinline__ uint32_t lg_update_one(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
+                                                  const uint32_t* __restrict__ qh, const uint32_t* __restrict__ sz,
+                                                  const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ V,
+                                                  const uint32_t* __restrict__ R, const uint32_t* __restrict__ Gs,
+                                                  uint32_t q, int B, uint32_t* __restrict__ rank,
+                                                  uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                                                  uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs,
+                                                  uint32_t* __restrict__ NGs, uint32_t* __restrict__ NGz,
+                                                  uint32_t* __restrict__ NR, uint32_t small_max, Policy pol)
 {
-    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= nl) return;
     const uint32_t i = LV[q];
     const uint32_t x = V[i];
     uint32_t r = 0;
@@ -348,7 +374,31 @@ __global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __r
     NV[p] = x;
     NHs[p] = s + lt;
     NGs[p] = gs + lt;
-    NGz[p] = c;
+    NGz[p] = kp ? c : 0u;
+    return kp ? 1u + ((c > small_max) ? (1u << 16) : 0u) : 0u;
+}
+
+template <typename Policy>
+__global__ void k_lg_update(const uint64_t* __restrict__ LK, const uint32_t* __restrict__ LV,
+                            const uint32_t* __restrict__ qh, const uint32_t* __restrict__ sz,
+                            const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ V,
+                            const uint32_t* __restrict__ R, const uint32_t* __restrict__ Gs, uint32_t nl, int B,
+                            uint32_t* __restrict__ rank, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                            uint32_t* __restrict__ NV, uint32_t* __restrict__ NHs, uint32_t* __restrict__ NGs,
+                            uint32_t* __restrict__ NGz, uint32_t* __restrict__ NR, uint32_t small_max,
+                            uint32_t* __restrict__ extra, Policy pol)
+{
+    __shared__ uint32_t red[kWaves];
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kept = 0;  // as in k_seg_small: kept | kept in large sub-groups << 16
+    if (q < nl) kept = lg_update_one(LK, LV, qh, sz, lpos, V, R, Gs, q, B, rank, SA, keep, NV, NHs, NGs, NGz, NR,
+                                     small_max, pol);
+    uint32_t tot;
+    (void)block_exclusive_scan(kept, 0u, OpSum{}, red, &tot);
+    if (threadIdx.x == 0 && tot) {
+        atomicAdd(extra, tot & 0xFFFFu);
+        atomicAdd(extra + 1, tot >> 16);
+    }
 }
 
 // Compaction of the kept members in one pass over chained tiles of kCompactTile slots: the
@@ -484,6 +534,46 @@ static __global__ __launch_bounds__(kBlock) void k_sum_tiles(const uint32_t* __r
     }
 }
 
+// Next round's counts from the per-block counts of k_seg_small (kept | kept in large sub-groups
+// << 16): every block sums a contiguous chunk and adds it to extra[0], extra[1], which already
+// hold the totals of k_lg_update; k_round_publish then copies them to the scalar slots and
+// publishes them like k_sum_tiles.
+constexpr uint32_t kRoundSumBlocks = 128;
+static __global__ __launch_bounds__(kBlock) void k_round_sum(const uint32_t* __restrict__ part, uint32_t blocks,
+                                                            uint32_t* __restrict__ extra)
+{
+    __shared__ uint32_t lds[kWaves];
+    const uint32_t chunk = (blocks + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = blockIdx.x * chunk, hi = lo + chunk < blocks ? lo + chunk : blocks;
+    uint32_t a = 0, l = 0;
+    for (uint32_t t = lo + threadIdx.x; t < hi; t += kBlock) {
+        const uint32_t v = part[t];
+        a += v & 0xFFFFu;
+        l += v >> 16;
+    }
+    uint32_t ta, tl;
+    (void)block_exclusive_scan(a, 0u, OpSum{}, lds, &ta);
+    (void)block_exclusive_scan(l, 0u, OpSum{}, lds, &tl);
+    if (threadIdx.x == 0) {
+        if (ta) atomicAdd(extra, ta);
+        if (tl) atomicAdd(extra + 1, tl);
+    }
+}
+
+static __global__ void k_round_publish(const uint32_t* __restrict__ extra, uint32_t* __restrict__ active,
+                                       uint32_t* __restrict__ large, uint32_t* __restrict__ sig, uint32_t seq)
+{
+    if (threadIdx.x != 0) return;
+    const uint32_t ta = extra[0], tl = extra[1];
+    *active = ta;
+    *large = tl;
+    if (sig) {
+        __hip_atomic_store(sig, ta, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(sig + 1, tl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(sig + Storage::kSigSeq, seq, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
 struct DoublingResult {
     int rounds = 0;
     double bytes = 0;
@@ -541,15 +631,29 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)N * (8 * 2 + 4 * 9);
     }
-    bool first = true;
-    uint32_t m = N;
+    // A round normally starts with the compaction of the kept sub-groups.  For the string
+    // families, while most slots stay active (an estimated kLazyActive of them), the compaction is
+    // skipped: the round's outputs, written at the same slots, become the next round's layout,
+    // and the slots of finished elements are holes (group size 0) that the kernels pass over;
+    // the counts then come from k_seg_small / k_lg_update.  (The chain policy steps its pointers
+    // over the compacted active set and always compacts.)
+    constexpr double kLazyActive = 0.8;
+    static const bool in_place_ok = [] {  // MIBWT_INPLACE_ROUNDS=0: compact every round
+        const char* e = getenv("MIBWT_INPLACE_ROUNDS");
+        return !(e && e[0] == '0');
+    }();
+    bool first = true, compact = true;
+    uint32_t m = N;  // slots of the current layout (after a compaction: the active elements)
+    uint32_t prev_active = N;
+    uint32_t* part = S.buf<uint32_t>(B_DB_PART, grid_for(N));
+    uint32_t* extra = S.buf<uint32_t>(B_DB_EXTRA, 2);
+    uint32_t seq = 0;
     for (;;) {
-        // ---- compaction of the kept sub-groups with next round's counts
         // the counts come back through the mapped signal words, or, in arena mode (no mapped
         // memory), by a plain read of the scalar slots
         uint32_t* sigd = S.arena ? nullptr : S.signal_dev();
-        const uint32_t seq = S.arena ? 0u : S.signal_next();
-        {
+        if (compact) {
+            seq = S.arena ? 0u : S.signal_next();
             const uint32_t tiles = std::max<uint32_t>(1, ceil_div(m, kCompactTile));
             uint32_t* lpart = S.buf<uint32_t>(B_DB_LPART, tiles);
             const Chain c = S.chain(tiles);
@@ -564,12 +668,21 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             MIBWT_LAUNCH_CHECK();
             k_sum_tiles<<<1, kBlock, 0, S.stream>>>(lpart, tiles, sc + SC_LARGE, sc + SC_ACTIVE, sigd, seq);
             MIBWT_LAUNCH_CHECK();
+        } else {
+            // last round's outputs are this round's layout (its counts are on their way: seq)
+            std::swap(V, NV);
+            std::swap(Hs, NHs);
+            std::swap(Gs, NGs);
+            std::swap(Gz, NGz);
+            if constexpr (Policy::kCarry) std::swap(R, NR);
         }
         // ---- order each group by the rank of the next position: secondary keys and the sort of
         // the small groups; with Policy::kSpeculate enqueued before the count comes back (the
         // kernels read it on the device, grids sized for the previous count), so the device
         // does not idle while the host waits
-        const uint32_t* mdev = sc + SC_ACTIVE;
+        const uint32_t* mdev = compact ? sc + SC_ACTIVE : nullptr;
+        const uint32_t* live = compact ? nullptr : Gz;
+        uint32_t blocks = 0;
         auto sort_round = [&](uint32_t mb) {
             // Policy::kAdvanceFirst: the step to this round's positions is taken at the start of
             // the round (from the second on) over the compacted active set only.  For chains this
@@ -577,11 +690,12 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             // an element that was active in that round, so it was stepped at that round's start.
             if constexpr (Policy::kAdvanceFirst)
                 if (res.rounds > 1) advance(pol, V, mb);
-            k_key2<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, R, mdev, rank, pol, key2);
+            k_key2<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, R, mdev, m, live, rank, pol, key2);
             MIBWT_LAUNCH_CHECK();
             if constexpr (!Policy::kAdvanceFirst) advance(pol, V, mb);
-            k_seg_small<Policy><<<grid_for(mb), kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, mdev, kSmallGroup, rank,
-                                                                       SA, keep, NV, NHs, NGs, NGz, NR, large, pol);
+            blocks = grid_for(mb);
+            k_seg_small<Policy><<<blocks, kBlock, 0, S.stream>>>(V, Hs, Gs, Gz, key2, R, mdev, m, kSmallGroup, rank,
+                                                                 SA, keep, NV, NHs, NGs, NGz, NR, large, part, pol);
             MIBWT_LAUNCH_CHECK();
         };
         if constexpr (Policy::kSpeculate) sort_round(m);
@@ -597,14 +711,15 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         const uint32_t m2 = hv[0], nl = hv[1];
         res.rounds++;
         if (S.verbose)
-            fprintf(stderr, "mibwt: refinement round %d: %u of %u elements active, %u in large groups\n", res.rounds,
-                    m2, N, nl);
+            fprintf(stderr, "mibwt: refinement round %d: %u of %u elements active, %u in large groups (%s)\n",
+                    res.rounds, m2, N, nl, compact ? "compacted" : "in place");
         MIBWT_REQUIRE(m2 == 0 || res.rounds < max_rounds, MIBWT_HASH_COLLISION, "groups do not separate");
-        res.bytes += (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4);
+        res.bytes += compact ? (double)m * 4 + (double)m2 * (4 * 5 + 4 * 4) : 0.0;
         if (m2 == 0) break;
-        m = m2;
+        if (compact) m = m2;
         if constexpr (!Policy::kSpeculate) sort_round(m);
         res.bytes += (double)m * (4 * 12);
+        fill<uint32_t>(S, extra, 2, 0u);
         if (nl > 0) {
             uint32_t* lpos = pos;
             exclusive_sum<uint32_t>(S, large, lpos, m);
@@ -624,10 +739,23 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             k_lg_sizes<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, qh, nl, sz);
             MIBWT_LAUNCH_CHECK();
             k_lg_update<Policy><<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, qh, sz, lpos, V, R, Gs, nl, B, rank,
-                                                                       SA, keep, NV, NHs, NGs, NGz, NR, pol);
+                                                                       SA, keep, NV, NHs, NGs, NGz, NR, kSmallGroup,
+                                                                       extra, pol);
             MIBWT_LAUNCH_CHECK();
             res.large_items += nl;
             res.bytes += (double)nl * (12 * 2 * ((kb + 7) / 8) + 48);
+        }
+        // the next round in place while most slots stay active: its active count is estimated
+        // from this round's shrink (m2 of prev_active) against the m slots of this layout
+        const double next_active = (double)m2 * ((double)m2 / (double)prev_active);
+        prev_active = m2;
+        compact = !(Policy::kSpeculate && in_place_ok && next_active >= kLazyActive * (double)m);
+        if (!compact) {
+            seq = S.arena ? 0u : S.signal_next();
+            k_round_sum<<<std::min(kRoundSumBlocks, blocks), kBlock, 0, S.stream>>>(part, blocks, extra);
+            MIBWT_LAUNCH_CHECK();
+            k_round_publish<<<1, 64, 0, S.stream>>>(extra, sc + SC_ACTIVE, sc + SC_LARGE, sigd, seq);
+            MIBWT_LAUNCH_CHECK();
         }
     }
     return res;
