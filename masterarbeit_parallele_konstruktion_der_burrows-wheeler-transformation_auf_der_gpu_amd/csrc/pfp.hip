@@ -303,6 +303,8 @@ struct TrigScan {
     uint32_t pshift;  // P = 2^pshift * q, q odd
     uint64_t qinv;    // q^-1 mod 2^64
     uint64_t plim;    // floor((2^bits - 1) / P) for the value width in use
+    uint32_t inv24;   // odd P: P^-1 mod 2^24
+    uint32_t plim24;  // odd P: floor((2^24 - 1) / P) << 8 | 0xFF
     uint32_t tiles;
     uint32_t cap;     // capacity of E
     uint32_t* E;
@@ -310,10 +312,21 @@ struct TrigScan {
     Chain chain;
 };
 
-template <typename V, bool kPow2>
+// Divisibility of the window value by P: kDiv 1 (P = 2^s) a mask test; kDiv 2 (odd P, window
+// values below 2^24, i.e. W <= 16) P | v <=> v * P^-1 mod 2^24 <= floor((2^24 - 1) / P), with the
+// full-rate 24-bit multiply (v_mul_u32_u24) whose low 24 bits are compared after a shift by 8
+// (plim24 holds the bound << 8 | 0xFF); kDiv 0: P = 2^s q, rotr(v * q^-1, s) <= floor((2^b - 1) / P).
+constexpr int kDivGeneral = 0, kDivPow2 = 1, kDivOdd24 = 2;
+template <typename V, int kDiv>
 __device__ __forceinline__ bool trig_divisible(V v, const TrigScan& p)
 {
-    if (kPow2) return (v & (V)(p.P - 1)) == 0;
+    if constexpr (kDiv == kDivPow2) return (v & (V)(p.P - 1)) == 0;
+    if constexpr (kDiv == kDivOdd24) {
+        // the compiler turns a 24-bit multiply of a small value into the quarter-rate 32-bit one
+        uint32_t x;
+        asm("v_mul_u32_u24 %0, %1, %2" : "=v"(x) : "v"((uint32_t)v), "s"(p.inv24));
+        return (x << 8) <= p.plim24;
+    }
     const V x = v * (V)p.qinv;
     V r;
     if (sizeof(V) == 4) r = (V)__builtin_rotateright32((uint32_t)x, p.pshift);
@@ -325,7 +338,7 @@ __device__ __forceinline__ bool trig_divisible(V v, const TrigScan& p)
 // sub-tile's bytes from kScanHalo before its first window end (LDS), a/b the thread's own 32
 // bytes; window ends lo <= e < hi of the sub-tile's text count, and its sentinel end `sent` is
 // always set.
-template <typename V, bool kPow2>
+template <typename V, int kDiv>
 __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc, const uint4& a, const uint4& b,
                                                 const TrigScan& p, V v0, const uint8_t* first, uint64_t e0,
                                                 uint64_t lo, uint64_t hi, uint64_t sent)
@@ -351,7 +364,7 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
         const uint32_t cin = (in[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         const uint32_t cout = (out[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         v = (v << 1) + (V)cin - ((V)cout << W);
-        mh = (mh << 1) + (trig_divisible<V, kPow2>(v, p) ? 1u : 0u);
+        mh = (mh << 1) + (trig_divisible<V, kDiv>(v, p) ? 1u : 0u);
         mc = (mc << 1) + ((v == v0) ? 1u : 0u);
     }
     uint32_t mask = __brev(mh);
@@ -443,7 +456,7 @@ __device__ __forceinline__ uint32_t trig_mask32_swar(const uint8_t* txt, uint32_
 // mask per thread; the masks wait in LDS until the tile's offset is known.  Eight sub-tiles
 // per tile balance the number of chained tiles (one ticket each) against the LDS that limits
 // the resident blocks (measured faster than 2, 4, 6, 12, 16 or 32 on 256 MiB and 1 GiB texts).
-template <typename V, bool kPow2, int kScanSub, bool kSwar = false>
+template <typename V, int kDiv, int kScanSub, bool kSwar = false>
 __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
 {
     __shared__ __align__(16) uint8_t txt[kScanHalo + kScanTileW + 16];
@@ -508,7 +521,7 @@ __global__ __launch_bounds__(kBlock) void k_trig_scan(TrigScan p)
             if constexpr (kSwar)
                 m = trig_mask32_swar(txt, loc, a, b, p, first, e0, (uint64_t)base + W, (uint64_t)base + nt, sent);
             else
-                m = trig_mask32<V, kPow2>(txt, loc, a, b, p, v0s, first, e0, (uint64_t)base + W, (uint64_t)base + nt,
+                m = trig_mask32<V, kDiv>(txt, loc, a, b, p, v0s, first, e0, (uint64_t)base + W, (uint64_t)base + nt,
                                           sent);
         }
         masks[sub][threadIdx.x] = m;
@@ -586,16 +599,21 @@ uint32_t trig_scan(Storage& S, const uint8_t* T, uint64_t n, uint32_t n1, const 
     p.total = total_dev;
     p.chain = S.chain(p.tiles);
     const bool pow2 = (q == 1);
+    // odd P and windows of at most 16 bytes (values below 2^24): the 24-bit test
+    const bool odd24 = (s == 0) && W <= 16;
+    p.inv24 = (uint32_t)(p.qinv & 0xFFFFFFu);
+    p.plim24 = ((uint32_t)(0xFFFFFFu / P) << 8) | 0xFFu;
     // P = 2^s with s <= 5: four window ends per word (MIBWT_SCAN_SWAR=0 takes the rolling value)
     const char* sw = getenv("MIBWT_SCAN_SWAR");
     const bool swar = pow2 && s <= 5 && !(sw && atoi(sw) == 0);
     auto launch = [&](auto sub_tag) {
         constexpr int kSub = decltype(sub_tag)::value;
-        if (swar) k_trig_scan<uint32_t, true, kSub, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
-        else if (narrow && pow2) k_trig_scan<uint32_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
-        else if (narrow) k_trig_scan<uint32_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
-        else if (pow2) k_trig_scan<uint64_t, true, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
-        else k_trig_scan<uint64_t, false, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        if (swar) k_trig_scan<uint32_t, kDivPow2, kSub, true><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (narrow && pow2) k_trig_scan<uint32_t, kDivPow2, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (odd24) k_trig_scan<uint32_t, kDivOdd24, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (narrow) k_trig_scan<uint32_t, kDivGeneral, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else if (pow2) k_trig_scan<uint64_t, kDivPow2, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
+        else k_trig_scan<uint64_t, kDivGeneral, kSub><<<p.tiles, kBlock, 0, S.stream>>>(p);
     };
     if (sub == 1) launch(std::integral_constant<int, 1>{});
     else if (sub == 2) launch(std::integral_constant<int, 2>{});

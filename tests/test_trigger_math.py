@@ -44,6 +44,19 @@ def test_divisibility_exhaustive_16bit(P):
     assert (got == (vs % np.uint64(P) == 0)).all()
 
 
+@pytest.mark.parametrize("P", [3, 11, 13, 17, 23, 31, 41, 53, 61, 71, 81, 255, 4095, 65535])
+def test_divisibility_24bit_odd_exhaustive(P):
+    # the kernel's test for odd P and window values below 2^24 (W <= 16): a 24 x 24-bit multiply
+    # (low 32 bits kept), shifted left by 8 within 32 bits, against floor((2^24 - 1) / P) << 8 | 0xFF;
+    # every v < 2^24, so a dropped mask, a wrong inverse width or bound fails here
+    inv24 = inv_odd(P, 64) & 0xFFFFFF
+    lim = ((0xFFFFFF // P) << 8) | 0xFF
+    vs = np.arange(1 << 24, dtype=np.uint64)
+    x = (vs * np.uint64(inv24)) & np.uint64(0xFFFFFFFF)
+    got = ((x << np.uint64(8)) & np.uint64(0xFFFFFFFF)) <= np.uint64(lim)
+    assert (got == (vs % np.uint64(P) == 0)).all()
+
+
 @pytest.mark.parametrize("bits", [32, 64])
 def test_divisibility_random(bits):
     rng = np.random.default_rng(bits)
