@@ -1036,33 +1036,32 @@ __global__ __launch_bounds__(kBlock) void k_asm_fill(const uint32_t* __restrict_
             if (rs[wb + m] <= first) a = m;
             else b = m - 1;
         }
-        uint32_t nxt = (a + 1 < (int)nv) ? rs[wb + a + 1] : 0xFFFFFFFFu;
-        {
-            // common case: the whole granule lies inside one fill run
-            const uint32_t re = rs[wb + a] + rl[wb + a];
-            if (aligned && first == p0 && p0 + 16 <= hi && p0 + 16 <= nxt && p0 + 16 <= re) {
-                const uint32_t c4 = (uint32_t)rc[wb + a] * 0x01010101u;
-                *reinterpret_cast<uint4*>(O + p0) = make_uint4(c4, c4, c4, c4);
-                continue;
-            }
-        }
+        // the fill bytes of the runs that overlap the granule (usually one or two), as a
+        // 16-bit byte mask and four words
         uint32_t bytes[4] = {0, 0, 0, 0};
         uint32_t mask = 0;
+        const uint32_t gend = (p0 + 16 < hi) ? p0 + 16 : hi;
+        for (int r = a; r < (int)nv; r++) {
+            const uint32_t st = rs[wb + r];
+            if (st >= gend) break;
+            const uint32_t ln = rl[wb + r];
+            const uint32_t b0 = (st > first ? st : first) - p0;
+            const uint32_t e = st + ln < gend ? st + ln : gend;
+            if (ln == 0 || e <= p0 + b0) continue;
+            const uint32_t m = ((1u << (e - p0)) - 1u) & ~((1u << b0) - 1u);
+            mask |= m;
+            const uint32_t c4 = (uint32_t)rc[wb + r] * 0x01010101u;
 #pragma unroll
-        for (int k = 0; k < 16; k++) {
-            const uint32_t p = p0 + k;
-            if (p >= first && p < hi) {
-                while (p >= nxt) {
-                    a++;
-                    nxt = (a + 1 < (int)nv) ? rs[wb + a + 1] : 0xFFFFFFFFu;
-                }
-                if (p - rs[wb + a] < rl[wb + a]) {
-                    bytes[k >> 2] |= (uint32_t)rc[wb + a] << (8 * (k & 3));
-                    mask |= 1u << k;
-                }
+            for (int d = 0; d < 4; d++) {
+                const uint32_t nib = (m >> (4 * d)) & 0xFu;  // 4 byte flags -> byte mask
+                bytes[d] |= c4 & ((((nib * 0x00204081u) & 0x01010101u)) * 0xFFu);
             }
         }
-        if (mask == 0xFFFFu && aligned) {
+        if (mask == 0) continue;
+        // a granule inside this wavefront's span holds only its own positions' slots: the bytes
+        // that are not fill bytes belong to ordered groups, which the entry sort writes later,
+        // so one 16-byte store may carry anything there
+        if (aligned && (mask == 0xFFFFu || (p0 >= lo && p0 + 16 <= hi))) {
             *reinterpret_cast<uint4*>(O + p0) = make_uint4(bytes[0], bytes[1], bytes[2], bytes[3]);
         } else {
             while (mask) {
