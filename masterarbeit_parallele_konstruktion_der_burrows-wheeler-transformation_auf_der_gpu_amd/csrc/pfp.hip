@@ -757,16 +757,17 @@ __global__ void k_dt_owner(const uint64_t* __restrict__ off, const uint32_t* __r
 // symbol code 0 = past the phrase end, 1 = terminator, 2.. = dense code of the byte (codes
 // follow byte order, so key order is string order).
 
-// Initial parse key: `per` consecutive phrase ranks (+1; 0 past the end) of B bits each.
+// Initial parse key: `per` consecutive phrase ranks (+1; 0 past the end) as digits in base
+// du + 1.
 __global__ void k_parse_initkey(const uint32_t* __restrict__ lexrank, const uint32_t* __restrict__ pid,
-                                uint32_t d, int B, int per, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
+                                uint32_t d, uint64_t base, int per, uint64_t* __restrict__ K, uint32_t* __restrict__ V)
 {
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= d) return;
     uint64_t key = 0;
     for (int i = 0; i < per; i++) {
         const uint64_t s = (k + i < d) ? (uint64_t)lexrank[pid[k + i]] + 1 : 0;
-        key = (key << B) | s;
+        key = key * base + s;
     }
     K[k] = key;
     V[k] = k;
@@ -857,17 +858,19 @@ __global__ void k_lexrank(const uint32_t* __restrict__ rank, const uint64_t* __r
 //   bit  41      whole phrase (j == 0).
 constexpr uint64_t kRecAlpha = 1ull << 40, kRecWhole = 1ull << 41;
 
-// Per dictionary position x (dictionary order): the initial sort key (the text id of its
-// phrase in the `tbits` bits above the first `per` symbols of its suffix, which are dense codes
-// of `bits` bits, 0 past the phrase end, 1 terminator), V = x, and its DictPos (record: frequency,
-// preceding character, flags; phrase end).  The text id keeps the suffixes of different texts
-// in different groups.
+// Per dictionary position x (dictionary order): the initial sort key (the first `per` symbols
+// of its suffix as digits in base `base` = sigma + 2: 0 past the phrase end, 1 terminator, then
+// the dense codes in byte order; with many texts the text id of its phrase times tpow =
+// base^per above them), V = x, and its DictPos (record: frequency, preceding character, flags;
+// phrase end).  Mixed-radix digits order like the symbol strings and need fewer key bits than
+// whole bits per symbol (DNA: 21 symbols in 55 bits instead of 63, one radix pass less).  The
+// text id keeps the suffixes of different texts in different groups.
 __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restrict__ owner,
                           const uint64_t* __restrict__ off, const uint32_t* __restrict__ len,
                           const uint32_t* __restrict__ tlen, const uint32_t* __restrict__ dtext,
                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
-                          const uint32_t* __restrict__ code, int bits, int per, int tbits, uint32_t L, int W,
-                          uint64_t* __restrict__ K, uint32_t* __restrict__ V, DictPos* __restrict__ dp)
+                          const uint32_t* __restrict__ code, uint32_t base, int per, uint64_t tpow, uint32_t L,
+                          int W, uint64_t* __restrict__ K, uint32_t* __restrict__ V, DictPos* __restrict__ dp)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
@@ -888,9 +891,9 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restr
         if ((uint32_t)i >= rem) s = 0;
         else if ((uint32_t)i >= tr) s = 1;
         else s = cs[(w >> (8 * (i & 7))) & 0xFF];
-        key = (key << bits) | s;
+        key = key * base + s;
     }
-    if (tbits) key |= (uint64_t)dtext[u] << (bits * per);  // bits * per <= 64 - tbits
+    if (tpow) key += (uint64_t)dtext[u] * tpow;  // the text id above the symbols
     K[x] = key;
     V[x] = x;
     uint64_t r = freq[u];
@@ -1380,6 +1383,30 @@ static int init_key_bits(const Storage& S, bool parse = false)
     return v;
 }
 
+// Mixed-radix initial keys: the most digits of base `base` that fit, with a leading field of
+// `lead` values (1: none) above them, in keys below 2^cap.  Returns the digit count (at least
+// 1); *key_bits receives the width of the largest key and *lead_pow base^digits (0 without a
+// leading field).
+static int mixed_radix_digits(uint64_t base, uint64_t lead, int cap, int* key_bits, uint64_t* lead_pow)
+{
+    const unsigned __int128 lim = (unsigned __int128)1 << (cap < 64 ? cap : 64);
+    unsigned __int128 pw = 1;
+    int per = 0;
+    while (per < 64 && pw * base * lead <= lim) {
+        pw *= base;
+        per++;
+    }
+    if (per == 0) {
+        per = 1;
+        pw = base;
+    }
+    const unsigned __int128 top = pw * lead - 1;
+    MIBWT_REQUIRE(top >> 64 == 0, LIBCUBWT_BAD_PARAMETER, "initial keys exceed 64 bits");
+    *key_bits = std::max(1, bits_for((uint64_t)top));
+    *lead_pow = lead > 1 ? (uint64_t)pw : 0;
+    return per;
+}
+
 // Host description of the texts of one build: one text of n1 bytes at 0, or `count` texts laid
 // out as TextSet describes (device arrays tbase, tlen, obase, sub_text).
 struct BuildTexts {
@@ -1621,16 +1648,15 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     // symbols per initial key: dense codes of the bytes that occur (the reference's small-
     // alphabet variant packs two symbols per radix digit, PFPs16:995-1002; here a 64-bit key
     // holds as many codes as fit, 21 for DNA)
-    const int abits = bits_for((uint64_t)sigma + 1);
     // initial key width: whole radix passes trade against one more doubling round; with many
-    // texts the text id takes the bits above the symbols
-    const int tbits = multi ? bits_for(bt.count - 1) : 0;
-    const int key_cap = init_key_bits(S);
-    const int aper = std::max(1, (key_cap - tbits) / abits);
-    MIBWT_REQUIRE(tbits + abits * aper <= 64, LIBCUBWT_BAD_PARAMETER, "too many texts for one build");
+    // texts the text id takes the digits above the symbols
+    int dkey_bits = 0;
+    uint64_t tpow = 0;
+    const int aper = mixed_radix_digits((uint64_t)sigma + 2, multi ? bt.count : 1, init_key_bits(S), &dkey_bits,
+                                        &tpow);
     DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
-    k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dtl, dtext, dstart, freq, codes, abits, aper,
-                                                     tbits, L, W, K0, V0, dp);
+    k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dtl, dtext, dstart, freq, codes,
+                                                     (uint32_t)sigma + 2, aper, tpow, L, W, K0, V0, dp);
     MIBWT_LAUNCH_CHECK();
     // per dictionary position a record (frequency, preceding character, flags); the sort
     // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
@@ -1639,8 +1665,8 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     uint32_t* wbits = S.buf<uint32_t>(B_AS_WBITS, wwords);
     fill<uint32_t>(S, wbits, wwords, 0u);
     LimDict limd{dp};
-    DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, tbits + abits * aper,
-                                                           (uint32_t)aper, limd, drank, nullptr,
+    DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, dkey_bits, (uint32_t)aper, limd,
+                                                           drank, nullptr,
                                                            PlaceRec{dp, RH, wbits});
     S.phase_end(ph, rd.bytes);
     S.stats.doubling_rounds_dict = rd.rounds;
@@ -1658,13 +1684,14 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     K1 = S.buf<uint64_t>(B_SORT_K1, d);
     V0 = S.buf<uint32_t>(B_SORT_V0, d);
     V1 = S.buf<uint32_t>(B_SORT_V1, d);
-    const int B0 = bits_for((uint64_t)du + 1);
-    const int pper = std::max(1, init_key_bits(S, true) / B0);
-    k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, B0, pper, K0, V0);
+    int pkey_bits = 0;
+    uint64_t unused_pow = 0;
+    const int pper = mixed_radix_digits((uint64_t)du + 1, 1, init_key_bits(S, true), &pkey_bits, &unused_pow);
+    k_parse_initkey<<<grid_for(d), kBlock, 0, S.stream>>>(lexrank, pid, d, (uint64_t)du + 1, pper, K0, V0);
     MIBWT_LAUNCH_CHECK();
     uint32_t* prank = S.buf<uint32_t>(B_P_RANK, d);
     uint32_t* psa = S.buf<uint32_t>(B_P_SA, d);
-    DoublingResult rp = prefix_doubling<LimParse>(S, K0, K1, V0, V1, d, B0 * pper, (uint32_t)pper, LimParse{d},
+    DoublingResult rp = prefix_doubling<LimParse>(S, K0, K1, V0, V1, d, pkey_bits, (uint32_t)pper, LimParse{d},
                                                   prank, psa);
     S.phase_end(ph, rp.bytes);
     S.stats.doubling_rounds_parse = rp.rounds;

@@ -200,7 +200,7 @@ int autotune_run(Storage& S, uint64_t max_bytes, int reps, const char* path)
     uint8_t* dO = S.buf<uint8_t>(B_BATCH_OUT, (size_t)max_bytes);
     Table t;
     const Tuning def{};
-    const int subs[4] = {8, 4, 2, 1}, items[2] = {16, 8}, chains[2] = {64, 16}, keys[3] = {64, 48, 32};
+    const int subs[4] = {8, 4, 2, 1}, items[2] = {16, 8}, chains[2] = {64, 16}, keys[5] = {64, 56, 48, 40, 32};
     int tuned = 0;
     for (int b = kMinBucket; b <= top; b++) {
         const uint64_t n = std::min<uint64_t>(max_bytes, (3ull << b) / 4);
