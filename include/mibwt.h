@@ -198,8 +198,9 @@ int64_t mibwt_set_kernel_timing(void* storage, const char* kernel);
  * text of the last PFP call; "copy": a plain 16-byte-per-thread device copy of
  * bytes_per_launch bytes (input), used to calibrate hardware byte counters; "radix": the LSD
  * radix sort of bytes_per_launch (input: an item count) uniformly random 64-bit keys with
- * 32-bit values, its mean duration per digit pass (8 per sort) and 24 bytes per item
- * (returns LIBCUBWT_GPU_ERROR if a sorted result is wrong). */
+ * 32-bit values, with the launch shape of the storage's last build (on a fresh storage: that of
+ * a build of that many bytes), its mean duration per digit pass (8 per sort) and 24 bytes per
+ * item (returns LIBCUBWT_GPU_ERROR if a sorted result is wrong). */
 int64_t mibwt_time_kernel(void* storage, const char* name, int32_t reps, double* ms,
                           double* bytes_per_launch);
 

@@ -2074,7 +2074,8 @@ __global__ void k_sorted_check(const uint64_t* __restrict__ K, const uint32_t* _
 double time_radix(Storage& S, uint64_t m, int reps, double* bytes)
 {
     MIBWT_REQUIRE(m >= 1 && m < (1ull << 31), LIBCUBWT_BAD_PARAMETER, "radix timing: item count out of range");
-    apply_tuning(S, m);  // the launch shape of a build of m bytes
+    // the launch shape of the storage's last build, or on a fresh storage that of a build of m bytes
+    if (S.last_n == 0) apply_tuning(S, m);
     uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, m);
     uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, m);
     uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, m);

@@ -361,9 +361,11 @@ struct EmitPair {
     __device__ __forceinline__ void val(uint64_t pos, V v) const { vals_out[pos] = v; }
 };
 
-// Items keys per thread: 16 for 8-byte keys, 32 for 4-byte keys (the same 32 KiB of staging).
+// Keys per thread: 16 for 8-byte and 4-byte keys.  (32 for 4-byte keys, the same 32 KiB of
+// staging, needs over 190 VGPRs: two wavefronts per SIMD, and its passes ran 2-3x slower per key
+// than the 8-byte ones.)
 template <typename K>
-constexpr int sort_items() { return sizeof(K) <= 4 ? 2 * kSortItems : kSortItems; }
+constexpr int sort_items() { return kSortItems; }
 
 // Tile of block b in launches without tickets.  The blocks of a launch are dealt to the 8 XCDs
 // in turn (block b to XCD b mod 8); XCD x is given the contiguous tiles
