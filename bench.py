@@ -103,6 +103,8 @@ def main():
                     help="timed fixed builds with the default launch shapes (0: skip)")
     ap.add_argument("--grid-steps", type=int, default=0,
                     help="also time every (W,P) candidate with this many builds (0: skip)")
+    ap.add_argument("--radix-pass-items", type=int, default=1 << 25,
+                    help="time the whole radix digit pass on this many random pairs (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a plain 256 MiB device copy kernel (byte-counter reference)")
     a = ap.parse_args()
@@ -269,12 +271,14 @@ def main():
         cms, cbytes = st.time_kernel("copy", reps=3, nbytes=float(256 << 20))
         log(f"copy kernel: {cbytes / (cms / 1000) / 1e9 * 2:.1f} GB/s (read+write) over {cbytes:.0f} B")
     # the whole digit pass (digit counts, their scan, downsweep) on uniformly random pairs
-    rp_items = 1 << 25
-    rp_ms, rp_bytes = st.time_kernel("radix", 3, float(rp_items))
-    radix_whole = {"items": rp_items, "ms_per_pass": round(rp_ms, 4),
-                   "GBps": round(rp_bytes / (rp_ms / 1e3) / 1e9, 1),
-                   "frac": round(rp_bytes / (rp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
-                   "kernels": "k_radix_up + k_scan_chain + k_radix_pass"}
+    radix_whole = None
+    if a.radix_pass_items > 0:
+        rp_items = a.radix_pass_items
+        rp_ms, rp_bytes = st.time_kernel("radix", 3, float(rp_items))
+        radix_whole = {"items": rp_items, "ms_per_pass": round(rp_ms, 4),
+                       "GBps": round(rp_bytes / (rp_ms / 1e3) / 1e9, 1),
+                       "frac": round(rp_bytes / (rp_ms / 1e3) / 1e9 / HBM_PEAK_GBS, 4),
+                       "kernels": "k_radix_up + k_scan_chain + k_radix_pass"}
     avg_ms = ks["kt_ms_total"] / max(1, ks["kt_launches"])
     avg_bytes = ks["kt_bytes_total"] / max(1, ks["kt_launches"])
     achieved = avg_bytes / (avg_ms / 1000.0) / 1e9
