@@ -184,6 +184,8 @@ struct mibwt_phase_stats {
     int64_t retries;                     /* Lyndon: builds repeated after a fingerprint collision */
     int64_t device_bytes;                /* device memory the storage holds after the call (its peak:
                                             buffers only grow) */
+    int64_t induction;                   /* Lyndon: 1 grammar induction over distinct labels,
+                                            2 position-level induction (fallback) */
 };
 int64_t mibwt_last_phase_stats(void* storage, struct mibwt_phase_stats* out);
 

@@ -104,6 +104,7 @@ class PhaseStats(ctypes.Structure):
         ("kt_bytes_total", ctypes.c_double),
         ("retries", ctypes.c_int64),
         ("device_bytes", ctypes.c_int64),
+        ("induction", ctypes.c_int64),
     ]
 
 

@@ -469,6 +469,53 @@ enum BufId : int {
     B_LY_GL,
     B_LY_GF,
     B_LY_SETTLED,
+    // Lyndon grammar: distinct labels, child slots, up-contexts
+    B_GR_SC,        // scalars of the grammar phases (GrSlot)
+    B_GR_REP,       // representative occurrence of each distinct label
+    B_GR_CNT,       // occurrences of each distinct label
+    B_GR_KEY,       // sorted distinct label keys (lookups)
+    B_GR_OFF,       // first child slot of each label (D + 1)
+    B_GR_LAST,      // label of the last child of each label
+    B_GR_ROOT,      // root positions (the stack)
+    B_GR_CHL,       // label of each slot's child
+    B_GR_CHC,       // character before each slot's child
+    B_GR_CONT,      // label owning each slot (kNone: the stack)
+    B_GR_NXT,       // next slot after the slot's run
+    B_GR_RUN,       // run length from each slot; later the scan of the run ends
+    B_GR_CCNT,      // positions an item at each slot stands for
+    B_GR_RKEY,      // dense key of each slot's run (label rank, length)
+    B_GR_LRANK,     // rank of each distinct label
+    B_GR_UCNT,      // up-context count / offsets per slot
+    B_GR_UOFF,      // up-context list offsets per label (D + 1)
+    B_GR_UK0,
+    B_GR_UK1,
+    B_GR_UV0,
+    B_GR_UV1,
+    // Lyndon grammar: items of the induction (double-buffered) and per-round arrays
+    B_IT_Q0,
+    B_IT_Q1,
+    B_IT_G0,
+    B_IT_G1,
+    B_IT_ID0,
+    B_IT_ID1,
+    B_IT_CH0,
+    B_IT_CH1,
+    B_IT_SQ,
+    B_IT_SG,
+    B_IT_SCH,
+    B_IT_P,
+    B_IT_NH,
+    B_IT_OF,
+    B_IT_GST,
+    B_IT_GFI,
+    B_IT_GEXP,
+    B_IT_XID,
+    B_IT_RF,
+    B_IT_RST,
+    B_IT_RLEN,
+    B_IT_RCH,
+    B_IT_EC,
+    B_IT_HR,
     B_AS_R,
     B_AS_WBITS,
     B_AS_WSCAN,

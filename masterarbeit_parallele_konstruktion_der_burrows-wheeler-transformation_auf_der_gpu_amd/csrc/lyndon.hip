@@ -16,15 +16,21 @@
 //              stay open and are resolved by a merge tree over neighbouring ranges, where an
 //              open position bisects the right range's prefix minima.  Suffix comparisons
 //              use the fingerprints for long common prefixes (O(log lcp) per comparison).
-//  3. rsort    positions ordered by P_i: radix sort of the first symbols packed with dense
-//              alphabet codes, then groups that tie on that prefix are ordered with a
-//              fingerprint-LCP comparator (equal fingerprints over the whole prefix = the same
-//              rule); the final order is checked pair by pair byte-exactly and the
-//              fingerprint base is re-drawn if a collision ever broke it.
-//  4. induce   groups of equal P_i are refined by the rank of the following suffix, along
-//              the next-smaller-suffix chain with pointer jumping (segmented group refinement
-//              shared with the prefix-free-parsing pipeline).
-//  5. output   O[j] = T[SA[j]-1] with the wrap to T[n-1].
+//  3. labels   the grammar: one 64-bit key per position (short labels by their symbols, long
+//              ones by fingerprint), one radix sort, the distinct labels with their counts, and
+//              for each distinct label its children (the Lyndon factorization of the label
+//              without its first symbol), runs of equal children and up-contexts.
+//  4. rsort    the distinct labels in lexicographic order: radix sort of the first symbols, tie
+//              groups ordered with a fingerprint-LCP comparator, checked pair by pair.
+//  5. induce   induction over rule rank groups: items are child slots of distinct labels, each
+//              standing for all occurrences of its label; groups are refined by the next run of
+//              labels, and a group whose items share their preceding character fills its BWT
+//              range with that character at once (see section 4).
+//  6. verify   the BWT is inverted on the device and compared with the text; a mismatch (a
+//              fingerprint collision) repeats the build with a new fingerprint base.
+// The position-level induction (every position ordered by its label, groups refined along the
+// next-smaller-suffix chain with pointer jumping) remains as the fallback when the grammar
+// induction outgrows its budget.
 #include <chrono>
 #include <cstdlib>
 
@@ -328,35 +334,53 @@ HashPair* prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base)
     return pre;
 }
 
-// ------------------------------------------------------------------- 3. rule order
+// ------------------------------------------------------------------- 3. label order
+//
+// The labels being ordered are strings of the text: item x stands for
+// T[start(x) .. start(x) + len(x)).  The position-level induction orders every position by its
+// longest Lyndon prefix (PosLab); the grammar orders each distinct label once, read at a
+// representative occurrence (RepLab).
+struct PosLab {
+    const uint32_t* lam;
+    __device__ __forceinline__ uint32_t start(uint32_t x) const { return x; }
+    __device__ __forceinline__ uint32_t len(uint32_t x) const { return lam[x]; }
+};
+struct RepLab {
+    const uint32_t* rep;
+    const uint32_t* lam;
+    __device__ __forceinline__ uint32_t start(uint32_t x) const { return rep[x]; }
+    __device__ __forceinline__ uint32_t len(uint32_t x) const { return lam[rep[x]]; }
+};
 
-// Key of position i: the first `per` symbols of P_i as dense codes (0 past its end).
-__global__ void k_lyn_initkey(const uint8_t* __restrict__ T, uint64_t n, const uint32_t* __restrict__ lam,
+// Key of item x: the first `per` symbols of its label as dense codes (0 past its end).
+template <typename Lab>
+__global__ void k_lyn_initkey(const uint8_t* __restrict__ T, uint64_t n, Lab lab, uint32_t m,
                               const uint32_t* __restrict__ code, int bits, int per, uint64_t* __restrict__ K,
                               uint32_t* __restrict__ V)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
     __syncthreads();
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t l = lam[i];
+    const uint64_t x = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= m) return;
+    const uint32_t i = lab.start((uint32_t)x), l = lab.len((uint32_t)x);
     uint64_t key = 0, w = 0;
     for (int k = 0; k < per; k++) {
         if ((k & 7) == 0) w = (i + k < n) ? load8(T, i + k) : 0ull;
         const uint64_t s = ((uint32_t)k < l) ? cs[(w >> (8 * (k & 7))) & 0xFF] : 0ull;
         key = (key << bits) | s;
     }
-    K[i] = key;
-    V[i] = (uint32_t)i;
+    K[x] = key;
+    V[x] = (uint32_t)x;
 }
 
-// fingerprint of every label P_i (compared before any longest-common-prefix search: most tie
+// fingerprint of every label (compared before any longest-common-prefix search: most tie
 // groups of a collection are copies of one label)
-__global__ void k_label_hash(PrefixHash ph, const uint32_t* __restrict__ lam, uint64_t n, uint64_t* __restrict__ F)
+template <typename Lab>
+__global__ void k_label_hash(PrefixHash ph, Lab lab, uint32_t m, uint64_t* __restrict__ F)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) F[i] = ph.sub((uint32_t)i, lam[i]);
+    const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < m) F[x] = ph.sub(lab.start(x), lab.len(x));
 }
 
 // Common prefix of T[x..x+m) and T[y..y+m) by fingerprints, knowing that the first `known`
@@ -375,20 +399,22 @@ __device__ __forceinline__ uint32_t label_lcp(const PrefixHash& ph, uint32_t x, 
     return lo;
 }
 
+template <typename Lab>
 struct LabelCmp {
     const uint8_t* T;
-    const uint32_t* lam;
+    Lab lab;
     PrefixHash ph;
     const uint64_t* F;  // label fingerprints
-    // -1 / 0 / +1 for P_x against P_y, knowing their first `known` symbols agree
+    // -1 / 0 / +1 for label x against label y, knowing their first `known` symbols agree
     __device__ int operator()(uint32_t x, uint32_t y, uint32_t known) const
     {
         if (x == y) return 0;
-        const uint32_t lx = lam[x], ly = lam[y];
+        const uint32_t lx = lab.len(x), ly = lab.len(y);
         const uint32_t m = lx < ly ? lx : ly;
         if (lx == ly && F[x] == F[y]) return 0;
-        const uint32_t lo = label_lcp(ph, x, y, known, m);
-        if (lo < m) return T[x + lo] < T[y + lo] ? -1 : 1;
+        const uint32_t sx = lab.start(x), sy = lab.start(y);
+        const uint32_t lo = label_lcp(ph, sx, sy, known, m);
+        if (lo < m) return T[sx + lo] < T[sy + lo] ? -1 : 1;
         return lx < ly ? -1 : (lx > ly ? 1 : 0);
     }
 };
@@ -403,47 +429,49 @@ __global__ void k_lab_heads_from_keys(const uint64_t* __restrict__ K, uint64_t n
 
 // Per element: is its group a tie group (>= 2 members whose labels all reach past `known`
 // symbols), and is that group small or large?  cls: 0 = settled, 1 = small tie, 2 = large tie.
+template <typename Lab>
 __global__ void k_lab_classify(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
-                               const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ lam, uint64_t n,
-                               uint32_t known, uint32_t small, const uint8_t* __restrict__ settled,
-                               uint8_t* __restrict__ cls, uint32_t* __restrict__ large)
+                               const uint32_t* __restrict__ gofs, Lab lab, uint32_t m, uint32_t known, uint32_t small,
+                               const uint8_t* __restrict__ settled, uint8_t* __restrict__ cls,
+                               uint32_t* __restrict__ large)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t s = gofs[g], e = gofs[g + 1];
     uint8_t c = 0;
-    if (e - s >= 2 && lam[V[s]] >= known && !settled[s]) c = (e - s > small) ? 2 : 1;
+    if (e - s >= 2 && lab.len(V[s]) >= known && !settled[s]) c = (e - s > small) ? 2 : 1;
     cls[i] = c;
     large[i] = (c == 2) ? 1u : 0u;
 }
 
-// Small tie groups: comparator counting sort; settled groups are copied.  hf receives the
-// heads of the refined order (first of every run of equal labels).
-// label length and fingerprint of every position in the current order (the comparisons of a
-// tie group then read them coalesced; most members of a collection's groups are equal labels)
-__global__ void k_lab_gather(const uint32_t* __restrict__ V, const uint32_t* __restrict__ lam,
-                             const uint64_t* __restrict__ F, uint64_t n, uint32_t* __restrict__ GL,
-                             uint64_t* __restrict__ GF)
+// label length and fingerprint of every item in the current order (the comparisons of a tie
+// group then read them coalesced; most members of a collection's groups are equal labels)
+template <typename Lab>
+__global__ void k_lab_gather(const uint32_t* __restrict__ V, Lab lab, const uint64_t* __restrict__ F, uint32_t m,
+                             uint32_t* __restrict__ GL, uint64_t* __restrict__ GF)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
     const uint32_t x = V[i];
-    GL[i] = lam[x];
+    GL[i] = lab.len(x);
     GF[i] = F[x];
 }
 
+// Small tie groups: comparator counting sort; settled groups are copied.  hf receives the
+// heads of the refined order (first of every run of equal labels).
+template <typename Lab>
 __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
-                            const uint32_t* __restrict__ gofs, const uint8_t* __restrict__ cls, uint64_t n,
-                            uint32_t known, LabelCmp cmp, const uint32_t* __restrict__ GL,
+                            const uint32_t* __restrict__ gofs, const uint8_t* __restrict__ cls, uint32_t m,
+                            uint32_t known, LabelCmp<Lab> cmp, const uint32_t* __restrict__ GL,
                             const uint64_t* __restrict__ GF, uint32_t* __restrict__ V2, uint32_t* __restrict__ hf)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t s = gofs[g], e = gofs[g + 1];
     const uint32_t x = V[i];
-    if (cls[i] == 2) return;  // large groups are written by the chunk pass
+    if (cls[i] == 2) return;  // large groups are written by the pivot pass
     if (cls[i] == 0) {
         V2[i] = x;
         hf[i] = (i == s) ? 1u : 0u;
@@ -455,8 +483,8 @@ __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __re
     for (uint32_t j = s; j < e; j++) {
         if (j == i) continue;
         const int c = (GL[j] == lx && GF[j] == fx) ? 0 : cmp(V[j], x, known);
-        before += (c < 0) || (c == 0 && j < (uint32_t)i);
-        equal_before += (c == 0 && j < (uint32_t)i);
+        before += (c < 0) || (c == 0 && j < i);
+        equal_before += (c == 0 && j < i);
     }
     V2[s + before] = x;
     hf[s + before] = (equal_before == 0) ? 1u : 0u;
@@ -472,27 +500,28 @@ __global__ void k_lab_small(const uint32_t* __restrict__ V, const uint32_t* __re
 // at l); the dense id of the large group goes above it (k_lab_pivot_ids).  Every round removes
 // at least the pivot's class from each group, so a group of g labels needs at most g rounds,
 // and a run such as a^k b (labels a^j b that all share long prefixes) is sorted in one.
-
-__global__ void k_lab_pivot_gather(PrefixHash ph, const uint8_t* __restrict__ T, uint64_t n,
+template <typename Lab>
+__global__ void k_lab_pivot_gather(PrefixHash ph, const uint8_t* __restrict__ T, uint32_t m,
                                    const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                                    const uint32_t* __restrict__ gofs, const uint32_t* __restrict__ large,
-                                   const uint32_t* __restrict__ lpos, const uint32_t* __restrict__ lam, uint32_t known,
-                                   int lbits, uint64_t* __restrict__ LK, uint32_t* __restrict__ LV, uint32_t* __restrict__ LI,
+                                   const uint32_t* __restrict__ lpos, Lab lab, uint32_t known, int lbits,
+                                   uint64_t* __restrict__ LK, uint32_t* __restrict__ LV, uint32_t* __restrict__ LI,
                                    uint32_t* __restrict__ LG)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !large[i]) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m || !large[i]) return;
     const uint32_t g = gid1[i] - 1;
     const uint32_t x = V[i], p = V[gofs[g]];
-    const uint32_t lx = lam[x], lp = lam[p];
-    const uint32_t m = lx < lp ? lx : lp;
-    const uint32_t l = (x == p) ? m : label_lcp(ph, x, p, known, m);
+    const uint32_t lx = lab.len(x), lp = lab.len(p);
+    const uint32_t sx = lab.start(x), sp = lab.start(p);
+    const uint32_t mm = lx < lp ? lx : lp;
+    const uint32_t l = (x == p) ? mm : label_lcp(ph, sx, sp, known, mm);
     uint64_t cls, lpart = 0, ch = 0;
-    if (l == m && lx == lp) {
+    if (l == mm && lx == lp) {
         cls = 1;  // EQUAL
     } else {
-        const uint32_t cx = (l < lx) ? (uint32_t)T[x + l] + 1u : 0u;  // 0: x ends at l
-        const uint32_t cp = (l < lp) ? (uint32_t)T[p + l] + 1u : 0u;
+        const uint32_t cx = (l < lx) ? (uint32_t)T[sx + l] + 1u : 0u;  // 0: x ends at l
+        const uint32_t cp = (l < lp) ? (uint32_t)T[sp + l] + 1u : 0u;
         const uint32_t lmax = (uint32_t)((1ull << lbits) - 1);
         const uint32_t d = (l > known) ? l - known : 0u;
         const uint32_t lc = d < lmax ? d : lmax;
@@ -508,7 +537,7 @@ __global__ void k_lab_pivot_gather(PrefixHash ph, const uint8_t* __restrict__ T,
     const uint32_t q = lpos[i];
     LK[q] = (cls << (9 + lbits)) | (lpart << 9) | ch;
     LV[q] = x;
-    LI[q] = (uint32_t)i;
+    LI[q] = i;
     LG[q] = g;
 }
 
@@ -543,19 +572,20 @@ __global__ void k_lab_pivot_scatter(const uint64_t* __restrict__ LK, const uint3
 // they share a label group.  The common prefix of two labels is found through the fingerprints
 // (O(log lcp), so long runs of equal labels cost no more than short ones); a fingerprint
 // collision that slips through is caught by the exact inversion check of the finished BWT.
-__global__ void k_lab_verify(SuffixCmp sc, const uint32_t* __restrict__ V, const uint32_t* __restrict__ hf,
-                             const uint32_t* __restrict__ lam, uint32_t* __restrict__ bad)
+template <typename Lab>
+__global__ void k_lab_verify(SuffixCmp sc, Lab lab, uint32_t m, const uint32_t* __restrict__ V,
+                             const uint32_t* __restrict__ hf, uint32_t* __restrict__ bad)
 {
-    const uint64_t n = sc.n;
     const uint8_t* T = sc.T;
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 || i >= n) return;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 || i >= m) return;
     const uint32_t a = V[i - 1], b = V[i];
-    const uint32_t la = lam[a], lb = lam[b];
-    const uint32_t m = la < lb ? la : lb;
-    const uint32_t k = (a == b) ? m : sc.lcp_upto(a, b, m);
+    const uint32_t la = lab.len(a), lb = lab.len(b);
+    const uint32_t sa = lab.start(a), sb = lab.start(b);
+    const uint32_t mm = la < lb ? la : lb;
+    const uint32_t k = (sa == sb) ? mm : sc.lcp_upto(sa, sb, mm);
     int c;
-    if (k < m) c = T[a + k] < T[b + k] ? -1 : 1;
+    if (k < mm) c = T[sa + k] < T[sb + k] ? -1 : 1;
     else c = la < lb ? -1 : (la > lb ? 1 : 0);
     const bool head = hf[i] != 0;
     if (c > 0 || (c == 0) == head) atomicOr(bad, 1u);
@@ -567,7 +597,838 @@ __global__ void k_u32_to_key(const uint32_t* __restrict__ a, uint64_t n, uint64_
     if (i < n) K[i] = a[i];
 }
 
-// ------------------------------------------------------------------- 4. induction
+// Orders m labels.  On return *v_out holds the items in label order and *hf_out flags the first
+// item of every run of equal labels.  Returns false when the final neighbour check fails (a
+// fingerprint collision broke the order).  Labels that agree on their first `per` symbols (the
+// packed initial key) are refined in rounds: tie groups of up to kLabSmall members by the
+// comparator counting sort, larger ones by the pivot partition.
+template <typename Lab>
+static bool label_order(Storage& S, const uint8_t* T, uint64_t n, const PrefixHash& phh, Lab lab, uint32_t m,
+                        const uint32_t* codes, uint32_t sigma, uint32_t** v_out, uint32_t** hf_out, double* bytes)
+{
+    uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
+    const int bits = bits_for((uint64_t)sigma + 1);
+    const int per = 64 / bits;
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, m);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, m);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, m);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, m);
+    uint32_t* hf = S.buf<uint32_t>(B_LY_HF, m);
+    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, m);
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, (size_t)m + 1);
+    uint32_t* large = S.buf<uint32_t>(B_DB_LARGE, m);
+    uint32_t* lpos = S.buf<uint32_t>(B_ACT_X, m);
+    uint8_t* cls = S.buf<uint8_t>(B_LY_CLS, m);
+    uint32_t* Vl = S.buf<uint32_t>(B_LY_VL, m);
+    k_lyn_initkey<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(T, n, lab, m, codes, bits, per, K0, V0);
+    MIBWT_LAUNCH_CHECK();
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, m, 0, bits * per);
+    k_lab_heads_from_keys<<<grid_for(m), kBlock, 0, S.stream>>>(K0, m, hf);
+    MIBWT_LAUNCH_CHECK();
+    double rbytes = (double)m * 24 * ((bits * per + 7) / 8);
+    uint64_t* LF = S.buf<uint64_t>(B_LY_LABH, m);
+    k_label_hash<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(phh, lab, m, LF);
+    MIBWT_LAUNCH_CHECK();
+    LabelCmp<Lab> cmp{T, lab, phh, LF};
+    // working copies: V (order) and H (heads) of the packed-key order
+    uint32_t* V = Vl;
+    MIBWT_HIP(hipMemcpyAsync(V, V0, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, S.stream));
+    uint32_t* H = S.buf<uint32_t>(B_LY_HF2, m);
+    MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, S.stream));
+    const uint32_t known = (uint32_t)per;  // every tie group agrees on its first `per` symbols
+    uint32_t small = kLabSmall;
+    if (const char* e = getenv("MIBWT_LAB_SMALL")) {
+        const int v = atoi(e);
+        if (v >= 2 && v <= 256) small = (uint32_t)v;
+    }
+    uint8_t* settled = S.buf<uint8_t>(B_LY_SETTLED, m);
+    fill<uint8_t>(S, settled, m, (uint8_t)0);
+    for (int round = 0;; round++) {
+        MIBWT_REQUIRE(round < 1 << 20, LIBCUBWT_GPU_ERROR, "label refinement does not converge");
+        inclusive_sum<uint32_t>(S, H, gid1, m, sc + SC_GROUPS);
+        k_grp_offsets<<<grid_for(m), kBlock, 0, S.stream>>>(H, gid1, m, gofs);
+        MIBWT_LAUNCH_CHECK();
+        k_lab_classify<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(V, gid1, gofs, lab, m, known, small, settled, cls,
+                                                                   large);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, large, lpos, m, sc + SC_LARGE);
+        uint32_t hv[SC_COUNT_];
+        S.read_fast(hv, sc, SC_COUNT_);
+        const uint32_t nl = hv[SC_LARGE];
+        uint32_t* V2 = (V == Vl) ? V1 : Vl;
+        uint32_t* GL = S.buf<uint32_t>(B_LY_GL, m);
+        uint64_t* GF = S.buf<uint64_t>(B_LY_GF, m);
+        k_lab_gather<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(V, lab, LF, m, GL, GF);
+        MIBWT_LAUNCH_CHECK();
+        k_lab_small<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(V, gid1, gofs, cls, m, known, cmp, GL, GF, V2, H);
+        MIBWT_LAUNCH_CHECK();
+        if (nl > 0) {
+            uint64_t* LK0 = S.buf<uint64_t>(B_LG_K0, nl);
+            uint64_t* LK1 = S.buf<uint64_t>(B_LG_K1, nl);
+            uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
+            uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
+            uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
+            uint32_t* LG = S.buf<uint32_t>(B_LG_SZ, nl);
+            uint32_t* hq = S.buf<uint32_t>(B_LY_SCAN0, nl);
+            // large groups hold more than `small` members each, so their dense ids take gb
+            // bits; the l part takes what is left of the 64-bit key (at most 31 bits)
+            const int gb = bits_for((uint64_t)nl / (small + 1));
+            const int lbits = std::min(31, 64 - 11 - gb);
+            k_lab_pivot_gather<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(phh, T, m, V, gid1, gofs, large, lpos, lab,
+                                                                          known, lbits, LK0, LV0, LI, LG);
+            MIBWT_LAUNCH_CHECK();
+            k_lab_pivot_heads<<<grid_for(nl), kBlock, 0, S.stream>>>(LG, nl, hq);
+            MIBWT_LAUNCH_CHECK();
+            inclusive_sum<uint32_t>(S, hq, hq, nl, sc + SC_GROUPS);
+            k_lab_pivot_ids<<<grid_for(nl), kBlock, 0, S.stream>>>(hq, nl, 11 + lbits, LK0);
+            MIBWT_LAUNCH_CHECK();
+            radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, 11 + lbits + gb);
+            k_lab_pivot_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, lbits, V2, H, settled);
+            MIBWT_LAUNCH_CHECK();
+            rbytes += (double)nl * 24 * 8;
+        }
+        V = V2;
+        if (nl == 0) break;
+    }
+    // check of the label order
+    fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
+    k_lab_verify<Lab><<<grid_for(m), kBlock, 0, S.stream>>>(SuffixCmp{T, n, phh}, lab, m, V, H, sc + SC_COLLISION);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t bad = 0;
+    S.read_fast(&bad, sc + SC_COLLISION);
+    *bytes += rbytes;
+    if (bad != 0) return false;
+    if (V != Vl) MIBWT_HIP(hipMemcpyAsync(Vl, V, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(hf, H, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, S.stream));
+    *v_out = Vl;
+    *hf_out = hf;
+    return true;
+}
+
+// ------------------------------------------------------------------- 4. the grammar
+//
+// Every position i starts exactly one maximal node of the Lyndon forest, its label
+// P_i = T[i .. i + lam[i]) (the reference's rules: right children and stack words).  A label M of
+// length >= 2 has the children c_1 = i + 1, c_{k+1} = c_k + lam[c_k] < i + lam[i]: the Lyndon
+// factorization of M without its first symbol, non-increasing.  The children's labels are labels
+// themselves, so the distinct labels with their child lists are the grammar; the roots (positions
+// that no earlier label covers: the Lyndon factorization of T) are the stack.
+//
+// The suffix at a position is the sequence of labels along its next-smaller-suffix chain (its
+// Lyndon factorization), and suffixes compare like these sequences, a run of k equal labels
+// comparing as one symbol (label, k): more copies is larger, since the label after a run is
+// smaller.  A run never reaches past the end of the label whose children it belongs to, because
+// what follows a label M is at most M and M is smaller than each of its children; so the run
+// length from every child slot is a property of the label alone.
+//
+// Slots: child k of distinct label d is slot off[d] + k; root s is slot C + s; slot Z = C + S is
+// the end of the text.  An item is a slot standing for the positions of that child in every
+// occurrence of its label (cnt[d] of them; one for a root slot), all preceded by the same
+// character.  After the run that starts at slot q an item continues at nxt[q], the slot after
+// the run in the same label (or the next root), or, when the run ends its label d, at each of
+// d's up-contexts: the slots that follow d where d occurs as a child.  d occurs exactly at the
+// ends of the last-child chains (v, lastchild(v), ...) of the children v that are not the last
+// in their label (the context is the next child slot) and of the roots (the next root slot).
+// The counts of the up-contexts of d add up to cnt[d].
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+
+// Slots of B_GR_SC.
+enum GrSlot : int {
+    GR_D = 0,     // distinct labels
+    GR_S,         // roots
+    GR_C,         // child slots
+    GR_UT,        // up-context pairs
+    GR_NR,        // distinct run keys
+    GR_ERR,       // an inconsistent grammar (a child label missing, a run past the end): colliding keys
+    GR_M,         // items of the next round
+    GR_G,         // their groups
+    GR_RUNS,      // runs written this round
+    GR_RB,        // bytes of those runs
+    GR_NG,        // groups of this round
+    GR_PIDX,      // primary index (BWT position of suffix 0, plus one)
+    GR_COUNT_ = 16
+};
+
+// Identity of the label at position i as one 64-bit key: labels of at most `per` symbols by
+// their dense codes (exact; top bit clear), longer ones by their fingerprint mixed with the
+// length (top bit set).  Equal labels always get equal keys.
+struct LabelKey {
+    const uint8_t* T;
+    uint64_t n;
+    const uint32_t* lam;
+    PrefixHash ph;
+    int bits, per;
+    __device__ __forceinline__ uint64_t operator()(uint32_t i, const uint32_t* cs) const
+    {
+        const uint32_t l = lam[i];
+        if (l <= (uint32_t)per) {
+            uint64_t key = 0, w = 0;
+            for (int k = 0; k < per; k++) {
+                if ((k & 7) == 0) w = ((uint64_t)i + k < n) ? load8(T, (uint64_t)i + k) : 0ull;
+                const uint64_t s = ((uint32_t)k < l) ? cs[(w >> (8 * (k & 7))) & 0xFF] : 0ull;
+                key = (key << bits) | s;
+            }
+            return key;
+        }
+        const uint64_t f = ph.sub(i, l) ^ ((uint64_t)l * 0x9E3779B97F4A7C15ull);
+        return (1ull << 63) | (f >> 1);
+    }
+};
+
+// index of `key` in the sorted distinct keys U[0..D); sets *err when it is missing
+__device__ __forceinline__ uint32_t key_lookup(const uint64_t* __restrict__ U, uint32_t D, uint64_t key,
+                                               uint32_t* err)
+{
+    uint32_t lo = 0, hi = D;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (U[mid] < key) lo = mid + 1;
+        else hi = mid;
+    }
+    if (lo >= D || U[lo] != key) {
+        atomicOr(err, 1u);
+        return 0;
+    }
+    return lo;
+}
+
+__global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint64_t* __restrict__ K,
+                          uint32_t* __restrict__ V)
+{
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    __syncthreads();
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= lk.n) return;
+    K[i] = lk((uint32_t)i, cs);
+    V[i] = (uint32_t)i;
+}
+
+// per distinct label d (the runs of equal sorted keys): representative, count, key
+__global__ void k_gl_dedup(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V,
+                           const uint32_t* __restrict__ gofs, uint32_t D, uint32_t* __restrict__ rep,
+                           uint32_t* __restrict__ cnt, uint64_t* __restrict__ U)
+{
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    const uint32_t j = gofs[d];
+    rep[d] = V[j];
+    cnt[d] = gofs[d + 1] - j;
+    U[d] = K[j];
+}
+
+__global__ void k_gl_ends(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ E)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) E[i] = (uint32_t)i + lam[i];
+}
+
+// root flags from the prefix maxima PM of i + lam[i]: no label starting before k reaches past k
+__global__ void k_gl_rootflag(const uint32_t* __restrict__ PM, uint64_t n, uint32_t* __restrict__ rf)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) rf[k] = (k == 0 || PM[k - 1] <= k) ? 1u : 0u;
+}
+
+__global__ void k_gl_roots(const uint32_t* __restrict__ PM, const uint32_t* __restrict__ rpos, uint64_t n,
+                           uint32_t* __restrict__ roots)
+{
+    const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n && (k == 0 || PM[k - 1] <= k)) roots[rpos[k]] = (uint32_t)k;
+}
+
+// number of children of each distinct label
+__global__ void k_gl_nchild(const uint32_t* __restrict__ rep, const uint32_t* __restrict__ lam, uint32_t D,
+                            uint32_t* __restrict__ t)
+{
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    const uint32_t p = rep[d], e = p + lam[p];
+    uint32_t c = 0;
+    for (uint32_t x = p + 1; x < e; x += lam[x]) c++;
+    t[d] = c;
+}
+
+// child slots of each distinct label: child label, preceding character, owner; last child
+__global__ void k_gl_children(LabelKey lk, const uint32_t* __restrict__ code, const uint32_t* __restrict__ rep,
+                              const uint32_t* __restrict__ off, uint32_t D, const uint64_t* __restrict__ U,
+                              uint32_t* __restrict__ chl, uint8_t* __restrict__ chc, uint32_t* __restrict__ cont,
+                              uint32_t* __restrict__ last, uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];
+    __syncthreads();
+    const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
+    if (d >= D) return;
+    const uint32_t p = rep[d], e = p + lk.lam[p];
+    uint32_t q = off[d], l = kNone;
+    for (uint32_t x = p + 1; x < e; x += lk.lam[x], q++) {
+        l = key_lookup(U, D, lk(x, cs), err);
+        chl[q] = l;
+        chc[q] = lk.T[x - 1];
+        cont[q] = d;
+    }
+    last[d] = l;
+}
+
+// root slots C + s
+__global__ void k_gl_rootslots(LabelKey lk, const uint32_t* __restrict__ code, const uint32_t* __restrict__ roots,
+                               uint32_t S, uint32_t C, const uint64_t* __restrict__ U, uint32_t D,
+                               uint32_t* __restrict__ chl, uint8_t* __restrict__ chc, uint32_t* __restrict__ cont,
+                               uint32_t* __restrict__ err)
+{
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];
+    __syncthreads();
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S) return;
+    const uint32_t k = roots[s];
+    chl[C + s] = key_lookup(U, D, lk(k, cs), err);
+    chc[C + s] = k ? lk.T[k - 1] : lk.T[lk.n - 1];
+    cont[C + s] = kNone;
+}
+
+// run breaks, reversed for a prefix-maximum scan: RV[Z-1-q] = Z-1-q where a run ends at slot q
+__global__ void k_gl_brk(const uint32_t* __restrict__ chl, const uint32_t* __restrict__ cont, uint32_t Z,
+                         uint32_t* __restrict__ RV)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Z) return;
+    const bool brk = (q + 1 == Z) || cont[q] != cont[q + 1] || chl[q] != chl[q + 1];
+    const uint32_t r = Z - 1 - q;
+    RV[r] = brk ? r : 0u;
+}
+
+// per slot: run length (in place of the scanned breaks), next slot, positions per item
+__global__ void k_gl_runs(uint32_t* __restrict__ RV, const uint32_t* __restrict__ cont, const uint32_t* __restrict__ cnt,
+                          uint32_t Z, uint32_t* __restrict__ mrun, uint32_t* __restrict__ nxt, uint32_t* __restrict__ ccnt)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q > Z) return;
+    if (q == Z) {  // the end of the text
+        nxt[Z] = kNone;
+        ccnt[Z] = 1;
+        return;
+    }
+    const uint32_t re = Z - 1 - RV[Z - 1 - q];  // last slot of the run starting at q
+    const uint32_t c = cont[q], nx = re + 1;
+    mrun[q] = re - q + 1;
+    if (c == kNone) nxt[q] = nx;  // the next root, or the end
+    else nxt[q] = (nx < Z && cont[nx] == c) ? nx : kNone;
+    ccnt[q] = (c == kNone) ? 1u : cnt[c];
+}
+
+// Up-contexts: a slot q that is a root or not the last child of its label sends every label on
+// the last-child chain of its child to slot q + 1.  Pass 1 counts, pass 2 writes the pairs.
+__device__ __forceinline__ bool up_source(const uint32_t* __restrict__ cont, uint32_t q, uint32_t Z)
+{
+    const uint32_t c = cont[q];
+    return c == kNone || (q + 1 < Z && cont[q + 1] == c);
+}
+
+__global__ void k_gl_upcount(const uint32_t* __restrict__ chl, const uint32_t* __restrict__ cont,
+                             const uint32_t* __restrict__ off, const uint32_t* __restrict__ last, uint32_t Z,
+                             uint32_t* __restrict__ ucnt)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Z) return;
+    uint32_t c = 0;
+    if (up_source(cont, q, Z))
+        for (uint32_t v = chl[q]; off[v + 1] > off[v]; v = last[v]) c++;
+    ucnt[q] = c;
+}
+
+__global__ void k_gl_upfill(const uint32_t* __restrict__ chl, const uint32_t* __restrict__ cont,
+                            const uint32_t* __restrict__ off, const uint32_t* __restrict__ last, uint32_t Z,
+                            const uint32_t* __restrict__ uoff, uint32_t* __restrict__ UK, uint32_t* __restrict__ UV)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Z || !up_source(cont, q, Z)) return;
+    uint32_t o = uoff[q];
+    for (uint32_t v = chl[q]; off[v + 1] > off[v]; v = last[v], o++) {
+        UK[o] = v;
+        UV[o] = q + 1;
+    }
+}
+
+// list offsets of the up-contexts sorted by label: uo[v] = first pair of label v, uo[D] = UT
+__global__ void k_gl_csr(const uint32_t* __restrict__ UK, uint32_t UT, uint32_t D, uint32_t* __restrict__ uo)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= UT) return;
+    const uint32_t a = (j == 0) ? 0u : UK[j - 1] + 1u;
+    for (uint32_t v = a; v <= UK[j]; v++) uo[v] = j;
+    if (j == UT - 1)
+        for (uint32_t v = UK[j] + 1; v <= D; v++) uo[v] = UT;
+}
+
+__global__ void k_gl_lrank(const uint32_t* __restrict__ V, const uint32_t* __restrict__ rk, uint32_t D,
+                           uint32_t* __restrict__ lrank)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < D) lrank[V[j]] = rk[j];
+}
+
+// run key of every slot: (rank of its child's label, run length), to be ranked densely
+__global__ void k_gl_rkinit(const uint32_t* __restrict__ chl, const uint32_t* __restrict__ mrun,
+                            const uint32_t* __restrict__ lrank, uint32_t Z, int mb, uint64_t* __restrict__ K,
+                            uint32_t* __restrict__ V)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Z) return;
+    K[q] = ((uint64_t)lrank[chl[q]] << mb) | mrun[q];
+    V[q] = q;
+}
+
+__global__ void k_gl_rkey(const uint32_t* __restrict__ V, const uint32_t* __restrict__ rk, uint32_t Z,
+                          uint32_t* __restrict__ rkey)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < Z) rkey[V[j]] = rk[j];  // 1-based; the end of the text keeps key 0
+    if (j == 0) rkey[Z] = 0;
+}
+
+// ---- items
+
+__global__ void k_it_init(const uint8_t* __restrict__ chc, uint32_t Z, uint32_t C, uint32_t* __restrict__ IQ,
+                          uint32_t* __restrict__ IG, uint32_t* __restrict__ ID, uint32_t* __restrict__ ICH)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Z) return;
+    IQ[q] = q;
+    IG[q] = 0;
+    ID[q] = 0;
+    ICH[q] = (uint32_t)chc[q] | ((q == C) ? 0x100u : 0u);  // slot C: the root at position 0
+}
+
+__global__ void k_it_hasrule(const uint32_t* __restrict__ IQ, const uint32_t* __restrict__ ID, uint32_t m,
+                             uint32_t C, uint32_t* __restrict__ HR)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m && IQ[i] < C) HR[ID[i]] = 1;
+}
+
+// Sort key: the item's group, then its next factor run.  A group with only root items (and the
+// end) orders them by position alone: a later root's suffix is smaller (it lies on the earlier
+// one's next-smaller-suffix chain).
+__global__ void k_it_key(const uint32_t* __restrict__ IQ, const uint32_t* __restrict__ ID, uint32_t m, uint32_t C,
+                         uint32_t Z, const uint32_t* __restrict__ HR, const uint32_t* __restrict__ rkey, int kb,
+                         uint64_t* __restrict__ K, uint32_t* __restrict__ V)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t q = IQ[i], g = ID[i];
+    const uint32_t k = (q >= C && !HR[g]) ? (Z - q) : rkey[q];
+    K[i] = ((uint64_t)g << kb) | k;
+    V[i] = i;
+}
+
+__global__ void k_it_gather(const uint32_t* __restrict__ V, uint32_t m, const uint32_t* __restrict__ IQ,
+                            const uint32_t* __restrict__ IG, const uint32_t* __restrict__ ICH,
+                            const uint32_t* __restrict__ ccnt, uint32_t* __restrict__ SQ, uint32_t* __restrict__ SG,
+                            uint32_t* __restrict__ SCH, uint32_t* __restrict__ CN)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t i = V[j], q = IQ[i];
+    SQ[j] = q;
+    SG[j] = IG[i];
+    SCH[j] = ICH[i];
+    CN[j] = ccnt[q];
+}
+
+// first item of each old group (OF) and head flags of the new groups (NH)
+__global__ void k_it_heads(const uint64_t* __restrict__ K, uint32_t m, int kb, uint32_t* __restrict__ OF,
+                           uint32_t* __restrict__ NH)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint64_t k = K[j];
+    const bool first = (j == 0) || (K[j - 1] >> kb) != (k >> kb);
+    if (first) OF[k >> kb] = j;
+    NH[j] = (j == 0 || K[j - 1] != k) ? 1u : 0u;
+}
+
+// New groups: BWT start (GST) and first item (GFI) of each; GEXP flags the groups that must be
+// refined further (two characters, or the item of suffix 0 among several items).
+__global__ void k_it_groups(const uint64_t* __restrict__ K, uint32_t m, int kb, const uint32_t* __restrict__ SG,
+                            const uint32_t* __restrict__ P, const uint32_t* __restrict__ OF,
+                            const uint32_t* __restrict__ NG1, const uint32_t* __restrict__ SCH,
+                            uint32_t* __restrict__ GST, uint32_t* __restrict__ GFI, uint32_t* __restrict__ GEXP)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t ng = NG1[j] - 1;
+    const bool head = (j == 0) || NG1[j - 1] != NG1[j];
+    if (head) {
+        GST[ng] = SG[j] + P[j] - P[OF[K[j] >> kb]];
+        GFI[ng] = j;
+    } else {
+        const uint32_t c = SCH[j], pc = SCH[j - 1];
+        if (((c ^ pc) & 0xFFu) || ((c | pc) & 0x100u)) GEXP[ng] = 1;
+    }
+}
+
+// Per item: the expansion count of a refined group's item, the run record of a finished group's
+// first item; the primary index when suffix 0 finishes.
+__global__ void k_it_flags(uint32_t m, const uint32_t* __restrict__ NG1, const uint32_t* __restrict__ GEXP,
+                           const uint32_t* __restrict__ SQ, const uint32_t* __restrict__ SCH,
+                           const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ cont,
+                           const uint32_t* __restrict__ uo, uint32_t Z, const uint32_t* __restrict__ GST,
+                           uint32_t* __restrict__ RF, uint32_t* __restrict__ EC, uint32_t* __restrict__ sc)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t ng = NG1[j] - 1;
+    const bool head = (j == 0) || NG1[j - 1] != NG1[j];
+    const bool exp = GEXP[ng] != 0;
+    RF[j] = (!exp && head) ? 1u : 0u;
+    uint32_t e = 0;
+    if (exp) {
+        const uint32_t q = SQ[j];
+        if (q >= Z) {
+            sc[GR_ERR] = 1;  // the end of the text is never refined further
+        } else if (nxt[q] != kNone) {
+            e = 1;
+        } else {
+            const uint32_t c = cont[q];
+            e = uo[c + 1] - uo[c];
+        }
+    } else if (SCH[j] & 0x100u) {
+        sc[GR_PIDX] = GST[ng] + 1;
+    }
+    EC[j] = e;
+}
+
+// run records of the finished groups: start, length (the group's count), character; RIDX is the
+// exclusive scan of the run flags
+__global__ void k_it_runs(uint32_t m, const uint32_t* __restrict__ RIDX, const uint32_t* __restrict__ NG1,
+                          const uint32_t* __restrict__ GEXP, const uint32_t* __restrict__ GST,
+                          const uint32_t* __restrict__ GFI, const uint32_t* __restrict__ P,
+                          const uint32_t* __restrict__ SCH, const uint32_t* __restrict__ sc,
+                          uint32_t* __restrict__ RST, uint32_t* __restrict__ RLEN, uint8_t* __restrict__ RCH)
+{
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m) return;
+    const uint32_t ng = NG1[j] - 1;
+    const bool head = (j == 0) || NG1[j - 1] != NG1[j];
+    if (!head || GEXP[ng]) return;
+    const uint32_t NG = sc[GR_NG];
+    const uint32_t end = (ng + 1 < NG) ? GFI[ng + 1] : m;
+    const uint32_t r = RIDX[j];
+    RST[r] = GST[ng];
+    RLEN[r] = P[end] - P[j];
+    RCH[r] = (uint8_t)(SCH[j] & 0xFFu);
+}
+
+// Writes the finished runs, 16 bytes of their concatenation per thread and step (grid-stride;
+// the total comes from the device).
+__global__ void k_it_fill(const uint32_t* __restrict__ ROFF, const uint32_t* __restrict__ RST,
+                          const uint8_t* __restrict__ RCH, uint32_t nruns, const uint32_t* __restrict__ total,
+                          uint8_t* __restrict__ O, uint64_t n, uint32_t* __restrict__ err)
+{
+    const uint64_t bytes = *total;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * 16;
+    for (uint64_t b0 = ((uint64_t)blockIdx.x * blockDim.x + threadIdx.x) * 16; b0 < bytes; b0 += stride) {
+        uint32_t lo = 0, hi = nruns;  // last run with ROFF <= b0
+        while (hi - lo > 1) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            if (ROFF[mid] <= b0) lo = mid;
+            else hi = mid;
+        }
+        uint32_t r = lo;
+        const uint64_t b1 = (b0 + 16 < bytes) ? b0 + 16 : bytes;
+        for (uint64_t b = b0; b < b1; b++) {
+            while (r + 1 < nruns && ROFF[r + 1] <= b) r++;
+            const uint64_t at = RST[r] + (b - ROFF[r]);
+            if (at < n) O[at] = RCH[r];
+            else *err = 1;  // a run past the end: never for consistent groups
+        }
+    }
+}
+
+// Items of the next round: output o belongs to the item whose expansion range holds it.
+__global__ void k_it_expand(uint32_t m, uint32_t mo, const uint32_t* __restrict__ EOFF, const uint32_t* __restrict__ SQ,
+                            const uint32_t* __restrict__ SCH, const uint32_t* __restrict__ NG1,
+                            const uint32_t* __restrict__ GST, const uint32_t* __restrict__ XID,
+                            const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ cont,
+                            const uint32_t* __restrict__ uo, const uint32_t* __restrict__ UV, uint32_t* __restrict__ OQ,
+                            uint32_t* __restrict__ OG, uint32_t* __restrict__ OID, uint32_t* __restrict__ OCH)
+{
+    const uint32_t o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= mo) return;
+    uint32_t lo = 0, hi = m;  // last item with EOFF <= o
+    while (hi - lo > 1) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (EOFF[mid] <= o) lo = mid;
+        else hi = mid;
+    }
+    const uint32_t i = lo, q = SQ[i], ng = NG1[i] - 1;
+    const uint32_t nq = nxt[q];
+    OQ[o] = (nq != kNone) ? nq : UV[uo[cont[q]] + (o - EOFF[i])];
+    OG[o] = GST[ng];
+    OID[o] = XID[ng];
+    OCH[o] = SCH[i];
+}
+
+// The grammar build: labels, their order, the induction.  Returns the primary index, 0 when the
+// label order check found a fingerprint collision, or kNone when the induction outgrew its
+// budget (the caller then runs the position-level induction).
+static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const PrefixHash& phh, const uint32_t* lam,
+                               const uint32_t* codes, uint32_t sigma, uint8_t* dO)
+{
+    uint32_t* gsc = S.buf<uint32_t>(B_GR_SC, GR_COUNT_);
+    fill<uint32_t>(S, gsc, GR_COUNT_, 0u);
+    uint32_t hv[GR_COUNT_];
+
+    // ---- distinct labels
+    int ph = S.phase_begin("labels");
+    const int bits = bits_for((uint64_t)sigma + 1);
+    const LabelKey lk{T, n, lam, phh, bits, 63 / bits};
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, n);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, n);
+    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, K0, V0);
+    MIBWT_LAUNCH_CHECK();
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, n, 0, 64);
+    uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
+    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 1);
+    k_dbl_headflag<<<grid_for(n), kBlock, 0, S.stream>>>(K0, (uint32_t)n, hf);
+    MIBWT_LAUNCH_CHECK();
+    inclusive_sum<uint32_t>(S, hf, gid1, n, gsc + GR_D);
+    k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(hf, gid1, (uint32_t)n, gofs);
+    MIBWT_LAUNCH_CHECK();
+    // roots: prefix maxima of i + lam[i]
+    uint32_t* PM = S.buf<uint32_t>(B_LY_JMP0, n);
+    uint32_t* rpos = S.buf<uint32_t>(B_LY_JMP1, n);
+    k_gl_ends<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, PM);
+    MIBWT_LAUNCH_CHECK();
+    inclusive_max<uint32_t>(S, PM, PM, n);
+    k_gl_rootflag<<<grid_for(n), kBlock, 0, S.stream>>>(PM, n, rpos);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, rpos, rpos, n, gsc + GR_S);
+    S.read_fast(hv, gsc, GR_COUNT_);
+    const uint32_t D = hv[GR_D], NS = hv[GR_S];
+    uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, NS);
+    k_gl_roots<<<grid_for(n), kBlock, 0, S.stream>>>(PM, rpos, n, roots);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* rep = S.buf<uint32_t>(B_GR_REP, D);
+    uint32_t* cnt = S.buf<uint32_t>(B_GR_CNT, D);
+    uint64_t* U = S.buf<uint64_t>(B_GR_KEY, D);
+    k_gl_dedup<<<grid_for(D), kBlock, 0, S.stream>>>(K0, V0, gofs, D, rep, cnt, U);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* off = S.buf<uint32_t>(B_GR_OFF, (size_t)D + 1);
+    k_gl_nchild<<<grid_for(D), kBlock, 0, S.stream>>>(rep, lam, D, off);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, off, off, D, off + D);
+    MIBWT_HIP(hipMemcpyAsync(gsc + GR_C, off + D, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    S.read_fast(hv, gsc, GR_COUNT_);
+    const uint32_t C = hv[GR_C];
+    const uint32_t Z = C + NS;  // slots; slot Z is the end of the text
+    uint32_t* last = S.buf<uint32_t>(B_GR_LAST, D);
+    uint32_t* chl = S.buf<uint32_t>(B_GR_CHL, Z);
+    uint8_t* chc = S.buf<uint8_t>(B_GR_CHC, Z);
+    uint32_t* cont = S.buf<uint32_t>(B_GR_CONT, Z);
+    k_gl_children<<<grid_for(D), kBlock, 0, S.stream>>>(lk, codes, rep, off, D, U, chl, chc, cont, last, gsc + GR_ERR);
+    MIBWT_LAUNCH_CHECK();
+    k_gl_rootslots<<<grid_for(NS), kBlock, 0, S.stream>>>(lk, codes, roots, NS, C, U, D, chl, chc, cont, gsc + GR_ERR);
+    MIBWT_LAUNCH_CHECK();
+    // runs of equal children
+    uint32_t* mrun = S.buf<uint32_t>(B_GR_RUN, Z);
+    uint32_t* nxt = S.buf<uint32_t>(B_GR_NXT, (size_t)Z + 1);
+    uint32_t* ccnt = S.buf<uint32_t>(B_GR_CCNT, (size_t)Z + 1);
+    uint32_t* RV = S.buf<uint32_t>(B_GR_UCNT, Z);
+    k_gl_brk<<<grid_for(Z), kBlock, 0, S.stream>>>(chl, cont, Z, RV);
+    MIBWT_LAUNCH_CHECK();
+    inclusive_max<uint32_t>(S, RV, RV, Z);
+    k_gl_runs<<<grid_for((uint64_t)Z + 1), kBlock, 0, S.stream>>>(RV, cont, cnt, Z, mrun, nxt, ccnt);
+    MIBWT_LAUNCH_CHECK();
+    // up-contexts
+    uint32_t* ucnt = RV;
+    k_gl_upcount<<<grid_for(Z), kBlock, 0, S.stream>>>(chl, cont, off, last, Z, ucnt);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, ucnt, ucnt, Z, gsc + GR_UT);
+    S.read_fast(hv, gsc, GR_COUNT_);
+    if (hv[GR_ERR] != 0) {  // inconsistent keys: only colliding fingerprints produce them
+        S.phase_end(ph, 0);
+        return 0;
+    }
+    const uint32_t UT = hv[GR_UT];
+    uint32_t* UK0 = S.buf<uint32_t>(B_GR_UK0, UT);
+    uint32_t* UK1 = S.buf<uint32_t>(B_GR_UK1, UT);
+    uint32_t* UV0 = S.buf<uint32_t>(B_GR_UV0, UT);
+    uint32_t* UV1 = S.buf<uint32_t>(B_GR_UV1, UT);
+    uint32_t* uo = S.buf<uint32_t>(B_GR_UOFF, (size_t)D + 1);
+    fill<uint32_t>(S, uo, (uint64_t)D + 1, 0u);
+    if (UT > 0) {
+        k_gl_upfill<<<grid_for(Z), kBlock, 0, S.stream>>>(chl, cont, off, last, Z, ucnt, UK0, UV0);
+        MIBWT_LAUNCH_CHECK();
+        radix_sort<uint32_t, uint32_t>(S, UK0, UK1, UV0, UV1, UT, 0, bits_for(D));
+        k_gl_csr<<<grid_for(UT), kBlock, 0, S.stream>>>(UK0, UT, D, uo);
+        MIBWT_LAUNCH_CHECK();
+    }
+    S.phase_end(ph, (double)n * (4 + 8 + 4 + 4) + (double)n * 24 * 8 + (double)Z * 32);
+    S.stats.distinct_phrases = D;
+    S.stats.dict_length = C;
+    S.stats.triggers = NS;
+    S.stats.mixed_entries = UT;
+
+    // ---- label order (the rule sort over distinct labels) and run keys
+    ph = S.phase_begin("rsort");
+    double rbytes = 0;
+    uint32_t* Vs = nullptr;
+    uint32_t* Hs = nullptr;
+    if (!label_order<RepLab>(S, T, n, phh, RepLab{rep, lam}, D, codes, sigma, &Vs, &Hs, &rbytes)) {
+        S.phase_end(ph, rbytes);
+        return 0;
+    }
+    uint32_t* lrank = S.buf<uint32_t>(B_GR_LRANK, D);
+    {
+        uint32_t* rk = S.buf<uint32_t>(B_DB_GID, D);
+        inclusive_sum<uint32_t>(S, Hs, rk, D);
+        k_gl_lrank<<<grid_for(D), kBlock, 0, S.stream>>>(Vs, rk, D, lrank);
+        MIBWT_LAUNCH_CHECK();
+    }
+    uint32_t* rkey = S.buf<uint32_t>(B_GR_RKEY, (size_t)Z + 1);
+    {
+        const int mb = bits_for(Z);
+        uint64_t* RK0 = S.buf<uint64_t>(B_SORT_K0, Z);
+        uint64_t* RK1 = S.buf<uint64_t>(B_SORT_K1, Z);
+        uint32_t* RV0 = S.buf<uint32_t>(B_SORT_V0, Z);
+        uint32_t* RV1 = S.buf<uint32_t>(B_SORT_V1, Z);
+        k_gl_rkinit<<<grid_for(Z), kBlock, 0, S.stream>>>(chl, mrun, lrank, Z, mb, RK0, RV0);
+        MIBWT_LAUNCH_CHECK();
+        radix_sort<uint64_t, uint32_t>(S, RK0, RK1, RV0, RV1, Z, 0, bits_for(D) + mb);
+        uint32_t* h = S.buf<uint32_t>(B_LY_HF, Z);
+        k_dbl_headflag<<<grid_for(Z), kBlock, 0, S.stream>>>(RK0, Z, h);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_sum<uint32_t>(S, h, h, Z, gsc + GR_NR);
+        k_gl_rkey<<<grid_for(Z), kBlock, 0, S.stream>>>(RV0, h, Z, rkey);
+        MIBWT_LAUNCH_CHECK();
+        rbytes += (double)Z * 24 * 8;
+    }
+    S.read_fast(hv, gsc, GR_COUNT_);
+    const uint32_t NR = hv[GR_NR];
+    S.phase_end(ph, rbytes);
+
+    // ---- induction over the items
+    ph = S.phase_begin("induce");
+    const int kb = bits_for(std::max<uint64_t>(NR, NS));
+    uint32_t m = Z, G = 1;
+    uint32_t* IQ = S.buf<uint32_t>(B_IT_Q0, m);
+    uint32_t* IG = S.buf<uint32_t>(B_IT_G0, m);
+    uint32_t* ID = S.buf<uint32_t>(B_IT_ID0, m);
+    uint32_t* ICH = S.buf<uint32_t>(B_IT_CH0, m);
+    int cur = 0;
+    k_it_init<<<grid_for(m), kBlock, 0, S.stream>>>(chc, Z, C, IQ, IG, ID, ICH);
+    MIBWT_LAUNCH_CHECK();
+    uint64_t budget = 8 * (uint64_t)n + (1u << 22);
+    if (const char* e = getenv("MIBWT_LYN_ITEM_BUDGET")) budget = strtoull(e, nullptr, 10);  // tests
+    uint64_t items_total = 0;
+    double ibytes = 0;
+    int rounds = 0;
+    bool over = false;
+    while (m > 0) {
+        items_total += m;
+        if (items_total > budget || rounds >= 4096) {
+            over = true;
+            break;
+        }
+        rounds++;
+        uint32_t* HR = S.buf<uint32_t>(B_IT_HR, G);
+        fill<uint32_t>(S, HR, G, 0u);
+        k_it_hasrule<<<grid_for(m), kBlock, 0, S.stream>>>(IQ, ID, m, C, HR);
+        MIBWT_LAUNCH_CHECK();
+        uint64_t* IK0 = S.buf<uint64_t>(B_SORT_K0, m);
+        uint64_t* IK1 = S.buf<uint64_t>(B_SORT_K1, m);
+        uint32_t* IV0 = S.buf<uint32_t>(B_SORT_V0, m);
+        uint32_t* IV1 = S.buf<uint32_t>(B_SORT_V1, m);
+        k_it_key<<<grid_for(m), kBlock, 0, S.stream>>>(IQ, ID, m, C, Z, HR, rkey, kb, IK0, IV0);
+        MIBWT_LAUNCH_CHECK();
+        const int sbits = bits_for(G - 1) + kb;
+        radix_sort<uint64_t, uint32_t>(S, IK0, IK1, IV0, IV1, m, 0, sbits);
+        uint32_t* SQ = S.buf<uint32_t>(B_IT_SQ, m);
+        uint32_t* SG = S.buf<uint32_t>(B_IT_SG, m);
+        uint32_t* SCH = S.buf<uint32_t>(B_IT_SCH, m);
+        uint32_t* P = S.buf<uint32_t>(B_IT_P, (size_t)m + 1);
+        k_it_gather<<<grid_for(m), kBlock, 0, S.stream>>>(IV0, m, IQ, IG, ICH, ccnt, SQ, SG, SCH, P);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, P, P, m, P + m);
+        uint32_t* OF = S.buf<uint32_t>(B_IT_OF, G);
+        uint32_t* NG1 = S.buf<uint32_t>(B_IT_NH, m);
+        k_it_heads<<<grid_for(m), kBlock, 0, S.stream>>>(IK0, m, kb, OF, NG1);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_sum<uint32_t>(S, NG1, NG1, m, gsc + GR_NG);
+        uint32_t* GST = S.buf<uint32_t>(B_IT_GST, m);
+        uint32_t* GFI = S.buf<uint32_t>(B_IT_GFI, m);
+        uint32_t* GEXP = S.buf<uint32_t>(B_IT_GEXP, m);
+        fill<uint32_t>(S, GEXP, m, 0u);
+        k_it_groups<<<grid_for(m), kBlock, 0, S.stream>>>(IK0, m, kb, SG, P, OF, NG1, SCH, GST, GFI, GEXP);
+        MIBWT_LAUNCH_CHECK();
+        uint32_t* RF = S.buf<uint32_t>(B_IT_RF, m);   // run flags, then run indices
+        uint32_t* EC = S.buf<uint32_t>(B_IT_EC, m);   // expansion counts, then output offsets
+        k_it_flags<<<grid_for(m), kBlock, 0, S.stream>>>(m, NG1, GEXP, SQ, SCH, nxt, cont, uo, Z, GST, RF, EC, gsc);
+        MIBWT_LAUNCH_CHECK();
+        // dense ids of the refined groups (GEXP beyond the last group is 0), run indices, offsets
+        uint32_t* XID = S.buf<uint32_t>(B_IT_XID, m);
+        exclusive_sum<uint32_t>(S, GEXP, XID, m, gsc + GR_G);
+        exclusive_sum<uint32_t>(S, RF, RF, m, gsc + GR_RUNS);
+        exclusive_sum<uint32_t>(S, EC, EC, m, gsc + GR_M);
+        uint32_t* RST = S.buf<uint32_t>(B_IT_RST, m);
+        uint32_t* RLEN = S.buf<uint32_t>(B_IT_RLEN, m);  // lengths, then offsets
+        uint8_t* RCH = S.buf<uint8_t>(B_IT_RCH, m);
+        k_it_runs<<<grid_for(m), kBlock, 0, S.stream>>>(m, RF, NG1, GEXP, GST, GFI, P, SCH, gsc, RST, RLEN, RCH);
+        MIBWT_LAUNCH_CHECK();
+        S.read_fast(hv, gsc, GR_COUNT_);
+        if (hv[GR_ERR] != 0) {  // an inconsistent grammar (colliding fingerprints): new base
+            S.phase_end(ph, ibytes);
+            return 0;
+        }
+        const uint32_t nruns = hv[GR_RUNS], mo = hv[GR_M], Go = hv[GR_G];
+        if (nruns > 0) {
+            exclusive_sum<uint32_t>(S, RLEN, RLEN, nruns, gsc + GR_RB);
+            const uint32_t fb = std::max<uint32_t>(1, std::min<uint32_t>(2048, ceil_div(nruns, 16)));
+            k_it_fill<<<fb, kBlock, 0, S.stream>>>(RLEN, RST, RCH, nruns, gsc + GR_RB, dO, n, gsc + GR_ERR);
+            MIBWT_LAUNCH_CHECK();
+        }
+        if (S.verbose)
+            fprintf(stderr, "[mibwt] lyndon induction round %d: %u items in %u groups, %u runs, %u items next\n",
+                    rounds, m, G, nruns, mo);
+        ibytes += (double)m * (24 * ((sbits + 7) / 8) + 64) + (double)mo * 28 + (double)nruns * 12;
+        if (mo == 0) {
+            m = 0;
+            break;
+        }
+        const int nx = cur ^ 1;
+        uint32_t* OQ = S.buf<uint32_t>(nx ? B_IT_Q1 : B_IT_Q0, mo);
+        uint32_t* OG = S.buf<uint32_t>(nx ? B_IT_G1 : B_IT_G0, mo);
+        uint32_t* OID = S.buf<uint32_t>(nx ? B_IT_ID1 : B_IT_ID0, mo);
+        uint32_t* OCH = S.buf<uint32_t>(nx ? B_IT_CH1 : B_IT_CH0, mo);
+        k_it_expand<<<grid_for(mo), kBlock, 0, S.stream>>>(m, mo, EC, SQ, SCH, NG1, GST, XID, nxt, cont, uo, UV0, OQ, OG,
+                                                            OID, OCH);
+        MIBWT_LAUNCH_CHECK();
+        IQ = OQ;
+        IG = OG;
+        ID = OID;
+        ICH = OCH;
+        cur = nx;
+        m = mo;
+        G = Go;
+    }
+    S.stats.phrases = (int64_t)items_total;
+    S.stats.doubling_rounds_parse = rounds;
+    S.phase_end(ph, ibytes);
+    if (over) return kNone;
+    S.read_fast(hv, gsc, GR_COUNT_);
+    if (hv[GR_ERR] != 0) return 0;  // a run past the end: an inconsistent grammar, new base
+    const uint32_t pidx = hv[GR_PIDX];
+    return (pidx >= 1 && pidx <= n) ? pidx : 0xFFFFFFFEu;  // an invalid index fails the inversion
+}
+
+// ------------------------------------------------------------------- 5. position-level induction
 
 struct ChainPolicy {
     static constexpr bool kCarry = false;
@@ -631,130 +1492,27 @@ static uint64_t lyndon_base(uint64_t seed)
     return (1ull << 32) + z % (kM61 - (1ull << 32));
 }
 
-// One build attempt with fingerprint base `base` (fingerprints compared under `hmask`): the
-// phases rules .. output.  Returns the primary index, or 0 when the label order check found a
-// fingerprint collision.
-static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_t base, uint64_t hmask,
-                               uint32_t* codes, uint32_t* sigma_io, uint8_t* dO)
+// The position-level induction: every position ordered by its label, then groups of equal labels
+// refined by the rank at the next smaller suffix with pointer jumping.  It runs when the grammar
+// induction outgrows its budget (or when MIBWT_LYN_INDUCTION=positions asks for it).  Returns the
+// primary index, or 0 when a fingerprint collision broke the label order.
+static uint32_t lyndon_positions(Storage& S, const uint8_t* T, uint64_t n, const PrefixHash& phh,
+                                 const uint32_t* lam, const uint32_t* codes, uint32_t sigma, uint8_t* dO)
 {
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
-    // ---- 1. prefix fingerprints
-    int ph = S.phase_begin("rules");
-    HashPair* pre = prefix_hashes(S, T, n, base);
-    const PrefixHash phh{pre, hmask};
-    S.phase_end(ph, (double)n * (1 + 16 * 4));
-
-    // ---- 2. Lyndon array
-    ph = S.phase_begin("lyndon");
-    uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
-    const int levels = lyndon_array(S, T, n, phh, lam);
-    alphabet_codes(S, T, n, codes);
-    uint32_t sigma = 0;
-    S.read(&sigma, codes + 256);
-    *sigma_io = sigma;
-    S.phase_end(ph, (double)n * (1 + 4 + 4 + 1) * 2);
-    S.stats.doubling_rounds_dict = levels;
-
-    // ---- 3. label order
-    ph = S.phase_begin("rsort");
-    const int bits = bits_for((uint64_t)sigma + 1);
-    const int per = 64 / bits;
-    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
-    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
-    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, n);
-    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, n);
-    uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
-    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
-    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 1);
-    uint32_t* large = S.buf<uint32_t>(B_DB_LARGE, n);
-    uint32_t* lpos = S.buf<uint32_t>(B_ACT_X, n);
-    uint8_t* cls = S.buf<uint8_t>(B_LY_CLS, n);
-    uint32_t* Vl = S.buf<uint32_t>(B_LY_VL, n);
-    k_lyn_initkey<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, lam, codes, bits, per, K0, V0);
-    MIBWT_LAUNCH_CHECK();
-    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, n, 0, bits * per);
-    k_lab_heads_from_keys<<<grid_for(n), kBlock, 0, S.stream>>>(K0, n, hf);
-    MIBWT_LAUNCH_CHECK();
-    double rbytes = (double)n * 24 * ((bits * per + 7) / 8);
-    {
-        uint64_t* LF = S.buf<uint64_t>(B_LY_LABH, n);
-        k_label_hash<<<grid_for(n), kBlock, 0, S.stream>>>(phh, lam, n, LF);
-        MIBWT_LAUNCH_CHECK();
-        LabelCmp cmp{T, lam, phh, LF};
-        // working copies: V (order) and hf (heads) of the packed-key order
-        uint32_t* V = Vl;
-        MIBWT_HIP(hipMemcpyAsync(V, V0, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-        uint32_t* H = S.buf<uint32_t>(B_LY_HF2, n);
-        MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-        const uint32_t known = (uint32_t)per;  // every tie group agrees on its first `per` symbols
-        uint32_t small = kLabSmall;
-        if (const char* e = getenv("MIBWT_LAB_SMALL")) {
-            const int v = atoi(e);
-            if (v >= 2 && v <= 256) small = (uint32_t)v;
-        }
-        uint8_t* settled = S.buf<uint8_t>(B_LY_SETTLED, n);
-        fill<uint8_t>(S, settled, n, (uint8_t)0);
-        for (int round = 0;; round++) {
-            MIBWT_REQUIRE(round < 1 << 20, LIBCUBWT_GPU_ERROR, "label refinement does not converge");
-            inclusive_sum<uint32_t>(S, H, gid1, n, sc + SC_GROUPS);
-            k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(H, gid1, (uint32_t)n, gofs);
-            MIBWT_LAUNCH_CHECK();
-            k_lab_classify<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, lam, n, known, small, settled, cls,
-                                                                 large);
-            MIBWT_LAUNCH_CHECK();
-            exclusive_sum<uint32_t>(S, large, lpos, n, sc + SC_LARGE);
-            uint32_t hv[SC_COUNT_];
-            S.read(hv, sc, SC_COUNT_);
-            const uint32_t nl = hv[SC_LARGE];
-            uint32_t* V2 = (V == Vl) ? V1 : Vl;
-            uint32_t* GL = S.buf<uint32_t>(B_LY_GL, n);
-            uint64_t* GF = S.buf<uint64_t>(B_LY_GF, n);
-            k_lab_gather<<<grid_for(n), kBlock, 0, S.stream>>>(V, lam, LF, n, GL, GF);
-            MIBWT_LAUNCH_CHECK();
-            k_lab_small<<<grid_for(n), kBlock, 0, S.stream>>>(V, gid1, gofs, cls, n, known, cmp, GL, GF, V2, H);
-            MIBWT_LAUNCH_CHECK();
-            if (nl > 0) {
-                uint64_t* LK0 = S.buf<uint64_t>(B_LG_K0, nl);
-                uint64_t* LK1 = S.buf<uint64_t>(B_LG_K1, nl);
-                uint32_t* LV0 = S.buf<uint32_t>(B_LG_V0, nl);
-                uint32_t* LV1 = S.buf<uint32_t>(B_LG_V1, nl);
-                uint32_t* LI = S.buf<uint32_t>(B_LG_I, nl);
-                uint32_t* LG = S.buf<uint32_t>(B_LG_SZ, nl);
-                uint32_t* hq = S.buf<uint32_t>(B_LY_SCAN0, nl);
-                // large groups hold more than `small` members each, so their dense ids take gb
-                // bits; the l part takes what is left of the 64-bit key (at most 31 bits)
-                const int gb = bits_for((uint64_t)nl / (small + 1));
-                const int lbits = std::min(31, 64 - 11 - gb);
-                k_lab_pivot_gather<<<grid_for(n), kBlock, 0, S.stream>>>(phh, T, n, V, gid1, gofs, large, lpos, lam,
-                                                                         known, lbits, LK0, LV0, LI, LG);
-                MIBWT_LAUNCH_CHECK();
-                k_lab_pivot_heads<<<grid_for(nl), kBlock, 0, S.stream>>>(LG, nl, hq);
-                MIBWT_LAUNCH_CHECK();
-                inclusive_sum<uint32_t>(S, hq, hq, nl, sc + SC_GROUPS);
-                k_lab_pivot_ids<<<grid_for(nl), kBlock, 0, S.stream>>>(hq, nl, 11 + lbits, LK0);
-                MIBWT_LAUNCH_CHECK();
-                radix_sort<uint64_t, uint32_t>(S, LK0, LK1, LV0, LV1, nl, 0, 11 + lbits + gb);
-                k_lab_pivot_scatter<<<grid_for(nl), kBlock, 0, S.stream>>>(LK0, LV0, LI, nl, lbits, V2, H, settled);
-                MIBWT_LAUNCH_CHECK();
-                rbytes += (double)nl * 24 * 8;
-            }
-            V = V2;
-            if (nl == 0) break;
-        }
-        // check of the label order
-        fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
-        k_lab_verify<<<grid_for(n), kBlock, 0, S.stream>>>(SuffixCmp{T, n, phh}, V, H, lam, sc + SC_COLLISION);
-        MIBWT_LAUNCH_CHECK();
-        uint32_t bad = 0;
-        S.read(&bad, sc + SC_COLLISION);
-        if (bad != 0) {
-            S.phase_end(ph, rbytes);
-            return 0;
-        }
-        if (V != Vl) MIBWT_HIP(hipMemcpyAsync(Vl, V, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
-        MIBWT_HIP(hipMemcpyAsync(hf, H, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, S.stream));
+    // ---- 3. label order of every position
+    int ph = S.phase_begin("rsort");
+    double rbytes = 0;
+    uint32_t* Vl = nullptr;
+    uint32_t* hf = nullptr;
+    if (!label_order<PosLab>(S, T, n, phh, PosLab{lam}, (uint32_t)n, codes, sigma, &Vl, &hf, &rbytes)) {
+        S.phase_end(ph, rbytes);
+        return 0;
     }
     // label group ids as the initial keys of the induction
+    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, n);
     inclusive_sum<uint32_t>(S, hf, gid1, n, sc + SC_GROUPS);
     k_u32_to_key<<<grid_for(n), kBlock, 0, S.stream>>>(gid1, n, K0);
     MIBWT_LAUNCH_CHECK();
@@ -803,9 +1561,43 @@ static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_
     S.phase_end(ph, (double)n * 6);
     uint32_t pidx = 0;
     S.read(&pidx, sc + SC_PIDX);
-    return (pidx >= 1 && pidx <= n) ? pidx : 0xFFFFFFFFu;  // an invalid index fails the inversion
+    return (pidx >= 1 && pidx <= n) ? pidx : 0xFFFFFFFEu;  // an invalid index fails the inversion
 }
 
+// One build attempt with fingerprint base `base` (fingerprints compared under `hmask`):
+// fingerprints, Lyndon array, then the grammar induction (or the position-level one).  Returns
+// the primary index, or 0 when the label order check found a fingerprint collision.
+static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_t base, uint64_t hmask,
+                               uint32_t* codes, uint8_t* dO, bool positions)
+{
+    // ---- 1. prefix fingerprints
+    int ph = S.phase_begin("rules");
+    HashPair* pre = prefix_hashes(S, T, n, base);
+    const PrefixHash phh{pre, hmask};
+    S.phase_end(ph, (double)n * (1 + 16 * 4));
+
+    // ---- 2. Lyndon array
+    ph = S.phase_begin("lyndon");
+    uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
+    const int levels = lyndon_array(S, T, n, phh, lam);
+    alphabet_codes(S, T, n, codes);
+    uint32_t sigma = 0;
+    S.read_fast(&sigma, codes + 256);
+    S.phase_end(ph, (double)n * (1 + 4 + 4 + 1) * 2);
+    S.stats.doubling_rounds_dict = levels;
+    S.stats.alpha_groups = sigma;
+
+    if (!positions) {
+        const uint32_t r = lyndon_grammar(S, T, n, phh, lam, codes, sigma, dO);
+        if (r != kNone) {
+            S.stats.induction = 1;
+            return r;
+        }
+        if (S.verbose) fprintf(stderr, "[mibwt] lyndon: grammar induction over budget, position-level induction\n");
+    }
+    S.stats.induction = 2;
+    return lyndon_positions(S, T, n, phh, lam, codes, sigma, dO);
+}
 
 int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* dO, int64_t n64)
 {
@@ -827,26 +1619,28 @@ int64_t lyndon_bwt_run(Storage& S, const uint8_t* src, bool src_host, uint8_t* d
         T = stage_text(S, src, false, n);
     }
 
-    // The suffix comparisons behind the Lyndon array and the label order trust Karp-Rabin
-    // fingerprints; the finished BWT is therefore inverted on the device and compared with the
-    // text, and a mismatch (a fingerprint collision) repeats the build with a new base.  The base
-    // is drawn per call.  MIBWT_LYN_HASH_BITS narrows the first attempt's fingerprints (tests).
+    // The Lyndon array, the label keys and the label order trust Karp-Rabin fingerprints; the
+    // finished BWT is therefore inverted on the device and compared with the text, and a mismatch
+    // (a fingerprint collision) repeats the build with a new base.  The base is drawn per call.
+    // MIBWT_LYN_HASH_BITS narrows the first attempt's fingerprints (tests);
+    // MIBWT_LYN_INDUCTION=positions selects the position-level induction.
     uint64_t mask0 = ~0ull;
     if (const char* hb = getenv("MIBWT_LYN_HASH_BITS")) {
         const int v = atoi(hb);
         if (v >= 1 && v < 64) mask0 = (1ull << v) - 1;
     }
+    const char* ind = getenv("MIBWT_LYN_INDUCTION");
+    const bool positions = ind && strcmp(ind, "positions") == 0;
     S.lyn_calls++;
     uint64_t base = lyndon_base(S.lyn_calls ^ (uint64_t)std::chrono::steady_clock::now().time_since_epoch().count());
     uint32_t* codes = S.buf<uint32_t>(B_ALPHA, 272);
-    uint32_t sigma = 0;
     uint32_t pidx = 0;
     for (int attempt = 0;; attempt++) {
         MIBWT_REQUIRE(attempt < 4, MIBWT_HASH_COLLISION, "Lyndon fingerprint collisions persist");
         if (attempt > 0) base = lyndon_base(base * 0x9E3779B97F4A7C15ull + 0x632BE59BD9B4E019ull);
         const uint64_t hmask = (attempt == 0) ? mask0 : ~0ull;
         S.stats.retries = attempt;  // the timing covers every attempt
-        pidx = lyndon_attempt(S, T, n, base, hmask, codes, &sigma, dO);
+        pidx = lyndon_attempt(S, T, n, base, hmask, codes, dO, positions);
         if (pidx == 0) continue;  // the label order check failed: new base
         const int ph = S.phase_begin("verify");
         const bool ok = bwt_inverts_to(S, dO, (int64_t)n, pidx, T);

@@ -34,10 +34,26 @@ def test_lyndon_array(storage, name):
     assert (storage.lyndon_array(T).astype(np.int64) == oracle.lyndon_array(T)).all()
 
 
+INDUCTIONS = {
+    # grammar induction over distinct labels (the default)
+    "grammar": ({}, 1),
+    # the position-level induction, asked for directly
+    "positions": ({"MIBWT_LYN_INDUCTION": "positions"}, 2),
+    # the grammar induction over its item budget from the first round: the position-level one
+    # takes over in the same build
+    "fallback": ({"MIBWT_LYN_ITEM_BUDGET": "0"}, 2),
+}
+
+
+@pytest.mark.parametrize("mode", sorted(INDUCTIONS))
 @pytest.mark.parametrize("name", sorted(TEXTS))
-def test_lyndon_bwt(storage, mibwt, name):
+def test_lyndon_bwt(storage, mibwt, monkeypatch, name, mode):
+    env, expect = INDUCTIONS[mode]
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
     T = np.frombuffer(TEXTS[name], dtype=np.uint8)
     L, idx = storage.bwt(T, "lyndon")
+    assert storage.stats()["induction"] == expect
     O, oidx = oracle.bwt(T)
     assert idx == oidx
     assert (mibwt.cubwt_to_reference(L, idx) == O).all()
