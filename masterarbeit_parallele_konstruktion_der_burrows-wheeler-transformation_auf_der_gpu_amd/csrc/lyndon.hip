@@ -41,13 +41,16 @@ namespace mibwt {
 
 bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT);
 
-constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass
+constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass (MIBWT_LYN_BLOCK overrides)
 // Label tie groups of more members go to the pivot partition (linear work per round; a group of
 // equal labels settles in one round) instead of the comparator counting sort (quadratic in the
-// group size, but one pass); MIBWT_LAB_SMALL overrides it (2..256).  Measured on the 256 MiB DNA
-// collection: 256 -> rsort 83 ms, 64 -> 97 ms, 16 -> 148 ms, 4 -> 174 ms (more pivot rounds,
-// each a radix sort, outweigh the cheaper comparisons).
-constexpr uint32_t kLabSmall = 256;
+// group size, but one pass); MIBWT_LAB_SMALL overrides it (2..256).  Ordering every position
+// (many copies of each label), measured on the 256 MiB DNA collection: 256 -> rsort 83 ms,
+// 64 -> 97 ms, 16 -> 148 ms, 4 -> 174 ms (more pivot rounds, each a radix sort, outweigh the
+// cheaper comparisons).  Ordering distinct labels (no equal members, so every pivot round
+// splits well): 4 -> 5.7 ms, 8 -> 5.8, 16 -> 7.0, 32 -> 9.2, 256 -> 11.6.
+constexpr uint32_t kLabSmall = 256;      // every position
+constexpr uint32_t kLabSmallDistinct = 4;  // distinct labels
 constexpr uint64_t kM61 = (1ull << 61) - 1;
 
 // ------------------------------------------------------------------- 1. Lyndon array
@@ -195,13 +198,13 @@ constexpr uint32_t kNoNss = 0xFFFFFFFFu;
 // Positions without a smaller suffix inside the block stay open (kNoNss).  Per block: the
 // open positions in ascending order (O, increasing suffixes) and the chain of prefix minima
 // from the block start (M = s, NSS[s], NSS[NSS[s]], ..., decreasing suffixes).
-__global__ void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
+__global__ void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t bs, uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
                             uint32_t* __restrict__ oc, uint32_t* __restrict__ M, uint32_t* __restrict__ mc)
 {
     const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nb) return;
-    const uint32_t s = b * kLynBlock;
-    const uint32_t e = (uint32_t)(((uint64_t)s + kLynBlock < sc.n) ? s + kLynBlock : sc.n);
+    const uint32_t s = b * bs;
+    const uint32_t e = (uint32_t)(((uint64_t)s + bs < sc.n) ? s + bs : sc.n);
     for (uint32_t i = e; i-- > s;) {
         uint32_t j = i + 1;
         while (j < e && !sc.less(j, i)) j = nss[j];  // kNoNss ends the walk as well
@@ -220,7 +223,7 @@ __global__ void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t* __restrict__ ns
 // of L finds its NSS among R's prefix minima: they decrease, so the first one below suffix(i)
 // is found by bisection.  The merged range keeps L's unresolved positions followed by R's
 // open ones, and L's prefix minima followed by the prefix minima of R below min(L).
-__global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uint32_t size,
+__global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uint32_t bs, uint32_t size,
                                                   uint32_t* __restrict__ nss, const uint32_t* __restrict__ Oin,
                                                   const uint32_t* __restrict__ ocin, const uint32_t* __restrict__ Min,
                                                   const uint32_t* __restrict__ mcin, uint32_t* __restrict__ Oout,
@@ -231,7 +234,7 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
     const uint64_t a = (uint64_t)blockIdx.x * 2 * size;
     if (a >= nb) return;
     const uint64_t rb = a + size;
-    const uint64_t sL = a * kLynBlock;
+    const uint64_t sL = a * bs;
     const uint32_t oL = ocin[a], mL = mcin[a];
     const uint32_t* OL = Oin + sL;
     const uint32_t* ML = Min + sL;
@@ -244,7 +247,7 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
         }
         return;
     }
-    const uint64_t sR = rb * kLynBlock;
+    const uint64_t sR = rb * bs;
     const uint32_t oR = ocin[rb], mR = mcin[rb];
     const uint32_t* OR = Oin + sR;
     const uint32_t* MR = Min + sR;
@@ -296,22 +299,27 @@ __global__ void k_nss_to_lam(uint32_t* __restrict__ a, uint64_t n)
 // merge levels.
 int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32_t* lam)
 {
-    const uint32_t nb = ceil_div(n, kLynBlock);
-    uint32_t* oa = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * kLynBlock);
-    uint32_t* ob = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * kLynBlock);
-    uint32_t* ma = S.buf<uint32_t>(B_LY_RES, (size_t)nb * kLynBlock);
-    uint32_t* mb = S.buf<uint32_t>(B_LY_JMP0, (size_t)nb * kLynBlock);
+    uint32_t bs = kLynBlock;
+    if (const char* e = getenv("MIBWT_LYN_BLOCK")) {
+        const int v = atoi(e);
+        if (v >= 8 && v <= 4096 && (v & (v - 1)) == 0) bs = (uint32_t)v;
+    }
+    const uint32_t nb = ceil_div(n, bs);
+    uint32_t* oa = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * bs);
+    uint32_t* ob = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * bs);
+    uint32_t* ma = S.buf<uint32_t>(B_LY_RES, (size_t)nb * bs);
+    uint32_t* mb = S.buf<uint32_t>(B_LY_JMP0, (size_t)nb * bs);
     uint32_t* oca = S.buf<uint32_t>(B_LY_CNT0, nb);
     uint32_t* ocb = S.buf<uint32_t>(B_LY_CNT1, nb);
     uint32_t* mca = S.buf<uint32_t>(B_LY_HF, nb);
     uint32_t* mcb = S.buf<uint32_t>(B_LY_HF2, nb);
     const SuffixCmp sc{T, n, ph};
-    k_nss_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(sc, nb, lam, oa, oca, ma, mca);
+    k_nss_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(sc, nb, bs, lam, oa, oca, ma, mca);
     MIBWT_LAUNCH_CHECK();
     int levels = 0;
     for (uint64_t size = 1; size < nb; size *= 2) {
         const uint32_t pairs = (uint32_t)((nb + 2 * size - 1) / (2 * size));
-        k_nss_merge<<<pairs, 64, 0, S.stream>>>(sc, nb, (uint32_t)size, lam, oa, oca, ma, mca, ob, ocb, mb, mcb);
+        k_nss_merge<<<pairs, 64, 0, S.stream>>>(sc, nb, bs, (uint32_t)size, lam, oa, oca, ma, mca, ob, ocb, mb, mcb);
         MIBWT_LAUNCH_CHECK();
         std::swap(oa, ob);
         std::swap(oca, ocb);
@@ -600,11 +608,12 @@ __global__ void k_u32_to_key(const uint32_t* __restrict__ a, uint64_t n, uint64_
 // Orders m labels.  On return *v_out holds the items in label order and *hf_out flags the first
 // item of every run of equal labels.  Returns false when the final neighbour check fails (a
 // fingerprint collision broke the order).  Labels that agree on their first `per` symbols (the
-// packed initial key) are refined in rounds: tie groups of up to kLabSmall members by the
+// packed initial key) are refined in rounds: tie groups of up to `small` members by the
 // comparator counting sort, larger ones by the pivot partition.
 template <typename Lab>
 static bool label_order(Storage& S, const uint8_t* T, uint64_t n, const PrefixHash& phh, Lab lab, uint32_t m,
-                        const uint32_t* codes, uint32_t sigma, uint32_t** v_out, uint32_t** hf_out, double* bytes)
+                        const uint32_t* codes, uint32_t sigma, uint32_t small, uint32_t** v_out, uint32_t** hf_out,
+                        double* bytes)
 {
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     const int bits = bits_for((uint64_t)sigma + 1);
@@ -636,7 +645,6 @@ static bool label_order(Storage& S, const uint8_t* T, uint64_t n, const PrefixHa
     uint32_t* H = S.buf<uint32_t>(B_LY_HF2, m);
     MIBWT_HIP(hipMemcpyAsync(H, hf, sizeof(uint32_t) * m, hipMemcpyDeviceToDevice, S.stream));
     const uint32_t known = (uint32_t)per;  // every tie group agrees on its first `per` symbols
-    uint32_t small = kLabSmall;
     if (const char* e = getenv("MIBWT_LAB_SMALL")) {
         const int v = atoi(e);
         if (v >= 2 && v <= 256) small = (uint32_t)v;
@@ -1281,7 +1289,7 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     double rbytes = 0;
     uint32_t* Vs = nullptr;
     uint32_t* Hs = nullptr;
-    if (!label_order<RepLab>(S, T, n, phh, RepLab{rep, lam}, D, codes, sigma, &Vs, &Hs, &rbytes)) {
+    if (!label_order<RepLab>(S, T, n, phh, RepLab{rep, lam}, D, codes, sigma, kLabSmallDistinct, &Vs, &Hs, &rbytes)) {
         S.phase_end(ph, rbytes);
         return 0;
     }
@@ -1505,7 +1513,7 @@ static uint32_t lyndon_positions(Storage& S, const uint8_t* T, uint64_t n, const
     double rbytes = 0;
     uint32_t* Vl = nullptr;
     uint32_t* hf = nullptr;
-    if (!label_order<PosLab>(S, T, n, phh, PosLab{lam}, (uint32_t)n, codes, sigma, &Vl, &hf, &rbytes)) {
+    if (!label_order<PosLab>(S, T, n, phh, PosLab{lam}, (uint32_t)n, codes, sigma, kLabSmall, &Vl, &hf, &rbytes)) {
         S.phase_end(ph, rbytes);
         return 0;
     }

@@ -66,6 +66,8 @@ def lib():
         L.oracle_lyndon_ranks.restype = i64
         L.oracle_lyndon_bwt.argtypes = [u8p, i64, u8p]
         L.oracle_lyndon_bwt.restype = i64
+        L.oracle_lyndon_grammar_bwt.argtypes = [u8p, i64, u8p, ctypes.POINTER(LyndonGrammarInfo)]
+        L.oracle_lyndon_grammar_bwt.restype = i64
         _lib = L
     return _lib
 
@@ -216,6 +218,20 @@ def lyndon_ranks(T):
     r = np.empty(len(T), dtype=np.int32)
     g = lib().oracle_lyndon_ranks(_u8(T), len(T), _i32(r))
     return r, int(g)
+
+
+class LyndonGrammarInfo(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("labels", "slots", "roots", "ups", "rounds", "items")]
+
+
+def lyndon_grammar_bwt(T):
+    """Lyndon-grammar BWT by induction over distinct labels: (reference-format BWT, primary
+    index, sizes dict with labels, slots, roots, ups, rounds, items)."""
+    T = _as_text(T)
+    O = np.empty(len(T), dtype=np.uint8)
+    info = LyndonGrammarInfo()
+    idx = lib().oracle_lyndon_grammar_bwt(_u8(T), len(T), _u8(O), ctypes.byref(info)) if len(T) else 0
+    return O, int(idx), {k: int(getattr(info, k)) for k, _ in LyndonGrammarInfo._fields_}
 
 
 def lyndon_bwt(T):

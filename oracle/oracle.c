@@ -18,6 +18,10 @@
  *     BWT by induction over rank groups (LYN:4628-4896), here as one left-to-right induced
  *     pass: suffixes are grouped by their Lyndon prefix and, inside a group, ordered by the
  *     suffix that follows the prefix, which is smaller and therefore already placed.
+ *  4. The same BWT by induction over the grammar itself (LYN:4114-4896 with runs of equal
+ *     children): distinct labels with their child lists, items standing for a child slot in
+ *     every occurrence of a label, groups refined by the next run of labels and filled at once
+ *     when their items share the preceding character.
  */
 #define _GNU_SOURCE
 #include "oracle.h"
@@ -754,4 +758,219 @@ int64_t oracle_lyndon_bwt(const uint8_t* T, int64_t n, uint8_t* O)
     }
     free(SA); free(cfill); free(child); free(cstart); free(bucket); free(rank); free(lam);
     return index;
+}
+
+/* ------------------------------------------------------------------ 4. grammar induction
+ *
+ * The labels P_i = T[i .. i + lam[i]) are the maximal nodes of the Lyndon forest (LYN's rules
+ * and stack words).  Label M of length >= 2 has the children i + 1, then each next child at the
+ * end of the previous one, up to i + lam[i] (the Lyndon factorization of M without its first
+ * symbol).  The roots are the Lyndon factorization of T.  A suffix compares like its sequence
+ * of labels along the next-smaller-suffix chain, where a run of k equal labels compares as one
+ * symbol (label, k); runs never reach past the label whose children they are.  Items are slots
+ * (child k of distinct label d, or root s) standing for all positions in that role; a round
+ * sorts the items of every group by (label rank, run length) of their next run, fills every new
+ * group whose items share their preceding character, and moves the others past the run: to
+ * the next slot of the label, or, at the label's end, to every slot that follows the label
+ * where it occurs as a child (its up-contexts: the slots after the last-child chains of the
+ * non-final children and of the roots; LYN:4114-4256 builds the same lists as "right parents").
+ */
+
+typedef struct {
+    int64_t grp, key, q;
+    int32_t ch;  /* preceding character; 256 added for the item of suffix 0 */
+} lg_item;
+
+static int cmp_lg_item(const void* a, const void* b)
+{
+    const lg_item* x = (const lg_item*)a;
+    const lg_item* y = (const lg_item*)b;
+    if (x->grp != y->grp) return x->grp < y->grp ? -1 : 1;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return 0;
+}
+
+static int cmp_i64(const void* a, const void* b)
+{
+    const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+int64_t oracle_lyndon_grammar_bwt(const uint8_t* T, int64_t n, uint8_t* O,
+                                  struct oracle_lyndon_grammar_info* info)
+{
+    if (n <= 0) return 0;
+    int32_t* lam = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int32_t* ord = (int32_t*)malloc(sizeof(int32_t) * (size_t)n);
+    int64_t* lid = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    oracle_lyndon_array(T, n, lam);
+    /* distinct labels in lexicographic order: the label id is its rank */
+    for (int64_t i = 0; i < n; i++) ord[i] = (int32_t)i;
+    lyn_ctx lc = {T, lam};
+    qsort_r(ord, (size_t)n, sizeof(int32_t), cmp_lyn, &lc);
+    int64_t D = 0;
+    int64_t* rep = (int64_t*)malloc(sizeof(int64_t) * (size_t)n);
+    int64_t* cnt = (int64_t*)calloc((size_t)n, sizeof(int64_t));
+    for (int64_t i = 0; i < n; i++) {
+        if (i == 0 || cmp_lyn(&ord[i - 1], &ord[i], &lc) != 0) rep[D++] = ord[i];
+        lid[ord[i]] = D - 1;
+        cnt[D - 1]++;
+    }
+    /* child slots of every label, then the roots */
+    int64_t* off = (int64_t*)calloc((size_t)D + 1, sizeof(int64_t));
+    for (int64_t d = 0; d < D; d++) {
+        int64_t t = 0;
+        for (int64_t c = rep[d] + 1; c < rep[d] + lam[rep[d]]; c += lam[c]) t++;
+        off[d + 1] = off[d] + t;
+    }
+    const int64_t C = off[D];
+    int64_t S = 0;
+    for (int64_t c = 0; c < n; c += lam[c]) S++;
+    const int64_t Z = C + S;
+    int64_t* chl = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+    int64_t* cont = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+    int32_t* chc = (int32_t*)malloc(sizeof(int32_t) * (size_t)(Z + 1));
+    int64_t* last = (int64_t*)malloc(sizeof(int64_t) * (size_t)D);
+    for (int64_t d = 0; d < D; d++) {
+        int64_t q = off[d];
+        last[d] = -1;
+        for (int64_t c = rep[d] + 1; c < rep[d] + lam[rep[d]]; c += lam[c], q++) {
+            chl[q] = lid[c];
+            chc[q] = T[c - 1];
+            cont[q] = d;
+            last[d] = lid[c];
+        }
+    }
+    {
+        int64_t q = C;
+        for (int64_t c = 0; c < n; c += lam[c], q++) {
+            chl[q] = lid[c];
+            chc[q] = c ? T[c - 1] : T[n - 1];
+            cont[q] = -1;
+        }
+    }
+    /* runs of equal children: length from each slot and the slot after the run */
+    int64_t* mrun = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+    int64_t* nxt = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+    for (int64_t q = Z - 1; q >= 0; q--) {
+        const int same = q + 1 < Z && cont[q + 1] == cont[q] && chl[q + 1] == chl[q];
+        mrun[q] = same ? mrun[q + 1] + 1 : 1;
+        const int64_t nx = q + mrun[q];
+        if (cont[q] < 0) nxt[q] = nx;
+        else nxt[q] = (nx < Z && cont[nx] == cont[q]) ? nx : -1;
+    }
+    /* dense run keys (label rank, run length); 0 is the end of the text */
+    int64_t* rkey = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+    {
+        int64_t* pk = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));
+        int64_t* dr = (int64_t*)malloc(sizeof(int64_t) * (size_t)(Z + 1));  /* distinct values before k */
+        for (int64_t q = 0; q < Z; q++) pk[q] = chl[q] * (Z + 1) + mrun[q];
+        qsort(pk, (size_t)Z, sizeof(int64_t), cmp_i64);
+        for (int64_t k = 0; k < Z; k++) dr[k] = (k == 0) ? 0 : dr[k - 1] + (pk[k] != pk[k - 1]);
+        for (int64_t q = 0; q < Z; q++) {
+            const int64_t v = chl[q] * (Z + 1) + mrun[q];
+            int64_t lo = 0, hi = Z;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) / 2;
+                if (pk[mid] < v) lo = mid + 1;
+                else hi = mid;
+            }
+            rkey[q] = dr[lo] + 1;
+        }
+        free(dr);
+        free(pk);
+    }
+    rkey[Z] = 0;
+    /* up-contexts: CSR lists per label (uo offsets, ul targets) */
+    int64_t* uo = (int64_t*)calloc((size_t)D + 1, sizeof(int64_t));
+    for (int64_t q = 0; q < Z; q++)
+        if (cont[q] < 0 || (q + 1 < Z && cont[q + 1] == cont[q]))
+            for (int64_t v = chl[q]; off[v + 1] > off[v]; v = last[v]) uo[v + 1]++;
+    for (int64_t d = 0; d < D; d++) uo[d + 1] += uo[d];
+    const int64_t UT = uo[D];
+    int64_t* ul = (int64_t*)malloc(sizeof(int64_t) * (size_t)(UT + 1));
+    {
+        int64_t* fp = (int64_t*)malloc(sizeof(int64_t) * (size_t)(D + 1));
+        memcpy(fp, uo, sizeof(int64_t) * (size_t)(D + 1));
+        for (int64_t q = 0; q < Z; q++)
+            if (cont[q] < 0 || (q + 1 < Z && cont[q + 1] == cont[q]))
+                for (int64_t v = chl[q]; off[v + 1] > off[v]; v = last[v]) ul[fp[v]++] = q + 1;
+        free(fp);
+    }
+    /* the induction */
+    int64_t m = Z, rounds = 0, items = 0, pidx = -1;
+    lg_item* it = (lg_item*)malloc(sizeof(lg_item) * (size_t)(m + 1));
+    for (int64_t q = 0; q < Z; q++) {
+        it[q].grp = 0;
+        it[q].q = q;
+        it[q].ch = chc[q] + (q == C ? 256 : 0);  /* slot C: the root at position 0 */
+    }
+    while (m > 0) {
+        rounds++;
+        items += m;
+        /* keys: the next run; a group holding only root items orders them by position (a later
+         * root's suffix is smaller) */
+        for (int64_t i = 0; i < m; i++) it[i].key = 0;
+        qsort(it, (size_t)m, sizeof(lg_item), cmp_lg_item);
+        for (int64_t a = 0; a < m;) {
+            int64_t b = a, rule = 0;
+            while (b < m && it[b].grp == it[a].grp) rule |= it[b++].q < C;
+            for (int64_t i = a; i < b; i++) it[i].key = (it[i].q >= C && !rule) ? Z - it[i].q : rkey[it[i].q];
+            a = b;
+        }
+        qsort(it, (size_t)m, sizeof(lg_item), cmp_lg_item);
+        int64_t mo = 0, cap = m + 16;
+        lg_item* nx = (lg_item*)malloc(sizeof(lg_item) * (size_t)cap);
+        int64_t start = 0;
+        for (int64_t b = 0; b < m;) {
+            /* one new group: equal (group, key); its BWT range follows the earlier new groups of
+             * the same old group */
+            int64_t e = b, tot = 0;
+            int uni = 1;
+            if (b == 0 || it[b - 1].grp != it[b].grp) start = it[b].grp;
+            while (e < m && it[e].grp == it[b].grp && it[e].key == it[b].key) {
+                tot += (it[e].q < C) ? cnt[cont[it[e].q]] : 1;
+                if (e > b && (it[e].ch != it[e - 1].ch || it[e].ch >= 256 || it[e - 1].ch >= 256)) uni = 0;
+                e++;
+            }
+            if (uni) {
+                memset(O + start, it[b].ch & 0xFF, (size_t)tot);
+                if (it[b].ch >= 256) pidx = start + 1;
+            } else {
+                for (int64_t i = b; i < e; i++) {
+                    const int64_t q = it[i].q;
+                    const int64_t k0 = (nxt[q] >= 0) ? 0 : uo[cont[q]];
+                    const int64_t k1 = (nxt[q] >= 0) ? 1 : uo[cont[q] + 1];
+                    for (int64_t k = k0; k < k1; k++) {
+                        if (mo >= cap) {
+                            cap = 2 * cap;
+                            nx = (lg_item*)realloc(nx, sizeof(lg_item) * (size_t)cap);
+                        }
+                        nx[mo].grp = start;
+                        nx[mo].q = (nxt[q] >= 0) ? nxt[q] : ul[k];
+                        nx[mo].ch = it[i].ch;
+                        nx[mo].key = 0;
+                        mo++;
+                    }
+                }
+            }
+            start += tot;
+            b = e;
+        }
+        free(it);
+        it = nx;
+        m = mo;
+    }
+    free(it);
+    if (info) {
+        info->labels = D;
+        info->slots = C;
+        info->roots = S;
+        info->ups = UT;
+        info->rounds = rounds;
+        info->items = items;
+    }
+    free(ul); free(uo); free(rkey); free(nxt); free(mrun); free(last); free(chc); free(cont); free(chl);
+    free(off); free(cnt); free(rep); free(lid); free(ord); free(lam);
+    return pidx;
 }

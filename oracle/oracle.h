@@ -76,6 +76,18 @@ int64_t oracle_lyndon_ranks(const uint8_t* T, int64_t n, int32_t* rank);
 /* Lyndon-grammar BWT, reference format; returns the libcubwt primary index. */
 int64_t oracle_lyndon_bwt(const uint8_t* T, int64_t n, uint8_t* O);
 
+/* Sizes of the grammar induction: distinct labels, their child slots, roots (stack words),
+ * up-context pairs, induction rounds and items summed over the rounds. */
+struct oracle_lyndon_grammar_info {
+    int64_t labels, slots, roots, ups, rounds, items;
+};
+
+/* The Lyndon-grammar BWT by induction over distinct labels (the grammar restated with exact
+ * string comparisons), reference format; returns the libcubwt primary index and fills `info`
+ * (may be NULL). */
+int64_t oracle_lyndon_grammar_bwt(const uint8_t* T, int64_t n, uint8_t* O,
+                                  struct oracle_lyndon_grammar_info* info);
+
 #ifdef __cplusplus
 }
 #endif

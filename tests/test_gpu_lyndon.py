@@ -59,6 +59,37 @@ def test_lyndon_bwt(storage, mibwt, monkeypatch, name, mode):
     assert (mibwt.cubwt_to_reference(L, idx) == O).all()
 
 
+def _grammar_stats(st):
+    s = st.stats()
+    return s["induction"], s["retries"], dict(
+        labels=s["distinct_phrases"], slots=s["dict_length"], roots=s["triggers"], ups=s["mixed_entries"],
+        rounds=s["doubling_rounds_parse"], items=s["phrases"])
+
+
+@pytest.mark.parametrize("name", sorted(TEXTS))
+def test_grammar_sizes_match_oracle(storage, name):
+    # the grammar phases against the oracle's restatement with exact string comparisons:
+    # distinct labels, child slots, roots, up-context pairs, induction rounds and items
+    T = np.frombuffer(TEXTS[name], dtype=np.uint8)
+    storage.bwt(T, "lyndon")
+    induction, retries, got = _grammar_stats(storage)
+    _, _, info = oracle.lyndon_grammar_bwt(T)
+    assert induction == 1 and retries == 0
+    assert got == info
+
+
+def test_grammar_sizes_match_oracle_dna(mibwt):
+    from mibwt import gen
+    T = gen.workload("dna256m", scale=1 / 64)  # 4 MiB: 64 KiB base x 64 copies
+    with mibwt.Storage(len(T)) as st:
+        L, idx = st.bwt(T, "lyndon")
+        induction, retries, got = _grammar_stats(st)
+    O, oidx, info = oracle.lyndon_grammar_bwt(T)
+    assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
+    assert induction == 1 and retries == 0 and got == info
+    assert info["items"] < len(T) // 4  # one item per distinct label slot, not per position
+
+
 def test_lyndon_dna32m(mibwt):
     from mibwt import gen
     T = gen.workload("dna32m")
@@ -107,9 +138,12 @@ def test_long_runs_are_fast(mibwt, kind):
         t0 = time.perf_counter()
         L, idx = st.bwt(T, "lyndon")
         dt = time.perf_counter() - t0
+        induction, _, got = _grammar_stats(st)
     O, oidx = oracle.bwt(T)
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
     assert dt < 10.0, dt
+    # the run of equal children is one key (label, run length): no round per symbol
+    assert induction == 1 and got["rounds"] < 100, got
 
 
 @pytest.mark.timeout(600)

@@ -283,6 +283,62 @@ def test_lyndon_bwt_restatement_repetitive():
     assert 0 < g < len(T)
 
 
+def _duval(w: bytes):
+    """Lyndon factorization of w by Duval's algorithm (starts of the factors)."""
+    out, i, n = [], 0, len(w)
+    while i < n:
+        j, k = i + 1, i
+        while j < n and w[k] <= w[j]:
+            k = i if w[k] < w[j] else k + 1
+            j += 1
+        while i <= k:
+            out.append(i)
+            i += j - k
+    return out
+
+
+def _grammar_sizes(T: bytes):
+    """Sizes of the label grammar from the definitions alone: the distinct longest Lyndon
+    prefixes (brute force), their children (the Lyndon factorization of the label without its
+    first symbol, by Duval) and the roots (the Lyndon factorization of T)."""
+    lam = _brute_lyndon_array(T)
+    labels = {T[i:i + lam[i]] for i in range(len(T))}
+    slots = sum(len(_duval(M[1:])) for M in labels)
+    return len(labels), slots, len(_duval(T))
+
+
+@settings(max_examples=300, deadline=None)
+@given(st.binary(min_size=1, max_size=14).map(lambda b: bytes(x % 4 for x in b))
+       | st.lists(st.sampled_from([0, 1, 255]), min_size=1, max_size=14).map(bytes))
+def test_lyndon_grammar_restatement_small(T):
+    O, idx, info = oracle.lyndon_grammar_bwt(T)
+    assert (bytes(O), idx) == brute_bwt(T)
+    assert (info["labels"], info["slots"], info["roots"]) == _grammar_sizes(T)
+    # every position is one child slot of one label occurrence, or a root
+    assert info["items"] >= info["slots"] + info["roots"]
+
+
+def test_lyndon_grammar_restatement_shapes():
+    # repetitive DNA (few labels, many rounds), runs of one symbol before a smaller one (long
+    # runs of equal children, settled in one round by their run length), descending bytes (many
+    # roots), periodic text, all byte values
+    rng = np.random.default_rng(16)
+    base = rng.choice(np.frombuffer(b"ACGT", dtype=np.uint8), size=20000)
+    dna = np.concatenate([base, base[11:], base[:9000], base])
+    nrun = np.concatenate([base[:5000], np.full(20000, ord("N"), np.uint8), base[5000:]])
+    cases = {"dna": dna, "nrun": nrun, "descending": np.arange(255, -1, -1, dtype=np.uint8).repeat(3),
+             "period": np.tile(np.frombuffer(b"ACGTA", dtype=np.uint8), 3000),
+             "bytes": rng.integers(0, 256, size=5000).astype(np.uint8)}
+    for name, T in cases.items():
+        O, idx, info = oracle.lyndon_grammar_bwt(T)
+        Ob, ib = oracle.bwt(T)
+        assert (O == Ob).all() and idx == ib, name
+    _, _, info = oracle.lyndon_grammar_bwt(dna)
+    assert info["labels"] < len(dna) // 10 and info["items"] < len(dna)  # the grammar compresses
+    _, _, info = oracle.lyndon_grammar_bwt(nrun)
+    assert info["rounds"] < 100  # a run of 20000 equal children does not take a round per child
+
+
 @pytest.mark.parametrize("threads", [1, 2, 5, 16])
 def test_unbwt_mt_matches_sequential_inverse(threads):
     # the multi-threaded inverse (segments of the LF cycle walked in parallel) gives the text
