@@ -231,6 +231,10 @@ def main():
         sl = st.stats()
         lyndon = {"value": round(n / 1e6 / (ms_l / 1000.0), 2), "ms_per_step": round(ms_l, 4),
                   "steps": a.lyndon_steps, "retries": sl["retries"],
+                  "induction": {1: "grammar (distinct labels)", 2: "position-level"}.get(int(sl["induction"])),
+                  "grammar": {"labels": sl["distinct_phrases"], "child_slots": sl["dict_length"],
+                              "roots": sl["triggers"], "up_contexts": sl["mixed_entries"],
+                              "rounds": sl["doubling_rounds_parse"], "items": sl["phrases"]},
                   "device_memory_bytes": int(sl["device_bytes"]),
                   "phases": {p["name"]: round(p["ms"], 3) for p in sl["phases"]}}
         if O_fixed is not None:
