@@ -392,6 +392,15 @@ def main():
                 sys.exit(1)
             checked += 1
         dU.free()
+        # the same texts built one by one through the single-text device entry (learned (W,P)
+        # per text): what one multi-text build saves
+        stb.bwt_device(dB.address, dBO.address, len(texts[0]), "pfp", 0, 0)
+        mibwt.synchronize()
+        t0 = time.perf_counter()
+        for i, t in enumerate(texts):
+            stb.bwt_device(dB.address + int(offs[i]), dBO.address + int(offs[i]), len(t), "pfp", 0, 0)
+        mibwt.synchronize()
+        per_ms = (time.perf_counter() - t0) * 1000.0
         t0 = time.perf_counter()
         res = stb.batch_bwt(texts, "pfp", 0, 0)
         hms = (time.perf_counter() - t0) * 1000.0
@@ -404,9 +413,12 @@ def main():
                  "device_memory_bytes": int(sb["device_bytes"]),
                  "host_buffers": {"ms": round(hms, 2), "value": round(tot / 1e6 / (hms / 1000.0), 1),
                                   "entry": "mibwt_batch_bwt (host texts in, host BWTs out, PCIe included)"},
+                 "per_text": {"ms": round(per_ms, 2), "value": round(tot / 1e6 / (per_ms / 1000.0), 1),
+                              "entry": "mibwt_pfp_bwt_device once per text (learned (W,P) per text)"},
+                 "speedup_vs_per_text": round(per_ms / dms, 3),
                  "verified_texts": checked}
         log(f"batch: {len(texts)} texts, {tot / 1e6:.0f} MB: device {dms:.1f} ms = {batch['value']} MB/s, "
-            f"host buffers {hms:.1f} ms = {batch['host_buffers']['value']} MB/s")
+            f"host buffers {hms:.1f} ms = {batch['host_buffers']['value']} MB/s, one by one {per_ms:.1f} ms")
         stb.close()
         dB.free()
         dBO.free()

@@ -14,12 +14,12 @@ SEED_BASE = 0xB17C0DE
 
 
 def mutate(rng: np.random.Generator, base_idx: np.ndarray, sub_rate: float, indel_rate: float,
-           indel_max: int) -> np.ndarray:
-    """One mutated copy of a base given as symbol indices 0..3; returns indices."""
+           indel_max: int, sigma: int = 4) -> np.ndarray:
+    """One mutated copy of a base given as symbol indices 0..sigma-1; returns indices."""
     x = base_idx.copy()
     if sub_rate > 0:
         hit = np.flatnonzero(rng.random(len(x)) < sub_rate)
-        x[hit] = (x[hit] + rng.integers(1, 4, size=len(hit))) % 4
+        x[hit] = (x[hit].astype(np.int64) + rng.integers(1, sigma, size=len(hit))) % sigma
     if indel_rate <= 0:
         return x
     nev = rng.binomial(len(x), indel_rate)
@@ -35,7 +35,7 @@ def mutate(rng: np.random.Generator, base_idx: np.ndarray, sub_rate: float, inde
             continue
         pieces.append(x[cur:p])
         if ins:
-            pieces.append(rng.integers(0, 4, size=ln).astype(np.uint8))
+            pieces.append(rng.integers(0, sigma, size=ln).astype(np.uint8))
             cur = p
         else:
             cur = min(len(x), p + ln)
@@ -49,6 +49,16 @@ def repetitive_dna(base_len: int, copies: int, sub_rate: float, indel_rate: floa
     base = rng.integers(0, 4, size=base_len).astype(np.uint8)
     parts = [mutate(rng, base, sub_rate, indel_rate, indel_max) for _ in range(copies)]
     return ACGT[np.concatenate(parts)]
+
+
+def repetitive_bytes(base_len: int, copies: int, sub_rate: float, indel_rate: float,
+                     indel_max: int, seed: int) -> np.ndarray:
+    """Like repetitive_dna over all 256 byte values: a uniform random byte base, each copy with
+    substitutions to another byte and insertions of random bytes."""
+    rng = np.random.default_rng(seed)
+    base = rng.integers(0, 256, size=base_len).astype(np.uint8)
+    parts = [mutate(rng, base, sub_rate, indel_rate, indel_max, sigma=256) for _ in range(copies)]
+    return np.concatenate(parts)
 
 
 def markov2_printable(n: int, seed: int, alpha: float = 0.5, chains: int = 1024) -> np.ndarray:
@@ -94,9 +104,10 @@ def batch_collection(count: int, seed: int = SEED_BASE + 4, threads: int = 16, m
                      max_len: int = 16 << 20, max_copies: int = 64):
     """`count` independent texts shaped like the batched configuration: lengths log-uniform in
     [min_len, max_len], each a random base replicated 2..max_copies times with substitutions at a
-    log-uniform rate in [1e-5, 1e-2] (indels at a tenth of it, length 1..10); every other text is
-    mapped onto 256 byte values.  The parameters of text i are drawn from one seeded stream, the
-    text itself from its own seed, so the result does not depend on `threads`."""
+    log-uniform rate in [1e-5, 1e-2] (indels at a tenth of it, length 1..10); even texts over
+    A/C/G/T, odd texts over all 256 byte values.  The parameters of text i are drawn from one
+    seeded stream, the text itself from its own seed, so the result does not depend on
+    `threads`."""
     from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(seed)
     specs = []
@@ -108,10 +119,8 @@ def batch_collection(count: int, seed: int = SEED_BASE + 4, threads: int = 16, m
 
     def make(spec):
         i, n_total, copies, rate, s = spec
-        T = repetitive_dna(max(1024, n_total // copies), copies, rate, rate / 10, 10, seed=s)
-        if i % 2:
-            T = (T.astype(np.uint32) * 97 + np.arange(len(T)) % 7 * 31).astype(np.uint8)
-        return T
+        make_text = repetitive_bytes if i % 2 else repetitive_dna
+        return make_text(max(1024, n_total // copies), copies, rate, rate / 10, 10, seed=s)
 
     with ThreadPoolExecutor(max(1, threads)) as ex:
         return list(ex.map(make, specs))
