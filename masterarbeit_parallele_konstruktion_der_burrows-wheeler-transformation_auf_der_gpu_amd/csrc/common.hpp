@@ -491,6 +491,17 @@ enum BufId : int {
     B_GR_UK1,
     B_GR_UV0,
     B_GR_UV1,
+    B_GL_FA,        // label keys of one word: flags, then their positions in the short list
+    B_GL_KA0,       // short label keys (high word) and positions, sorted
+    B_GL_KA1,
+    B_GL_VA0,
+    B_GL_VA1,
+    B_GL_REPA,      // distinct short labels: representative, count, key
+    B_GL_CNTA,
+    B_GL_UA,
+    B_GL_REPB,      // distinct long labels: representative, count, key
+    B_GL_CNTB,
+    B_GL_UB,
     // Lyndon grammar: items of the induction (double-buffered) and per-round arrays
     B_IT_Q0,
     B_IT_Q1,

@@ -755,6 +755,9 @@ enum GrSlot : int {
     GR_RB,        // bytes of those runs
     GR_NG,        // groups of this round
     GR_PIDX,      // primary index (BWT position of suffix 0, plus one)
+    GR_NA,        // positions whose label key fits its high word
+    GR_DA,        // distinct labels among them
+    GR_DB,        // distinct labels among the others
     GR_COUNT_ = 16
 };
 
@@ -801,29 +804,92 @@ __device__ __forceinline__ uint32_t key_lookup(const uint64_t* __restrict__ U, u
     return lo;
 }
 
-__global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint64_t* __restrict__ K,
-                          uint32_t* __restrict__ V)
+// Label keys whose low word is zero (labels of at most `per32` symbols, exact in the high word)
+// are sorted as 32-bit keys, the others as 64-bit keys: fa[i] = 1 for the former.
+__global__ void k_gl_short(const uint32_t* __restrict__ lam, uint64_t n, uint32_t per32, uint32_t* __restrict__ fa)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) fa[i] = (lam[i] <= per32) ? 1u : 0u;
+}
+
+// the key of every position into the short list (high word; fa = exclusive scan of the flags)
+// or the long list (all 64 bits), with the position as value
+__global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32_t per32,
+                          const uint32_t* __restrict__ fa, uint32_t* __restrict__ KA, uint32_t* __restrict__ VA,
+                          uint64_t* __restrict__ KB, uint32_t* __restrict__ VB)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
     __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lk.n) return;
-    K[i] = lk((uint32_t)i, cs);
-    V[i] = (uint32_t)i;
+    const uint64_t key = lk((uint32_t)i, cs);
+    const uint32_t a = fa[i];
+    if (lk.lam[i] <= per32) {
+        KA[a] = (uint32_t)(key >> 32);
+        VA[a] = (uint32_t)i;
+    } else {
+        KB[i - a] = key;
+        VB[i - a] = (uint32_t)i;
+    }
 }
 
-// per distinct label d (the runs of equal sorted keys): representative, count, key
-__global__ void k_gl_dedup(const uint64_t* __restrict__ K, const uint32_t* __restrict__ V,
-                           const uint32_t* __restrict__ gofs, uint32_t D, uint32_t* __restrict__ rep,
-                           uint32_t* __restrict__ cnt, uint64_t* __restrict__ U)
+__global__ void k_gl_heads32(const uint32_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) hf[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
+}
+
+// per distinct label d (the runs of equal sorted keys): representative, count, 64-bit key
+// (a short key is the high word of its 64-bit key)
+template <typename K>
+__global__ void k_gl_dedup(const K* __restrict__ Ks, const uint32_t* __restrict__ V, const uint32_t* __restrict__ gofs,
+                           uint32_t D, uint32_t* __restrict__ rep, uint32_t* __restrict__ cnt,
+                           uint64_t* __restrict__ U)
 {
     const uint32_t d = blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= D) return;
     const uint32_t j = gofs[d];
     rep[d] = V[j];
     cnt[d] = gofs[d + 1] - j;
-    U[d] = K[j];
+    U[d] = (sizeof(K) == 4) ? ((uint64_t)Ks[j] << 32) : (uint64_t)Ks[j];
+}
+
+// number of keys in the sorted X[0..m) below x
+__device__ __forceinline__ uint32_t count_below64(const uint64_t* __restrict__ X, uint32_t m, uint64_t x)
+{
+    uint32_t lo = 0, hi = m;
+    while (lo < hi) {
+        const uint32_t mid = lo + (hi - lo) / 2;
+        if (X[mid] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// the two sorted lists of distinct labels (their keys never coincide: a short key's low word is
+// zero, a long label has more symbols or the fingerprint flag) merged into one sorted list
+__global__ void k_gl_merge(const uint64_t* __restrict__ UA, const uint32_t* __restrict__ RA,
+                           const uint32_t* __restrict__ CA, uint32_t DA, const uint64_t* __restrict__ UB,
+                           const uint32_t* __restrict__ RB, const uint32_t* __restrict__ CB, uint32_t DB,
+                           uint64_t* __restrict__ U, uint32_t* __restrict__ rep, uint32_t* __restrict__ cnt)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= DA + DB) return;
+    if (t < DA) {
+        const uint64_t k = UA[t];
+        const uint32_t o = t + count_below64(UB, DB, k);
+        U[o] = k;
+        rep[o] = RA[t];
+        cnt[o] = CA[t];
+    } else {
+        const uint32_t d = t - DA;
+        const uint64_t k = UB[d];
+        const uint32_t o = d + count_below64(UA, DA, k);
+        U[o] = k;
+        rep[o] = RB[d];
+        cnt[o] = CB[d];
+    }
 }
 
 __global__ void k_gl_ends(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ E)
@@ -1193,21 +1259,42 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     // ---- distinct labels
     int ph = S.phase_begin("labels");
     const int bits = bits_for((uint64_t)sigma + 1);
-    const LabelKey lk{T, n, lam, phh, bits, 63 / bits};
-    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
-    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
-    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, n);
-    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, n);
-    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, K0, V0);
+    const int per = 63 / bits;
+    const LabelKey lk{T, n, lam, phh, bits, per};
+    // symbols wholly inside the key's high word
+    const uint32_t per32 = (uint32_t)std::max(0, per - (32 + bits - 1) / bits);
+    uint32_t* fa = S.buf<uint32_t>(B_GL_FA, n);
+    k_gl_short<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, per32, fa);
     MIBWT_LAUNCH_CHECK();
-    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, n, 0, 64);
+    exclusive_sum<uint32_t>(S, fa, fa, n, gsc + GR_NA);
+    S.read_fast(hv, gsc, GR_COUNT_);
+    const uint32_t nA = hv[GR_NA], nB = (uint32_t)n - nA;
+    uint32_t* KA0 = S.buf<uint32_t>(B_GL_KA0, nA);
+    uint32_t* KA1 = S.buf<uint32_t>(B_GL_KA1, nA);
+    uint32_t* VA0 = S.buf<uint32_t>(B_GL_VA0, nA);
+    uint32_t* VA1 = S.buf<uint32_t>(B_GL_VA1, nA);
+    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, nB);
+    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, nB);
+    uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, nB);
+    uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, nB);
+    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, K0, V0);
+    MIBWT_LAUNCH_CHECK();
+    radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, 31);
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, nB, 0, 64);
     uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
     uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
-    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 1);
-    k_dbl_headflag<<<grid_for(n), kBlock, 0, S.stream>>>(K0, (uint32_t)n, hf);
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 2);
+    uint32_t* hfB = hf + nA;
+    uint32_t* gidB = gid1 + nA;
+    uint32_t* gofsB = gofs + nA + 1;
+    k_gl_heads32<<<grid_for(nA), kBlock, 0, S.stream>>>(KA0, nA, hf);
     MIBWT_LAUNCH_CHECK();
-    inclusive_sum<uint32_t>(S, hf, gid1, n, gsc + GR_D);
-    k_grp_offsets<<<grid_for(n), kBlock, 0, S.stream>>>(hf, gid1, (uint32_t)n, gofs);
+    k_dbl_headflag<<<grid_for(nB), kBlock, 0, S.stream>>>(K0, nB, hfB);
+    MIBWT_LAUNCH_CHECK();
+    inclusive_sum<uint32_t>(S, hf, gid1, nA, gsc + GR_DA);
+    inclusive_sum<uint32_t>(S, hfB, gidB, nB, gsc + GR_DB);
+    if (nA) k_grp_offsets<<<grid_for(nA), kBlock, 0, S.stream>>>(hf, gid1, nA, gofs);
+    if (nB) k_grp_offsets<<<grid_for(nB), kBlock, 0, S.stream>>>(hfB, gidB, nB, gofsB);
     MIBWT_LAUNCH_CHECK();
     // roots: prefix maxima of i + lam[i]
     uint32_t* PM = S.buf<uint32_t>(B_LY_JMP0, n);
@@ -1219,14 +1306,25 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     MIBWT_LAUNCH_CHECK();
     exclusive_sum<uint32_t>(S, rpos, rpos, n, gsc + GR_S);
     S.read_fast(hv, gsc, GR_COUNT_);
-    const uint32_t D = hv[GR_D], NS = hv[GR_S];
+    const uint32_t DA = hv[GR_DA], DB = hv[GR_DB], NS = hv[GR_S];
+    const uint32_t D = DA + DB;
     uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, NS);
     k_gl_roots<<<grid_for(n), kBlock, 0, S.stream>>>(PM, rpos, n, roots);
+    MIBWT_LAUNCH_CHECK();
+    uint32_t* repA = S.buf<uint32_t>(B_GL_REPA, DA);
+    uint32_t* cntA = S.buf<uint32_t>(B_GL_CNTA, DA);
+    uint64_t* UA = S.buf<uint64_t>(B_GL_UA, DA);
+    uint32_t* repB = S.buf<uint32_t>(B_GL_REPB, DB);
+    uint32_t* cntB = S.buf<uint32_t>(B_GL_CNTB, DB);
+    uint64_t* UB = S.buf<uint64_t>(B_GL_UB, DB);
+    k_gl_dedup<uint32_t><<<grid_for(DA), kBlock, 0, S.stream>>>(KA0, VA0, gofs, DA, repA, cntA, UA);
+    MIBWT_LAUNCH_CHECK();
+    k_gl_dedup<uint64_t><<<grid_for(DB), kBlock, 0, S.stream>>>(K0, V0, gofsB, DB, repB, cntB, UB);
     MIBWT_LAUNCH_CHECK();
     uint32_t* rep = S.buf<uint32_t>(B_GR_REP, D);
     uint32_t* cnt = S.buf<uint32_t>(B_GR_CNT, D);
     uint64_t* U = S.buf<uint64_t>(B_GR_KEY, D);
-    k_gl_dedup<<<grid_for(D), kBlock, 0, S.stream>>>(K0, V0, gofs, D, rep, cnt, U);
+    k_gl_merge<<<grid_for(D), kBlock, 0, S.stream>>>(UA, repA, cntA, DA, UB, repB, cntB, DB, U, rep, cnt);
     MIBWT_LAUNCH_CHECK();
     uint32_t* off = S.buf<uint32_t>(B_GR_OFF, (size_t)D + 1);
     k_gl_nchild<<<grid_for(D), kBlock, 0, S.stream>>>(rep, lam, D, off);
@@ -1278,7 +1376,7 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
         k_gl_csr<<<grid_for(UT), kBlock, 0, S.stream>>>(UK0, UT, D, uo);
         MIBWT_LAUNCH_CHECK();
     }
-    S.phase_end(ph, (double)n * (4 + 8 + 4 + 4) + (double)n * 24 * 8 + (double)Z * 32);
+    S.phase_end(ph, (double)n * (4 + 8 + 4 + 4) + (double)nA * 16 * 4 + (double)nB * 24 * 8 + (double)Z * 32);
     S.stats.distinct_phrases = D;
     S.stats.dict_length = C;
     S.stats.triggers = NS;
