@@ -8,6 +8,7 @@ indels of length 1..10).  Prints one JSON line.
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload dna1g] [--W 5 --P 16]
                   [--wp auto|fixed] [--lyndon-steps 1] [--batch-texts 256] [--grid-steps 0]
                   [--control markov256m] [--control-steps 3] [--no-verify] [--no-cpu-baseline]
+                  [--method pfp|lyndon]
 
 With --wp auto (default) every timed step runs the learned (W,P) selector on the resident text
 and then the build with its choice; `value` is that rate.  Beside it: the fixed W/P build
@@ -16,7 +17,8 @@ text with fixed and learned (W,P) ("control"), the Lyndon-grammar BWT of the sam
 byte for byte ("lyndon"), the host-buffer entry with its PCIe copies ("pcie_inclusive"), a
 batch of independent texts of uneven length ("batch"), the peak device memory of the storage
 ("device_memory_bytes"), and the multi-threaded C++ restatement of the reference pipeline on the
-host cores ("cpu_baseline").
+host cores ("cpu_baseline").  --method lyndon times the Lyndon-grammar build of the workload as
+the step instead (its own JSON line; used for the Lyndon profiles).
 """
 from __future__ import annotations
 
@@ -79,6 +81,40 @@ def cpu_baseline(T, W, P, sais_sample_bytes):
                              "sample": f"first {len(sample) >> 20} MiB, SA-IS oracle, 1 thread, {ds:.2f} s"}}
 
 
+def lyndon_only(a, mibwt, st, T, dT, dO):
+    """The Lyndon-grammar build of the workload as the timed step (one JSON line)."""
+    n = len(T)
+    idx = None
+    for i in range(a.warmup):
+        idx = st.bwt_device(dT.address, dO.address, n, "lyndon")
+    mibwt.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        idx = st.bwt_device(dT.address, dO.address, n, "lyndon")
+    mibwt.synchronize()
+    ms = (time.perf_counter() - t0) * 1000.0 / a.steps
+    s = st.stats()
+    verified = None
+    if not a.no_verify:
+        dU = mibwt.DeviceBuffer(n)
+        st.unbwt_device(dO.address, dU.address, n, idx)
+        verified = "unbwt_roundtrip_ok_gpu" if bool((dU.download() == T).all()) else "MISMATCH"
+        dU.free()
+    out = {"metric": "Lyndon-grammar BWT build MB/s on 1 MI355X, bit-exact", "value": round(n / 1e6 / (ms / 1000.0), 2),
+           "unit": "MB/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u8", "data": "synthetic",
+           "config": {"workload": a.workload, "n_bytes": n, "method": "lyndon", "parallelism": "single-gpu"},
+           "induction": {1: "grammar (distinct labels)", 2: "position-level"}.get(int(s["induction"])),
+           "grammar": {"labels": s["distinct_phrases"], "child_slots": s["dict_length"], "roots": s["triggers"],
+                       "up_contexts": s["mixed_entries"], "rounds": s["doubling_rounds_parse"], "items": s["phrases"]},
+           "retries": s["retries"], "device_memory_bytes": int(s["device_bytes"]),
+           "phases": {p["name"]: round(p["ms"], 3) for p in s["phases"]}, "verified": verified}
+    print(json.dumps(out), flush=True)
+    st.close()
+    if verified == "MISMATCH":
+        sys.exit(1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,6 +143,9 @@ def main():
                     help="time the whole radix digit pass on this many random pairs (0: skip)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also launch a plain 256 MiB device copy kernel (byte-counter reference)")
+    ap.add_argument("--method", choices=["pfp", "lyndon"], default="pfp",
+                    help="lyndon: time the Lyndon-grammar build as the step (checked by the GPU inverse "
+                         "BWT); the PFP legs are skipped")
     a = ap.parse_args()
 
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -126,6 +165,9 @@ def main():
     dO = mibwt.DeviceBuffer(n)
     mibwt.synchronize()
     st = mibwt.Storage(n)
+    if a.method == "lyndon":
+        lyndon_only(a, mibwt, st, T, dT, dO)
+        return
 
     def run(step, steps, warmup, label, size=n):
         idx = None
