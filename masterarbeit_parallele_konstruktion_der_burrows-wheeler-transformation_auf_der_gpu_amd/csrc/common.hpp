@@ -429,6 +429,8 @@ enum BufId : int {
     B_UN_D1,
     B_UN_IO,
     B_UN_AUX,
+    B_UN_STASH,     // the first kUnStash bytes of every splitter's walk
+    B_UN_ROW,       // the row each longer walk reached after kUnStash steps
     B_SCALARS,
     B_STAGE,
     B_DB_KEY2,

@@ -22,6 +22,7 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
 
 constexpr uint32_t kUnSample = 64;
 constexpr uint32_t kUnEnd = 0xFFFFFFFFu;
+constexpr uint32_t kUnStash = 256;  // bytes kept per walk (a walk exceeds it with probability ~e^-4)
 
 __device__ __forceinline__ uint8_t bwt_row(const uint8_t* __restrict__ L, uint64_t r, uint64_t index, bool ref)
 {
@@ -51,9 +52,11 @@ struct EmitLF {
     }
 };
 
-// walk from splitter s to the next splitter: rows visited, next splitter (kUnEnd: the '$' row)
+// walk from splitter s to the next splitter: rows visited, next splitter (kUnEnd: the '$' row);
+// with `stash`, the bytes of the first kUnStash rows of the walk and the row after them (rowk)
 __global__ void k_un_walk1(const uint64_t* __restrict__ rec, uint64_t m, uint64_t index, uint32_t S,
-                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ w, uint32_t* __restrict__ bad)
+                           uint32_t* __restrict__ nxt, uint32_t* __restrict__ w, uint32_t* __restrict__ bad,
+                           uint8_t* __restrict__ stash, uint32_t* __restrict__ rowk)
 {
     const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= S) return;
@@ -65,9 +68,13 @@ __global__ void k_un_walk1(const uint64_t* __restrict__ rec, uint64_t m, uint64_
     }
     uint64_t r = r0, steps = 0;
     uint32_t next = kUnEnd;
+    uint8_t* st = stash ? stash + (uint64_t)s * kUnStash : nullptr;
     for (;;) {
-        r = (uint32_t)rec[r];
+        const uint64_t x = rec[r];
+        if (st && steps < kUnStash) st[steps] = (uint8_t)(x >> 32);
+        r = (uint32_t)x;
         steps++;
+        if (st && steps == kUnStash) rowk[s] = (uint32_t)r;
         if (r == index) break;
         if (r % kUnSample == 0) {
             next = (uint32_t)(r / kUnSample);
@@ -117,6 +124,43 @@ __global__ void k_un_walk2(const uint64_t* __restrict__ rec, const uint32_t* __r
             if (U) U[n - 1 - k] = (uint8_t)(x >> 32);
             if (I && k > 0 && (n - k) % rate == 0) I[(n - k) / rate] = (uint32_t)r;
         }
+        k++;
+        r = (uint32_t)x;
+    }
+}
+
+// The kept bytes of walk s (one wavefront per walk, four bytes per lane) to their places in the
+// text: step i of the walk is step k = n - dist(s) + i of the whole walk, preceded by T[n-1-k].
+__global__ __launch_bounds__(256) void k_un_emit(const uint8_t* __restrict__ stash, const uint32_t* __restrict__ w,
+                                                 const uint32_t* __restrict__ dist, uint64_t n, uint32_t S,
+                                                 uint8_t* __restrict__ U)
+{
+    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    if (s >= S) return;
+    const uint32_t ws = w[s];
+    if (ws == 0) return;
+    const uint32_t cnt = ws < kUnStash ? ws : kUnStash;
+    const uint64_t k0 = n - dist[s];
+    const uint8_t* st = stash + (uint64_t)s * kUnStash;
+    for (uint32_t i = lane; i < cnt; i += 64) {
+        const uint64_t k = k0 + i;
+        if (k < n) U[n - 1 - k] = st[i];
+    }
+}
+
+// the rest of the walks longer than kUnStash, from the row they reached
+__global__ void k_un_tail(const uint64_t* __restrict__ rec, const uint32_t* __restrict__ w,
+                          const uint32_t* __restrict__ dist, const uint32_t* __restrict__ rowk, uint64_t n,
+                          uint32_t S, uint8_t* __restrict__ U)
+{
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= S || w[s] <= kUnStash) return;
+    uint64_t k = n - dist[s] + kUnStash;
+    uint64_t r = rowk[s];
+    for (uint32_t i = kUnStash; i < w[s]; i++) {
+        const uint64_t x = rec[r];
+        if (k < n) U[n - 1 - k] = (uint8_t)(x >> 32);
         k++;
         r = (uint32_t)x;
     }
@@ -198,7 +242,10 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     uint32_t* d0 = S.buf<uint32_t>(B_UN_D0, Sn);
     uint32_t* d1 = S.buf<uint32_t>(B_UN_D1, Sn);
     fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
-    k_un_walk1<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, m, index, Sn, nxt0, w, sc + SC_COLLISION);
+    // the text (not the sampled rows) is written from the bytes kept by the first walk
+    uint8_t* stash = dI ? nullptr : S.buf<uint8_t>(B_UN_STASH, (size_t)Sn * kUnStash);
+    uint32_t* rowk = dI ? nullptr : S.buf<uint32_t>(B_UN_ROW, Sn);
+    k_un_walk1<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, m, index, Sn, nxt0, w, sc + SC_COLLISION, stash, rowk);
     MIBWT_LAUNCH_CHECK();
     MIBWT_HIP(hipMemcpyAsync(d0, w, sizeof(uint32_t) * Sn, hipMemcpyDeviceToDevice, S.stream));
     for (uint64_t span = 1; span < Sn; span *= 2) {
@@ -220,7 +267,13 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
 
     ph = phase_begin("text");
     if (dI) fill<uint32_t>(S, dI, 1, (uint32_t)index);  // position 0: the primary index
-    k_un_walk2<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, n, Sn, dU, dI, dI ? (uint64_t)rate : 1);
+    if (dI) {
+        k_un_walk2<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, n, Sn, dU, dI, (uint64_t)rate);
+    } else {
+        k_un_emit<<<ceil_div(Sn, 4), 256, 0, S.stream>>>(stash, w, d0, n, Sn, dU);
+        MIBWT_LAUNCH_CHECK();
+        k_un_tail<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, rowk, n, Sn, dU);
+    }
     MIBWT_LAUNCH_CHECK();
     phase_end(ph, 9.0 * m);
     if (own_phases) S.phases_collect();
