@@ -110,10 +110,61 @@ struct OpHash {
     }
 };
 
-__global__ void k_hash_items(const uint8_t* __restrict__ T, uint64_t n, uint64_t base, HashPair* __restrict__ out)
+// Prefix fingerprints straight from the text, reduce-then-scan with 16 bytes per thread (one
+// 16-byte load): a run of bytes c_1..c_k hashes to (sum (c_i + 1) base^(k-i), base^k), so each
+// byte costs one modular multiply while the thread runs over it, and runs combine with OpHash.
+constexpr int kPhItems = 16;
+constexpr int kPhTile = kBlock * kPhItems;
+
+__device__ __forceinline__ HashPair ph_run(const uint8_t* __restrict__ T, uint64_t n, uint64_t b0, uint64_t base,
+                                           uint64_t base16, uint4& w)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = HashPair{(uint64_t)T[i] + 1, base};
+    w = *reinterpret_cast<const uint4*>(T + b0);  // the padded copy reads zeros past n
+    const uint32_t cnt = (b0 >= n) ? 0u : (uint32_t)((n - b0 < (uint64_t)kPhItems) ? n - b0 : kPhItems);
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+    uint64_t h = 0, p = 1;
+#pragma unroll
+    for (uint32_t i = 0; i < (uint32_t)kPhItems; i++) {
+        if (i < cnt) {
+            h = addmod61(mulmod61(h, base), (uint64_t)((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) + 1);
+            if (cnt < (uint32_t)kPhItems) p = mulmod61(p, base);
+        }
+    }
+    return HashPair{h, cnt == (uint32_t)kPhItems ? base16 : p};
+}
+
+__global__ __launch_bounds__(kBlock) void k_ph_reduce(const uint8_t* __restrict__ T, uint64_t n, uint64_t base,
+                                                     uint64_t base16, HashPair* __restrict__ partial)
+{
+    __shared__ HashPair lds[kWaves];
+    uint4 w;
+    const HashPair acc = ph_run(T, n, (uint64_t)blockIdx.x * kPhTile + (uint64_t)threadIdx.x * kPhItems, base, base16, w);
+    HashPair tot;
+    (void)block_exclusive_scan(acc, HashPair{0, 1}, OpHash{}, lds, &tot);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
+}
+
+// pre[k] = (H(T[0..k]), base^(k+1)) from the tiles' exclusive prefixes
+__global__ __launch_bounds__(kBlock) void k_ph_down(const uint8_t* __restrict__ T, uint64_t n, uint64_t base,
+                                                   uint64_t base16, const HashPair* __restrict__ partial,
+                                                   HashPair* __restrict__ pre)
+{
+    __shared__ HashPair lds[kWaves];
+    const uint64_t b0 = (uint64_t)blockIdx.x * kPhTile + (uint64_t)threadIdx.x * kPhItems;
+    uint4 w;
+    const HashPair acc = ph_run(T, n, b0, base, base16, w);
+    HashPair tot;
+    HashPair run = block_exclusive_scan(acc, HashPair{0, 1}, OpHash{}, lds, &tot);
+    run = OpHash{}(partial[blockIdx.x], run);
+    const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+    for (uint32_t i = 0; i < (uint32_t)kPhItems; i++) {
+        if (b0 + i < n) {
+            run.h = addmod61(mulmod61(run.h, base), (uint64_t)((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) + 1);
+            run.p = mulmod61(run.p, base);
+            pre[b0 + i] = run;
+        }
+    }
 }
 
 struct PrefixHash {
@@ -332,13 +383,30 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
     return levels;
 }
 
-// Prefix fingerprints pre[k] = (H(T[0..k]), base^(k+1)) mod 2^61-1 by one device scan.
+// Prefix fingerprints pre[k] = (H(T[0..k]), base^(k+1)) mod 2^61-1: per-tile fingerprints of the
+// text, their exclusive scan, and the tiles' prefixes (the text is read twice, pre written once).
+static uint64_t powmod61(uint64_t b, uint64_t e)
+{
+    unsigned __int128 r = 1, x = b % kM61;
+    while (e) {
+        if (e & 1) r = (r * x) % kM61;
+        x = (x * x) % kM61;
+        e >>= 1;
+    }
+    return (uint64_t)r;
+}
+
 HashPair* prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base)
 {
     HashPair* pre = S.buf<HashPair>(B_LY_HASH, n);
-    k_hash_items<<<grid_for(n), kBlock, 0, S.stream>>>(T, n, base, pre);
+    const uint32_t tiles = ceil_div(n, (uint64_t)kPhTile);
+    HashPair* partial = S.buf<HashPair>(B_LY_SCAN0, tiles);
+    const uint64_t base16 = powmod61(base, kPhItems);
+    k_ph_reduce<<<tiles, kBlock, 0, S.stream>>>(T, n, base, base16, partial);
     MIBWT_LAUNCH_CHECK();
-    device_scan<HashPair, OpHash>(S, pre, pre, n, true, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN0);
+    device_scan<HashPair, OpHash>(S, partial, partial, tiles, false, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN1);
+    k_ph_down<<<tiles, kBlock, 0, S.stream>>>(T, n, base, base16, partial, pre);
+    MIBWT_LAUNCH_CHECK();
     return pre;
 }
 
