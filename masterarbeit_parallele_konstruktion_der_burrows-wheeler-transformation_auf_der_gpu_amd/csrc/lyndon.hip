@@ -11,8 +11,8 @@
 // sorts rules (:3129-4083) and induces the BWT over rank groups (:4114-4896).  Here:
 //
 //  1. rules    Karp-Rabin prefix fingerprints (mod 2^61-1) of the text in one device scan.
-//  2. lyndon   Lyndon array as next-smaller-suffix distances: every 256-byte block chases
-//              NSS pointers right to left; positions without a smaller suffix in the block
+//  2. lyndon   Lyndon array as next-smaller-suffix distances: every 128-byte block chases
+//              NSS pointers right to left over its text staged in LDS; positions without a smaller suffix in the block
 //              stay open and are resolved by a merge tree over neighbouring ranges, where an
 //              open position bisects the right range's prefix minima.  Suffix comparisons
 //              use the fingerprints for long common prefixes (O(log lcp) per comparison).
@@ -41,7 +41,10 @@ namespace mibwt {
 
 bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT);
 
-constexpr uint32_t kLynBlock = 256;   // characters per block of the local pass (MIBWT_LYN_BLOCK overrides)
+// Characters per block of the local pass (MIBWT_LYN_BLOCK overrides: 16..256).  Measured on the
+// 1 GiB DNA collection (local pass + merge tree): 64 -> 38.8 ms, 128 -> 32.2 ms, 256 -> 43.3 ms
+// (256 halves the resident workgroups: 34 KiB of LDS each against 18 KiB).
+constexpr uint32_t kLynBlock = 128;
 // Label tie groups of more members go to the pivot partition (linear work per round; a group of
 // equal labels settles in one round) instead of the comparator counting sort (quadratic in the
 // group size, but one pass); MIBWT_LAB_SMALL overrides it (2..256).  Ordering every position
@@ -245,28 +248,119 @@ struct SuffixCmp {
 
 constexpr uint32_t kNoNss = 0xFFFFFFFFu;
 
-// One thread per 256-byte block, right to left: NSS by pointer chasing inside the block.
-// Positions without a smaller suffix inside the block stay open (kNoNss).  Per block: the
-// open positions in ascending order (O, increasing suffixes) and the chain of prefix minima
-// from the block start (M = s, NSS[s], NSS[NSS[s]], ..., decreasing suffixes).
-__global__ void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t bs, uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
-                            uint32_t* __restrict__ oc, uint32_t* __restrict__ M, uint32_t* __restrict__ mc)
+// Local pass: one thread per block of bs <= kNssMaxBlock characters, 64 blocks per workgroup.
+// The workgroup stages its blocks' text in LDS, each block followed by the first kNssLook bytes
+// of the next one (one coalesced read of the text), and keeps each block's NSS values as 8-bit
+// offsets (0: no smaller suffix inside the block) in LDS; the neighbours compared inside a block
+// almost always differ within 16 bytes, so a comparison is a few LDS reads, and only 16 equal
+// bytes fall back to the global comparator.  Per-thread rows are padded to an odd number of
+// words so that the 64 lanes' rows start in different LDS banks.
+constexpr uint32_t kNssMaxBlock = 256;
+constexpr uint32_t kNssLook = 16;
+__host__ __device__ constexpr uint32_t nss_text_row(uint32_t bs) { return bs + kNssLook + 4; }
+__host__ __device__ constexpr uint32_t nss_off_row(uint32_t bs) { return bs + 4; }
+
+// 8 bytes of an LDS row from byte offset o (rows are word aligned and hold o + 12 bytes)
+__device__ __forceinline__ uint64_t lds_read8(const uint8_t* row, uint32_t o)
 {
-    const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= nb) return;
-    const uint32_t s = b * bs;
-    const uint32_t e = (uint32_t)(((uint64_t)s + bs < sc.n) ? s + bs : sc.n);
-    for (uint32_t i = e; i-- > s;) {
-        uint32_t j = i + 1;
-        while (j < e && !sc.less(j, i)) j = nss[j];  // kNoNss ends the walk as well
-        nss[i] = (j < e) ? j : kNoNss;
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(row + (o & ~3u));
+    const uint32_t sh = o & 3u, w0 = w[0], w1 = w[1], w2 = w[2];
+    return ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, sh) << 32) | __builtin_amdgcn_alignbyte(w1, w0, sh);
+}
+
+// One thread per block, right to left: NSS by pointer chasing inside the block.  Positions
+// without a smaller suffix inside the block stay open (kNoNss).  Per block: the open positions
+// in ascending order (O, increasing suffixes) and the chain of prefix minima from the block start
+// (M = s, NSS[s], NSS[NSS[s]], ..., decreasing suffixes).  Launched with 64 threads and
+// 64 * (nss_text_row(bs) + nss_off_row(bs)) bytes of dynamic LDS; bs is a power of two in [16, 256].
+__global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t bs, int lbs,
+                                                  uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
+                                                  uint32_t* __restrict__ oc, uint32_t* __restrict__ M,
+                                                  uint32_t* __restrict__ mc)
+{
+    extern __shared__ __align__(16) uint8_t nss_lds[];
+    const uint32_t TS = nss_text_row(bs), NS = nss_off_row(bs);
+    uint8_t* tx = nss_lds;
+    uint8_t* of = nss_lds + 64 * TS;
+    const uint32_t lane = threadIdx.x;
+    const uint64_t b0 = (uint64_t)blockIdx.x * 64;
+    const uint64_t s0 = b0 * bs;
+    // stage T[s0, s0 + 64 bs + kNssLook) in 16-byte pieces (every piece that holds text bytes
+    // lies inside the kPad zero bytes after the text)
+    const uint32_t span = 64 * bs + kNssLook;
+    for (uint32_t q = lane * 16; q < span; q += 64 * 16) {
+        uint4 v = make_uint4(0, 0, 0, 0);
+        if (s0 + q + 16 <= sc.n + kPad) v = *reinterpret_cast<const uint4*>(sc.T + s0 + q);
+        const uint32_t t = q >> lbs, k = q & (bs - 1);
+        if (t < 64) {
+            uint32_t* d = reinterpret_cast<uint32_t*>(tx + t * TS + k);
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
+        if (t >= 1 && k == 0) {  // the previous block's look-ahead
+            uint32_t* d = reinterpret_cast<uint32_t*>(tx + (t - 1) * TS + bs);
+            d[0] = v.x; d[1] = v.y; d[2] = v.z; d[3] = v.w;
+        }
     }
+    __syncthreads();
+    const uint64_t b = b0 + lane;
+    const uint8_t* mt = tx + lane * TS;
+    uint8_t* mo = of + lane * NS;
+    uint32_t len = 0;
+    if (b < nb) {
+        const uint64_t s = b * bs;
+        len = (uint32_t)((s + bs < sc.n) ? bs : sc.n - s);
+        for (uint32_t k = len; k-- > 0;) {
+            uint32_t jk = k + 1;
+            uint32_t off = 0;
+            while (jk < len) {
+                // suffix(s + jk) < suffix(s + k)?  m = comparable length = n - (s + jk)
+                const uint64_t m = sc.n - (s + jk);
+                int res = -1;  // -1: undecided within the staged bytes
+#pragma unroll
+                for (uint32_t l0 = 0; l0 < kNssLook; l0 += 8) {
+                    if (res < 0) {
+                        const uint64_t xi = lds_read8(mt, k + l0), xj = lds_read8(mt, jk + l0);
+                        const uint64_t d = xi ^ xj;
+                        const uint32_t l = l0 + (d ? (uint32_t)(__builtin_ctzll(d) >> 3) : 8u);
+                        if (l >= m) res = 1;  // suffix s + jk ends first: it is smaller
+                        else if (d) {
+                            const uint32_t sh = 8 * (l - l0);
+                            res = ((xj >> sh) & 0xFFu) < ((xi >> sh) & 0xFFu) ? 1 : 0;
+                        }
+                    }
+                }
+                if (res < 0) res = sc.less((uint32_t)(s + jk), (uint32_t)(s + k)) ? 1 : 0;
+                if (res) {
+                    off = jk - k;
+                    break;
+                }
+                const uint32_t o2 = mo[jk];
+                if (o2 == 0) break;
+                jk += o2;
+            }
+            mo[k] = (uint8_t)off;
+        }
+    }
+    __syncthreads();
+    // NSS values of the workgroup's positions, coalesced
+    const uint64_t tot = (sc.n - s0 < (uint64_t)64 * bs) ? sc.n - s0 : (uint64_t)64 * bs;
+    for (uint32_t q = lane; q < tot; q += 64) {
+        const uint32_t o = of[(q >> lbs) * NS + (q & (bs - 1))];
+        nss[s0 + q] = o ? (uint32_t)(s0 + q + o) : kNoNss;
+    }
+    if (b >= nb) return;
+    const uint32_t s = (uint32_t)(b * bs);
     uint32_t c = 0;
-    for (uint32_t i = s; i < e; i++)
-        if (nss[i] == kNoNss) O[s + c++] = i;
+    for (uint32_t k = 0; k < len; k++)
+        if (mo[k] == 0) O[s + c++] = s + k;
     oc[b] = c;
     c = 0;
-    for (uint32_t m = s; m != kNoNss; m = nss[m]) M[s + c++] = m;
+    for (uint32_t k = 0;;) {
+        M[s + c++] = s + k;
+        const uint32_t o = mo[k];
+        if (o == 0) break;
+        k += o;
+    }
     mc[b] = c;
 }
 
@@ -353,8 +447,10 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
     uint32_t bs = kLynBlock;
     if (const char* e = getenv("MIBWT_LYN_BLOCK")) {
         const int v = atoi(e);
-        if (v >= 8 && v <= 4096 && (v & (v - 1)) == 0) bs = (uint32_t)v;
+        if (v >= 16 && v <= (int)kNssMaxBlock && (v & (v - 1)) == 0) bs = (uint32_t)v;
     }
+    int lbs = 0;
+    while ((1u << lbs) < bs) lbs++;
     const uint32_t nb = ceil_div(n, bs);
     uint32_t* oa = S.buf<uint32_t>(B_LY_LIST0, (size_t)nb * bs);
     uint32_t* ob = S.buf<uint32_t>(B_LY_LIST1, (size_t)nb * bs);
@@ -365,7 +461,8 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
     uint32_t* mca = S.buf<uint32_t>(B_LY_HF, nb);
     uint32_t* mcb = S.buf<uint32_t>(B_LY_HF2, nb);
     const SuffixCmp sc{T, n, ph};
-    k_nss_local<<<ceil_div(nb, 64), 64, 0, S.stream>>>(sc, nb, bs, lam, oa, oca, ma, mca);
+    const size_t lds = (size_t)64 * (nss_text_row(bs) + nss_off_row(bs));
+    k_nss_local<<<ceil_div(nb, 64), 64, lds, S.stream>>>(sc, nb, bs, lbs, lam, oa, oca, ma, mca);
     MIBWT_LAUNCH_CHECK();
     int levels = 0;
     for (uint64_t size = 1; size < nb; size *= 2) {
