@@ -939,13 +939,13 @@ struct LabelKey {
     {
         const uint32_t l = lam[i];
         if (l <= (uint32_t)per) {
+            // the l symbols, then zero codes up to `per` (l <= per < 64, so i + l <= n)
             uint64_t key = 0, w = 0;
-            for (int k = 0; k < per; k++) {
-                if ((k & 7) == 0) w = ((uint64_t)i + k < n) ? load8(T, (uint64_t)i + k) : 0ull;
-                const uint64_t s = ((uint32_t)k < l) ? cs[(w >> (8 * (k & 7))) & 0xFF] : 0ull;
-                key = (key << bits) | s;
+            for (uint32_t k = 0; k < l; k++) {
+                if ((k & 7) == 0) w = load8(T, (uint64_t)i + k);
+                key = (key << bits) | cs[(w >> (8 * (k & 7))) & 0xFF];
             }
-            return key;
+            return key << (bits * ((uint32_t)per - l));
         }
         const uint64_t f = ph.sub(i, l) ^ ((uint64_t)l * 0x9E3779B97F4A7C15ull);
         return (1ull << 63) | (f >> 1);
@@ -1055,6 +1055,24 @@ __global__ void k_gl_merge(const uint64_t* __restrict__ UA, const uint32_t* __re
         rep[o] = RB[d];
         cnt[o] = CB[d];
     }
+}
+
+// The roots are the starts of the Lyndon factorization of T: 0, then r + lam[r] (the first factor
+// of a suffix is its longest Lyndon prefix).  One thread follows that chain for at most `limit`
+// factors and writes their number, or kNone when the text has more (then the prefix maxima of
+// i + lam[i] below find them in parallel).
+constexpr uint32_t kRootWalk = 1024;
+__global__ void k_gl_rootwalk(const uint32_t* __restrict__ lam, uint64_t n, uint32_t limit,
+                              uint32_t* __restrict__ roots, uint32_t* __restrict__ count)
+{
+    if (threadIdx.x != 0) return;
+    uint64_t k = 0;
+    uint32_t c = 0;
+    while (k < n && c < limit) {
+        roots[c++] = (uint32_t)k;
+        k += lam[k];
+    }
+    *count = (k >= n) ? c : kNone;
 }
 
 __global__ void k_gl_ends(const uint32_t* __restrict__ lam, uint64_t n, uint32_t* __restrict__ E)
@@ -1461,21 +1479,27 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     if (nA) k_grp_offsets<<<grid_for(nA), kBlock, 0, S.stream>>>(hf, gid1, nA, gofs);
     if (nB) k_grp_offsets<<<grid_for(nB), kBlock, 0, S.stream>>>(hfB, gidB, nB, gofsB);
     MIBWT_LAUNCH_CHECK();
-    // roots: prefix maxima of i + lam[i]
-    uint32_t* PM = S.buf<uint32_t>(B_LY_JMP0, n);
-    uint32_t* rpos = S.buf<uint32_t>(B_LY_JMP1, n);
-    k_gl_ends<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, PM);
+    // roots: the factorization chain, or the prefix maxima of i + lam[i] for many factors
+    uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, kRootWalk);
+    k_gl_rootwalk<<<1, 64, 0, S.stream>>>(lam, n, kRootWalk, roots, gsc + GR_S);
     MIBWT_LAUNCH_CHECK();
-    inclusive_max<uint32_t>(S, PM, PM, n);
-    k_gl_rootflag<<<grid_for(n), kBlock, 0, S.stream>>>(PM, n, rpos);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, rpos, rpos, n, gsc + GR_S);
     S.read_fast(hv, gsc, GR_COUNT_);
+    if (hv[GR_S] == kNone) {
+        uint32_t* PM = S.buf<uint32_t>(B_LY_JMP0, n);
+        uint32_t* rpos = S.buf<uint32_t>(B_LY_JMP1, n);
+        k_gl_ends<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, PM);
+        MIBWT_LAUNCH_CHECK();
+        inclusive_max<uint32_t>(S, PM, PM, n);
+        k_gl_rootflag<<<grid_for(n), kBlock, 0, S.stream>>>(PM, n, rpos);
+        MIBWT_LAUNCH_CHECK();
+        exclusive_sum<uint32_t>(S, rpos, rpos, n, gsc + GR_S);
+        S.read_fast(hv, gsc, GR_COUNT_);
+        roots = S.buf<uint32_t>(B_GR_ROOT, hv[GR_S]);
+        k_gl_roots<<<grid_for(n), kBlock, 0, S.stream>>>(PM, rpos, n, roots);
+        MIBWT_LAUNCH_CHECK();
+    }
     const uint32_t DA = hv[GR_DA], DB = hv[GR_DB], NS = hv[GR_S];
     const uint32_t D = DA + DB;
-    uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, NS);
-    k_gl_roots<<<grid_for(n), kBlock, 0, S.stream>>>(PM, rpos, n, roots);
-    MIBWT_LAUNCH_CHECK();
     uint32_t* repA = S.buf<uint32_t>(B_GL_REPA, DA);
     uint32_t* cntA = S.buf<uint32_t>(B_GL_CNTA, DA);
     uint64_t* UA = S.buf<uint64_t>(B_GL_UA, DA);
