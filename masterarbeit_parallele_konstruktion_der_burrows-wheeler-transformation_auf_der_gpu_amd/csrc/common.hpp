@@ -457,6 +457,7 @@ enum BufId : int {
     B_LY_CNT1,
     B_LY_LAM,
     B_LY_HASH,
+    B_LY_POW,
     B_LY_SCAN0,     // 4 consecutive ids for the fingerprint scan partials
     B_LY_SCAN1,
     B_LY_SCAN2,

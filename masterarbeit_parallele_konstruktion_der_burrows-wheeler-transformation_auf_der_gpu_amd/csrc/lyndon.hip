@@ -147,13 +147,17 @@ __global__ __launch_bounds__(kBlock) void k_ph_reduce(const uint8_t* __restrict_
     if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// pre[k] = (H(T[0..k]), base^(k+1)) from the tiles' exclusive prefixes
+// pre[k] = H(T[0..k]) from the tiles' exclusive prefixes.  Each thread's values go through LDS
+// (rows of kPhItems + 1 words against bank conflicts) so that the tile leaves as one coalesced
+// write.
 __global__ __launch_bounds__(kBlock) void k_ph_down(const uint8_t* __restrict__ T, uint64_t n, uint64_t base,
                                                    uint64_t base16, const HashPair* __restrict__ partial,
-                                                   HashPair* __restrict__ pre)
+                                                   uint64_t* __restrict__ pre)
 {
     __shared__ HashPair lds[kWaves];
-    const uint64_t b0 = (uint64_t)blockIdx.x * kPhTile + (uint64_t)threadIdx.x * kPhItems;
+    __shared__ uint64_t out[kBlock * (kPhItems + 1)];
+    const uint64_t t0 = (uint64_t)blockIdx.x * kPhTile;
+    const uint64_t b0 = t0 + (uint64_t)threadIdx.x * kPhItems;
     uint4 w;
     const HashPair acc = ph_run(T, n, b0, base, base16, w);
     HashPair tot;
@@ -162,26 +166,35 @@ __global__ __launch_bounds__(kBlock) void k_ph_down(const uint8_t* __restrict__ 
     const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
     for (uint32_t i = 0; i < (uint32_t)kPhItems; i++) {
-        if (b0 + i < n) {
-            run.h = addmod61(mulmod61(run.h, base), (uint64_t)((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) + 1);
-            run.p = mulmod61(run.p, base);
-            pre[b0 + i] = run;
-        }
+        run.h = addmod61(mulmod61(run.h, base), (uint64_t)((ws[i >> 2] >> (8 * (i & 3))) & 0xFFu) + 1);
+        out[threadIdx.x * (kPhItems + 1) + i] = run.h;
     }
+    __syncthreads();
+    const uint32_t valid = (n - t0 < (uint64_t)kPhTile) ? (uint32_t)(n - t0) : (uint32_t)kPhTile;
+    for (uint32_t q = threadIdx.x; q < valid; q += kBlock) pre[t0 + q] = out[(q / kPhItems) * (kPhItems + 1) + q % kPhItems];
 }
 
+// Powers of the base in two tables of 2^16 entries: base^i and base^(2^16 i), so base^len for
+// any 32-bit len is one entry or the product of two (1 MiB in all, cache resident).
+constexpr uint32_t kPowBits = 16;
+constexpr uint32_t kPowSize = 1u << kPowBits;
+
 struct PrefixHash {
-    const HashPair* pre;  // pre[k] = (H[k+1], base^(k+1))
+    const uint64_t* pre;  // pre[k] = H(T[0..k])
+    const uint64_t* pw;   // pw[i] = base^i, pw[kPowSize + j] = base^(kPowSize j)
     uint64_t mask;        // bits of the fingerprints compared (all 61 but in collision tests)
+    __device__ __forceinline__ uint64_t power(uint32_t len) const
+    {
+        const uint64_t lo = pw[len & (kPowSize - 1)];
+        return (len < kPowSize) ? lo : mulmod61(lo, pw[kPowSize + (len >> kPowBits)]);
+    }
     // fingerprint of T[a..a+len)
     __device__ __forceinline__ uint64_t sub(uint32_t a, uint32_t len) const
     {
         if (len == 0) return 0;
-        const uint64_t hb = pre[a + len - 1].h;
+        const uint64_t hb = pre[a + len - 1];
         if (a == 0) return hb & mask;
-        const uint64_t ha = pre[a - 1].h;
-        const uint64_t pw = pre[len - 1].p;
-        const uint64_t x = mulmod61(ha, pw);
+        const uint64_t x = mulmod61(pre[a - 1], power(len));
         return (hb >= x ? hb - x : hb + kM61 - x) & mask;
     }
 };
@@ -480,8 +493,27 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
     return levels;
 }
 
-// Prefix fingerprints pre[k] = (H(T[0..k]), base^(k+1)) mod 2^61-1: per-tile fingerprints of the
-// text, their exclusive scan, and the tiles' prefixes (the text is read twice, pre written once).
+// pw[i] = b^i and pw[kPowSize + i] = b^(kPowSize i), i < kPowSize (b16 = b^kPowSize)
+__global__ void k_ph_pow(uint64_t b, uint64_t b16, uint64_t* __restrict__ pw)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= kPowSize) return;
+    uint64_t r0 = 1, r1 = 1, x0 = b, x1 = b16;
+    for (uint32_t e = i; e; e >>= 1) {
+        if (e & 1) {
+            r0 = mulmod61(r0, x0);
+            r1 = mulmod61(r1, x1);
+        }
+        x0 = mulmod61(x0, x0);
+        x1 = mulmod61(x1, x1);
+    }
+    pw[i] = r0;
+    pw[kPowSize + i] = r1;
+}
+
+// Prefix fingerprints pre[k] = H(T[0..k]) mod 2^61-1: per-tile fingerprints of the text, their
+// exclusive scan, and the tiles' prefixes (the text is read twice, pre written once); the power
+// tables of the base beside them.
 static uint64_t powmod61(uint64_t b, uint64_t e)
 {
     unsigned __int128 r = 1, x = b % kM61;
@@ -493,9 +525,12 @@ static uint64_t powmod61(uint64_t b, uint64_t e)
     return (uint64_t)r;
 }
 
-HashPair* prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base)
+PrefixHash prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base, uint64_t hmask)
 {
-    HashPair* pre = S.buf<HashPair>(B_LY_HASH, n);
+    uint64_t* pre = S.buf<uint64_t>(B_LY_HASH, n);
+    uint64_t* pw = S.buf<uint64_t>(B_LY_POW, 2 * kPowSize);
+    k_ph_pow<<<kPowSize / kBlock, kBlock, 0, S.stream>>>(base, powmod61(base, kPowSize), pw);
+    MIBWT_LAUNCH_CHECK();
     const uint32_t tiles = ceil_div(n, (uint64_t)kPhTile);
     HashPair* partial = S.buf<HashPair>(B_LY_SCAN0, tiles);
     const uint64_t base16 = powmod61(base, kPhItems);
@@ -504,7 +539,7 @@ HashPair* prefix_hashes(Storage& S, const uint8_t* T, uint64_t n, uint64_t base)
     device_scan<HashPair, OpHash>(S, partial, partial, tiles, false, HashPair{0, 1}, OpHash{}, nullptr, B_LY_SCAN1);
     k_ph_down<<<tiles, kBlock, 0, S.stream>>>(T, n, base, base16, partial, pre);
     MIBWT_LAUNCH_CHECK();
-    return pre;
+    return PrefixHash{pre, pw, hmask};
 }
 
 // ------------------------------------------------------------------- 3. label order
@@ -1867,9 +1902,8 @@ static uint32_t lyndon_attempt(Storage& S, const uint8_t* T, uint64_t n, uint64_
 {
     // ---- 1. prefix fingerprints
     int ph = S.phase_begin("rules");
-    HashPair* pre = prefix_hashes(S, T, n, base);
-    const PrefixHash phh{pre, hmask};
-    S.phase_end(ph, (double)n * (1 + 16 * 4));
+    const PrefixHash phh = prefix_hashes(S, T, n, base, hmask);
+    S.phase_end(ph, (double)n * (2 + 8));
 
     // ---- 2. Lyndon array
     ph = S.phase_begin("lyndon");
@@ -1957,8 +1991,7 @@ int64_t lyndon_array_run(Storage& S, const uint8_t* src, int64_t n64, uint32_t* 
     const uint64_t n = (uint64_t)n64;
     uint8_t* T = stage_text(S, src, true, n);
     uint32_t* lam = S.buf<uint32_t>(B_LY_LAM, n);
-    HashPair* pre = prefix_hashes(S, T, n, 0x1F3D5B79A2C4E681ull % kM61);
-    const int levels = lyndon_array(S, T, n, PrefixHash{pre, ~0ull}, lam);
+    const int levels = lyndon_array(S, T, n, prefix_hashes(S, T, n, 0x1F3D5B79A2C4E681ull % kM61, ~0ull), lam);
     MIBWT_HIP(hipMemcpyAsync(lam_host, lam, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, S.stream));
     MIBWT_HIP(hipStreamSynchronize(S.stream));
     return levels;
