@@ -961,18 +961,31 @@ enum GrSlot : int {
     GR_COUNT_ = 16
 };
 
-// Identity of the label at position i as one 64-bit key: labels of at most `per` symbols by
-// their dense codes (exact; top bit clear), longer ones by their fingerprint mixed with the
-// length (top bit set).  Equal labels always get equal keys.
+// Identity of the label at position i as one 64-bit key.  Labels of at most per32 symbols: the
+// number 1 d_0 d_1 .. d_{l-1} in base `radix` (d_k = dense symbol code - 2, radix = max(sigma, 2)),
+// below 2^32, one number per string.  Labels of at most `per` symbols: their codes (0 past the
+// end) packed `bits` bits each from the top, exact; the first code is at least 2, so such a key is
+// at least 2^(bits (per - 1) + 1) >= 2^33.  Longer labels: their fingerprint mixed with the length,
+// top bit set.  Equal labels always get equal keys, and the three kinds never meet.
 struct LabelKey {
     const uint8_t* T;
     uint64_t n;
     const uint32_t* lam;
     PrefixHash ph;
     int bits, per;
+    uint32_t per32, radix;
     __device__ __forceinline__ uint64_t operator()(uint32_t i, const uint32_t* cs) const
     {
         const uint32_t l = lam[i];
+        if (l <= per32) {
+            uint32_t key = 1;
+            uint64_t w = 0;
+            for (uint32_t k = 0; k < l; k++) {
+                if ((k & 7) == 0) w = load8(T, (uint64_t)i + k);
+                key = key * radix + (cs[(w >> (8 * (k & 7))) & 0xFF] - 2);
+            }
+            return key;
+        }
         if (l <= (uint32_t)per) {
             // the l symbols, then zero codes up to `per` (l <= per < 64, so i + l <= n)
             uint64_t key = 0, w = 0;
@@ -1004,16 +1017,16 @@ __device__ __forceinline__ uint32_t key_lookup(const uint64_t* __restrict__ U, u
     return lo;
 }
 
-// Label keys whose low word is zero (labels of at most `per32` symbols, exact in the high word)
-// are sorted as 32-bit keys, the others as 64-bit keys: fa[i] = 1 for the former.
+// Keys of labels of at most `per32` symbols are below 2^32 and sorted as 32-bit keys, the others
+// as 64-bit keys: fa[i] = 1 for the former.
 __global__ void k_gl_short(const uint32_t* __restrict__ lam, uint64_t n, uint32_t per32, uint32_t* __restrict__ fa)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) fa[i] = (lam[i] <= per32) ? 1u : 0u;
 }
 
-// the key of every position into the short list (high word; fa = exclusive scan of the flags)
-// or the long list (all 64 bits), with the position as value
+// the key of every position into the short list (32 bits; fa = exclusive scan of the flags) or
+// the long list (64 bits), with the position as value
 __global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32_t per32,
                           const uint32_t* __restrict__ fa, uint32_t* __restrict__ KA, uint32_t* __restrict__ VA,
                           uint64_t* __restrict__ KB, uint32_t* __restrict__ VB)
@@ -1026,7 +1039,7 @@ __global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32
     const uint64_t key = lk((uint32_t)i, cs);
     const uint32_t a = fa[i];
     if (lk.lam[i] <= per32) {
-        KA[a] = (uint32_t)(key >> 32);
+        KA[a] = (uint32_t)key;
         VA[a] = (uint32_t)i;
     } else {
         KB[i - a] = key;
@@ -1041,7 +1054,6 @@ __global__ void k_gl_heads32(const uint32_t* __restrict__ K, uint32_t m, uint32_
 }
 
 // per distinct label d (the runs of equal sorted keys): representative, count, 64-bit key
-// (a short key is the high word of its 64-bit key)
 template <typename K>
 __global__ void k_gl_dedup(const K* __restrict__ Ks, const uint32_t* __restrict__ V, const uint32_t* __restrict__ gofs,
                            uint32_t D, uint32_t* __restrict__ rep, uint32_t* __restrict__ cnt,
@@ -1052,7 +1064,7 @@ __global__ void k_gl_dedup(const K* __restrict__ Ks, const uint32_t* __restrict_
     const uint32_t j = gofs[d];
     rep[d] = V[j];
     cnt[d] = gofs[d + 1] - j;
-    U[d] = (sizeof(K) == 4) ? ((uint64_t)Ks[j] << 32) : (uint64_t)Ks[j];
+    U[d] = (uint64_t)Ks[j];
 }
 
 // number of keys in the sorted X[0..m) below x
@@ -1067,8 +1079,8 @@ __device__ __forceinline__ uint32_t count_below64(const uint64_t* __restrict__ X
     return lo;
 }
 
-// the two sorted lists of distinct labels (their keys never coincide: a short key's low word is
-// zero, a long label has more symbols or the fingerprint flag) merged into one sorted list
+// the two sorted lists of distinct labels (their keys never coincide: short keys are below 2^32,
+// the others at least 2^33) merged into one sorted list
 __global__ void k_gl_merge(const uint64_t* __restrict__ UA, const uint32_t* __restrict__ RA,
                            const uint32_t* __restrict__ CA, uint32_t DA, const uint64_t* __restrict__ UB,
                            const uint32_t* __restrict__ RB, const uint32_t* __restrict__ CB, uint32_t DB,
@@ -1478,9 +1490,19 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     int ph = S.phase_begin("labels");
     const int bits = bits_for((uint64_t)sigma + 1);
     const int per = 63 / bits;
-    const LabelKey lk{T, n, lam, phh, bits, per};
-    // symbols wholly inside the key's high word
-    const uint32_t per32 = (uint32_t)std::max(0, per - (32 + bits - 1) / bits);
+    // short labels: keys below 2^24 (three radix passes) when that holds 8 symbols, else below 2^32
+    const uint32_t radix = std::max<uint32_t>(sigma, 2);
+    auto fit = [&](uint64_t lim) {
+        uint32_t L = 0;
+        for (uint64_t v = 2ull * radix; v <= lim; v *= radix) L++;
+        return L;
+    };
+    uint32_t per32 = fit(1ull << 24);
+    if (per32 < 8) per32 = fit(1ull << 32);
+    uint64_t amax = 2;
+    for (uint32_t k = 0; k < per32; k++) amax *= radix;
+    const int bitsA = bits_for(amax - 1);  // keys below 2 radix^per32
+    const LabelKey lk{T, n, lam, phh, bits, per, per32, radix};
     uint32_t* fa = S.buf<uint32_t>(B_GL_FA, n);
     k_gl_short<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, per32, fa);
     MIBWT_LAUNCH_CHECK();
@@ -1497,7 +1519,13 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, nB);
     k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, K0, V0);
     MIBWT_LAUNCH_CHECK();
-    radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, 31);
+    int whereA = 0;  // an odd number of passes leaves the short list in buffer 1
+    radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, bitsA,
+                                   static_cast<const EmitPair<uint32_t, uint32_t>*>(nullptr), &whereA);
+    if (whereA) {
+        std::swap(KA0, KA1);
+        std::swap(VA0, VA1);
+    }
     radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, nB, 0, 64);
     uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
     uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
