@@ -67,12 +67,13 @@ static __global__ void k_grp_offsets(const uint32_t* __restrict__ hf, const uint
 }
 
 // Element i of the initial order: rank and SA slot, whether its group stays active, and the
-// group size (the first compaction reads the rest of the group record from gid1 / gofs).
+// group size (the first compaction reads the rest of the group record from gid1 / gofs).  With
+// `pairs` the rank is not scattered but written as pairs[i] = x << 32 | rank, in order.
 template <typename Policy>
 __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                            const uint32_t* __restrict__ gofs, uint32_t m, uint32_t* __restrict__ rank,
-                           uint32_t* __restrict__ SA, uint32_t* __restrict__ keep, uint32_t* __restrict__ NGz,
-                           uint32_t* __restrict__ NR, Policy pol)
+                           uint64_t* __restrict__ pairs, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                           uint32_t* __restrict__ NGz, uint32_t* __restrict__ NR, Policy pol)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m) return;
@@ -89,10 +90,29 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
         if (!kp) pol.place(SA, i, x, s);
     }
     if constexpr (Policy::kCarry) NR[i] = r;
-    rank[x] = s + 1;
+    if (pairs) pairs[i] = ((uint64_t)x << 32) | (s + 1);
+    else rank[x] = s + 1;
     keep[i] = kp ? 1u : 0u;
     NGz[i] = z;
 }
+
+// rank[x] = r for the (x << 32 | r) pairs, partitioned by the high bits of x so that the writes of
+// neighbouring blocks fall into one window of the rank array
+static __global__ void k_dbl_rank_pairs(const uint64_t* __restrict__ P, uint32_t m, uint32_t* __restrict__ rank)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint64_t p = P[i];
+    rank[(uint32_t)(p >> 32)] = (uint32_t)p;
+}
+
+// The first placement writes the rank of every element, a permutation of [0, m).  Scattered
+// directly, each 4-byte write costs a random access to HBM once the rank array outgrows the
+// last-level cache; from this many elements on, the (position, rank) pairs are first partitioned
+// by the top 8 bits of the position (one radix pass) and then written window by window.
+// Measured with a random permutation on MI355X: 369 M elements 14.6 ms direct against 1.9 + 4.7 ms,
+// 64 M elements 0.88 ms against 0.35 + 0.49 ms.
+constexpr uint32_t kRankPartMin = 96u << 20;
 
 // ---- one refinement round over the m active elements (groups contiguous)
 
@@ -591,9 +611,11 @@ struct DoublingResult {
 // compaction; one host read-back per round (active and large counts).
 // `max_rounds` bounds the loop: a caller whose keys can be inconsistent (elements that never
 // separate) gets MIBWT_HASH_COLLISION instead of an endless refinement.
+// K1 (optional, N entries) is scratch for the partitioned first placement.
 template <typename Policy, typename Advance>
 DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V1, uint32_t N, Policy& pol,
-                             Advance advance, uint32_t* rank, uint32_t* SA, int max_rounds = 1 << 30)
+                             Advance advance, uint32_t* rank, uint32_t* SA, int max_rounds = 1 << 30,
+                             uint64_t* K1 = nullptr)
 {
     DoublingResult res;
     uint32_t* V = S.buf<uint32_t>(B_ACT_X2, N);
@@ -627,9 +649,21 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         inclusive_sum<uint32_t>(S, hf, gid1, N);
         k_grp_offsets<<<grid_for(N), kBlock, 0, S.stream>>>(hf, gid1, N, gofs);
         MIBWT_LAUNCH_CHECK();
-        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, SA, keep, NGz, NR, pol);
+        // the keys are dead once the heads are flagged: K0 takes the (position, rank) pairs
+        const bool part = K1 && N >= kRankPartMin;
+        k_dbl_init<Policy><<<grid_for(N), kBlock, 0, S.stream>>>(V0, gid1, gofs, N, rank, part ? K0 : nullptr, SA,
+                                                                  keep, NGz, NR, pol);
         MIBWT_LAUNCH_CHECK();
         res.bytes += (double)N * (8 * 2 + 4 * 9);
+        if (part) {
+            const int xb = bits_for((uint64_t)N - 1);
+            int where = 0;
+            radix_sort<uint64_t, uint32_t>(S, K0, K1, nullptr, nullptr, N, 32 + std::max(0, xb - kRadixBits), 32 + xb,
+                                           static_cast<const EmitPair<uint64_t, uint32_t>*>(nullptr), &where);
+            k_dbl_rank_pairs<<<grid_for(N), kBlock, 0, S.stream>>>(where ? K1 : K0, N, rank);
+            MIBWT_LAUNCH_CHECK();
+            res.bytes += (double)N * (8 * 3 + 4);
+        }
     }
     // A round normally starts with the compaction of the kept sub-groups.  For the string
     // families, while most slots stay active (an estimated kLazyActive of them), the compaction is

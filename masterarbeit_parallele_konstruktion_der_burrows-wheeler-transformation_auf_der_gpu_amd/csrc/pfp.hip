@@ -825,7 +825,8 @@ DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t*
     pol.h = h0;
     pol.place_fn = place;
     DoublingResult r =
-        refine_groups(S, K0, V0, V1, N, pol, [](Pol& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA);
+        refine_groups(S, K0, V0, V1, N, pol, [](Pol& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA,
+                      1 << 30, K1);
     r.bytes += (double)N * 24 * ((key_bits + 7) / 8);
     return r;
 }
