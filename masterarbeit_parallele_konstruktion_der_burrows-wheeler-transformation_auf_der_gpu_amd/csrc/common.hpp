@@ -423,14 +423,16 @@ enum BufId : int {
     B_E_DELTAC,
     B_UN_REC,
     B_UN_NXT0,
-    B_UN_NXT1,
     B_UN_W,
     B_UN_D0,
-    B_UN_D1,
     B_UN_IO,
     B_UN_AUX,
     B_UN_STASH,     // the first kUnStash bytes of every splitter's walk
     B_UN_ROW,       // the row each longer walk reached after kUnStash steps
+    B_UN_RN0,       // rulers of the splitter list: next ruler, rows to it (two of each)
+    B_UN_RN1,
+    B_UN_RD0,
+    B_UN_RD1,
     B_SCALARS,
     B_STAGE,
     B_DB_KEY2,

@@ -10,9 +10,12 @@
 //     pass's emitter stores rec[r] = LF(r) | B[r] << 32, so a walk step is one 8-byte load.
 //  2. The LF cycle is ranked by sampling: every kUnSample-th row is a splitter; each splitter
 //     walks to the next splitter (or the '$' row) and records the number of rows and the next
-//     splitter; the splitter list is ranked by pointer jumping (suffix sums, a fixed number of
-//     rounds); a second walk writes the text.  An input that is not a BWT with this primary
-//     index (LF cycle from row 0 shorter than n) is reported, not written.
+//     splitter.  The splitter list is ranked through a ruling set: every kUnRule-th splitter
+//     is a ruler that follows the list to the next ruler, the ruler list is ranked by pointer
+//     jumping (suffix sums, a fixed number of rounds), and the rulers walk their stretch again
+//     to hand every splitter its distance to the end; a second walk writes the text.  An input
+//     that is not a BWT with this primary index (LF cycle from row 0 shorter than n) is
+//     reported, not written.
 #include "prims.hpp"
 
 namespace mibwt {
@@ -21,6 +24,7 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
                   uint32_t* dI, int64_t rate, bool own_phases = true);
 
 constexpr uint32_t kUnSample = 64;
+constexpr uint32_t kUnRule = 64;  // splitters per ruler
 constexpr uint32_t kUnEnd = 0xFFFFFFFFu;
 constexpr uint32_t kUnStash = 256;  // bytes kept per walk (a walk exceeds it with probability ~e^-4)
 
@@ -103,6 +107,47 @@ __global__ void k_un_jump(const uint32_t* __restrict__ nxt, const uint32_t* __re
     }
     d2[s] = d[s] + d[t];
     nxt2[s] = nxt[t];
+}
+
+// Ruler q (splitter q kUnRule) follows the splitter list to the next ruler or the end: the rows
+// on the way (its own walk's included) and that ruler (kUnEnd: the end).  A stretch longer than
+// the list (a cycle without a ruler: not a BWT) sets *bad.
+__global__ void k_un_rule1(const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ w, uint32_t Sn, uint32_t Rn,
+                           uint32_t* __restrict__ rnext, uint32_t* __restrict__ rsum, uint32_t* __restrict__ bad)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Rn) return;
+    const uint32_t s = q * kUnRule;
+    uint32_t sum = w[s], t = nxt[s], steps = 0;
+    while (t != kUnEnd && t % kUnRule != 0) {
+        if (++steps > Sn) {
+            atomicOr(bad, 1u);
+            break;
+        }
+        sum += w[t];
+        t = nxt[t];
+    }
+    rnext[q] = (t == kUnEnd) ? kUnEnd : t / kUnRule;
+    rsum[q] = sum;
+}
+
+// Ruler q, with its rows to the end rdist[q], walks its stretch again: dist(s) = rows from
+// splitter s to the end, its own included.
+__global__ void k_un_rule2(const uint32_t* __restrict__ nxt, const uint32_t* __restrict__ w, uint32_t Sn, uint32_t Rn,
+                           const uint32_t* __restrict__ rdist, uint32_t* __restrict__ dist)
+{
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= Rn) return;
+    const uint32_t s = q * kUnRule;
+    uint32_t acc = rdist[q];
+    dist[s] = acc;
+    acc -= w[s];
+    uint32_t t = nxt[s], steps = 0;
+    while (t != kUnEnd && t % kUnRule != 0 && ++steps <= Sn) {
+        dist[t] = acc;
+        acc -= w[t];
+        t = nxt[t];
+    }
 }
 
 // second walk: splitter s starts at step k = n - dist(s) (dist = rows from s to the end); the
@@ -237,32 +282,39 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     ph = phase_begin("rank");
     const uint32_t Sn = (uint32_t)ceil_div(m, (uint64_t)kUnSample);
     uint32_t* nxt0 = S.buf<uint32_t>(B_UN_NXT0, Sn);
-    uint32_t* nxt1 = S.buf<uint32_t>(B_UN_NXT1, Sn);
     uint32_t* w = S.buf<uint32_t>(B_UN_W, Sn);
     uint32_t* d0 = S.buf<uint32_t>(B_UN_D0, Sn);
-    uint32_t* d1 = S.buf<uint32_t>(B_UN_D1, Sn);
     fill<uint32_t>(S, sc + SC_COLLISION, 1, 0u);
     // the text (not the sampled rows) is written from the bytes kept by the first walk
     uint8_t* stash = dI ? nullptr : S.buf<uint8_t>(B_UN_STASH, (size_t)Sn * kUnStash);
     uint32_t* rowk = dI ? nullptr : S.buf<uint32_t>(B_UN_ROW, Sn);
     k_un_walk1<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, m, index, Sn, nxt0, w, sc + SC_COLLISION, stash, rowk);
     MIBWT_LAUNCH_CHECK();
-    MIBWT_HIP(hipMemcpyAsync(d0, w, sizeof(uint32_t) * Sn, hipMemcpyDeviceToDevice, S.stream));
-    for (uint64_t span = 1; span < Sn; span *= 2) {
-        k_un_jump<<<grid_for(Sn), kBlock, 0, S.stream>>>(nxt0, d0, Sn, nxt1, d1);
+    // ruling set over the splitter list (ruler 0 is splitter 0, the start of the cycle)
+    const uint32_t Rn = (uint32_t)ceil_div((uint64_t)Sn, (uint64_t)kUnRule);
+    uint32_t* rn0 = S.buf<uint32_t>(B_UN_RN0, Rn);
+    uint32_t* rn1 = S.buf<uint32_t>(B_UN_RN1, Rn);
+    uint32_t* rd0 = S.buf<uint32_t>(B_UN_RD0, Rn);
+    uint32_t* rd1 = S.buf<uint32_t>(B_UN_RD1, Rn);
+    k_un_rule1<<<grid_for(Rn), kBlock, 0, S.stream>>>(nxt0, w, Sn, Rn, rn0, rd0, sc + SC_COLLISION);
+    MIBWT_LAUNCH_CHECK();
+    for (uint64_t span = 1; span < Rn; span *= 2) {
+        k_un_jump<<<grid_for(Rn), kBlock, 0, S.stream>>>(rn0, rd0, Rn, rn1, rd1);
         MIBWT_LAUNCH_CHECK();
-        std::swap(nxt0, nxt1);
-        std::swap(d0, d1);
+        std::swap(rn0, rn1);
+        std::swap(rd0, rd1);
     }
     // the walk from row 0 must reach the '$' row after exactly n byte rows
     uint32_t hv[3];
-    MIBWT_HIP(hipMemcpyAsync(sc + SC_TOTAL, d0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
-    MIBWT_HIP(hipMemcpyAsync(sc + SC_GROUPS, nxt0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(sc + SC_TOTAL, rd0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
+    MIBWT_HIP(hipMemcpyAsync(sc + SC_GROUPS, rn0, sizeof(uint32_t), hipMemcpyDeviceToDevice, S.stream));
     S.read(hv, sc + SC_COLLISION, 1);
     S.read(hv + 1, sc + SC_TOTAL, 1);
     S.read(hv + 2, sc + SC_GROUPS, 1);
     MIBWT_REQUIRE(hv[0] == 0 && hv[1] == (uint32_t)n && hv[2] == kUnEnd, LIBCUBWT_BAD_PARAMETER,
                   "not a BWT with this primary index");
+    k_un_rule2<<<grid_for(Rn), kBlock, 0, S.stream>>>(nxt0, w, Sn, Rn, rd0, d0);
+    MIBWT_LAUNCH_CHECK();
     phase_end(ph, 16.0 * m);
 
     ph = phase_begin("text");
