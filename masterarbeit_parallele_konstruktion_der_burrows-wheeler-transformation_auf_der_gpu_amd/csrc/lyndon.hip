@@ -322,10 +322,14 @@ __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uin
     if (b < nb) {
         const uint64_t s = b * bs;
         len = (uint32_t)((s + bs < sc.n) ? bs : sc.n - s);
-        for (uint32_t k = len; k-- > 0;) {
-            uint32_t jk = k + 1;
+        // One flat loop of single steps (a comparison, or the end of a walk) over the whole block,
+        // so that a lane whose position needs a long walk does not hold the others back at
+        // every position: the lanes' total step counts are nearly equal.
+        uint32_t k = len - 1, jk = len;
+        for (;;) {
+            bool done = jk >= len;  // no smaller suffix inside the block
             uint32_t off = 0;
-            while (jk < len) {
+            if (!done) {
                 // suffix(s + jk) < suffix(s + k)?  m = comparable length = n - (s + jk)
                 const uint64_t m = sc.n - (s + jk);
                 int res = -1;  // -1: undecided within the staged bytes
@@ -345,13 +349,19 @@ __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uin
                 if (res < 0) res = sc.less((uint32_t)(s + jk), (uint32_t)(s + k)) ? 1 : 0;
                 if (res) {
                     off = jk - k;
-                    break;
+                    done = true;
+                } else {
+                    const uint32_t o2 = mo[jk];
+                    if (o2 == 0) done = true;
+                    else jk += o2;
                 }
-                const uint32_t o2 = mo[jk];
-                if (o2 == 0) break;
-                jk += o2;
             }
-            mo[k] = (uint8_t)off;
+            if (done) {
+                mo[k] = (uint8_t)off;
+                if (k == 0) break;
+                k--;
+                jk = k + 1;
+            }
         }
     }
     __syncthreads();
@@ -377,10 +387,13 @@ __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uin
     mc[b] = c;
 }
 
-// One wavefront per pair of neighbouring ranges L, R of `size` blocks.  An open position i
-// of L finds its NSS among R's prefix minima: they decrease, so the first one below suffix(i)
-// is found by bisection.  The merged range keeps L's unresolved positions followed by R's
-// open ones, and L's prefix minima followed by the prefix minima of R below min(L).
+// G lanes (8 or 64) per pair of neighbouring ranges L, R of `size` blocks, 64 / G pairs per
+// wavefront.  An open position i of L finds its NSS among R's prefix minima: they decrease, so the
+// first one below suffix(i) is found by bisection.  The merged range keeps L's unresolved
+// positions followed by R's open ones, and L's prefix minima followed by the prefix minima of R
+// below min(L).  The lists are short on most texts (a handful of entries per range), so the
+// many pairs of the low levels share wavefronts.
+template <int G>
 __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uint32_t bs, uint32_t size,
                                                   uint32_t* __restrict__ nss, const uint32_t* __restrict__ Oin,
                                                   const uint32_t* __restrict__ ocin, const uint32_t* __restrict__ Min,
@@ -388,8 +401,10 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
                                                   uint32_t* __restrict__ ocout, uint32_t* __restrict__ Mout,
                                                   uint32_t* __restrict__ mcout)
 {
-    const int lane = threadIdx.x;
-    const uint64_t a = (uint64_t)blockIdx.x * 2 * size;
+    static_assert(G == 8 || G == 64, "lanes per pair");
+    const int lane = threadIdx.x & (G - 1);
+    const int grp = threadIdx.x / G;
+    const uint64_t a = ((uint64_t)blockIdx.x * (64 / G) + grp) * 2 * size;
     if (a >= nb) return;
     const uint64_t rb = a + size;
     const uint64_t sL = a * bs;
@@ -397,8 +412,8 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
     const uint32_t* OL = Oin + sL;
     const uint32_t* ML = Min + sL;
     if (rb >= nb) {
-        for (uint32_t q = lane; q < oL; q += 64) Oout[sL + q] = OL[q];
-        for (uint32_t q = lane; q < mL; q += 64) Mout[sL + q] = ML[q];
+        for (uint32_t q = lane; q < oL; q += G) Oout[sL + q] = OL[q];
+        for (uint32_t q = lane; q < mL; q += G) Mout[sL + q] = ML[q];
         if (lane == 0) {
             ocout[a] = oL;
             mcout[a] = mL;
@@ -419,9 +434,11 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
         }
         return lo;
     };
-    // L's open positions: resolve or keep (ordered compaction)
+    // L's open positions: resolve or keep (ordered compaction within the group's lanes)
+    const uint32_t gshift = (uint32_t)(grp * G);
+    const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gshift);
     uint32_t kept = 0;
-    for (uint32_t q0 = 0; q0 < oL; q0 += 64) {
+    for (uint32_t q0 = 0; q0 < oL; q0 += G) {
         const uint32_t q = q0 + lane;
         bool keep = false;
         if (q < oL) {
@@ -430,15 +447,15 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
             if (t < mR) nss[i] = MR[t];
             else keep = true;
         }
-        const uint64_t bal = __ballot(keep);
+        const uint64_t bal = __ballot(keep) & gmask;
         if (keep) Oout[sL + kept + (uint32_t)__popcll(bal & lanemask_lt())] = OL[q];
         kept += (uint32_t)__popcll(bal);
     }
-    for (uint32_t q = lane; q < oR; q += 64) Oout[sL + kept + q] = OR[q];
+    for (uint32_t q = lane; q < oR; q += G) Oout[sL + kept + q] = OR[q];
     // prefix minima: all of L's, then R's below min(L) = the last of L's
-    const uint32_t tstar = first_below(ML[mL - 1]);  // every lane computes the same value
-    for (uint32_t q = lane; q < mL; q += 64) Mout[sL + q] = ML[q];
-    for (uint32_t q = tstar + lane; q < mR; q += 64) Mout[sL + mL + (q - tstar)] = MR[q];
+    const uint32_t tstar = first_below(ML[mL - 1]);  // every lane of the group computes the same value
+    for (uint32_t q = lane; q < mL; q += G) Mout[sL + q] = ML[q];
+    for (uint32_t q = tstar + lane; q < mR; q += G) Mout[sL + mL + (q - tstar)] = MR[q];
     if (lane == 0) {
         ocout[a] = kept + oR;
         mcout[a] = mL + (mR - tstar);
@@ -480,7 +497,13 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
     int levels = 0;
     for (uint64_t size = 1; size < nb; size *= 2) {
         const uint32_t pairs = (uint32_t)((nb + 2 * size - 1) / (2 * size));
-        k_nss_merge<<<pairs, 64, 0, S.stream>>>(sc, nb, bs, (uint32_t)size, lam, oa, oca, ma, mca, ob, ocb, mb, mcb);
+        // eight lanes per pair while the pairs are many, a wavefront per pair above
+        if (pairs >= (1u << 16))
+            k_nss_merge<8><<<ceil_div(pairs, 8), 64, 0, S.stream>>>(sc, nb, bs, (uint32_t)size, lam, oa, oca, ma, mca,
+                                                                    ob, ocb, mb, mcb);
+        else
+            k_nss_merge<64><<<pairs, 64, 0, S.stream>>>(sc, nb, bs, (uint32_t)size, lam, oa, oca, ma, mca, ob, ocb,
+                                                        mb, mcb);
         MIBWT_LAUNCH_CHECK();
         std::swap(oa, ob);
         std::swap(oca, ocb);
