@@ -174,23 +174,25 @@ __global__ void k_un_walk2(const uint64_t* __restrict__ rec, const uint32_t* __r
     }
 }
 
-// The kept bytes of walk s (one wavefront per walk, four bytes per lane) to their places in the
-// text: step i of the walk is step k = n - dist(s) + i of the whole walk, preceded by T[n-1-k].
+// The kept bytes of the walks to their places in the text: step i of walk s is step
+// k = n - dist(s) + i of the whole walk, preceded by T[n-1-k].  A wavefront copies kUnEmitWalks
+// walks one after the other, a byte per lane (a walk keeps 64 bytes on average).
+constexpr uint32_t kUnEmitWalks = 16;
 __global__ __launch_bounds__(256) void k_un_emit(const uint8_t* __restrict__ stash, const uint32_t* __restrict__ w,
                                                  const uint32_t* __restrict__ dist, uint64_t n, uint32_t S,
                                                  uint8_t* __restrict__ U)
 {
-    const uint32_t s = blockIdx.x * 4 + (threadIdx.x >> 6);
     const uint32_t lane = threadIdx.x & 63;
-    if (s >= S) return;
-    const uint32_t ws = w[s];
-    if (ws == 0) return;
-    const uint32_t cnt = ws < kUnStash ? ws : kUnStash;
-    const uint64_t k0 = n - dist[s];
-    const uint8_t* st = stash + (uint64_t)s * kUnStash;
-    for (uint32_t i = lane; i < cnt; i += 64) {
-        const uint64_t k = k0 + i;
-        if (k < n) U[n - 1 - k] = st[i];
+    const uint32_t s0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * kUnEmitWalks;
+    for (uint32_t s = s0; s < s0 + kUnEmitWalks && s < S; s++) {
+        const uint32_t ws = w[s];
+        const uint32_t cnt = ws < kUnStash ? ws : kUnStash;
+        const uint64_t k0 = n - dist[s];
+        const uint8_t* st = stash + (uint64_t)s * kUnStash;
+        for (uint32_t i = lane; i < cnt; i += 64) {
+            const uint64_t k = k0 + i;
+            if (k < n) U[n - 1 - k] = st[i];
+        }
     }
 }
 
@@ -322,7 +324,7 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     if (dI) {
         k_un_walk2<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, n, Sn, dU, dI, (uint64_t)rate);
     } else {
-        k_un_emit<<<ceil_div(Sn, 4), 256, 0, S.stream>>>(stash, w, d0, n, Sn, dU);
+        k_un_emit<<<ceil_div(Sn, 4 * kUnEmitWalks), 256, 0, S.stream>>>(stash, w, d0, n, Sn, dU);
         MIBWT_LAUNCH_CHECK();
         k_un_tail<<<grid_for(Sn), kBlock, 0, S.stream>>>(rec, w, d0, rowk, n, Sn, dU);
     }
