@@ -1041,11 +1041,40 @@ __device__ __forceinline__ uint32_t key_lookup(const uint64_t* __restrict__ U, u
 }
 
 // Keys of labels of at most `per32` symbols are below 2^32 and sorted as 32-bit keys, the others
-// as 64-bit keys: fa[i] = 1 for the former.
-__global__ void k_gl_short(const uint32_t* __restrict__ lam, uint64_t n, uint32_t per32, uint32_t* __restrict__ fa)
+// as 64-bit keys: the scan input "position i has a short label".
+struct ScanShort {
+    const uint32_t* lam;
+    uint32_t per32;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return lam[i] <= per32 ? 1u : 0u; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const
+    {
+        const uint4 v = *reinterpret_cast<const uint4*>(lam + i);
+        return make_uint4(v.x <= per32, v.y <= per32, v.z <= per32, v.w <= per32);
+    }
+};
+
+// The scan input "sorted key i starts a run of equal keys" (K of 4 or 8 bytes).
+template <typename K>
+struct ScanHeads {
+    const K* key;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return (i == 0 || key[i] != key[i - 1]) ? 1u : 0u; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const
+    {
+        const K a = key[i], b = key[i + 1], c = key[i + 2], d = key[i + 3];
+        const uint32_t h0 = (i == 0 || a != key[i - 1]) ? 1u : 0u;
+        return make_uint4(h0, b != a, c != b, d != c);
+    }
+};
+
+// gofs[g] = first index of group g (0-based) from the inclusive head counts gid1 alone,
+// gofs[ngroups] = m
+__global__ void k_gl_gofs(const uint32_t* __restrict__ gid1, uint32_t m, uint32_t* __restrict__ gofs)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) fa[i] = (lam[i] <= per32) ? 1u : 0u;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t g = gid1[i];
+    if (i == 0 || gid1[i - 1] != g) gofs[g - 1] = i;
+    if (i == m - 1) gofs[g] = m;
 }
 
 // the key of every position into the short list (32 bits; fa = exclusive scan of the flags) or
@@ -1068,12 +1097,6 @@ __global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32
         KB[i - a] = key;
         VB[i - a] = (uint32_t)i;
     }
-}
-
-__global__ void k_gl_heads32(const uint32_t* __restrict__ K, uint32_t m, uint32_t* __restrict__ hf)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < m) hf[i] = (i == 0 || K[i] != K[i - 1]) ? 1u : 0u;
 }
 
 // per distinct label d (the runs of equal sorted keys): representative, count, 64-bit key
@@ -1527,9 +1550,7 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     const int bitsA = bits_for(amax - 1);  // keys below 2 radix^per32
     const LabelKey lk{T, n, lam, phh, bits, per, per32, radix};
     uint32_t* fa = S.buf<uint32_t>(B_GL_FA, n);
-    k_gl_short<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n, per32, fa);
-    MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, fa, fa, n, gsc + GR_NA);
+    sum_scan_of(S, ScanShort{lam, per32}, (uintptr_t)lam % 16 == 0, fa, n, false, gsc + GR_NA);
     S.read_fast(hv, gsc, GR_COUNT_);
     const uint32_t nA = hv[GR_NA], nB = (uint32_t)n - nA;
     uint32_t* KA0 = S.buf<uint32_t>(B_GL_KA0, nA);
@@ -1550,20 +1571,15 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
         std::swap(VA0, VA1);
     }
     radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, nB, 0, 64);
-    uint32_t* hf = S.buf<uint32_t>(B_LY_HF, n);
+    // distinct keys of each list: inclusive counts of the run heads, then the run starts
     uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
     uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 2);
-    uint32_t* hfB = hf + nA;
     uint32_t* gidB = gid1 + nA;
     uint32_t* gofsB = gofs + nA + 1;
-    k_gl_heads32<<<grid_for(nA), kBlock, 0, S.stream>>>(KA0, nA, hf);
-    MIBWT_LAUNCH_CHECK();
-    k_dbl_headflag<<<grid_for(nB), kBlock, 0, S.stream>>>(K0, nB, hfB);
-    MIBWT_LAUNCH_CHECK();
-    inclusive_sum<uint32_t>(S, hf, gid1, nA, gsc + GR_DA);
-    inclusive_sum<uint32_t>(S, hfB, gidB, nB, gsc + GR_DB);
-    if (nA) k_grp_offsets<<<grid_for(nA), kBlock, 0, S.stream>>>(hf, gid1, nA, gofs);
-    if (nB) k_grp_offsets<<<grid_for(nB), kBlock, 0, S.stream>>>(hfB, gidB, nB, gofsB);
+    sum_scan_of(S, ScanHeads<uint32_t>{KA0}, true, gid1, nA, true, gsc + GR_DA);
+    sum_scan_of(S, ScanHeads<uint64_t>{K0}, true, gidB, nB, true, gsc + GR_DB);
+    if (nA) k_gl_gofs<<<grid_for(nA), kBlock, 0, S.stream>>>(gid1, nA, gofs);
+    if (nB) k_gl_gofs<<<grid_for(nB), kBlock, 0, S.stream>>>(gidB, nB, gofsB);
     MIBWT_LAUNCH_CHECK();
     // roots: the factorization chain, or the prefix maxima of i + lam[i] for many factors
     uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, kRootWalk);

@@ -211,8 +211,17 @@ __global__ __launch_bounds__(kBlock) void k_scan_down(const T* __restrict__ in, 
 // on MI355X, so a tile must carry far more than 11 ns of memory traffic.
 // The items per thread are a launch shape (Storage::tune.chain_items: 64 by default, 16 for
 // short arrays, whose tiles would otherwise be too few to fill the chip).
-template <typename Op, bool kVec, int kChainItems>
-__global__ __launch_bounds__(kBlock) void k_scan_chain(const uint32_t* in, uint32_t* out, uint64_t n,
+// The items come through `in`: in(i) is item i, in.load4(i) items i..i+3 (i a multiple of 4).
+// ScanArray reads a plain array; other inputs compute the items on the fly (a flag of each
+// element, say), which saves materialising them.
+struct ScanArray {
+    const uint32_t* p;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return p[i]; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const { return *reinterpret_cast<const uint4*>(p + i); }
+};
+
+template <typename Op, bool kVec, int kChainItems, typename In = ScanArray>
+__global__ __launch_bounds__(kBlock) void k_scan_chain(In in, uint32_t* out, uint64_t n,
                                                       uint32_t identity, Op op, int inclusive,
                                                       uint32_t* __restrict__ total_out, Chain c, uint32_t tiles)
 {
@@ -226,7 +235,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_chain(const uint32_t* in, uint3
     if (kVec && full) {
 #pragma unroll
         for (int q = 0; q < kChainItems / 4; q++) {
-            const uint4 a = *reinterpret_cast<const uint4*>(in + base + 4 * q);
+            const uint4 a = in.load4(base + 4 * q);
             v[4 * q] = a.x;
             v[4 * q + 1] = a.y;
             v[4 * q + 2] = a.z;
@@ -234,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_chain(const uint32_t* in, uint3
         }
     } else {
 #pragma unroll
-        for (int i = 0; i < kChainItems; i++) v[i] = (base + i < n) ? in[base + i] : identity;
+        for (int i = 0; i < kChainItems; i++) v[i] = (base + i < n) ? in(base + i) : identity;
     }
     uint32_t acc = identity;
 #pragma unroll
@@ -288,12 +297,13 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
         const bool vec = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
         const int inc = inclusive ? 1 : 0;
         const uint32_t id = (uint32_t)identity;
+        const ScanArray a{in32};
         if (items == 16) {
-            if (vec) k_scan_chain<Op, true, 16><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
-            else k_scan_chain<Op, false, 16><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+            if (vec) k_scan_chain<Op, true, 16><<<tiles, kBlock, 0, S.stream>>>(a, out32, n, id, op, inc, tot32, c, tiles);
+            else k_scan_chain<Op, false, 16><<<tiles, kBlock, 0, S.stream>>>(a, out32, n, id, op, inc, tot32, c, tiles);
         } else {
-            if (vec) k_scan_chain<Op, true, 64><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
-            else k_scan_chain<Op, false, 64><<<tiles, kBlock, 0, S.stream>>>(in32, out32, n, id, op, inc, tot32, c, tiles);
+            if (vec) k_scan_chain<Op, true, 64><<<tiles, kBlock, 0, S.stream>>>(a, out32, n, id, op, inc, tot32, c, tiles);
+            else k_scan_chain<Op, false, 64><<<tiles, kBlock, 0, S.stream>>>(a, out32, n, id, op, inc, tot32, c, tiles);
         }
         MIBWT_LAUNCH_CHECK();
         return;
@@ -306,6 +316,34 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
     device_scan<T, Op>(S, partial, partial, tiles, false, identity, op, nullptr, tmp_id, depth + 1);
     k_scan_down<T, Op><<<tiles, kBlock, 0, S.stream>>>(in, out, partial, n, identity, op,
                                                         inclusive ? 1 : 0, total_dev);
+    MIBWT_LAUNCH_CHECK();
+}
+
+template <typename T>
+void fill(Storage& S, T* p, uint64_t n, T v);
+
+// Sum scan of n items computed by `in` (a ScanArray-like input whose load4 reads 16-byte aligned
+// words when `aligned`): out[i] = the sum of the items before i (inclusive: up to i), and the total
+// into total_dev.  One chained launch for any n.
+template <typename In>
+void sum_scan_of(Storage& S, In in, bool aligned, uint32_t* out, uint64_t n, bool inclusive, uint32_t* total_dev)
+{
+    if (n == 0) {
+        if (total_dev) fill<uint32_t>(S, total_dev, 1, 0u);
+        return;
+    }
+    const int items = (S.tune.chain_items == 16) ? 16 : 64;
+    const uint32_t tiles = ceil_div(n, (uint64_t)kBlock * items);
+    const Chain c = S.chain(tiles);
+    const bool vec = aligned && ((uintptr_t)out % 16 == 0);
+    const int inc = inclusive ? 1 : 0;
+    if (items == 16) {
+        if (vec) k_scan_chain<OpSum, true, 16, In><<<tiles, kBlock, 0, S.stream>>>(in, out, n, 0u, OpSum{}, inc, total_dev, c, tiles);
+        else k_scan_chain<OpSum, false, 16, In><<<tiles, kBlock, 0, S.stream>>>(in, out, n, 0u, OpSum{}, inc, total_dev, c, tiles);
+    } else {
+        if (vec) k_scan_chain<OpSum, true, 64, In><<<tiles, kBlock, 0, S.stream>>>(in, out, n, 0u, OpSum{}, inc, total_dev, c, tiles);
+        else k_scan_chain<OpSum, false, 64, In><<<tiles, kBlock, 0, S.stream>>>(in, out, n, 0u, OpSum{}, inc, total_dev, c, tiles);
+    }
     MIBWT_LAUNCH_CHECK();
 }
 
