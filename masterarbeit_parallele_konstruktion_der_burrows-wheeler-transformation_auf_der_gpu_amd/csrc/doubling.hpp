@@ -66,6 +66,17 @@ static __global__ void k_grp_offsets(const uint32_t* __restrict__ hf, const uint
     if (i == m - 1) gofs[gid1[i]] = m;
 }
 
+// gofs[g] = first index of group g (0-based) from the inclusive head counts gid1 alone,
+// gofs[ngroups] = m
+static __global__ void k_grp_offsets_gid(const uint32_t* __restrict__ gid1, uint32_t m, uint32_t* __restrict__ gofs)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    const uint32_t g = gid1[i];
+    if (i == 0 || gid1[i - 1] != g) gofs[g - 1] = i;
+    if (i == m - 1) gofs[g] = m;
+}
+
 // Element i of the initial order: rank and SA slot, whether its group stays active, and the
 // group size (the first compaction reads the rest of the group record from gid1 / gofs).  With
 // `pairs` the rank is not scattered but written as pairs[i] = x << 32 | rank, in order.
@@ -643,11 +654,8 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t* gid1 = large;
     uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS0, N + 1);
     {
-        uint32_t* hf = keep;
-        k_dbl_headflag<<<grid_for(N), kBlock, 0, S.stream>>>(K0, N, hf);
-        MIBWT_LAUNCH_CHECK();
-        inclusive_sum<uint32_t>(S, hf, gid1, N);
-        k_grp_offsets<<<grid_for(N), kBlock, 0, S.stream>>>(hf, gid1, N, gofs);
+        sum_scan_of(S, ScanHeads<uint64_t>{K0}, true, gid1, N, true, nullptr);
+        k_grp_offsets_gid<<<grid_for(N), kBlock, 0, S.stream>>>(gid1, N, gofs);
         MIBWT_LAUNCH_CHECK();
         // the keys are dead once the heads are flagged: K0 takes the (position, rank) pairs
         const bool part = K1 && N >= kRankPartMin;

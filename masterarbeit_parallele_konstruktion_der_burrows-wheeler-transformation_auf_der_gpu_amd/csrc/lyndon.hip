@@ -1053,30 +1053,6 @@ struct ScanShort {
     }
 };
 
-// The scan input "sorted key i starts a run of equal keys" (K of 4 or 8 bytes).
-template <typename K>
-struct ScanHeads {
-    const K* key;
-    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return (i == 0 || key[i] != key[i - 1]) ? 1u : 0u; }
-    __device__ __forceinline__ uint4 load4(uint64_t i) const
-    {
-        const K a = key[i], b = key[i + 1], c = key[i + 2], d = key[i + 3];
-        const uint32_t h0 = (i == 0 || a != key[i - 1]) ? 1u : 0u;
-        return make_uint4(h0, b != a, c != b, d != c);
-    }
-};
-
-// gofs[g] = first index of group g (0-based) from the inclusive head counts gid1 alone,
-// gofs[ngroups] = m
-__global__ void k_gl_gofs(const uint32_t* __restrict__ gid1, uint32_t m, uint32_t* __restrict__ gofs)
-{
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m) return;
-    const uint32_t g = gid1[i];
-    if (i == 0 || gid1[i - 1] != g) gofs[g - 1] = i;
-    if (i == m - 1) gofs[g] = m;
-}
-
 // the key of every position into the short list (32 bits; fa = exclusive scan of the flags) or
 // the long list (64 bits), with the position as value
 __global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32_t per32,
@@ -1578,8 +1554,8 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     uint32_t* gofsB = gofs + nA + 1;
     sum_scan_of(S, ScanHeads<uint32_t>{KA0}, true, gid1, nA, true, gsc + GR_DA);
     sum_scan_of(S, ScanHeads<uint64_t>{K0}, true, gidB, nB, true, gsc + GR_DB);
-    if (nA) k_gl_gofs<<<grid_for(nA), kBlock, 0, S.stream>>>(gid1, nA, gofs);
-    if (nB) k_gl_gofs<<<grid_for(nB), kBlock, 0, S.stream>>>(gidB, nB, gofsB);
+    if (nA) k_grp_offsets_gid<<<grid_for(nA), kBlock, 0, S.stream>>>(gid1, nA, gofs);
+    if (nB) k_grp_offsets_gid<<<grid_for(nB), kBlock, 0, S.stream>>>(gidB, nB, gofsB);
     MIBWT_LAUNCH_CHECK();
     // roots: the factorization chain, or the prefix maxima of i + lam[i] for many factors
     uint32_t* roots = S.buf<uint32_t>(B_GR_ROOT, kRootWalk);

@@ -319,6 +319,19 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
     MIBWT_LAUNCH_CHECK();
 }
 
+// The scan input "sorted key i starts a run of equal keys" (K of 4 or 8 bytes).
+template <typename K>
+struct ScanHeads {
+    const K* key;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return (i == 0 || key[i] != key[i - 1]) ? 1u : 0u; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const
+    {
+        const K a = key[i], b = key[i + 1], c = key[i + 2], d = key[i + 3];
+        const uint32_t h0 = (i == 0 || a != key[i - 1]) ? 1u : 0u;
+        return make_uint4(h0, b != a, c != b, d != c);
+    }
+};
+
 template <typename T>
 void fill(Storage& S, T* p, uint64_t n, T v);
 
