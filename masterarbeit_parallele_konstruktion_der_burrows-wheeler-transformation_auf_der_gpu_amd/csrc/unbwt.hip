@@ -72,10 +72,19 @@ __global__ void k_un_walk1(const uint64_t* __restrict__ rec, uint64_t m, uint64_
     }
     uint64_t r = r0, steps = 0;
     uint32_t next = kUnEnd;
-    uint8_t* st = stash ? stash + (uint64_t)s * kUnStash : nullptr;
+    // the kept bytes leave 8 at a time (aligned 8-byte stores; a walk's last word may carry bytes
+    // past its end, which nothing reads)
+    uint64_t* st = stash ? reinterpret_cast<uint64_t*>(stash + (uint64_t)s * kUnStash) : nullptr;
+    uint64_t acc = 0;
     for (;;) {
         const uint64_t x = rec[r];
-        if (st && steps < kUnStash) st[steps] = (uint8_t)(x >> 32);
+        if (st && steps < kUnStash) {
+            acc |= ((x >> 32) & 0xFFull) << (8 * (steps & 7));
+            if ((steps & 7) == 7) {
+                st[steps >> 3] = acc;
+                acc = 0;
+            }
+        }
         r = (uint32_t)x;
         steps++;
         if (st && steps == kUnStash) rowk[s] = (uint32_t)r;
@@ -89,6 +98,7 @@ __global__ void k_un_walk1(const uint64_t* __restrict__ rec, uint64_t m, uint64_
             break;
         }
     }
+    if (st && steps < kUnStash && (steps & 7)) st[steps >> 3] = acc;
     nxt[s] = next;
     w[s] = (uint32_t)steps;
 }

@@ -34,6 +34,16 @@ def test_lyndon_array(storage, name):
     assert (storage.lyndon_array(T).astype(np.int64) == oracle.lyndon_array(T)).all()
 
 
+@pytest.mark.parametrize("bs", ["16", "64", "256"])
+def test_lyndon_array_block_sizes(storage, monkeypatch, bs):
+    # the local pass over LDS at other block sizes (the merge tree then starts from smaller or
+    # larger blocks; 16 makes every workgroup span several blocks of a short text)
+    monkeypatch.setenv("MIBWT_LYN_BLOCK", bs)
+    for name in ("period5", "repetitive", "long_period", "unary", "descending", "rand3", "rand7"):
+        T = TEXTS[name]
+        assert (storage.lyndon_array(T).astype(np.int64) == oracle.lyndon_array(T)).all(), name
+
+
 INDUCTIONS = {
     # grammar induction over distinct labels (the default)
     "grammar": ({}, 1),
