@@ -1000,15 +1000,22 @@ struct LabelKey {
     __device__ __forceinline__ uint64_t operator()(uint32_t i, const uint32_t* cs) const
     {
         const uint32_t l = lam[i];
-        if (l <= per32) {
-            uint32_t key = 1;
-            uint64_t w = 0;
-            for (uint32_t k = 0; k < l; k++) {
-                if ((k & 7) == 0) w = load8(T, (uint64_t)i + k);
-                key = key * radix + (cs[(w >> (8 * (k & 7))) & 0xFF] - 2);
-            }
-            return key;
+        return (l <= per32) ? short_key(i, l, cs) : long_key(i, l, cs);
+    }
+    // labels of l <= per32 symbols
+    __device__ __forceinline__ uint32_t short_key(uint32_t i, uint32_t l, const uint32_t* cs) const
+    {
+        uint32_t key = 1;
+        uint64_t w = 0;
+        for (uint32_t k = 0; k < l; k++) {
+            if ((k & 7) == 0) w = load8(T, (uint64_t)i + k);
+            key = key * radix + (cs[(w >> (8 * (k & 7))) & 0xFF] - 2);
         }
+        return key;
+    }
+    // labels of l > per32 symbols
+    __device__ __forceinline__ uint64_t long_key(uint32_t i, uint32_t l, const uint32_t* cs) const
+    {
         if (l <= (uint32_t)per) {
             // the l symbols, then zero codes up to `per` (l <= per < 64, so i + l <= n)
             uint64_t key = 0, w = 0;
@@ -1055,24 +1062,37 @@ struct ScanShort {
 
 // the key of every position into the short list (32 bits; fa = exclusive scan of the flags) or
 // the long list (64 bits), with the position as value
+// The key work is split by label kind so that a wavefront runs one kind of loop: this pass keys
+// the short labels and lists the positions of the others, k_gl_keys_long keys those.
 __global__ void k_gl_keys(LabelKey lk, const uint32_t* __restrict__ code, uint32_t per32,
                           const uint32_t* __restrict__ fa, uint32_t* __restrict__ KA, uint32_t* __restrict__ VA,
-                          uint64_t* __restrict__ KB, uint32_t* __restrict__ VB)
+                          uint32_t* __restrict__ VB)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
     __syncthreads();
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= lk.n) return;
-    const uint64_t key = lk((uint32_t)i, cs);
+    const uint32_t l = lk.lam[i];
     const uint32_t a = fa[i];
-    if (lk.lam[i] <= per32) {
-        KA[a] = (uint32_t)key;
+    if (l <= per32) {
+        KA[a] = lk.short_key((uint32_t)i, l, cs);
         VA[a] = (uint32_t)i;
     } else {
-        KB[i - a] = key;
         VB[i - a] = (uint32_t)i;
     }
+}
+
+__global__ void k_gl_keys_long(LabelKey lk, const uint32_t* __restrict__ code, const uint32_t* __restrict__ VB,
+                               uint32_t nB, uint64_t* __restrict__ KB)
+{
+    __shared__ uint32_t cs[256];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    __syncthreads();
+    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= nB) return;
+    const uint32_t i = VB[j];
+    KB[j] = lk.long_key(i, lk.lam[i], cs);
 }
 
 // per distinct label d (the runs of equal sorted keys): representative, count, 64-bit key
@@ -1537,7 +1557,9 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, nB);
     uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, nB);
     uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, nB);
-    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, K0, V0);
+    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, V0);
+    MIBWT_LAUNCH_CHECK();
+    if (nB) k_gl_keys_long<<<grid_for(nB), kBlock, 0, S.stream>>>(lk, codes, V0, nB, K0);
     MIBWT_LAUNCH_CHECK();
     int whereA = 0;  // an odd number of passes leaves the short list in buffer 1
     radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, bitsA,
