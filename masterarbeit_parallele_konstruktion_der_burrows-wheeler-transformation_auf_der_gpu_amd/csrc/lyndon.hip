@@ -11,7 +11,7 @@
 // sorts rules (:3129-4083) and induces the BWT over rank groups (:4114-4896).  Here:
 //
 //  1. rules    Karp-Rabin prefix fingerprints (mod 2^61-1) of the text in one device scan.
-//  2. lyndon   Lyndon array as next-smaller-suffix distances: every 128-byte block chases
+//  2. lyndon   Lyndon array as next-smaller-suffix distances: every 64-byte block chases
 //              NSS pointers right to left over its text staged in LDS; positions without a smaller suffix in the block
 //              stay open and are resolved by a merge tree over neighbouring ranges, where an
 //              open position bisects the right range's prefix minima.  Suffix comparisons
@@ -42,9 +42,10 @@ namespace mibwt {
 bool bwt_inverts_to(Storage& S, const uint8_t* dO, int64_t n, int64_t index, const uint8_t* dT);
 
 // Characters per block of the local pass (MIBWT_LYN_BLOCK overrides: 16..256).  Measured on the
-// 1 GiB DNA collection (local pass + merge tree): 64 -> 38.8 ms, 128 -> 32.2 ms, 256 -> 43.3 ms
-// (256 halves the resident workgroups: 34 KiB of LDS each against 18 KiB).
-constexpr uint32_t kLynBlock = 128;
+// 1 GiB DNA collection (local pass + merge tree with eight lanes per pair on the wide levels):
+// 32 -> 25.4 ms, 64 -> 21.4 ms, 128 -> 23.0 ms, 256 -> 43 ms (the workgroups' LDS, 10 / 18 / 34 KiB
+// at 64 / 128 / 256, bounds the resident wavefronts of this latency-bound pass).
+constexpr uint32_t kLynBlock = 64;
 // Label tie groups of more members go to the pivot partition (linear work per round; a group of
 // equal labels settles in one round) instead of the comparator counting sort (quadratic in the
 // group size, but one pass); MIBWT_LAB_SMALL overrides it (2..256).  Ordering every position
