@@ -282,13 +282,14 @@ __device__ __forceinline__ uint64_t lds_read8(const uint8_t* row, uint32_t o)
     return ((uint64_t)__builtin_amdgcn_alignbyte(w2, w1, sh) << 32) | __builtin_amdgcn_alignbyte(w1, w0, sh);
 }
 
-// One thread per block, right to left: NSS by pointer chasing inside the block.  Positions
-// without a smaller suffix inside the block stay open (kNoNss).  Per block: the open positions
+// One thread per block, right to left: NSS by pointer chasing inside the block; lam[i] receives
+// NSS(i) - i.  Positions without a smaller suffix inside the block stay open (kNoNss, set by the
+// merge tree or, for the positions still open at its end, by k_nss_last).  Per block: the open positions
 // in ascending order (O, increasing suffixes) and the chain of prefix minima from the block start
 // (M = s, NSS[s], NSS[NSS[s]], ..., decreasing suffixes).  Launched with 64 threads and
 // 64 * (nss_text_row(bs) + nss_off_row(bs)) bytes of dynamic LDS; bs is a power of two in [16, 256].
 __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uint32_t bs, int lbs,
-                                                  uint32_t* __restrict__ nss, uint32_t* __restrict__ O,
+                                                  uint32_t* __restrict__ lam, uint32_t* __restrict__ O,
                                                   uint32_t* __restrict__ oc, uint32_t* __restrict__ M,
                                                   uint32_t* __restrict__ mc)
 {
@@ -366,11 +367,11 @@ __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uin
         }
     }
     __syncthreads();
-    // NSS values of the workgroup's positions, coalesced
+    // NSS distances (lam) of the workgroup's positions, coalesced; kNoNss for the open ones
     const uint64_t tot = (sc.n - s0 < (uint64_t)64 * bs) ? sc.n - s0 : (uint64_t)64 * bs;
     for (uint32_t q = lane; q < tot; q += 64) {
         const uint32_t o = of[(q >> lbs) * NS + (q & (bs - 1))];
-        nss[s0 + q] = o ? (uint32_t)(s0 + q + o) : kNoNss;
+        lam[s0 + q] = o ? o : kNoNss;
     }
     if (b >= nb) return;
     const uint32_t s = (uint32_t)(b * bs);
@@ -390,13 +391,13 @@ __global__ __launch_bounds__(64) void k_nss_local(SuffixCmp sc, uint32_t nb, uin
 
 // G lanes (8 or 64) per pair of neighbouring ranges L, R of `size` blocks, 64 / G pairs per
 // wavefront.  An open position i of L finds its NSS among R's prefix minima: they decrease, so the
-// first one below suffix(i) is found by bisection.  The merged range keeps L's unresolved
+// first one below suffix(i) is found by bisection (lam[i] = NSS(i) - i).  The merged range keeps L's unresolved
 // positions followed by R's open ones, and L's prefix minima followed by the prefix minima of R
 // below min(L).  The lists are short on most texts (a handful of entries per range), so the
 // many pairs of the low levels share wavefronts.
 template <int G>
 __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uint32_t bs, uint32_t size,
-                                                  uint32_t* __restrict__ nss, const uint32_t* __restrict__ Oin,
+                                                  uint32_t* __restrict__ lam, const uint32_t* __restrict__ Oin,
                                                   const uint32_t* __restrict__ ocin, const uint32_t* __restrict__ Min,
                                                   const uint32_t* __restrict__ mcin, uint32_t* __restrict__ Oout,
                                                   uint32_t* __restrict__ ocout, uint32_t* __restrict__ Mout,
@@ -445,7 +446,7 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
         if (q < oL) {
             const uint32_t i = OL[q];
             const uint32_t t = first_below(i);
-            if (t < mR) nss[i] = MR[t];
+            if (t < mR) lam[i] = MR[t] - i;
             else keep = true;
         }
         const uint64_t bal = __ballot(keep) & gmask;
@@ -463,12 +464,13 @@ __global__ __launch_bounds__(64) void k_nss_merge(SuffixCmp sc, uint32_t nb, uin
     }
 }
 
-__global__ void k_nss_to_lam(uint32_t* __restrict__ a, uint64_t n)
+// the positions still open after the last level (no smaller suffix follows): lam = n - i
+__global__ void k_nss_last(const uint32_t* __restrict__ O, const uint32_t* __restrict__ oc, uint64_t n,
+                           uint32_t* __restrict__ lam)
 {
-    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t j = a[i];
-    a[i] = (j == kNoNss ? (uint32_t)n : j) - (uint32_t)i;
+    const uint32_t m = oc[0];
+    for (uint32_t q = blockIdx.x * blockDim.x + threadIdx.x; q < m; q += gridDim.x * blockDim.x)
+        lam[O[q]] = (uint32_t)n - O[q];
 }
 
 // lam[] of T[0..n) (the padded text copy T, prefix fingerprints ph).  Returns the number of
@@ -512,7 +514,7 @@ int lyndon_array(Storage& S, const uint8_t* T, uint64_t n, PrefixHash ph, uint32
         std::swap(mca, mcb);
         levels++;
     }
-    k_nss_to_lam<<<grid_for(n), kBlock, 0, S.stream>>>(lam, n);
+    k_nss_last<<<256, kBlock, 0, S.stream>>>(oa, oca, n, lam);
     MIBWT_LAUNCH_CHECK();
     return levels;
 }
