@@ -1098,6 +1098,79 @@ __global__ void k_gl_keys_long(LabelKey lk, const uint32_t* __restrict__ code, c
     KB[j] = lk.long_key(i, lk.lam[i], cs);
 }
 
+// Short labels whose keys lie below 2^24 are counted in a table indexed by the key instead of
+// being sorted: per key the occurrences and the smallest position (the representative the
+// stable sort would pick).  Keys below kCountBins (on DNA the labels of up to 5 symbols, 80% of
+// the positions) are counted per workgroup in LDS first, kCountPer positions per thread, and
+// leave as one device atomic per key and workgroup; the others go to the table directly.  The
+// long labels' positions are listed as in k_gl_keys.
+constexpr int kTabBitsMax = 24;
+constexpr uint32_t kCountBins = 2048;
+constexpr int kCountPer = 8;
+__global__ __launch_bounds__(kBlock) void k_gl_count(LabelKey lk, const uint32_t* __restrict__ code,
+                                                    const uint32_t* __restrict__ fa, uint32_t* __restrict__ tcnt,
+                                                    uint32_t* __restrict__ trep, uint32_t* __restrict__ VB)
+{
+    __shared__ uint32_t cs[256];
+    __shared__ uint32_t bc[kCountBins], br[kCountBins];
+    cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
+    for (uint32_t b = threadIdx.x; b < kCountBins; b += kBlock) {
+        bc[b] = 0;
+        br[b] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kCountPer);
+    for (int j = 0; j < kCountPer; j++) {
+        const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
+        if (i >= lk.n) break;
+        const uint32_t l = lk.lam[i];
+        if (l <= lk.per32) {
+            const uint32_t key = lk.short_key((uint32_t)i, l, cs);
+            if (key < kCountBins) {
+                atomicAdd(&bc[key], 1u);
+                atomicMin(&br[key], (uint32_t)i);
+            } else {
+                atomicAdd(&tcnt[key], 1u);
+                atomicMin(&trep[key], (uint32_t)i);
+            }
+        } else {
+            VB[i - fa[i]] = (uint32_t)i;
+        }
+    }
+    __syncthreads();
+    for (uint32_t b = threadIdx.x; b < kCountBins; b += kBlock)
+        if (bc[b]) {
+            atomicAdd(&tcnt[b], bc[b]);
+            atomicMin(&trep[b], br[b]);
+        }
+}
+
+// the scan input "table entry k is used"
+struct ScanNonzero {
+    const uint32_t* c;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return c[i] ? 1u : 0u; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const
+    {
+        const uint4 v = *reinterpret_cast<const uint4*>(c + i);
+        return make_uint4(v.x != 0, v.y != 0, v.z != 0, v.w != 0);
+    }
+};
+
+// the used table entries in key order: representative, count, key
+__global__ void k_gl_tab_dedup(const uint32_t* __restrict__ tcnt, const uint32_t* __restrict__ trep,
+                               const uint32_t* __restrict__ tpos, uint32_t K, uint32_t* __restrict__ rep,
+                               uint32_t* __restrict__ cnt, uint64_t* __restrict__ U)
+{
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= K) return;
+    const uint32_t c = tcnt[k];
+    if (!c) return;
+    const uint32_t d = tpos[k];
+    rep[d] = trep[k];
+    cnt[d] = c;
+    U[d] = k;
+}
+
 // per distinct label d (the runs of equal sorted keys): representative, count, 64-bit key
 template <typename K>
 __global__ void k_gl_dedup(const K* __restrict__ Ks, const uint32_t* __restrict__ V, const uint32_t* __restrict__ gofs,
@@ -1552,34 +1625,55 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     sum_scan_of(S, ScanShort{lam, per32}, (uintptr_t)lam % 16 == 0, fa, n, false, gsc + GR_NA);
     S.read_fast(hv, gsc, GR_COUNT_);
     const uint32_t nA = hv[GR_NA], nB = (uint32_t)n - nA;
-    uint32_t* KA0 = S.buf<uint32_t>(B_GL_KA0, nA);
-    uint32_t* KA1 = S.buf<uint32_t>(B_GL_KA1, nA);
-    uint32_t* VA0 = S.buf<uint32_t>(B_GL_VA0, nA);
-    uint32_t* VA1 = S.buf<uint32_t>(B_GL_VA1, nA);
     uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, nB);
     uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, nB);
     uint32_t* V0 = S.buf<uint32_t>(B_SORT_V0, nB);
     uint32_t* V1 = S.buf<uint32_t>(B_SORT_V1, nB);
-    k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, V0);
-    MIBWT_LAUNCH_CHECK();
+    // short labels: counted in a table of 2^bitsA keys, or (wider keys) sorted as 32-bit keys
+    const bool table = bitsA <= kTabBitsMax;
+    const uint32_t tabK = table ? (1u << bitsA) : 0u;
+    uint32_t *KA0 = nullptr, *VA0 = nullptr, *tcnt = nullptr, *trep = nullptr, *tpos = nullptr;
+    if (table) {
+        tcnt = S.buf<uint32_t>(B_GL_KA0, tabK);
+        trep = S.buf<uint32_t>(B_GL_VA0, tabK);
+        tpos = S.buf<uint32_t>(B_GL_KA1, tabK);
+        fill<uint32_t>(S, tcnt, tabK, 0u);
+        fill<uint32_t>(S, trep, tabK, 0xFFFFFFFFu);
+        k_gl_count<<<ceil_div(n, (uint64_t)kBlock * kCountPer), kBlock, 0, S.stream>>>(lk, codes, fa, tcnt, trep, V0);
+        MIBWT_LAUNCH_CHECK();
+    } else {
+        KA0 = S.buf<uint32_t>(B_GL_KA0, nA);
+        VA0 = S.buf<uint32_t>(B_GL_VA0, nA);
+        k_gl_keys<<<grid_for(n), kBlock, 0, S.stream>>>(lk, codes, per32, fa, KA0, VA0, V0);
+        MIBWT_LAUNCH_CHECK();
+    }
     if (nB) k_gl_keys_long<<<grid_for(nB), kBlock, 0, S.stream>>>(lk, codes, V0, nB, K0);
     MIBWT_LAUNCH_CHECK();
-    int whereA = 0;  // an odd number of passes leaves the short list in buffer 1
-    radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, bitsA,
-                                   static_cast<const EmitPair<uint32_t, uint32_t>*>(nullptr), &whereA);
-    if (whereA) {
-        std::swap(KA0, KA1);
-        std::swap(VA0, VA1);
+    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n + 4);
+    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 8);
+    if (table) {
+        sum_scan_of(S, ScanNonzero{tcnt}, true, tpos, tabK, false, gsc + GR_DA);
+    } else {
+        uint32_t* KA1 = S.buf<uint32_t>(B_GL_KA1, nA);
+        uint32_t* VA1 = S.buf<uint32_t>(B_GL_VA1, nA);
+        int whereA = 0;  // an odd number of passes leaves the short list in buffer 1
+        radix_sort<uint32_t, uint32_t>(S, KA0, KA1, VA0, VA1, nA, 0, bitsA,
+                                       static_cast<const EmitPair<uint32_t, uint32_t>*>(nullptr), &whereA);
+        if (whereA) {
+            std::swap(KA0, KA1);
+            std::swap(VA0, VA1);
+        }
+        // distinct keys: inclusive counts of the run heads, then the run starts
+        sum_scan_of(S, ScanHeads<uint32_t>{KA0}, true, gid1, nA, true, gsc + GR_DA);
+        if (nA) k_grp_offsets_gid<<<grid_for(nA), kBlock, 0, S.stream>>>(gid1, nA, gofs);
+        MIBWT_LAUNCH_CHECK();
     }
     radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, nB, 0, 64);
-    // distinct keys of each list: inclusive counts of the run heads, then the run starts
-    uint32_t* gid1 = S.buf<uint32_t>(B_DB_GID, n);
-    uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS, n + 2);
-    uint32_t* gidB = gid1 + nA;
-    uint32_t* gofsB = gofs + nA + 1;
-    sum_scan_of(S, ScanHeads<uint32_t>{KA0}, true, gid1, nA, true, gsc + GR_DA);
+    // the long list's group ids and starts behind the short list's (16-byte aligned)
+    const uint32_t offB = table ? 0u : ((nA + 3) & ~3u);
+    uint32_t* gidB = gid1 + offB;
+    uint32_t* gofsB = gofs + offB + 4;
     sum_scan_of(S, ScanHeads<uint64_t>{K0}, true, gidB, nB, true, gsc + GR_DB);
-    if (nA) k_grp_offsets_gid<<<grid_for(nA), kBlock, 0, S.stream>>>(gid1, nA, gofs);
     if (nB) k_grp_offsets_gid<<<grid_for(nB), kBlock, 0, S.stream>>>(gidB, nB, gofsB);
     MIBWT_LAUNCH_CHECK();
     // roots: the factorization chain, or the prefix maxima of i + lam[i] for many factors
@@ -1609,7 +1703,8 @@ static uint32_t lyndon_grammar(Storage& S, const uint8_t* T, uint64_t n, const P
     uint32_t* repB = S.buf<uint32_t>(B_GL_REPB, DB);
     uint32_t* cntB = S.buf<uint32_t>(B_GL_CNTB, DB);
     uint64_t* UB = S.buf<uint64_t>(B_GL_UB, DB);
-    k_gl_dedup<uint32_t><<<grid_for(DA), kBlock, 0, S.stream>>>(KA0, VA0, gofs, DA, repA, cntA, UA);
+    if (table) k_gl_tab_dedup<<<grid_for(tabK), kBlock, 0, S.stream>>>(tcnt, trep, tpos, tabK, repA, cntA, UA);
+    else k_gl_dedup<uint32_t><<<grid_for(DA), kBlock, 0, S.stream>>>(KA0, VA0, gofs, DA, repA, cntA, UA);
     MIBWT_LAUNCH_CHECK();
     k_gl_dedup<uint64_t><<<grid_for(DB), kBlock, 0, S.stream>>>(K0, V0, gofsB, DB, repB, cntB, UB);
     MIBWT_LAUNCH_CHECK();
