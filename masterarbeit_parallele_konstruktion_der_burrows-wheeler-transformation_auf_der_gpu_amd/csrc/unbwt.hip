@@ -34,27 +34,110 @@ __device__ __forceinline__ uint8_t bwt_row(const uint8_t* __restrict__ L, uint64
     return (r < index) ? L[r] : L[r - 1];
 }
 
-// keys of the n byte rows: byte << 32 | row
-__global__ void k_un_keys(const uint8_t* __restrict__ L, uint64_t n, uint64_t index, int ref,
-                          uint64_t* __restrict__ K)
+// LF of the n byte rows (every row but the '$' row) without sorting them: LF(r) = 1 + (rows whose
+// byte is smaller) + (rows before r with the same byte), the '$' row being row 0 of the sorted
+// order.  Tiles of kLfTile byte rows, one wavefront per kLfWave consecutive rows (16 per lane);
+// k_lf_count writes every tile's byte counts digit-major (cnt[c * tiles + tile]), one flat
+// exclusive scan turns them into each (byte, tile)'s first LF - 1, and k_lf_fill ranks the rows
+// of its tile per byte (stable: the lanes with equal bytes found by one ballot per byte bit) and
+// writes rec[r] = LF(r) | B[r] << 32 in row order.
+constexpr int kLfPer = 16;
+constexpr uint32_t kLfWave = 64 * kLfPer;
+constexpr uint32_t kLfTile = kWaves * kLfWave;
+
+// byte of byte row q (row q, or q + 1 past the '$' row)
+__device__ __forceinline__ uint32_t lf_byte(const uint8_t* __restrict__ L, uint64_t q, uint64_t index, bool ref)
 {
-    const uint64_t q = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= n) return;
-    const uint64_t r = (q < index) ? q : q + 1;  // skip the '$' row
-    K[q] = ((uint64_t)bwt_row(L, r, index, ref != 0) << 32) | r;
+    return bwt_row(L, (q < index) ? q : q + 1, index, ref);
 }
 
-// last pass of the row sort: row r lands at sorted position pos + 1 ('$' is position 0)
-struct EmitLF {
-    static constexpr double bytes_per_item = 8;
-    static constexpr bool kValues = false;
-    uint64_t* rec;
-    __device__ __forceinline__ void val(uint64_t, uint32_t) const {}
-    __device__ __forceinline__ void key(uint64_t pos, uint64_t key) const
-    {
-        rec[(uint32_t)key] = (pos + 1) | ((key >> 32) << 32);
+// per-wavefront stable ranking of the byte rows [w0, w0 + kLfWave): rank[k] = the rows before
+// item k in the wavefront with the same byte; wcnt (LDS, 256 entries, zeroed) ends with the
+// wavefront's counts
+__device__ __forceinline__ void lf_rank(const uint32_t (&c)[kLfPer], const bool (&valid)[kLfPer], uint32_t* wcnt,
+                                        uint32_t (&rank)[kLfPer])
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int k = 0; k < kLfPer; k++) {
+        const uint64_t vm = __ballot(valid[k]);
+        uint32_t plo = (uint32_t)vm, phi = (uint32_t)(vm >> 32);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const uint32_t mk = (uint32_t)((int32_t)(c[k] << (31 - b)) >> 31);
+            const uint64_t m = __ballot(mk != 0u);
+            plo &= ~((uint32_t)m ^ mk);
+            phi &= ~((uint32_t)(m >> 32) ^ mk);
+        }
+        const int leader = plo ? (__ffs(plo) - 1) : (31 + __ffs(phi));
+        uint32_t old = 0;
+        if (valid[k] && lane == leader) old = atomicAdd(&wcnt[c[k]], (uint32_t)(__popc(plo) + __popc(phi)));
+        old = __shfl(old, valid[k] ? leader : 0, 64);
+        rank[k] = old + __builtin_amdgcn_mbcnt_hi(phi, __builtin_amdgcn_mbcnt_lo(plo, 0u));
     }
-};
+}
+
+__global__ __launch_bounds__(kBlock) void k_lf_count(const uint8_t* __restrict__ L, uint64_t n, uint64_t index,
+                                                    int ref, uint32_t tiles, uint32_t* __restrict__ cnt)
+{
+    __shared__ uint32_t wcnt[kWaves][256];
+    for (int i = threadIdx.x; i < kWaves * 256; i += kBlock) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t w0 = (uint64_t)blockIdx.x * kLfTile + (uint64_t)w * kLfWave;
+    uint32_t c[kLfPer], rank[kLfPer];
+    bool valid[kLfPer];
+#pragma unroll
+    for (int k = 0; k < kLfPer; k++) {
+        const uint64_t q = w0 + (uint64_t)k * 64 + lane;
+        valid[k] = q < n;
+        c[k] = valid[k] ? lf_byte(L, q, index, ref != 0) : 0u;
+    }
+    lf_rank(c, valid, wcnt[w], rank);
+    __syncthreads();
+    const uint32_t b = threadIdx.x;  // kBlock == 256 bytes
+    cnt[(uint64_t)b * tiles + blockIdx.x] = wcnt[0][b] + wcnt[1][b] + wcnt[2][b] + wcnt[3][b];
+}
+
+__global__ __launch_bounds__(kBlock) void k_lf_fill(const uint8_t* __restrict__ L, uint64_t n, uint64_t index,
+                                                   int ref, uint32_t tiles, const uint32_t* __restrict__ start,
+                                                   uint64_t* __restrict__ rec)
+{
+    __shared__ uint32_t wcnt[kWaves][256];
+    for (int i = threadIdx.x; i < kWaves * 256; i += kBlock) (&wcnt[0][0])[i] = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t w0 = (uint64_t)blockIdx.x * kLfTile + (uint64_t)w * kLfWave;
+    uint32_t c[kLfPer], rank[kLfPer];
+    bool valid[kLfPer];
+#pragma unroll
+    for (int k = 0; k < kLfPer; k++) {
+        const uint64_t q = w0 + (uint64_t)k * 64 + lane;
+        valid[k] = q < n;
+        c[k] = valid[k] ? lf_byte(L, q, index, ref != 0) : 0u;
+    }
+    lf_rank(c, valid, wcnt[w], rank);
+    __syncthreads();
+    {
+        // per byte: the tile's first LF - 1, then the earlier wavefronts' rows of that byte
+        const uint32_t b = threadIdx.x;
+        uint32_t run = start[(uint64_t)b * tiles + blockIdx.x];
+#pragma unroll
+        for (int q = 0; q < kWaves; q++) {
+            const uint32_t x = wcnt[q][b];
+            wcnt[q][b] = run;
+            run += x;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kLfPer; k++) {
+        if (!valid[k]) continue;
+        const uint64_t q = w0 + (uint64_t)k * 64 + lane;
+        const uint64_t r = (q < index) ? q : q + 1;
+        rec[r] = (uint64_t)(1 + wcnt[w][c[k]] + rank[k]) | ((uint64_t)c[k] << 32);
+    }
+}
 
 // walk from splitter s to the next splitter: rows visited, next splitter (kUnEnd: the '$' row);
 // with `stash`, the bytes of the first kUnStash rows of the walk and the row after them (rowk)
@@ -281,15 +364,16 @@ int64_t unbwt_run(Storage& S, const uint8_t* dL, uint8_t* dU, int64_t n64, int64
     };
     uint32_t* sc = S.buf<uint32_t>(B_SCALARS, SC_COUNT_);
     int ph = phase_begin("lf");
-    uint64_t* K0 = S.buf<uint64_t>(B_SORT_K0, n);
-    uint64_t* K1 = S.buf<uint64_t>(B_SORT_K1, n);
     uint64_t* rec = S.buf<uint64_t>(B_UN_REC, m);
-    k_un_keys<<<grid_for(n), kBlock, 0, S.stream>>>(dL, n, index, ref_format ? 1 : 0, K0);
-    MIBWT_LAUNCH_CHECK();
     fill<uint64_t>(S, rec + index, 1, 0ull);  // LF('$' row) = row 0; its byte field is unused
-    EmitLF em{rec};
-    radix_sort<uint64_t, uint32_t, EmitLF>(S, K0, K1, nullptr, nullptr, n, 32, 40, &em);
-    phase_end(ph, 8.0 * n * 4);
+    const uint32_t lft = (uint32_t)ceil_div(n, (uint64_t)kLfTile);
+    uint32_t* lfc = S.buf<uint32_t>(B_SORT_TCNT, (size_t)lft * 256);
+    k_lf_count<<<lft, kBlock, 0, S.stream>>>(dL, n, index, ref_format ? 1 : 0, lft, lfc);
+    MIBWT_LAUNCH_CHECK();
+    exclusive_sum<uint32_t>(S, lfc, lfc, (uint64_t)lft * 256);
+    k_lf_fill<<<lft, kBlock, 0, S.stream>>>(dL, n, index, ref_format ? 1 : 0, lft, lfc, rec);
+    MIBWT_LAUNCH_CHECK();
+    phase_end(ph, 2.0 * n + 8.0 * n);
 
     ph = phase_begin("rank");
     const uint32_t Sn = (uint32_t)ceil_div(m, (uint64_t)kUnSample);
