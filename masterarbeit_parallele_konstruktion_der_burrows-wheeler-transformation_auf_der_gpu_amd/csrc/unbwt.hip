@@ -6,8 +6,8 @@
 // B[0] = O[index-1], B[r] = O[r-1]).  LF maps a row to the row of the preceding rotation; the
 // text is T[n-1-k] = B[r_k] with r_0 = 0, r_{k+1} = LF(r_k), ending at the '$' row after n steps.
 //
-//  1. LF by one stable 8-bit radix pass over the n byte rows (the '$' row sorts first): the
-//     pass's emitter stores rec[r] = LF(r) | B[r] << 32, so a walk step is one 8-byte load.
+//  1. LF of every row, computed in row order from per-tile byte counts (the '$' row sorts
+//     first): rec[r] = LF(r) | B[r] << 32, so a walk step is one 8-byte load.
 //  2. The LF cycle is ranked by sampling: every kUnSample-th row is a splitter; each splitter
 //     walks to the next splitter (or the '$' row) and records the number of rows and the next
 //     splitter.  The splitter list is ranked through a ruling set: every kUnRule-th splitter
