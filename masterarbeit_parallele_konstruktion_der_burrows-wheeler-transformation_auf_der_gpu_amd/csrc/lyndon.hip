@@ -1104,13 +1104,11 @@ __global__ void k_gl_keys_long(LabelKey lk, const uint32_t* __restrict__ code, c
 // first, kCountPer positions per thread, and leave as one device atomic per key and workgroup
 // (representative: the smallest position); the others go to the table directly, one atomic
 // count and a plain store of the position (any occurrence serves: a label's string, children
-// and order are the same at each).  The lanes of a wavefront with the same small key are found
-// by one ballot per key bit, and only the lowest of them updates the bin, so that a frequent
-// label costs one LDS atomic per wavefront instead of one per lane.  The long labels' positions
-// are listed as in k_gl_keys.
+// and order are the same at each).  (Combining the lanes of a wavefront with equal small keys
+// first, one ballot per key bit, measured no faster: the kernel waits on memory, not on LDS
+// atomics.)  The long labels' positions are listed as in k_gl_keys.
 constexpr int kTabBitsMax = 24;
-constexpr int kCountBinBits = 11;
-constexpr uint32_t kCountBins = 1u << kCountBinBits;
+constexpr uint32_t kCountBins = 2048;
 constexpr int kCountPer = 8;
 __global__ __launch_bounds__(kBlock) void k_gl_count(LabelKey lk, const uint32_t* __restrict__ code,
                                                     const uint32_t* __restrict__ fa, uint32_t* __restrict__ tcnt,
@@ -1125,7 +1123,6 @@ __global__ __launch_bounds__(kBlock) void k_gl_count(LabelKey lk, const uint32_t
     }
     __syncthreads();
     const uint64_t base = (uint64_t)blockIdx.x * (kBlock * kCountPer);
-    const int lane = threadIdx.x & 63;
     for (int j = 0; j < kCountPer; j++) {
         const uint64_t i = base + (uint64_t)j * kBlock + threadIdx.x;
         const bool valid = i < lk.n;
@@ -1139,22 +1136,9 @@ __global__ __launch_bounds__(kBlock) void k_gl_count(LabelKey lk, const uint32_t
             atomicAdd(&tcnt[key], 1u);
             trep[key] = (uint32_t)i;  // any occurrence: the label's string is the same at each
         }
-        // lanes with the same small key: per bit, the lanes that agree with this lane's bit
-        const uint64_t act = __ballot(small);
-        if (act) {
-            uint32_t plo = (uint32_t)act, phi = (uint32_t)(act >> 32);
-#pragma unroll
-            for (int b = 0; b < kCountBinBits; b++) {
-                const uint32_t mk = (uint32_t)((int32_t)(key << (31 - b)) >> 31);
-                const uint64_t m = __ballot(mk != 0u);
-                plo &= ~((uint32_t)m ^ mk);
-                phi &= ~((uint32_t)(m >> 32) ^ mk);
-            }
-            const int leader = plo ? (__ffs(plo) - 1) : (31 + __ffs(phi));
-            if (small && lane == leader) {
-                atomicAdd(&bc[key], (uint32_t)(__popc(plo) + __popc(phi)));
-                atomicMin(&br[key], (uint32_t)i);
-            }
+        if (small) {
+            atomicAdd(&bc[key], 1u);
+            atomicMin(&br[key], (uint32_t)i);
         }
     }
     __syncthreads();
