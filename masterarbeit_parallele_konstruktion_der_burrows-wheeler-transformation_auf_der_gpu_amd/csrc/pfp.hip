@@ -366,8 +366,25 @@ __device__ __forceinline__ uint32_t trig_mask32(const uint8_t* txt, uint32_t loc
         const uint32_t cin = (in[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         const uint32_t cout = (out[i >> 2] >> (8 * (i & 3))) & 0xFFu;
         v = (v << 1) + (V)cin - ((V)cout << W);
-        mh = (mh << 1) + (trig_divisible<V, kDiv>(v, p) ? 1u : 0u);
-        mc = (mc << 1) + ((v == v0) ? 1u : 0u);
+        if constexpr (kDiv == kDivOdd24) {
+            // each test's lane mask goes straight into its bit mask as the carry of
+            // mask + mask (two instructions per test instead of a compare, a select and a share of
+            // the shifts and ors that the compiler otherwise emits)
+            uint32_t x;
+            asm("v_mul_u32_u24 %0, %1, %2" : "=v"(x) : "v"((uint32_t)v), "s"(p.inv24));
+            x <<= 8;
+            asm volatile("v_cmp_ge_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+                         : "+v"(mh)
+                         : "s"(p.plim24), "v"(x)
+                         : "vcc");
+            asm volatile("v_cmp_eq_u32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+                         : "+v"(mc)
+                         : "s"((uint32_t)v0), "v"((uint32_t)v)
+                         : "vcc");
+        } else {
+            mh = (mh << 1) + (trig_divisible<V, kDiv>(v, p) ? 1u : 0u);
+            mc = (mc << 1) + ((v == v0) ? 1u : 0u);
+        }
     }
     uint32_t mask = __brev(mh);
     uint32_t cand = __brev(mc) & ~mask;
