@@ -79,11 +79,13 @@ static __global__ void k_grp_offsets_gid(const uint32_t* __restrict__ gid1, uint
 
 // Element i of the initial order: rank and SA slot, whether its group stays active, and the
 // group size (the first compaction reads the rest of the group record from gid1 / gofs).  With
-// `pairs` the rank is not scattered but written as pairs[i] = x << 32 | rank, in order.
+// `pairs` the rank is not scattered but written as pairs[i] = x << 32 | rank, in order.  `pairs`
+// may be the sorted key array, which a policy's first() can still read at i (not restrict: the
+// read stays before the write).
 template <typename Policy>
 __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __restrict__ gid1,
                            const uint32_t* __restrict__ gofs, uint32_t m, uint32_t* __restrict__ rank,
-                           uint64_t* __restrict__ pairs, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
+                           uint64_t* pairs, uint32_t* __restrict__ SA, uint32_t* __restrict__ keep,
                            uint32_t* __restrict__ NGz, uint32_t* __restrict__ NR, Policy pol)
 {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -108,10 +110,16 @@ __global__ void k_dbl_init(const uint32_t* __restrict__ V, const uint32_t* __res
 }
 
 // rank[x] = r for the (x << 32 | r) pairs, partitioned by the high bits of x so that the writes of
-// neighbouring blocks fall into one window of the rank array
+// neighbouring blocks fall into one window of the rank array.  Workgroups go to the 8 XCDs in
+// turn (workgroup b to XCD b mod 8), and each XCD has its own L2: the blocks are renumbered so
+// that XCD q takes the q-th eighth of the pairs, a contiguous range of windows, and its L2
+// gathers whole lines of them instead of eight XCDs writing parts of the same lines.  The grid
+// is a multiple of 8.
 static __global__ void k_dbl_rank_pairs(const uint64_t* __restrict__ P, uint32_t m, uint32_t* __restrict__ rank)
 {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t per = gridDim.x >> 3;
+    const uint32_t b = (blockIdx.x & 7u) * per + (blockIdx.x >> 3);
+    const uint32_t i = b * blockDim.x + threadIdx.x;
     if (i >= m) return;
     const uint64_t p = P[i];
     rank[(uint32_t)(p >> 32)] = (uint32_t)p;
@@ -626,7 +634,7 @@ struct DoublingResult {
 template <typename Policy, typename Advance>
 DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V1, uint32_t N, Policy& pol,
                              Advance advance, uint32_t* rank, uint32_t* SA, int max_rounds = 1 << 30,
-                             uint64_t* K1 = nullptr)
+                             uint64_t* K1 = nullptr, uint64_t key_mask = ~0ull)
 {
     DoublingResult res;
     uint32_t* V = S.buf<uint32_t>(B_ACT_X2, N);
@@ -654,7 +662,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
     uint32_t* gid1 = large;
     uint32_t* gofs = S.buf<uint32_t>(B_DB_GOFS0, N + 1);
     {
-        sum_scan_of(S, ScanHeads<uint64_t>{K0}, true, gid1, N, true, nullptr);
+        sum_scan_of(S, ScanHeads<uint64_t>{K0, key_mask}, true, gid1, N, true, nullptr);
         k_grp_offsets_gid<<<grid_for(N), kBlock, 0, S.stream>>>(gid1, N, gofs);
         MIBWT_LAUNCH_CHECK();
         // the keys are dead once the heads are flagged: K0 takes the (position, rank) pairs
@@ -668,7 +676,7 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
             int where = 0;
             radix_sort<uint64_t, uint32_t>(S, K0, K1, nullptr, nullptr, N, 32 + std::max(0, xb - kRadixBits), 32 + xb,
                                            static_cast<const EmitPair<uint64_t, uint32_t>*>(nullptr), &where);
-            k_dbl_rank_pairs<<<grid_for(N), kBlock, 0, S.stream>>>(where ? K1 : K0, N, rank);
+            k_dbl_rank_pairs<<<(grid_for(N) + 7) & ~7u, kBlock, 0, S.stream>>>(where ? K1 : K0, N, rank);
             MIBWT_LAUNCH_CHECK();
             res.bytes += (double)N * (8 * 3 + 4);
         }

@@ -755,6 +755,12 @@ struct alignas(16) DictPos {
 struct LimDict {
     static constexpr bool kCarry = true;  // the rest length travels with the element
     const DictPos* pos;
+    // Record payload above the initial key bits (pay == nullptr: none): from bit pshift, the
+    // frequency in fbits bits (all ones: too large, read the DictPos), the preceding character
+    // (8 bits), then the alpha and whole-phrase flags.
+    const uint64_t* pay = nullptr;
+    int pshift = 64;
+    int fbits = 0;
     __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return pos[x].lim; }
 };
 struct LimParse {
@@ -835,15 +841,25 @@ DoublingResult prefix_doubling(Storage& S, uint64_t* K0, uint64_t* K1, uint32_t*
                                int key_bits, uint32_t h0, Lim lim, uint32_t* rank, uint32_t* SA,
                                Place place = Place{})
 {
-    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, N, 0, key_bits);
+    // the sorted pairs stay in whichever buffer the last pass wrote (no copy back)
+    int where = 0;
+    radix_sort<uint64_t, uint32_t>(S, K0, K1, V0, V1, N, 0, key_bits,
+                                   static_cast<const EmitPair<uint64_t, uint32_t>*>(nullptr), &where);
+    if (where) {
+        std::swap(K0, K1);
+        std::swap(V0, V1);
+    }
     using Pol = typename PolicyFor<Lim, Place>::type;
     Pol pol{};
     pol.lim = lim;
+    if constexpr (std::is_same_v<Lim, LimDict>)
+        if (pol.lim.pay) pol.lim.pay = K0;  // the record payload travelled with the keys
     pol.h = h0;
     pol.place_fn = place;
+    const uint64_t key_mask = key_bits >= 64 ? ~0ull : (1ull << key_bits) - 1;
     DoublingResult r =
         refine_groups(S, K0, V0, V1, N, pol, [](Pol& p, const uint32_t*, uint32_t) { p.h *= 2; }, rank, SA,
-                      1 << 30, K1);
+                      1 << 30, K1, key_mask);
     r.bytes += (double)N * 24 * ((key_bits + 7) / 8);
     return r;
 }
@@ -890,7 +906,8 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restr
                           const uint32_t* __restrict__ tlen, const uint32_t* __restrict__ dtext,
                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
                           const uint32_t* __restrict__ code, uint32_t base, int per, uint64_t tpow, uint32_t L,
-                          int W, uint64_t* __restrict__ K, uint32_t* __restrict__ V, DictPos* __restrict__ dp)
+                          int W, int pshift, int fbits, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
+                          DictPos* __restrict__ dp)
 {
     __shared__ uint32_t cs[256];
     cs[threadIdx.x] = code[threadIdx.x];  // blockDim == 256
@@ -914,12 +931,20 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restr
         key = key * base + s;
     }
     if (tpow) key += (uint64_t)dtext[u] * tpow;  // the text id above the symbols
-    K[x] = key;
-    V[x] = x;
     uint64_t r = freq[u];
     if (rem > (uint32_t)W) r |= kRecAlpha;
     if (j == 0) r |= kRecWhole;
     else if (rem > (uint32_t)W) r |= (uint64_t)T[tp - 1] << 32;
+    if (pshift < 64) {
+        // the record rides above the key bits through the sort (see LimDict)
+        const uint32_t fm = (1u << fbits) - 1u, f = (uint32_t)r;
+        const uint64_t pl = (uint64_t)(f < fm ? f : fm) | (((r >> 32) & 0xFFu) << fbits) |
+                            ((uint64_t)((r & kRecAlpha) != 0) << (fbits + 8)) |
+                            ((uint64_t)((r & kRecWhole) != 0) << (fbits + 9));
+        key |= pl << pshift;
+    }
+    K[x] = key;
+    V[x] = x;
     dp[x] = DictPos{r, o + len[u], 0};
 }
 
@@ -948,11 +973,26 @@ struct PlaceRec {
 };
 
 // Dictionary suffixes: at the start one 16-byte read of a position's DictPos gives both its
-// rest length and, when it is placed at once, its record.
+// rest length and, when it is placed at once, its record.  A position alone in its initial group
+// whose record came through the sort with its key (LimDict::pay) is placed without that read:
+// in a dictionary of mostly distinct phrases that is nearly every position, and the read is a
+// random access per position.
 struct DictPolicy : StringPolicy<LimDict, PlaceRec> {
     __device__ __forceinline__ bool first(uint32_t*, uint32_t pos, uint32_t x, uint32_t s, uint32_t z,
                                           uint32_t& r) const
     {
+        if (lim.pay && z == 1) {
+            const uint64_t w = lim.pay[pos] >> lim.pshift;
+            const uint32_t fm = (1u << lim.fbits) - 1u, f = (uint32_t)w & fm;
+            if (f != fm) {
+                uint64_t rec = (uint64_t)f | (((w >> lim.fbits) & 0xFFu) << 32);
+                if ((w >> (lim.fbits + 8)) & 1u) rec |= kRecAlpha;
+                if ((w >> (lim.fbits + 9)) & 1u) rec |= kRecWhole;
+                place_fn.put(pos, x, s, rec);
+                r = 0;
+                return false;
+            }
+        }
         typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
         // rec lo, rec hi, lim, pad in one streaming read (no reuse: the next read of this
         // position comes rounds later)
@@ -1682,9 +1722,29 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
         dcap = std::min(dcap, std::max(16, (need + 7) / 8 * 8));
     }
     const int aper = mixed_radix_digits((uint64_t)sigma + 2, multi ? bt.count : 1, dcap, &dkey_bits, &tpow);
+    // The records of the dictionary positions ride above the key bits through the initial sort
+    // when the flags, the preceding character and at least 4 frequency bits fit there (keys of
+    // mostly distinct phrases: the order-2 Markov text's 47-bit keys leave 7 frequency bits).
+    // MIBWT_DICT_PAYLOAD: 0 = never, k > 0 = at most k frequency bits.
+    int pshift = 64, fbits = 0;
+    {
+        int fcap = 24;
+        bool on = true;
+        if (const char* e = getenv("MIBWT_DICT_PAYLOAD")) {
+            const int v = atoi(e);
+            on = v != 0;
+            if (v > 0) fcap = std::min(fcap, v);
+        }
+        const int fb = std::min(64 - dkey_bits - 10, fcap);
+        if (on && fb >= std::min(4, fcap)) {
+            pshift = dkey_bits;
+            fbits = fb;
+        }
+    }
     DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
     k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dtl, dtext, dstart, freq, codes,
-                                                     (uint32_t)sigma + 2, aper, tpow, L, W, K0, V0, dp);
+                                                     (uint32_t)sigma + 2, aper, tpow, L, W, pshift, fbits, K0, V0,
+                                                     dp);
     MIBWT_LAUNCH_CHECK();
     // per dictionary position a record (frequency, preceding character, flags); the sort
     // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
@@ -1693,6 +1753,11 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     uint32_t* wbits = S.buf<uint32_t>(B_AS_WBITS, wwords);
     fill<uint32_t>(S, wbits, wwords, 0u);
     LimDict limd{dp};
+    if (pshift < 64) {
+        limd.pay = K0;  // prefix_doubling points it at the buffer that holds the sorted keys
+        limd.pshift = pshift;
+        limd.fbits = fbits;
+    }
     DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, dkey_bits, (uint32_t)aper, limd,
                                                            drank, nullptr,
                                                            PlaceRec{dp, RH, wbits});

@@ -323,12 +323,17 @@ void device_scan(Storage& S, const T* in, T* out, uint64_t n, bool inclusive, T 
 template <typename K>
 struct ScanHeads {
     const K* key;
-    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return (i == 0 || key[i] != key[i - 1]) ? 1u : 0u; }
+    K mask = (K)~(K)0;  // key bits that count (higher bits carry data along with the key)
+    __device__ __forceinline__ bool diff(K a, K b) const { return ((a ^ b) & mask) != 0; }
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const
+    {
+        return (i == 0 || diff(key[i], key[i - 1])) ? 1u : 0u;
+    }
     __device__ __forceinline__ uint4 load4(uint64_t i) const
     {
         const K a = key[i], b = key[i + 1], c = key[i + 2], d = key[i + 3];
-        const uint32_t h0 = (i == 0 || a != key[i - 1]) ? 1u : 0u;
-        return make_uint4(h0, b != a, c != b, d != c);
+        const uint32_t h0 = (i == 0 || diff(a, key[i - 1])) ? 1u : 0u;
+        return make_uint4(h0, diff(b, a), diff(c, b), diff(d, c));
     }
 };
 

@@ -202,6 +202,28 @@ def test_wide_entries(mibwt, monkeypatch, kind):
     assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all()
 
 
+@pytest.mark.parametrize("kind", ["random", "repetitive"])
+@pytest.mark.parametrize("payload", ["0", "1", "2", None])
+def test_dictionary_record_payload(mibwt, monkeypatch, kind, payload):
+    # the dictionary positions' records ride above the initial key bits through the sort; with
+    # 1 or 2 frequency bits most frequencies do not fit and those positions read their record
+    # instead ("0": never carried), same exact result on distinct and on repeated phrases
+    rng = np.random.default_rng(31)
+    if kind == "random":
+        T = rng.integers(0, 256, size=250000).astype(np.uint8)
+    else:
+        base = rng.integers(0, 4, size=15000).astype(np.uint8)
+        T = np.frombuffer(b"ACGT", dtype=np.uint8)[np.tile(base, 16)].copy()
+        T[rng.integers(0, len(T), size=150)] = ord("G")
+    if payload is not None:
+        monkeypatch.setenv("MIBWT_DICT_PAYLOAD", payload)
+    O, oidx = oracle.bwt(T)
+    with mibwt.Storage(len(T)) as st:
+        for W, P in [(5, 16), (4, 7)]:
+            L, idx = st.bwt(T, "pfp", W, P)
+            assert idx == oidx and (mibwt.cubwt_to_reference(L, idx) == O).all(), (W, P)
+
+
 _LARGE_BWT = {}
 
 
