@@ -128,9 +128,12 @@ static __global__ void k_dbl_rank_pairs(const uint64_t* __restrict__ P, uint32_t
 // The first placement writes the rank of every element, a permutation of [0, m).  Scattered
 // directly, each 4-byte write costs a random access to HBM once the rank array outgrows the
 // last-level cache; from this many elements on, the (position, rank) pairs are first partitioned
-// by the top 8 bits of the position (one radix pass) and then written window by window.
-// Measured with a random permutation on MI355X: 369 M elements 14.6 ms direct against 1.9 + 4.7 ms,
-// 64 M elements 0.88 ms against 0.35 + 0.49 ms.
+// by the top kRankPartBits bits of the position (two radix passes) and then written window by
+// window.  Measured with a random permutation on MI355X: 369 M elements 14.6 ms direct against
+// 1.9 + 4.7 ms with one 8-bit pass, 64 M elements 0.88 ms against 0.35 + 0.49 ms; in the 256 MiB
+// order-2 Markov text's dictionary sort (352 M positions) 16 bits in two passes take 1.1 ms less
+// than 8 bits in one (12 bits: the same as 16).
+constexpr int kRankPartBits = 16;
 constexpr uint32_t kRankPartMin = 96u << 20;
 
 // ---- one refinement round over the m active elements (groups contiguous)
@@ -674,7 +677,8 @@ DoublingResult refine_groups(Storage& S, uint64_t* K0, uint32_t* V0, uint32_t* V
         if (part) {
             const int xb = bits_for((uint64_t)N - 1);
             int where = 0;
-            radix_sort<uint64_t, uint32_t>(S, K0, K1, nullptr, nullptr, N, 32 + std::max(0, xb - kRadixBits), 32 + xb,
+            radix_sort<uint64_t, uint32_t>(S, K0, K1, nullptr, nullptr, N, 32 + std::max(0, xb - kRankPartBits),
+                                           32 + xb,
                                            static_cast<const EmitPair<uint64_t, uint32_t>*>(nullptr), &where);
             k_dbl_rank_pairs<<<(grid_for(N) + 7) & ~7u, kBlock, 0, S.stream>>>(where ? K1 : K0, N, rank);
             MIBWT_LAUNCH_CHECK();
