@@ -755,12 +755,14 @@ struct alignas(16) DictPos {
 struct LimDict {
     static constexpr bool kCarry = true;  // the rest length travels with the element
     const DictPos* pos;
-    // Record payload above the initial key bits (pay == nullptr: none): from bit pshift, the
-    // frequency in fbits bits (all ones: too large, read the DictPos), the preceding character
-    // (8 bits), then the alpha and whole-phrase flags.
+    // Payload above the initial key bits (pay == nullptr: none), from bit pshift: with fbits > 0
+    // the record: the frequency in fbits bits (all ones: too large, read the DictPos), the
+    // preceding character (8 bits), then the alpha and whole-phrase flags; with rbits > 0 the
+    // rest length of the suffix in rbits bits (all ones: too large).
     const uint64_t* pay = nullptr;
     int pshift = 64;
     int fbits = 0;
+    int rbits = 0;
     __device__ __forceinline__ uint32_t operator()(uint32_t x) const { return pos[x].lim; }
 };
 struct LimParse {
@@ -906,7 +908,8 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restr
                           const uint32_t* __restrict__ tlen, const uint32_t* __restrict__ dtext,
                           const uint32_t* __restrict__ start, const uint32_t* __restrict__ freq,
                           const uint32_t* __restrict__ code, uint32_t base, int per, uint64_t tpow, uint32_t L,
-                          int W, int pshift, int fbits, uint64_t* __restrict__ K, uint32_t* __restrict__ V,
+                          int W, int pshift, int fbits, int rbits, uint64_t* __restrict__ K,
+                          uint32_t* __restrict__ V,
                           DictPos* __restrict__ dp)
 {
     __shared__ uint32_t cs[256];
@@ -935,13 +938,17 @@ __global__ void k_dt_init(const uint8_t* __restrict__ T, const uint32_t* __restr
     if (rem > (uint32_t)W) r |= kRecAlpha;
     if (j == 0) r |= kRecWhole;
     else if (rem > (uint32_t)W) r |= (uint64_t)T[tp - 1] << 32;
-    if (pshift < 64) {
+    if (pshift < 64 && fbits) {
         // the record rides above the key bits through the sort (see LimDict)
         const uint32_t fm = (1u << fbits) - 1u, f = (uint32_t)r;
         const uint64_t pl = (uint64_t)(f < fm ? f : fm) | (((r >> 32) & 0xFFu) << fbits) |
                             ((uint64_t)((r & kRecAlpha) != 0) << (fbits + 8)) |
                             ((uint64_t)((r & kRecWhole) != 0) << (fbits + 9));
         key |= pl << pshift;
+    } else if (pshift < 64) {
+        // the rest length instead
+        const uint32_t rm = (1u << rbits) - 1u;
+        key |= (uint64_t)(rem < rm ? rem : rm) << pshift;
     }
     K[x] = key;
     V[x] = x;
@@ -976,12 +983,21 @@ struct PlaceRec {
 // rest length and, when it is placed at once, its record.  A position alone in its initial group
 // whose record came through the sort with its key (LimDict::pay) is placed without that read:
 // in a dictionary of mostly distinct phrases that is nearly every position, and the read is a
-// random access per position.
+// random access per position.  Where only the rest length fits above the key, a position that
+// stays in a larger group takes it from there (its record is read when it is placed).
 struct DictPolicy : StringPolicy<LimDict, PlaceRec> {
     __device__ __forceinline__ bool first(uint32_t*, uint32_t pos, uint32_t x, uint32_t s, uint32_t z,
                                           uint32_t& r) const
     {
-        if (lim.pay && z == 1) {
+        if (lim.pay && lim.rbits && z > 1) {
+            const uint32_t rm = (1u << lim.rbits) - 1u;
+            const uint32_t rr = (uint32_t)(lim.pay[pos] >> lim.pshift) & rm;
+            if (rr != rm && open(x, rr)) {
+                r = rr;
+                return true;
+            }
+        }
+        if (lim.pay && lim.fbits && z == 1) {
             const uint64_t w = lim.pay[pos] >> lim.pshift;
             const uint32_t fm = (1u << lim.fbits) - 1u, f = (uint32_t)w & fm;
             if (f != fm) {
@@ -1721,30 +1737,47 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
         const int need = (int)std::ceil((std::ceil(std::log2((double)L + 1) / lg) + 2) * lg);
         dcap = std::min(dcap, std::max(16, (need + 7) / 8 * 8));
     }
-    const int aper = mixed_radix_digits((uint64_t)sigma + 2, multi ? bt.count : 1, dcap, &dkey_bits, &tpow);
     // The records of the dictionary positions ride above the key bits through the initial sort
     // when the flags, the preceding character and at least 4 frequency bits fit there (keys of
-    // mostly distinct phrases: the order-2 Markov text's 47-bit keys leave 7 frequency bits).
-    // MIBWT_DICT_PAYLOAD: 0 = never, k > 0 = at most k frequency bits.
-    int pshift = 64, fbits = 0;
-    {
-        int fcap = 24;
-        bool on = true;
-        if (const char* e = getenv("MIBWT_DICT_PAYLOAD")) {
-            const int v = atoi(e);
-            on = v != 0;
-            if (v > 0) fcap = std::min(fcap, v);
+    // mostly distinct phrases: the order-2 Markov text's 47-bit keys leave 7 frequency bits);
+    // else the rest lengths when at least 5 bits are free.  A tuned width that leaves fewer
+    // gives up 8 bits of key for them (DNA: 21 symbols in 55 bits instead of 24 in 63, 9 bits of
+    // rest length; the 1 GiB collection at W=10,P=71: dictionary sort 10.9 -> 10.1 ms, at
+    // W=5,P=16 4.70 -> 4.43 ms; the narrower key alone changed nothing).
+    // MIBWT_DICT_PAYLOAD: 0 = never, k > 0 = at most k frequency bits, -k = the rest lengths in
+    // at most k bits.
+    int fcap = 24, rcap = 16;
+    bool pay_on = true, rest_only = false;
+    if (const char* e = getenv("MIBWT_DICT_PAYLOAD")) {
+        const int v = atoi(e);
+        pay_on = v != 0;
+        if (v > 0) fcap = std::min(fcap, v);
+        if (v < 0) {
+            rest_only = true;
+            rcap = std::min(rcap, -v);
         }
-        const int fb = std::min(64 - dkey_bits - 10, fcap);
-        if (on && fb >= std::min(4, fcap)) {
+    }
+    int aper = mixed_radix_digits((uint64_t)sigma + 2, multi ? bt.count : 1, dcap, &dkey_bits, &tpow);
+    if (pay_on && 64 - dkey_bits < 5 && dcap > 24 && !getenv("MIBWT_INIT_KEY_BITS"))
+        aper = mixed_radix_digits((uint64_t)sigma + 2, multi ? bt.count : 1, dcap - 8, &dkey_bits, &tpow);
+    int pshift = 64, fbits = 0, rbits = 0;
+    {
+        const bool on = pay_on;
+        const int room = 64 - dkey_bits;
+        const int fb = std::min(room - 10, fcap);
+        const int rb = std::min(room, rcap);
+        if (on && !rest_only && fb >= std::min(4, fcap)) {
             pshift = dkey_bits;
             fbits = fb;
+        } else if (on && rb >= std::min(5, rcap)) {
+            pshift = dkey_bits;
+            rbits = rb;
         }
     }
     DictPos* dp = reinterpret_cast<DictPos*>(S.buf<uint4>(B_DT_POS, L));
     k_dt_init<<<grid_for(L), kBlock, 0, S.stream>>>(T, owner, doff, dlen, dtl, dtext, dstart, freq, codes,
-                                                     (uint32_t)sigma + 2, aper, tpow, L, W, pshift, fbits, K0, V0,
-                                                     dp);
+                                                     (uint32_t)sigma + 2, aper, tpow, L, W, pshift, fbits, rbits,
+                                                     K0, V0, dp);
     MIBWT_LAUNCH_CHECK();
     // per dictionary position a record (frequency, preceding character, flags); the sort
     // places it with its group head in sorted order (RH) and marks the whole phrases (wbits)
@@ -1757,6 +1790,7 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
         limd.pay = K0;  // prefix_doubling points it at the buffer that holds the sorted keys
         limd.pshift = pshift;
         limd.fbits = fbits;
+        limd.rbits = rbits;
     }
     DoublingResult rd = prefix_doubling<LimDict, PlaceRec>(S, K0, K1, V0, V1, L, dkey_bits, (uint32_t)aper, limd,
                                                            drank, nullptr,

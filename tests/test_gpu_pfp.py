@@ -203,11 +203,12 @@ def test_wide_entries(mibwt, monkeypatch, kind):
 
 
 @pytest.mark.parametrize("kind", ["random", "repetitive"])
-@pytest.mark.parametrize("payload", ["0", "1", "2", None])
+@pytest.mark.parametrize("payload", ["0", "1", "2", "-5", "-16", None])
 def test_dictionary_record_payload(mibwt, monkeypatch, kind, payload):
     # the dictionary positions' records ride above the initial key bits through the sort; with
     # 1 or 2 frequency bits most frequencies do not fit and those positions read their record
-    # instead ("0": never carried), same exact result on distinct and on repeated phrases
+    # instead ("0": never carried; "-k": the rest lengths in k bits instead of the records, too
+    # short for most suffixes at 5), same exact result on distinct and on repeated phrases
     rng = np.random.default_rng(31)
     if kind == "random":
         T = rng.integers(0, 256, size=250000).astype(np.uint8)
