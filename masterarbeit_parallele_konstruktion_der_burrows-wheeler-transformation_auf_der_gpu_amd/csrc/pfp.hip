@@ -1044,7 +1044,7 @@ __global__ void k_asm_mix(const SortedRec* __restrict__ RH, uint32_t L, uint8_t*
 // groups them by phrase for the entry generation.
 __global__ void k_asm_counts(const SortedRec* __restrict__ RH, const uint8_t* __restrict__ gmix, uint32_t L,
                              uint32_t* __restrict__ cnt, uint32_t* __restrict__ ecnt, uint16_t* __restrict__ fc,
-                             uint32_t* __restrict__ gflag, uint32_t* __restrict__ mxf)
+                             uint8_t* __restrict__ gflag, uint8_t* __restrict__ mxf)
 {
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= L) return;
@@ -1167,7 +1167,7 @@ struct Ent {
 
 template <typename EntT>
 __global__ void k_mx_write(const DictPos* __restrict__ dp, const uint32_t* __restrict__ rank,
-                           const uint32_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
+                           const uint8_t* __restrict__ mxf, const uint32_t* __restrict__ mxs,
                            const uint32_t* __restrict__ cid, uint32_t L, EntT* __restrict__ mx)
 {
     const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1176,7 +1176,7 @@ __global__ void k_mx_write(const DictPos* __restrict__ dp, const uint32_t* __res
 }
 
 // slot offset minus entry offset of each entry group, by its dense id
-__global__ void k_ent_delta(const uint32_t* __restrict__ gflag, const uint32_t* __restrict__ cid,
+__global__ void k_ent_delta(const uint8_t* __restrict__ gflag, const uint32_t* __restrict__ cid,
                             const uint32_t* __restrict__ goff, const uint32_t* __restrict__ escan, uint32_t L,
                             uint32_t* __restrict__ deltac)
 {
@@ -1846,16 +1846,16 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     uint32_t* goff = S.buf<uint32_t>(B_G_OFF, L);
     uint32_t* ecnt = S.buf<uint32_t>(B_G_ECNT, L);
     uint16_t* fc = S.buf<uint16_t>(B_G_FILLC, L);
-    uint32_t* gflag = S.buf<uint32_t>(B_E_GFLAG, L);
+    uint8_t* gflag = S.buf<uint8_t>(B_E_GFLAG, L + 16);
     uint32_t* cid = S.buf<uint32_t>(B_E_CID, L);
-    uint32_t* mxf = S.buf<uint32_t>(B_MX_FLAG, L + 1);
+    uint8_t* mxf = S.buf<uint8_t>(B_MX_FLAG, L + 16);
     uint32_t* mxs = S.buf<uint32_t>(B_MX_SCAN, L + 1);
-    fill<uint32_t>(S, gflag, L, 0u);
-    fill<uint32_t>(S, mxf, L + 1, 0u);
+    fill<uint8_t>(S, gflag, L, (uint8_t)0);
+    fill<uint8_t>(S, mxf, L + 1, (uint8_t)0);
     k_asm_counts<<<grid_for(L), kBlock, 0, S.stream>>>(RH, gmix, L, goff, ecnt, fc, gflag, mxf);
     MIBWT_LAUNCH_CHECK();
     // dense ids of the entry groups
-    exclusive_sum<uint32_t>(S, gflag, cid, L, sc + SC_NGROUPS);
+    sum_scan_of(S, ScanBytes{gflag}, true, cid, L, false, sc + SC_NGROUPS);
     exclusive_sum<uint32_t>(S, goff, goff, L, sc + SC_TOTAL);
     exclusive_sum<uint32_t>(S, ecnt, ecnt, L);
     uint32_t hv0[SC_COUNT_];
@@ -1868,7 +1868,7 @@ int64_t pfp_build(Storage& S, uint8_t* T, const BuildTexts& bt, uint8_t* dO, int
     uint32_t* deltac = S.buf<uint32_t>(B_E_DELTAC, std::max<uint32_t>(1, ngroups));
     k_ent_delta<<<grid_for(L), kBlock, 0, S.stream>>>(gflag, cid, goff, ecnt, L, deltac);
     MIBWT_LAUNCH_CHECK();
-    exclusive_sum<uint32_t>(S, mxf, mxs, L + 1, sc + SC_ENTRIES);
+    sum_scan_of(S, ScanBytes{mxf}, true, mxs, L + 1, false, sc + SC_ENTRIES);
     uint32_t nmx = 0;
     S.read_fast(&nmx, sc + SC_ENTRIES);
     PhraseRec* prec = S.buf<PhraseRec>(B_AS_PREC, du);

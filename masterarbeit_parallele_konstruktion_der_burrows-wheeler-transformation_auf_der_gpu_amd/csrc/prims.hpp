@@ -220,6 +220,17 @@ struct ScanArray {
     __device__ __forceinline__ uint4 load4(uint64_t i) const { return *reinterpret_cast<const uint4*>(p + i); }
 };
 
+// Byte items (flags): a quarter of the bytes of ScanArray to read (and to clear beforehand).
+struct ScanBytes {
+    const uint8_t* p;
+    __device__ __forceinline__ uint32_t operator()(uint64_t i) const { return p[i]; }
+    __device__ __forceinline__ uint4 load4(uint64_t i) const
+    {
+        const uint32_t w = *reinterpret_cast<const uint32_t*>(p + i);
+        return make_uint4(w & 0xFFu, (w >> 8) & 0xFFu, (w >> 16) & 0xFFu, w >> 24);
+    }
+};
+
 template <typename Op, bool kVec, int kChainItems, typename In = ScanArray>
 __global__ __launch_bounds__(kBlock) void k_scan_chain(In in, uint32_t* out, uint64_t n,
                                                       uint32_t identity, Op op, int inclusive,
